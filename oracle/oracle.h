@@ -1,0 +1,120 @@
+/*
+ * oracle.h -- CPU restatement of the dmrg_C superblock eigensolve path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg may load liboracle.so, and only as the checker.  The product
+ * (dmrg_c_amd/) never links, loads or calls it.
+ *
+ * Every function cites the reference file:line it restates.  Pinning (what checks
+ * each function against something independent of this file) is listed per function;
+ * see tests/test_oracle_*.py.
+ */
+#ifndef DMRG_ORACLE_H
+#define DMRG_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+   int32_t row_dim;
+   int32_t col_dim;
+   const int64_t *row;
+   const int32_t *col;
+   const double *val;
+} or_crs;
+
+/* layout identical to dmrg_c_amd.snapshot.TERM_DTYPE (7 x int32 + f64, packed at 36->40) */
+typedef struct {
+   int32_t kind, op1, op2, ele0, ele1, ele2, sign_mode;
+   double coeff;
+} __attribute__((packed)) or_term;
+
+/* counter-based start vector, same formula as the product's (splitmix64) */
+void or_start_vector(int64_t n, uint64_t seed, double *out);
+
+/* y = H x, H lower-triangle CRS incl. diagonal as last entry of each row
+ * (sml/MATRIX_VECTOR_PRODUCT_SYM.c:7-51, one thread). Pinned: dense numpy H@x. */
+void or_spmv_sym(int32_t n, const int64_t *row, const int32_t *col, const double *val,
+                 const double *x, double *y);
+/* y = H x, full CRS (sml/MATRIX_VECTOR_PRODUCT.c:6-21). Pinned: dense numpy H@x. */
+void or_spmv_full(int32_t n, const int64_t *row, const int32_t *col, const double *val,
+                  const double *x, double *y);
+/* lower-triangle -> full CRS, columns ascending; returns full nnz (2*nnz_tri - n).
+ * Pass NULL outputs to only count. Pinned: scipy tril/triu round trip. */
+int64_t or_expand_sym(int32_t n, const int64_t *row, const int32_t *col, const double *val,
+                      int64_t *frow, int32_t *fcol, double *fval);
+
+/* Superblock assembly, two passes like GET_HAM_LLLRRRRL.c:11-114 with the element
+ * makers dmrg/DMRG_MAKE_ELEM_*_LLLRRRRL.c.  Pass 1 (fcol == NULL): fills row_out[n+1]
+ * and returns nnz.  Pass 2: fills fcol/fval (sized nnz) given row_out.
+ * Pinned: bit-exact Row/Col (and Val) against the compiled reference's CRS in the
+ * golden snapshots, and the first DMRG step against a brute-force 4-site Hamiltonian. */
+int64_t or_assemble(int32_t n, int32_t dim_LL, int32_t dim_RR, int32_t d,
+                    const int16_t *LL, const int16_t *LR, const int16_t *RR, const int16_t *RL,
+                    const int32_t *inv_whole,
+                    int32_t n_ops, const or_crs *ops,
+                    int32_t n_eles, const int32_t *const *eles,
+                    int32_t n_terms, const or_term *terms,
+                    int64_t *row_out, int32_t *col_out, double *val_out);
+
+/* Lowest eigenpair of the symmetric tridiagonal (diag[k], off[k-1]) by Sturm bisection
+ * and inverse iteration (replaces dstev_ at sml/LANCZOS_SLOW_SYM.c:137).
+ * evec may be NULL.  Pinned: scipy.linalg.eigh_tridiagonal. */
+int or_tridiag_lowest(int32_t k, const double *diag, const double *off, double *eval, double *evec);
+
+/* Matrix operators for the solvers */
+typedef void (*or_matvec_fn)(void *ctx, const double *x, double *y);
+
+/* LANCZOS_SLOW_SYM (sml/LANCZOS_SLOW_SYM.c:57-274) with the start vector given
+ * (Guess == "Yes").  type: 0 "Normal", 1 "Tight".  Returns steps (>0) or -1 when it
+ * does not converge within max_step (the reference exits at :276-287).
+ * Pinned: numpy eigh on the densified matrix; residual; the reference's energies. */
+int or_lanczos_slow(int64_t n, or_matvec_fn mv, void *ctx, const double *start,
+                    double acc, int min_step, int max_step, int type,
+                    double *eig_val, double *eig_vec);
+
+/* INVERSE_ITERATION_SYM (sml/INVERSE_ITERATION_SYM.c:9-166) with
+ * CONJUGATE_GRADIENT_SYM (sml/CONJUGATE_GRADIENT_SYM.c:8-123) and
+ * CONVERGE_CHECK_SYM (sml/CONVERGE_CHECK_SYM.c:7-44).  The shift (ii_diag_add - E)
+ * is applied as y = Hx + shift*x, as DMRG_CONJUGATE_GRADIENT_OBC.c:29,40 does,
+ * instead of DIAG_ADD_CRS1 on the matrix.  Returns the last error; cg_steps gets the
+ * total CG iterations.  Pinned: residual ||Hv - Ev||_1 and the dense eigenvector. */
+double or_inverse_iteration(int64_t n, or_matvec_fn mv, void *ctx, double eig_val, double *vec,
+                            double ii_acc, double ii_diag_add, int ii_max_step,
+                            double cg_acc, int cg_max_step, int *cg_steps, int *ii_steps);
+
+/* matvec contexts usable from ctypes */
+typedef struct { int32_t n; const int64_t *row; const int32_t *col; const double *val; int32_t sym; } or_crs_op;
+void or_crs_matvec(void *ctx, const double *x, double *y);
+
+/* DMRG_DIAGONALIZE_SUPERBLOCK_SYM (dmrg/DMRG_DIAGONALIZE_SUPERBLOCK_SYM.c:7-63) on a
+ * lower-triangle CRS: Lanczos-slow then inverse iteration.  The n < 1000 dense branch
+ * of the reference (LAPACK_DSYEV_CRS1) is not restated here: callers compare those
+ * sizes against numpy.linalg.eigh.  Returns 0 or -1 (Lanczos did not converge). */
+int or_diagonalize_superblock_sym(int32_t n, const int64_t *row, const int32_t *col, const double *val,
+                                  const double *start, double diag_acc, int min_step, int max_step,
+                                  int lan_type, double ii_acc, double ii_diag_add, int ii_max_step,
+                                  double cg_acc, int cg_max_step,
+                                  double *gs_val, double *gs_error, double *gs_vec,
+                                  int *lan_steps, int *cg_steps);
+
+/* DMRG_MATRIX_VECTOR_PRODUCT_OBC (dmrg/DMRG_MATRIX_VECTOR_PRODUCT_OBC.c:6-86). */
+typedef struct {
+   int32_t n, dim_RR, d;
+   const int16_t *LL, *LR, *RR, *RL;
+   const int32_t *inv_whole;
+   const int32_t *inv_LLLR;   /* [dim_LL][d]  */
+   const int32_t *inv_LRRL;   /* [d][d]       */
+   const int32_t *inv_RRRL;   /* [dim_RR][d]  */
+   const int16_t *LL_LLLR, *LR_LLLR, *LR_LRRL, *RL_LRRL, *RR_RRRL, *RL_RRRL;
+   or_crs M_LLLR, M_LRRL, M_LRRL_Sign, M_RRRL;
+   const int32_t *Ele_RR;
+} or_obc_op;
+void or_obc_matvec(void *ctx, const double *x, double *y);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

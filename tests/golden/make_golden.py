@@ -1,0 +1,114 @@
+"""Regenerate the golden fixtures in tests/golden/ from the compiled reference.
+
+Runs the reference's own DMRG programs (built by ``make -C oracle ref`` from the
+sources under /root/reference into oracle/_ref/, with oracle/harness/snap.c as main)
+and converts the superblock snapshots they dump into compact ``.npz`` files:
+
+  <name>_s<step>.npz   inputs of GET_HAM_LLLRRRRL (basis, operators, term list),
+                       the reference's assembled lower-triangle CRS (``ham_*``),
+                       and the reference's ground state (``gs_energy``, ``gs_error``,
+                       ``gs_vec``) from DMRG_DIAGONALIZE_SUPERBLOCK_SYM
+  <name>_energies.txt  per-step: step sweep LL_site RR_site n E gs_error seconds
+
+This script is only run in the build container (the reference is not on the GPU box).
+Usage:  python tests/golden/make_golden.py [--only NAME] [--big]
+"""
+from __future__ import annotations
+
+import argparse
+import glob
+import os
+import shutil
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, REPO)
+from dmrg_c_amd import snapshot  # noqa: E402
+
+REF_BIN = os.path.join(REPO, "oracle", "_ref")
+
+# name -> (program, args, steps to keep as fixtures ("all" or list), keep vectors)
+CASES = {
+    "xxz_L8_m16": ("snap_xxz", "L=8 m=16 sweep=1 J_xy=1.0 J_z=1.0 h_z=0.0", "all", True),
+    "xxz_L12_m24_jz05": ("snap_xxz", "L=12 m=24 sweep=1 J_xy=1.0 J_z=0.5 h_z=0.1", [1, 4, 6, 9, 13], True),
+    "hubbard_L8_m32": ("snap_hubbard", "L=8 N=8 m=32 sweep=1 t1=-1 U=4 h_z=0.1", "all", True),
+    "klm_L6_m32": ("snap_klm", "L=6 N=4 m=32 sweep=1 t=-1 J=2 h_z=0.1", "all", True),
+    "klm_L6_m24_ixy": ("snap_klm", "L=6 N=6 m=24 sweep=1 t=-1 J=1.5 I_xy=0.5 I_z=0.3 h_z=0.05", [2, 4], True),
+}
+# per-step energies only
+ENERGY_RUNS = {
+    "xxz_L32_m64_sw4": ("snap_xxz", "L=32 m=64 sweep=4 J_xy=1.0 J_z=1.0 h_z=0.0 diag_acc=1e-12"),
+}
+# bench input (not a parity fixture): Hubbard L=64 N=64 m=400 U=4 h=0.1, last infinite step
+BIG = {
+    "hubbard_L64_m400": ("snap_hubbard", "L=64 N=64 m=400 sweep=3 t1=-1 U=4 h_z=0.1", 31),
+}
+
+
+def run(prog: str, args: str, workdir: str, extra: str) -> None:
+    env = dict(os.environ, MKL_THREADING_LAYER="GNU", OMP_NUM_THREADS="8")
+    cmd = [os.path.join(REF_BIN, prog)] + args.split() + extra.split() + [f"out={workdir}", "threads=8"]
+    with open(os.path.join(workdir, "log.txt"), "w") as log:
+        subprocess.run(cmd, cwd=workdir, env=env, stdout=log, stderr=subprocess.STDOUT, check=True)
+
+
+def convert(workdir: str, name: str, steps, keep_vec: bool) -> None:
+    for path in sorted(glob.glob(os.path.join(workdir, "snap_*.bin"))):
+        step = int(os.path.basename(path)[5:9])
+        if steps != "all" and step not in steps:
+            continue
+        snap = snapshot.load_bin(path)
+        extra = {}
+        gs = os.path.join(workdir, f"gs_{step:04d}.bin")
+        if os.path.exists(gs):
+            e, err, vec = snapshot.load_gs_bin(gs)
+            extra["gs_energy"] = np.array([e])
+            extra["gs_error"] = np.array([err])
+            if keep_vec:
+                extra["gs_vec"] = vec
+        out = os.path.join(HERE, f"{name}_s{step:02d}.npz")
+        snapshot.save_npz(snap, out, **extra)
+        print("wrote", os.path.relpath(out, REPO), "n =", snap.n)
+
+
+def main() -> None:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--only", default=None)
+    ap.add_argument("--big", action="store_true")
+    a = ap.parse_args()
+    for name, (prog, args, steps, keep_vec) in CASES.items():
+        if a.only and a.only != name:
+            continue
+        wd = tempfile.mkdtemp(prefix="golden_")
+        dump = "all" if steps == "all" else ",".join(str(s) for s in steps)
+        run(prog, args, wd, f"dump={dump}")
+        convert(wd, name, steps, keep_vec)
+        shutil.copy(os.path.join(wd, "energies.txt"), os.path.join(HERE, f"{name}_energies.txt"))
+        shutil.rmtree(wd)
+    for name, (prog, args) in ENERGY_RUNS.items():
+        if a.only and a.only != name:
+            continue
+        wd = tempfile.mkdtemp(prefix="golden_")
+        run(prog, args, wd, "dump=")
+        shutil.copy(os.path.join(wd, "energies.txt"), os.path.join(HERE, f"{name}_energies.txt"))
+        shutil.rmtree(wd)
+        print("wrote", f"{name}_energies.txt")
+    if a.big:
+        for name, (prog, args, step) in BIG.items():
+            wd = tempfile.mkdtemp(prefix="golden_")
+            run(prog, args, wd, f"dump={step} dump_ham=0 stop_after={step}")
+            snap = snapshot.load_bin(os.path.join(wd, f"snap_{step:04d}.bin"))
+            e, err, _ = snapshot.load_gs_bin(os.path.join(wd, f"gs_{step:04d}.bin"))
+            out = os.path.join(HERE, f"{name}_s{step:02d}.npz")
+            snapshot.save_npz(snap, out, gs_energy=np.array([e]), gs_error=np.array([err]))
+            print("wrote", os.path.relpath(out, REPO), "n =", snap.n)
+            shutil.rmtree(wd)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,245 @@
+"""ctypes binding of oracle/liboracle.so (the CPU restatement) -- test helper only."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_DIR = os.path.join(REPO, "oracle")
+_LIB = None
+
+i32p = np.ctypeslib.ndpointer(np.int32, flags="C")
+i64p = np.ctypeslib.ndpointer(np.int64, flags="C")
+i16p = np.ctypeslib.ndpointer(np.int16, flags="C")
+f64p = np.ctypeslib.ndpointer(np.float64, flags="C")
+
+
+class OrCrs(C.Structure):
+    _fields_ = [("row_dim", C.c_int32), ("col_dim", C.c_int32),
+                ("row", C.c_void_p), ("col", C.c_void_p), ("val", C.c_void_p)]
+
+
+class OrCrsOp(C.Structure):
+    _fields_ = [("n", C.c_int32), ("row", C.c_void_p), ("col", C.c_void_p),
+                ("val", C.c_void_p), ("sym", C.c_int32)]
+
+
+class OrObcOp(C.Structure):
+    _fields_ = [("n", C.c_int32), ("dim_RR", C.c_int32), ("d", C.c_int32),
+                ("LL", C.c_void_p), ("LR", C.c_void_p), ("RR", C.c_void_p), ("RL", C.c_void_p),
+                ("inv_whole", C.c_void_p), ("inv_LLLR", C.c_void_p), ("inv_LRRL", C.c_void_p),
+                ("inv_RRRL", C.c_void_p),
+                ("LL_LLLR", C.c_void_p), ("LR_LLLR", C.c_void_p), ("LR_LRRL", C.c_void_p),
+                ("RL_LRRL", C.c_void_p), ("RR_RRRL", C.c_void_p), ("RL_RRRL", C.c_void_p),
+                ("M_LLLR", OrCrs), ("M_LRRL", OrCrs), ("M_LRRL_Sign", OrCrs), ("M_RRRL", OrCrs),
+                ("Ele_RR", C.c_void_p)]
+
+
+MATVEC = C.CFUNCTYPE(None, C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double))
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(ORACLE_DIR, "liboracle.so")
+        if not os.path.exists(path):
+            subprocess.run(["make", "-C", ORACLE_DIR, "liboracle.so"], check=True,
+                           stdout=subprocess.DEVNULL)
+        L = C.CDLL(path)
+        L.or_start_vector.argtypes = [C.c_int64, C.c_uint64, f64p]
+        L.or_spmv_sym.argtypes = [C.c_int32, i64p, i32p, f64p, f64p, f64p]
+        L.or_spmv_full.argtypes = [C.c_int32, i64p, i32p, f64p, f64p, f64p]
+        L.or_expand_sym.argtypes = [C.c_int32, i64p, i32p, f64p, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.or_expand_sym.restype = C.c_int64
+        L.or_assemble.argtypes = [C.c_int32] * 4 + [i16p] * 4 + [i32p, C.c_int32, C.c_void_p,
+                                                               C.c_int32, C.c_void_p, C.c_int32,
+                                                               C.c_void_p, i64p, C.c_void_p, C.c_void_p]
+        L.or_assemble.restype = C.c_int64
+        L.or_tridiag_lowest.argtypes = [C.c_int32, f64p, f64p, C.POINTER(C.c_double), C.c_void_p]
+        L.or_lanczos_slow.argtypes = [C.c_int64, C.c_void_p, C.c_void_p, f64p, C.c_double, C.c_int,
+                                      C.c_int, C.c_int, C.POINTER(C.c_double), f64p]
+        L.or_inverse_iteration.argtypes = [C.c_int64, C.c_void_p, C.c_void_p, C.c_double, f64p,
+                                           C.c_double, C.c_double, C.c_int, C.c_double, C.c_int,
+                                           C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        L.or_inverse_iteration.restype = C.c_double
+        L.or_diagonalize_superblock_sym.argtypes = [C.c_int32, i64p, i32p, f64p, f64p, C.c_double,
+                                                    C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
+                                                    C.c_int, C.c_double, C.c_int,
+                                                    C.POINTER(C.c_double), C.POINTER(C.c_double), f64p,
+                                                    C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        _LIB = L
+    return _LIB
+
+
+def start_vector(n: int, seed: int) -> np.ndarray:
+    out = np.empty(n)
+    lib().or_start_vector(n, seed, out)
+    return out
+
+
+def spmv_sym(row, col, val, x):
+    y = np.empty(len(x))
+    lib().or_spmv_sym(len(x), row, col, val, np.ascontiguousarray(x, dtype=np.float64), y)
+    return y
+
+
+def spmv_full(row, col, val, x):
+    y = np.empty(len(x))
+    lib().or_spmv_full(len(x), row, col, val, np.ascontiguousarray(x, dtype=np.float64), y)
+    return y
+
+
+def expand_sym(n, row, col, val):
+    nnz = lib().or_expand_sym(n, row, col, val, None, None, None)
+    frow = np.empty(n + 1, np.int64)
+    fcol = np.empty(nnz, np.int32)
+    fval = np.empty(nnz)
+    lib().or_expand_sym(n, row, col, val, frow.ctypes.data, fcol.ctypes.data, fval.ctypes.data)
+    return frow, fcol, fval
+
+
+def _crs_struct(c):
+    return OrCrs(c.row_dim, c.col_dim, c.row.ctypes.data, c.col.ctypes.data, c.val.ctypes.data)
+
+
+def assemble(snap):
+    """Oracle superblock assembly from a dmrg_c_amd.snapshot.Snapshot -> (row, col, val)."""
+    L = lib()
+    ops = [o for o in snap.ops]
+    for o in ops:
+        o.row = np.ascontiguousarray(o.row, np.int64)
+        o.col = np.ascontiguousarray(o.col, np.int32)
+        o.val = np.ascontiguousarray(o.val, np.float64)
+    op_arr = (OrCrs * max(1, len(ops)))(*[_crs_struct(o) for o in ops])
+    eles = [np.ascontiguousarray(e, np.int32) for e in snap.eles]
+    ele_arr = (C.c_void_p * max(1, len(eles)))(*[e.ctypes.data for e in eles])
+    terms = np.ascontiguousarray(snap.terms)
+    inv = snap.inv_table()
+    LL, LR, RR, RL = (np.ascontiguousarray(a, np.int16) for a in (snap.LL, snap.LR, snap.RR, snap.RL))
+    row = np.zeros(snap.n + 1, np.int64)
+    args = [snap.n, snap.dim_LL, snap.dim_RR, snap.dim_onsite, LL, LR, RR, RL, inv,
+            len(ops), C.cast(op_arr, C.c_void_p), len(eles), C.cast(ele_arr, C.c_void_p),
+            len(terms), terms.ctypes.data, row]
+    nnz = L.or_assemble(*args, None, None)
+    col = np.empty(nnz, np.int32)
+    val = np.empty(nnz)
+    L.or_assemble(*args, col.ctypes.data, val.ctypes.data)
+    return row, col, val
+
+
+def tridiag_lowest(a, b, want_vec=True):
+    a = np.ascontiguousarray(a, np.float64)
+    b = np.ascontiguousarray(b if len(b) else np.zeros(1), np.float64)
+    ev = C.c_double()
+    vec = np.empty(len(a)) if want_vec else None
+    lib().or_tridiag_lowest(len(a), a, b, C.byref(ev), vec.ctypes.data if want_vec else None)
+    return ev.value, vec
+
+
+class CrsOperator:
+    """Keeps the arrays alive while the oracle solvers hold pointers to them."""
+
+    def __init__(self, row, col, val, sym=True):
+        self.row = np.ascontiguousarray(row, np.int64)
+        self.col = np.ascontiguousarray(col, np.int32)
+        self.val = np.ascontiguousarray(val, np.float64)
+        self.n = len(self.row) - 1
+        self.op = OrCrsOp(self.n, self.row.ctypes.data, self.col.ctypes.data, self.val.ctypes.data,
+                          1 if sym else 0)
+        self.fn = C.cast(lib().or_crs_matvec, C.c_void_p)
+
+    @property
+    def ctx(self):
+        return C.cast(C.pointer(self.op), C.c_void_p)
+
+
+def lanczos_slow(op, start, acc=1e-14, min_step=1, max_step=800, tight=False):
+    e = C.c_double()
+    vec = np.empty(op.n)
+    k = lib().or_lanczos_slow(op.n, op.fn, op.ctx, np.ascontiguousarray(start, np.float64), acc,
+                              min_step, max_step, 1 if tight else 0, C.byref(e), vec)
+    return k, e.value, vec
+
+
+def inverse_iteration(op, e, vec, ii_acc=1e-9, ii_diag_add=1e-11, ii_max_step=3, cg_acc=1e-7,
+                      cg_max_step=2000):
+    v = np.ascontiguousarray(vec, np.float64).copy()
+    cg, ii = C.c_int(), C.c_int()
+    err = lib().or_inverse_iteration(op.n, op.fn, op.ctx, e, v, ii_acc, ii_diag_add, ii_max_step,
+                                     cg_acc, cg_max_step, C.byref(cg), C.byref(ii))
+    return err, v, cg.value, ii.value
+
+
+def diagonalize_superblock_sym(row, col, val, start, diag_acc=1e-14, min_step=1, max_step=800,
+                               tight=False, ii_acc=1e-9, ii_diag_add=1e-11, ii_max_step=3,
+                               cg_acc=1e-7, cg_max_step=2000):
+    n = len(row) - 1
+    e, err = C.c_double(), C.c_double()
+    vec = np.empty(n)
+    ls, cs = C.c_int(), C.c_int()
+    rc = lib().or_diagonalize_superblock_sym(
+        n, np.ascontiguousarray(row, np.int64), np.ascontiguousarray(col, np.int32),
+        np.ascontiguousarray(val, np.float64), np.ascontiguousarray(start, np.float64), diag_acc,
+        min_step, max_step, 1 if tight else 0, ii_acc, ii_diag_add, ii_max_step, cg_acc, cg_max_step,
+        C.byref(e), C.byref(err), vec, C.byref(ls), C.byref(cs))
+    return rc, e.value, err.value, vec, ls.value, cs.value
+
+
+def obc_operator(snap, ele_rr=None):
+    """OBC matrix-free operator context for an XXZ snapshot with its sub-Hamiltonians."""
+    o = snap.obc
+    d, dim_LL, dim_RR = snap.dim_onsite, snap.dim_LL, snap.dim_RR
+    inv_LLLR = np.full(dim_LL * d, -1, np.int32)
+    inv_LLLR[o["LL_LLLR"].astype(np.int64) * d + o["LR_LLLR"]] = np.arange(len(o["LL_LLLR"]), dtype=np.int32)
+    inv_LRRL = np.full(d * d, -1, np.int32)
+    inv_LRRL[o["LR_LRRL"].astype(np.int64) * d + o["RL_LRRL"]] = np.arange(len(o["LR_LRRL"]), dtype=np.int32)
+    inv_RRRL = np.full(dim_RR * d, -1, np.int32)
+    inv_RRRL[o["RR_RRRL"].astype(np.int64) * d + o["RL_RRRL"]] = np.arange(len(o["RR_RRRL"]), dtype=np.int32)
+    keep = {"inv_whole": snap.inv_table(), "inv_LLLR": inv_LLLR, "inv_LRRL": inv_LRRL,
+            "inv_RRRL": inv_RRRL}
+    for k in ("LL", "LR", "RR", "RL"):
+        keep[k] = np.ascontiguousarray(getattr(snap, k), np.int16)
+    for k in ("LL_LLLR", "LR_LLLR", "LR_LRRL", "RL_LRRL", "RR_RRRL", "RL_RRRL"):
+        keep[k] = np.ascontiguousarray(o[k], np.int16)
+    mats = {}
+    for k in ("M_LLLR", "M_LRRL", "M_RRRL"):
+        c = o[k]
+        c.row = np.ascontiguousarray(c.row, np.int64)
+        c.col = np.ascontiguousarray(c.col, np.int32)
+        c.val = np.ascontiguousarray(c.val, np.float64)
+        mats[k] = _crs_struct(c)
+        keep[k] = c
+    # empty sign matrix and zero parities, as DIAGONALIZE_SUPERBLOCK.c:18-19 passes
+    sign_row = np.zeros(len(o["LL_LLLR"]) + 1, np.int64)
+    keep["sign_row"] = sign_row
+    keep["sign_col"] = np.zeros(1, np.int32)
+    keep["sign_val"] = np.zeros(1)
+    ele = np.zeros(max(dim_RR * d, 1), np.int32) if ele_rr is None else np.ascontiguousarray(ele_rr, np.int32)
+    keep["ele"] = ele
+    sgn = OrCrs(len(o["LL_LLLR"]), 1, sign_row.ctypes.data, keep["sign_col"].ctypes.data,
+                keep["sign_val"].ctypes.data)
+    ptr = lambda k: keep[k].ctypes.data
+    st = OrObcOp(snap.n, dim_RR, d, ptr("LL"), ptr("LR"), ptr("RR"), ptr("RL"), ptr("inv_whole"),
+                 ptr("inv_LLLR"), ptr("inv_LRRL"), ptr("inv_RRRL"), ptr("LL_LLLR"), ptr("LR_LLLR"),
+                 ptr("LR_LRRL"), ptr("RL_LRRL"), ptr("RR_RRRL"), ptr("RL_RRRL"),
+                 mats["M_LLLR"], mats["M_LRRL"], sgn, mats["M_RRRL"], ptr("ele"))
+
+    class _Op:
+        pass
+
+    op = _Op()
+    op.keep, op.st, op.n = keep, st, snap.n
+    op.fn = C.cast(C.cast(lib().or_obc_matvec, C.c_void_p), C.c_void_p)
+    op.ctx = C.cast(C.pointer(st), C.c_void_p)
+    return op
+
+
+def obc_matvec(op, x):
+    y = np.empty(op.n)
+    f = C.cast(lib().or_obc_matvec, C.CFUNCTYPE(None, C.c_void_p, C.c_void_p, C.c_void_p))
+    xx = np.ascontiguousarray(x, np.float64)
+    f(op.ctx, xx.ctypes.data, y.ctypes.data)
+    return y
