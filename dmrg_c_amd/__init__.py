@@ -1,0 +1,442 @@
+"""Python view of libdmrg_c_amd.so, the MI355X superblock eigensolve.
+
+The library is C ABI only (include/dmrg_c_amd.h).  This module mirrors the reference's
+structs (include/SML.h, include/dmrg.h) with ctypes so that tests, the benchmark and
+Python callers pass the same objects a C caller would.  There is no CPU fallback: if the
+shared library is missing, every entry point raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Optional, Tuple
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libdmrg_c_amd.so")
+INCLUDE_DIR = os.path.join(os.path.dirname(_HERE), "include")
+
+_lib: Optional[C.CDLL] = None
+
+
+class LibraryMissing(RuntimeError):
+    pass
+
+
+class DmrgxError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"dmrgx error {code}: {msg}")
+        self.code = code
+
+
+# ------------------------------------------------------------------ structs
+class CRS1(C.Structure):
+    """include/SML.h:17-28"""
+    _fields_ = [("max_val", C.c_long), ("max_row", C.c_int), ("row_dim", C.c_int),
+                ("col_dim", C.c_int), ("Col", C.POINTER(C.c_int)), ("Row", C.POINTER(C.c_long)),
+                ("Val", C.POINTER(C.c_double))]
+
+
+class BOX_CG(C.Structure):
+    """include/SML.h:56-65"""
+    _fields_ = [("M", C.POINTER(CRS1)), ("acc", C.c_double), ("Vec", C.POINTER(C.c_double)),
+                ("Out_Vec", C.POINTER(C.c_double)), ("max_step", C.c_int), ("p_threads", C.c_int)]
+
+
+class BOX_II(C.Structure):
+    """include/SML.h:67-86"""
+    _fields_ = [("M", C.POINTER(CRS1)), ("ii_acc", C.c_double), ("ii_diag_add", C.c_double),
+                ("ii_max_step", C.c_int), ("cg_acc", C.c_double), ("cg_max_step", C.c_int),
+                ("error", C.c_double), ("Eig_Vec", C.POINTER(C.c_double)),
+                ("Eig_Val", C.POINTER(C.c_double)), ("p_threads", C.c_int), ("Type", C.c_char * 100)]
+
+
+class BOX_LAN(C.Structure):
+    """include/SML.h:122-135"""
+    _fields_ = [("M", C.POINTER(CRS1)), ("Eig_Vec", C.POINTER(C.c_double)),
+                ("eig_val", C.POINTER(C.c_double)), ("acc", C.c_double), ("min_step", C.c_int),
+                ("max_step", C.c_int), ("max_block_step", C.c_int), ("p_threads", C.c_int),
+                ("Guess", C.c_char * 100), ("Type", C.c_char * 100)]
+
+
+class DMRG_PARAMETER(C.Structure):
+    """include/dmrg.h:6-38"""
+    _fields_ = [("Initial_Guess", C.c_char * 1000), ("Enviro_Copy", C.c_char * 1000),
+                ("II_Type", C.c_char * 1000), ("Lan_Con", C.c_char * 1000),
+                ("max_dim_system", C.c_int), ("sweep", C.c_int), ("Diag_Method", C.c_char * 1000),
+                ("diag_acc", C.c_double), ("diag_min_step", C.c_int), ("diag_max_step", C.c_int),
+                ("cg_acc", C.c_double), ("cg_max_step", C.c_int), ("inv_iter_acc", C.c_double),
+                ("inv_iter_diag_add", C.c_double), ("inv_iter_max_step", C.c_int),
+                ("sp_LL", C.c_double), ("param_iter", C.c_int), ("param_iter_now", C.c_int),
+                ("dim_LLLRRRRL_limit", C.c_int)]
+
+
+class DMRG_TIME(C.Structure):
+    """include/dmrg.h:40-51"""
+    _fields_ = [(k, C.c_double) for k in ("diag", "inv_iter", "make_ham", "make_basis",
+                                          "make_dens_mat", "trans_main", "trans_exp", "total")]
+
+
+class DMRG_STATUS(C.Structure):
+    """include/dmrg.h:236-267"""
+    _fields_ = [("BC", C.c_char * 1000), ("Enviro_Copy", C.c_char * 100), ("LL_site", C.c_int),
+                ("RR_site", C.c_int), ("max_dim_system", C.c_int), ("dim_LLLRRRRL", C.c_int),
+                ("dim_LL", C.c_int), ("dim_onsite", C.c_int), ("dim_RR", C.c_int), ("spin", C.c_int),
+                ("tot_sz", C.c_int), ("tot_ele", C.c_int), ("tot_ele_1", C.c_int),
+                ("tot_ele_2", C.c_int), ("tot_parity", C.c_int), ("GS_Vec", C.POINTER(C.c_double)),
+                ("gs_val", C.c_double), ("gs_error", C.c_double), ("percent_LL", C.c_double),
+                ("sweep_now", C.c_int), ("param_iter_now", C.c_int), ("tot_iter_now", C.c_int),
+                ("sweep", C.c_int), ("param_iter", C.c_int), ("tot_iter", C.c_int)]
+
+
+class SolveParams(C.Structure):
+    _fields_ = [("diag_acc", C.c_double), ("diag_min_step", C.c_int), ("diag_max_step", C.c_int),
+                ("lan_type", C.c_int), ("guess", C.c_int), ("seed", C.c_uint64),
+                ("ii_acc", C.c_double), ("ii_diag_add", C.c_double), ("ii_max_step", C.c_int),
+                ("cg_acc", C.c_double), ("cg_max_step", C.c_int), ("skip_ii", C.c_int),
+                ("precision", C.c_int)]
+
+
+class SolveStats(C.Structure):
+    _fields_ = [("eig_val", C.c_double), ("gs_error", C.c_double), ("lanczos_steps", C.c_int),
+                ("ii_steps", C.c_int), ("cg_steps", C.c_int), ("spmv_calls", C.c_int),
+                ("seconds", C.c_double)]
+
+
+def _dp(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+# ------------------------------------------------------------------ loader
+def lib() -> C.CDLL:
+    """Load libdmrg_c_amd.so (built in-tree by __graft_entry__.build())."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise LibraryMissing(f"{LIB_PATH} not built: run `python -c 'import __graft_entry__ as g; g.build()'`")
+    L = C.CDLL(LIB_PATH)
+    vp, i32, i64, f64 = C.c_void_p, C.c_int32, C.c_int64, C.c_double
+    L.dmrgx_last_error.restype = C.c_char_p
+    L.dmrgx_init.argtypes = [C.c_int]
+    L.dmrgx_set_seed.argtypes = [C.c_uint64]
+    L.dmrgx_start_vector.argtypes = [i64, C.c_uint64, vp]
+    L.dmrgx_matrix_from_arrays.argtypes = [i32, vp, vp, vp, C.c_int, C.POINTER(vp)]
+    L.dmrgx_matrix_free.argtypes = [vp]
+    L.dmrgx_matrix_nnz.argtypes = [vp]
+    L.dmrgx_matrix_nnz.restype = i64
+    L.dmrgx_matrix_dim.argtypes = [vp]
+    L.dmrgx_matrix_dim.restype = i32
+    L.dmrgx_matrix_download.argtypes = [vp, vp, vp, vp]
+    L.dmrgx_spmv_device.argtypes = [vp, vp, vp]
+    L.dmrgx_spmv_host.argtypes = [vp, vp, vp]
+    L.dmrgx_default_params.argtypes = [C.POINTER(SolveParams)]
+    L.dmrgx_ground_state.argtypes = [vp, C.POINTER(SolveParams), vp, C.POINTER(SolveStats)]
+    L.MATRIX_VECTOR_PRODUCT_SYM.argtypes = [C.POINTER(CRS1), vp, vp, vp, C.c_int]
+    L.MATRIX_VECTOR_PRODUCT.argtypes = [C.POINTER(CRS1), vp, vp, C.c_int]
+    for nm in ("LANCZOS_SLOW_SYM", "LANCZOS_SYM", "LANCZOS_SLOW", "LANCZOS"):
+        getattr(L, nm).argtypes = [C.POINTER(BOX_LAN)]
+    for nm in ("INVERSE_ITERATION_SYM", "INVERSE_ITERATION"):
+        getattr(L, nm).argtypes = [C.POINTER(BOX_II)]
+    for nm in ("CONJUGATE_GRADIENT_SYM", "CONJUGATE_GRADIENT"):
+        getattr(L, nm).argtypes = [C.POINTER(BOX_CG)]
+    for nm in ("CONVERGE_CHECK_SYM", "CONVERGE_CHECK"):
+        getattr(L, nm).argtypes = [C.POINTER(CRS1), vp, f64, C.c_int]
+        getattr(L, nm).restype = f64
+    for nm in ("DMRG_DIAGONALIZE_SUPERBLOCK_SYM", "DMRG_DIAGONALIZE_SUPERBLOCK"):
+        getattr(L, nm).argtypes = [C.POINTER(CRS1), C.POINTER(DMRG_TIME), C.POINTER(DMRG_PARAMETER),
+                                   C.POINTER(DMRG_STATUS), C.c_int]
+    _lib = L
+    return L
+
+
+def check(code: int) -> None:
+    if code != 0:
+        raise DmrgxError(code, lib().dmrgx_last_error().decode())
+
+
+# ------------------------------------------------------------------ helpers
+class HostCRS:
+    """A CRS1 over numpy arrays (kept alive by this object)."""
+
+    def __init__(self, row, col, val, n: Optional[int] = None):
+        self.row = np.ascontiguousarray(row, dtype=np.int64)
+        self.col = np.ascontiguousarray(col, dtype=np.int32)
+        self.val = np.ascontiguousarray(val, dtype=np.float64)
+        n = len(self.row) - 1 if n is None else n
+        self.n = n
+        self.crs = CRS1(len(self.val), n + 1, n, n,
+                        self.col.ctypes.data_as(C.POINTER(C.c_int)),
+                        self.row.ctypes.data_as(C.POINTER(C.c_long)), _dp(self.val))
+
+    @property
+    def ptr(self):
+        return C.pointer(self.crs)
+
+
+class DeviceMatrix:
+    """Full CSR resident on the GPU (dmrgx_matrix)."""
+
+    def __init__(self, row, col, val, sym: bool = True):
+        L = lib()
+        self._keep = HostCRS(row, col, val)
+        h = C.c_void_p()
+        check(L.dmrgx_matrix_from_arrays(self._keep.n, self._keep.row.ctypes.data,
+                                         self._keep.col.ctypes.data, self._keep.val.ctypes.data,
+                                         1 if sym else 0, C.byref(h)))
+        self.handle = h
+        self.n = self._keep.n
+        self._keep = None
+
+    @classmethod
+    def _wrap(cls, handle, n):
+        obj = cls.__new__(cls)
+        obj.handle = handle
+        obj.n = n
+        obj._keep = None
+        return obj
+
+    @property
+    def nnz(self) -> int:
+        return int(lib().dmrgx_matrix_nnz(self.handle))
+
+    def download(self) -> Tuple[np.ndarray, np.ndarray, np.ndarray]:
+        row = np.empty(self.n + 1, np.int64)
+        col = np.empty(self.nnz, np.int32)
+        val = np.empty(self.nnz)
+        check(lib().dmrgx_matrix_download(self.handle, row.ctypes.data, col.ctypes.data, val.ctypes.data))
+        return row, col, val
+
+    def matvec(self, x: np.ndarray) -> np.ndarray:
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.empty(self.n)
+        check(lib().dmrgx_spmv_host(self.handle, x.ctypes.data, y.ctypes.data))
+        return y
+
+    def ground_state(self, start: Optional[np.ndarray] = None, **kw):
+        p = SolveParams()
+        lib().dmrgx_default_params(C.byref(p))
+        for k, v in kw.items():
+            setattr(p, k, v)
+        vec = np.zeros(self.n)
+        if start is not None:
+            p.guess = 1
+            vec[:] = start
+        st = SolveStats()
+        check(lib().dmrgx_ground_state(self.handle, C.byref(p), vec.ctypes.data, C.byref(st)))
+        return st.eig_val, vec, st
+
+    def free(self):
+        if getattr(self, "handle", None):
+            lib().dmrgx_matrix_free(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def start_vector(n: int, seed: int) -> np.ndarray:
+    out = np.empty(n)
+    lib().dmrgx_start_vector(n, seed, out.ctypes.data)
+    return out
+
+
+def default_dmrg_parameter(method="Lanczos", lan_con="Normal", diag_acc=1e-14, cg_acc=1e-7,
+                           cg_max_step=2000) -> DMRG_PARAMETER:
+    """Defaults of main/dmrg/hubbard_vf/src/main/main.c:16-49."""
+    P = DMRG_PARAMETER()
+    P.Initial_Guess = b"No"
+    P.Enviro_Copy = b"Yes"
+    P.II_Type = b"CG"
+    P.Lan_Con = lan_con.encode()
+    P.Diag_Method = method.encode()
+    P.diag_acc = diag_acc
+    P.diag_min_step = 1
+    P.diag_max_step = 800
+    P.cg_acc = cg_acc
+    P.cg_max_step = cg_max_step
+    P.inv_iter_acc = 1e-9
+    P.inv_iter_diag_add = 1e-11
+    P.inv_iter_max_step = 3
+    P.dim_LLLRRRRL_limit = 2 ** 31 - 1
+    return P
+
+
+def diagonalize_superblock_sym(row, col, val, param: Optional[DMRG_PARAMETER] = None):
+    """DMRG_DIAGONALIZE_SUPERBLOCK_SYM on a host lower-triangle CRS -> (E, gs_error, vec, time)."""
+    L = lib()
+    H = HostCRS(row, col, val)
+    P = param if param is not None else default_dmrg_parameter()
+    st = DMRG_STATUS()
+    vec = np.zeros(H.n)
+    st.GS_Vec = _dp(vec)
+    t = DMRG_TIME()
+    L.DMRG_DIAGONALIZE_SUPERBLOCK_SYM(H.ptr, C.byref(t), C.byref(P), C.byref(st), 1)
+    return st.gs_val, st.gs_error, vec, t
+
+
+# ------------------------------------------------------------ superblock assembly
+class DMRG_TERM(C.Structure):
+    """dmrgx_term of include/dmrg_c_amd.h"""
+    _fields_ = [(k, C.c_int32) for k in ("kind", "op1", "op2", "ele0", "ele1", "ele2", "sign_mode", "reserved")] + \
+               [("coeff", C.c_double)]
+
+
+class DMRG_BASIS_ONSITE(C.Structure):
+    """include/dmrg.h:198-208"""
+    _fields_ = [(k, C.c_int) for k in ("LL", "LR", "RR", "RL", "row", "dim_RR", "dim_onsite")]
+
+
+class DMRG_A_BASIS(C.Structure):
+    """include/dmrg.h:128-138"""
+    _fields_ = [("LL_LLLRRRRL", C.POINTER(C.c_short)), ("LR_LLLRRRRL", C.POINTER(C.c_short)),
+                ("RL_LLLRRRRL", C.POINTER(C.c_short)), ("RR_LLLRRRRL", C.POINTER(C.c_short)),
+                ("Inv_LLLRRRRL", C.POINTER(C.c_int)), ("Val_LLLRRRRL", C.POINTER(C.c_double)),
+                ("elem_num", C.c_int)]
+
+
+_S = C.POINTER(C.c_short)
+_I = C.POINTER(C.c_int)
+_II = C.POINTER(C.POINTER(C.c_int))
+
+
+class DMRG_BASIS(C.Structure):
+    """include/dmrg.h:54-99"""
+    _fields_ = [("LL_LLLRRRRL", _S), ("LR_LLLRRRRL", _S), ("RL_LLLRRRRL", _S), ("RR_LLLRRRRL", _S),
+                ("Inv_LLLRRRRL", _I), ("dim_LLLRRRRL", C.c_int), ("dim_LL", C.c_int), ("dim_RR", C.c_int),
+                ("dim_onsite", C.c_int), ("tot_sz_LLLRRRRL", C.c_int), ("tot_ele_LLLRRRRL", C.c_int),
+                ("tot_ele_1_LLLRRRRL", C.c_int), ("tot_ele_2_LLLRRRRL", C.c_int),
+                ("tot_parity_LLLRRRRL", C.c_int), ("LL_LLLR", _S), ("LR_LLLR", _S), ("Inv_LLLR", _II),
+                ("dim_LLLR", C.c_int), ("LR_LRRL", _S), ("RL_LRRL", _S), ("Inv_LRRL", _II), ("dim_LRRL", C.c_int),
+                ("RR_RRRL", _S), ("RL_RRRL", _S), ("Inv_RRRL", _II), ("dim_RRRL", C.c_int), ("Sum_LLLR", _I),
+                ("Tot_Sz_LLLR", _I), ("Tot_Ele_LLLR", _I), ("Tot_Ele_1_LLLR", _I), ("Tot_Ele_2_LLLR", _I),
+                ("Tot_Parity_LLLR", _I)]
+
+
+class DMRG_BOX_LAN(C.Structure):
+    """include/dmrg.h:269-285"""
+    _fields_ = [("Eig_Vec", C.POINTER(C.c_double)), ("eig_val", C.POINTER(C.c_double)), ("acc", C.c_double),
+                ("min_step", C.c_int), ("max_step", C.c_int), ("p_threads", C.c_int), ("Guess", C.c_char * 100),
+                ("Type", C.c_char * 100), ("M_LLLR", C.POINTER(CRS1)), ("M_LRRL", C.POINTER(CRS1)),
+                ("M_LRRL_Sign", C.POINTER(CRS1)), ("M_RRRL", C.POINTER(CRS1)), ("Ele_RR", _I),
+                ("Dmrg_Basis", C.POINTER(DMRG_BASIS))]
+
+
+class DMRG_BOX_CG(C.Structure):
+    """include/dmrg.h:287-302"""
+    _fields_ = [("acc", C.c_double), ("diag_val", C.c_double), ("Vec", C.POINTER(C.c_double)),
+                ("Out_Vec", C.POINTER(C.c_double)), ("max_step", C.c_int), ("p_threads", C.c_int),
+                ("M_LLLR", C.POINTER(CRS1)), ("M_LRRL", C.POINTER(CRS1)), ("M_LRRL_Sign", C.POINTER(CRS1)),
+                ("M_RRRL", C.POINTER(CRS1)), ("Ele_RR", _I), ("Dmrg_Basis", C.POINTER(DMRG_BASIS))]
+
+
+class DMRG_BOX_II(C.Structure):
+    """include/dmrg.h:304-326"""
+    _fields_ = [("ii_acc", C.c_double), ("ii_diag_add", C.c_double), ("ii_max_step", C.c_int),
+                ("cg_acc", C.c_double), ("cg_max_step", C.c_int), ("Eig_Vec", C.POINTER(C.c_double)),
+                ("eig_val", C.POINTER(C.c_double)), ("error", C.c_double), ("p_threads", C.c_int),
+                ("M_LLLR", C.POINTER(CRS1)), ("M_LRRL", C.POINTER(CRS1)), ("M_LRRL_Sign", C.POINTER(CRS1)),
+                ("M_RRRL", C.POINTER(CRS1)), ("Ele_RR", _I), ("Dmrg_Basis", C.POINTER(DMRG_BASIS))]
+
+
+def _bind_superblock(L):
+    vp = C.c_void_p
+    L.dmrgx_superblock_create_arrays.argtypes = [C.c_int32] * 4 + [vp] * 4 + [C.c_int32, vp, C.c_int32, vp, vp,
+                                                                              C.c_int32, vp, C.POINTER(vp)]
+    L.dmrgx_superblock_create.argtypes = [C.POINTER(DMRG_BASIS), vp, C.POINTER(vp)]
+    L.dmrgx_superblock_assemble.argtypes = [vp, C.POINTER(C.c_int64), C.POINTER(vp)]
+    L.dmrgx_superblock_download_lower.argtypes = [vp, vp, vp, vp]
+    L.dmrgx_superblock_free.argtypes = [vp]
+    L.dmrgx_record_begin.argtypes = []
+    L.dmrgx_record_end.restype = vp
+    L.dmrgx_terms_free.argtypes = [vp]
+    L.dmrgx_terms_count.argtypes = [vp]
+    L.dmrgx_get_ham_lllrrrrl.argtypes = [C.POINTER(DMRG_BASIS), vp]
+    L.dmrgx_get_ham_lllrrrrl.restype = C.POINTER(CRS1)
+    L.DMRG_MATRIX_VECTOR_PRODUCT_OBC.argtypes = [C.POINTER(CRS1)] * 4 + [vp, vp, _I, C.c_int, C.POINTER(DMRG_BASIS)]
+    L.DMRG_CONVERGE_CHECK_OBC.argtypes = [C.c_double] + [C.POINTER(CRS1)] * 4 + [vp, vp, _I, C.c_int,
+                                                                                C.POINTER(DMRG_BASIS)]
+    L.DMRG_CONVERGE_CHECK_OBC.restype = C.c_double
+    L.DMRG_LANCZOS_SLOW_OBC.argtypes = [C.POINTER(DMRG_BOX_LAN)]
+    L.DMRG_CONJUGATE_GRADIENT_OBC.argtypes = [C.POINTER(DMRG_BOX_CG)]
+    L.DMRG_INVERSE_ITERATION_OBC.argtypes = [C.POINTER(DMRG_BOX_II)]
+    elem1 = [C.POINTER(DMRG_BASIS_ONSITE), C.POINTER(DMRG_A_BASIS), _I, C.POINTER(CRS1), C.c_double, _I]
+    for k in ("LL", "LR", "RR", "RL"):
+        getattr(L, f"DMRG_MAKE_ELEM_{k}_LLLRRRRL").argtypes = elem1
+    two = [C.POINTER(DMRG_BASIS_ONSITE), C.POINTER(DMRG_A_BASIS), _I, C.POINTER(CRS1), C.POINTER(CRS1), C.c_double]
+    L.DMRG_MAKE_ELEM_LLLR_LLLRRRRL.argtypes = two + [_I, _I, C.c_char_p, C.c_char_p]
+    L.DMRG_MAKE_ELEM_LLRL_LLLRRRRL.argtypes = two + [_I, _I, _I, _I, C.c_char_p, C.c_char_p]
+    L.DMRG_MAKE_ELEM_LLRR_LLLRRRRL.argtypes = two + [_I, _I, _I, C.c_char_p, C.c_char_p]
+    L.DMRG_MAKE_ELEM_LRRL_LLLRRRRL.argtypes = two + [_I, _I, _I, C.c_char_p, C.c_char_p]
+    L.DMRG_MAKE_ELEM_LRRR_LLLRRRRL.argtypes = two + [_I, _I, C.c_char_p, C.c_char_p]
+    L.DMRG_MAKE_ELEM_RRRL_LLLRRRRL.argtypes = two + [_I, _I, C.c_char_p, C.c_char_p]
+    L.DMRG_MAKE_ELEM_ZERO_LLLRRRRL.argtypes = [C.POINTER(DMRG_BASIS_ONSITE), C.POINTER(DMRG_A_BASIS), _I, _I]
+
+
+_orig_lib = lib
+
+
+def lib() -> C.CDLL:  # noqa: F811  (extends the loader with the superblock bindings)
+    L = _orig_lib()
+    if not getattr(L, "_sb_bound", False):
+        _bind_superblock(L)
+        L._sb_bound = True
+    return L
+
+
+class Superblock:
+    """Device-resident superblock inputs (basis, operators, term list) from a Snapshot."""
+
+    def __init__(self, snap):
+        L = lib()
+        self.snap = snap
+        self._ops = [HostCRS(o.row, o.col, o.val, o.row_dim) for o in snap.ops]
+        for h, o in zip(self._ops, snap.ops):
+            h.crs.col_dim = o.col_dim
+        ops = (CRS1 * max(1, len(self._ops)))(*[h.crs for h in self._ops])
+        self._eles = [np.ascontiguousarray(e, np.int32) for e in snap.eles]
+        eptr = (C.c_void_p * max(1, len(self._eles)))(*[e.ctypes.data for e in self._eles])
+        elen = np.array([len(e) for e in self._eles] or [0], np.int32)
+        terms = (DMRG_TERM * max(1, len(snap.terms)))()
+        for i, t in enumerate(snap.terms):
+            terms[i] = DMRG_TERM(int(t["kind"]), int(t["op1"]), int(t["op2"]), int(t["ele0"]), int(t["ele1"]),
+                                 int(t["ele2"]), int(t["sign_mode"]), 0, float(t["coeff"]))
+        self._basis = [np.ascontiguousarray(a, np.int16) for a in (snap.LL, snap.LR, snap.RR, snap.RL)]
+        h = C.c_void_p()
+        check(L.dmrgx_superblock_create_arrays(snap.n, snap.dim_LL, snap.dim_RR, snap.dim_onsite,
+                                               *[a.ctypes.data for a in self._basis], len(self._ops),
+                                               C.cast(ops, C.c_void_p), len(self._eles), C.cast(eptr, C.c_void_p),
+                                               elen.ctypes.data, len(snap.terms), C.cast(terms, C.c_void_p),
+                                               C.byref(h)))
+        self.handle = h
+        self.n = snap.n
+        self.nnz_lower = -1
+
+    def assemble(self, full: bool = True) -> Optional[DeviceMatrix]:
+        nnz = C.c_int64()
+        A = C.c_void_p()
+        check(lib().dmrgx_superblock_assemble(self.handle, C.byref(nnz), C.byref(A) if full else None))
+        self.nnz_lower = nnz.value
+        return DeviceMatrix._wrap(A, self.n) if full else None
+
+    def download_lower(self):
+        row = np.empty(self.n + 1, np.int64)
+        col = np.empty(max(self.nnz_lower, 1), np.int32)
+        val = np.empty(max(self.nnz_lower, 1))
+        check(lib().dmrgx_superblock_download_lower(self.handle, row.ctypes.data, col.ctypes.data, val.ctypes.data))
+        return row, col[:self.nnz_lower], val[:self.nnz_lower]
+
+    def free(self):
+        if getattr(self, "handle", None):
+            lib().dmrgx_superblock_free(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass

@@ -1,0 +1,502 @@
+// extern "C" entry points of libdmrg_c_amd.so.
+//
+// Part 1: the reference's own symbols (include/SML.h, include/dmrg.h) with the reference's
+// argument meaning and error behaviour: fatal conditions print the reference's message and
+// exit(1); soft ones warn, append to ./SML_out/error.txt and continue; step counts are
+// appended to ./SML_out/<SOLVER>_Step.txt (DMRGX_SML_OUT=0 turns the files off).
+// Part 2: dmrgx_* with status codes.
+#include <sys/stat.h>
+
+#include <cstdarg>
+
+#include <chrono>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+
+#include "internal.h"
+
+using namespace dmrgx;
+
+namespace dmrgx {
+const std::string &last_error();
+void launch_start_vector(int64_t n, uint64_t seed, double *out, hipStream_t s);
+}  // namespace dmrgx
+
+namespace {
+
+std::recursive_mutex g_api_mu;
+
+bool sml_out_enabled() {
+  const char *e = getenv("DMRGX_SML_OUT");
+  return !(e && strcmp(e, "0") == 0);
+}
+
+void sml_append(const char *file, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
+void sml_append(const char *file, const char *fmt, ...) {
+  if (!sml_out_enabled()) return;
+  mkdir("SML_out", 0777);
+  char path[256];
+  snprintf(path, sizeof(path), "./SML_out/%s", file);
+  FILE *f = fopen(path, "a+");
+  if (!f) {
+    printf("Can't open file.\n");
+    exit(1);
+  }
+  va_list ap;
+  va_start(ap, fmt);
+  vfprintf(f, fmt, ap);
+  va_end(ap);
+  fclose(f);
+}
+
+[[noreturn]] void fatal(const char *fn, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
+[[noreturn]] void fatal(const char *fn, const char *fmt, ...) {
+  printf("Error in %s\n", fn);
+  va_list ap;
+  va_start(ap, fmt);
+  vprintf(fmt, ap);
+  va_end(ap);
+  fflush(stdout);
+  exit(1);
+}
+
+double now_s() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+// Run `body`; on a library failure print it and exit(1) (the drop-in has no status channel).
+template <class F>
+void run_or_die(const char *fn, F &&body) {
+  std::lock_guard<std::recursive_mutex> lk(g_api_mu);
+  try {
+    body();
+  } catch (const Failure &f) {
+    fatal(fn, "%s\n", f.what());
+  }
+}
+
+// Run `body` for a dmrgx_* entry point; failures become status codes.
+template <class F>
+int run_status(F &&body) {
+  std::lock_guard<std::recursive_mutex> lk(g_api_mu);
+  try {
+    body();
+    return DMRGX_OK;
+  } catch (const Failure &f) {
+    set_last_error(f.what());
+    return f.code;
+  } catch (const std::exception &e) {
+    set_last_error(e.what());
+    return DMRGX_ERR_HIP;
+  }
+}
+
+void check_square(const char *fn, const CRS1 *M) {
+  if (M->row_dim != M->col_dim)
+    fatal(fn, "The input matrix is not a square matirx(row=%d,col=%d)\n", M->row_dim, M->col_dim);
+  if (M->row_dim <= 0 || M->col_dim <= 0)
+    fatal(fn, "row_dim(%d) or col_dim(%d) is illegal value\n", M->row_dim, M->col_dim);
+}
+
+struct DevVec {
+  DevBuf<double> d;
+  int64_t n;
+  explicit DevVec(int64_t n_) : d(n_), n(n_) {}
+  void up(const double *h) { HIPCHK(hipMemcpyAsync(d.p, h, sizeof(double) * n, hipMemcpyHostToDevice, ctx().stream)); }
+  void down(double *h) {
+    HIPCHK(hipMemcpyAsync(h, d.p, sizeof(double) * n, hipMemcpyDeviceToHost, ctx().stream));
+    HIPCHK(hipStreamSynchronize(ctx().stream));
+  }
+};
+
+// Lanczos entry shared by the four LANCZOS* symbols.
+void lanczos_entry(const char *fn, const char *step_file, BOX_LAN *Box, bool sym, bool slow) {
+  check_square(fn, Box->M);
+  if (Box->max_step <= Box->min_step)
+    fatal(fn, "Box->max_step=%d,Box->min_step=%d\n", Box->max_step, Box->min_step);
+  const bool guess = strcmp(Box->Guess, "Yes") == 0;
+  if (!guess && strcmp(Box->Guess, "No") != 0) fatal(fn, "Box->Guess=%s is error\n", Box->Guess);
+  int type;
+  if (strcmp(Box->Type, "Normal") == 0) type = 0;
+  else if (strcmp(Box->Type, "Tight") == 0) type = 1;
+  else fatal(fn, "Box->Type=%s is error\n", Box->Type);
+  const int n = Box->M->row_dim;
+  run_or_die(fn, [&] {
+    auto A = upload_csr(n, Box->M->Row, Box->M->Col, Box->M->Val, sym);
+    DevVec v(n);
+    if (guess) v.up(Box->Eig_Vec);
+    else launch_start_vector(n, ctx().seed, v.d.p, ctx().stream);
+    LanczosOpts o;
+    o.acc = Box->acc;
+    o.min_step = Box->min_step;
+    o.max_step = Box->max_step;
+    o.max_block_step = slow ? 0 : Box->max_block_step;
+    o.type = type;
+    SolveOut out;
+    if (n == 1) {
+      Box->eig_val[0] = Box->M->Val[0];
+      Box->Eig_Vec[0] = 1.0;
+      sml_append(step_file, "%d\n", 0);
+      return;
+    }
+    lanczos(op_of(*A), o, v.d.p, out);
+    if (!out.lan_converged) {
+      sml_append("error.txt", "Error in %s\nDoes not converge (max step is %d)\n", fn, Box->max_step);
+      fatal(fn, "Does not converge (max step is %d)\n", Box->max_step);
+    }
+    Box->eig_val[0] = out.eig;
+    v.down(Box->Eig_Vec);
+    sml_append(step_file, "%d\n", out.lan_steps);
+  });
+}
+
+void ii_entry(const char *fn, BOX_II *B, bool sym) {
+  if (B->M->row_dim != B->M->col_dim)
+    fatal(fn, "The input matrix is not a square matirx(row=%d,col=%d)\n", B->M->row_dim, B->M->col_dim);
+  if (strcmp(B->Type, "CG") != 0) {
+    if (strcmp(B->Type, "MR") == 0)
+      fatal(fn, "Box_II->Type=MR (MINIMUM_RESIDUAL) is not provided by libdmrg_c_amd; use CG\n");
+    fatal(fn, "Invalid Box_II->Type=%s\n", B->Type);
+  }
+  const int n = B->M->row_dim;
+  run_or_die(fn, [&] {
+    auto A = upload_csr(n, B->M->Row, B->M->Col, B->M->Val, sym);
+    DevVec v(n);
+    v.up(B->Eig_Vec);
+    SolveOut out;
+    inverse_iteration(op_of(*A), B->Eig_Val[0], v.d.p, B->ii_acc, B->ii_diag_add, B->ii_max_step, B->cg_acc,
+                      B->cg_max_step, out);
+    v.down(B->Eig_Vec);
+    B->error = out.err;
+    if (out.ii_converged) {
+      sml_append("Inv_iter_step.txt", "%d\n", out.ii_steps);
+    } else {
+      printf("Error in %s\n", fn);
+      printf("Does not converge(%e > %e) (max step is %d)\n", out.err, B->ii_acc, B->ii_max_step);
+      printf("continue...\n");
+      sml_append("error.txt", "Error in %s\nDoes not converge(%e > %e) (max step is %d)\ncontinue...\n", fn,
+                 out.err, B->ii_acc, B->ii_max_step);
+    }
+  });
+}
+
+void cg_entry(const char *fn, BOX_CG *B, bool sym) {
+  check_square(fn, B->M);
+  const int n = B->M->row_dim;
+  run_or_die(fn, [&] {
+    auto A = upload_csr(n, B->M->Row, B->M->Col, B->M->Val, sym);
+    DevVec b(n), x(n);
+    b.up(B->Vec);
+    x.up(B->Out_Vec);
+    SolveOut out;
+    bool conv = false;
+    int it = cg_solve(op_of(*A), 0.0, b.d.p, x.d.p, B->acc, B->max_step, &conv, out);
+    x.down(B->Out_Vec);
+    const char *sf = sym ? "CONJUGATE_GRADIENT_SYM_Step.txt" : "CONJUGATE_GRADIENT_Step.txt";
+    if (!conv) {
+      printf("Error in %s\nDoes not converge (max step is %d)\nContinue...\n", fn, B->max_step);
+      sml_append("error.txt", "Error in %s\nDoes not converge (max step is %d)\nContinue...\n", fn, B->max_step);
+    }
+    sml_append(sf, "%d\n", it);
+  });
+}
+
+double check_entry(const char *fn, CRS1 *M, double *v, double eig, bool sym) {
+  double r = 0.0;
+  run_or_die(fn, [&] {
+    auto A = upload_csr(M->row_dim, M->Row, M->Col, M->Val, sym);
+    DevVec d(M->row_dim);
+    d.up(v);
+    r = converge_check(op_of(*A), d.d.p, eig);
+  });
+  return r;
+}
+
+void diagonalize_entry(const char *fn, CRS1 *Ham, DMRG_TIME *Time, DMRG_PARAMETER *Param, DMRG_STATUS *St,
+                       bool sym) {
+  const bool lan = strcmp(Param->Diag_Method, "Lanczos") == 0;
+  const bool slow = strcmp(Param->Diag_Method, "Lanczos_Slow") == 0;
+  if (!lan && !slow) fatal("DMRG_DIAGONALIZE_HAMILTONIAN", "%s", "");
+  int type;
+  if (strcmp(Param->Lan_Con, "Normal") == 0) type = 0;
+  else if (strcmp(Param->Lan_Con, "Tight") == 0) type = 1;
+  else fatal(slow ? "LANCZOS_SLOW_SYM" : "LANCZOS_SYM", "Box->Type=%s is error\n", Param->Lan_Con);
+  if (strcmp(Param->II_Type, "CG") != 0)
+    fatal("INVERSE_ITERATION_SYM", "Invalid Box_II->Type=%s (only CG is provided)\n", Param->II_Type);
+  check_square(slow ? "LANCZOS_SLOW_SYM" : "LANCZOS_SYM", Ham);
+  if (Param->diag_max_step <= Param->diag_min_step)
+    fatal(slow ? "LANCZOS_SLOW_SYM" : "LANCZOS_SYM", "Box->max_step=%d,Box->min_step=%d\n",
+          Param->diag_max_step, Param->diag_min_step);
+  const int n = Ham->row_dim;
+  run_or_die(fn, [&] {
+    double t0 = now_s();
+    std::unique_ptr<dmrgx_matrix> A(sym ? take_pending(Ham) : nullptr);
+    if (!A) A = upload_csr(n, Ham->Row, Ham->Col, Ham->Val, sym);
+    DevVec v(n);
+    SolveOut out;
+    double eig;
+    if (n == 1) {
+      eig = Ham->Val[0];
+      double one = 1.0;
+      v.up(&one);
+    } else {
+      launch_start_vector(n, ctx().seed, v.d.p, ctx().stream);  // Guess is forced to "No" (:24)
+      LanczosOpts o;
+      o.acc = Param->diag_acc;
+      o.min_step = Param->diag_min_step;
+      o.max_step = Param->diag_max_step;
+      o.max_block_step = slow ? 0 : Param->diag_max_step;
+      o.type = type;
+      lanczos(op_of(*A), o, v.d.p, out);
+      if (!out.lan_converged) {
+        const char *ln = slow ? "LANCZOS_SLOW_SYM" : "LANCZOS_SYM";
+        sml_append("error.txt", "Error in %s\nDoes not converge (max step is %d)\n", ln, Param->diag_max_step);
+        fatal(ln, "Does not converge (max step is %d)\n", Param->diag_max_step);
+      }
+      eig = out.eig;
+      sml_append(slow ? "LANCZOS_SLOW_SYM_Step.txt" : "LANCZOS_SYM_Step.txt", "%d\n", out.lan_steps);
+    }
+    double t1 = now_s();
+    inverse_iteration(op_of(*A), eig, v.d.p, Param->inv_iter_acc, Param->inv_iter_diag_add, Param->inv_iter_max_step,
+                      Param->cg_acc, Param->cg_max_step, out);
+    v.down(St->GS_Vec);
+    double t2 = now_s();
+    if (out.ii_converged) {
+      sml_append("Inv_iter_step.txt", "%d\n", out.ii_steps);
+    } else {
+      printf("Error in INVERSE_ITERATION_SYM\nDoes not converge(%e > %e) (max step is %d)\ncontinue...\n", out.err,
+             Param->inv_iter_acc, Param->inv_iter_max_step);
+      sml_append("error.txt", "Error in INVERSE_ITERATION_SYM\nDoes not converge(%e > %e) (max step is %d)\ncontinue...\n",
+                 out.err, Param->inv_iter_acc, Param->inv_iter_max_step);
+    }
+    St->gs_val = eig;       // the Lanczos eigenvalue (DMRG_DIAGONALIZE_SUPERBLOCK_SYM.c:56)
+    St->gs_error = out.err;
+    if (Time) {
+      Time->diag = t1 - t0;
+      Time->inv_iter = t2 - t1;
+    }
+  });
+}
+
+}  // namespace
+
+namespace dmrgx {
+std::unique_ptr<dmrgx_matrix> upload_csr(int32_t n, const int64_t *row, const int32_t *col, const double *val,
+                                         bool sym) {
+  if (n <= 0) throw Failure(DMRGX_ERR_ARG, "matrix dimension must be positive");
+  Context &C = ctx();
+  hipStream_t st = C.stream;
+  const int64_t nnz = row[n] - row[0];
+  if (row[0] != 0 || nnz < 0) throw Failure(DMRGX_ERR_ARG, "CRS Row[0] must be 0 and Row non-decreasing");
+  auto A = std::make_unique<dmrgx_matrix>();
+  if (sym) {
+    DevBuf<int64_t> tr((size_t)n + 1);
+    DevBuf<int32_t> tc(nnz);
+    DevBuf<double> tv(nnz);
+    HIPCHK(hipMemcpyAsync(tr.p, row, sizeof(int64_t) * ((size_t)n + 1), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(tc.p, col, sizeof(int32_t) * nnz, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(tv.p, val, sizeof(double) * nnz, hipMemcpyHostToDevice, st));
+    launch_expand_lower(n, nnz, tr.p, tc.p, tv.p, *A, st);
+  } else {
+    A->n = n;
+    A->nnz = nnz;
+    A->row.resize((size_t)n + 1);
+    A->col.resize(nnz);
+    A->val.resize(nnz);
+    HIPCHK(hipMemcpyAsync(A->row.p, row, sizeof(int64_t) * ((size_t)n + 1), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(A->col.p, col, sizeof(int32_t) * nnz, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(A->val.p, val, sizeof(double) * nnz, hipMemcpyHostToDevice, st));
+    HIPCHK(hipStreamSynchronize(st));
+  }
+  A->lanes = pick_lanes(A->n, A->nnz);
+  return A;
+}
+}  // namespace dmrgx
+
+// ================================================================ Part 1
+extern "C" {
+
+void MATRIX_VECTOR_PRODUCT_SYM(CRS1 *M, double *V1, double *Out, double **Temp, int p_threads) {
+  (void)Temp;
+  (void)p_threads;
+  run_or_die("MATRIX_VECTOR_PRODUCT_SYM", [&] {
+    auto A = upload_csr(M->row_dim, M->Row, M->Col, M->Val, true);
+    DevVec x(M->row_dim), y(M->row_dim);
+    x.up(V1);
+    launch_spmv(op_of(*A), x.d.p, y.d.p, ctx().stream);
+    y.down(Out);
+  });
+}
+
+void MATRIX_VECTOR_PRODUCT(CRS1 *M, double *V1, double *Out, int p_threads) {
+  (void)p_threads;
+  run_or_die("MATRIX_VECTOR_PRODUCT", [&] {
+    auto A = upload_csr(M->row_dim, M->Row, M->Col, M->Val, false);
+    DevVec x(M->row_dim), y(M->row_dim);
+    x.up(V1);
+    launch_spmv(op_of(*A), x.d.p, y.d.p, ctx().stream);
+    y.down(Out);
+  });
+}
+
+void LANCZOS_SLOW_SYM(BOX_LAN *Box) { lanczos_entry("LANCZOS_SLOW_SYM", "LANCZOS_SLOW_SYM_Step.txt", Box, true, true); }
+void LANCZOS_SYM(BOX_LAN *Box) { lanczos_entry("LANCZOS_SYM", "LANCZOS_SYM_Step.txt", Box, true, false); }
+void LANCZOS_SLOW(BOX_LAN *Box) { lanczos_entry("LANCZOS_SLOW", "LANCZOS_SLOW_Step.txt", Box, false, true); }
+void LANCZOS(BOX_LAN *Box) { lanczos_entry("LANCZOS", "LANCZOS_Step.txt", Box, false, false); }
+
+void INVERSE_ITERATION_SYM(BOX_II *B) { ii_entry("INVERSE_ITERATION_SYM", B, true); }
+void INVERSE_ITERATION(BOX_II *B) { ii_entry("INVERSE_ITERATION", B, false); }
+void CONJUGATE_GRADIENT_SYM(BOX_CG *B) { cg_entry("CONJUGATE_GRADIENT_SYM", B, true); }
+void CONJUGATE_GRADIENT(BOX_CG *B) { cg_entry("CONJUGATE_GRADIENT", B, false); }
+
+double CONVERGE_CHECK_SYM(CRS1 *M, double *Eig_Vec, double eig_val, int p_threads) {
+  (void)p_threads;
+  return check_entry("CONVERGE_CHECK_SYM", M, Eig_Vec, eig_val, true);
+}
+double CONVERGE_CHECK(CRS1 *M, double *Eig_Vec, double eig_val, int p_threads) {
+  (void)p_threads;
+  return check_entry("CONVERGE_CHECK", M, Eig_Vec, eig_val, false);
+}
+
+void DMRG_DIAGONALIZE_SUPERBLOCK_SYM(CRS1 *Ham, DMRG_TIME *Time, DMRG_PARAMETER *Param, DMRG_STATUS *St,
+                                     int p_threads) {
+  (void)p_threads;
+  diagonalize_entry("DMRG_DIAGONALIZE_SUPERBLOCK_SYM", Ham, Time, Param, St, true);
+}
+void DMRG_DIAGONALIZE_SUPERBLOCK(CRS1 *Ham, DMRG_TIME *Time, DMRG_PARAMETER *Param, DMRG_STATUS *St,
+                                 int p_threads) {
+  (void)p_threads;
+  diagonalize_entry("DMRG_DIAGONALIZE_SUPERBLOCK", Ham, Time, Param, St, false);
+}
+
+// ================================================================ Part 2
+const char *dmrgx_last_error(void) { return last_error().c_str(); }
+
+int dmrgx_init(int device) {
+  return run_status([&] {
+    if (device >= 0) HIPCHK(hipSetDevice(device));
+    ensure_init();
+  });
+}
+
+void dmrgx_set_seed(uint64_t seed) { ctx().seed = seed; }
+
+static uint64_t splitmix64(uint64_t z) {
+  z += 0x9E3779B97F4A7C15ULL;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  return z ^ (z >> 31);
+}
+
+void dmrgx_start_vector(int64_t n, uint64_t seed, double *out) {
+  for (int64_t i = 0; i < n; i++) {
+    const uint64_t h = splitmix64(seed * 0xD1B54A32D192ED03ULL + (uint64_t)i);
+    out[i] = (double)(int64_t)(h % 100000000ULL) - 50000000.0;
+  }
+}
+
+int dmrgx_matrix_from_crs1(const CRS1 *M, int sym, dmrgx_matrix **out) {
+  if (!M || !out) return DMRGX_ERR_ARG;
+  return dmrgx_matrix_from_arrays(M->row_dim, M->Row, M->Col, M->Val, sym, out);
+}
+
+int dmrgx_matrix_from_arrays(int32_t n, const int64_t *row, const int32_t *col, const double *val, int sym,
+                             dmrgx_matrix **out) {
+  if (!row || !out || n <= 0) return DMRGX_ERR_ARG;
+  return run_status([&] { *out = upload_csr(n, row, col, val, sym != 0).release(); });
+}
+
+void dmrgx_matrix_free(dmrgx_matrix *A) {
+  std::lock_guard<std::recursive_mutex> lk(g_api_mu);
+  delete A;
+}
+int64_t dmrgx_matrix_nnz(const dmrgx_matrix *A) { return A ? A->nnz : -1; }
+int32_t dmrgx_matrix_dim(const dmrgx_matrix *A) { return A ? A->n : -1; }
+
+int dmrgx_matrix_download(const dmrgx_matrix *A, int64_t *row, int32_t *col, double *val) {
+  if (!A) return DMRGX_ERR_ARG;
+  return run_status([&] {
+    hipStream_t st = ctx().stream;
+    HIPCHK(hipMemcpyAsync(row, A->row.p, sizeof(int64_t) * ((size_t)A->n + 1), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(col, A->col.p, sizeof(int32_t) * A->nnz, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(val, A->val.p, sizeof(double) * A->nnz, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+  });
+}
+
+int dmrgx_spmv_device(const dmrgx_matrix *A, const double *x_dev, double *y_dev) {
+  if (!A) return DMRGX_ERR_ARG;
+  return run_status([&] { launch_spmv(op_of(*A), x_dev, y_dev, ctx().stream); });
+}
+
+int dmrgx_spmv_host(const dmrgx_matrix *A, const double *x, double *y) {
+  if (!A) return DMRGX_ERR_ARG;
+  return run_status([&] {
+    DevVec dx(A->n), dy(A->n);
+    dx.up(x);
+    launch_spmv(op_of(*A), dx.d.p, dy.d.p, ctx().stream);
+    dy.down(y);
+  });
+}
+
+void dmrgx_default_params(dmrgx_solve_params *p) {
+  memset(p, 0, sizeof(*p));
+  p->diag_acc = 1e-14;     // main/dmrg/hubbard_vf/src/main/main.c:40-42
+  p->diag_min_step = 1;
+  p->diag_max_step = 800;
+  p->lan_type = 0;
+  p->guess = 0;
+  p->seed = 1;
+  p->ii_acc = 1e-9;        // :47-49
+  p->ii_diag_add = 1e-11;
+  p->ii_max_step = 3;
+  p->cg_acc = 1e-7;        // :44-45
+  p->cg_max_step = 2000;
+}
+
+int dmrgx_ground_state(const dmrgx_matrix *A, const dmrgx_solve_params *p, double *vec_inout,
+                       dmrgx_solve_stats *stats) {
+  if (!A || !p || !vec_inout) return DMRGX_ERR_ARG;
+  if (p->diag_max_step <= p->diag_min_step) return DMRGX_ERR_ARG;
+  return run_status([&] {
+    const double t0 = now_s();
+    hipStream_t st = ctx().stream;
+    DevVec v(A->n);
+    if (p->guess) v.up(vec_inout);
+    else launch_start_vector(A->n, p->seed, v.d.p, st);
+    SolveOut out;
+    LanczosOpts o;
+    o.acc = p->diag_acc;
+    o.min_step = p->diag_min_step;
+    o.max_step = p->diag_max_step;
+    o.type = p->lan_type;
+    if (A->n == 1) {
+      double h;
+      HIPCHK(hipMemcpy(&h, A->val.p, sizeof(double), hipMemcpyDeviceToHost));
+      out.eig = h;
+      const double one = 1.0;
+      v.up(&one);
+      out.lan_converged = true;
+    } else {
+      lanczos(op_of(*A), o, v.d.p, out);
+    }
+    if (!out.lan_converged) throw Failure(DMRGX_ERR_NOCONV, "Lanczos did not converge within diag_max_step");
+    if (!p->skip_ii)
+      inverse_iteration(op_of(*A), out.eig, v.d.p, p->ii_acc, p->ii_diag_add, p->ii_max_step, p->cg_acc, p->cg_max_step,
+                        out);
+    v.down(vec_inout);
+    if (stats) {
+      stats->eig_val = out.eig;
+      stats->gs_error = out.err;
+      stats->lanczos_steps = out.lan_steps;
+      stats->ii_steps = out.ii_steps;
+      stats->cg_steps = out.cg_steps;
+      stats->spmv_calls = out.spmv;
+      stats->seconds = now_s() - t0;
+    }
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,262 @@
+// Internal declarations of libdmrg_c_amd.so (not installed, not part of the ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <memory>
+#include <stdexcept>
+#include <unordered_map>
+#include <string>
+#include <vector>
+
+#include "dmrg_c_amd.h"
+
+namespace dmrgx {
+
+constexpr int kBlock = 256;          // threads per workgroup (4 wave64)
+constexpr int kVecGrid = 1024;       // fixed grid of the streaming vector kernels
+
+// Error carried from the HIP layer up to the C entry points.
+struct Failure : std::runtime_error {
+  int code;
+  Failure(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+void set_last_error(const std::string &msg);
+[[noreturn]] void hip_fail(hipError_t e, const char *what, const char *file, int line);
+
+#define HIPCHK(expr)                                                   \
+  do {                                                                 \
+    hipError_t e_ = (expr);                                            \
+    if (e_ != hipSuccess) ::dmrgx::hip_fail(e_, #expr, __FILE__, __LINE__); \
+  } while (0)
+
+// One process-wide context per device: stream, pinned staging, allocation cache.
+struct Context {
+  int device = -1;
+  hipStream_t stream = nullptr;
+  uint64_t seed = 1;
+  double *pinned = nullptr;        // host staging for scalars read back per chunk
+  size_t pinned_cap = 0;
+  std::vector<std::pair<size_t, void *>> free_list;  // cached device blocks
+  size_t cached_bytes = 0;
+  std::unordered_map<void *, size_t> live;            // block sizes of handed-out pointers
+
+  void *alloc(size_t bytes);
+  void release(void *p, size_t bytes);
+  double *host_stage(size_t count);
+  void sizes_note(void *p, size_t bytes) { live[p] = bytes; }
+  size_t sizes_take(void *p) {
+    auto it = live.find(p);
+    size_t b = it == live.end() ? 0 : it->second;
+    if (it != live.end()) live.erase(it);
+    return b;
+  }
+};
+
+Context &ctx();          // lazily initialised on the current device
+void ensure_init();
+
+// RAII device buffer from the context cache.
+template <class T>
+struct DevBuf {
+  T *p = nullptr;
+  size_t count = 0;
+  DevBuf() = default;
+  explicit DevBuf(size_t n) { resize(n); }
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+  DevBuf(DevBuf &&o) noexcept : p(o.p), count(o.count) { o.p = nullptr; o.count = 0; }
+  DevBuf &operator=(DevBuf &&o) noexcept {
+    if (this != &o) { reset(); p = o.p; count = o.count; o.p = nullptr; o.count = 0; }
+    return *this;
+  }
+  ~DevBuf() { reset(); }
+  void resize(size_t n) {
+    reset();
+    if (n) { p = static_cast<T *>(ctx().alloc(n * sizeof(T))); count = n; }
+  }
+  void reset() {
+    if (p) ctx().release(p, count * sizeof(T));
+    p = nullptr;
+    count = 0;
+  }
+};
+
+}  // namespace dmrgx
+
+// Full CSR resident on the device.  Columns ascending in every row.
+struct dmrgx_matrix {
+  int32_t n = 0;
+  int64_t nnz = 0;
+  dmrgx::DevBuf<int64_t> row;
+  dmrgx::DevBuf<int32_t> col;
+  dmrgx::DevBuf<double> val;
+  dmrgx::DevBuf<float> val32;   // FP32 copy of val for the mixed-precision Lanczos
+  int lanes = 8;                // lanes per row of the SpMV kernel
+};
+
+namespace dmrgx {
+
+// Matrix-free superblock operator of dmrg/DMRG_MATRIX_VECTOR_PRODUCT_OBC.c:18-83: the row's
+// contributions from M_LLLR, M_LRRL, M_LRRL_Sign and M_RRRL are one flattened list the
+// lanes stride over.
+struct ObcRows {
+  int32_t dim_RR, d;
+  const int16_t *LL, *LR, *RR, *RL;
+  const int32_t *inv;
+  const int32_t *inv_LLLR, *inv_LRRL, *inv_RRRL;
+  const int16_t *LL_LLLR, *LR_LLLR, *LR_LRRL, *RL_LRRL, *RR_RRRL, *RL_RRRL;
+  const int64_t *r1, *r2, *r3, *r4;
+  const int32_t *c1, *c2, *c3, *c4;
+  const double *v1, *v2, *v3, *v4;
+  const int32_t *ele_RR;
+  template <int G>
+  __device__ __forceinline__ double lane_sum(int32_t r, int lane, const double *__restrict__ x) const {
+    const int ll = LL[r], lr = LR[r], rr = RR[r], rl = RL[r];
+    const int a = inv_LLLR[ll * d + lr], b = inv_LRRL[lr * d + rl], c = inv_RRRL[rr * d + rl];
+    const int64_t s1 = r1[a], n1 = r1[a + 1] - s1;
+    const int64_t s2 = r2[b], n2 = r2[b + 1] - s2;
+    const int64_t s3 = r3[b], n3 = r3[b + 1] - s3;
+    const int64_t s4 = r4[c], n4 = r4[c + 1] - s4;
+    const double sign = (ele_RR[rr] % 2 == 1) ? -1.0 : 1.0;
+    const int64_t base = (int64_t)ll * d * dim_RR * d + (int64_t)lr * dim_RR * d;
+    double s = 0.0;
+    for (int64_t t = lane; t < n1 + n2 + n3 + n4; t += G) {
+      int64_t idx;
+      double v;
+      if (t < n1) {
+        const int32_t cc = c1[s1 + t];
+        idx = ((int64_t)LL_LLLR[cc] * d + LR_LLLR[cc]) * dim_RR * d + (int64_t)rr * d + rl;
+        v = v1[s1 + t];
+      } else if (t < n1 + n2) {
+        const int64_t j = s2 + (t - n1);
+        const int32_t cc = c2[j];
+        idx = (int64_t)ll * d * dim_RR * d + (int64_t)LR_LRRL[cc] * dim_RR * d + (int64_t)rr * d + RL_LRRL[cc];
+        v = v2[j];
+      } else if (t < n1 + n2 + n3) {
+        const int64_t j = s3 + (t - n1 - n2);
+        const int32_t cc = c3[j];
+        idx = (int64_t)ll * d * dim_RR * d + (int64_t)LR_LRRL[cc] * dim_RR * d + (int64_t)rr * d + RL_LRRL[cc];
+        v = v3[j] * sign;
+      } else {
+        const int64_t j = s4 + (t - n1 - n2 - n3);
+        const int32_t cc = c4[j];
+        idx = base + (int64_t)RR_RRRL[cc] * d + RL_RRRL[cc];
+        v = v4[j];
+      }
+      const int32_t iv = inv[idx];
+      if (iv >= 0) s += x[iv] * v;
+    }
+    return s;
+  }
+};
+
+// Device arrays of the matrix-free operator (uploaded from DMRG_BASIS + the three
+// sub-Hamiltonians, obc.cpp).
+struct ObcDev {
+  int32_t n = 0, dim_LL = 0, dim_RR = 0, d = 0, lanes = 8;
+  DevBuf<int16_t> LL, LR, RR, RL, LL_LLLR, LR_LLLR, LR_LRRL, RL_LRRL, RR_RRRL, RL_RRRL;
+  DevBuf<int32_t> inv, inv_LLLR, inv_LRRL, inv_RRRL, ele_RR;
+  DevBuf<int64_t> r1, r2, r3, r4;
+  DevBuf<int32_t> c1, c2, c3, c4;
+  DevBuf<double> v1, v2, v3, v4;
+  ObcRows rows() const {
+    return ObcRows{dim_RR, d, LL.p, LR.p, RR.p, RL.p, inv.p, inv_LLLR.p, inv_LRRL.p, inv_RRRL.p,
+                   LL_LLLR.p, LR_LLLR.p, LR_LRRL.p, RL_LRRL.p, RR_RRRL.p, RL_RRRL.p,
+                   r1.p, r2.p, r3.p, r4.p, c1.p, c2.p, c3.p, c4.p, v1.p, v2.p, v3.p, v4.p, ele_RR.p};
+  }
+};
+
+// The operator a solver applies: an assembled full CSR, or the matrix-free OBC form.
+struct Operator {
+  int32_t n = 0;
+  int lanes = 8;
+  const dmrgx_matrix *A = nullptr;
+  const ObcDev *obc = nullptr;
+};
+inline Operator op_of(const dmrgx_matrix &A) { return Operator{A.n, A.lanes, &A, nullptr}; }
+inline Operator op_of(const ObcDev &O) { return Operator{O.n, O.lanes, nullptr, &O}; }
+
+// ---- kernel launchers (kernels.hip)
+struct Reduce {              // deterministic grid reduction workspace
+  DevBuf<double> partial;    // one slot per block
+  DevBuf<unsigned> counter;  // last-block ticket, self-resetting
+  void ensure(size_t blocks);
+};
+
+int pick_lanes(int32_t n, int64_t nnz);
+int spmv_blocks(int32_t n, int lanes);
+
+void launch_spmv(const Operator &op, const double *x, double *y, hipStream_t s);
+void launch_expand_lower(int32_t n, int64_t nnz_tri, const int64_t *trow, const int32_t *tcol,
+                         const double *tval, dmrgx_matrix &A, hipStream_t s);
+void launch_to_fp32(const dmrgx_matrix &A, hipStream_t s);
+void launch_start_vector(int64_t n, uint64_t seed, double *out, hipStream_t s);
+
+// Lanczos step kernels.  Scalars live in device arrays indexed by step.
+void launch_norm2(int64_t n, const double *x, Reduce &R, double *out, hipStream_t s);
+void launch_lanczos_a(const Operator &op, bool fp32, int k, const double *w, const double *nrm2,
+                      double *vk, double *y, double *alpha, Reduce &R, hipStream_t s);
+void launch_lanczos_b(int64_t n, int k, const double *y, const double *vk, const double *vkm1,
+                      const double *alpha, double *beta, double *w, double *nrm2, Reduce &R,
+                      hipStream_t s);
+void launch_ritz(int64_t n, int k, const double *const *vecs_dev, const double *coef_dev, double *out,
+                 hipStream_t s);
+void launch_scale_by_rnorm(int64_t n, const double *nrm2, const double *x, double *y, hipStream_t s);
+void launch_copy(int64_t n, const double *x, double *y, hipStream_t s);
+
+// inverse iteration / CG kernels
+void launch_residual_l1(const Operator &op, const double *v, double eig, double *out, Reduce &R,
+                        hipStream_t s);
+void launch_cg_init(const Operator &op, double shift, const double *nrm2, const double *x_in,
+                    double *x_out, const double *b, double *r, double *p, double *rr0, Reduce &R,
+                    hipStream_t s);
+void launch_cg_ap(const Operator &op, double shift, const double *p, double *y, double *pap,
+                  const int *done, Reduce &R, hipStream_t s);
+void launch_cg_update(int64_t n, int it, double acc, const double *p, const double *y, double *x,
+                      double *r, const double *pap, double *rr, int *status, Reduce &R,
+                      hipStream_t s);
+void launch_cg_dir(int64_t n, int it, const double *r, const double *rr, double *p, const int *status,
+                   hipStream_t s);
+
+// ---- host tridiagonal solver (tridiag.cpp)
+// lowest eigenvalue of T(a[0..k-1], b[0..k-2]); lam_hint: an upper bound or +inf
+double tridiag_lowest_value(int k, const double *a, const double *b, double upper_hint);
+// eigenvector of T for eigenvalue lam, normalised
+void tridiag_vector(int k, const double *a, const double *b, double lam, double *v);
+
+// ---- solvers (solver.hip)
+struct SolveOut {
+  double eig = 0, err = 0;
+  int lan_steps = 0, ii_steps = 0, cg_steps = 0, spmv = 0;
+  bool lan_converged = false, cg_all_converged = true, ii_converged = false;
+};
+struct LanczosOpts {
+  double acc = 1e-14;
+  int min_step = 1, max_step = 800, max_block_step = 0;  // 0: no restart
+  int type = 0;        // 0 Normal, 1 Tight
+  bool fp32 = false;
+  bool dense_small = true;  // n < 1000: residual-bound stop standing in for the dense dsyev_ branch
+};
+// vec: device in/out (start on input, Ritz vector on output)
+void lanczos(const Operator &A, const LanczosOpts &o, double *vec, SolveOut &out);
+// Inverse iteration (shift folded into the operator), CG inner solver; vec device in/out.
+void inverse_iteration(const Operator &A, double eig, double *vec, double ii_acc, double ii_diag_add,
+                       int ii_max_step, double cg_acc, int cg_max_step, SolveOut &out);
+// CG solve of (A + shift I) x = b with x normalised first; returns iterations, sets converged.
+int cg_solve(const Operator &A, double shift, const double *b, double *x, double acc, int max_step,
+             bool *converged, SolveOut &out);
+double converge_check(const Operator &A, const double *v, double eig);
+
+// device matrix handed over by dmrgx_get_ham_lllrrrrl for the CRS1 it returned (assemble.hip)
+dmrgx_matrix *take_pending(const CRS1 *M);
+void launch_exclusive_scan(int32_t n, const int64_t *len, int64_t *out, hipStream_t s);
+
+// upload helpers (abi.cpp)
+std::unique_ptr<dmrgx_matrix> upload_csr(int32_t n, const int64_t *row, const int32_t *col,
+                                         const double *val, bool sym);
+
+}  // namespace dmrgx

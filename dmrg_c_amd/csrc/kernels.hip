@@ -1,0 +1,638 @@
+// Device kernels of the superblock eigensolve for gfx950 (wave64, 256-thread workgroups).
+//
+// * CSR SpMV with G lanes per row (G in {4..64}, picked from the mean row length) and a
+//   fused per-row epilogue (Lanczos normalise + alpha dot, CG A*p + shift + p.Ap, the
+//   L1 residual of the inverse-iteration convergence check).
+// * Streaming vector kernels for the Lanczos / CG updates with the next reduction fused.
+// * Every reduction is deterministic: each block writes one partial, the last block to
+//   arrive (agent-scope ticket) sums the partials in index order.  Results do not depend
+//   on scheduling, so a solve gives the same bits alone or next to other work.
+// * Lower-triangle -> full CSR expansion (counting, scan, scatter, per-row sort).
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "internal.h"
+
+namespace dmrgx {
+namespace {
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  return v;
+}
+
+// Sum over the workgroup; the result is valid in thread 0.  Uses LDS, so callers that
+// reduce twice in one kernel must separate the calls with __syncthreads().
+__device__ __forceinline__ double block_sum(double v) {
+  __shared__ double lds[kBlock / 64];
+  v = wave_sum(v);
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) lds[w] = v;
+  __syncthreads();
+  double t = 0.0;
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int i = 0; i < kBlock / 64; i++) t += lds[i];
+  }
+  return t;
+}
+
+// Publish this block's partial; the last block sums all partials in index order.
+// Returns true in every thread of the last block; *total is valid in its thread 0.
+__device__ bool grid_reduce(double block_total, double *partial, unsigned *counter, double *total) {
+  __shared__ int is_last;
+  if (threadIdx.x == 0) {
+    __hip_atomic_store(&partial[blockIdx.x], block_total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    unsigned t = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    is_last = (t == gridDim.x - 1);
+  }
+  __syncthreads();
+  if (!is_last) return false;
+  __threadfence();
+  double s = 0.0;
+  for (unsigned i = threadIdx.x; i < gridDim.x; i += kBlock)
+    s += __hip_atomic_load(&partial[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __syncthreads();
+  double t = block_sum(s);
+  if (threadIdx.x == 0) {
+    __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    *total = t;
+  }
+  return true;
+}
+
+// ------------------------------------------------------------------ SpMV + epilogue
+// Row sources: each returns this lane's share of sum_j H_rj x_j for row r.
+template <class VT>
+struct CsrRows {
+  const int64_t *__restrict__ row;
+  const int32_t *__restrict__ col;
+  const VT *__restrict__ val;
+  template <int G>
+  __device__ __forceinline__ double lane_sum(int32_t r, int lane, const double *__restrict__ x) const {
+    double s = 0.0;
+    const int64_t b = row[r], e = row[r + 1];
+    for (int64_t j = b + lane; j < e; j += G) s += (double)val[j] * x[col[j]];
+    return s;
+  }
+};
+
+template <int G, class Rows, class Epi>
+__global__ __launch_bounds__(kBlock) void k_apply(int32_t n, Rows rows, const double *__restrict__ x, Epi epi,
+                                                  double *partial, unsigned *counter) {
+  if (epi.skip()) return;
+  constexpr int RPB = kBlock / G;
+  const int g = threadIdx.x / G;
+  const int lane = threadIdx.x % G;
+  const int32_t r = blockIdx.x * RPB + g;
+  double s = 0.0;
+  if (r < n) s = rows.template lane_sum<G>(r, lane, x);
+#pragma unroll
+  for (int off = G / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, G);
+  epi.begin();
+  double c = 0.0;
+  if (lane == 0 && r < n) c = epi.row(r, s);
+  if constexpr (Epi::kReduce) {
+    double bt = block_sum(c);
+    double tot;
+    if (grid_reduce(bt, partial, counter, &tot) && threadIdx.x == 0) epi.finish(tot);
+  }
+}
+
+struct EpiPlain {
+  double *y;
+  static constexpr bool kReduce = false;
+  __device__ bool skip() const { return false; }
+  __device__ void begin() {}
+  __device__ double row(int32_t r, double s) { y[r] = s; return 0.0; }
+  __device__ void finish(double) {}
+};
+
+// K_a of a Lanczos step: v_k = w / ||w||, y = H v_k, alpha_k = <y, v_k>
+// (sml/LANCZOS_SLOW_SYM.c:114-122 with NORMALIZE folded into the product).
+struct EpiLanczosA {
+  const double *w;
+  const double *nrm2;
+  double *vk, *y, *alpha;
+  int k;
+  double scale;
+  static constexpr bool kReduce = true;
+  __device__ bool skip() const { return false; }
+  __device__ void begin() { scale = 1.0 / sqrt(*nrm2); }
+  __device__ double row(int32_t r, double s) {
+    const double xs = scale * w[r];
+    vk[r] = xs;
+    const double yv = scale * s;
+    y[r] = yv;
+    return yv * xs;
+  }
+  __device__ void finish(double t) { alpha[k] = t; }
+};
+
+// CG start (sml/CONJUGATE_GRADIENT_SYM.c:202-210): x = x0/||x0||, r = b - (H + shift) x, p = r
+struct EpiCgInit {
+  const double *nrm2, *x_in, *b;
+  double *x_out, *r_out, *p_out, *rr0;
+  double shift, scale;
+  static constexpr bool kReduce = true;
+  __device__ bool skip() const { return false; }
+  __device__ void begin() { scale = 1.0 / sqrt(*nrm2); }
+  __device__ double row(int32_t r, double s) {
+    const double xs = scale * x_in[r];
+    x_out[r] = xs;
+    const double rv = b[r] - (scale * s + shift * xs);
+    r_out[r] = rv;
+    p_out[r] = rv;
+    return rv * rv;
+  }
+  __device__ void finish(double t) { *rr0 = t; }
+};
+
+// CG product (CONJUGATE_GRADIENT_SYM.c:214-218, shift as DMRG_CONJUGATE_GRADIENT_OBC.c:40)
+struct EpiCgAp {
+  const double *p;
+  double *y, *pap;
+  const int *status;
+  double shift;
+  static constexpr bool kReduce = true;
+  __device__ bool skip() const { return status[0] != 0; }
+  __device__ void begin() {}
+  __device__ double row(int32_t r, double s) {
+    const double yv = s + shift * p[r];
+    y[r] = yv;
+    return p[r] * yv;
+  }
+  __device__ void finish(double t) { *pap = t; }
+};
+
+// CONVERGE_CHECK_SYM (sml/CONVERGE_CHECK_SYM.c:24-33): sum_i |E v_i - (H v)_i|
+struct EpiResidualL1 {
+  const double *v;
+  double *out;
+  double eig;
+  static constexpr bool kReduce = true;
+  __device__ bool skip() const { return false; }
+  __device__ void begin() {}
+  __device__ double row(int32_t r, double s) { return fabs(eig * v[r] - s); }
+  __device__ void finish(double t) { *out = t; }
+};
+
+template <class Rows, class Epi>
+void apply_rows(int32_t n, int lanes, const Rows &rows, const double *x, const Epi &epi, Reduce *R,
+                hipStream_t s) {
+  const int blocks = spmv_blocks(n, lanes);
+  if (blocks == 0) return;
+  double *part = nullptr;
+  unsigned *cnt = nullptr;
+  if (Epi::kReduce) {
+    R->ensure(blocks);
+    part = R->partial.p;
+    cnt = R->counter.p;
+  }
+  switch (lanes) {
+#define L_CASE(G) \
+  case G: k_apply<G, Rows, Epi><<<blocks, kBlock, 0, s>>>(n, rows, x, epi, part, cnt); break;
+    L_CASE(1) L_CASE(2) L_CASE(4) L_CASE(8) L_CASE(16) L_CASE(32) L_CASE(64)
+#undef L_CASE
+    default: throw Failure(DMRGX_ERR_ARG, "bad lanes per row");
+  }
+  HIPCHK(hipGetLastError());
+}
+
+template <class Epi>
+void apply_op(const Operator &op, bool fp32, const double *x, const Epi &epi, Reduce *R, hipStream_t s) {
+  if (op.obc) {
+    apply_rows(op.n, op.lanes, op.obc->rows(), x, epi, R, s);
+  } else if (fp32) {
+    if (!op.A->val32.p) throw Failure(DMRGX_ERR_ARG, "FP32 values not prepared");
+    apply_rows(op.n, op.lanes, CsrRows<float>{op.A->row.p, op.A->col.p, op.A->val32.p}, x, epi, R, s);
+  } else {
+    apply_rows(op.n, op.lanes, CsrRows<double>{op.A->row.p, op.A->col.p, op.A->val.p}, x, epi, R, s);
+  }
+}
+
+// -------------------------------------------------------------- vector kernels
+__global__ __launch_bounds__(kBlock) void k_norm2(int64_t n, const double *__restrict__ x,
+                                                  double *out, double *partial, unsigned *counter) {
+  double s = 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
+    s += x[i] * x[i];
+  double bt = block_sum(s), tot;
+  if (grid_reduce(bt, partial, counter, &tot) && threadIdx.x == 0) *out = tot;
+}
+
+// K_b of a Lanczos step (sml/LANCZOS_SLOW_SYM.c:183-186 and :116 of the next step):
+// w = y - alpha_k v_k - beta_k v_{k-1}; nrm2 = <w,w>; beta_{k+1} = sqrt(nrm2)
+__global__ __launch_bounds__(kBlock) void k_lanczos_b(int64_t n, int k, const double *__restrict__ y,
+                                                      const double *__restrict__ vk,
+                                                      const double *__restrict__ vkm1,
+                                                      const double *alpha, double *beta,
+                                                      double *__restrict__ w, double *nrm2,
+                                                      double *partial, unsigned *counter) {
+  const double a = alpha[k];
+  const double b = (k > 0) ? beta[k] : 0.0;
+  double s = 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+    double v = y[i] - a * vk[i];
+    if (k > 0) v -= b * vkm1[i];
+    w[i] = v;
+    s += v * v;
+  }
+  double bt = block_sum(s), tot;
+  if (grid_reduce(bt, partial, counter, &tot) && threadIdx.x == 0) {
+    *nrm2 = tot;
+    beta[k + 1] = sqrt(tot);
+  }
+}
+
+// Ritz vector sum_j c_j v_j over the stored Lanczos basis (sml/LANCZOS_SYM.c:195-202)
+__global__ __launch_bounds__(kBlock) void k_ritz(int64_t n, int k, const double *const *vecs,
+                                                 const double *coef, double *__restrict__ out) {
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+    double t = 0.0;
+    for (int j = 0; j <= k; j++) t += coef[j] * vecs[j][i];
+    out[i] = t;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_scale_rnorm(int64_t n, const double *nrm2,
+                                                        const double *__restrict__ x,
+                                                        double *__restrict__ y) {
+  const double sc = 1.0 / sqrt(*nrm2);
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
+    y[i] = sc * x[i];
+}
+
+__global__ __launch_bounds__(kBlock) void k_copy(int64_t n, const double *__restrict__ x,
+                                                 double *__restrict__ y) {
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
+    y[i] = x[i];
+}
+
+// CG update (CONJUGATE_GRADIENT_SYM.c:216-236): alpha = rr/pAp, x += alpha p, r -= alpha y,
+// rr_new = <r,r>; the last block records convergence in status[0..1].
+__global__ __launch_bounds__(kBlock) void k_cg_update(int64_t n, int it, double acc,
+                                                      const double *__restrict__ p,
+                                                      const double *__restrict__ y,
+                                                      double *__restrict__ x, double *__restrict__ r,
+                                                      const double *pap, double *rr, int *status,
+                                                      double *partial, unsigned *counter) {
+  if (status[0] != 0) return;
+  const double alpha = rr[it - 1] / pap[it];
+  double s = 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+    x[i] = x[i] + alpha * p[i];
+    const double rv = r[i] - alpha * y[i];
+    r[i] = rv;
+    s += rv * rv;
+  }
+  double bt = block_sum(s), tot;
+  if (grid_reduce(bt, partial, counter, &tot) && threadIdx.x == 0) {
+    rr[it] = tot;
+    status[1] = it;
+    if (tot < acc) status[0] = 1;
+  }
+}
+
+// CG direction (CONJUGATE_GRADIENT_SYM.c:257-262): p = r + (rr_it / rr_{it-1}) p
+__global__ __launch_bounds__(kBlock) void k_cg_dir(int64_t n, int it, const double *__restrict__ r,
+                                                   const double *rr, double *__restrict__ p,
+                                                   const int *status) {
+  if (status[0] != 0) return;
+  const double beta = rr[it] / rr[it - 1];
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
+    p[i] = r[i] + beta * p[i];
+}
+
+// ------------------------------------------------- lower triangle -> full CSR
+// bad[0] != 0 if a row is empty, its last entry is not the diagonal, or a column is out
+// of [0, row).  tcnt[c] counts the strictly-lower entries in column c.
+__global__ __launch_bounds__(kBlock) void k_tri_count(int32_t n, const int64_t *__restrict__ trow,
+                                                      const int32_t *__restrict__ tcol,
+                                                      int64_t *__restrict__ own,
+                                                      unsigned *__restrict__ tcnt, int *bad) {
+  const int32_t r = blockIdx.x * kBlock + threadIdx.x;
+  if (r >= n) return;
+  const int64_t b = trow[r], e = trow[r + 1];
+  own[r] = e - b;
+  if (e <= b || tcol[e - 1] != r) {
+    atomicOr(bad, 1);
+    return;
+  }
+  for (int64_t j = b; j < e - 1; j++) {
+    const int32_t c = tcol[j];
+    if (c < 0 || c >= r) {
+      atomicOr(bad, 2);
+      continue;
+    }
+    atomicAdd(&tcnt[c], 1u);
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_row_len(int32_t n, const int64_t *own,
+                                                    const unsigned *tcnt, int64_t *len) {
+  const int32_t r = blockIdx.x * kBlock + threadIdx.x;
+  if (r < n) len[r] = own[r] + tcnt[r];
+}
+
+// exclusive scan of len[0..n) into out[0..n], 1024 elements per block
+constexpr int kScanPer = 4;
+constexpr int kScanChunk = kBlock * kScanPer;
+
+__device__ int64_t block_excl_scan(int64_t v, int64_t *total) {
+  __shared__ int64_t lds[kBlock];
+  lds[threadIdx.x] = v;
+  __syncthreads();
+  for (int off = 1; off < kBlock; off <<= 1) {
+    int64_t t = (threadIdx.x >= (unsigned)off) ? lds[threadIdx.x - off] : 0;
+    __syncthreads();
+    lds[threadIdx.x] += t;
+    __syncthreads();
+  }
+  const int64_t incl = lds[threadIdx.x];
+  *total = lds[kBlock - 1];
+  __syncthreads();
+  return incl - v;
+}
+
+__global__ __launch_bounds__(kBlock) void k_scan_blocks(int32_t n, const int64_t *len, int64_t *bsum) {
+  const int64_t base = (int64_t)blockIdx.x * kScanChunk + threadIdx.x * kScanPer;
+  int64_t s = 0;
+  for (int q = 0; q < kScanPer; q++)
+    if (base + q < n) s += len[base + q];
+  int64_t tot;
+  block_excl_scan(s, &tot);
+  if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(kBlock) void k_scan_top(int32_t nb, int64_t *bsum) {
+  // single workgroup: in-place exclusive scan of nb block sums
+  __shared__ int64_t carry;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  for (int32_t base = 0; base < nb; base += kBlock) {
+    const int32_t i = base + threadIdx.x;
+    const int64_t v = (i < nb) ? bsum[i] : 0;
+    int64_t tot;
+    const int64_t ex = block_excl_scan(v, &tot);
+    if (i < nb) bsum[i] = carry + ex;
+    __syncthreads();
+    if (threadIdx.x == 0) carry += tot;
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_scan_apply(int32_t n, const int64_t *len, const int64_t *bsum,
+                                                       int64_t *out) {
+  const int64_t base = (int64_t)blockIdx.x * kScanChunk + threadIdx.x * kScanPer;
+  int64_t v[kScanPer];
+  int64_t s = 0;
+  for (int q = 0; q < kScanPer; q++) {
+    v[q] = (base + q < n) ? len[base + q] : 0;
+    s += v[q];
+  }
+  int64_t tot;
+  int64_t ex = block_excl_scan(s, &tot) + bsum[blockIdx.x];
+  for (int q = 0; q < kScanPer; q++) {
+    if (base + q < n) out[base + q] = ex;
+    ex += v[q];
+  }
+  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == kBlock - 1) out[n] = ex;
+}
+
+__global__ __launch_bounds__(kBlock) void k_tri_fill(int32_t n, const int64_t *__restrict__ trow,
+                                                     const int32_t *__restrict__ tcol,
+                                                     const double *__restrict__ tval,
+                                                     const int64_t *__restrict__ frow,
+                                                     const int64_t *__restrict__ own,
+                                                     unsigned *__restrict__ tfill,
+                                                     int32_t *__restrict__ fcol,
+                                                     double *__restrict__ fval) {
+  const int32_t r = blockIdx.x * kBlock + threadIdx.x;
+  if (r >= n) return;
+  const int64_t b = trow[r], e = trow[r + 1];
+  const int64_t dst = frow[r];
+  for (int64_t j = b; j < e; j++) {
+    fcol[dst + (j - b)] = tcol[j];
+    fval[dst + (j - b)] = tval[j];
+  }
+  for (int64_t j = b; j < e - 1; j++) {
+    const int32_t c = tcol[j];
+    const int64_t pos = frow[c] + own[c] + atomicAdd(&tfill[c], 1u);
+    fcol[pos] = r;
+    fval[pos] = tval[j];
+  }
+}
+
+// the transposed part of each row was filled in arrival order: sort it by column
+__global__ __launch_bounds__(kBlock) void k_sort_tail(int32_t n, const int64_t *__restrict__ frow,
+                                                      const int64_t *__restrict__ own,
+                                                      int32_t *__restrict__ fcol,
+                                                      double *__restrict__ fval) {
+  const int32_t r = blockIdx.x * kBlock + threadIdx.x;
+  if (r >= n) return;
+  const int64_t b = frow[r] + own[r], e = frow[r + 1];
+  for (int64_t i = b + 1; i < e; i++) {
+    const int32_t kc = fcol[i];
+    const double kv = fval[i];
+    int64_t j = i - 1;
+    while (j >= b && fcol[j] > kc) {
+      fcol[j + 1] = fcol[j];
+      fval[j + 1] = fval[j];
+      j--;
+    }
+    fcol[j + 1] = kc;
+    fval[j + 1] = kv;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_to_fp32(int64_t nnz, const double *__restrict__ v,
+                                                    float *__restrict__ f) {
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < nnz; i += (int64_t)gridDim.x * kBlock)
+    f[i] = (float)v[i];
+}
+
+// Start vector: splitmix64 of a counter, integers in [-5e7, 5e7) as the
+// rand()%100000000 - 50000000 of sml/LANCZOS_SLOW_SYM.c:82 (seeded, reproducible).
+__device__ __forceinline__ uint64_t splitmix64(uint64_t z) {
+  z += 0x9E3779B97F4A7C15ULL;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  return z ^ (z >> 31);
+}
+
+__global__ __launch_bounds__(kBlock) void k_start_vector(int64_t n, uint64_t seed, double *__restrict__ out) {
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+    const uint64_t h = splitmix64(seed * 0xD1B54A32D192ED03ULL + (uint64_t)i);
+    out[i] = (double)(int64_t)(h % 100000000ULL) - 50000000.0;
+  }
+}
+
+inline int vec_grid(int64_t n) {
+  int64_t b = (n + kBlock - 1) / kBlock;
+  return (int)(b < kVecGrid ? (b > 0 ? b : 1) : kVecGrid);
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ launchers
+void Reduce::ensure(size_t blocks) {
+  if (partial.count < blocks) partial.resize(blocks);
+  if (!counter.p) {
+    counter.resize(1);
+    HIPCHK(hipMemsetAsync(counter.p, 0, sizeof(unsigned), ctx().stream));
+  }
+}
+
+int pick_lanes(int32_t n, int64_t nnz) {
+  const double avg = n > 0 ? (double)nnz / n : 0.0;
+  int g = 4;
+  while (g < 64 && g * 3 < avg) g <<= 1;
+  return g;
+}
+
+int spmv_blocks(int32_t n, int lanes) {
+  const int rpb = kBlock / lanes;
+  return (int)((n + rpb - 1) / rpb);
+}
+
+void launch_spmv(const Operator &op, const double *x, double *y, hipStream_t s) {
+  apply_op(op, false, x, EpiPlain{y}, nullptr, s);
+}
+
+void launch_norm2(int64_t n, const double *x, Reduce &R, double *out, hipStream_t s) {
+  const int g = vec_grid(n);
+  R.ensure(g);
+  k_norm2<<<g, kBlock, 0, s>>>(n, x, out, R.partial.p, R.counter.p);
+  HIPCHK(hipGetLastError());
+}
+
+void launch_lanczos_a(const Operator &op, bool fp32, int k, const double *w, const double *nrm2,
+                      double *vk, double *y, double *alpha, Reduce &R, hipStream_t s) {
+  apply_op(op, fp32, w, EpiLanczosA{w, nrm2, vk, y, alpha, k, 0.0}, &R, s);
+}
+
+void launch_lanczos_b(int64_t n, int k, const double *y, const double *vk, const double *vkm1,
+                      const double *alpha, double *beta, double *w, double *nrm2, Reduce &R,
+                      hipStream_t s) {
+  const int g = vec_grid(n);
+  R.ensure(g);
+  k_lanczos_b<<<g, kBlock, 0, s>>>(n, k, y, vk, vkm1 ? vkm1 : vk, alpha, beta, w, nrm2, R.partial.p,
+                                   R.counter.p);
+  HIPCHK(hipGetLastError());
+}
+
+void launch_ritz(int64_t n, int k, const double *const *vecs_dev, const double *coef_dev, double *out,
+                 hipStream_t s) {
+  k_ritz<<<vec_grid(n), kBlock, 0, s>>>(n, k, vecs_dev, coef_dev, out);
+  HIPCHK(hipGetLastError());
+}
+
+void launch_scale_by_rnorm(int64_t n, const double *nrm2, const double *x, double *y, hipStream_t s) {
+  k_scale_rnorm<<<vec_grid(n), kBlock, 0, s>>>(n, nrm2, x, y);
+  HIPCHK(hipGetLastError());
+}
+
+void launch_copy(int64_t n, const double *x, double *y, hipStream_t s) {
+  k_copy<<<vec_grid(n), kBlock, 0, s>>>(n, x, y);
+  HIPCHK(hipGetLastError());
+}
+
+void launch_residual_l1(const Operator &op, const double *v, double eig, double *out, Reduce &R,
+                        hipStream_t s) {
+  apply_op(op, false, v, EpiResidualL1{v, out, eig}, &R, s);
+}
+
+void launch_cg_init(const Operator &op, double shift, const double *nrm2, const double *x_in,
+                    double *x_out, const double *b, double *r, double *p, double *rr0, Reduce &R,
+                    hipStream_t s) {
+  apply_op(op, false, x_in, EpiCgInit{nrm2, x_in, b, x_out, r, p, rr0, shift, 0.0}, &R, s);
+}
+
+void launch_cg_ap(const Operator &op, double shift, const double *p, double *y, double *pap,
+                  const int *done, Reduce &R, hipStream_t s) {
+  apply_op(op, false, p, EpiCgAp{p, y, pap, done, shift}, &R, s);
+}
+
+void launch_cg_update(int64_t n, int it, double acc, const double *p, const double *y, double *x,
+                      double *r, const double *pap, double *rr, int *status, Reduce &R,
+                      hipStream_t s) {
+  const int g = vec_grid(n);
+  R.ensure(g);
+  k_cg_update<<<g, kBlock, 0, s>>>(n, it, acc, p, y, x, r, pap, rr, status, R.partial.p, R.counter.p);
+  HIPCHK(hipGetLastError());
+}
+
+void launch_cg_dir(int64_t n, int it, const double *r, const double *rr, double *p, const int *status,
+                   hipStream_t s) {
+  k_cg_dir<<<vec_grid(n), kBlock, 0, s>>>(n, it, r, rr, p, status);
+  HIPCHK(hipGetLastError());
+}
+
+void launch_exclusive_scan(int32_t n, const int64_t *len, int64_t *out, hipStream_t s) {
+  const int sb = (n + kScanChunk - 1) / kScanChunk;
+  if (sb == 0) {
+    HIPCHK(hipMemsetAsync(out, 0, sizeof(int64_t), s));
+    return;
+  }
+  DevBuf<int64_t> bsum(sb);
+  k_scan_blocks<<<sb, kBlock, 0, s>>>(n, len, bsum.p);
+  k_scan_top<<<1, kBlock, 0, s>>>(sb, bsum.p);
+  k_scan_apply<<<sb, kBlock, 0, s>>>(n, len, bsum.p, out);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(s));
+}
+
+void launch_start_vector(int64_t n, uint64_t seed, double *out, hipStream_t s) {
+  k_start_vector<<<vec_grid(n), kBlock, 0, s>>>(n, seed, out);
+  HIPCHK(hipGetLastError());
+}
+
+void launch_to_fp32(const dmrgx_matrix &A, hipStream_t s) {
+  k_to_fp32<<<vec_grid(A.nnz), kBlock, 0, s>>>(A.nnz, A.val.p, A.val32.p);
+  HIPCHK(hipGetLastError());
+}
+
+void launch_expand_lower(int32_t n, int64_t nnz_tri, const int64_t *trow, const int32_t *tcol,
+                         const double *tval, dmrgx_matrix &A, hipStream_t s) {
+  (void)nnz_tri;
+  const int nb = (n + kBlock - 1) / kBlock;
+  DevBuf<int64_t> own(n), len(n);
+  DevBuf<unsigned> tcnt(n), tfill(n);
+  DevBuf<int> bad(1);
+  HIPCHK(hipMemsetAsync(tcnt.p, 0, sizeof(unsigned) * n, s));
+  HIPCHK(hipMemsetAsync(tfill.p, 0, sizeof(unsigned) * n, s));
+  HIPCHK(hipMemsetAsync(bad.p, 0, sizeof(int), s));
+  k_tri_count<<<nb, kBlock, 0, s>>>(n, trow, tcol, own.p, tcnt.p, bad.p);
+  HIPCHK(hipGetLastError());
+  int hbad = 0;
+  HIPCHK(hipMemcpyAsync(&hbad, bad.p, sizeof(int), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  if (hbad & 1) throw Failure(DMRGX_ERR_NODIAG, "a row has no diagonal element as its last entry");
+  if (hbad & 2) throw Failure(DMRGX_ERR_ARG, "lower-triangle CRS has a column outside [0,row)");
+  k_row_len<<<nb, kBlock, 0, s>>>(n, own.p, tcnt.p, len.p);
+  HIPCHK(hipGetLastError());
+  A.row.resize((size_t)n + 1);
+  const int sb = (n + kScanChunk - 1) / kScanChunk;
+  DevBuf<int64_t> bsum(sb > 0 ? sb : 1);
+  k_scan_blocks<<<sb, kBlock, 0, s>>>(n, len.p, bsum.p);
+  k_scan_top<<<1, kBlock, 0, s>>>(sb, bsum.p);
+  k_scan_apply<<<sb, kBlock, 0, s>>>(n, len.p, bsum.p, A.row.p);
+  HIPCHK(hipGetLastError());
+  int64_t nnz = 0;
+  HIPCHK(hipMemcpyAsync(&nnz, A.row.p + n, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  A.n = n;
+  A.nnz = nnz;
+  A.col.resize(nnz);
+  A.val.resize(nnz);
+  k_tri_fill<<<nb, kBlock, 0, s>>>(n, trow, tcol, tval, A.row.p, own.p, tfill.p, A.col.p, A.val.p);
+  k_sort_tail<<<nb, kBlock, 0, s>>>(n, A.row.p, own.p, A.col.p, A.val.p);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(s));
+}
+
+}  // namespace dmrgx

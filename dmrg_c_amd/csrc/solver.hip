@@ -1,0 +1,243 @@
+// Device-resident ground-state drivers: Lanczos (stored basis, host tridiagonal checks
+// every chunk of steps), inverse iteration with CG, convergence check.
+//
+// Algorithms follow sml/LANCZOS_SLOW_SYM.c:57-274, sml/LANCZOS_SYM.c:58-258,
+// sml/INVERSE_ITERATION_SYM.c:9-166, sml/CONJUGATE_GRADIENT_SYM.c:8-123 and
+// sml/CONVERGE_CHECK_SYM.c:7-44.  Differences, all without effect on the converged result:
+//  * the Lanczos vectors are kept in HBM, so the eigenvector is one pass over them
+//    instead of the reference's replay of the whole recurrence (LANCZOS_SLOW_SYM.c:193-261,
+//    which recomputes the same vectors);
+//  * alpha/beta are read back once per chunk of steps; steps past the converged one are
+//    computed and discarded;
+//  * the lowest Ritz value comes from Sturm bisection (tridiag.cpp) instead of dstev_;
+//  * the inverse-iteration shift is applied inside the operator (as
+//    dmrg/DMRG_CONJUGATE_GRADIENT_OBC.c:29,40 does) instead of DIAG_ADD_CRS1 on the matrix;
+//  * below n = 1000, where the reference calls dense dsyev_ (LANCZOS_SLOW_SYM.c:38), the
+//    Lanczos run stops on the residual bound beta_{k+1}|s_k| <= 1e-12 max(1,|E|) (or on an
+//    invariant subspace), which gives the extremal eigenvalue to working precision.
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <limits>
+
+#include "internal.h"
+
+namespace dmrgx {
+
+namespace {
+
+// Lanczos basis stored in chunks of 32 vectors, allocated as the recurrence grows.
+struct Basis {
+  int64_t n;
+  static constexpr int kPer = 32;
+  std::vector<DevBuf<double>> chunks;
+  explicit Basis(int64_t n_) : n(n_) {}
+  double *vec(int k) {
+    const size_t c = (size_t)k / kPer;
+    while (chunks.size() <= c) chunks.emplace_back((size_t)n * kPer);
+    return chunks[c].p + (size_t)(k % kPer) * n;
+  }
+};
+
+void ritz(Basis &V, int k, const std::vector<double> &s, double *out, hipStream_t st) {
+  std::vector<const double *> ptrs(k + 1);
+  for (int j = 0; j <= k; j++) ptrs[j] = V.vec(j);
+  DevBuf<const double *> dptr(k + 1);
+  DevBuf<double> dcoef(k + 1);
+  HIPCHK(hipMemcpyAsync(dptr.p, ptrs.data(), sizeof(double *) * (k + 1), hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(dcoef.p, s.data(), sizeof(double) * (k + 1), hipMemcpyHostToDevice, st));
+  launch_ritz(V.n, k, dptr.p, dcoef.p, out, st);
+  HIPCHK(hipStreamSynchronize(st));  // host vectors above go out of scope
+}
+
+double read_scalar(const double *d, hipStream_t st) {
+  double *h = ctx().host_stage(1);
+  HIPCHK(hipMemcpyAsync(h, d, sizeof(double), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  return h[0];
+}
+
+}  // namespace
+
+void lanczos(const Operator &A, const LanczosOpts &o, double *vec, SolveOut &out) {
+  Context &C = ctx();
+  hipStream_t st = C.stream;
+  const int64_t n = A.n;
+  const bool small = o.dense_small && n < 1000;
+  const int max_step = o.max_step;
+  const int block = (o.max_block_step > 0 && o.max_block_step < max_step) ? o.max_block_step : max_step;
+  const bool restarts = (block < max_step);
+
+  DevBuf<double> w(n), y(n);
+  DevBuf<double> sc(2 * ((size_t)block + 2) + 1);
+  double *alpha = sc.p, *beta = sc.p + block + 2, *nrm2 = sc.p + 2 * (block + 2);
+  Reduce R;
+  Basis V(n);
+  std::vector<double> ha(block + 2), hb(block + 2), s(block + 2);
+
+  double E = 1e15;  // sml/LANCZOS_SYM.c:77
+  int total_step = 0;
+  launch_copy(n, vec, w.p, st);
+  launch_norm2(n, w.p, R, nrm2, st);
+
+  while (true) {
+    // step 0 of this block: sml/LANCZOS_SLOW_SYM.c:98-106
+    launch_lanczos_a(A, o.fp32, 0, w.p, nrm2, V.vec(0), y.p, alpha, R, st);
+    launch_lanczos_b(n, 0, y.p, V.vec(0), nullptr, alpha, beta, w.p, nrm2, R, st);
+    out.spmv++;
+    int next = 1, kconv = -1, chunk = small ? 16 : 8;
+    double lam = E;
+    bool block_done = false;
+    while (next < block && total_step < max_step) {
+      const int c = std::min({chunk, block - next, max_step - total_step});
+      for (int k = next; k < next + c; k++) {
+        launch_lanczos_a(A, o.fp32, k, w.p, nrm2, V.vec(k), y.p, alpha, R, st);
+        launch_lanczos_b(n, k, y.p, V.vec(k), V.vec(k - 1), alpha, beta, w.p, nrm2, R, st);
+      }
+      double *h = C.host_stage(2 * ((size_t)block + 2));
+      HIPCHK(hipMemcpyAsync(h, alpha, sizeof(double) * (next + c), hipMemcpyDeviceToHost, st));
+      HIPCHK(hipMemcpyAsync(h + block + 2, beta, sizeof(double) * (next + c + 1), hipMemcpyDeviceToHost, st));
+      HIPCHK(hipStreamSynchronize(st));
+      std::copy(h, h + next + c, ha.begin());
+      std::copy(h + block + 2, h + block + 2 + next + c + 1, hb.begin());
+      for (int k = next; k < next + c; k++) {
+        total_step++;
+        out.spmv++;
+        const bool check = small || k >= o.min_step || (restarts && k == block - 1);
+        if (!check) continue;
+        // T_{k+1}: diag ha[0..k], off hb[1..k]
+        lam = tridiag_lowest_value(k + 1, ha.data(), hb.data() + 1,
+                                   std::numeric_limits<double>::infinity());
+        bool conv;
+        const double bnext = hb[k + 1];
+        const double tscale = std::max(1.0, std::fabs(lam));
+        if (small) {
+          // dense-branch equivalent: residual bound of the Ritz pair, or invariant subspace
+          tridiag_vector(k + 1, ha.data(), hb.data() + 1, lam, s.data());
+          conv = !(bnext > 1e-12 * tscale) || bnext * std::fabs(s[k]) <= 1e-12 * tscale;
+        } else if (!(bnext > 1e-14 * tscale)) {
+          conv = true;  // invariant subspace: the Ritz values are exact
+        } else if (o.type == 1) {
+          tridiag_vector(k + 1, ha.data(), hb.data() + 1, lam, s.data());
+          conv = std::fabs(s[k]) < o.acc;     // "Tight": sml/LANCZOS_SLOW_SYM.c:140-146
+        } else {
+          conv = std::fabs(E - lam) < o.acc;  // "Normal": sml/LANCZOS_SLOW_SYM.c:147-153
+        }
+        if (conv) {
+          kconv = k;
+          break;
+        }
+        if (restarts && k == block - 1) {
+          block_done = true;
+          break;
+        }
+        E = lam;
+      }
+      if (kconv >= 0 || block_done) break;
+      next += c;
+      chunk = std::min(chunk * 2, 32);
+    }
+    if (kconv >= 0) {
+      tridiag_vector(kconv + 1, ha.data(), hb.data() + 1, lam, s.data());
+      ritz(V, kconv, s, vec, st);
+      launch_norm2(n, vec, R, nrm2, st);
+      launch_scale_by_rnorm(n, nrm2, vec, vec, st);
+      HIPCHK(hipStreamSynchronize(st));
+      out.eig = lam;
+      out.lan_steps = total_step;
+      out.lan_converged = true;
+      return;
+    }
+    if (!restarts || total_step >= max_step) break;
+    // restart from the Ritz vector of this block (sml/LANCZOS_SYM.c:249-256)
+    const int kb = block - 1;
+    tridiag_vector(kb + 1, ha.data(), hb.data() + 1, lam, s.data());
+    ritz(V, kb, s, w.p, st);
+    launch_norm2(n, w.p, R, nrm2, st);
+  }
+  out.lan_steps = total_step;
+  out.lan_converged = false;
+}
+
+double converge_check(const Operator &A, const double *v, double eig) {
+  Context &C = ctx();
+  Reduce R;
+  DevBuf<double> res(1);
+  launch_residual_l1(A, v, eig, res.p, R, C.stream);
+  return read_scalar(res.p, C.stream);
+}
+
+int cg_solve(const Operator &A, double shift, const double *b, double *x, double acc, int max_step,
+             bool *converged, SolveOut &out) {
+  Context &C = ctx();
+  hipStream_t st = C.stream;
+  const int64_t n = A.n;
+  DevBuf<double> r(n), p(n), y(n), xw(n);
+  const int slots = std::max(max_step, 1) + 1;
+  DevBuf<double> sc(2 * (size_t)slots + 1);
+  double *rr = sc.p, *pap = sc.p + slots, *nrm2 = sc.p + 2 * slots;
+  DevBuf<int> status(2);
+  HIPCHK(hipMemsetAsync(status.p, 0, 2 * sizeof(int), st));
+  Reduce R;
+  launch_norm2(n, x, R, nrm2, st);
+  launch_cg_init(A, shift, nrm2, x, xw.p, b, r.p, p.p, rr, R, st);
+  out.spmv++;
+  int it = 1, chunk = 4, done_it = -1;
+  int *hs = reinterpret_cast<int *>(C.host_stage(1));
+  while (it < max_step) {
+    const int c = std::min(chunk, max_step - it);
+    for (int i = it; i < it + c; i++) {
+      launch_cg_ap(A, shift, p.p, y.p, pap + i, status.p, R, st);
+      launch_cg_update(n, i, acc, p.p, y.p, xw.p, r.p, pap, rr, status.p, R, st);
+      launch_cg_dir(n, i, r.p, rr, p.p, status.p, st);
+    }
+    HIPCHK(hipMemcpyAsync(hs, status.p, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    if (hs[0]) {
+      done_it = hs[1];
+      break;
+    }
+    it += c;
+    chunk = std::min(chunk * 2, 64);
+  }
+  launch_copy(n, xw.p, x, st);
+  HIPCHK(hipStreamSynchronize(st));
+  *converged = done_it >= 0;
+  const int iters = (done_it >= 0) ? done_it : max_step;
+  out.spmv += (done_it >= 0) ? done_it : std::max(max_step - 1, 0);
+  return iters;
+}
+
+void inverse_iteration(const Operator &A, double eig, double *vec, double ii_acc, double ii_diag_add,
+                       int ii_max_step, double cg_acc, int cg_max_step, SolveOut &out) {
+  Context &C = ctx();
+  hipStream_t st = C.stream;
+  const int64_t n = A.n;
+  const double shift = ii_diag_add - eig;  // sml/INVERSE_ITERATION_SYM.c:24
+  DevBuf<double> un(n), nrm2(1);
+  Reduce R;
+  launch_copy(n, vec, un.p, st);
+  double err = 1000.0;
+  int iter;
+  out.ii_converged = false;
+  for (iter = 0; iter <= ii_max_step; iter++) {
+    err = converge_check(A, vec, eig);
+    out.spmv++;
+    if (err < ii_acc) {
+      out.ii_converged = true;
+      break;
+    }
+    bool conv = false;
+    out.cg_steps += cg_solve(A, shift, vec, un.p, cg_acc, cg_max_step, &conv, out);
+    if (!conv) out.cg_all_converged = false;
+    launch_norm2(n, un.p, R, nrm2.p, st);
+    launch_scale_by_rnorm(n, nrm2.p, un.p, un.p, st);
+    launch_copy(n, un.p, vec, st);
+  }
+  HIPCHK(hipStreamSynchronize(st));
+  out.err = err;
+  out.ii_steps = iter;
+}
+
+}  // namespace dmrgx

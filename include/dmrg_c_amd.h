@@ -1,0 +1,473 @@
+/*
+ * dmrg_c_amd.h -- C ABI of libdmrg_c_amd.so, the MI355X (gfx950) superblock eigensolve.
+ *
+ * Part 1 declares the reference's own entry points for the hot path, with the
+ * reference's signatures and struct layouts, so a dmrg_C program links against this
+ * library instead of libsml.a / libdmrg.a / libexact.a for these symbols and runs the
+ * eigensolve on the GPU.  Each declaration cites the reference prototype it replaces.
+ *
+ * Part 2 declares new entry points (prefix dmrgx_) that return status codes instead of
+ * calling exit(), take device-resident data, and expose the forms the reference lacks
+ * (device assembly from a recorded term list, precision / batching options).
+ *
+ * Struct layouts are field-for-field those of include/SML.h, include/dmrg.h and
+ * include/exact.h so the same objects can be passed.  If the reference headers were
+ * included first, their definitions are used.
+ */
+#ifndef DMRG_C_AMD_H
+#define DMRG_C_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ===================================================================== types */
+#ifndef SML_h
+/* include/SML.h:17-28 */
+typedef struct {
+   long max_val;
+   int max_row;
+   int row_dim;
+   int col_dim;
+   int *Col;
+   long *Row;
+   double *Val;
+} CRS1;
+
+/* include/SML.h:56-65 */
+typedef struct {
+   CRS1 *M;
+   double acc;
+   double *Vec;
+   double *Out_Vec;
+   int max_step;
+   int p_threads;
+} BOX_CG;
+
+/* include/SML.h:67-86 */
+typedef struct {
+   CRS1 *M;
+   double ii_acc;
+   double ii_diag_add;
+   int ii_max_step;
+   double cg_acc;
+   int cg_max_step;
+   double error;
+   double *Eig_Vec;
+   double *Eig_Val;
+   int p_threads;
+   char Type[100];
+} BOX_II;
+
+/* include/SML.h:122-135 */
+typedef struct {
+   CRS1 *M;
+   double *Eig_Vec;
+   double *eig_val;
+   double acc;
+   int min_step;
+   int max_step;
+   int max_block_step;
+   int p_threads;
+   char Guess[100];
+   char Type[100];
+} BOX_LAN;
+#endif /* SML_h */
+
+#ifndef dmrg_h
+/* include/dmrg.h:6-38 */
+typedef struct {
+   char Initial_Guess[1000];
+   char Enviro_Copy[1000];
+   char II_Type[1000];
+   char Lan_Con[1000];
+   int max_dim_system;
+   int sweep;
+   char Diag_Method[1000];
+   double diag_acc;
+   int diag_min_step;
+   int diag_max_step;
+   double cg_acc;
+   int cg_max_step;
+   double inv_iter_acc;
+   double inv_iter_diag_add;
+   int inv_iter_max_step;
+   double sp_LL;
+   int param_iter;
+   int param_iter_now;
+   int dim_LLLRRRRL_limit;
+} DMRG_PARAMETER;
+
+/* include/dmrg.h:40-51 */
+typedef struct {
+   double diag;
+   double inv_iter;
+   double make_ham;
+   double make_basis;
+   double make_dens_mat;
+   double trans_main;
+   double trans_exp;
+   double total;
+} DMRG_TIME;
+
+/* include/dmrg.h:54-99 */
+typedef struct {
+   short *LL_LLLRRRRL;
+   short *LR_LLLRRRRL;
+   short *RL_LLLRRRRL;
+   short *RR_LLLRRRRL;
+   int *Inv_LLLRRRRL;
+   int dim_LLLRRRRL;
+   int dim_LL;
+   int dim_RR;
+   int dim_onsite;
+   int tot_sz_LLLRRRRL;
+   int tot_ele_LLLRRRRL;
+   int tot_ele_1_LLLRRRRL;
+   int tot_ele_2_LLLRRRRL;
+   int tot_parity_LLLRRRRL;
+   short *LL_LLLR;
+   short *LR_LLLR;
+   int **Inv_LLLR;
+   int dim_LLLR;
+   short *LR_LRRL;
+   short *RL_LRRL;
+   int **Inv_LRRL;
+   int dim_LRRL;
+   short *RR_RRRL;
+   short *RL_RRRL;
+   int **Inv_RRRL;
+   int dim_RRRL;
+   int *Sum_LLLR;
+   int *Tot_Sz_LLLR;
+   int *Tot_Ele_LLLR;
+   int *Tot_Ele_1_LLLR;
+   int *Tot_Ele_2_LLLR;
+   int *Tot_Parity_LLLR;
+} DMRG_BASIS;
+
+/* include/dmrg.h:128-138 */
+typedef struct {
+   short *LL_LLLRRRRL;
+   short *LR_LLLRRRRL;
+   short *RL_LLLRRRRL;
+   short *RR_LLLRRRRL;
+   int *Inv_LLLRRRRL;
+   double *Val_LLLRRRRL;
+   int elem_num;
+} DMRG_A_BASIS;
+
+/* include/dmrg.h:198-208 */
+typedef struct {
+   int LL;
+   int LR;
+   int RR;
+   int RL;
+   int row;
+   int dim_RR;
+   int dim_onsite;
+} DMRG_BASIS_ONSITE;
+
+/* include/dmrg.h:236-267 */
+typedef struct {
+   char BC[1000];
+   char Enviro_Copy[100];
+   int LL_site;
+   int RR_site;
+   int max_dim_system;
+   int dim_LLLRRRRL;
+   int dim_LL;
+   int dim_onsite;
+   int dim_RR;
+   int spin;
+   int tot_sz;
+   int tot_ele;
+   int tot_ele_1;
+   int tot_ele_2;
+   int tot_parity;
+   double *GS_Vec;
+   double gs_val;
+   double gs_error;
+   double percent_LL;
+   int sweep_now;
+   int param_iter_now;
+   int tot_iter_now;
+   int sweep;
+   int param_iter;
+   int tot_iter;
+} DMRG_STATUS;
+
+/* include/dmrg.h:269-285 */
+typedef struct {
+   double *Eig_Vec;
+   double *eig_val;
+   double acc;
+   int min_step;
+   int max_step;
+   int p_threads;
+   char Guess[100];
+   char Type[100];
+   CRS1 *M_LLLR;
+   CRS1 *M_LRRL;
+   CRS1 *M_LRRL_Sign;
+   CRS1 *M_RRRL;
+   int *Ele_RR;
+   DMRG_BASIS *Dmrg_Basis;
+} DMRG_BOX_LAN;
+
+/* include/dmrg.h:287-302 */
+typedef struct {
+   double acc;
+   double diag_val;
+   double *Vec;
+   double *Out_Vec;
+   int max_step;
+   int p_threads;
+   CRS1 *M_LLLR;
+   CRS1 *M_LRRL;
+   CRS1 *M_LRRL_Sign;
+   CRS1 *M_RRRL;
+   int *Ele_RR;
+   DMRG_BASIS *Dmrg_Basis;
+} DMRG_BOX_CG;
+
+/* include/dmrg.h:304-326 */
+typedef struct {
+   double ii_acc;
+   double ii_diag_add;
+   int ii_max_step;
+   double cg_acc;
+   int cg_max_step;
+   double *Eig_Vec;
+   double *eig_val;
+   double error;
+   int p_threads;
+   CRS1 *M_LLLR;
+   CRS1 *M_LRRL;
+   CRS1 *M_LRRL_Sign;
+   CRS1 *M_RRRL;
+   int *Ele_RR;
+   DMRG_BASIS *Dmrg_Basis;
+} DMRG_BOX_II;
+#endif /* dmrg_h */
+
+#ifndef exact_h
+/* include/exact.h:14-37 */
+typedef struct {
+   int diag_num;
+   int est_max_row_elem_num;
+   char Diag_Method[1000];
+   double diag_acc;
+   int diag_min_step;
+   int diag_max_step;
+   double cg_acc;
+   int cg_max_step;
+   double inv_iter_acc;
+   double inv_iter_diag_add;
+   int inv_iter_max_step;
+   int param_iter;
+   int param_iter_now;
+} EXACT_PARAMETER;
+
+/* include/exact.h:39-51 */
+typedef struct {
+   double diag;
+   double inv_iter;
+   double make_ham;
+   double find_dim;
+   double make_basis;
+   double exp_values;
+   double cf;
+   double sc;
+   double total;
+} EXACT_TIME;
+
+/* include/exact.h:54-64 */
+typedef struct {
+   int dim;
+   int diag_num;
+   double **Vector;
+   double *Value;
+   double *Error;
+   double mem_ham;
+   CRS1 *Ham;
+} EXACT_HAM_INFO;
+
+/* include/exact.h:66-74 */
+typedef struct {
+   int dim;
+   long *Basis;
+   int tot_sz;
+   int tot_ele;
+   int tot_parity;
+} EXACT_BASIS_INFO;
+#endif /* exact_h */
+
+/* ========================================== Part 1: reference entry points */
+/* Semantics follow the reference: void return, exit(1) with the reference's message
+ * on the conditions it treats as fatal, warnings + ./SML_out/error.txt for the soft
+ * ones, step counts appended to ./SML_out/<SOLVER>_Step.txt (set DMRGX_SML_OUT=0 to
+ * skip the files).  Work runs on the current HIP device; p_threads is accepted and
+ * ignored.  Guess == "No" starts from a counter-hash random vector seeded by
+ * dmrgx_set_seed() (default 1) instead of srand(time(NULL)). */
+
+/* include/SML.h:291 -- y = M x, M lower-triangle CRS with the diagonal stored last */
+void MATRIX_VECTOR_PRODUCT_SYM(CRS1 *M, double *V1, double *Out, double **Temp, int p_threads);
+/* include/SML.h:290 -- y = M x, full CRS */
+void MATRIX_VECTOR_PRODUCT(CRS1 *M, double *V1, double *Out, int p_threads);
+/* include/SML.h:270,272 */
+void LANCZOS_SLOW_SYM(BOX_LAN *Box);
+void LANCZOS_SYM(BOX_LAN *Box);
+/* include/SML.h:269,271 (full CRS) */
+void LANCZOS_SLOW(BOX_LAN *Box);
+void LANCZOS(BOX_LAN *Box);
+/* include/SML.h:266-267 */
+void INVERSE_ITERATION(BOX_II *Box_II);
+void INVERSE_ITERATION_SYM(BOX_II *Box_II);
+/* include/SML.h:172-173 */
+void CONJUGATE_GRADIENT(BOX_CG *Box);
+void CONJUGATE_GRADIENT_SYM(BOX_CG *Box);
+/* include/SML.h:174-175 */
+double CONVERGE_CHECK(CRS1 *M, double *Eig_Vec, double eig_val, int p_threads);
+double CONVERGE_CHECK_SYM(CRS1 *M, double *Eig_Vec, double eig_val, int p_threads);
+
+/* include/dmrg.h:406 and :353 */
+void DMRG_DIAGONALIZE_SUPERBLOCK_SYM(CRS1 *Ham, DMRG_TIME *Time, DMRG_PARAMETER *Param, DMRG_STATUS *Dmrg_Status, int p_threads);
+void DMRG_DIAGONALIZE_SUPERBLOCK(CRS1 *Ham, DMRG_TIME *Time, DMRG_PARAMETER *Param, DMRG_STATUS *Dmrg_Status, int p_threads);
+
+/* include/dmrg.h:401-405 -- matrix-free OBC superblock operator and its solvers */
+void DMRG_MATRIX_VECTOR_PRODUCT_OBC(CRS1 *M_LLLR, CRS1 *M_LRRL, CRS1 *M_LRRL_Sign, CRS1 *M_RRRL, double *V, double *Out_V, int *Ele_RR, int p_threads, DMRG_BASIS *Dmrg_Basis);
+void DMRG_LANCZOS_SLOW_OBC(DMRG_BOX_LAN *Box);
+void DMRG_CONJUGATE_GRADIENT_OBC(DMRG_BOX_CG *Box);
+void DMRG_INVERSE_ITERATION_OBC(DMRG_BOX_II *Box_II);
+double DMRG_CONVERGE_CHECK_OBC(double eig_val, CRS1 *M_LLLR, CRS1 *M_LRRL, CRS1 *M_LRRL_Sign, CRS1 *M_RRRL, double *Eig_V, double *Temp_V, int *Ele_RR, int p_threads, DMRG_BASIS *Dmrg_Basis);
+
+/* include/dmrg.h:330-340 -- superblock element makers.  Outside a recording they
+ * behave as the reference (accumulate one row into A_Basis on the host).  Between
+ * dmrgx_record_begin() and dmrgx_record_end() they only append to the term list. */
+void DMRG_MAKE_ELEM_LL_LLLRRRRL(DMRG_BASIS_ONSITE *Basis, DMRG_A_BASIS *A_Basis, int *Inv_LLLRRRRL, CRS1 *M, double coeef, int *elem_num);
+void DMRG_MAKE_ELEM_LR_LLLRRRRL(DMRG_BASIS_ONSITE *Basis, DMRG_A_BASIS *A_Basis, int *Inv_LLLRRRRL, CRS1 *M, double coeef, int *elem_num);
+void DMRG_MAKE_ELEM_RR_LLLRRRRL(DMRG_BASIS_ONSITE *Basis, DMRG_A_BASIS *A_Basis, int *Inv_LLLRRRRL, CRS1 *M, double coeef, int *elem_num);
+void DMRG_MAKE_ELEM_RL_LLLRRRRL(DMRG_BASIS_ONSITE *Basis, DMRG_A_BASIS *A_Basis, int *Inv_LLLRRRRL, CRS1 *M, double coeef, int *elem_num);
+void DMRG_MAKE_ELEM_LLLR_LLLRRRRL(DMRG_BASIS_ONSITE *Basis, DMRG_A_BASIS *A_Basis, int *Inv_LLLRRRRL, CRS1 *M_LL, CRS1 *M_LR, double coeef, int *LL_Ele, int *elem_num, char Type[], char Sign_Flag[]);
+void DMRG_MAKE_ELEM_LLRL_LLLRRRRL(DMRG_BASIS_ONSITE *Basis, DMRG_A_BASIS *A_Basis, int *Inv_LLLRRRRL, CRS1 *M_LL, CRS1 *M_RL, double coeef, int *Ele_LL, int *Ele_RR, int *Ele_LR, int *elem_num, char Type[], char Sign_Flag[]);
+void DMRG_MAKE_ELEM_LLRR_LLLRRRRL(DMRG_BASIS_ONSITE *Basis, DMRG_A_BASIS *A_Basis, int *Inv_LLLRRRRL, CRS1 *M_LL, CRS1 *M_RR, double coeef, int *Ele_LL, int *Ele_LR, int *elem_num, char Type[], char Sign_Flag[]);
+void DMRG_MAKE_ELEM_LRRL_LLLRRRRL(DMRG_BASIS_ONSITE *Basis, DMRG_A_BASIS *A_Basis, int *Inv_LLLRRRRL, CRS1 *M_LR, CRS1 *M_RL, double coeef, int *Ele_RR, int *Ele_LR, int *elem_num, char Type[], char Sign_Flag[]);
+void DMRG_MAKE_ELEM_LRRR_LLLRRRRL(DMRG_BASIS_ONSITE *Basis, DMRG_A_BASIS *A_Basis, int *Inv_LLLRRRRL, CRS1 *M_LR, CRS1 *M_RR, double coeef, int *Ele_LR, int *elem_num, char Type[], char Sign_Flag[]);
+void DMRG_MAKE_ELEM_RRRL_LLLRRRRL(DMRG_BASIS_ONSITE *Basis, DMRG_A_BASIS *A_Basis, int *Inv_LLLRRRRL, CRS1 *M_RR, CRS1 *M_RL, double coeef, int *Ele_RR, int *elem_num, char Type[], char Sign_Flag[]);
+void DMRG_MAKE_ELEM_ZERO_LLLRRRRL(DMRG_BASIS_ONSITE *Basis, DMRG_A_BASIS *A_Basis, int *Inv_LLLRRRRL, int *elem_num);
+
+/* ================================================= Part 2: dmrgx_ entry points */
+/* Status codes */
+#define DMRGX_OK              0
+#define DMRGX_ERR_ARG        -1   /* invalid argument (sizes, strings, NULL)         */
+#define DMRGX_ERR_HIP        -2   /* HIP runtime error (message in dmrgx_last_error) */
+#define DMRGX_ERR_NOCONV     -3   /* Lanczos did not converge within max_step        */
+#define DMRGX_ERR_NODIAG     -4   /* a row of the matrix has no diagonal element     */
+#define DMRGX_ERR_NOMEM      -5   /* device allocation failed                        */
+#define DMRGX_ERR_NOGPU      -6   /* no usable HIP device                            */
+
+const char *dmrgx_last_error(void);
+int dmrgx_init(int device);
+void dmrgx_set_seed(uint64_t seed);
+/* Same splitmix64 counter hash as the Guess == "No" start vector: values are integers
+ * in [-5e7, 5e7), the range of rand()%100000000 - 50000000 (sml/LANCZOS_SLOW_SYM.c:82). */
+void dmrgx_start_vector(int64_t n, uint64_t seed, double *out);
+
+/* ---- device matrix (full CSR, columns ascending, resident in HBM) */
+typedef struct dmrgx_matrix dmrgx_matrix;
+/* Upload a host CRS1.  sym != 0: M is the lower triangle incl. diagonal (the layout
+ * GET_HAM_LLLRRRRL produces) and is expanded to the full matrix on the device. */
+int dmrgx_matrix_from_crs1(const CRS1 *M, int sym, dmrgx_matrix **out);
+int dmrgx_matrix_from_arrays(int32_t n, const int64_t *row, const int32_t *col, const double *val,
+                             int sym, dmrgx_matrix **out);
+void dmrgx_matrix_free(dmrgx_matrix *A);
+int64_t dmrgx_matrix_nnz(const dmrgx_matrix *A);
+int32_t dmrgx_matrix_dim(const dmrgx_matrix *A);
+/* Copy the full CSR back to the host (row: n+1, col/val: nnz). */
+int dmrgx_matrix_download(const dmrgx_matrix *A, int64_t *row, int32_t *col, double *val);
+/* y = A x with device pointers (x, y of length n), on the library stream. */
+int dmrgx_spmv_device(const dmrgx_matrix *A, const double *x_dev, double *y_dev);
+/* y = A x with host pointers. */
+int dmrgx_spmv_host(const dmrgx_matrix *A, const double *x, double *y);
+
+/* ---- ground-state solve (Lanczos -> inverse iteration with CG), all on the device */
+typedef struct {
+   double diag_acc;         /* Lanczos convergence threshold                     */
+   int diag_min_step;
+   int diag_max_step;
+   int lan_type;            /* 0 "Normal" |E_k - E_k-1| < acc, 1 "Tight" |s_k| < acc */
+   int guess;               /* 0: counter-hash start vector (seed), 1: vec_inout    */
+   uint64_t seed;
+   double ii_acc;           /* inverse iteration: stop when sum|Hv - Ev| < ii_acc */
+   double ii_diag_add;
+   int ii_max_step;
+   double cg_acc;           /* CG: stop when ||r||^2 < cg_acc                      */
+   int cg_max_step;
+   int skip_ii;             /* 1: Lanczos only                                    */
+   int precision;           /* 0: FP64; 1: FP32 CRS values + FP32 SpMV in Lanczos,
+                                FP64 recurrence scalars and FP64 II/CG refinement  */
+} dmrgx_solve_params;
+
+typedef struct {
+   double eig_val;
+   double gs_error;         /* sum |Hv - Ev| after inverse iteration               */
+   int lanczos_steps;
+   int ii_steps;
+   int cg_steps;
+   int spmv_calls;
+   double seconds;
+} dmrgx_solve_stats;
+
+void dmrgx_default_params(dmrgx_solve_params *p);
+/* vec_inout: host array of length n; start vector when guess == 1, ground state out. */
+int dmrgx_ground_state(const dmrgx_matrix *A, const dmrgx_solve_params *p, double *vec_inout,
+                       dmrgx_solve_stats *stats);
+
+
+/* ---- superblock assembly on the device from a recorded term list
+ * Recording: between dmrgx_record_begin() and dmrgx_record_end() the DMRG_MAKE_ELEM_*
+ * symbols of this library append (kind, operators, coefficient, sign rule, parity arrays)
+ * to a term list instead of computing; run the model's MAKE_ELEMENT_HAM_LLLRRRRL once
+ * (any row) to capture it.  The recorded pointers must stay valid until assembly. */
+typedef struct dmrgx_terms dmrgx_terms;
+typedef struct dmrgx_superblock dmrgx_superblock;
+int dmrgx_record_begin(void);
+dmrgx_terms *dmrgx_record_end(void);
+void dmrgx_terms_free(dmrgx_terms *t);
+int dmrgx_terms_count(const dmrgx_terms *t);
+
+/* Term of the array form.  kind: 0 LL, 1 LR, 2 RR, 3 RL, 4 LLLR, 5 LLRL, 6 LLRR, 7 LRRL,
+ * 8 LRRR, 9 RRRL, 10 ZERO (the element makers of include/dmrg.h:330-340, in that order).
+ * op1/op2 index ops[]; ele0..2 index eles[] in the argument order of the element maker
+ * (e.g. LRRL: Ele_RR, Ele_LR); sign_mode 0 = Sign_Flag "No", 1 = first Type ("LL_LR",
+ * "LL_RL", "LL_RR", "LR_RL", "LR_RR", "RR_RL"), 2 = the reversed Type. */
+typedef struct {
+   int32_t kind, op1, op2, ele0, ele1, ele2, sign_mode, reserved;
+   double coeff;
+} dmrgx_term;
+
+/* Upload a superblock (basis from GET_BASIS_SUPERBLOCK, operators, term list). */
+int dmrgx_superblock_create(const DMRG_BASIS *B, const dmrgx_terms *T, dmrgx_superblock **out);
+int dmrgx_superblock_create_arrays(int32_t n, int32_t dim_LL, int32_t dim_RR, int32_t dim_onsite,
+                                   const int16_t *LL, const int16_t *LR, const int16_t *RR, const int16_t *RL,
+                                   int32_t n_ops, const CRS1 *ops, int32_t n_eles, const int32_t *const *eles,
+                                   const int32_t *ele_len, int32_t n_terms, const dmrgx_term *terms,
+                                   dmrgx_superblock **out);
+/* Assemble on the device: the lower-triangle CRS of GET_HAM_LLLRRRRL (kept in the handle,
+ * nnz in *nnz_lower) and, if full_out != NULL, its full symmetric expansion ready to solve. */
+int dmrgx_superblock_assemble(dmrgx_superblock *S, int64_t *nnz_lower, dmrgx_matrix **full_out);
+int dmrgx_superblock_download_lower(const dmrgx_superblock *S, int64_t *row, int32_t *col, double *val);
+void dmrgx_superblock_free(dmrgx_superblock *S);
+/* Drop-in body of GET_HAM_LLLRRRRL (main/dmrg/<model>_vf/src/allocations/GET_HAM_LLLRRRRL.c:11):
+ * assembles on the device, returns a malloc'ed CRS1 (free with FREE_CRS1) holding the
+ * reference-layout lower triangle, and keeps the expanded device copy for the next
+ * DMRG_DIAGONALIZE_SUPERBLOCK_SYM call on that CRS1 (no re-upload).  exit(1) on error. */
+CRS1 *dmrgx_get_ham_lllrrrrl(const DMRG_BASIS *B, const dmrgx_terms *T);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DMRG_C_AMD_H */
