@@ -1,0 +1,253 @@
+#!/usr/bin/env python3
+"""Ground-state solves/s of the superblock eigensolve on one MI355X.
+
+Workload: the Hubbard superblock of the reference's own DMRG run at L=64, N=64, S^z=0,
+t=1 (t1=-1), U=4, h=0.1, m=400 (last infinite-system step, 64 sites, n=144542 rows),
+stored as tests/golden/hubbard_L64_m400_s31.npz (basis, block operators, term list and the
+reference's ground-state energy).  Inputs are uploaded to HBM before timing.
+
+One step = one ground-state solve as GET_GROUND_STATE does it: device assembly of the
+superblock CRS (bit-identical to GET_HAM_LLLRRRRL), expansion to the full matrix, Lanczos
+(reference defaults of main/dmrg/hubbard_vf/src/main/main.c: "Lanczos", "Normal",
+diag_acc 1e-14), inverse iteration with CG (cg_acc 1e-7, ii_acc 1e-9), vector to host.
+
+Reported beside it:
+* roofline of the dominant kernel (CSR SpMV inside Lanczos/CG), timed live with HIP events
+  on the library's stream during the timed steps; algorithmic bytes per launch
+  nnz*(8+4) + (n+1)*8 + 2*n*8; MI355X HBM peak 8 TB/s;
+* traffic: HBM bytes per SpMV launch from the committed rocprofv3 --pmc summary;
+* cpu_baseline: the reference's DMRG_DIAGONALIZE_SUPERBLOCK_SYM (oracle/_ref/libdmrgref.so,
+  built from the reference sources, OpenMP) on the same superblock, assembled on the host by
+  the oracle; eigensolve only (the reference's assembly time is not counted).
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, REPO)
+WORKLOAD = os.path.join(REPO, "tests", "golden", "hubbard_L64_m400_s31.npz")
+TRAFFIC = os.path.join(REPO, "profiles", "spmv_hubbard_L64_m400_traffic.json")
+PEAK_HBM_GBS = 8000.0
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def gpu_solve(D, sb, params):
+    A = sb.assemble(full=True)
+    E, vec, st = A.ground_state(**params)
+    nnz = A.nnz
+    A.free()
+    return E, st, nnz
+
+
+def cpu_baseline(snap, e_ref, budget_s):
+    """The reference's own eigensolve on the host cores (bounded sample)."""
+    import dmrg_c_amd as D
+    sys.path.insert(0, os.path.join(REPO, "tests"))
+    import oracle_py as O
+
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    ref = os.path.join(REPO, "oracle", "_ref", "libdmrgref.so")
+    row, col, val = O.assemble(snap)
+    kind, times, energies = None, [], []
+    if os.path.exists(ref):
+        os.environ.setdefault("MKL_THREADING_LAYER", "GNU")
+        try:
+            R = C.CDLL(ref)
+            R.DMRG_DIAGONALIZE_SUPERBLOCK_SYM.argtypes = [C.POINTER(D.CRS1), C.POINTER(D.DMRG_TIME),
+                                                          C.POINTER(D.DMRG_PARAMETER), C.POINTER(D.DMRG_STATUS),
+                                                          C.c_int]
+            kind = "reference"
+        except OSError as e:
+            log(f"cpu baseline: cannot load {ref}: {e}; using the oracle port")
+    old = os.getcwd()
+    os.chdir(tempfile.mkdtemp(prefix="cpu_baseline_"))  # the reference appends to ./SML_out
+    # the reference prints progress on C stdout: keep it off the JSON line
+    sys.stdout.flush()
+    saved_fd = os.dup(1)
+    sink = os.open(os.devnull, os.O_WRONLY)
+    os.dup2(sink, 1)
+    try:
+        t_start = None
+        warm = 1
+        while True:
+            if kind == "reference":
+                H = D.HostCRS(row, col, val)
+                P = D.default_dmrg_parameter()
+                st = D.DMRG_STATUS()
+                vec = np.zeros(snap.n)
+                st.GS_Vec = vec.ctypes.data_as(C.POINTER(C.c_double))
+                t0 = time.perf_counter()
+                R.DMRG_DIAGONALIZE_SUPERBLOCK_SYM(H.ptr, C.byref(D.DMRG_TIME()), C.byref(P), C.byref(st), threads)
+                times.append(time.perf_counter() - t0)
+                energies.append(st.gs_val)
+            else:
+                kind, threads = "port", 1
+                t0 = time.perf_counter()
+                rc, e, err, vec, ls, cs = O.diagonalize_superblock_sym(row, col, val, O.start_vector(snap.n, 1))
+                times.append(time.perf_counter() - t0)
+                energies.append(e)
+            if warm:  # first solve pays page faults and library start-up: not counted
+                warm = 0
+                times.pop()
+                energies.pop()
+                t_start = time.perf_counter()
+                continue
+            log(f"cpu baseline solve {len(times)}: {times[-1]:.2f} s  E={energies[-1]:.15f}")
+            if time.perf_counter() - t_start > budget_s or len(times) >= 10:
+                break
+    finally:
+        C.CDLL(None).fflush(None)
+        os.dup2(saved_fd, 1)
+        os.close(saved_fd)
+        os.close(sink)
+        os.chdir(old)
+    med = float(np.median(times))
+    return {
+        "value": round(1.0 / med, 4),
+        "unit": "solves/s",
+        "cores": threads,
+        "kind": kind,
+        "sample": (f"{len(times)} solve(s) of the same n={snap.n} superblock after 1 untimed, median {med:.2f} s; "
+                   + ("DMRG_DIAGONALIZE_SUPERBLOCK_SYM of the reference (LANCZOS_SYM + INVERSE_ITERATION_SYM, "
+                      "-Ofast -fopenmp, MKL dstev_) on the oracle-assembled CRS; assembly not counted"
+                      if kind == "reference" else "oracle port of the eigensolve, 1 thread; assembly not counted")),
+        "max_rel_energy_error": float(max(abs(e - e_ref) / abs(e_ref) for e in energies)),
+    }
+
+
+def traffic_probe(D, L, sb, snap):
+    L.dmrgx_device_alloc.argtypes = [C.c_int64, C.POINTER(C.c_void_p)]
+    L.dmrgx_device_free.argtypes = [C.c_void_p]
+    L.dmrgx_copy_device.argtypes = [C.c_void_p, C.c_void_p, C.c_int64]
+    A = sb.assemble(full=True)
+    n, ncopy = snap.n, 1 << 26
+    bufs = []
+    for nbytes in (8 * n, 8 * n, 8 * ncopy, 8 * ncopy):
+        p = C.c_void_p()
+        D.check(L.dmrgx_device_alloc(nbytes, C.byref(p)))
+        bufs.append(p)
+    x, y, src, dst = bufs
+    D.check(L.dmrgx_copy_device(src, dst, ncopy))  # first touch
+    for _ in range(5):
+        D.check(L.dmrgx_spmv_device(A.handle, x, y))
+    D.check(L.dmrgx_copy_device(src, dst, ncopy))
+    alg = A.nnz * 12 + (n + 1) * 8 + 2 * n * 8
+    print(json.dumps({"spmv_launches": 5, "n": n, "nnz": A.nnz, "spmv_alg_bytes": alg,
+                      "copy_doubles": ncopy, "copy_bytes_read": 8 * ncopy, "copy_bytes_written": 8 * ncopy}))
+    for p in bufs:
+        L.dmrgx_device_free(p)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU baseline work")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--traffic-probe", action="store_true",
+                    help="run 5 plain SpMV launches on the workload and one copy kernel of 2^26 doubles "
+                         "(traffic calibration under rocprofv3 --pmc), then exit")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: the superblock eigensolve runs on a single MI355X "
+              f"(--gpus {args.gpus} is not supported; multi-GPU sharding is a design note only)", file=sys.stderr)
+        sys.exit(2)
+    os.environ.setdefault("DMRGX_SML_OUT", "0")
+    import dmrg_c_amd as D
+    from dmrg_c_amd import snapshot
+
+    L = D.lib()
+    D.check(L.dmrgx_init(0))
+    snap = snapshot.load(WORKLOAD)
+    e_ref = float(snap.meta["gs_energy"][0])
+    params = dict(diag_acc=1e-14, diag_max_step=800, cg_acc=1e-7, cg_max_step=2000, ii_acc=1e-9,
+                  ii_diag_add=1e-11, ii_max_step=3, seed=1)
+    sb = D.Superblock(snap)  # inputs resident in HBM from here on
+    if args.traffic_probe:
+        traffic_probe(D, L, sb, snap)
+        return
+    log(f"workload: Hubbard L=64 m=400 superblock n={snap.n}")
+    for i in range(args.warmup):
+        E, st, nnz = gpu_solve(D, sb, params)
+        log(f"warmup {i}: E={E:.15f} lanczos={st.lanczos_steps} cg={st.cg_steps}")
+
+    L.dmrgx_profile_begin()
+    lat, errs, lan, cg = [], [], [], []
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        ts = time.perf_counter()
+        E, st, nnz = gpu_solve(D, sb, params)
+        lat.append(time.perf_counter() - ts)
+        errs.append(abs(E - e_ref) / abs(e_ref))
+        lan.append(st.lanczos_steps)
+        cg.append(st.cg_steps)
+        log(f"step {i}: {lat[-1] * 1e3:.2f} ms  E={E:.15f}  dE/E={errs[-1]:.2e}")
+    t1 = time.perf_counter()
+    spmv_ms, launches, alg_bytes = C.c_double(), C.c_int64(), C.c_double()
+    L.dmrgx_profile_end.argtypes = [C.POINTER(C.c_double), C.POINTER(C.c_int64), C.POINTER(C.c_double)]
+    D.check(L.dmrgx_profile_end(C.byref(spmv_ms), C.byref(launches), C.byref(alg_bytes)))
+
+    total = t1 - t0
+    avg_launch_s = spmv_ms.value / 1e3 / max(launches.value, 1)
+    bytes_per_launch = alg_bytes.value / max(launches.value, 1)
+    achieved = bytes_per_launch / avg_launch_s / 1e9 if avg_launch_s > 0 else 0.0
+    traffic = None
+    if os.path.exists(TRAFFIC):
+        with open(TRAFFIC) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+    line = {
+        "metric": "ground-state solves/s + p99 latency at |dE/E|<=1e-9; SpMV HBM GB/s vs peak",
+        "value": round(args.steps / total, 4),
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(total / args.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "superblock snapshot from the reference's own DMRG run (tests/golden/hubbard_L64_m400_s31.npz)",
+        "config": {
+            "workload": "Hubbard OBC L=64 N=64 Sz=0 t=1 U=4 h=0.1 m=400, superblock n=144542: "
+                        "device assembly + Lanczos + inverse iteration/CG per solve",
+            "n": snap.n,
+            "nnz_full": int(nnz),
+            "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 3),
+            "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 3),
+            "max_rel_energy_error": float(max(errs)),
+            "lanczos_steps": int(np.median(lan)),
+            "cg_steps": int(np.median(cg)),
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "k_apply<CsrRows<double>> (CSR SpMV + fused Lanczos/CG epilogue)",
+            "achieved": round(achieved, 1),
+            "peak": PEAK_HBM_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / PEAK_HBM_GBS, 4),
+            "traffic": traffic,
+            "alg_bytes_per_launch": int(bytes_per_launch),
+            "avg_launch_us": round(avg_launch_s * 1e6, 2),
+            "launches": int(launches.value),
+        },
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline(snap, e_ref, args.cpu_budget)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

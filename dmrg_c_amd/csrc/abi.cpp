@@ -499,4 +499,48 @@ int dmrgx_ground_state(const dmrgx_matrix *A, const dmrgx_solve_params *p, doubl
   });
 }
 
+int dmrgx_profile_begin(void) {
+  return run_status([&] {
+    Context &C = ctx();
+    HIPCHK(hipStreamSynchronize(C.stream));
+    C.prof = true;
+    C.prof_used = 0;
+    C.prof_bytes = 0.0;
+  });
+}
+
+int dmrgx_profile_end(double *spmv_ms, int64_t *spmv_launches, double *spmv_alg_bytes) {
+  return run_status([&] {
+    Context &C = ctx();
+    HIPCHK(hipStreamSynchronize(C.stream));
+    double tot = 0.0;
+    for (size_t i = 0; i < C.prof_used; i++) {
+      float ms = 0.f;
+      HIPCHK(hipEventElapsedTime(&ms, C.prof_ev[i].first, C.prof_ev[i].second));
+      tot += ms;
+    }
+    if (spmv_ms) *spmv_ms = tot;
+    if (spmv_launches) *spmv_launches = (int64_t)C.prof_used;
+    if (spmv_alg_bytes) *spmv_alg_bytes = C.prof_bytes;
+    C.prof = false;
+    C.prof_used = 0;
+    C.prof_bytes = 0.0;
+  });
+}
+
+int dmrgx_copy_device(const double *src, double *dst, int64_t count) {
+  return run_status([&] {
+    launch_copy(count, src, dst, ctx().stream);
+    HIPCHK(hipStreamSynchronize(ctx().stream));
+  });
+}
+
+int dmrgx_device_alloc(int64_t bytes, void **out) {
+  return run_status([&] { HIPCHK(hipMalloc(out, (size_t)bytes)); });
+}
+
+int dmrgx_device_free(void *p) {
+  return run_status([&] { HIPCHK(hipFree(p)); });
+}
+
 }  // extern "C"

@@ -94,6 +94,16 @@ void Context::release(void *p, size_t) {
   }
 }
 
+std::pair<hipEvent_t, hipEvent_t> *Context::prof_slot() {
+  if (prof_used == prof_ev.size()) {
+    std::pair<hipEvent_t, hipEvent_t> p;
+    HIPCHK(hipEventCreate(&p.first));
+    HIPCHK(hipEventCreate(&p.second));
+    prof_ev.push_back(p);
+  }
+  return &prof_ev[prof_used++];
+}
+
 double *Context::host_stage(size_t count) {
   if (count > pinned_cap) {
     if (pinned) (void)hipHostFree(pinned);

@@ -47,6 +47,12 @@ struct Context {
   void *alloc(size_t bytes);
   void release(void *p, size_t bytes);
   double *host_stage(size_t count);
+  // live kernel timing of the SpMV launches (dmrgx_profile_begin/end)
+  bool prof = false;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> prof_ev;
+  size_t prof_used = 0;
+  double prof_bytes = 0.0;
+  std::pair<hipEvent_t, hipEvent_t> *prof_slot();
   void sizes_note(void *p, size_t bytes) { live[p] = bytes; }
   size_t sizes_take(void *p) {
     auto it = live.find(p);
@@ -113,6 +119,8 @@ struct ObcRows {
   const int32_t *c1, *c2, *c3, *c4;
   const double *v1, *v2, *v3, *v4;
   const int32_t *ele_RR;
+  static constexpr bool kTimed = false;
+  double alg_bytes(int32_t) const { return 0.0; }
   template <int G>
   __device__ __forceinline__ double lane_sum(int32_t r, int lane, const double *__restrict__ x) const {
     const int ll = LL[r], lr = LR[r], rr = RR[r], rl = RL[r];

@@ -71,6 +71,12 @@ struct CsrRows {
   const int64_t *__restrict__ row;
   const int32_t *__restrict__ col;
   const VT *__restrict__ val;
+  int64_t nnz;
+  static constexpr bool kTimed = true;
+  // algorithmic bytes of one y = H x: values + columns + row pointers + x once + y once
+  double alg_bytes(int32_t n) const {
+    return (double)nnz * (sizeof(VT) + 4) + (double)(n + 1) * 8 + 2.0 * n * 8;
+  }
   template <int G>
   __device__ __forceinline__ double lane_sum(int32_t r, int lane, const double *__restrict__ x) const {
     double s = 0.0;
@@ -192,6 +198,13 @@ void apply_rows(int32_t n, int lanes, const Rows &rows, const double *x, const E
     part = R->partial.p;
     cnt = R->counter.p;
   }
+  Context &C = ctx();
+  std::pair<hipEvent_t, hipEvent_t> *ev = nullptr;
+  if (C.prof && Rows::kTimed) {
+    ev = C.prof_slot();
+    HIPCHK(hipEventRecord(ev->first, s));
+    C.prof_bytes += rows.alg_bytes(n);
+  }
   switch (lanes) {
 #define L_CASE(G) \
   case G: k_apply<G, Rows, Epi><<<blocks, kBlock, 0, s>>>(n, rows, x, epi, part, cnt); break;
@@ -200,6 +213,7 @@ void apply_rows(int32_t n, int lanes, const Rows &rows, const double *x, const E
     default: throw Failure(DMRGX_ERR_ARG, "bad lanes per row");
   }
   HIPCHK(hipGetLastError());
+  if (ev) HIPCHK(hipEventRecord(ev->second, s));
 }
 
 template <class Epi>
@@ -208,9 +222,9 @@ void apply_op(const Operator &op, bool fp32, const double *x, const Epi &epi, Re
     apply_rows(op.n, op.lanes, op.obc->rows(), x, epi, R, s);
   } else if (fp32) {
     if (!op.A->val32.p) throw Failure(DMRGX_ERR_ARG, "FP32 values not prepared");
-    apply_rows(op.n, op.lanes, CsrRows<float>{op.A->row.p, op.A->col.p, op.A->val32.p}, x, epi, R, s);
+    apply_rows(op.n, op.lanes, CsrRows<float>{op.A->row.p, op.A->col.p, op.A->val32.p, op.A->nnz}, x, epi, R, s);
   } else {
-    apply_rows(op.n, op.lanes, CsrRows<double>{op.A->row.p, op.A->col.p, op.A->val.p}, x, epi, R, s);
+    apply_rows(op.n, op.lanes, CsrRows<double>{op.A->row.p, op.A->col.p, op.A->val.p, op.A->nnz}, x, epi, R, s);
   }
 }
 

@@ -467,6 +467,18 @@ void dmrgx_superblock_free(dmrgx_superblock *S);
  * DMRG_DIAGONALIZE_SUPERBLOCK_SYM call on that CRS1 (no re-upload).  exit(1) on error. */
 CRS1 *dmrgx_get_ham_lllrrrrl(const DMRG_BASIS *B, const dmrgx_terms *T);
 
+/* ---- measurement
+ * Between begin and end, every CSR SpMV launch of the library is bracketed by HIP events
+ * on the library's stream; end returns the summed kernel time, the launch count and the
+ * algorithmic bytes (nnz*(sizeof(val)+4) + (n+1)*8 + 2*n*8 per launch). */
+int dmrgx_profile_begin(void);
+int dmrgx_profile_end(double *spmv_ms, int64_t *spmv_launches, double *spmv_alg_bytes);
+/* Plain streaming copy of count doubles (device pointers): traffic calibration for
+ * rocprofv3 FETCH_SIZE / WRITE_SIZE. */
+int dmrgx_copy_device(const double *src, double *dst, int64_t count);
+int dmrgx_device_alloc(int64_t bytes, void **out);
+int dmrgx_device_free(void *p);
+
 #ifdef __cplusplus
 }
 #endif
