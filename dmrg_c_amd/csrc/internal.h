@@ -162,6 +162,31 @@ struct ObcRows {
   }
 };
 
+// One operator acting on one slot (LL, LR, RR or RL) of the superblock state,
+// dmrg/DMRG_V_M_LL_Q0.c:14-29 and its three siblings.
+struct SlotRows {
+  int32_t slot, dim_RR, d;
+  const int16_t *LL, *LR, *RR, *RL;
+  const int32_t *inv;
+  const int64_t *row;
+  const int32_t *col;
+  const double *val;
+  static constexpr bool kTimed = false;
+  double alg_bytes(int32_t) const { return 0.0; }
+  template <int G>
+  __device__ __forceinline__ double lane_sum(int32_t r, int lane, const double *__restrict__ x) const {
+    int q[4] = {LL[r], LR[r], RR[r], RL[r]};
+    const int own = q[slot];
+    double s = 0.0;
+    for (int64_t j = row[own] + lane; j < row[own + 1]; j += G) {
+      q[slot] = col[j];
+      const int32_t iv = inv[(((int64_t)q[0] * d + q[1]) * dim_RR + q[2]) * d + q[3]];
+      if (iv >= 0) s += x[iv] * val[j];
+    }
+    return s;
+  }
+};
+
 // Device arrays of the matrix-free operator (uploaded from DMRG_BASIS + the three
 // sub-Hamiltonians, obc.cpp).
 struct ObcDev {
@@ -229,6 +254,11 @@ void launch_cg_update(int64_t n, int it, double acc, const double *p, const doub
                       hipStream_t s);
 void launch_cg_dir(int64_t n, int it, const double *r, const double *rr, double *p, const int *status,
                    hipStream_t s);
+
+// single-slot operator products: y = O x (y may be null) and, if dot_with != null,
+// *dot_out = <dot_with, O x> (deterministic reduction)
+void launch_slot_apply(int32_t n, const SlotRows &rows, const double *x, double *y, const double *dot_with,
+                       double *dot_out, Reduce &R, hipStream_t s);
 
 // ---- host tridiagonal solver (tridiag.cpp)
 // lowest eigenvalue of T(a[0..k-1], b[0..k-2]); lam_hint: an upper bound or +inf

@@ -186,6 +186,23 @@ struct EpiResidualL1 {
   __device__ void finish(double t) { *out = t; }
 };
 
+// y[r] = (O x)_r (if y) and sum_r u[r] (O x)_r (if u)
+struct EpiStoreDot {
+  double *y;
+  const double *u;
+  double *out;
+  static constexpr bool kReduce = true;
+  __device__ bool skip() const { return false; }
+  __device__ void begin() {}
+  __device__ double row(int32_t r, double s) {
+    if (y) y[r] = s;
+    return u ? u[r] * s : 0.0;
+  }
+  __device__ void finish(double t) {
+    if (out) *out = t;
+  }
+};
+
 template <class Rows, class Epi>
 void apply_rows(int32_t n, int lanes, const Rows &rows, const double *x, const Epi &epi, Reduce *R,
                 hipStream_t s) {
@@ -515,6 +532,11 @@ int spmv_blocks(int32_t n, int lanes) {
 
 void launch_spmv(const Operator &op, const double *x, double *y, hipStream_t s) {
   apply_op(op, false, x, EpiPlain{y}, nullptr, s);
+}
+
+void launch_slot_apply(int32_t n, const SlotRows &rows, const double *x, double *y, const double *dot_with,
+                       double *dot_out, Reduce &R, hipStream_t s) {
+  apply_rows(n, 8, rows, x, EpiStoreDot{y, dot_with, dot_out}, &R, s);
 }
 
 void launch_norm2(int64_t n, const double *x, Reduce &R, double *out, hipStream_t s) {

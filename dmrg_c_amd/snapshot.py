@@ -211,3 +211,71 @@ def load_gs_bin(path: str):
         e, err = (float(v) for v in _read(f, "<f8", 2))
         vec = _read(f, "<f8", n)
     return e, err, vec
+
+
+@dataclass
+class ExpectationCall:
+    """Inputs and outputs of one DMRG_EXPECTATION_ONSITE (kind 0) or
+    DMRG_EXPECTATION_INTERSITE_Q0 (kind 1) call (dmrg/DMRG_EXPECTATION_ONSITE.c:3-28,
+    dmrg/DMRG_EXPECTATION_INTERSITE_Q0.c:3-43)."""
+    kind: int
+    n: int
+    dim_LL: int
+    dim_RR: int
+    dim_onsite: int
+    LL_site: int
+    RR_site: int
+    origin: int
+    LL: np.ndarray
+    LR: np.ndarray
+    RR: np.ndarray
+    RL: np.ndarray
+    M_LL: List[Crs]
+    M_On: Crs
+    M_RR: List[Crs]
+    M_CF: List[Crs]
+    vec: np.ndarray
+    out: np.ndarray
+
+
+def load_expect_bin(path: str) -> ExpectationCall:
+    with open(path, "rb") as f:
+        if f.read(8) != b"DMRGEXP1":
+            raise ValueError(f"{path}: not a DMRGEXP1 file")
+        kind, n, dl, dr, d, lls, rrs, origin = (int(v) for v in _read(f, "<i4", 8))
+        LL, LR, RR, RL = (_read(f, "<i2", n) for _ in range(4))
+        mll = [_read_crs(f) for _ in range(int(_read(f, "<i4", 1)[0]))]
+        mon = _read_crs(f)
+        mrr = [_read_crs(f) for _ in range(int(_read(f, "<i4", 1)[0]))]
+        mcf = [_read_crs(f) for _ in range(int(_read(f, "<i4", 1)[0]))]
+        vec = _read(f, "<f8", n)
+        out = _read(f, "<f8", int(_read(f, "<i4", 1)[0]))
+    return ExpectationCall(kind, n, dl, dr, d, lls, rrs, origin, LL, LR, RR, RL, mll, mon, mrr, mcf, vec, out)
+
+
+def save_expect_npz(e: ExpectationCall, path: str) -> None:
+    out: Dict[str, np.ndarray] = {
+        "header": np.array([e.kind, e.n, e.dim_LL, e.dim_RR, e.dim_onsite, e.LL_site, e.RR_site, e.origin]),
+        "LL": e.LL, "LR": e.LR, "RR": e.RR, "RL": e.RL, "vec": e.vec, "out": e.out,
+        "counts": np.array([len(e.M_LL), len(e.M_RR), len(e.M_CF)]),
+    }
+    for i, c in enumerate(e.M_LL):
+        _crs_to_dict(f"ll{i}", c, out)
+    for i, c in enumerate(e.M_RR):
+        _crs_to_dict(f"rr{i}", c, out)
+    for i, c in enumerate(e.M_CF):
+        _crs_to_dict(f"cf{i}", c, out)
+    _crs_to_dict("on", e.M_On, out)
+    np.savez_compressed(path, **out)
+
+
+def load_expect(path: str) -> ExpectationCall:
+    if path.endswith(".bin"):
+        return load_expect_bin(path)
+    z = np.load(path, allow_pickle=False)
+    kind, n, dl, dr, d, lls, rrs, origin = (int(v) for v in z["header"])
+    nll, nrr, ncf = (int(v) for v in z["counts"])
+    return ExpectationCall(kind, n, dl, dr, d, lls, rrs, origin, z["LL"], z["LR"], z["RR"], z["RL"],
+                           [_crs_from_dict(f"ll{i}", z) for i in range(nll)], _crs_from_dict("on", z),
+                           [_crs_from_dict(f"rr{i}", z) for i in range(nrr)],
+                           [_crs_from_dict(f"cf{i}", z) for i in range(ncf)], z["vec"], z["out"])

@@ -344,6 +344,18 @@ void DMRG_CONJUGATE_GRADIENT_OBC(DMRG_BOX_CG *Box);
 void DMRG_INVERSE_ITERATION_OBC(DMRG_BOX_II *Box_II);
 double DMRG_CONVERGE_CHECK_OBC(double eig_val, CRS1 *M_LLLR, CRS1 *M_LRRL, CRS1 *M_LRRL_Sign, CRS1 *M_RRRL, double *Eig_V, double *Temp_V, int *Ele_RR, int p_threads, DMRG_BASIS *Dmrg_Basis);
 
+/* include/dmrg.h:367-370 -- one operator on one slot of the superblock state, y = O x */
+void DMRG_V_M_LL_Q0(CRS1 *M_LL, double *V, double *Out_V, int dim, int p_threads, DMRG_BASIS *Dmrg_Basis);
+void DMRG_V_M_LR_Q0(CRS1 *M_LR, double *V, double *Out_V, int dim, int p_threads, DMRG_BASIS *Dmrg_Basis);
+void DMRG_V_M_RR_Q0(CRS1 *M_RR, double *V, double *Out_V, int dim, int p_threads, DMRG_BASIS *Dmrg_Basis);
+void DMRG_V_M_RL_Q0(CRS1 *M_RL, double *V, double *Out_V, int dim, int p_threads, DMRG_BASIS *Dmrg_Basis);
+/* include/dmrg.h:363-364 -- on-site values and correlation functions on the ground state */
+void DMRG_EXPECTATION_ONSITE(CRS1 **M_LL, CRS1 *M_On, CRS1 **M_RR, double *Out, double *Vec, double *Temp_V,
+                             int p_threads, DMRG_BASIS *Dmrg_Basis, DMRG_STATUS *Dmrg_Status);
+void DMRG_EXPECTATION_INTERSITE_Q0(CRS1 **M_CF, CRS1 **M_LL, CRS1 *M_On, CRS1 **M_RR, double *Out, int origin,
+                                   double *Vec, double *Temp_V1, double *Temp_V2, int p_threads,
+                                   DMRG_BASIS *Dmrg_Basis, DMRG_STATUS *Dmrg_Status);
+
 /* include/dmrg.h:330-340 -- superblock element makers.  Outside a recording they
  * behave as the reference (accumulate one row into A_Basis on the host).  Between
  * dmrgx_record_begin() and dmrgx_record_end() they only append to the term list. */

@@ -323,6 +323,62 @@ void __wrap_DMRG_DIAGONALIZE_SUPERBLOCK_SYM(CRS1 *Ham, DMRG_TIME *Time, DMRG_PAR
    }
 }
 
+/* ----------------------------------------------------- expectation values */
+/* dmrg/DMRG_EXPECTATION_ONSITE.c and DMRG_EXPECTATION_INTERSITE_Q0.c: dump inputs and
+ * the reference's outputs of every call (file expect_<step>_<k>.bin). */
+static int g_expect_calls = 0;
+
+static void dump_expect(int kind, CRS1 **M_CF, CRS1 **M_LL, CRS1 *M_On, CRS1 **M_RR, double *Out, int origin,
+                        double *Vec, DMRG_BASIS *B, DMRG_STATUS *St) {
+   char path[4096];
+   snprintf(path, sizeof(path), "%s/expect_%04d_%02d.bin", g_outdir, St->tot_iter_now, g_expect_calls++);
+   FILE *f = fopen(path, "wb");
+   if (!f) { perror(path); exit(2); }
+   int n = St->dim_LLLRRRRL, LL_site = St->LL_site, RR_site = St->RR_site;
+   fwrite("DMRGEXP1", 1, 8, f);
+   w_i32(f, kind);
+   w_i32(f, n);
+   w_i32(f, B->dim_LL);
+   w_i32(f, B->dim_RR);
+   w_i32(f, B->dim_onsite);
+   w_i32(f, LL_site);
+   w_i32(f, RR_site);
+   w_i32(f, origin);
+   fwrite(B->LL_LLLRRRRL, 2, (size_t)n, f);
+   fwrite(B->LR_LLLRRRRL, 2, (size_t)n, f);
+   fwrite(B->RR_LLLRRRRL, 2, (size_t)n, f);
+   fwrite(B->RL_LLLRRRRL, 2, (size_t)n, f);
+   w_i32(f, LL_site + 1);
+   for (int s = 0; s <= LL_site; s++) w_crs(f, M_LL[s]);
+   w_crs(f, M_On);
+   w_i32(f, RR_site + 1);
+   for (int s = 0; s <= RR_site; s++) w_crs(f, M_RR[s]);
+   int ncf = (kind == 1 && origin <= LL_site) ? LL_site - origin + 1 : 0;
+   w_i32(f, ncf);
+   for (int r = 0; r < ncf; r++) w_crs(f, M_CF[r]);
+   fwrite(Vec, 8, (size_t)n, f);
+   int nout = (kind == 0) ? LL_site + RR_site + 4 : LL_site + RR_site + 4 - origin;
+   w_i32(f, nout);
+   fwrite(Out, 8, (size_t)nout, f);
+   fclose(f);
+}
+
+void __real_DMRG_EXPECTATION_ONSITE(CRS1 **, CRS1 *, CRS1 **, double *, double *, double *, int, DMRG_BASIS *, DMRG_STATUS *);
+void __wrap_DMRG_EXPECTATION_ONSITE(CRS1 **M_LL, CRS1 *M_On, CRS1 **M_RR, double *Out, double *Vec, double *Temp_V,
+                                    int p_threads, DMRG_BASIS *B, DMRG_STATUS *St) {
+   __real_DMRG_EXPECTATION_ONSITE(M_LL, M_On, M_RR, Out, Vec, Temp_V, p_threads, B, St);
+   if (step_selected(St->tot_iter_now)) dump_expect(0, NULL, M_LL, M_On, M_RR, Out, 0, Vec, B, St);
+}
+
+void __real_DMRG_EXPECTATION_INTERSITE_Q0(CRS1 **, CRS1 **, CRS1 *, CRS1 **, double *, int, double *, double *, double *, int, DMRG_BASIS *, DMRG_STATUS *);
+void __wrap_DMRG_EXPECTATION_INTERSITE_Q0(CRS1 **M_CF, CRS1 **M_LL, CRS1 *M_On, CRS1 **M_RR, double *Out, int origin,
+                                          double *Vec, double *T1, double *T2, int p_threads, DMRG_BASIS *B,
+                                          DMRG_STATUS *St) {
+   __real_DMRG_EXPECTATION_INTERSITE_Q0(M_CF, M_LL, M_On, M_RR, Out, origin, Vec, T1, T2, p_threads, B, St);
+   if (step_selected(St->tot_iter_now) && origin <= St->LL_site + 1)
+      dump_expect(1, M_CF, M_LL, M_On, M_RR, Out, origin, Vec, B, St);
+}
+
 /* ---------------------------------------------------------------------- main */
 static double argd(int argc, char **argv, const char *key, double def) {
    size_t k = strlen(key);

@@ -555,3 +555,20 @@ void or_obc_matvec(void *ctx, const double *V, double *Out) {
       Out[b] = val;
    }
 }
+
+/* ------------------------------------------------------ expectation values */
+void or_v_m_q0(int slot, int32_t n, int32_t dim_RR, int32_t d, const int16_t *LL, const int16_t *LR,
+               const int16_t *RR, const int16_t *RL, const int32_t *inv_whole, const int64_t *row,
+               const int32_t *col, const double *val, const double *x, double *y) {
+   for (int32_t b = 0; b < n; b++) {
+      int q[4] = {LL[b], LR[b], RR[b], RL[b]};
+      const int own = q[slot];
+      double v = 0.0;
+      for (int64_t j = row[own]; j < row[own + 1]; j++) {
+         q[slot] = col[j];
+         int32_t inv = inv_whole[sb_index(q, dim_RR, d)];
+         if (inv >= 0) v = v + x[inv] * val[j];   /* DMRG_V_M_LL_Q0.c:24-26 */
+      }
+      y[b] = v;
+   }
+}

@@ -114,6 +114,13 @@ typedef struct {
 } or_obc_op;
 void or_obc_matvec(void *ctx, const double *x, double *y);
 
+/* DMRG_V_M_{LL,LR,RR,RL}_Q0 (dmrg/DMRG_V_M_LL_Q0.c:14-29 and siblings): y = O x with O
+ * acting on slot (0 LL, 1 LR, 2 RR, 3 RL).  Pinned: the reference's expectation values
+ * (DMRG_EXPECTATION_ONSITE / _INTERSITE_Q0 outputs on the same inputs). */
+void or_v_m_q0(int slot, int32_t n, int32_t dim_RR, int32_t d, const int16_t *LL, const int16_t *LR,
+               const int16_t *RR, const int16_t *RL, const int32_t *inv_whole, const int64_t *row,
+               const int32_t *col, const double *val, const double *x, double *y);
+
 #ifdef __cplusplus
 }
 #endif

@@ -18,6 +18,8 @@ def pytest_configure(config):
 def golden_snapshots(max_n=None, pattern="*_s*.npz", exclude_big=True):
     out = []
     for p in sorted(glob.glob(os.path.join(GOLDEN, pattern))):
+        if "_expect_" in p:
+            continue
         if exclude_big and "L64" in p:
             continue
         out.append(p)

@@ -8,6 +8,8 @@ and converts the superblock snapshots they dump into compact ``.npz`` files:
                        the reference's assembled lower-triangle CRS (``ham_*``),
                        and the reference's ground state (``gs_energy``, ``gs_error``,
                        ``gs_vec``) from DMRG_DIAGONALIZE_SUPERBLOCK_SYM
+  <name>_expect_s<step>_<k>.npz  one DMRG_EXPECTATION_ONSITE / _INTERSITE_Q0 call of the
+                       reference: block/on-site operators, the GS vector, the reference's output
   <name>_energies.txt  per-step: step sweep LL_site RR_site n E gs_error seconds
 
 This script is only run in the build container (the reference is not on the GPU box).
@@ -74,6 +76,14 @@ def convert(workdir: str, name: str, steps, keep_vec: bool) -> None:
         out = os.path.join(HERE, f"{name}_s{step:02d}.npz")
         snapshot.save_npz(snap, out, **extra)
         print("wrote", os.path.relpath(out, REPO), "n =", snap.n)
+    for path in sorted(glob.glob(os.path.join(workdir, "expect_*.bin"))):
+        step, k = int(os.path.basename(path)[7:11]), int(os.path.basename(path)[12:14])
+        if steps != "all" and step not in steps:
+            continue
+        e = snapshot.load_expect_bin(path)
+        out = os.path.join(HERE, f"{name}_expect_s{step:02d}_{k}.npz")
+        snapshot.save_expect_npz(e, out)
+        print("wrote", os.path.relpath(out, REPO), "kind =", e.kind)
 
 
 def main() -> None:

@@ -70,6 +70,8 @@ def lib():
                                                     C.c_int, C.c_double, C.c_int,
                                                     C.POINTER(C.c_double), C.POINTER(C.c_double), f64p,
                                                     C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        L.or_v_m_q0.argtypes = [C.c_int, C.c_int32, C.c_int32, C.c_int32, i16p, i16p, i16p, i16p, i32p, i64p,
+                                i32p, f64p, f64p, f64p]
         _LIB = L
     return _LIB
 
@@ -243,3 +245,46 @@ def obc_matvec(op, x):
     xx = np.ascontiguousarray(x, np.float64)
     f(op.ctx, xx.ctypes.data, y.ctypes.data)
     return y
+
+
+def _inv_from(e):
+    d, dr = e.dim_onsite, e.dim_RR
+    idx = ((e.LL.astype(np.int64) * d + e.LR) * dr + e.RR) * d + e.RL
+    inv = np.full(e.dim_LL * d * dr * d, -1, np.int32)
+    inv[idx] = np.arange(e.n, dtype=np.int32)
+    return inv
+
+
+def v_m_q0(e, slot, op, x, inv=None):
+    """Oracle DMRG_V_M_*_Q0 for an ExpectationCall / Snapshot-like basis holder."""
+    inv = _inv_from(e) if inv is None else inv
+    y = np.empty(e.n)
+    lib().or_v_m_q0(slot, e.n, e.dim_RR, e.dim_onsite, *(np.ascontiguousarray(a, np.int16) for a in (e.LL, e.LR, e.RR, e.RL)),
+                    inv, np.ascontiguousarray(op.row, np.int64), np.ascontiguousarray(op.col, np.int32),
+                    np.ascontiguousarray(op.val, np.float64), np.ascontiguousarray(x, np.float64), y)
+    return y
+
+
+def expectation(e, vec=None):
+    """DMRG_EXPECTATION_ONSITE / _INTERSITE_Q0 restated with the oracle's V_M (same loops)."""
+    v = e.vec if vec is None else vec
+    inv = _inv_from(e)
+    L, R, o = e.LL_site, e.RR_site, e.origin
+    if e.kind == 0:
+        out = np.zeros(L + R + 4)
+        for s in range(L + 1):
+            out[s] = np.dot(v, v_m_q0(e, 0, e.M_LL[s], v, inv))
+        out[L + 1] = np.dot(v, v_m_q0(e, 1, e.M_On, v, inv))
+        out[L + 2] = np.dot(v, v_m_q0(e, 3, e.M_On, v, inv))
+        for s in range(R, -1, -1):
+            out[R + L + 3 - s] = np.dot(v, v_m_q0(e, 2, e.M_RR[s], v, inv))
+        return out
+    out = np.zeros(L + R + 4 - o)
+    t1 = v_m_q0(e, 1, e.M_On, v, inv) if o == L + 1 else v_m_q0(e, 0, e.M_LL[o], v, inv)
+    for s in range(o, L + 1):
+        out[s - o] = np.dot(v, v_m_q0(e, 0, e.M_CF[s - o], v, inv))
+    out[L + 1 - o] = np.dot(t1, v_m_q0(e, 1, e.M_On, v, inv))
+    out[L + 2 - o] = np.dot(t1, v_m_q0(e, 3, e.M_On, v, inv))
+    for s in range(R, -1, -1):
+        out[R + L + 3 - s - o] = np.dot(t1, v_m_q0(e, 2, e.M_RR[s], v, inv))
+    return out
