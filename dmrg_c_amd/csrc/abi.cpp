@@ -15,25 +15,24 @@
 #include <cstring>
 #include <mutex>
 
-#include "internal.h"
+#include "entry.h"
 
 using namespace dmrgx;
 
 namespace dmrgx {
 const std::string &last_error();
 void launch_start_vector(int64_t n, uint64_t seed, double *out, hipStream_t s);
-}  // namespace dmrgx
 
-namespace {
+std::recursive_mutex &api_mutex() {
+  static std::recursive_mutex mu;
+  return mu;
+}
 
-std::recursive_mutex g_api_mu;
-
-bool sml_out_enabled() {
+static bool sml_out_enabled() {
   const char *e = getenv("DMRGX_SML_OUT");
   return !(e && strcmp(e, "0") == 0);
 }
 
-void sml_append(const char *file, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
 void sml_append(const char *file, const char *fmt, ...) {
   if (!sml_out_enabled()) return;
   mkdir("SML_out", 0777);
@@ -51,8 +50,7 @@ void sml_append(const char *file, const char *fmt, ...) {
   fclose(f);
 }
 
-[[noreturn]] void fatal(const char *fn, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
-[[noreturn]] void fatal(const char *fn, const char *fmt, ...) {
+void fatal(const char *fn, const char *fmt, ...) {
   printf("Error in %s\n", fn);
   va_list ap;
   va_start(ap, fmt);
@@ -66,33 +64,6 @@ double now_s() {
   return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
-// Run `body`; on a library failure print it and exit(1) (the drop-in has no status channel).
-template <class F>
-void run_or_die(const char *fn, F &&body) {
-  std::lock_guard<std::recursive_mutex> lk(g_api_mu);
-  try {
-    body();
-  } catch (const Failure &f) {
-    fatal(fn, "%s\n", f.what());
-  }
-}
-
-// Run `body` for a dmrgx_* entry point; failures become status codes.
-template <class F>
-int run_status(F &&body) {
-  std::lock_guard<std::recursive_mutex> lk(g_api_mu);
-  try {
-    body();
-    return DMRGX_OK;
-  } catch (const Failure &f) {
-    set_last_error(f.what());
-    return f.code;
-  } catch (const std::exception &e) {
-    set_last_error(e.what());
-    return DMRGX_ERR_HIP;
-  }
-}
-
 void check_square(const char *fn, const CRS1 *M) {
   if (M->row_dim != M->col_dim)
     fatal(fn, "The input matrix is not a square matirx(row=%d,col=%d)\n", M->row_dim, M->col_dim);
@@ -100,16 +71,9 @@ void check_square(const char *fn, const CRS1 *M) {
     fatal(fn, "row_dim(%d) or col_dim(%d) is illegal value\n", M->row_dim, M->col_dim);
 }
 
-struct DevVec {
-  DevBuf<double> d;
-  int64_t n;
-  explicit DevVec(int64_t n_) : d(n_), n(n_) {}
-  void up(const double *h) { HIPCHK(hipMemcpyAsync(d.p, h, sizeof(double) * n, hipMemcpyHostToDevice, ctx().stream)); }
-  void down(double *h) {
-    HIPCHK(hipMemcpyAsync(h, d.p, sizeof(double) * n, hipMemcpyDeviceToHost, ctx().stream));
-    HIPCHK(hipStreamSynchronize(ctx().stream));
-  }
-};
+}  // namespace dmrgx
+
+namespace {
 
 // Lanczos entry shared by the four LANCZOS* symbols.
 void lanczos_entry(const char *fn, const char *step_file, BOX_LAN *Box, bool sym, bool slow) {
@@ -409,7 +373,7 @@ int dmrgx_matrix_from_arrays(int32_t n, const int64_t *row, const int32_t *col, 
 }
 
 void dmrgx_matrix_free(dmrgx_matrix *A) {
-  std::lock_guard<std::recursive_mutex> lk(g_api_mu);
+  std::lock_guard<std::recursive_mutex> lk(api_mutex());
   delete A;
 }
 int64_t dmrgx_matrix_nnz(const dmrgx_matrix *A) { return A ? A->nnz : -1; }

@@ -32,7 +32,7 @@
 #include <cstring>
 #include <mutex>
 
-#include "internal.h"
+#include "entry.h"
 
 #pragma clang fp contract(off)
 
@@ -692,7 +692,7 @@ int dmrgx_superblock_create_arrays(int32_t n, int32_t dim_LL, int32_t dim_RR, in
                                    const int32_t *ele_len, int32_t n_terms, const dmrgx_term *terms,
                                    dmrgx_superblock **out) {
   if (!out || !LL || !LR || !RR || !RL || n_ops < 0 || n_eles < 0 || n_terms < 0) return DMRGX_ERR_ARG;
-  try {
+  return run_status([&] {
     std::vector<ArrOp> o(n_ops);
     for (int i = 0; i < n_ops; i++)
       o[i] = ArrOp{ops[i].row_dim, ops[i].col_dim, reinterpret_cast<const int64_t *>(ops[i].Row), ops[i].Col, ops[i].Val};
@@ -703,16 +703,12 @@ int dmrgx_superblock_create_arrays(int32_t n, int32_t dim_LL, int32_t dim_RR, in
       t[i] = DTerm{terms[i].kind, terms[i].op1, terms[i].op2, -1, -1, terms[i].sign_mode, terms[i].ele0, terms[i].ele1,
                    terms[i].ele2, 0, terms[i].coeff};
     *out = create_superblock(n, dim_LL, dim_RR, d, LL, LR, RR, RL, o, e, t);
-    return DMRGX_OK;
-  } catch (const Failure &f) {
-    set_last_error(f.what());
-    return f.code;
-  }
+  });
 }
 
 int dmrgx_superblock_create(const DMRG_BASIS *B, const dmrgx_terms *T, dmrgx_superblock **out) {
   if (!B || !T || !out) return DMRGX_ERR_ARG;
-  try {
+  {
     std::vector<const CRS1 *> opp;
     std::vector<std::pair<const int *, int>> elp;
     std::vector<dmrgx_term> terms;
@@ -758,28 +754,21 @@ int dmrgx_superblock_create(const DMRG_BASIS *B, const dmrgx_terms *T, dmrgx_sup
                                           B->LR_LLLRRRRL, B->RR_LLLRRRRL, B->RL_LLLRRRRL, (int)ops.size(),
                                           ops.data(), (int)ep.size(), ep.data(), el.data(), (int)terms.size(),
                                           terms.data(), out);
-  } catch (const Failure &f) {
-    set_last_error(f.what());
-    return f.code;
   }
 }
 
 int dmrgx_superblock_assemble(dmrgx_superblock *S, int64_t *nnz_lower, dmrgx_matrix **full_out) {
   if (!S) return DMRGX_ERR_ARG;
-  try {
+  return run_status([&] {
     const int64_t nnz = assemble_lower(*S);
     if (nnz_lower) *nnz_lower = nnz;
     if (full_out) *full_out = expand_last(*S);
-    return DMRGX_OK;
-  } catch (const Failure &f) {
-    set_last_error(f.what());
-    return f.code;
-  }
+  });
 }
 
 int dmrgx_superblock_download_lower(const dmrgx_superblock *S, int64_t *row, int32_t *col, double *val) {
   if (!S || S->lnnz < 0) return DMRGX_ERR_ARG;
-  try {
+  return run_status([&] {
     hipStream_t st = ctx().stream;
     HIPCHK(hipMemcpyAsync(row, S->lrow.p, sizeof(int64_t) * ((size_t)S->n + 1), hipMemcpyDeviceToHost, st));
     if (S->lnnz) {
@@ -787,14 +776,13 @@ int dmrgx_superblock_download_lower(const dmrgx_superblock *S, int64_t *row, int
       HIPCHK(hipMemcpyAsync(val, S->lval.p, sizeof(double) * S->lnnz, hipMemcpyDeviceToHost, st));
     }
     HIPCHK(hipStreamSynchronize(st));
-    return DMRGX_OK;
-  } catch (const Failure &f) {
-    set_last_error(f.what());
-    return f.code;
-  }
+  });
 }
 
-void dmrgx_superblock_free(dmrgx_superblock *S) { delete S; }
+void dmrgx_superblock_free(dmrgx_superblock *S) {
+  std::lock_guard<std::recursive_mutex> lk(api_mutex());
+  delete S;
+}
 
 CRS1 *dmrgx_get_ham_lllrrrrl(const DMRG_BASIS *B, const dmrgx_terms *T) {
   dmrgx_superblock *S = nullptr;

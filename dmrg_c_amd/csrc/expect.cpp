@@ -9,7 +9,7 @@
 // end.  Products and inner products use the library's deterministic reductions.
 #include <cstring>
 
-#include "internal.h"
+#include "entry.h"
 
 namespace dmrgx {
 void fill_inverse(int32_t n, int32_t dim_RR, int32_t d, const int16_t *LL, const int16_t *LR, const int16_t *RR,
@@ -80,14 +80,8 @@ SlotRows rows_of(const DevBasis &D, int slot, const DevOp &O) {
 
 int slot_dim(const DevBasis &D, int slot) { return slot == 0 ? D.dim_LL : slot == 2 ? D.dim_RR : D.d; }
 
-[[noreturn]] void die(const char *fn, const char *msg) {
-  printf("Error in %s\n%s\n", fn, msg);
-  fflush(stdout);
-  exit(1);
-}
-
 void v_m(const char *fn, int slot, CRS1 *M, double *V, double *Out, int dim, DMRG_BASIS *B) {
-  try {
+  run_or_die(fn, [&] {
     auto D = upload_basis(B, dim);
     DevOp O;
     upload_op(M, slot_dim(*D, slot), O);
@@ -98,9 +92,7 @@ void v_m(const char *fn, int slot, CRS1 *M, double *V, double *Out, int dim, DMR
     launch_slot_apply(dim, rows_of(*D, slot, O), x.p, y.p, nullptr, nullptr, R, st);
     HIPCHK(hipMemcpyAsync(Out, y.p, sizeof(double) * dim, hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
-  } catch (const Failure &f) {
-    die(fn, f.what());
-  }
+  });
 }
 
 }  // namespace
@@ -128,7 +120,7 @@ void DMRG_EXPECTATION_ONSITE(CRS1 **M_LL, CRS1 *M_On, CRS1 **M_RR, double *Out, 
                              int p_threads, DMRG_BASIS *B, DMRG_STATUS *St) {
   (void)p_threads;
   static const char *fn = "DMRG_EXPECTATION_ONSITE";
-  try {
+  run_or_die(fn, [&] {
     const int LL_site = St->LL_site, RR_site = St->RR_site, n = St->dim_LLLRRRRL;
     const int nout = LL_site + RR_site + 4;
     auto D = upload_basis(B, n);
@@ -152,9 +144,7 @@ void DMRG_EXPECTATION_ONSITE(CRS1 **M_LL, CRS1 *M_On, CRS1 **M_RR, double *Out, 
     HIPCHK(hipMemcpyAsync(Out, res.p, sizeof(double) * nout, hipMemcpyDeviceToHost, st));
     if (Temp_V) HIPCHK(hipMemcpyAsync(Temp_V, t.p, sizeof(double) * n, hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
-  } catch (const Failure &f) {
-    die(fn, f.what());
-  }
+  });
 }
 
 void DMRG_EXPECTATION_INTERSITE_Q0(CRS1 **M_CF, CRS1 **M_LL, CRS1 *M_On, CRS1 **M_RR, double *Out, int origin,
@@ -164,7 +154,7 @@ void DMRG_EXPECTATION_INTERSITE_Q0(CRS1 **M_CF, CRS1 **M_LL, CRS1 *M_On, CRS1 **
   static const char *fn = "DMRG_EXPECTATION_INTERSITE_Q0";
   const int LL_site = St->LL_site, RR_site = St->RR_site, n = St->dim_LLLRRRRL;
   if (origin > LL_site + 1) return;  // :10-12
-  try {
+  run_or_die(fn, [&] {
     const int nout = LL_site + RR_site + 4 - origin;
     auto D = upload_basis(B, n);
     hipStream_t st = ctx().stream;
@@ -199,9 +189,7 @@ void DMRG_EXPECTATION_INTERSITE_Q0(CRS1 **M_CF, CRS1 **M_LL, CRS1 *M_On, CRS1 **
     if (Temp_V1) HIPCHK(hipMemcpyAsync(Temp_V1, t1.p, sizeof(double) * n, hipMemcpyDeviceToHost, st));
     if (Temp_V2) HIPCHK(hipMemcpyAsync(Temp_V2, t2.p, sizeof(double) * n, hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
-  } catch (const Failure &f) {
-    die(fn, f.what());
-  }
+  });
 }
 
 }  // extern "C"

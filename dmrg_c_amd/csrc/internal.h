@@ -187,6 +187,56 @@ struct SlotRows {
   }
 };
 
+// Binary search of an ascending int64 array: index of target or -1
+// (sml/BINARY_SEARCH_LINT1.c, iterative).
+__device__ __forceinline__ int64_t find_code(const int64_t *__restrict__ a, int64_t n, int64_t target) {
+  int64_t lo = 0, hi = n - 1;
+  while (lo <= hi) {
+    const int64_t mid = lo + ((hi - lo) >> 1);
+    const int64_t v = a[mid];
+    if (v == target) return mid;
+    if (v < target) lo = mid + 1;
+    else hi = mid - 1;
+  }
+  return -1;
+}
+
+// One on-site operator at one site of an ED state (exact/EXACT_V_M_Q0.c:20-35, Q1-Q3):
+// row r of the output basis is code c; the operator row of c's local state a at `site`
+// sends c to c - (a - col) d^site, looked up in the input basis; with sign_flag the
+// result is negated when sites < site hold an odd number of electrons (EXACT_V_M_Q2.c:31-44).
+struct EdRows {
+  int32_t d, site, sign_flag;
+  int64_t pw, n_in;
+  const int64_t *basis_out, *basis_in;
+  const int64_t *row;
+  const int32_t *col;
+  const double *val;
+  const int32_t *n_ele;
+  static constexpr bool kTimed = false;
+  double alg_bytes(int32_t) const { return 0.0; }
+  template <int G>
+  __device__ __forceinline__ double lane_sum(int32_t r, int lane, const double *__restrict__ x) const {
+    const int64_t code = basis_out[r];
+    const int a = (int)((code / pw) % d);
+    double s = 0.0;
+    for (int64_t j = row[a] + lane; j < row[a + 1]; j += G) {
+      const int64_t inv = find_code(basis_in, n_in, code - (int64_t)(a - col[j]) * pw);
+      if (inv >= 0) s += x[inv] * val[j];
+    }
+    if (sign_flag) {
+      int ne = 0;
+      int64_t c = code;
+      for (int k = 0; k < site; k++) {
+        ne += n_ele[c % d];
+        c /= d;
+      }
+      if (ne & 1) s = -s;
+    }
+    return s;
+  }
+};
+
 // Device arrays of the matrix-free operator (uploaded from DMRG_BASIS + the three
 // sub-Hamiltonians, obc.cpp).
 struct ObcDev {
@@ -259,6 +309,9 @@ void launch_cg_dir(int64_t n, int it, const double *r, const double *rr, double 
 // *dot_out = <dot_with, O x> (deterministic reduction)
 void launch_slot_apply(int32_t n, const SlotRows &rows, const double *x, double *y, const double *dot_with,
                        double *dot_out, Reduce &R, hipStream_t s);
+// the same for an on-site operator on an ED state (n = rows of the output basis)
+void launch_ed_apply(int32_t n, const EdRows &rows, const double *x, double *y, const double *dot_with,
+                     double *dot_out, Reduce &R, hipStream_t s);
 
 // ---- host tridiagonal solver (tridiag.cpp)
 // lowest eigenvalue of T(a[0..k-1], b[0..k-2]); lam_hint: an upper bound or +inf

@@ -539,6 +539,12 @@ void launch_slot_apply(int32_t n, const SlotRows &rows, const double *x, double 
   apply_rows(n, 8, rows, x, EpiStoreDot{y, dot_with, dot_out}, &R, s);
 }
 
+void launch_ed_apply(int32_t n, const EdRows &rows, const double *x, double *y, const double *dot_with,
+                     double *dot_out, Reduce &R, hipStream_t s) {
+  // an on-site row has at most d entries: one lane per row
+  apply_rows(n, 1, rows, x, EpiStoreDot{y, dot_with, dot_out}, &R, s);
+}
+
 void launch_norm2(int64_t n, const double *x, Reduce &R, double *out, hipStream_t s) {
   const int g = vec_grid(n);
   R.ensure(g);

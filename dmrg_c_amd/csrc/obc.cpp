@@ -7,11 +7,9 @@
 // arrays (the same scatter GET_BASIS_SUPERBLOCK.c uses to fill Inv_LLLRRRRL), so the
 // dim_LL*d*dim_RR*d table does not cross PCIe.
 #include <cmath>
-#include <cstdarg>
 #include <cstring>
-#include <sys/stat.h>
 
-#include "internal.h"
+#include "entry.h"
 
 namespace dmrgx {
 void fill_inverse(int32_t n, int32_t dim_RR, int32_t d, const int16_t *LL, const int16_t *LR, const int16_t *RR,
@@ -21,34 +19,6 @@ void fill_inverse(int32_t n, int32_t dim_RR, int32_t d, const int16_t *LL, const
 using namespace dmrgx;
 
 namespace {
-
-[[noreturn]] void die(const char *fn, const char *fmt, ...) {
-  printf("Error in %s\n", fn);
-  va_list ap;
-  va_start(ap, fmt);
-  vprintf(fmt, ap);
-  va_end(ap);
-  fflush(stdout);
-  exit(1);
-}
-
-void log_append(const char *file, const char *fmt, ...) {
-  const char *e = getenv("DMRGX_SML_OUT");
-  if (e && strcmp(e, "0") == 0) return;
-  mkdir("SML_out", 0777);
-  char path[256];
-  snprintf(path, sizeof(path), "./SML_out/%s", file);
-  FILE *f = fopen(path, "a+");
-  if (!f) {
-    printf("Can't open file.\n");
-    exit(1);
-  }
-  va_list ap;
-  va_start(ap, fmt);
-  vfprintf(f, fmt, ap);
-  va_end(ap);
-  fclose(f);
-}
 
 template <class T>
 void up(DevBuf<T> &b, const T *h, size_t n) {
@@ -149,15 +119,6 @@ std::unique_ptr<ObcDev> upload_obc(const CRS1 *M_LLLR, const CRS1 *M_LRRL, const
   return O;
 }
 
-template <class F>
-void guarded(const char *fn, F &&body) {
-  try {
-    body();
-  } catch (const Failure &f) {
-    die(fn, "%s\n", f.what());
-  }
-}
-
 }  // namespace
 
 extern "C" {
@@ -165,7 +126,7 @@ extern "C" {
 void DMRG_MATRIX_VECTOR_PRODUCT_OBC(CRS1 *M_LLLR, CRS1 *M_LRRL, CRS1 *M_LRRL_Sign, CRS1 *M_RRRL, double *V,
                                     double *Out_V, int *Ele_RR, int p_threads, DMRG_BASIS *B) {
   (void)p_threads;
-  guarded("DMRG_MATRIX_VECTOR_PRODUCT_OBC", [&] {
+  run_or_die("DMRG_MATRIX_VECTOR_PRODUCT_OBC", [&] {
     auto O = upload_obc(M_LLLR, M_LRRL, M_LRRL_Sign, M_RRRL, Ele_RR, B);
     const int64_t n = O->n;
     DevBuf<double> x(n), y(n);
@@ -181,7 +142,7 @@ double DMRG_CONVERGE_CHECK_OBC(double eig_val, CRS1 *M_LLLR, CRS1 *M_LRRL, CRS1 
                                double *Eig_V, double *Temp_V, int *Ele_RR, int p_threads, DMRG_BASIS *B) {
   (void)p_threads;
   double r = 0.0;
-  guarded("DMRG_CONVERGE_CHECK_OBC", [&] {
+  run_or_die("DMRG_CONVERGE_CHECK_OBC", [&] {
     auto O = upload_obc(M_LLLR, M_LRRL, M_LRRL_Sign, M_RRRL, Ele_RR, B);
     const int64_t n = O->n;
     DevBuf<double> x(n), y(n);
@@ -198,13 +159,13 @@ double DMRG_CONVERGE_CHECK_OBC(double eig_val, CRS1 *M_LLLR, CRS1 *M_LRRL, CRS1 
 
 void DMRG_LANCZOS_SLOW_OBC(DMRG_BOX_LAN *Box) {
   static const char *fn = "DMRG_LANCZOS_SLOW_OBC";
-  if (Box->max_step <= Box->min_step) die(fn, "Box->max_step=%d,Box->min_step=%d\n", Box->max_step, Box->min_step);
+  if (Box->max_step <= Box->min_step) fatal(fn, "Box->max_step=%d,Box->min_step=%d\n", Box->max_step, Box->min_step);
   const bool guess = strcmp(Box->Guess, "Yes") == 0;
-  if (!guess && strcmp(Box->Guess, "No") != 0) die(fn, "Box->Guess=%s is error\n", Box->Guess);
+  if (!guess && strcmp(Box->Guess, "No") != 0) fatal(fn, "Box->Guess=%s is error\n", Box->Guess);
   int type = 0;
   if (strcmp(Box->Type, "Tight") == 0) type = 1;
-  else if (strcmp(Box->Type, "Normal") != 0) die(fn, "Box->Type=%s is error\n", Box->Type);
-  guarded(fn, [&] {
+  else if (strcmp(Box->Type, "Normal") != 0) fatal(fn, "Box->Type=%s is error\n", Box->Type);
+  run_or_die(fn, [&] {
     auto O = upload_obc(Box->M_LLLR, Box->M_LRRL, Box->M_LRRL_Sign, Box->M_RRRL, Box->Ele_RR, Box->Dmrg_Basis);
     const int64_t n = O->n;
     hipStream_t st = ctx().stream;
@@ -220,19 +181,19 @@ void DMRG_LANCZOS_SLOW_OBC(DMRG_BOX_LAN *Box) {
     SolveOut out;
     lanczos(op_of(*O), o, v.p, out);
     if (!out.lan_converged) {
-      log_append("error.txt", "Error in %s\nDoes not converge (max step is %d)\n", fn, Box->max_step);
-      die(fn, "Does not converge (max step is %d)\n", Box->max_step);
+      sml_append("error.txt", "Error in %s\nDoes not converge (max step is %d)\n", fn, Box->max_step);
+      fatal(fn, "Does not converge (max step is %d)\n", Box->max_step);
     }
     Box->eig_val[0] = out.eig;
     HIPCHK(hipMemcpyAsync(Box->Eig_Vec, v.p, sizeof(double) * n, hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
-    log_append("LANCZOS_SLOW_OBC_Step.txt", "%d\n", out.lan_steps);
+    sml_append("LANCZOS_SLOW_OBC_Step.txt", "%d\n", out.lan_steps);
   });
 }
 
 void DMRG_CONJUGATE_GRADIENT_OBC(DMRG_BOX_CG *Box) {
   static const char *fn = "DMRG_CONJUGATE_GRADIENT_OBC";
-  guarded(fn, [&] {
+  run_or_die(fn, [&] {
     auto O = upload_obc(Box->M_LLLR, Box->M_LRRL, Box->M_LRRL_Sign, Box->M_RRRL, Box->Ele_RR, Box->Dmrg_Basis);
     const int64_t n = O->n;
     hipStream_t st = ctx().stream;
@@ -246,15 +207,15 @@ void DMRG_CONJUGATE_GRADIENT_OBC(DMRG_BOX_CG *Box) {
     HIPCHK(hipStreamSynchronize(st));
     if (!conv) {
       printf("Error in %s\nDoes not converge (max step is %d)\nContinue...\n", fn, Box->max_step);
-      log_append("error.txt", "Error in %s\nDoes not converge (max step is %d)\nContinue...\n", fn, Box->max_step);
+      sml_append("error.txt", "Error in %s\nDoes not converge (max step is %d)\nContinue...\n", fn, Box->max_step);
     }
-    log_append("CONJUGATE_GRADIENT_Step.txt", "%d\n", it);
+    sml_append("CONJUGATE_GRADIENT_Step.txt", "%d\n", it);
   });
 }
 
 void DMRG_INVERSE_ITERATION_OBC(DMRG_BOX_II *Box) {
   static const char *fn = "DMRG_INVERSE_ITERATION_OBC";
-  guarded(fn, [&] {
+  run_or_die(fn, [&] {
     auto O = upload_obc(Box->M_LLLR, Box->M_LRRL, Box->M_LRRL_Sign, Box->M_RRRL, Box->Ele_RR, Box->Dmrg_Basis);
     const int64_t n = O->n;
     hipStream_t st = ctx().stream;
@@ -267,11 +228,11 @@ void DMRG_INVERSE_ITERATION_OBC(DMRG_BOX_II *Box) {
     HIPCHK(hipStreamSynchronize(st));
     Box->error = out.err;
     if (out.ii_converged) {
-      log_append("Inv_iter_step.txt", "%d\n", out.ii_steps);
+      sml_append("Inv_iter_step.txt", "%d\n", out.ii_steps);
     } else {
       printf("Error in %s\nDoes not converge(%e > %e) (max step is %d)\ncontinue...\n", fn, out.err, Box->ii_acc,
              Box->ii_max_step);
-      log_append("error.txt", "Error in %s\nDoes not converge(%e > %e) (max step is %d)\ncontinue...\n", fn, out.err,
+      sml_append("error.txt", "Error in %s\nDoes not converge(%e > %e) (max step is %d)\ncontinue...\n", fn, out.err,
                  Box->ii_acc, Box->ii_max_step);
     }
   });

@@ -303,6 +303,34 @@ typedef struct {
    int tot_ele;
    int tot_parity;
 } EXACT_BASIS_INFO;
+
+/* include/exact.h:76-87 */
+typedef struct {
+   long *Basis;
+   long elem_num;
+   long *Check;
+   double *Val;
+   int max_row;
+   int dim_onsite;
+   long dim_whole;
+} EXACT_A_BASIS;
+
+/* include/exact.h:89-111 */
+typedef struct {
+   int *Dim;
+   long **Basis;
+   int max_dim;
+} EXACT_WHOLE_BASIS_Q1;
+typedef struct {
+   int **Dim;
+   long ***Basis;
+   int max_dim;
+} EXACT_WHOLE_BASIS_Q2;
+typedef struct {
+   int ***Dim;
+   long ****Basis;
+   int max_dim;
+} EXACT_WHOLE_BASIS_Q3;
 #endif /* exact_h */
 
 /* ========================================== Part 1: reference entry points */
@@ -370,6 +398,30 @@ void DMRG_MAKE_ELEM_LRRL_LLLRRRRL(DMRG_BASIS_ONSITE *Basis, DMRG_A_BASIS *A_Basi
 void DMRG_MAKE_ELEM_LRRR_LLLRRRRL(DMRG_BASIS_ONSITE *Basis, DMRG_A_BASIS *A_Basis, int *Inv_LLLRRRRL, CRS1 *M_LR, CRS1 *M_RR, double coeef, int *Ele_LR, int *elem_num, char Type[], char Sign_Flag[]);
 void DMRG_MAKE_ELEM_RRRL_LLLRRRRL(DMRG_BASIS_ONSITE *Basis, DMRG_A_BASIS *A_Basis, int *Inv_LLLRRRRL, CRS1 *M_RR, CRS1 *M_RL, double coeef, int *Ele_RR, int *elem_num, char Type[], char Sign_Flag[]);
 void DMRG_MAKE_ELEM_ZERO_LLLRRRRL(DMRG_BASIS_ONSITE *Basis, DMRG_A_BASIS *A_Basis, int *Inv_LLLRRRRL, int *elem_num);
+
+/* ---- exact diagonalization (include/exact.h:113-125) */
+/* include/exact.h:118 -- Lanczos ("Lanczos" | "Lanczos_Slow"), inverse iteration with CG and
+ * the residual check on the full CRS Ham_Info->Ham, on the device.  Fills Vector[0],
+ * Value[0] (the Lanczos eigenvalue), Error[0], dim, mem_ham and Time->diag / inv_iter. */
+void EXACT_DIAGONALIZE_HAMILTONIAN(EXACT_HAM_INFO *Ham_Info, EXACT_PARAMETER *Param, EXACT_TIME *Time, int p_threads);
+/* include/exact.h:119,122-124 -- one on-site operator at one site on an ED vector (device);
+ * Q1..Q3 map between the quantum-number sectors of W_Basis, Q2/Q3 apply the fermion sign
+ * (-1)^(electrons on sites < op_site) when Sign_Flag is "Yes". */
+void EXACT_V_M_Q0(CRS1 *M_On, double *Vec, double *Out_Vec, int dim_onsite, int site, int p_threads, EXACT_BASIS_INFO *Basis_Info);
+void EXACT_V_M_Q1(CRS1 *M_On, int qn_out, double *Vec, int qn_in, double *Out_Vec, int dim_onsite, int site, int p_threads, EXACT_WHOLE_BASIS_Q1 *W_Basis);
+void EXACT_V_M_Q2(CRS1 *M_On, int qn1_out, int qn2_out, double *Vec, int qn1_in, int qn2_in, double *Out_Vec, char Sign_Flag[], int *N_Ele, int dim_onsite, int op_site, int p_threads, EXACT_WHOLE_BASIS_Q2 *W_Basis);
+void EXACT_V_M_Q3(CRS1 *M_On, int qn1_out, int qn2_out, int qn3_out, double *Vec, int qn1_in, int qn2_in, int qn3_in, double *Out_Vec, char Sign_Flag[], int *N_Ele, int dim_onsite, int op_site, int p_threads, EXACT_WHOLE_BASIS_Q3 *W_Basis);
+/* include/exact.h:120-121 */
+void EXACT_EXPECTATION_ONSITE(CRS1 *M_On, double *Out, double *Vec, double *T_Vec, int dim_onsite, int tot_site, int p_threads, EXACT_BASIS_INFO *Basis_Info);
+void EXACT_EXPECTATION_INTERSITE_Q0(CRS1 *M_O, CRS1 *M_R, int start, int end, double *Out, double *Vec, double *T_Vec1, double *T_Vec2, int dim_onsite, int p_threads, EXACT_BASIS_INFO *Basis_Info);
+/* include/exact.h:113-117,125 -- host helpers of the programs' GET_HAM (row accumulation
+ * into A_Basis, kept sorted by target code), with the reference's results */
+EXACT_A_BASIS **EXACT_GET_A_BASIS(int p_threads, int max_row);
+void EXACT_FREE_A_BASIS(EXACT_A_BASIS **A_Basis, int p_threads);
+int EXACT_FIND_SITE_STATE(long basis, int site, int dim_onsite);
+void EXACT_MAKE_ELEM_ON(long basis, int site, int dim_onsite, CRS1 *M_On, long *elem_num, double coeef, EXACT_A_BASIS *A_Basis);
+void EXACT_MAKE_ELEM_INTER(long basis, int site1, int site2, int dim_onsite, CRS1 *M1, CRS1 *M2, long *elem_num, double coeef, int sign, EXACT_A_BASIS *A_Basis);
+double EXACT_SSD_COEFF(int site, int tot_site, char BC[], char Inter_Name[]);
 
 /* ================================================= Part 2: dmrgx_ entry points */
 /* Status codes */
@@ -478,6 +530,66 @@ void dmrgx_superblock_free(dmrgx_superblock *S);
  * reference-layout lower triangle, and keeps the expanded device copy for the next
  * DMRG_DIAGONALIZE_SUPERBLOCK_SYM call on that CRS1 (no re-upload).  exit(1) on error. */
 CRS1 *dmrgx_get_ham_lllrrrrl(const DMRG_BASIS *B, const dmrgx_terms *T);
+
+/* ---- exact diagonalization on the device
+ * A problem is a sorted basis of base-d codes (site s is digit s, site 0 least
+ * significant: exact/EXACT_FIND_SITE_STATE.c), on-site operators (d x d CRS) and a term
+ * list the device assembler turns into the full CRS of GET_HAM
+ * (main/exact_diag/klm_vf/src/GET_HAM.c:11-92), bit-identical to the reference's element
+ * makers driven by the same terms.
+ *
+ * Term: kind 0 = coeff * op1 on site1 (EXACT_MAKE_ELEM_ON), kind 1 = s * coeff *
+ * op1(site1) op2(site2) (EXACT_MAKE_ELEM_INTER) with integer sign s = sign, negated when
+ * par_lo < par_hi and the row state holds an even number of electrons (n_ele[] of the
+ * local states) on sites [par_lo, par_hi) -- the fermion sign rule of
+ * main/exact_diag/klm_vf/src/MAKE_ELEMENT_HAM.c:31-37,44-64. */
+typedef struct {
+   int32_t kind, site1, site2, op1, op2, par_lo, par_hi, sign;
+   double coeff;
+} dmrgx_ed_term;
+
+typedef struct dmrgx_ed dmrgx_ed;
+/* basis: n sorted codes (host); ops: n_ops CRS1 of dimension d; n_ele: d entries (may be
+ * NULL when no term has a parity range). */
+int dmrgx_ed_create(int32_t L, int32_t d, int64_t n, const int64_t *basis, int32_t n_ops, const CRS1 *ops,
+                    const int32_t *n_ele, int32_t n_terms, const dmrgx_ed_term *terms, dmrgx_ed **out);
+/* Same with the basis enumerated on the device: all codes of d^L whose per-site sums of
+ * qn1[] and (if qn2 != NULL) qn2[] equal target1 / target2, ascending. */
+int dmrgx_ed_create_sector(int32_t L, int32_t d, const int32_t *qn1, int32_t target1, const int32_t *qn2,
+                           int32_t target2, int32_t n_ops, const CRS1 *ops, const int32_t *n_ele,
+                           int32_t n_terms, const dmrgx_ed_term *terms, dmrgx_ed **out);
+int64_t dmrgx_ed_dim(const dmrgx_ed *E);
+int dmrgx_ed_download_basis(const dmrgx_ed *E, int64_t *basis);
+/* Full CRS of the Hamiltonian on the device (columns ascending), ready for
+ * dmrgx_ground_state. */
+int dmrgx_ed_assemble(const dmrgx_ed *E, dmrgx_matrix **out);
+/* y = O x on device vectors: O = ops[op] of E_in at site, from E_in's basis to E_out's
+ * (the same handle for a sector-preserving operator); sign_flag: fermion sign of sites
+ * < site (EXACT_V_M_Q2.c:31-44).  E_out may carry no terms (only its basis is used). */
+int dmrgx_ed_apply_onsite(const dmrgx_ed *E_out, const dmrgx_ed *E_in, int32_t op, int32_t site, int32_t sign_flag,
+                          const double *x_dev, double *y_dev);
+void dmrgx_ed_free(dmrgx_ed *E);
+
+/* Models with no ED driver in the reference, written on the klm_vf pattern with each
+ * DMRG program's coupling order:
+ *   model 0: spin-S XXZ  sum J_z SzSz + J_xy/2 (S+S- + S-S+) + sum D_z Sz^2 + h_z Sz;
+ *            ops 0 Ham_On, 1 S+, 2 S-, 3 Sz, 4 zero diagonal; sector 2*Sz_tot = sz
+ *   model 1: Hubbard     sum t (c+_{i s} c_{i+1 s} + h.c.) + V (nup nup + ndn ndn)
+ *            + sum U n_up n_dn + h_z Sz + mu n;  ops 0 Ham_On, 1 c+up, 2 cup, 3 c+dn, 4 cdn,
+ *            5 nup, 6 ndn, 7 zero diagonal, 8 n, 9 Sz;  sector (N, 2*Sz_tot = sz)
+ * bc: 0 OBC, 1 PBC, 2 SSD (EXACT_SSD_COEFF weights).  spin = 2S (model 0). */
+typedef struct {
+   int32_t model, L, spin, N, sz, bc;
+   double J_xy, J_z, h_z, D_z, t, U, V, mu;
+} dmrgx_ed_model;
+int dmrgx_ed_model_build(const dmrgx_ed_model *m, dmrgx_ed **out);
+/* Host only (no device work): what dmrgx_ed_model_build uploads.  Sizes come back in
+ * d, n_ops, n_terms (and op_nnz[n_ops]); any array argument may be NULL.  Operators are
+ * packed: op_row[k*(d+1) ..], op_col/op_val at the running sum of op_nnz.  targets:
+ * {target1, target2, qn2 used (0|1)}. */
+int dmrgx_ed_model_describe(const dmrgx_ed_model *m, int32_t *d, int32_t *n_ops, int64_t *op_nnz, int64_t *op_row,
+                            int32_t *op_col, double *op_val, int32_t *n_terms, dmrgx_ed_term *terms, int32_t *n_ele,
+                            int32_t *qn1, int32_t *qn2, int32_t *targets);
 
 /* ---- measurement
  * Between begin and end, every CSR SpMV launch of the library is bracketed by HIP events

@@ -572,3 +572,160 @@ void or_v_m_q0(int slot, int32_t n, int32_t dim_RR, int32_t d, const int16_t *LL
       y[b] = v;
    }
 }
+
+/* ------------------------------------------------------ exact diagonalization */
+/* sml/BINARY_SEARCH_LINT1.c: index of target in the ascending array, -1 when absent */
+static int64_t bsearch_i64(const int64_t *a, int64_t lo, int64_t hi, int64_t target) {
+   hi -= 1;
+   while (lo <= hi) {
+      int64_t mid = lo + (hi - lo) / 2;
+      if (a[mid] == target) return mid;
+      if (a[mid] < target) lo = mid + 1;
+      else hi = mid - 1;
+   }
+   return -1;
+}
+
+static int ed_local(int64_t code, int32_t site, int32_t d) { /* exact/EXACT_FIND_SITE_STATE.c:3-15 */
+   for (int32_t i = 0; i < site; i++) code /= d;
+   return (int)(code % d);
+}
+
+static int64_t ed_pow(int32_t d, int32_t site) {
+   int64_t p = 1;
+   for (int32_t i = 0; i < site; i++) p *= d;
+   return p;
+}
+
+typedef struct { int64_t code; double val; int64_t order; } ed_contrib;
+
+static int ed_cmp(const void *a, const void *b) {
+   const ed_contrib *x = a, *y = b;
+   if (x->code != y->code) return x->code < y->code ? -1 : 1;
+   return x->order < y->order ? -1 : (x->order > y->order);
+}
+
+/* one row: contributions in call order (EXACT_MAKE_ELEM_ON.c:18-29, EXACT_MAKE_ELEM_INTER.c:17-34),
+ * grouped by target code, summed first-to-last.  Returns the number of distinct targets;
+ * codes/vals hold them ascending. */
+static int64_t ed_row(int64_t code, int32_t L, int32_t d, const or_crs *ops, const int32_t *n_ele,
+                      int32_t n_terms, const or_ed_term *terms, ed_contrib **buf, int64_t *cap,
+                      int64_t *codes, double *vals) {
+   int64_t m = 0;
+   (void)L;
+   for (int32_t t = 0; t < n_terms; t++) {
+      const or_ed_term *tm = &terms[t];
+      const or_crs *a = &ops[tm->op1];
+      const int l1 = ed_local(code, tm->site1, d);
+      const int64_t p1 = ed_pow(d, tm->site1);
+      if (tm->kind == 0) {
+         for (int64_t i = a->row[l1]; i < a->row[l1 + 1]; i++) {
+            if (m == *cap) { *cap *= 2; *buf = realloc(*buf, sizeof(ed_contrib) * (size_t)*cap); }
+            (*buf)[m].code = code + (a->col[i] - l1) * p1;
+            (*buf)[m].val = tm->coeff * a->val[i];
+            (*buf)[m].order = m;
+            m++;
+         }
+      } else {
+         const or_crs *b = &ops[tm->op2];
+         const int l2 = ed_local(code, tm->site2, d);
+         const int64_t p2 = ed_pow(d, tm->site2);
+         int sign = tm->sign;
+         if (tm->par_lo < tm->par_hi) {
+            int ne = 0;
+            for (int32_t s = tm->par_lo; s < tm->par_hi; s++) ne += n_ele[ed_local(code, s, d)];
+            if (ne % 2 == 0) sign = -sign;
+         }
+         for (int64_t i = a->row[l1]; i < a->row[l1 + 1]; i++) {
+            for (int64_t j = b->row[l2]; j < b->row[l2 + 1]; j++) {
+               if (m == *cap) { *cap *= 2; *buf = realloc(*buf, sizeof(ed_contrib) * (size_t)*cap); }
+               (*buf)[m].code = code + (a->col[i] - l1) * p1 + (b->col[j] - l2) * p2;
+               (*buf)[m].val = sign * tm->coeff * a->val[i] * b->val[j];
+               (*buf)[m].order = m;
+               m++;
+            }
+         }
+      }
+   }
+   qsort(*buf, (size_t)m, sizeof(ed_contrib), ed_cmp);
+   int64_t u = 0;
+   for (int64_t i = 0; i < m; i++) {
+      if (u > 0 && codes[u - 1] == (*buf)[i].code) {
+         vals[u - 1] = vals[u - 1] + (*buf)[i].val;
+      } else {
+         codes[u] = (*buf)[i].code;
+         vals[u] = (*buf)[i].val;
+         u++;
+      }
+   }
+   return u;
+}
+
+int64_t or_ed_assemble(int32_t L, int32_t d, int32_t n, const int64_t *basis, int32_t n_ops,
+                       const or_crs *ops, const int32_t *n_ele, int32_t n_terms, const or_ed_term *terms,
+                       int64_t *row_out, int32_t *col_out, double *val_out) {
+   (void)n_ops;
+   int64_t cap = 4096;
+   ed_contrib *buf = malloc(sizeof(ed_contrib) * (size_t)cap);
+   int64_t ucap = 4096;
+   int64_t *codes = malloc(sizeof(int64_t) * (size_t)ucap);
+   double *vals = malloc(sizeof(double) * (size_t)ucap);
+   int64_t nnz = 0;
+   if (!col_out) row_out[0] = 0;
+   for (int32_t r = 0; r < n; r++) {
+      /* the number of targets never exceeds the number of contributions */
+      int64_t need = 0;
+      for (int32_t t = 0; t < n_terms; t++) {
+         const or_crs *a = &ops[terms[t].op1];
+         int64_t k = a->row[ed_local(basis[r], terms[t].site1, d) + 1] - a->row[ed_local(basis[r], terms[t].site1, d)];
+         if (terms[t].kind == 1) {
+            const or_crs *b = &ops[terms[t].op2];
+            const int l2 = ed_local(basis[r], terms[t].site2, d);
+            k *= b->row[l2 + 1] - b->row[l2];
+         }
+         need += k;
+      }
+      if (need > ucap) {
+         ucap = need;
+         codes = realloc(codes, sizeof(int64_t) * (size_t)ucap);
+         vals = realloc(vals, sizeof(double) * (size_t)ucap);
+      }
+      int64_t u = ed_row(basis[r], L, d, ops, n_ele, n_terms, terms, &buf, &cap, codes, vals);
+      int64_t p = col_out ? row_out[r] : 0;
+      for (int64_t i = 0; i < u; i++) {
+         int64_t inv = bsearch_i64(basis, 0, n, codes[i]);
+         if ((fabs(vals[i]) > OR_ZERO && inv >= 0) || inv == r) { /* GET_HAM.c:35,63 */
+            if (col_out) { col_out[p] = (int32_t)inv; val_out[p] = vals[i]; }
+            p++;
+         }
+      }
+      if (!col_out) row_out[r + 1] = row_out[r] + p;
+      nnz += col_out ? p - row_out[r] : p;
+   }
+   free(buf);
+   free(codes);
+   free(vals);
+   return nnz;
+}
+
+void or_ed_v_m(int32_t d, int32_t site, const or_crs *op, int32_t n_out, const int64_t *basis_out,
+               int32_t n_in, const int64_t *basis_in, const double *x, int32_t sign_flag,
+               const int32_t *n_ele, double *y) {
+   const int64_t p = ed_pow(d, site);
+   for (int32_t i = 0; i < n_out; i++) {
+      const int64_t code = basis_out[i];
+      const int a = ed_local(code, site, d);
+      int sign = 1;
+      if (sign_flag) {
+         int ne = 0;
+         for (int32_t s = 0; s < site; s++) ne += n_ele[ed_local(code, s, d)];
+         sign = (ne % 2 == 0) ? 1 : -1;   /* EXACT_V_M_Q2.c:35-42 */
+      }
+      double v = 0.0;
+      for (int64_t j = op->row[a]; j < op->row[a + 1]; j++) {
+         int64_t inv = bsearch_i64(basis_in, 0, n_in, code - (a - op->col[j]) * p);
+         if (inv >= 0) v = v + x[inv] * op->val[j];
+      }
+      y[i] = v * sign;
+   }
+}

@@ -121,6 +121,39 @@ void or_v_m_q0(int slot, int32_t n, int32_t dim_RR, int32_t d, const int16_t *LL
                const int16_t *RR, const int16_t *RL, const int32_t *inv_whole, const int64_t *row,
                const int32_t *col, const double *val, const double *x, double *y);
 
+/* ------------------------------------------------------------ exact diagonalization */
+/* One term of an ED Hamiltonian row (layout identical to dmrgx_ed_term):
+ *   kind 0: coeff * O1 on site1               (exact/EXACT_MAKE_ELEM_ON.c:7-31)
+ *   kind 1: sign * coeff * O1(site1) O2(site2) (exact/EXACT_MAKE_ELEM_INTER.c:6-36)
+ * The integer sign is term.sign, times -1 when par_lo < par_hi and the row state holds
+ * an even number of electrons (n_ele of each site's local state) on sites
+ * [par_lo, par_hi): the rule main/exact_diag/klm_vf/src/MAKE_ELEMENT_HAM.c:31-37,44-64
+ * writes out per row. */
+typedef struct {
+   int32_t kind, site1, site2, op1, op2, par_lo, par_hi, sign;
+   double coeff;
+} or_ed_term;
+
+/* Full ED Hamiltonian CRS, main/exact_diag/klm_vf/src/GET_HAM.c:11-92: per row the
+ * terms' contributions accumulate by target code (first one sets, later ones add, in call
+ * order), targets are looked up in the sorted basis (sml/BINARY_SEARCH_LINT1.c) and kept
+ * when (|val| > 1e-15 && inv >= 0) || inv == row; columns ascending.  Pass 1 (col_out ==
+ * NULL) fills row_out and returns nnz; pass 2 fills col_out/val_out.
+ * Pinned: bit-exact against the reference's element makers driven by
+ * oracle/harness/ed_ref.c (golden ED fixtures); energies against closed forms. */
+int64_t or_ed_assemble(int32_t L, int32_t d, int32_t n, const int64_t *basis, int32_t n_ops,
+                       const or_crs *ops, const int32_t *n_ele, int32_t n_terms, const or_ed_term *terms,
+                       int64_t *row_out, int32_t *col_out, double *val_out);
+
+/* EXACT_V_M_Q0..Q3 (exact/EXACT_V_M_Q0.c:14-37, Q1.c:8-38, Q2.c:10-63, Q3.c:10-63):
+ * y[i] = sign_i * sum_j O[a, c_j] x[inv(code_i - (a - c_j) d^site)], a = local state of
+ * basis_out[i] at site, inv by binary search in basis_in (absent targets skipped),
+ * sign_i = (-1)^(electrons on sites < site) when sign_flag, else 1.
+ * Pinned: the reference's EXACT_V_M_Q1/Q2/Q3 and EXACT_EXPECTATION_* outputs. */
+void or_ed_v_m(int32_t d, int32_t site, const or_crs *op, int32_t n_out, const int64_t *basis_out,
+               int32_t n_in, const int64_t *basis_in, const double *x, int32_t sign_flag,
+               const int32_t *n_ele, double *y);
+
 #ifdef __cplusplus
 }
 #endif

@@ -11,6 +11,10 @@ and converts the superblock snapshots they dump into compact ``.npz`` files:
   <name>_expect_s<step>_<k>.npz  one DMRG_EXPECTATION_ONSITE / _INTERSITE_Q0 call of the
                        reference: block/on-site operators, the GS vector, the reference's output
   <name>_energies.txt  per-step: step sweep LL_site RR_site n E gs_error seconds
+  ed_<name>.npz        one exact diagonalization (oracle/harness/ed_ref.c): on-site operators,
+                       sector basis, the full Hamiltonian CRS built by the reference's
+                       EXACT_MAKE_ELEM_*, EXACT_DIAGONALIZE_HAMILTONIAN's ground state, and the
+                       outputs of EXACT_EXPECTATION_* and EXACT_V_M_Q1/Q2/Q3 on it
 
 This script is only run in the build container (the reference is not on the GPU box).
 Usage:  python tests/golden/make_golden.py [--only NAME] [--big]
@@ -30,7 +34,9 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 REPO = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, REPO)
+sys.path.insert(0, os.path.dirname(HERE))
 from dmrg_c_amd import snapshot  # noqa: E402
+import edhelp  # noqa: E402
 
 REF_BIN = os.path.join(REPO, "oracle", "_ref")
 
@@ -46,15 +52,30 @@ CASES = {
 ENERGY_RUNS = {
     "xxz_L32_m64_sw4": ("snap_xxz", "L=32 m=64 sweep=4 J_xy=1.0 J_z=1.0 h_z=0.0 diag_acc=1e-12"),
 }
+# exact diagonalization through oracle/_ref/ed_ref (the reference's exact/ + onsite_op/):
+# name -> harness arguments
+ED_CASES = {
+    "ed_xxz_L10": "model=xxz L=10 sz=0",
+    "ed_xxz_L12_jz05_h": "model=xxz L=12 sz=2 J_z=0.5 h_z=0.3",
+    "ed_xxz_L8_s1_dz": "model=xxz spin=2 L=8 sz=0 D_z=0.5 h_z=0.1",
+    "ed_xxz_L10_pbc": "model=xxz L=10 sz=0 J_z=1.5 BC=PBC",
+    "ed_xxz_L10_ssd": "model=xxz L=10 sz=0 BC=SSD",
+    "ed_hubbard_L6": "model=hubbard L=6 N=6 sz=0 t=-1 U=4 h_z=0.1",
+    "ed_hubbard_L6_v": "model=hubbard L=6 N=5 sz=1 t=-1 U=2 V=0.5 mu=0.3 h_z=0.2",
+    "ed_hubbard_L6_pbc": "model=hubbard L=6 N=6 sz=0 t=-1 U=4 BC=PBC",
+    "ed_hubbard_L5_pbc_odd": "model=hubbard L=5 N=3 sz=1 t=-1 U=3 BC=PBC",
+    "ed_hubbard_L7": "model=hubbard L=7 N=7 sz=1 t=-1 U=4 h_z=0.1",
+    "ed_hubbard_L6_ssd": "model=hubbard L=6 N=6 sz=0 t=-1 U=4 BC=SSD",
+}
 # bench input (not a parity fixture): Hubbard L=64 N=64 m=400 U=4 h=0.1, last infinite step
 BIG = {
     "hubbard_L64_m400": ("snap_hubbard", "L=64 N=64 m=400 sweep=3 t1=-1 U=4 h_z=0.1", 31),
 }
 
 
-def run(prog: str, args: str, workdir: str, extra: str) -> None:
+def run(prog: str, args: str, workdir: str, extra: str, out: str = "") -> None:
     env = dict(os.environ, MKL_THREADING_LAYER="GNU", OMP_NUM_THREADS="8")
-    cmd = [os.path.join(REF_BIN, prog)] + args.split() + extra.split() + [f"out={workdir}", "threads=8"]
+    cmd = [os.path.join(REF_BIN, prog)] + args.split() + extra.split() + [f"out={out or workdir}", "threads=8"]
     with open(os.path.join(workdir, "log.txt"), "w") as log:
         subprocess.run(cmd, cwd=workdir, env=env, stdout=log, stderr=subprocess.STDOUT, check=True)
 
@@ -108,6 +129,18 @@ def main() -> None:
         shutil.copy(os.path.join(wd, "energies.txt"), os.path.join(HERE, f"{name}_energies.txt"))
         shutil.rmtree(wd)
         print("wrote", f"{name}_energies.txt")
+    for name, args in ED_CASES.items():
+        if a.only and a.only != name:
+            continue
+        wd = tempfile.mkdtemp(prefix="golden_")
+        path = os.path.join(wd, "ed.bin")
+        run("ed_ref", args, wd, "", out=path)
+        kv = dict(x.split("=", 1) for x in args.split())
+        case = edhelp.load_ed_bin(path, kv)
+        out = os.path.join(HERE, f"{name}.npz")
+        edhelp.save_ed_npz(case, out)
+        print("wrote", os.path.relpath(out, REPO), "n =", case.n, "E =", case.energy)
+        shutil.rmtree(wd)
     if a.big:
         for name, (prog, args, step) in BIG.items():
             wd = tempfile.mkdtemp(prefix="golden_")

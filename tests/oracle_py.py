@@ -72,6 +72,11 @@ def lib():
                                                     C.POINTER(C.c_int), C.POINTER(C.c_int)]
         L.or_v_m_q0.argtypes = [C.c_int, C.c_int32, C.c_int32, C.c_int32, i16p, i16p, i16p, i16p, i32p, i64p,
                                 i32p, f64p, f64p, f64p]
+        L.or_ed_assemble.argtypes = [C.c_int32, C.c_int32, C.c_int32, i64p, C.c_int32, C.c_void_p, C.c_void_p,
+                                     C.c_int32, C.c_void_p, i64p, C.c_void_p, C.c_void_p]
+        L.or_ed_assemble.restype = C.c_int64
+        L.or_ed_v_m.argtypes = [C.c_int32, C.c_int32, C.POINTER(OrCrs), C.c_int32, i64p, C.c_int32, i64p, f64p,
+                                C.c_int32, C.c_void_p, f64p]
         _LIB = L
     return _LIB
 
@@ -288,3 +293,68 @@ def expectation(e, vec=None):
     for s in range(R, -1, -1):
         out[R + L + 3 - s - o] = np.dot(t1, v_m_q0(e, 2, e.M_RR[s], v, inv))
     return out
+
+
+# ------------------------------------------------------------ exact diagonalization
+def _or_crs_list(ops):
+    """ops: list of (row, col, val) -> (ctypes array of OrCrs, keep-alive list)."""
+    keep, arr = [], (OrCrs * max(1, len(ops)))()
+    for i, (r, c, v) in enumerate(ops):
+        r = np.ascontiguousarray(r, np.int64)
+        c = np.ascontiguousarray(c, np.int32)
+        v = np.ascontiguousarray(v, np.float64)
+        keep += [r, c, v]
+        arr[i] = OrCrs(len(r) - 1, len(r) - 1, r.ctypes.data, c.ctypes.data, v.ctypes.data)
+    return arr, keep
+
+
+def ed_assemble(L, d, basis, ops, terms, n_ele=None):
+    """Full ED Hamiltonian CRS (row, col, val) from a term list (oracle restatement of GET_HAM)."""
+    basis = np.ascontiguousarray(basis, np.int64)
+    n = len(basis)
+    arr, keep = _or_crs_list(ops)
+    t = np.ascontiguousarray(terms)
+    ne = None if n_ele is None else np.ascontiguousarray(n_ele, np.int32)
+    row = np.zeros(n + 1, np.int64)
+    lib().or_ed_assemble(L, d, n, basis, len(ops), C.cast(arr, C.c_void_p), None if ne is None else ne.ctypes.data,
+                         len(t), t.ctypes.data, row, None, None)
+    nnz = int(row[n])
+    col = np.zeros(max(nnz, 1), np.int32)
+    val = np.zeros(max(nnz, 1))
+    lib().or_ed_assemble(L, d, n, basis, len(ops), C.cast(arr, C.c_void_p), None if ne is None else ne.ctypes.data,
+                         len(t), t.ctypes.data, row, col.ctypes.data, val.ctypes.data)
+    del keep
+    return row, col[:nnz], val[:nnz]
+
+
+def ed_v_m(d, site, op, basis_out, basis_in, x, sign_flag=False, n_ele=None):
+    """EXACT_V_M_Q0..Q3 restatement: y = O(site) x from basis_in to basis_out."""
+    r, c, v = (np.ascontiguousarray(op[0], np.int64), np.ascontiguousarray(op[1], np.int32),
+               np.ascontiguousarray(op[2], np.float64))
+    cr = OrCrs(len(r) - 1, len(r) - 1, r.ctypes.data, c.ctypes.data, v.ctypes.data)
+    bo = np.ascontiguousarray(basis_out, np.int64)
+    bi = np.ascontiguousarray(basis_in, np.int64)
+    x = np.ascontiguousarray(x, np.float64)
+    ne = None if n_ele is None else np.ascontiguousarray(n_ele, np.int32)
+    y = np.zeros(len(bo))
+    lib().or_ed_v_m(d, site, C.byref(cr), len(bo), bo, len(bi), bi, x, 1 if sign_flag else 0,
+                    None if ne is None else ne.ctypes.data, y)
+    return y
+
+
+def ed_sector_basis(L, d, qn1, t1, qn2=None, t2=0):
+    """All codes of d^L (site 0 = least significant digit) in the sector, ascending (numpy)."""
+    codes = np.arange(d ** L, dtype=np.int64)
+    s1 = np.zeros(len(codes), np.int64)
+    s2 = np.zeros(len(codes), np.int64)
+    c = codes.copy()
+    for _ in range(L):
+        a = c % d
+        s1 += np.asarray(qn1)[a]
+        if qn2 is not None:
+            s2 += np.asarray(qn2)[a]
+        c //= d
+    keep = s1 == t1
+    if qn2 is not None:
+        keep &= s2 == t2
+    return codes[keep]

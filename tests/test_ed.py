@@ -1,0 +1,466 @@
+"""Exact diagonalization: EXACT_* entry points and the device ED assembler.
+
+Fixtures ed_*.npz hold the reference's own results (oracle/harness/ed_ref.c driving the
+compiled exact/ and onsite_op/ sources): on-site operators, sector basis, the full
+Hamiltonian built by EXACT_MAKE_ELEM_ON/INTER, EXACT_DIAGONALIZE_HAMILTONIAN's ground state,
+EXACT_EXPECTATION_* and EXACT_V_M_Q1/Q2/Q3 outputs.
+"""
+import ctypes as C
+import glob
+import os
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as sla
+
+import edhelp
+import oracle_py as O
+from conftest import GOLDEN
+
+CASES = sorted(glob.glob(os.path.join(GOLDEN, "ed_*.npz")))
+IDS = [os.path.basename(p)[:-4] for p in CASES]
+
+
+def _ed():
+    from dmrg_c_amd import ed
+    return ed
+
+
+def _desc(c):
+    return _ed().describe(edhelp.model_of(c))
+
+
+def _ops(c):
+    return [(o.row, o.col, o.val) for o in c.ops]
+
+
+def _dense_lowest(row, col, val, n):
+    H = sp.csr_matrix((val, col, row), shape=(n, n))
+    if n <= 1500:
+        return float(np.linalg.eigvalsh(H.toarray())[0])
+    return float(sla.eigsh(H, k=1, which="SA", tol=1e-13)[0][0])
+
+
+# ------------------------------------------------------------------ CPU: oracle + host logic
+@pytest.mark.parametrize("path", CASES, ids=IDS)
+def test_model_operators_match_reference(path):
+    """The library's on-site operators are bit-identical to the reference's onsite_op/ ones."""
+    c = edhelp.load_ed(path)
+    D = _desc(c)
+    assert D.d == c.d and len(D.ops) == len(c.ops)
+    for k, (mine, ref) in enumerate(zip(D.ops, c.ops)):
+        assert np.array_equal(mine[0], ref.row), k
+        assert np.array_equal(mine[1], ref.col), k
+        assert np.array_equal(mine[2], ref.val), k  # bit-exact
+    assert np.array_equal(D.n_ele, c.n_ele)
+
+
+@pytest.mark.parametrize("path", CASES, ids=IDS)
+def test_sector_basis_matches_reference(path):
+    c = edhelp.load_ed(path)
+    D = _desc(c)
+    qn2 = D.qn2 if D.targets[2] else None
+    b = O.ed_sector_basis(c.L, c.d, D.qn1, D.targets[0], qn2, D.targets[1])
+    assert np.array_equal(b, c.basis)
+
+
+@pytest.mark.parametrize("path", CASES, ids=IDS)
+def test_oracle_assembly_matches_reference(path):
+    """Oracle GET_HAM restatement on the library's term list == the reference's element
+    makers driven by the same model, bit for bit (Row, Col, Val)."""
+    c = edhelp.load_ed(path)
+    D = _desc(c)
+    row, col, val = O.ed_assemble(c.L, c.d, c.basis, _ops(c), D.terms, D.n_ele)
+    assert np.array_equal(row, c.ham.row)
+    assert np.array_equal(col, c.ham.col)
+    assert np.array_equal(val, c.ham.val)
+
+
+@pytest.mark.parametrize("path", CASES, ids=IDS)
+def test_oracle_v_m_matches_reference(path):
+    """EXACT_V_M_Q1/Q2/Q3 and EXACT_EXPECTATION_* restated: the reference's outputs."""
+    c = edhelp.load_ed(path)
+    ops = _ops(c)
+    for v in c.vm:
+        y = O.ed_v_m(c.d, v.site, ops[v.op], v.basis_out, c.basis, c.vec, v.sign_flag, c.n_ele)
+        assert np.allclose(y, v.out, rtol=1e-12, atol=1e-14), (v.q, v.site)
+    for ob in c.obs:
+        if ob.kind == 0:
+            out = [c.vec @ O.ed_v_m(c.d, s, ops[ob.op1], c.basis, c.basis, c.vec) for s in range(c.L)]
+        else:
+            t1 = O.ed_v_m(c.d, ob.origin, ops[ob.op1], c.basis, c.basis, c.vec)
+            out = [t1 @ O.ed_v_m(c.d, s, ops[ob.op2], c.basis, c.basis, c.vec) for s in range(ob.origin, c.L)]
+        assert np.allclose(out, ob.out, rtol=1e-12, atol=1e-13)
+
+
+@pytest.mark.parametrize("path", CASES, ids=IDS)
+def test_reference_energy_is_lowest_eigenvalue(path):
+    """The fixture's energy is the lowest eigenvalue of the fixture's own matrix."""
+    c = edhelp.load_ed(path)
+    e = _dense_lowest(c.ham.row, c.ham.col, c.ham.val, c.n)
+    assert abs(e - c.energy) <= 1e-10 * abs(e)
+
+
+def _oracle_energy(m):
+    ed = _ed()
+    D = ed.describe(m)
+    qn2 = D.qn2 if D.targets[2] else None
+    b = O.ed_sector_basis(m.L, D.d, D.qn1, D.targets[0], qn2, D.targets[1])
+    row, col, val = O.ed_assemble(m.L, D.d, b, D.ops, D.terms, D.n_ele)
+    return _dense_lowest(row, col, val, len(b)), (row, col, val, b)
+
+
+@pytest.mark.parametrize("L", [10, 12])
+def test_xx_chain_closed_form(L):
+    """J_z = 0, S^z = 0, OBC: free fermions, E0 = -J_xy sum_{k<=L/2} cos(pi k/(L+1))."""
+    m = _ed().Model("xxz", L=L, sz=0, J_z=0.0)
+    e, _ = _oracle_energy(m)
+    k = np.arange(1, L // 2 + 1)
+    assert abs(e + np.sum(np.cos(np.pi * k / (L + 1)))) < 1e-11
+
+
+@pytest.mark.parametrize("L", [4, 6])
+def test_hubbard_u0_closed_form(L):
+    """U = 0, half filling, S^z = 0, OBC: E0 = 2 sum_{k<=N/2} (-2|t| cos(pi k/(L+1)))."""
+    m = _ed().Model("hubbard", L=L, N=L, sz=0, U=0.0, t=-1.0)
+    e, _ = _oracle_energy(m)
+    k = np.arange(1, L // 2 + 1)
+    assert abs(e - 2 * np.sum(-2 * np.cos(np.pi * k / (L + 1)))) < 1e-11
+
+
+def test_two_site_closed_forms():
+    ed = _ed()
+    e, _ = _oracle_energy(ed.Model("xxz", L=2, sz=0))
+    assert abs(e + 0.75) < 1e-14          # Heisenberg singlet
+    U, t = 4.0, -1.0
+    e, _ = _oracle_energy(ed.Model("hubbard", L=2, N=2, sz=0, U=U, t=t))
+    assert abs(e - (U - np.sqrt(U * U + 16 * t * t)) / 2) < 1e-13
+
+
+def test_field_shifts_energy_by_sz():
+    """Uniform field in a fixed sector: E(h) = E(0) + h S^z_tot."""
+    ed = _ed()
+    e0, _ = _oracle_energy(ed.Model("xxz", L=10, sz=2, J_z=0.7))
+    e1, _ = _oracle_energy(ed.Model("xxz", L=10, sz=2, J_z=0.7, h_z=0.3))
+    assert abs(e1 - (e0 + 0.3 * 1.0)) < 1e-12
+    e0, _ = _oracle_energy(ed.Model("hubbard", L=6, N=5, sz=1, U=3.0))
+    e1, _ = _oracle_energy(ed.Model("hubbard", L=6, N=5, sz=1, U=3.0, h_z=0.4))
+    assert abs(e1 - (e0 + 0.4 * 0.5)) < 1e-12
+
+
+def _fock_hubbard(L, t, U, pbc):
+    """Spinful fermions on L sites by an explicit Jordan-Wigner construction over the 4^L
+    Fock space (mode order: site 0 up, site 0 down, site 1 up, ...); independent of the
+    on-site matrices and term lists.  Returns H and the (N, 2Sz) of every Fock state."""
+    nm = 2 * L
+    dim = 1 << nm
+    states = np.arange(dim)
+    occ = (states[:, None] >> np.arange(nm)[None, :]) & 1
+
+    def annihilate(mode):
+        rows, cols, vals = [], [], []
+        for s in range(dim):
+            if (s >> mode) & 1:
+                sign = (-1) ** int(bin(s & ((1 << mode) - 1)).count("1"))
+                rows.append(s ^ (1 << mode))
+                cols.append(s)
+                vals.append(sign)
+        return sp.csr_matrix((vals, (rows, cols)), shape=(dim, dim))
+
+    c = [annihilate(m) for m in range(nm)]
+    H = sp.csr_matrix((dim, dim))
+    bonds = [(i, i + 1) for i in range(L - 1)] + ([(0, L - 1)] if pbc and L > 2 else [])
+    for (i, j) in bonds:
+        for s in range(2):
+            a, b = c[2 * i + s], c[2 * j + s]
+            H = H + t * (a.T @ b + b.T @ a)
+    for i in range(L):
+        H = H + U * sp.diags(occ[:, 2 * i] * occ[:, 2 * i + 1])
+    N = occ.sum(1)
+    sz2 = occ[:, 0::2].sum(1) - occ[:, 1::2].sum(1)
+    return H.tocsr(), N, sz2
+
+
+@pytest.mark.parametrize("L,N,sz,pbc", [(4, 4, 0, False), (4, 4, 0, True), (5, 3, 1, True), (4, 3, 1, True),
+                                        (5, 4, 0, True), (3, 2, 0, True)])
+def test_hubbard_matches_jordan_wigner_fermions(L, N, sz, pbc):
+    """The sign rules (hopping parity, periodic wrap over sites 0..L-2) reproduce the
+    spectrum of genuine fermions: lowest eigenvalue in the sector equals that of an
+    explicit Jordan-Wigner Hamiltonian sum t (c+_i c_j + h.c.) + U n_up n_dn."""
+    t, U = -1.0, 2.5
+    e, _ = _oracle_energy(_ed().Model("hubbard", L=L, N=N, sz=sz, U=U, t=t, bc="PBC" if pbc else "OBC"))
+    H, Nf, szf = _fock_hubbard(L, t, U, pbc)
+    sel = np.where((Nf == N) & (szf == sz))[0]
+    ref = np.linalg.eigvalsh(H[sel][:, sel].toarray())[0]
+    assert abs(e - ref) < 1e-12
+
+
+def test_xxz_l8_matches_reference_dmrg():
+    """L = 8 with m = 16 keeps every block state, so the reference's DMRG energies at the end
+    of the sweep are exact: they must equal the ED ground state."""
+    path = os.path.join(GOLDEN, "xxz_L8_m16_energies.txt")
+    rows = np.loadtxt(path, ndmin=2)
+    e, _ = _oracle_energy(_ed().Model("xxz", L=8, sz=0))
+    assert abs(rows[-1][5] - e) < 1e-9
+
+
+def test_hubbard_four_sites_match_reference_dmrg_first_step():
+    """The first DMRG step of the L=8 Hubbard run is an exact 4-site problem: its energy is
+    the ED ground state of the 4-site open chain in the sector of the same dimension."""
+    from dmrg_c_amd import snapshot
+    s = snapshot.load(os.path.join(GOLDEN, "hubbard_L8_m32_s01.npz"))
+    assert s.dim_LL == 4 and s.dim_RR == 4
+    ed = _ed()
+    found = False
+    for N in range(0, 9):
+        for sz in range(-4, 5):
+            if (N + sz) % 2 or abs(sz) > N or N - abs(sz) > 8:
+                continue
+            m = ed.Model("hubbard", L=4, N=N, sz=sz, U=4.0, t=-1.0, h_z=0.1)
+            D = ed.describe(m)
+            b = O.ed_sector_basis(4, 4, D.qn1, N, D.qn2, sz)
+            if len(b) != s.n:
+                continue
+            row, col, val = O.ed_assemble(4, 4, b, D.ops, D.terms, D.n_ele)
+            e = _dense_lowest(row, col, val, len(b))
+            if abs(e - float(s.meta["gs_energy"][0])) < 1e-9:
+                found = True
+    assert found
+
+
+def test_host_element_makers_rebuild_reference_hamiltonian():
+    """EXACT_MAKE_ELEM_ON / _INTER / EXACT_GET_A_BASIS of this library, driven row by row as
+    the reference's GET_HAM does, give the reference's CRS bit for bit (host only)."""
+    from dmrg_c_amd import HostCRS
+    ed = _ed()
+    L = ed.lib()
+    c = edhelp.load_ed(os.path.join(GOLDEN, "ed_hubbard_L5_pbc_odd.npz"))
+    D = _desc(c)
+    hops = [HostCRS(o.row, o.col, o.val, c.d) for o in c.ops]
+    A = L.EXACT_GET_A_BASIS(1, 1000)
+    rows, cols, vals = [0], [], []
+    for r, code in enumerate(c.basis):
+        ne = C.c_long(0)
+        for t in D.terms:
+            if t["kind"] == 0:
+                L.EXACT_MAKE_ELEM_ON(int(code), int(t["site1"]), c.d, hops[t["op1"]].ptr, C.byref(ne),
+                                     float(t["coeff"]), A[0])
+            else:
+                s = int(t["sign"])
+                if t["par_lo"] < t["par_hi"]:
+                    p = sum(int(c.n_ele[(int(code) // c.d ** k) % c.d]) for k in range(t["par_lo"], t["par_hi"]))
+                    s = s if p % 2 else -s
+                L.EXACT_MAKE_ELEM_INTER(int(code), int(t["site1"]), int(t["site2"]), c.d, hops[t["op1"]].ptr,
+                                        hops[t["op2"]].ptr, C.byref(ne), float(t["coeff"]), s, A[0])
+        for i in range(ne.value):
+            tgt = A[0].contents.Basis[i]
+            v = A[0].contents.Val[i]
+            j = np.searchsorted(c.basis, tgt)
+            inv = j if j < len(c.basis) and c.basis[j] == tgt else -1
+            if (abs(v) > 1e-15 and inv >= 0) or inv == r:
+                cols.append(inv)
+                vals.append(v)
+        rows.append(len(cols))
+    L.EXACT_FREE_A_BASIS(A, 1)
+    assert np.array_equal(rows, c.ham.row)
+    assert np.array_equal(cols, c.ham.col)
+    assert np.array_equal(np.array(vals), c.ham.val)
+
+
+def test_ssd_coeff_and_site_state():
+    ed = _ed()
+    L = ed.lib()
+    for site in range(6):
+        a = L.EXACT_SSD_COEFF(site, 6, b"SSD", b"Onsite")
+        assert a == np.sin(np.pi * (site + 1 - 0.5) / 6) ** 2
+        assert L.EXACT_SSD_COEFF(site, 6, b"OBC", b"Intersite") == 1.0
+    assert L.EXACT_FIND_SITE_STATE(4 * 16 + 3 * 4 + 2, 1, 4) == 3
+
+
+# ------------------------------------------------------------------ GPU
+@pytest.mark.gpu
+@pytest.mark.parametrize("path", CASES, ids=IDS)
+def test_gpu_ed_assembly_bit_exact(path):
+    """Device sector enumeration + device assembly == the reference's basis and CRS."""
+    c = edhelp.load_ed(path)
+    P = _ed().EdProblem.from_model(edhelp.model_of(c))
+    try:
+        assert P.dim == c.n
+        assert np.array_equal(P.basis(), c.basis)
+        A = P.assemble()
+        row, col, val = A.download()
+        assert np.array_equal(row, c.ham.row)
+        assert np.array_equal(col, c.ham.col)
+        assert np.array_equal(val, c.ham.val)
+        A.free()
+    finally:
+        P.free()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("path", CASES, ids=IDS)
+@pytest.mark.parametrize("method", ["Lanczos", "Lanczos_Slow"])
+def test_gpu_exact_diagonalize(path, method, tmp_path, monkeypatch):
+    """EXACT_DIAGONALIZE_HAMILTONIAN on the reference's CRS: energy within 1e-10 relative,
+    eigenvector equal up to sign within 1e-8, residual small."""
+    monkeypatch.chdir(tmp_path)
+    c = edhelp.load_ed(path)
+    ed = _ed()
+    E, err, vec, _ = ed.exact_diagonalize(c.ham.row, c.ham.col, c.ham.val, ed.default_exact_parameter(method))
+    assert abs(E - c.energy) <= 1e-10 * abs(c.energy)
+    s = np.sign(vec @ c.vec)
+    assert np.abs(vec - s * c.vec).max() <= 1e-8
+    assert err < 1e-7
+
+
+def _whole_basis_q1(bases):
+    ed = _ed()
+    W = ed.EXACT_WHOLE_BASIS_Q1()
+    keep = [np.ascontiguousarray(b, np.int64) for b in bases]
+    dims = (C.c_int * len(bases))(*[len(b) for b in bases])
+    ptrs = (ed._L * len(bases))(*[b.ctypes.data_as(ed._L) for b in keep])
+    W.Dim = C.cast(dims, ed._I)
+    W.Basis = C.cast(ptrs, C.POINTER(ed._L))
+    return W, (keep, dims, ptrs)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("path", CASES, ids=IDS)
+def test_gpu_exact_v_m_and_expectations(path):
+    """EXACT_V_M_Q1/Q2/Q3 and EXACT_EXPECTATION_ONSITE/_INTERSITE_Q0 on the device, on the
+    reference's vector: within 1e-12 of the reference's outputs."""
+    from dmrg_c_amd import HostCRS
+    ed = _ed()
+    L = ed.lib()
+    c = edhelp.load_ed(path)
+    hops = [HostCRS(o.row, o.col, o.val, c.d) for o in c.ops]
+    vec = np.ascontiguousarray(c.vec)
+    basis = np.ascontiguousarray(c.basis, np.int64)
+    n_ele = np.ascontiguousarray(c.n_ele, np.int32)
+    for v in c.vm:
+        out = np.zeros(len(v.out))
+        bo = np.ascontiguousarray(v.basis_out, np.int64)
+        flag = b"Yes" if v.sign_flag else b"No"
+        if v.q == 1:
+            W, keep = _whole_basis_q1([basis, bo])
+            L.EXACT_V_M_Q1(hops[v.op].ptr, 1, vec.ctypes.data, 0, out.ctypes.data, c.d, v.site, 1, C.byref(W))
+        elif v.q == 2:
+            W = ed.EXACT_WHOLE_BASIS_Q2()
+            r0, r1 = (C.c_int * 2)(len(basis), 0), (C.c_int * 2)(0, len(bo))
+            dims = (ed._I * 2)(C.cast(r0, ed._I), C.cast(r1, ed._I))
+            b0 = (ed._L * 2)(basis.ctypes.data_as(ed._L), None)
+            b1 = (ed._L * 2)(None, bo.ctypes.data_as(ed._L))
+            bas = (C.POINTER(ed._L) * 2)(C.cast(b0, C.POINTER(ed._L)), C.cast(b1, C.POINTER(ed._L)))
+            W.Dim = C.cast(dims, C.POINTER(ed._I))
+            W.Basis = C.cast(bas, C.POINTER(C.POINTER(ed._L)))
+            L.EXACT_V_M_Q2(hops[v.op].ptr, 1, 1, vec.ctypes.data, 0, 0, out.ctypes.data, flag, n_ele.ctypes.data,
+                           c.d, v.site, 1, C.byref(W))
+        else:
+            W = ed.EXACT_WHOLE_BASIS_Q3()
+            d00, d11 = (C.c_int * 1)(len(basis)), (C.c_int * 1)(len(bo))
+            row0 = (ed._I * 2)(C.cast(d00, ed._I), None)
+            row1 = (ed._I * 2)(None, C.cast(d11, ed._I))
+            dims = (C.POINTER(ed._I) * 2)(C.cast(row0, C.POINTER(ed._I)), C.cast(row1, C.POINTER(ed._I)))
+            p00, p11 = (ed._L * 1)(basis.ctypes.data_as(ed._L)), (ed._L * 1)(bo.ctypes.data_as(ed._L))
+            brow0 = (C.POINTER(ed._L) * 2)(C.cast(p00, C.POINTER(ed._L)), None)
+            brow1 = (C.POINTER(ed._L) * 2)(None, C.cast(p11, C.POINTER(ed._L)))
+            bas = (C.POINTER(C.POINTER(ed._L)) * 2)(C.cast(brow0, C.POINTER(C.POINTER(ed._L))),
+                                                     C.cast(brow1, C.POINTER(C.POINTER(ed._L))))
+            W.Dim = C.cast(dims, C.POINTER(C.POINTER(ed._I)))
+            W.Basis = C.cast(bas, C.POINTER(C.POINTER(C.POINTER(ed._L))))
+            L.EXACT_V_M_Q3(hops[v.op].ptr, 1, 1, 0, vec.ctypes.data, 0, 0, 0, out.ctypes.data, flag,
+                           n_ele.ctypes.data, c.d, v.site, 1, C.byref(W))
+        assert np.allclose(out, v.out, rtol=1e-12, atol=1e-14), (v.q, v.site)
+    BI = ed.EXACT_BASIS_INFO()
+    BI.dim = c.n
+    BI.Basis = basis.ctypes.data_as(ed._L)
+    t1, t2 = np.zeros(c.n), np.zeros(c.n)
+    for ob in c.obs:
+        out = np.zeros(len(ob.out))
+        if ob.kind == 0:
+            L.EXACT_EXPECTATION_ONSITE(hops[ob.op1].ptr, out.ctypes.data, vec.ctypes.data, t1.ctypes.data, c.d, c.L,
+                                       1, C.byref(BI))
+        else:
+            L.EXACT_EXPECTATION_INTERSITE_Q0(hops[ob.op1].ptr, hops[ob.op2].ptr, ob.origin, c.L, out.ctypes.data,
+                                             vec.ctypes.data, t1.ctypes.data, t2.ctypes.data, c.d, 1, C.byref(BI))
+        assert np.allclose(out, ob.out, rtol=1e-12, atol=1e-13)
+    # EXACT_V_M_Q0 of the last on-site observable at the last site is what T_Vec holds
+    y = np.zeros(c.n)
+    L.EXACT_V_M_Q0(hops[c.obs[1].op1].ptr, vec.ctypes.data, y.ctypes.data, c.d, c.L - 1, 1, C.byref(BI))
+    assert np.allclose(y, O.ed_v_m(c.d, c.L - 1, _ops(c)[c.obs[1].op1], basis, basis, vec), rtol=1e-13, atol=1e-15)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("path", [p for p in CASES if "L10" in p or "L7" in p], ids=lambda p: os.path.basename(p))
+def test_gpu_ed_end_to_end_observables(path):
+    """Model -> device assembly -> device ground state -> expectation values: within 1e-8
+    of the reference's observables (the acceptance bound)."""
+    from dmrg_c_amd import HostCRS
+    ed = _ed()
+    L = ed.lib()
+    c = edhelp.load_ed(path)
+    P = ed.EdProblem.from_model(edhelp.model_of(c))
+    A = P.assemble()
+    E, vec, st = A.ground_state()
+    A.free()
+    P.free()
+    assert abs(E - c.energy) <= 1e-10 * abs(c.energy)
+    hops = [HostCRS(o.row, o.col, o.val, c.d) for o in c.ops]
+    basis = np.ascontiguousarray(c.basis, np.int64)
+    BI = ed.EXACT_BASIS_INFO()
+    BI.dim = c.n
+    BI.Basis = basis.ctypes.data_as(ed._L)
+    t1, t2 = np.zeros(c.n), np.zeros(c.n)
+    for ob in c.obs:
+        out = np.zeros(len(ob.out))
+        if ob.kind == 0:
+            L.EXACT_EXPECTATION_ONSITE(hops[ob.op1].ptr, out.ctypes.data, vec.ctypes.data, t1.ctypes.data, c.d, c.L,
+                                       1, C.byref(BI))
+        else:
+            L.EXACT_EXPECTATION_INTERSITE_Q0(hops[ob.op1].ptr, hops[ob.op2].ptr, ob.origin, c.L, out.ctypes.data,
+                                             vec.ctypes.data, t1.ctypes.data, t2.ctypes.data, c.d, 1, C.byref(BI))
+        assert np.abs(out - ob.out).max() <= 1e-8
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("model", [
+    dict(model="xxz", L=18, sz=0, J_z=0.8, h_z=0.2),
+    dict(model="hubbard", L=8, N=8, sz=0, U=4.0, h_z=0.1, bc="PBC"),
+    dict(model="xxz", L=12, spin=2, sz=2, D_z=-0.3, h_z=0.1),
+], ids=["xxz_L18", "hubbard_L8_pbc", "xxz_S1_L12"])
+def test_gpu_ed_assembly_matches_oracle_medium(model):
+    """Medium sizes (n = 48620, 4900, 73789): device basis + assembly == oracle, bit for bit."""
+    ed = _ed()
+    m = ed.Model(**model)
+    D = ed.describe(m)
+    qn2 = D.qn2 if D.targets[2] else None
+    b = O.ed_sector_basis(m.L, D.d, D.qn1, D.targets[0], qn2, D.targets[1])
+    row, col, val = O.ed_assemble(m.L, D.d, b, D.ops, D.terms, D.n_ele)
+    P = ed.EdProblem.from_model(m)
+    try:
+        assert np.array_equal(P.basis(), b)
+        A = P.assemble()
+        r2, c2, v2 = A.download()
+        A.free()
+        assert np.array_equal(r2, row) and np.array_equal(c2, col) and np.array_equal(v2, val)
+    finally:
+        P.free()
+
+
+@pytest.mark.gpu
+def test_gpu_xx_chain_l24_closed_form():
+    """L = 24, S^z = 0 (n = 2 704 156): the largest ED size of the batched workload.  XX
+    chain energy equals the free-fermion closed form; the assembled matrix is symmetric."""
+    ed = _ed()
+    P = ed.EdProblem.from_model(ed.Model("xxz", L=24, sz=0, J_z=0.0))
+    assert P.dim == 2704156
+    A = P.assemble()
+    P.free()
+    E, vec, st = A.ground_state()
+    k = np.arange(1, 13)
+    assert abs(E + np.sum(np.cos(np.pi * k / 25))) < 1e-9
+    x = np.random.default_rng(0).standard_normal(A.n)
+    y = np.random.default_rng(1).standard_normal(A.n)
+    assert abs(y @ A.matvec(x) - x @ A.matvec(y)) < 1e-9 * np.linalg.norm(x) * np.linalg.norm(y)
+    A.free()
