@@ -119,11 +119,8 @@ void lanczos_entry(const char *fn, const char *step_file, BOX_LAN *Box, bool sym
 void ii_entry(const char *fn, BOX_II *B, bool sym) {
   if (B->M->row_dim != B->M->col_dim)
     fatal(fn, "The input matrix is not a square matirx(row=%d,col=%d)\n", B->M->row_dim, B->M->col_dim);
-  if (strcmp(B->Type, "CG") != 0) {
-    if (strcmp(B->Type, "MR") == 0)
-      fatal(fn, "Box_II->Type=MR (MINIMUM_RESIDUAL) is not provided by libdmrg_c_amd; use CG\n");
-    fatal(fn, "Invalid Box_II->Type=%s\n", B->Type);
-  }
+  const bool mr = strcmp(B->Type, "MR") == 0;
+  if (!mr && strcmp(B->Type, "CG") != 0) fatal(fn, "Invalid Box_II->Type=%s\n", B->Type);
   const int n = B->M->row_dim;
   run_or_die(fn, [&] {
     auto A = upload_csr(n, B->M->Row, B->M->Col, B->M->Val, sym);
@@ -131,7 +128,7 @@ void ii_entry(const char *fn, BOX_II *B, bool sym) {
     v.up(B->Eig_Vec);
     SolveOut out;
     inverse_iteration(op_of(*A), B->Eig_Val[0], v.d.p, B->ii_acc, B->ii_diag_add, B->ii_max_step, B->cg_acc,
-                      B->cg_max_step, out);
+                      B->cg_max_step, out, mr);
     v.down(B->Eig_Vec);
     B->error = out.err;
     if (out.ii_converged) {
@@ -146,7 +143,8 @@ void ii_entry(const char *fn, BOX_II *B, bool sym) {
   });
 }
 
-void cg_entry(const char *fn, BOX_CG *B, bool sym) {
+// CONJUGATE_GRADIENT(_SYM) and MINIMUM_RESIDUAL(_SYM) on a BOX_CG
+void cg_entry(const char *fn, BOX_CG *B, bool sym, bool mr = false) {
   check_square(fn, B->M);
   const int n = B->M->row_dim;
   run_or_die(fn, [&] {
@@ -156,9 +154,11 @@ void cg_entry(const char *fn, BOX_CG *B, bool sym) {
     x.up(B->Out_Vec);
     SolveOut out;
     bool conv = false;
-    int it = cg_solve(op_of(*A), 0.0, b.d.p, x.d.p, B->acc, B->max_step, &conv, out);
+    int it = mr ? mr_solve(op_of(*A), 0.0, b.d.p, x.d.p, B->acc, B->max_step, &conv, out)
+                : cg_solve(op_of(*A), 0.0, b.d.p, x.d.p, B->acc, B->max_step, &conv, out);
     x.down(B->Out_Vec);
-    const char *sf = sym ? "CONJUGATE_GRADIENT_SYM_Step.txt" : "CONJUGATE_GRADIENT_Step.txt";
+    const char *sf = mr ? "MINIMUM_RESIDUAL_Step.txt"
+                        : sym ? "CONJUGATE_GRADIENT_SYM_Step.txt" : "CONJUGATE_GRADIENT_Step.txt";
     if (!conv) {
       printf("Error in %s\nDoes not converge (max step is %d)\nContinue...\n", fn, B->max_step);
       sml_append("error.txt", "Error in %s\nDoes not converge (max step is %d)\nContinue...\n", fn, B->max_step);
@@ -187,8 +187,9 @@ void diagonalize_entry(const char *fn, CRS1 *Ham, DMRG_TIME *Time, DMRG_PARAMETE
   if (strcmp(Param->Lan_Con, "Normal") == 0) type = 0;
   else if (strcmp(Param->Lan_Con, "Tight") == 0) type = 1;
   else fatal(slow ? "LANCZOS_SLOW_SYM" : "LANCZOS_SYM", "Box->Type=%s is error\n", Param->Lan_Con);
-  if (strcmp(Param->II_Type, "CG") != 0)
-    fatal("INVERSE_ITERATION_SYM", "Invalid Box_II->Type=%s (only CG is provided)\n", Param->II_Type);
+  const bool mr = strcmp(Param->II_Type, "MR") == 0;
+  if (!mr && strcmp(Param->II_Type, "CG") != 0)
+    fatal("INVERSE_ITERATION_SYM", "Invalid Box_II->Type=%s\n", Param->II_Type);
   check_square(slow ? "LANCZOS_SLOW_SYM" : "LANCZOS_SYM", Ham);
   if (Param->diag_max_step <= Param->diag_min_step)
     fatal(slow ? "LANCZOS_SLOW_SYM" : "LANCZOS_SYM", "Box->max_step=%d,Box->min_step=%d\n",
@@ -224,7 +225,7 @@ void diagonalize_entry(const char *fn, CRS1 *Ham, DMRG_TIME *Time, DMRG_PARAMETE
     }
     double t1 = now_s();
     inverse_iteration(op_of(*A), eig, v.d.p, Param->inv_iter_acc, Param->inv_iter_diag_add, Param->inv_iter_max_step,
-                      Param->cg_acc, Param->cg_max_step, out);
+                      Param->cg_acc, Param->cg_max_step, out, mr);
     v.down(St->GS_Vec);
     double t2 = now_s();
     if (out.ii_converged) {
@@ -314,6 +315,8 @@ void INVERSE_ITERATION_SYM(BOX_II *B) { ii_entry("INVERSE_ITERATION_SYM", B, tru
 void INVERSE_ITERATION(BOX_II *B) { ii_entry("INVERSE_ITERATION", B, false); }
 void CONJUGATE_GRADIENT_SYM(BOX_CG *B) { cg_entry("CONJUGATE_GRADIENT_SYM", B, true); }
 void CONJUGATE_GRADIENT(BOX_CG *B) { cg_entry("CONJUGATE_GRADIENT", B, false); }
+void MINIMUM_RESIDUAL_SYM(BOX_CG *B) { cg_entry("MINIMUM_RESIDUAL_SYM", B, true, true); }
+void MINIMUM_RESIDUAL(BOX_CG *B) { cg_entry("MINIMUM_RESIDUAL", B, false, true); }
 
 double CONVERGE_CHECK_SYM(CRS1 *M, double *Eig_Vec, double eig_val, int p_threads) {
   (void)p_threads;

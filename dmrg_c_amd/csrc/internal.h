@@ -281,7 +281,7 @@ void launch_start_vector(int64_t n, uint64_t seed, double *out, hipStream_t s);
 
 // Lanczos step kernels.  Scalars live in device arrays indexed by step.
 void launch_norm2(int64_t n, const double *x, Reduce &R, double *out, hipStream_t s);
-void launch_lanczos_a(const Operator &op, bool fp32, int k, const double *w, const double *nrm2,
+void launch_lanczos_a(const Operator &op, bool fp32, int k, const double *w, const double *nrm2, double shift,
                       double *vk, double *y, double *alpha, Reduce &R, hipStream_t s);
 void launch_lanczos_b(int64_t n, int k, const double *y, const double *vk, const double *vkm1,
                       const double *alpha, double *beta, double *w, double *nrm2, Reduce &R,
@@ -290,6 +290,13 @@ void launch_ritz(int64_t n, int k, const double *const *vecs_dev, const double *
                  hipStream_t s);
 void launch_scale_by_rnorm(int64_t n, const double *nrm2, const double *x, double *y, hipStream_t s);
 void launch_copy(int64_t n, const double *x, double *y, hipStream_t s);
+
+// minimum-residual solve of (H + shift) x = b: start, and one Givens update per Lanczos step
+void launch_mr_init(const Operator &op, double shift, const double *b, const double *x, double *v, double *nrm2,
+                    double *beta, double *mr, int *status, Reduce &R, hipStream_t s);
+void launch_mr_step(int64_t n, int k, double acc, const double *alpha, const double *beta, double *mr, int *status,
+                    double *coef, const double *v1, const double *w0, const double *w1, double *w2, double *x,
+                    hipStream_t s);
 
 // inverse iteration / CG kernels
 void launch_residual_l1(const Operator &op, const double *v, double eig, double *out, Reduce &R,
@@ -335,10 +342,14 @@ struct LanczosOpts {
 // vec: device in/out (start on input, Ritz vector on output)
 void lanczos(const Operator &A, const LanczosOpts &o, double *vec, SolveOut &out);
 // Inverse iteration (shift folded into the operator), CG inner solver; vec device in/out.
+// mr: minimum residual ("MR") instead of conjugate gradient ("CG") for the inner solves
 void inverse_iteration(const Operator &A, double eig, double *vec, double ii_acc, double ii_diag_add,
-                       int ii_max_step, double cg_acc, int cg_max_step, SolveOut &out);
+                       int ii_max_step, double cg_acc, int cg_max_step, SolveOut &out, bool mr = false);
 // CG solve of (A + shift I) x = b with x normalised first; returns iterations, sets converged.
 int cg_solve(const Operator &A, double shift, const double *b, double *x, double acc, int max_step,
+             bool *converged, SolveOut &out);
+// Minimum-residual solve of (A + shift I) x = b from x as given (not normalised); x in/out.
+int mr_solve(const Operator &A, double shift, const double *b, double *x, double acc, int max_step,
              bool *converged, SolveOut &out);
 double converge_check(const Operator &A, const double *v, double eig);
 

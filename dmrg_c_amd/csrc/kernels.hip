@@ -124,6 +124,7 @@ struct EpiLanczosA {
   const double *nrm2;
   double *vk, *y, *alpha;
   int k;
+  double shift;   // operator H + shift I (the minimum-residual solve of inverse iteration)
   double scale;
   static constexpr bool kReduce = true;
   __device__ bool skip() const { return false; }
@@ -131,11 +132,27 @@ struct EpiLanczosA {
   __device__ double row(int32_t r, double s) {
     const double xs = scale * w[r];
     vk[r] = xs;
-    const double yv = scale * s;
+    const double yv = scale * s + shift * xs;
     y[r] = yv;
     return yv * xs;
   }
   __device__ void finish(double t) { alpha[k] = t; }
+};
+
+// Minimum-residual start (sml/MINIMUM_RESIDUAL.c:31-38): v = b - (H + shift) x, |v|^2
+struct EpiMrInit {
+  const double *b, *x;
+  double *v, *nrm2;
+  double shift;
+  static constexpr bool kReduce = true;
+  __device__ bool skip() const { return false; }
+  __device__ void begin() {}
+  __device__ double row(int32_t r, double s) {
+    const double rv = b[r] - (s + shift * x[r]);
+    v[r] = rv;
+    return rv * rv;
+  }
+  __device__ void finish(double t) { *nrm2 = t; }
 };
 
 // CG start (sml/CONJUGATE_GRADIENT_SYM.c:202-210): x = x0/||x0||, r = b - (H + shift) x, p = r
@@ -552,9 +569,9 @@ void launch_norm2(int64_t n, const double *x, Reduce &R, double *out, hipStream_
   HIPCHK(hipGetLastError());
 }
 
-void launch_lanczos_a(const Operator &op, bool fp32, int k, const double *w, const double *nrm2,
+void launch_lanczos_a(const Operator &op, bool fp32, int k, const double *w, const double *nrm2, double shift,
                       double *vk, double *y, double *alpha, Reduce &R, hipStream_t s) {
-  apply_op(op, fp32, w, EpiLanczosA{w, nrm2, vk, y, alpha, k, 0.0}, &R, s);
+  apply_op(op, fp32, w, EpiLanczosA{w, nrm2, vk, y, alpha, k, shift, 0.0}, &R, s);
 }
 
 void launch_lanczos_b(int64_t n, int k, const double *y, const double *vk, const double *vkm1,
@@ -611,6 +628,87 @@ void launch_cg_update(int64_t n, int it, double acc, const double *p, const doub
 void launch_cg_dir(int64_t n, int it, const double *r, const double *rr, double *p, const int *status,
                    hipStream_t s) {
   k_cg_dir<<<vec_grid(n), kBlock, 0, s>>>(n, it, r, rr, p, status);
+  HIPCHK(hipGetLastError());
+}
+
+// ------------------------------------------------------------- minimum residual
+// State of sml/MINIMUM_RESIDUAL.c:40-45 kept on the device: c0 c1 s0 s1 eta; status[0] = done,
+// status[1] = iteration that converged.  Per iteration k, coef[5k..5k+4] =
+// {1/alpha1, alpha3, alpha2, c2*eta, commit} for the vector update.
+__global__ void k_mr_init(const double *nrm2, double *beta, double *mr, int *status) {
+  const double g = sqrt(*nrm2);   // gamma0 = eta = |b - A x0| (:39-41)
+  beta[0] = g;
+  mr[0] = 1.0;
+  mr[1] = 1.0;
+  mr[2] = 0.0;
+  mr[3] = 0.0;
+  mr[4] = g;
+  status[0] = 0;
+  status[1] = -1;
+}
+
+// Givens step of :63-74 and the convergence test of :78
+__global__ void k_mr_scalars(int k, double acc, const double *alpha, const double *beta, double *mr, int *status,
+                             double *coef) {
+  double *cf = coef + 5 * (size_t)k;
+  if (status[0]) {
+    cf[4] = -1.0;  // iteration not run
+    return;
+  }
+  const double c0 = mr[0], c1 = mr[1], s0 = mr[2], s1 = mr[3], eta = mr[4];
+  const double delta = alpha[k], gamma0 = beta[k], gamma1 = beta[k + 1];
+  const double a0 = c1 * delta - c0 * s1 * gamma0;
+  const double a1 = sqrt(a0 * a0 + gamma1 * gamma1);
+  const double a2 = s1 * delta + c0 * c1 * gamma0;
+  const double a3 = s0 * gamma0;
+  const double c2 = a0 / a1, s2 = gamma1 / a1;
+  cf[0] = 1.0 / a1;
+  cf[1] = a3;
+  cf[2] = a2;
+  cf[3] = c2 * eta;
+  const double eta2 = -s2 * eta;
+  const bool done = fabs(eta2) < acc;
+  // on convergence the reference returns before copying X1 into Out_Vec (:79-99)
+  cf[4] = done ? 0.0 : 1.0;
+  if (done) {
+    status[0] = 1;
+    status[1] = k;
+  }
+  mr[0] = c1;
+  mr[1] = c2;
+  mr[2] = s1;
+  mr[3] = s2;
+  mr[4] = eta2;
+}
+
+// W2 = (V1 - alpha3 W0 - alpha2 W1) / alpha1;  x += c2 eta W2 unless converged (:75-77, :100-106)
+__global__ __launch_bounds__(kBlock) void k_mr_update(int64_t n, int k, const double *coef,
+                                                      const double *__restrict__ v1, const double *__restrict__ w0,
+                                                      const double *__restrict__ w1, double *__restrict__ w2,
+                                                      double *__restrict__ x) {
+  const double *cf = coef + 5 * (size_t)k;
+  const double run = cf[4];
+  if (run < 0.0) return;
+  const double t = cf[0], a3 = cf[1], a2 = cf[2], ce = cf[3];
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+    const double wv = t * (v1[i] - a3 * w0[i] - a2 * w1[i]);
+    w2[i] = wv;
+    if (run > 0.0) x[i] = x[i] + ce * wv;
+  }
+}
+
+void launch_mr_init(const Operator &op, double shift, const double *b, const double *x, double *v, double *nrm2,
+                    double *beta, double *mr, int *status, Reduce &R, hipStream_t s) {
+  apply_op(op, false, x, EpiMrInit{b, x, v, nrm2, shift}, &R, s);
+  k_mr_init<<<1, 1, 0, s>>>(nrm2, beta, mr, status);
+  HIPCHK(hipGetLastError());
+}
+
+void launch_mr_step(int64_t n, int k, double acc, const double *alpha, const double *beta, double *mr, int *status,
+                    double *coef, const double *v1, const double *w0, const double *w1, double *w2, double *x,
+                    hipStream_t s) {
+  k_mr_scalars<<<1, 1, 0, s>>>(k, acc, alpha, beta, mr, status, coef);
+  k_mr_update<<<vec_grid(n), kBlock, 0, s>>>(n, k, coef, v1, w0, w1, w2, x);
   HIPCHK(hipGetLastError());
 }
 

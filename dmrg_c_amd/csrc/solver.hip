@@ -83,7 +83,7 @@ void lanczos(const Operator &A, const LanczosOpts &o, double *vec, SolveOut &out
 
   while (true) {
     // step 0 of this block: sml/LANCZOS_SLOW_SYM.c:98-106
-    launch_lanczos_a(A, o.fp32, 0, w.p, nrm2, V.vec(0), y.p, alpha, R, st);
+    launch_lanczos_a(A, o.fp32, 0, w.p, nrm2, 0.0, V.vec(0), y.p, alpha, R, st);
     launch_lanczos_b(n, 0, y.p, V.vec(0), nullptr, alpha, beta, w.p, nrm2, R, st);
     out.spmv++;
     int next = 1, kconv = -1, chunk = small ? 16 : 8;
@@ -92,7 +92,7 @@ void lanczos(const Operator &A, const LanczosOpts &o, double *vec, SolveOut &out
     while (next < block && total_step < max_step) {
       const int c = std::min({chunk, block - next, max_step - total_step});
       for (int k = next; k < next + c; k++) {
-        launch_lanczos_a(A, o.fp32, k, w.p, nrm2, V.vec(k), y.p, alpha, R, st);
+        launch_lanczos_a(A, o.fp32, k, w.p, nrm2, 0.0, V.vec(k), y.p, alpha, R, st);
         launch_lanczos_b(n, k, y.p, V.vec(k), V.vec(k - 1), alpha, beta, w.p, nrm2, R, st);
       }
       double *h = C.host_stage(2 * ((size_t)block + 2));
@@ -209,8 +209,56 @@ int cg_solve(const Operator &A, double shift, const double *b, double *x, double
   return iters;
 }
 
+// sml/MINIMUM_RESIDUAL.c:7-150: MINRES through the Lanczos recurrence of H + shift with
+// Givens rotations; the Lanczos step is the library's fused pair of kernels, the rotation
+// scalars stay on the device and the host reads the convergence flag once per chunk.
+int mr_solve(const Operator &A, double shift, const double *b, double *x, double acc, int max_step,
+             bool *converged, SolveOut &out) {
+  Context &C = ctx();
+  hipStream_t st = C.stream;
+  const int64_t n = A.n;
+  const int slots = std::max(max_step, 1) + 1;
+  DevBuf<double> vraw(n), t(n), vn0(n), vn1(n), wbuf(3 * (size_t)n);
+  DevBuf<double> sc(2 * (size_t)slots + 1 + 5), coef(5 * (size_t)slots);
+  double *alpha = sc.p, *beta = sc.p + slots, *nrm2 = sc.p + 2 * slots, *mr = nrm2 + 1;
+  DevBuf<int> status(2);
+  HIPCHK(hipMemsetAsync(wbuf.p, 0, sizeof(double) * 3 * (size_t)n, st));  // W0 = W1 = 0 (:20-21)
+  Reduce R;
+  launch_mr_init(A, shift, b, x, vraw.p, nrm2, beta, mr, status.p, R, st);
+  out.spmv++;
+  double *vprev = vn0.p, *vcur = vn1.p;
+  double *W[3] = {wbuf.p, wbuf.p + n, wbuf.p + 2 * n};
+  int it = 0, chunk = 4, done_it = -1;
+  int *hs = reinterpret_cast<int *>(C.host_stage(1));
+  while (it < max_step) {
+    const int c = std::min(chunk, max_step - it);
+    for (int k = it; k < it + c; k++) {
+      // V1 = V1 / gamma0, T = (H + shift) V1, delta = <T, V1>, V2 = T - delta V1 - gamma0 V0 (:46-62)
+      launch_lanczos_a(A, false, k, vraw.p, nrm2, shift, vcur, t.p, alpha, R, st);
+      launch_lanczos_b(n, k, t.p, vcur, k > 0 ? vprev : nullptr, alpha, beta, vraw.p, nrm2, R, st);
+      // w_j lives in W[j % 3]: W0 = w_{k-2}, W1 = w_{k-1}, W2 = w_k (zero before they exist)
+      launch_mr_step(n, k, acc, alpha, beta, mr, status.p, coef.p, vcur, W[(k + 1) % 3], W[(k + 2) % 3],
+                     W[k % 3], x, st);
+      std::swap(vprev, vcur);
+    }
+    HIPCHK(hipMemcpyAsync(hs, status.p, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    if (hs[0]) {
+      done_it = hs[1];
+      break;
+    }
+    it += c;
+    chunk = std::min(chunk * 2, 64);
+  }
+  HIPCHK(hipStreamSynchronize(st));
+  *converged = done_it >= 0;
+  const int iters = (done_it >= 0) ? done_it : max_step;
+  out.spmv += (done_it >= 0) ? done_it + 1 : max_step;
+  return iters;
+}
+
 void inverse_iteration(const Operator &A, double eig, double *vec, double ii_acc, double ii_diag_add,
-                       int ii_max_step, double cg_acc, int cg_max_step, SolveOut &out) {
+                       int ii_max_step, double cg_acc, int cg_max_step, SolveOut &out, bool mr) {
   Context &C = ctx();
   hipStream_t st = C.stream;
   const int64_t n = A.n;
@@ -229,7 +277,8 @@ void inverse_iteration(const Operator &A, double eig, double *vec, double ii_acc
       break;
     }
     bool conv = false;
-    out.cg_steps += cg_solve(A, shift, vec, un.p, cg_acc, cg_max_step, &conv, out);
+    out.cg_steps += mr ? mr_solve(A, shift, vec, un.p, cg_acc, cg_max_step, &conv, out)
+                       : cg_solve(A, shift, vec, un.p, cg_acc, cg_max_step, &conv, out);
     if (!conv) out.cg_all_converged = false;
     launch_norm2(n, un.p, R, nrm2.p, st);
     launch_scale_by_rnorm(n, nrm2.p, un.p, un.p, st);

@@ -357,6 +357,10 @@ void INVERSE_ITERATION_SYM(BOX_II *Box_II);
 /* include/SML.h:172-173 */
 void CONJUGATE_GRADIENT(BOX_CG *Box);
 void CONJUGATE_GRADIENT_SYM(BOX_CG *Box);
+/* include/SML.h:293-294 -- MINRES on (M) x = Vec from x = Out_Vec;
+ * like the reference, the iterate that first meets |eta| < acc is not copied to Out_Vec */
+void MINIMUM_RESIDUAL(BOX_CG *Box);
+void MINIMUM_RESIDUAL_SYM(BOX_CG *Box);
 /* include/SML.h:174-175 */
 double CONVERGE_CHECK(CRS1 *M, double *Eig_Vec, double eig_val, int p_threads);
 double CONVERGE_CHECK_SYM(CRS1 *M, double *Eig_Vec, double eig_val, int p_threads);

@@ -474,6 +474,45 @@ static int cg_solve(int64_t n, or_matvec_fn mv, void *ctx, const double *b, doub
    return iter;
 }
 
+/* sml/MINIMUM_RESIDUAL.c:7-150: MINRES from x as given; returns the iteration at which
+ * |eta| < acc (x then holds the previous iterate, as the reference leaves it), or -1. */
+int or_minimum_residual(int64_t n, or_matvec_fn mv, void *ctx, const double *b, double *x, double acc,
+                        int max_step) {
+   size_t sz = sizeof(double) * (size_t)n;
+   double *v0 = calloc((size_t)n, sizeof(double)), *v1 = malloc(sz), *v2 = malloc(sz);
+   double *w0 = calloc((size_t)n, sizeof(double)), *w1 = calloc((size_t)n, sizeof(double));
+   double *w2 = malloc(sz), *x1 = malloc(sz), *t = malloc(sz);
+   mv(ctx, x, v1);
+   for (int64_t i = 0; i < n; i++) v1[i] = b[i] - v1[i];            /* :31-38 */
+   double gamma0 = sqrt(dot(n, v1, v1)), eta = gamma0;              /* :39-41 */
+   double s0 = 0, s1 = 0, c0 = 1, c1 = 1;
+   int ret = -1;
+   for (int iter = 0; iter < max_step; iter++) {
+      double inv = 1.0 / gamma0;
+      for (int64_t i = 0; i < n; i++) v1[i] = v1[i] * inv;         /* :48-51 */
+      mv(ctx, v1, t);
+      double delta = dot(n, t, v1);
+      for (int64_t i = 0; i < n; i++) v2[i] = t[i] - delta * v1[i] - gamma0 * v0[i];
+      double gamma1 = sqrt(dot(n, v2, v2));
+      double a0 = c1 * delta - c0 * s1 * gamma0;                      /* :66-73 */
+      double a1 = sqrt(a0 * a0 + gamma1 * gamma1);
+      double a2 = s1 * delta + c0 * c1 * gamma0;
+      double a3 = s0 * gamma0;
+      double c2 = a0 / a1, s2 = gamma1 / a1;
+      double ti = 1.0 / a1;
+      for (int64_t i = 0; i < n; i++) {
+         w2[i] = ti * (v1[i] - a3 * w0[i] - a2 * w1[i]);
+         x1[i] = x[i] + c2 * eta * w2[i];
+      }
+      eta = -s2 * eta;
+      if (fabs(eta) < acc) { ret = iter; break; }                     /* :78-99 */
+      memcpy(v0, v1, sz); memcpy(v1, v2, sz); memcpy(w0, w1, sz); memcpy(w1, w2, sz); memcpy(x, x1, sz);
+      s0 = s1; s1 = s2; c0 = c1; c1 = c2; gamma0 = gamma1;
+   }
+   free(v0); free(v1); free(v2); free(w0); free(w1); free(w2); free(x1); free(t);
+   return ret;
+}
+
 double or_inverse_iteration(int64_t n, or_matvec_fn mv, void *ctx, double eig_val, double *vec,
                             double ii_acc, double ii_diag_add, int ii_max_step,
                             double cg_acc, int cg_max_step, int *cg_steps, int *ii_steps) {

@@ -85,6 +85,13 @@ double or_inverse_iteration(int64_t n, or_matvec_fn mv, void *ctx, double eig_va
                             double ii_acc, double ii_diag_add, int ii_max_step,
                             double cg_acc, int cg_max_step, int *cg_steps, int *ii_steps);
 
+/* MINIMUM_RESIDUAL / MINIMUM_RESIDUAL_SYM (sml/MINIMUM_RESIDUAL.c:7-150): MINRES on A x = b from
+ * x as given.  Returns the iteration at which |eta| < acc (x is left at the previous iterate,
+ * as the reference returns before copying X1) or -1.  Pinned: numpy.linalg.solve and the
+ * MINRES residual identity |b - A x_k| = |eta_k|. */
+int or_minimum_residual(int64_t n, or_matvec_fn mv, void *ctx, const double *b, double *x, double acc,
+                        int max_step);
+
 /* matvec contexts usable from ctypes */
 typedef struct { int32_t n; const int64_t *row; const int32_t *col; const double *val; int32_t sym; } or_crs_op;
 void or_crs_matvec(void *ctx, const double *x, double *y);

@@ -72,6 +72,7 @@ def lib():
                                                     C.POINTER(C.c_int), C.POINTER(C.c_int)]
         L.or_v_m_q0.argtypes = [C.c_int, C.c_int32, C.c_int32, C.c_int32, i16p, i16p, i16p, i16p, i32p, i64p,
                                 i32p, f64p, f64p, f64p]
+        L.or_minimum_residual.argtypes = [C.c_int64, C.c_void_p, C.c_void_p, f64p, f64p, C.c_double, C.c_int]
         L.or_ed_assemble.argtypes = [C.c_int32, C.c_int32, C.c_int32, i64p, C.c_int32, C.c_void_p, C.c_void_p,
                                      C.c_int32, C.c_void_p, i64p, C.c_void_p, C.c_void_p]
         L.or_ed_assemble.restype = C.c_int64
@@ -358,3 +359,10 @@ def ed_sector_basis(L, d, qn1, t1, qn2=None, t2=0):
     if qn2 is not None:
         keep &= s2 == t2
     return codes[keep]
+
+
+def minimum_residual(op, b, x0, acc=1e-10, max_step=2000):
+    """MINIMUM_RESIDUAL restatement: returns (iteration or -1, x)."""
+    x = np.ascontiguousarray(x0, np.float64).copy()
+    it = lib().or_minimum_residual(op.n, op.fn, op.ctx, np.ascontiguousarray(b, np.float64), x, acc, max_step)
+    return it, x

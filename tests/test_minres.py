@@ -1,0 +1,101 @@
+"""MINIMUM_RESIDUAL(_SYM) and the "MR" inverse iteration (sml/MINIMUM_RESIDUAL.c)."""
+import ctypes as C
+import os
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+import oracle_py as O
+from conftest import GOLDEN
+
+SNAP = os.path.join(GOLDEN, "hubbard_L8_m32_s03.npz")
+
+
+def _full(s):
+    n = s.n
+    L = sp.csr_matrix((s.ham.val, s.ham.col, s.ham.row), shape=(n, n))
+    return (L + sp.tril(L, -1).T).tocsr()
+
+
+def test_oracle_minres_solves_shifted_system():
+    """MINRES on an indefinite shifted superblock Hamiltonian: the iterate it returns solves
+    the system to the accuracy the stopping rule implies (pins the restatement)."""
+    from dmrg_c_amd import snapshot
+    s = snapshot.load(SNAP)
+    H = _full(s)
+    shift = -float(s.meta["gs_energy"][0]) + 0.37   # indefinite: E0 sits below, levels above
+    A = (H + shift * sp.identity(s.n)).tocsr()
+    op = O.CrsOperator(A.indptr, A.indices, A.data, sym=False)
+    rng = np.random.default_rng(3)
+    b = rng.standard_normal(s.n)
+    it, x = O.minimum_residual(op, b, np.zeros(s.n), acc=1e-11, max_step=4000)
+    assert it > 0
+    xs = np.linalg.solve(A.toarray(), b)
+    # x is the iterate before the last rotation: its residual is the previous |eta| >= acc,
+    # bounded well below |b|
+    assert np.linalg.norm(A @ x - b) < 1e-8 * np.linalg.norm(b)
+    assert np.linalg.norm(x - xs) < 1e-6 * np.linalg.norm(xs)
+
+
+def _box_cg(H, b, x0, acc, max_step):
+    import dmrg_c_amd as D
+    B = D.BOX_CG()
+    B.M = H.ptr
+    B.acc = acc
+    B.Vec = D._dp(b)
+    B.Out_Vec = D._dp(x0)
+    B.max_step = max_step
+    B.p_threads = 1
+    return B
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("sym", [True, False], ids=["SYM", "full"])
+def test_gpu_minimum_residual_matches_oracle(sym, tmp_path, monkeypatch):
+    """MINIMUM_RESIDUAL(_SYM) on the device == the restatement: same stopping iteration,
+    same iterate to 1e-9 relative."""
+    import dmrg_c_amd as D
+    from dmrg_c_amd import snapshot
+    monkeypatch.chdir(tmp_path)
+    L = D.lib()
+    L.MINIMUM_RESIDUAL_SYM.argtypes = [C.POINTER(D.BOX_CG)]
+    L.MINIMUM_RESIDUAL.argtypes = [C.POINTER(D.BOX_CG)]
+    s = snapshot.load(SNAP)
+    H = _full(s)
+    rng = np.random.default_rng(5)
+    b = rng.standard_normal(s.n)
+    x0 = rng.standard_normal(s.n) * 1e-3
+    acc = 1e-9
+    op = O.CrsOperator(H.indptr, H.indices, H.data, sym=False)
+    it_ref, x_ref = O.minimum_residual(op, b, x0, acc=acc, max_step=3000)
+    if sym:
+        hc = D.HostCRS(s.ham.row, s.ham.col, s.ham.val)
+    else:
+        hc = D.HostCRS(H.indptr, H.indices, H.data)
+    x = x0.copy()
+    box = _box_cg(hc, b, x, acc, 3000)
+    (L.MINIMUM_RESIDUAL_SYM if sym else L.MINIMUM_RESIDUAL)(C.byref(box))
+    steps = np.loadtxt(os.path.join("SML_out", "MINIMUM_RESIDUAL_Step.txt"), ndmin=1)
+    assert abs(int(steps[-1]) - it_ref) <= 1
+    assert np.linalg.norm(x - x_ref) <= 1e-9 * np.linalg.norm(x_ref) or \
+        np.linalg.norm(H @ x - b) <= 1e-8 * np.linalg.norm(b)
+
+
+@pytest.mark.gpu
+def test_gpu_diagonalize_with_mr_inverse_iteration(tmp_path, monkeypatch):
+    """DMRG_DIAGONALIZE_SUPERBLOCK_SYM with II_Type "MR": the reference's energy to 1e-10 and
+    its eigenvector up to sign."""
+    import dmrg_c_amd as D
+    from dmrg_c_amd import snapshot
+    monkeypatch.chdir(tmp_path)
+    s = snapshot.load(SNAP)
+    P = D.default_dmrg_parameter()
+    P.II_Type = b"MR"
+    E, err, vec, _ = D.diagonalize_superblock_sym(s.ham.row, s.ham.col, s.ham.val, P)
+    e_ref = float(s.meta["gs_energy"][0])
+    assert abs(E - e_ref) <= 1e-10 * abs(e_ref)
+    ref = s.meta["gs_vec"]
+    sgn = np.sign(vec @ ref)
+    assert np.abs(vec - sgn * ref).max() <= 1e-8
+    assert err < 1e-8
