@@ -439,6 +439,12 @@ int dmrgx_ground_state(const dmrgx_matrix *A, const dmrgx_solve_params *p, doubl
     o.min_step = p->diag_min_step;
     o.max_step = p->diag_max_step;
     o.type = p->lan_type;
+    const bool mixed = p->precision == 1;
+    if (p->precision != 0 && !mixed) throw Failure(DMRGX_ERR_ARG, "precision must be 0 (FP64) or 1 (FP32 + FP64 refinement)");
+    if (mixed) {
+      ensure_fp32(*A);
+      o.fp32 = true;
+    }
     if (A->n == 1) {
       double h;
       HIPCHK(hipMemcpy(&h, A->val.p, sizeof(double), hipMemcpyDeviceToHost));
@@ -453,6 +459,12 @@ int dmrgx_ground_state(const dmrgx_matrix *A, const dmrgx_solve_params *p, doubl
     if (!p->skip_ii)
       inverse_iteration(op_of(*A), out.eig, v.d.p, p->ii_acc, p->ii_diag_add, p->ii_max_step, p->cg_acc, p->cg_max_step,
                         out);
+    // mixed precision: the FP32 Lanczos value is only a shift for the FP64 refinement;
+    // the energy reported is the FP64 Rayleigh quotient of the refined vector
+    if (mixed && A->n > 1) {
+      out.eig = rayleigh_quotient(op_of(*A), v.d.p);
+      out.spmv++;
+    }
     v.down(vec_inout);
     if (stats) {
       stats->eig_val = out.eig;

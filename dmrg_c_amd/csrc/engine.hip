@@ -1,0 +1,866 @@
+// Batched ground-state engine: many independent solves advance together, one launch per
+// kernel per step for all of them.
+//
+// Each request (a device matrix + solve parameters) occupies a slot and goes through the
+// phases of dmrgx_ground_state -- Lanczos (sml/LANCZOS_SLOW_SYM.c:57-274 with the basis
+// kept in HBM), then inverse iteration with CG (sml/INVERSE_ITERATION_SYM.c:9-116,
+// sml/CONJUGATE_GRADIENT_SYM.c:8-123) -- but the per-step kernels are segmented: a launch
+// covers the rows of every slot in that phase, a block belongs to exactly one slot, and
+// each slot's block partials are summed in block order by a finish kernel (one block per
+// slot).  A slot gets
+// the same blocks, the same lanes per row and the same summation order as the single-solve
+// kernels, so its result is bit-identical to dmrgx_ground_state on the same matrix, alone
+// or next to any other requests.
+//
+// Per tick (one Lanczos step of every Lanczos slot, one CG iteration of every CG slot):
+//   k_seg_spmv<G, LANCZOS_A>  v_k = w/|w|, y = H v_k, alpha_k       (one launch per lanes bin)
+//   k_seg_lanczos_b           w = y - alpha v_k - beta v_{k-1}, |w|^2, beta_{k+1}
+//   k_seg_spmv<G, CG_AP>      y = (H + shift) p, p.y
+//   k_seg_cg_update           x += a p, r -= a y, |r|^2, done flag
+//   k_seg_cg_dir              p = r + b p
+// The per-tick descriptors of all slots are built on the host and copied once per chunk of
+// ticks; the host reads the recurrence scalars once per chunk, runs the tridiagonal checks,
+// and moves slots between phases.  New requests are admitted at chunk boundaries while the
+// HBM budget allows (continuous batching).
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <deque>
+#include <limits>
+#include <map>
+
+#include "entry.h"
+
+namespace dmrgx {
+namespace {
+
+enum Mode { LANCZOS_A = 0, CG_AP = 1 };
+
+// One slot's view for one launch of a segmented kernel.
+struct SegDesc {
+  const int64_t *row;
+  const int32_t *col;
+  const double *val;
+  const float *val32;
+  int32_t n, fp32;
+  int32_t blk_lo, nblk;   // this slot's blocks in the launch
+  int32_t k, pad;         // step (Lanczos) or iteration (CG)
+  double shift, acc;
+  const double *x;        // LANCZOS_A: w      CG_AP: p
+  double *o1, *o2;        // LANCZOS_A: v_k, y CG_AP: y
+  const double *o3;       // lanczos_b: v_{k-1} (null at k = 0); cg_update/dir: p
+  double *s1, *s2, *s3;   // LANCZOS_A: nrm2, alpha       lanczos_b: alpha, beta, nrm2
+                          // CG_AP: pap                   cg_update: pap, rr      cg_dir: rr
+  double *v1, *v2;        // cg_update: x, r
+  int *status;            // CG: {done, last iteration}
+};
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  return v;
+}
+
+__device__ __forceinline__ double block_sum(double v) {
+  __shared__ double lds[kBlock / 64];
+  v = wave_sum(v);
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) lds[w] = v;
+  __syncthreads();
+  double t = 0.0;
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int i = 0; i < kBlock / 64; i++) t += lds[i];
+  }
+  return t;
+}
+
+__device__ __forceinline__ void block_partial(double v, double *partial) {
+  const double t = block_sum(v);
+  if (threadIdx.x == 0) partial[blockIdx.x] = t;
+}
+
+enum FinKind { FIN_LANCZOS_A = 0, FIN_CG_AP = 1, FIN_LANCZOS_B = 2, FIN_CG_UPDATE = 3 };
+
+// One block per segment: its partials summed in block order (the single-solve k_finish
+// over the segment's blocks), then the kernel's scalar update.
+template <int KIND>
+__global__ __launch_bounds__(kBlock) void k_seg_finish(const SegDesc *desc, const double *partial) {
+  const SegDesc d = desc[blockIdx.x];
+  if ((KIND == FIN_CG_AP || KIND == FIN_CG_UPDATE) && d.status[0] != 0) return;
+  double s = 0.0;
+  for (int i = threadIdx.x; i < d.nblk; i += kBlock) s += partial[d.blk_lo + i];
+  const double t = block_sum(s);
+  if (threadIdx.x != 0) return;
+  if (KIND == FIN_LANCZOS_A) {
+    *d.s2 = t;                 // alpha_k
+  } else if (KIND == FIN_CG_AP) {
+    *d.s1 = t;                 // pAp
+  } else if (KIND == FIN_LANCZOS_B) {
+    *d.s3 = t;                 // |w|^2
+    d.s2[1] = sqrt(t);         // beta_{k+1}
+  } else {
+    d.s2[d.k] = t;             // rr_it
+    d.status[1] = d.k;
+    if (t < d.acc) d.status[0] = 1;
+  }
+}
+
+template <int G, int MODE>
+__global__ __launch_bounds__(kBlock) void k_seg_spmv(const SegDesc *desc, const int32_t *blk_seg, double *partial) {
+  const int seg = blk_seg[blockIdx.x];
+  const SegDesc d = desc[seg];
+  if (MODE == CG_AP && d.status[0] != 0) return;
+  constexpr int RPB = kBlock / G;
+  const int g = threadIdx.x / G;
+  const int lane = threadIdx.x % G;
+  const int32_t r = (blockIdx.x - d.blk_lo) * RPB + g;
+  double s = 0.0;
+  if (r < d.n) {
+    if (d.fp32) s = csr_lane_sum<G>(d.row, d.col, d.val32, r, lane, d.x);
+    else s = csr_lane_sum<G>(d.row, d.col, d.val, r, lane, d.x);
+  }
+#pragma unroll
+  for (int off = G / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, G);
+  double c = 0.0;
+  if (lane == 0 && r < d.n) {
+    if (MODE == LANCZOS_A) {
+      const double scale = 1.0 / sqrt(*d.s1);
+      const double xs = scale * d.x[r];
+      d.o1[r] = xs;
+      const double yv = scale * s + d.shift * xs;
+      d.o2[r] = yv;
+      c = yv * xs;
+    } else {
+      const double yv = s + d.shift * d.x[r];
+      d.o1[r] = yv;
+      c = d.x[r] * yv;
+    }
+  }
+  block_partial(c, partial);
+}
+
+// Vector kernels: a slot gets vec_grid(n) blocks and the single-solve grid-stride pattern.
+__global__ __launch_bounds__(kBlock) void k_seg_lanczos_b(const SegDesc *desc, const int32_t *blk_seg,
+                                                          double *partial) {
+  const int seg = blk_seg[blockIdx.x];
+  const SegDesc d = desc[seg];
+  const int lb = blockIdx.x - d.blk_lo;
+  const double a = *d.s1;
+  const double bb = (d.k > 0) ? *d.s2 : 0.0;
+  const double *vkm1 = d.o3 ? d.o3 : d.o1;
+  double s = 0.0;
+  for (int64_t i = lb * (int64_t)kBlock + threadIdx.x; i < d.n; i += (int64_t)d.nblk * kBlock) {
+    double v = d.o2[i] - a * d.o1[i];
+    if (d.k > 0) v -= bb * vkm1[i];
+    const_cast<double *>(d.x)[i] = v;
+    s += v * v;
+  }
+  block_partial(s, partial);
+}
+
+__global__ __launch_bounds__(kBlock) void k_seg_cg_update(const SegDesc *desc, const int32_t *blk_seg,
+                                                          double *partial) {
+  const int seg = blk_seg[blockIdx.x];
+  const SegDesc d = desc[seg];
+  if (d.status[0] != 0) return;
+  const int lb = blockIdx.x - d.blk_lo;
+  const int it = d.k;
+  const double alpha = d.s2[it - 1] / d.s1[it];
+  double s = 0.0;
+  for (int64_t i = lb * (int64_t)kBlock + threadIdx.x; i < d.n; i += (int64_t)d.nblk * kBlock) {
+    d.v1[i] = d.v1[i] + alpha * d.o3[i];
+    const double rv = d.v2[i] - alpha * d.o1[i];
+    d.v2[i] = rv;
+    s += rv * rv;
+  }
+  block_partial(s, partial);
+}
+
+__global__ __launch_bounds__(kBlock) void k_seg_cg_dir(const SegDesc *desc, const int32_t *blk_seg) {
+  const int seg = blk_seg[blockIdx.x];
+  const SegDesc d = desc[seg];
+  if (d.status[0] != 0) return;
+  const int lb = blockIdx.x - d.blk_lo;
+  const int it = d.k;
+  const double beta = d.s2[it] / d.s2[it - 1];
+  for (int64_t i = lb * (int64_t)kBlock + threadIdx.x; i < d.n; i += (int64_t)d.nblk * kBlock)
+    d.v1[i] = d.v2[i] + beta * d.v1[i];
+}
+
+int vec_blocks(int64_t n) {
+  int64_t b = (n + kBlock - 1) / kBlock;
+  return (int)(b < kVecGrid ? (b > 0 ? b : 1) : kVecGrid);
+}
+
+double now_sec() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+// Lanczos basis of one slot, chunks of 32 vectors (as solver.hip).
+struct SlotBasis {
+  int64_t n = 0;
+  std::vector<DevBuf<double>> chunks;
+  double *vec(int k) {
+    const size_t c = (size_t)k / 32;
+    while (chunks.size() <= c) chunks.emplace_back((size_t)n * 32);
+    return chunks[c].p + (size_t)(k % 32) * n;
+  }
+};
+
+}  // namespace
+
+enum Phase { P_LANCZOS, P_CG, P_DONE };
+
+struct Slot {
+  int64_t id = 0;
+  uint64_t tag = 0;
+  const dmrgx_matrix *A = nullptr;
+  dmrgx_matrix *owned = nullptr;
+  dmrgx_solve_params p{};
+  bool mixed = false;
+  Phase phase = P_LANCZOS;
+  int status = DMRGX_OK;
+  int32_t n = 0;
+  int lanes = 8;
+  size_t bytes = 0;
+  // device vectors
+  DevBuf<double> w, y, vec, un, r, pv, cy, xw;
+  SlotBasis V;
+  DevBuf<double> scal;   // alpha[KS] beta[KS] nrm2 | rr[CS] pap[CS] | residual
+  DevBuf<int> cgst;      // CG {done, last iteration}
+  int KS = 0, CS = 0;
+  // host Lanczos state (as solver.hip:lanczos)
+  int k_next = 0;        // next Lanczos step to compute
+  int k_read = 0;        // steps whose scalars the host has checked
+  double E = 1e15, lam = 0.0;
+  std::vector<double> ha, hb, s;
+  // II / CG state
+  double eig = 0.0, err = 1000.0;
+  int ii_iter = 0, cg_it = 0;
+  SolveOut out;
+  double t_submit = 0, t_start = 0, t_done = 0;
+  double *alpha() { return scal.p; }
+  double *beta() { return scal.p + KS; }
+  double *nrm2() { return scal.p + 2 * KS; }
+  double *rr() { return scal.p + 2 * KS + 1; }
+  double *pap() { return scal.p + 2 * KS + 1 + CS; }
+  double *resid() { return scal.p + 2 * KS + 1 + 2 * CS; }
+};
+
+}  // namespace dmrgx
+
+struct dmrgx_engine {
+  dmrgx_engine_opts opts{};
+  int64_t next_id = 1;
+  size_t bytes_in_use = 0;
+  std::deque<std::unique_ptr<dmrgx::Slot>> pending;
+  std::vector<std::unique_ptr<dmrgx::Slot>> active;
+  std::deque<std::unique_ptr<dmrgx::Slot>> done;
+  // launch scratch
+  dmrgx::DevBuf<dmrgx::SegDesc> ddesc;
+  dmrgx::DevBuf<int32_t> dblk;
+  dmrgx::DevBuf<double> partial;
+};
+
+namespace dmrgx {
+namespace {
+
+size_t slot_bytes(const dmrgx_engine &E, const Slot &s) {
+  // eight work vectors, the scalars and an estimate of the Lanczos basis
+  const size_t n = (size_t)s.n;
+  const size_t steps = (size_t)std::min(s.p.diag_max_step + 1, std::max(E.opts.basis_steps_estimate, 1));
+  size_t b = n * 8 * (8 + steps) + (size_t)(2 * s.KS + 2 * s.CS + 2) * 8;
+  if (s.owned) b += (size_t)s.owned->nnz * 12 + (size_t)(s.n + 1) * 8;
+  return b;
+}
+
+// Start a slot: start vector and |w|^2 (solver.hip:lanczos before its first step).
+void start_slot(Slot &s) {
+  hipStream_t st = ctx().stream;
+  const int64_t n = s.n;
+  s.w.resize(n);
+  s.y.resize(n);
+  s.vec.resize(n);
+  s.V.n = n;
+  s.KS = s.p.diag_max_step + 2;
+  s.CS = std::max(s.p.cg_max_step, 1) + 1;
+  s.scal.resize(2 * (size_t)s.KS + 2 * (size_t)s.CS + 2);
+  s.cgst.resize(2);
+  s.ha.assign(s.KS, 0.0);
+  s.hb.assign(s.KS, 0.0);
+  s.s.assign(s.KS, 0.0);
+  if (s.mixed) ensure_fp32(*s.A);
+  launch_start_vector(n, s.p.seed, s.vec.p, st);
+  launch_copy(n, s.vec.p, s.w.p, st);
+  Reduce R;
+  launch_norm2(n, s.w.p, R, s.nrm2(), st);
+  HIPCHK(hipGetLastError());
+  s.t_start = now_sec();
+}
+
+void finish_slot(Slot &s, int status) {
+  s.phase = P_DONE;
+  s.status = status;
+  s.t_done = now_sec();
+  // release the work vectors; vec (the result) stays until polled
+  s.w.reset();
+  s.y.reset();
+  s.un.reset();
+  s.r.reset();
+  s.pv.reset();
+  s.cy.reset();
+  s.xw.reset();
+  s.V.chunks.clear();
+  s.scal.reset();
+  s.cgst.reset();
+}
+
+// CG start for the current II iteration (solver.hip:cg_solve before its loop)
+void start_cg(Slot &s) {
+  hipStream_t st = ctx().stream;
+  const int64_t n = s.n;
+  Reduce R;
+  const double shift = s.p.ii_diag_add - s.eig;
+  HIPCHK(hipMemsetAsync(s.cgst.p, 0, 2 * sizeof(int), st));
+  launch_norm2(n, s.un.p, R, s.nrm2(), st);
+  launch_cg_init(op_of(*s.A), shift, s.nrm2(), s.un.p, s.xw.p, s.vec.p, s.r.p, s.pv.p, s.rr(), R, st);
+  s.out.spmv++;
+  s.cg_it = 1;
+  s.phase = P_CG;
+}
+
+// Inverse-iteration step after Lanczos or after a CG solve: residual check, then either
+// done or the next CG solve (solver.hip:inverse_iteration).  Synchronises.
+void ii_advance(Slot &s) {
+  hipStream_t st = ctx().stream;
+  const int64_t n = s.n;
+  Reduce R;
+  while (!s.p.skip_ii) {
+    if (s.ii_iter > s.p.ii_max_step) break;
+    launch_residual_l1(op_of(*s.A), s.vec.p, s.eig, s.resid(), R, st);
+    double *h = ctx().host_stage(1);
+    HIPCHK(hipMemcpyAsync(h, s.resid(), sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    s.err = h[0];
+    s.out.spmv++;
+    if (s.err < s.p.ii_acc) {
+      s.out.ii_converged = true;
+      break;
+    }
+    start_cg(s);
+    return;
+  }
+  if (!s.p.skip_ii) {
+    s.out.ii_steps = s.ii_iter;
+    s.out.err = s.err;
+  }
+  if (s.mixed && s.n > 1) {
+    s.eig = rayleigh_quotient(op_of(*s.A), s.vec.p);
+    s.out.spmv++;
+  }
+  s.out.eig = s.eig;
+  HIPCHK(hipStreamSynchronize(st));
+  finish_slot(s, DMRGX_OK);
+  (void)n;
+}
+
+// A CG solve ended (converged or out of steps): x -> un, normalise, vec = un, next II round.
+void cg_finished(Slot &s, int done_it) {
+  hipStream_t st = ctx().stream;
+  const int64_t n = s.n;
+  Reduce R;
+  const bool conv = done_it >= 0;
+  if (!conv) s.out.cg_all_converged = false;
+  s.out.cg_steps += conv ? done_it : s.p.cg_max_step;
+  s.out.spmv += conv ? done_it : std::max(s.p.cg_max_step - 1, 0);
+  launch_copy(n, s.xw.p, s.un.p, st);
+  launch_norm2(n, s.un.p, R, s.nrm2(), st);
+  launch_scale_by_rnorm(n, s.nrm2(), s.un.p, s.un.p, st);
+  launch_copy(n, s.un.p, s.vec.p, st);
+  s.ii_iter++;
+  ii_advance(s);
+}
+
+// The Lanczos run of this slot converged at step kconv: Ritz vector, then inverse iteration.
+void lanczos_finished(Slot &s, int kconv) {
+  hipStream_t st = ctx().stream;
+  const int64_t n = s.n;
+  tridiag_vector(kconv + 1, s.ha.data(), s.hb.data() + 1, s.lam, s.s.data());
+  std::vector<const double *> ptrs(kconv + 1);
+  for (int j = 0; j <= kconv; j++) ptrs[j] = s.V.vec(j);
+  DevBuf<const double *> dptr(kconv + 1);
+  DevBuf<double> dcoef(kconv + 1);
+  HIPCHK(hipMemcpyAsync(dptr.p, ptrs.data(), sizeof(double *) * (kconv + 1), hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(dcoef.p, s.s.data(), sizeof(double) * (kconv + 1), hipMemcpyHostToDevice, st));
+  launch_ritz(n, kconv, dptr.p, dcoef.p, s.vec.p, st);
+  Reduce R;
+  launch_norm2(n, s.vec.p, R, s.nrm2(), st);
+  launch_scale_by_rnorm(n, s.nrm2(), s.vec.p, s.vec.p, st);
+  HIPCHK(hipStreamSynchronize(st));
+  s.V.chunks.clear();
+  s.eig = s.lam;
+  s.out.lan_steps = kconv;   // steps after the first, as solver.hip:lanczos counts them
+  s.out.lan_converged = true;
+  if (s.p.skip_ii) {
+    ii_advance(s);
+    return;
+  }
+  s.un.resize(n);
+  s.r.resize(n);
+  s.pv.resize(n);
+  s.cy.resize(n);
+  s.xw.resize(n);
+  launch_copy(n, s.vec.p, s.un.p, st);
+  s.ii_iter = 0;
+  ii_advance(s);
+}
+
+// Host checks of the steps computed since the last chunk (solver.hip:lanczos, no restarts).
+void lanczos_check(Slot &s, const double *alpha_h, const double *beta_h) {
+  const bool small = s.n < 1000;
+  for (int k = s.k_read; k < s.k_next; k++) {
+    s.ha[k] = alpha_h[k];
+    s.hb[k + 1] = beta_h[k + 1];
+  }
+  for (int k = s.k_read; k < s.k_next; k++) {
+    s.k_read = k + 1;
+    s.out.spmv++;
+    if (k == 0) continue;  // step 0 is the start of the recurrence (solver.hip: block step 0)
+    const bool check = small || k >= s.p.diag_min_step;
+    if (!check) continue;
+    s.lam = tridiag_lowest_value(k + 1, s.ha.data(), s.hb.data() + 1, std::numeric_limits<double>::infinity());
+    bool conv;
+    const double bnext = s.hb[k + 1];
+    const double tscale = std::max(1.0, std::fabs(s.lam));
+    if (small) {
+      tridiag_vector(k + 1, s.ha.data(), s.hb.data() + 1, s.lam, s.s.data());
+      conv = !(bnext > 1e-12 * tscale) || bnext * std::fabs(s.s[k]) <= 1e-12 * tscale;
+    } else if (!(bnext > 1e-14 * tscale)) {
+      conv = true;
+    } else if (s.p.lan_type == 1) {
+      tridiag_vector(k + 1, s.ha.data(), s.hb.data() + 1, s.lam, s.s.data());
+      conv = std::fabs(s.s[k]) < s.p.diag_acc;
+    } else {
+      conv = std::fabs(s.E - s.lam) < s.p.diag_acc;
+    }
+    if (conv) {
+      lanczos_finished(s, k);
+      return;
+    }
+    s.E = s.lam;
+  }
+  if (s.k_next >= s.p.diag_max_step) {
+    s.out.lan_steps = s.k_next - 1;
+    s.out.eig = s.lam;
+    finish_slot(s, DMRGX_ERR_NOCONV);
+  }
+}
+
+// Lay out one segmented launch: blocks of each slot consecutively, desc index = slot order.
+struct LaunchPlan {
+  std::vector<SegDesc> desc;
+  std::vector<int32_t> blk;
+};
+
+void ensure_scratch(dmrgx_engine &E, size_t n_desc, size_t n_blk) {
+  if (E.ddesc.count < n_desc) E.ddesc.resize(std::max<size_t>(n_desc, 2 * E.ddesc.count));
+  if (E.dblk.count < n_blk) E.dblk.resize(std::max<size_t>(n_blk, 2 * E.dblk.count));
+  if (E.partial.count < n_blk) E.partial.resize(std::max<size_t>(n_blk, 2 * E.partial.count));
+}
+
+template <int MODE>
+void launch_seg_spmv(int G, int nblk, const SegDesc *desc, const int32_t *blk, double *partial, hipStream_t st) {
+  switch (G) {
+#define G_CASE(g) \
+  case g: k_seg_spmv<g, MODE><<<nblk, kBlock, 0, st>>>(desc, blk, partial); break;
+    G_CASE(1) G_CASE(2) G_CASE(4) G_CASE(8) G_CASE(16) G_CASE(32) G_CASE(64)
+#undef G_CASE
+    default: throw Failure(DMRGX_ERR_ARG, "bad lanes per row");
+  }
+}
+
+// One chunk of ticks over all active slots.
+void run_chunk(dmrgx_engine &E, int ticks) {
+  hipStream_t st = ctx().stream;
+  std::vector<Slot *> L, Cg;
+  for (auto &sp : E.active) {
+    if (sp->phase == P_LANCZOS) L.push_back(sp.get());
+    else if (sp->phase == P_CG) Cg.push_back(sp.get());
+  }
+  if (L.empty() && Cg.empty()) return;
+  // lanes bins (each bin one SpMV launch per tick)
+  std::map<int, std::vector<int>> binL, binC;
+  for (size_t i = 0; i < L.size(); i++) binL[L[i]->lanes].push_back((int)i);
+  for (size_t i = 0; i < Cg.size(); i++) binC[Cg[i]->lanes].push_back((int)i);
+  // Build every launch of the chunk: descriptors + block tables, one upload.
+  struct Launch {
+    int kind;  // 0 spmv-A, 1 lanczos_b, 2 spmv-cg, 3 cg_update, 4 cg_dir
+    int G, nblk, nseg;
+    size_t desc_off, blk_off;
+  };
+  std::vector<Launch> launches;
+  std::vector<SegDesc> hdesc;
+  std::vector<int32_t> hblk;
+  auto add_launch = [&](int kind, int G, const std::vector<SegDesc> &segs, const std::vector<int> &nb) {
+    Launch l{kind, G, 0, (int)segs.size(), hdesc.size(), hblk.size()};
+    int lo = 0;
+    for (size_t i = 0; i < segs.size(); i++) {
+      SegDesc d = segs[i];
+      d.blk_lo = lo;
+      d.nblk = nb[i];
+      hdesc.push_back(d);
+      for (int b = 0; b < nb[i]; b++) hblk.push_back((int32_t)i);
+      lo += nb[i];
+    }
+    l.nblk = lo;
+    if (lo > 0) launches.push_back(l);
+  };
+  std::vector<int> lan_k(L.size()), cg_it(Cg.size());
+  for (size_t i = 0; i < L.size(); i++) lan_k[i] = L[i]->k_next;
+  for (size_t i = 0; i < Cg.size(); i++) cg_it[i] = Cg[i]->cg_it;
+  for (int t = 0; t < ticks; t++) {
+    bool any = false;
+    // Lanczos: A then B per bin
+    for (auto &kv : binL) {
+      std::vector<SegDesc> segA, segB;
+      std::vector<int> nbA, nbB;
+      for (int i : kv.second) {
+        Slot &s = *L[i];
+        const int k = lan_k[i];
+        if (k >= s.p.diag_max_step) continue;
+        SegDesc a{};
+        a.row = s.A->row.p;
+        a.col = s.A->col.p;
+        a.val = s.A->val.p;
+        a.val32 = s.A->val32.p;
+        a.n = s.n;
+        a.fp32 = s.mixed ? 1 : 0;
+        a.k = k;
+        a.shift = 0.0;
+        a.x = s.w.p;
+        a.o1 = s.V.vec(k);
+        a.o2 = s.y.p;
+        a.s1 = s.nrm2();
+        a.s2 = s.alpha() + k;
+        segA.push_back(a);
+        nbA.push_back(spmv_blocks(s.n, kv.first));
+        SegDesc b{};
+        b.n = s.n;
+        b.k = k;
+        b.x = s.w.p;
+        b.o1 = s.V.vec(k);
+        b.o2 = s.y.p;
+        b.o3 = k > 0 ? s.V.vec(k - 1) : nullptr;
+        b.s1 = s.alpha() + k;
+        b.s2 = s.beta() + k;
+        b.s3 = s.nrm2();
+        segB.push_back(b);
+        nbB.push_back(vec_blocks(s.n));
+        lan_k[i] = k + 1;
+        any = true;
+      }
+      add_launch(0, kv.first, segA, nbA);
+      add_launch(1, 0, segB, nbB);
+    }
+    for (auto &kv : binC) {
+      std::vector<SegDesc> sp, su, sd;
+      std::vector<int> nbp, nbv;
+      for (int i : kv.second) {
+        Slot &s = *Cg[i];
+        const int it = cg_it[i];
+        if (it >= s.p.cg_max_step) continue;
+        SegDesc a{};
+        a.row = s.A->row.p;
+        a.col = s.A->col.p;
+        a.val = s.A->val.p;
+        a.n = s.n;
+        a.k = it;
+        a.shift = s.p.ii_diag_add - s.eig;
+        a.x = s.pv.p;
+        a.o1 = s.cy.p;
+        a.s1 = s.pap() + it;
+        a.status = s.cgst.p;
+        sp.push_back(a);
+        nbp.push_back(spmv_blocks(s.n, kv.first));
+        SegDesc u{};
+        u.n = s.n;
+        u.k = it;
+        u.acc = s.p.cg_acc;
+        u.o1 = s.cy.p;
+        u.o3 = s.pv.p;
+        u.s1 = s.pap();
+        u.s2 = s.rr();
+        u.v1 = s.xw.p;
+        u.v2 = s.r.p;
+        u.status = s.cgst.p;
+        su.push_back(u);
+        SegDesc dd{};
+        dd.n = s.n;
+        dd.k = it;
+        dd.s2 = s.rr();
+        dd.v1 = s.pv.p;
+        dd.v2 = s.r.p;
+        dd.status = s.cgst.p;
+        sd.push_back(dd);
+        nbv.push_back(vec_blocks(s.n));
+        cg_it[i] = it + 1;
+        any = true;
+      }
+      add_launch(2, kv.first, sp, nbp);
+      add_launch(3, 0, su, nbv);
+      add_launch(4, 0, sd, nbv);
+    }
+    if (!any) break;
+  }
+  if (launches.empty()) return;
+  ensure_scratch(E, hdesc.size(), hblk.size());
+  HIPCHK(hipMemcpyAsync(E.ddesc.p, hdesc.data(), sizeof(SegDesc) * hdesc.size(), hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(E.dblk.p, hblk.data(), sizeof(int32_t) * hblk.size(), hipMemcpyHostToDevice, st));
+  for (const Launch &l : launches) {
+    const SegDesc *dd = E.ddesc.p + l.desc_off;
+    const int32_t *bb = E.dblk.p + l.blk_off;
+    switch (l.kind) {
+      case 0:
+        launch_seg_spmv<LANCZOS_A>(l.G, l.nblk, dd, bb, E.partial.p, st);
+        k_seg_finish<FIN_LANCZOS_A><<<l.nseg, kBlock, 0, st>>>(dd, E.partial.p);
+        break;
+      case 1:
+        k_seg_lanczos_b<<<l.nblk, kBlock, 0, st>>>(dd, bb, E.partial.p);
+        k_seg_finish<FIN_LANCZOS_B><<<l.nseg, kBlock, 0, st>>>(dd, E.partial.p);
+        break;
+      case 2:
+        launch_seg_spmv<CG_AP>(l.G, l.nblk, dd, bb, E.partial.p, st);
+        k_seg_finish<FIN_CG_AP><<<l.nseg, kBlock, 0, st>>>(dd, E.partial.p);
+        break;
+      case 3:
+        k_seg_cg_update<<<l.nblk, kBlock, 0, st>>>(dd, bb, E.partial.p);
+        k_seg_finish<FIN_CG_UPDATE><<<l.nseg, kBlock, 0, st>>>(dd, E.partial.p);
+        break;
+      case 4: k_seg_cg_dir<<<l.nblk, kBlock, 0, st>>>(dd, bb); break;
+    }
+  }
+  HIPCHK(hipGetLastError());
+  for (size_t i = 0; i < L.size(); i++) L[i]->k_next = lan_k[i];
+  for (size_t i = 0; i < Cg.size(); i++) Cg[i]->cg_it = cg_it[i];
+  HIPCHK(hipStreamSynchronize(st));  // host descriptor vectors go out of scope; scalars read next
+  // host side: Lanczos scalars and CG status
+  for (Slot *s : L) {
+    double *h = ctx().host_stage(2 * (size_t)s->KS);
+    HIPCHK(hipMemcpyAsync(h, s->alpha(), sizeof(double) * 2 * s->KS, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    std::vector<double> a(h, h + s->KS), b(h + s->KS, h + 2 * s->KS);
+    lanczos_check(*s, a.data(), b.data());
+  }
+  for (Slot *s : Cg) {
+    int *hs = reinterpret_cast<int *>(ctx().host_stage(1));
+    HIPCHK(hipMemcpyAsync(hs, s->cgst.p, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    if (hs[0]) cg_finished(*s, hs[1]);
+    else if (s->cg_it >= s->p.cg_max_step) cg_finished(*s, -1);
+  }
+}
+
+void admit(dmrgx_engine &E) {
+  while (!E.pending.empty()) {
+    Slot &s = *E.pending.front();
+    if ((int)E.active.size() >= E.opts.max_active) break;
+    if (!E.active.empty() && E.bytes_in_use + s.bytes > (size_t)E.opts.hbm_budget_bytes) break;
+    E.bytes_in_use += s.bytes;
+    auto sp = std::move(E.pending.front());
+    E.pending.pop_front();
+    if (sp->n == 1) {
+      // 1x1: the eigenpair is the element itself (as dmrgx_ground_state)
+      sp->vec.resize(1);
+      double h;
+      HIPCHK(hipMemcpy(&h, sp->A->val.p, sizeof(double), hipMemcpyDeviceToHost));
+      const double one = 1.0;
+      HIPCHK(hipMemcpy(sp->vec.p, &one, sizeof(double), hipMemcpyHostToDevice));
+      sp->eig = sp->out.eig = h;
+      sp->out.lan_converged = true;
+      sp->t_start = now_sec();
+      finish_slot(*sp, DMRGX_OK);
+    } else {
+      start_slot(*sp);
+    }
+    E.active.push_back(std::move(sp));
+  }
+}
+
+void retire(dmrgx_engine &E) {
+  for (size_t i = 0; i < E.active.size();) {
+    if (E.active[i]->phase == P_DONE) {
+      E.bytes_in_use -= std::min(E.bytes_in_use, E.active[i]->bytes);
+      E.done.push_back(std::move(E.active[i]));
+      E.active.erase(E.active.begin() + i);
+    } else {
+      i++;
+    }
+  }
+}
+
+}  // namespace
+}  // namespace dmrgx
+
+using namespace dmrgx;
+
+extern "C" {
+
+void dmrgx_engine_default_opts(dmrgx_engine_opts *o) {
+  memset(o, 0, sizeof(*o));
+  o->hbm_budget_bytes = (int64_t)200 << 30;   // of the 288 GB of one MI355X
+  o->max_active = 1024;
+  o->chunk_ticks = 16;
+  o->basis_steps_estimate = 200;
+}
+
+int dmrgx_engine_create(const dmrgx_engine_opts *o, dmrgx_engine **out) {
+  if (!out) return DMRGX_ERR_ARG;
+  return run_status([&] {
+    ensure_init();
+    auto E = std::make_unique<dmrgx_engine>();
+    if (o) E->opts = *o;
+    else dmrgx_engine_default_opts(&E->opts);
+    if (E->opts.max_active <= 0 || E->opts.chunk_ticks <= 0 || E->opts.hbm_budget_bytes <= 0)
+      throw Failure(DMRGX_ERR_ARG, "engine options must be positive");
+    *out = E.release();
+  });
+}
+
+int dmrgx_engine_submit(dmrgx_engine *E, dmrgx_matrix *A, int take_ownership, const dmrgx_solve_params *p,
+                        uint64_t tag, int64_t *request_id) {
+  if (!E || !A || !p) return DMRGX_ERR_ARG;
+  if (p->diag_max_step <= p->diag_min_step || (p->precision != 0 && p->precision != 1) || p->guess)
+    return DMRGX_ERR_ARG;
+  return run_status([&] {
+    auto s = std::make_unique<Slot>();
+    s->id = E->next_id++;
+    s->tag = tag;
+    s->A = A;
+    s->owned = take_ownership ? A : nullptr;
+    s->p = *p;
+    s->mixed = p->precision == 1;
+    s->n = A->n;
+    s->lanes = A->lanes;
+    s->KS = p->diag_max_step + 2;
+    s->CS = std::max(p->cg_max_step, 1) + 1;
+    s->t_submit = now_sec();
+    s->bytes = slot_bytes(*E, *s);
+    if (request_id) *request_id = s->id;
+    E->pending.push_back(std::move(s));
+  });
+}
+
+int dmrgx_engine_run(dmrgx_engine *E, int32_t max_chunks) {
+  if (!E) return DMRGX_ERR_ARG;
+  int rc = run_status([&] {
+    for (int c = 0; c < max_chunks; c++) {
+      admit(*E);
+      if (E->active.empty()) break;
+      run_chunk(*E, E->opts.chunk_ticks);
+      retire(*E);
+    }
+    admit(*E);
+  });
+  return rc;
+}
+
+int32_t dmrgx_engine_pending(const dmrgx_engine *E) {
+  return E ? (int32_t)(E->pending.size() + E->active.size()) : -1;
+}
+
+int32_t dmrgx_engine_done(const dmrgx_engine *E) { return E ? (int32_t)E->done.size() : -1; }
+
+int dmrgx_engine_peek(const dmrgx_engine *E, uint64_t *tag, int32_t *n) {
+  if (!E) return DMRGX_ERR_ARG;
+  std::lock_guard<std::recursive_mutex> lk(api_mutex());
+  if (E->done.empty()) return 0;
+  if (tag) *tag = E->done.front()->tag;
+  if (n) *n = E->done.front()->n;
+  return 1;
+}
+
+int dmrgx_engine_poll(dmrgx_engine *E, dmrgx_engine_result *res, double *vec_out) {
+  if (!E || !res) return DMRGX_ERR_ARG;
+  int got = 0;
+  int rc = run_status([&] {
+    if (E->done.empty()) return;
+    auto s = std::move(E->done.front());
+    E->done.pop_front();
+    memset(res, 0, sizeof(*res));
+    res->request_id = s->id;
+    res->tag = s->tag;
+    res->status = s->status;
+    res->n = s->n;
+    res->eig_val = s->out.eig;
+    res->gs_error = s->out.err;
+    res->lanczos_steps = s->out.lan_steps;
+    res->ii_steps = s->out.ii_steps;
+    res->cg_steps = s->out.cg_steps;
+    res->spmv_calls = s->out.spmv;
+    res->t_submit = s->t_submit;
+    res->t_start = s->t_start;
+    res->t_done = s->t_done;
+    if (vec_out && s->vec.p) {
+      HIPCHK(hipMemcpyAsync(vec_out, s->vec.p, sizeof(double) * s->n, hipMemcpyDeviceToHost, ctx().stream));
+      HIPCHK(hipStreamSynchronize(ctx().stream));
+    }
+    if (s->owned) dmrgx_matrix_free(s->owned);
+    got = 1;
+  });
+  return rc != DMRGX_OK ? rc : got;
+}
+
+void dmrgx_engine_free(dmrgx_engine *E) {
+  if (!E) return;
+  std::lock_guard<std::recursive_mutex> lk(api_mutex());
+  for (auto *q : {&E->pending, &E->done})
+    for (auto &s : *q)
+      if (s->owned) dmrgx_matrix_free(s->owned);
+  for (auto &s : E->active)
+    if (s->owned) dmrgx_matrix_free(s->owned);
+  delete E;
+}
+
+int dmrgx_ground_state_batch(int32_t count, dmrgx_matrix *const *A, const dmrgx_solve_params *p,
+                             double *const *vec_out, dmrgx_solve_stats *stats) {
+  if (count < 0 || (count && (!A || !p))) return DMRGX_ERR_ARG;
+  dmrgx_engine *E = nullptr;
+  dmrgx_engine_opts o;
+  dmrgx_engine_default_opts(&o);
+  int rc = dmrgx_engine_create(&o, &E);
+  if (rc != DMRGX_OK) return rc;
+  for (int32_t i = 0; i < count && rc == DMRGX_OK; i++) rc = dmrgx_engine_submit(E, A[i], 0, p, (uint64_t)i, nullptr);
+  const double t0 = now_sec();
+  while (rc == DMRGX_OK && dmrgx_engine_pending(E) > 0) rc = dmrgx_engine_run(E, 64);
+  int worst = rc;
+  dmrgx_engine_result r;
+  uint64_t next_tag = 0;
+  while (rc == DMRGX_OK && dmrgx_engine_peek(E, &next_tag, nullptr) == 1) {
+    // the vector goes straight to the caller's buffer for this tag
+    const int i = (int)next_tag;
+    int got = dmrgx_engine_poll(E, &r, vec_out ? vec_out[i] : nullptr);
+    if (got < 0) {
+      rc = got;
+      break;
+    }
+    if (r.status != DMRGX_OK && worst == DMRGX_OK) worst = r.status;
+    if (stats) {
+      stats[i].eig_val = r.eig_val;
+      stats[i].gs_error = r.gs_error;
+      stats[i].lanczos_steps = r.lanczos_steps;
+      stats[i].ii_steps = r.ii_steps;
+      stats[i].cg_steps = r.cg_steps;
+      stats[i].spmv_calls = r.spmv_calls;
+      stats[i].seconds = r.t_done - t0;
+    }
+  }
+  dmrgx_engine_free(E);
+  return rc != DMRGX_OK ? rc : worst;
+}
+
+}  // extern "C"

@@ -187,6 +187,36 @@ struct SlotRows {
   }
 };
 
+// This lane's share of sum_j H_rj x_j over CSR row r: entries b+lane, b+lane+G, ... summed in
+// that order.  Four entries are loaded before their x gathers so each lane keeps several
+// loads in flight; the matrix streams past the caches (non-temporal loads) so that x stays
+// resident in L2.  The order of the additions is that of the plain loop.
+template <int G, class VT>
+__device__ __forceinline__ double csr_lane_sum(const int64_t *__restrict__ row, const int32_t *__restrict__ col,
+                                               const VT *__restrict__ val, int32_t r, int lane,
+                                               const double *__restrict__ x) {
+  double s = 0.0;
+  const int64_t e = row[r + 1];
+  int64_t j = row[r] + lane;
+  for (; j + 3 * G < e; j += 4 * G) {
+    const VT v0 = __builtin_nontemporal_load(val + j);
+    const VT v1 = __builtin_nontemporal_load(val + j + G);
+    const VT v2 = __builtin_nontemporal_load(val + j + 2 * G);
+    const VT v3 = __builtin_nontemporal_load(val + j + 3 * G);
+    const int32_t c0 = __builtin_nontemporal_load(col + j);
+    const int32_t c1 = __builtin_nontemporal_load(col + j + G);
+    const int32_t c2 = __builtin_nontemporal_load(col + j + 2 * G);
+    const int32_t c3 = __builtin_nontemporal_load(col + j + 3 * G);
+    const double x0 = x[c0], x1 = x[c1], x2 = x[c2], x3 = x[c3];
+    s += (double)v0 * x0;
+    s += (double)v1 * x1;
+    s += (double)v2 * x2;
+    s += (double)v3 * x3;
+  }
+  for (; j < e; j += G) s += (double)__builtin_nontemporal_load(val + j) * x[__builtin_nontemporal_load(col + j)];
+  return s;
+}
+
 // Binary search of an ascending int64 array: index of target or -1
 // (sml/BINARY_SEARCH_LINT1.c, iterative).
 __device__ __forceinline__ int64_t find_code(const int64_t *__restrict__ a, int64_t n, int64_t target) {
@@ -264,9 +294,8 @@ inline Operator op_of(const dmrgx_matrix &A) { return Operator{A.n, A.lanes, &A,
 inline Operator op_of(const ObcDev &O) { return Operator{O.n, O.lanes, nullptr, &O}; }
 
 // ---- kernel launchers (kernels.hip)
-struct Reduce {              // deterministic grid reduction workspace
-  DevBuf<double> partial;    // one slot per block
-  DevBuf<unsigned> counter;  // last-block ticket, self-resetting
+struct Reduce {              // deterministic two-kernel reduction workspace
+  DevBuf<double> partial;    // one slot per block, summed in order by a finish kernel
   void ensure(size_t blocks);
 };
 
@@ -297,6 +326,9 @@ void launch_mr_init(const Operator &op, double shift, const double *b, const dou
 void launch_mr_step(int64_t n, int k, double acc, const double *alpha, const double *beta, double *mr, int *status,
                     double *coef, const double *v1, const double *w0, const double *w1, double *w2, double *x,
                     hipStream_t s);
+
+// *out = <v, A v> in FP64 (the Rayleigh quotient of a normalised v)
+void launch_rayleigh(const Operator &op, const double *v, double *out, Reduce &R, hipStream_t s);
 
 // inverse iteration / CG kernels
 void launch_residual_l1(const Operator &op, const double *v, double eig, double *out, Reduce &R,
@@ -352,6 +384,10 @@ int cg_solve(const Operator &A, double shift, const double *b, double *x, double
 int mr_solve(const Operator &A, double shift, const double *b, double *x, double acc, int max_step,
              bool *converged, SolveOut &out);
 double converge_check(const Operator &A, const double *v, double eig);
+// <v, A v> / <v, v> in FP64
+double rayleigh_quotient(const Operator &A, const double *v);
+// FP32 copy of the values for the mixed-precision Lanczos (made once per matrix)
+void ensure_fp32(const dmrgx_matrix &A);
 
 // device matrix handed over by dmrgx_get_ham_lllrrrrl for the CRS1 it returned (assemble.hip)
 dmrgx_matrix *take_pending(const CRS1 *M);

@@ -4,9 +4,12 @@
 //   fused per-row epilogue (Lanczos normalise + alpha dot, CG A*p + shift + p.Ap, the
 //   L1 residual of the inverse-iteration convergence check).
 // * Streaming vector kernels for the Lanczos / CG updates with the next reduction fused.
-// * Every reduction is deterministic: each block writes one partial, the last block to
-//   arrive (agent-scope ticket) sums the partials in index order.  Results do not depend
-//   on scheduling, so a solve gives the same bits alone or next to other work.
+// * Every reduction is deterministic: each block stores one partial with a plain store and
+//   a one-block finish kernel sums the partials in index order.  (A last-block ticket with
+//   agent-scope acquire/release atomics would make every block write back its L2 for the
+//   other XCDs to see; with ~36k blocks per SpMV that costs more than the product.)
+//   Results do not depend on scheduling, so a solve gives the same bits alone or next to
+//   other work.
 // * Lower-triangle -> full CSR expansion (counting, scan, scatter, per-row sort).
 
 #include <hip/hip_runtime.h>
@@ -40,28 +43,20 @@ __device__ __forceinline__ double block_sum(double v) {
   return t;
 }
 
-// Publish this block's partial; the last block sums all partials in index order.
-// Returns true in every thread of the last block; *total is valid in its thread 0.
-__device__ bool grid_reduce(double block_total, double *partial, unsigned *counter, double *total) {
-  __shared__ int is_last;
-  if (threadIdx.x == 0) {
-    __hip_atomic_store(&partial[blockIdx.x], block_total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    unsigned t = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    is_last = (t == gridDim.x - 1);
-  }
-  __syncthreads();
-  if (!is_last) return false;
-  __threadfence();
+// This block's share of a reduction; k_finish sums the shares in block order.
+__device__ __forceinline__ void block_partial(double v, double *partial) {
+  const double t = block_sum(v);
+  if (threadIdx.x == 0) partial[blockIdx.x] = t;
+}
+
+// One block: sum of the nblk partials in index order, handed to f.finish().
+template <class Fin>
+__global__ __launch_bounds__(kBlock) void k_finish(Fin f, const double *partial, int nblk) {
+  if (f.skip()) return;
   double s = 0.0;
-  for (unsigned i = threadIdx.x; i < gridDim.x; i += kBlock)
-    s += __hip_atomic_load(&partial[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  __syncthreads();
-  double t = block_sum(s);
-  if (threadIdx.x == 0) {
-    __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    *total = t;
-  }
-  return true;
+  for (int i = threadIdx.x; i < nblk; i += kBlock) s += partial[i];
+  const double t = block_sum(s);
+  if (threadIdx.x == 0) f.finish(t);
 }
 
 // ------------------------------------------------------------------ SpMV + epilogue
@@ -79,16 +74,13 @@ struct CsrRows {
   }
   template <int G>
   __device__ __forceinline__ double lane_sum(int32_t r, int lane, const double *__restrict__ x) const {
-    double s = 0.0;
-    const int64_t b = row[r], e = row[r + 1];
-    for (int64_t j = b + lane; j < e; j += G) s += (double)val[j] * x[col[j]];
-    return s;
+    return csr_lane_sum<G>(row, col, val, r, lane, x);
   }
 };
 
 template <int G, class Rows, class Epi>
 __global__ __launch_bounds__(kBlock) void k_apply(int32_t n, Rows rows, const double *__restrict__ x, Epi epi,
-                                                  double *partial, unsigned *counter) {
+                                                  double *partial) {
   if (epi.skip()) return;
   constexpr int RPB = kBlock / G;
   const int g = threadIdx.x / G;
@@ -101,11 +93,7 @@ __global__ __launch_bounds__(kBlock) void k_apply(int32_t n, Rows rows, const do
   epi.begin();
   double c = 0.0;
   if (lane == 0 && r < n) c = epi.row(r, s);
-  if constexpr (Epi::kReduce) {
-    double bt = block_sum(c);
-    double tot;
-    if (grid_reduce(bt, partial, counter, &tot) && threadIdx.x == 0) epi.finish(tot);
-  }
+  if constexpr (Epi::kReduce) block_partial(c, partial);
 }
 
 struct EpiPlain {
@@ -226,11 +214,9 @@ void apply_rows(int32_t n, int lanes, const Rows &rows, const double *x, const E
   const int blocks = spmv_blocks(n, lanes);
   if (blocks == 0) return;
   double *part = nullptr;
-  unsigned *cnt = nullptr;
   if (Epi::kReduce) {
     R->ensure(blocks);
     part = R->partial.p;
-    cnt = R->counter.p;
   }
   Context &C = ctx();
   std::pair<hipEvent_t, hipEvent_t> *ev = nullptr;
@@ -241,13 +227,17 @@ void apply_rows(int32_t n, int lanes, const Rows &rows, const double *x, const E
   }
   switch (lanes) {
 #define L_CASE(G) \
-  case G: k_apply<G, Rows, Epi><<<blocks, kBlock, 0, s>>>(n, rows, x, epi, part, cnt); break;
+  case G: k_apply<G, Rows, Epi><<<blocks, kBlock, 0, s>>>(n, rows, x, epi, part); break;
     L_CASE(1) L_CASE(2) L_CASE(4) L_CASE(8) L_CASE(16) L_CASE(32) L_CASE(64)
 #undef L_CASE
     default: throw Failure(DMRGX_ERR_ARG, "bad lanes per row");
   }
   HIPCHK(hipGetLastError());
   if (ev) HIPCHK(hipEventRecord(ev->second, s));
+  if (Epi::kReduce) {
+    k_finish<Epi><<<1, kBlock, 0, s>>>(epi, part, blocks);
+    HIPCHK(hipGetLastError());
+  }
 }
 
 template <class Epi>
@@ -263,23 +253,26 @@ void apply_op(const Operator &op, bool fp32, const double *x, const Epi &epi, Re
 }
 
 // -------------------------------------------------------------- vector kernels
-__global__ __launch_bounds__(kBlock) void k_norm2(int64_t n, const double *__restrict__ x,
-                                                  double *out, double *partial, unsigned *counter) {
+__global__ __launch_bounds__(kBlock) void k_norm2(int64_t n, const double *__restrict__ x, double *partial) {
   double s = 0.0;
   for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
     s += x[i] * x[i];
-  double bt = block_sum(s), tot;
-  if (grid_reduce(bt, partial, counter, &tot) && threadIdx.x == 0) *out = tot;
+  block_partial(s, partial);
 }
+
+struct FinStore {
+  double *out;
+  __device__ bool skip() const { return false; }
+  __device__ void finish(double t) const { *out = t; }
+};
 
 // K_b of a Lanczos step (sml/LANCZOS_SLOW_SYM.c:183-186 and :116 of the next step):
 // w = y - alpha_k v_k - beta_k v_{k-1}; nrm2 = <w,w>; beta_{k+1} = sqrt(nrm2)
 __global__ __launch_bounds__(kBlock) void k_lanczos_b(int64_t n, int k, const double *__restrict__ y,
                                                       const double *__restrict__ vk,
                                                       const double *__restrict__ vkm1,
-                                                      const double *alpha, double *beta,
-                                                      double *__restrict__ w, double *nrm2,
-                                                      double *partial, unsigned *counter) {
+                                                      const double *alpha, const double *beta,
+                                                      double *__restrict__ w, double *partial) {
   const double a = alpha[k];
   const double b = (k > 0) ? beta[k] : 0.0;
   double s = 0.0;
@@ -289,12 +282,18 @@ __global__ __launch_bounds__(kBlock) void k_lanczos_b(int64_t n, int k, const do
     w[i] = v;
     s += v * v;
   }
-  double bt = block_sum(s), tot;
-  if (grid_reduce(bt, partial, counter, &tot) && threadIdx.x == 0) {
-    *nrm2 = tot;
-    beta[k + 1] = sqrt(tot);
-  }
+  block_partial(s, partial);
 }
+
+struct FinLanczosB {
+  double *nrm2, *beta;
+  int k;
+  __device__ bool skip() const { return false; }
+  __device__ void finish(double t) const {
+    *nrm2 = t;
+    beta[k + 1] = sqrt(t);
+  }
+};
 
 // Ritz vector sum_j c_j v_j over the stored Lanczos basis (sml/LANCZOS_SYM.c:195-202)
 __global__ __launch_bounds__(kBlock) void k_ritz(int64_t n, int k, const double *const *vecs,
@@ -326,8 +325,8 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(int64_t n, int it, double 
                                                       const double *__restrict__ p,
                                                       const double *__restrict__ y,
                                                       double *__restrict__ x, double *__restrict__ r,
-                                                      const double *pap, double *rr, int *status,
-                                                      double *partial, unsigned *counter) {
+                                                      const double *pap, const double *rr, const int *status,
+                                                      double *partial) {
   if (status[0] != 0) return;
   const double alpha = rr[it - 1] / pap[it];
   double s = 0.0;
@@ -337,13 +336,21 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(int64_t n, int it, double 
     r[i] = rv;
     s += rv * rv;
   }
-  double bt = block_sum(s), tot;
-  if (grid_reduce(bt, partial, counter, &tot) && threadIdx.x == 0) {
-    rr[it] = tot;
-    status[1] = it;
-    if (tot < acc) status[0] = 1;
-  }
+  block_partial(s, partial);
 }
+
+struct FinCgUpdate {
+  double *rr;
+  int *status;
+  int it;
+  double acc;
+  __device__ bool skip() const { return status[0] != 0; }
+  __device__ void finish(double t) const {
+    rr[it] = t;
+    status[1] = it;
+    if (t < acc) status[0] = 1;
+  }
+};
 
 // CG direction (CONJUGATE_GRADIENT_SYM.c:257-262): p = r + (rr_it / rr_{it-1}) p
 __global__ __launch_bounds__(kBlock) void k_cg_dir(int64_t n, int it, const double *__restrict__ r,
@@ -451,23 +458,26 @@ __global__ __launch_bounds__(kBlock) void k_scan_apply(int32_t n, const int64_t 
   if (blockIdx.x == gridDim.x - 1 && threadIdx.x == kBlock - 1) out[n] = ex;
 }
 
-__global__ __launch_bounds__(kBlock) void k_tri_fill(int32_t n, const int64_t *__restrict__ trow,
-                                                     const int32_t *__restrict__ tcol,
-                                                     const double *__restrict__ tval,
-                                                     const int64_t *__restrict__ frow,
-                                                     const int64_t *__restrict__ own,
-                                                     unsigned *__restrict__ tfill,
-                                                     int32_t *__restrict__ fcol,
-                                                     double *__restrict__ fval) {
-  const int32_t r = blockIdx.x * kBlock + threadIdx.x;
+// One wavefront per row: copy the row's own (lower) entries, then append each of them,
+// transposed, to the tail of row c (position by an atomic counter per row; the tails are
+// put in column order by k_sort_tail).
+__global__ __launch_bounds__(64) void k_tri_fill(int32_t n, const int64_t *__restrict__ trow,
+                                                 const int32_t *__restrict__ tcol,
+                                                 const double *__restrict__ tval,
+                                                 const int64_t *__restrict__ frow,
+                                                 const int64_t *__restrict__ own,
+                                                 unsigned *__restrict__ tfill,
+                                                 int32_t *__restrict__ fcol,
+                                                 double *__restrict__ fval) {
+  const int32_t r = blockIdx.x;
   if (r >= n) return;
   const int64_t b = trow[r], e = trow[r + 1];
   const int64_t dst = frow[r];
-  for (int64_t j = b; j < e; j++) {
+  for (int64_t j = b + threadIdx.x; j < e; j += 64) {
     fcol[dst + (j - b)] = tcol[j];
     fval[dst + (j - b)] = tval[j];
   }
-  for (int64_t j = b; j < e - 1; j++) {
+  for (int64_t j = b + threadIdx.x; j < e - 1; j += 64) {
     const int32_t c = tcol[j];
     const int64_t pos = frow[c] + own[c] + atomicAdd(&tfill[c], 1u);
     fcol[pos] = r;
@@ -475,25 +485,66 @@ __global__ __launch_bounds__(kBlock) void k_tri_fill(int32_t n, const int64_t *_
   }
 }
 
-// the transposed part of each row was filled in arrival order: sort it by column
-__global__ __launch_bounds__(kBlock) void k_sort_tail(int32_t n, const int64_t *__restrict__ frow,
-                                                      const int64_t *__restrict__ own,
-                                                      int32_t *__restrict__ fcol,
-                                                      double *__restrict__ fval) {
-  const int32_t r = blockIdx.x * kBlock + threadIdx.x;
+// One wavefront per row: sort the transposed tail by column (the columns are distinct).
+// Tails up to kTailCap entries are bitonic-sorted in LDS; longer ones by one lane.
+constexpr int kTailCap = 512;
+__global__ __launch_bounds__(64) void k_sort_tail(int32_t n, const int64_t *__restrict__ frow,
+                                                  const int64_t *__restrict__ own,
+                                                  int32_t *__restrict__ fcol,
+                                                  double *__restrict__ fval) {
+  __shared__ int32_t kc[kTailCap];
+  __shared__ double kv[kTailCap];
+  const int32_t r = blockIdx.x;
   if (r >= n) return;
+  const int lane = threadIdx.x;
   const int64_t b = frow[r] + own[r], e = frow[r + 1];
-  for (int64_t i = b + 1; i < e; i++) {
-    const int32_t kc = fcol[i];
-    const double kv = fval[i];
-    int64_t j = i - 1;
-    while (j >= b && fcol[j] > kc) {
-      fcol[j + 1] = fcol[j];
-      fval[j + 1] = fval[j];
-      j--;
+  const int m = (int)(e - b);
+  if (m <= 1) return;
+  if (m > kTailCap) {
+    if (lane != 0) return;
+    for (int64_t i = b + 1; i < e; i++) {
+      const int32_t c0 = fcol[i];
+      const double v0 = fval[i];
+      int64_t j = i - 1;
+      while (j >= b && fcol[j] > c0) {
+        fcol[j + 1] = fcol[j];
+        fval[j + 1] = fval[j];
+        j--;
+      }
+      fcol[j + 1] = c0;
+      fval[j + 1] = v0;
     }
-    fcol[j + 1] = kc;
-    fval[j + 1] = kv;
+    return;
+  }
+  int P = 1;
+  while (P < m) P <<= 1;
+  for (int i = lane; i < P; i += 64) {
+    kc[i] = i < m ? fcol[b + i] : INT32_MAX;
+    kv[i] = i < m ? fval[b + i] : 0.0;
+  }
+  __syncthreads();
+  for (int size = 2; size <= P; size <<= 1) {
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      for (int i = lane; i < P; i += 64) {
+        const int jx = i ^ stride;
+        if (jx > i) {
+          const bool up = (i & size) == 0;
+          const int32_t x = kc[i], y = kc[jx];
+          if ((x > y) == up) {
+            kc[i] = y;
+            kc[jx] = x;
+            const double t = kv[i];
+            kv[i] = kv[jx];
+            kv[jx] = t;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+  for (int i = lane; i < m; i += 64) {
+    fcol[b + i] = kc[i];
+    fval[b + i] = kv[i];
   }
 }
 
@@ -529,10 +580,6 @@ inline int vec_grid(int64_t n) {
 // ------------------------------------------------------------------ launchers
 void Reduce::ensure(size_t blocks) {
   if (partial.count < blocks) partial.resize(blocks);
-  if (!counter.p) {
-    counter.resize(1);
-    HIPCHK(hipMemsetAsync(counter.p, 0, sizeof(unsigned), ctx().stream));
-  }
 }
 
 int pick_lanes(int32_t n, int64_t nnz) {
@@ -565,7 +612,8 @@ void launch_ed_apply(int32_t n, const EdRows &rows, const double *x, double *y, 
 void launch_norm2(int64_t n, const double *x, Reduce &R, double *out, hipStream_t s) {
   const int g = vec_grid(n);
   R.ensure(g);
-  k_norm2<<<g, kBlock, 0, s>>>(n, x, out, R.partial.p, R.counter.p);
+  k_norm2<<<g, kBlock, 0, s>>>(n, x, R.partial.p);
+  k_finish<FinStore><<<1, kBlock, 0, s>>>(FinStore{out}, R.partial.p, g);
   HIPCHK(hipGetLastError());
 }
 
@@ -579,8 +627,8 @@ void launch_lanczos_b(int64_t n, int k, const double *y, const double *vk, const
                       hipStream_t s) {
   const int g = vec_grid(n);
   R.ensure(g);
-  k_lanczos_b<<<g, kBlock, 0, s>>>(n, k, y, vk, vkm1 ? vkm1 : vk, alpha, beta, w, nrm2, R.partial.p,
-                                   R.counter.p);
+  k_lanczos_b<<<g, kBlock, 0, s>>>(n, k, y, vk, vkm1 ? vkm1 : vk, alpha, beta, w, R.partial.p);
+  k_finish<FinLanczosB><<<1, kBlock, 0, s>>>(FinLanczosB{nrm2, beta, k}, R.partial.p, g);
   HIPCHK(hipGetLastError());
 }
 
@@ -598,6 +646,10 @@ void launch_scale_by_rnorm(int64_t n, const double *nrm2, const double *x, doubl
 void launch_copy(int64_t n, const double *x, double *y, hipStream_t s) {
   k_copy<<<vec_grid(n), kBlock, 0, s>>>(n, x, y);
   HIPCHK(hipGetLastError());
+}
+
+void launch_rayleigh(const Operator &op, const double *v, double *out, Reduce &R, hipStream_t s) {
+  apply_op(op, false, v, EpiStoreDot{nullptr, v, out}, &R, s);
 }
 
 void launch_residual_l1(const Operator &op, const double *v, double eig, double *out, Reduce &R,
@@ -621,7 +673,8 @@ void launch_cg_update(int64_t n, int it, double acc, const double *p, const doub
                       hipStream_t s) {
   const int g = vec_grid(n);
   R.ensure(g);
-  k_cg_update<<<g, kBlock, 0, s>>>(n, it, acc, p, y, x, r, pap, rr, status, R.partial.p, R.counter.p);
+  k_cg_update<<<g, kBlock, 0, s>>>(n, it, acc, p, y, x, r, pap, rr, status, R.partial.p);
+  k_finish<FinCgUpdate><<<1, kBlock, 0, s>>>(FinCgUpdate{rr, status, it, acc}, R.partial.p, g);
   HIPCHK(hipGetLastError());
 }
 
@@ -769,8 +822,8 @@ void launch_expand_lower(int32_t n, int64_t nnz_tri, const int64_t *trow, const 
   A.nnz = nnz;
   A.col.resize(nnz);
   A.val.resize(nnz);
-  k_tri_fill<<<nb, kBlock, 0, s>>>(n, trow, tcol, tval, A.row.p, own.p, tfill.p, A.col.p, A.val.p);
-  k_sort_tail<<<nb, kBlock, 0, s>>>(n, A.row.p, own.p, A.col.p, A.val.p);
+  k_tri_fill<<<n, 64, 0, s>>>(n, trow, tcol, tval, A.row.p, own.p, tfill.p, A.col.p, A.val.p);
+  k_sort_tail<<<n, 64, 0, s>>>(n, A.row.p, own.p, A.col.p, A.val.p);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(s));
 }

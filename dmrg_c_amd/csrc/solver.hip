@@ -168,6 +168,27 @@ double converge_check(const Operator &A, const double *v, double eig) {
   return read_scalar(res.p, C.stream);
 }
 
+double rayleigh_quotient(const Operator &A, const double *v) {
+  Context &C = ctx();
+  Reduce R;
+  DevBuf<double> res(2);
+  launch_rayleigh(A, v, res.p, R, C.stream);
+  launch_norm2(A.n, v, R, res.p + 1, C.stream);
+  double *h = C.host_stage(2);
+  HIPCHK(hipMemcpyAsync(h, res.p, 2 * sizeof(double), hipMemcpyDeviceToHost, C.stream));
+  HIPCHK(hipStreamSynchronize(C.stream));
+  return h[0] / h[1];
+}
+
+void ensure_fp32(const dmrgx_matrix &A) {
+  if (A.val32.p) return;
+  // the FP32 values are a cache of the FP64 ones; the matrix itself is not changed
+  dmrgx_matrix &M = const_cast<dmrgx_matrix &>(A);
+  M.val32.resize(std::max<int64_t>(A.nnz, 1));
+  launch_to_fp32(M, ctx().stream);
+  HIPCHK(hipStreamSynchronize(ctx().stream));
+}
+
 int cg_solve(const Operator &A, double shift, const double *b, double *x, double acc, int max_step,
              bool *converged, SolveOut &out) {
   Context &C = ctx();

@@ -595,6 +595,48 @@ int dmrgx_ed_model_describe(const dmrgx_ed_model *m, int32_t *d, int32_t *n_ops,
                             int32_t *op_col, double *op_val, int32_t *n_terms, dmrgx_ed_term *terms, int32_t *n_ele,
                             int32_t *qn1, int32_t *qn2, int32_t *targets);
 
+/* ---- batched solves and continuous batching
+ * An engine runs many independent ground-state solves together: every Lanczos step (and
+ * every CG iteration of the refinement) of all active requests is one launch per kernel,
+ * rows of all requests side by side; each request keeps the blocks, lanes and summation
+ * order of dmrgx_ground_state, so its result is bit-identical to a solve on its own.
+ * Requests are admitted at chunk boundaries while the HBM budget allows. */
+typedef struct dmrgx_engine dmrgx_engine;
+typedef struct {
+   int64_t hbm_budget_bytes;   /* device memory the active requests may hold            */
+   int32_t max_active;         /* requests advancing together                           */
+   int32_t chunk_ticks;        /* steps between host checks                             */
+   int32_t basis_steps_estimate; /* Lanczos vectors assumed per request for admission   */
+   int32_t reserved;
+} dmrgx_engine_opts;
+typedef struct {
+   int64_t request_id;
+   uint64_t tag;
+   int32_t status;             /* DMRGX_OK or DMRGX_ERR_NOCONV                            */
+   int32_t n;
+   double eig_val, gs_error;
+   int32_t lanczos_steps, ii_steps, cg_steps, spmv_calls;
+   double t_submit, t_start, t_done;   /* steady-clock seconds                            */
+} dmrgx_engine_result;
+void dmrgx_engine_default_opts(dmrgx_engine_opts *o);
+int dmrgx_engine_create(const dmrgx_engine_opts *o, dmrgx_engine **out);
+/* Queue a solve of A (a dmrgx_matrix; take_ownership != 0 frees it with the request).
+ * p->guess must be 0 (the start vector comes from p->seed). */
+int dmrgx_engine_submit(dmrgx_engine *E, dmrgx_matrix *A, int take_ownership, const dmrgx_solve_params *p,
+                        uint64_t tag, int64_t *request_id);
+/* Admit and advance up to max_chunks chunks of steps. */
+int dmrgx_engine_run(dmrgx_engine *E, int32_t max_chunks);
+int32_t dmrgx_engine_pending(const dmrgx_engine *E);   /* queued + active */
+int32_t dmrgx_engine_done(const dmrgx_engine *E);      /* finished, not yet polled */
+/* Tag and dimension of the result dmrgx_engine_poll would return next: 1, or 0 if none. */
+int dmrgx_engine_peek(const dmrgx_engine *E, uint64_t *tag, int32_t *n);
+/* Oldest finished request: returns 1 and fills *res (and vec_out[0..n) if given), 0 if none. */
+int dmrgx_engine_poll(dmrgx_engine *E, dmrgx_engine_result *res, double *vec_out);
+void dmrgx_engine_free(dmrgx_engine *E);
+/* count solves through one engine; vec_out[i] (host, n_i doubles) and stats[i] may be NULL. */
+int dmrgx_ground_state_batch(int32_t count, dmrgx_matrix *const *A, const dmrgx_solve_params *p,
+                             double *const *vec_out, dmrgx_solve_stats *stats);
+
 /* ---- measurement
  * Between begin and end, every CSR SpMV launch of the library is bracketed by HIP events
  * on the library's stream; end returns the summed kernel time, the launch count and the
