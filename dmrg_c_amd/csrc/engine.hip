@@ -617,10 +617,11 @@ void run_chunk(dmrgx_engine &E, int ticks) {
     }
     if (!any) break;
   }
-  if (launches.empty()) return;
-  ensure_scratch(E, hdesc.size(), hblk.size());
-  HIPCHK(hipMemcpyAsync(E.ddesc.p, hdesc.data(), sizeof(SegDesc) * hdesc.size(), hipMemcpyHostToDevice, st));
-  HIPCHK(hipMemcpyAsync(E.dblk.p, hblk.data(), sizeof(int32_t) * hblk.size(), hipMemcpyHostToDevice, st));
+  if (!launches.empty()) {
+    ensure_scratch(E, hdesc.size(), hblk.size());
+    HIPCHK(hipMemcpyAsync(E.ddesc.p, hdesc.data(), sizeof(SegDesc) * hdesc.size(), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(E.dblk.p, hblk.data(), sizeof(int32_t) * hblk.size(), hipMemcpyHostToDevice, st));
+  }
   for (const Launch &l : launches) {
     const SegDesc *dd = E.ddesc.p + l.desc_off;
     const int32_t *bb = E.dblk.p + l.blk_off;
