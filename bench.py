@@ -41,12 +41,28 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def gpu_solve(D, sb, params):
+def gpu_solve(D, sb, params, sched=None):
     A = sb.assemble(full=True)
+    if sched:
+        D.check(D.lib().dmrgx_matrix_set_schedule(A.handle, *sched))
     E, vec, st = A.ground_state(**params)
     nnz = A.nnz
     A.free()
     return E, st, nnz
+
+
+def autotune(D, sb):
+    """SpMV schedule (lanes per row, workgroup cap) for this workload, picked on the device by
+    dmrgx_matrix_autotune; the superblocks of one DMRG run share their row statistics."""
+    L = D.lib()
+    L.dmrgx_matrix_autotune.argtypes = [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                        C.POINTER(C.c_double)]
+    L.dmrgx_matrix_set_schedule.argtypes = [C.c_void_p, C.c_int32, C.c_int32]
+    A = sb.assemble(full=True)
+    g, cap, us = C.c_int32(), C.c_int32(), C.c_double()
+    D.check(L.dmrgx_matrix_autotune(A.handle, C.byref(g), C.byref(cap), C.byref(us)))
+    A.free()
+    return (g.value, cap.value), us.value
 
 
 def cpu_baseline(snap, e_ref, budget_s):
@@ -129,7 +145,9 @@ def traffic_probe(D, L, sb, snap):
     L.dmrgx_device_alloc.argtypes = [C.c_int64, C.POINTER(C.c_void_p)]
     L.dmrgx_device_free.argtypes = [C.c_void_p]
     L.dmrgx_copy_device.argtypes = [C.c_void_p, C.c_void_p, C.c_int64]
+    sched, _ = autotune(D, sb)
     A = sb.assemble(full=True)
+    D.check(L.dmrgx_matrix_set_schedule(A.handle, *sched))
     n, ncopy = snap.n, 1 << 26
     bufs = []
     for nbytes in (8 * n, 8 * n, 8 * ncopy, 8 * ncopy):
@@ -142,10 +160,71 @@ def traffic_probe(D, L, sb, snap):
         D.check(L.dmrgx_spmv_device(A.handle, x, y))
     D.check(L.dmrgx_copy_device(src, dst, ncopy))
     alg = A.nnz * 12 + (n + 1) * 8 + 2 * n * 8
-    print(json.dumps({"spmv_launches": 5, "n": n, "nnz": A.nnz, "spmv_alg_bytes": alg,
+    print(json.dumps({"spmv_launches": 5, "n": n, "nnz": A.nnz, "spmv_alg_bytes": alg, "schedule": sched,
                       "copy_doubles": ncopy, "copy_bytes_read": 8 * ncopy, "copy_bytes_written": 8 * ncopy}))
     for p in bufs:
         L.dmrgx_device_free(p)
+
+
+# ------------------------------------------------------------------ batched ED workload
+def ed_batch_models(count=256, seed=0):
+    """The batched-ED mix: XXZ chains L in {12, 14, ..., 24} at S^z = 0 (odd L has no S^z = 0
+    sector for spin 1/2) and Hubbard chains L in {6, ..., 10} at half filling (S^z = 0, or
+    S^z = 1/2 for odd L); J_z ~ U[0.5, 1.5], h ~ U[0, 0.5], t = 1, U = 4; one seeded draw."""
+    from dmrg_c_amd import ed
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(count):
+        if rng.random() < 0.5:
+            L = int(rng.choice(np.arange(12, 25, 2)))
+            out.append(ed.Model("xxz", L=L, sz=0, J_z=float(rng.uniform(0.5, 1.5)), h_z=float(rng.uniform(0, 0.5))))
+        else:
+            L = int(rng.integers(6, 11))
+            out.append(ed.Model("hubbard", L=L, N=L, sz=L % 2, t=-1.0, U=4.0, h_z=float(rng.uniform(0, 0.5))))
+    return out
+
+
+def run_ed_batch(D, count, precision_forms=(0, 1)):
+    from dmrg_c_amd import ed, engine
+    models = ed_batch_models(count)
+    t0 = time.perf_counter()
+    mats = []
+    for m in models:
+        P = ed.EdProblem.from_model(m)
+        mats.append(P.assemble())
+        P.free()
+    t_asm = time.perf_counter() - t0
+    dims = [A.n for A in mats]
+    log(f"ed-batch: {count} problems, n from {min(dims)} to {max(dims)}, total rows {sum(dims)}, "
+        f"device assembly {t_asm:.2f} s")
+    params = dict(diag_acc=1e-14, diag_max_step=800, cg_acc=1e-7, cg_max_step=2000, ii_acc=1e-9,
+                  ii_diag_add=1e-11, ii_max_step=3, seed=1)
+    res = {"problems": count, "n_min": min(dims), "n_max": max(dims), "rows_total": int(sum(dims)),
+           "device_assembly_s": round(t_asm, 3)}
+    single_e = None
+    for prec in precision_forms:
+        p = dict(params, precision=prec)
+        t0 = time.perf_counter()
+        es = [A.ground_state(**p)[0] for A in mats]
+        t_single = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        out = engine.ground_state_batch(mats, engine.default_params(**p), want_vec=False)
+        t_batch = time.perf_counter() - t0
+        eb = [o[0] for o in out]
+        same = all(a == b for a, b in zip(es, eb))
+        if prec == 0:
+            single_e = es
+            err = 0.0
+        else:
+            err = max(abs(a - b) / abs(b) for a, b in zip(eb, single_e))
+        tag = "fp64" if prec == 0 else "fp32_refined"
+        res[tag] = {"single_solves_per_s": round(count / t_single, 2), "batched_solves_per_s": round(count / t_batch, 2),
+                    "batched_over_single": round(t_single / t_batch, 2), "batched_equals_single": same,
+                    "max_rel_energy_diff_vs_fp64": err}
+        log(f"ed-batch {tag}: single {count / t_single:.1f}/s, batched {count / t_batch:.1f}/s, identical={same}")
+    for A in mats:
+        A.free()
+    return res
 
 
 def main():
@@ -158,6 +237,10 @@ def main():
     ap.add_argument("--traffic-probe", action="store_true",
                     help="run 5 plain SpMV launches on the workload and one copy kernel of 2^26 doubles "
                          "(traffic calibration under rocprofv3 --pmc), then exit")
+    ap.add_argument("--workload", choices=["superblock", "ed-batch"], default="superblock",
+                    help="superblock: the headline line; ed-batch: 256 seeded ED solves one by one and "
+                         "through the batched engine (a separate JSON line)")
+    ap.add_argument("--ed-count", type=int, default=256)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the superblock eigensolve runs on a single MI355X "
@@ -169,6 +252,10 @@ def main():
 
     L = D.lib()
     D.check(L.dmrgx_init(0))
+    if args.workload == "ed-batch":
+        print(json.dumps({"metric": "batched ED ground-state solves/s", "workload": "ed-batch",
+                          "result": run_ed_batch(D, args.ed_count)}), flush=True)
+        return
     snap = snapshot.load(WORKLOAD)
     e_ref = float(snap.meta["gs_energy"][0])
     params = dict(diag_acc=1e-14, diag_max_step=800, cg_acc=1e-7, cg_max_step=2000, ii_acc=1e-9,
@@ -178,8 +265,10 @@ def main():
         traffic_probe(D, L, sb, snap)
         return
     log(f"workload: Hubbard L=64 m=400 superblock n={snap.n}")
+    lanes, tuned_us = autotune(D, sb)
+    log(f"autotune: (lanes per row, workgroup cap) = {lanes}, {tuned_us:.1f} us per SpMV")
     for i in range(args.warmup):
-        E, st, nnz = gpu_solve(D, sb, params)
+        E, st, nnz = gpu_solve(D, sb, params, lanes)
         log(f"warmup {i}: E={E:.15f} lanczos={st.lanczos_steps} cg={st.cg_steps}")
 
     L.dmrgx_profile_begin()
@@ -187,7 +276,7 @@ def main():
     t0 = time.perf_counter()
     for i in range(args.steps):
         ts = time.perf_counter()
-        E, st, nnz = gpu_solve(D, sb, params)
+        E, st, nnz = gpu_solve(D, sb, params, lanes)
         lat.append(time.perf_counter() - ts)
         errs.append(abs(E - e_ref) / abs(e_ref))
         lan.append(st.lanczos_steps)
@@ -199,6 +288,14 @@ def main():
     D.check(L.dmrgx_profile_end(C.byref(spmv_ms), C.byref(launches), C.byref(alg_bytes)))
 
     total = t1 - t0
+    # the same solves in the mixed-precision form (FP32 SpMV in Lanczos, FP64 refinement)
+    mixed = dict(params, precision=1)
+    mlat, merrs = [], []
+    for i in range(max(2, args.steps // 4)):
+        ts = time.perf_counter()
+        E, st, _ = gpu_solve(D, sb, mixed, lanes)
+        mlat.append(time.perf_counter() - ts)
+        merrs.append(abs(E - e_ref) / abs(e_ref))
     avg_launch_s = spmv_ms.value / 1e3 / max(launches.value, 1)
     bytes_per_launch = alg_bytes.value / max(launches.value, 1)
     achieved = bytes_per_launch / avg_launch_s / 1e9 if avg_launch_s > 0 else 0.0
@@ -229,6 +326,14 @@ def main():
             "max_rel_energy_error": float(max(errs)),
             "lanczos_steps": int(np.median(lan)),
             "cg_steps": int(np.median(cg)),
+            "spmv_schedule": {"lanes_per_row": lanes[0], "workgroup_cap": lanes[1]},
+        },
+        "forms": {
+            "fp32_refined": {"solves_per_s": round(len(mlat) / sum(mlat), 4),
+                             "p99_ms": round(float(np.percentile(mlat, 99)) * 1e3, 3),
+                             "max_rel_energy_error": float(max(merrs)),
+                             "note": "FP32 CSR values in the Lanczos SpMV, FP64 recurrence scalars, FP64 "
+                                     "inverse iteration/CG and Rayleigh quotient"},
         },
         "roofline": {
             "bound": "hbm",

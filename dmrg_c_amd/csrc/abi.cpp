@@ -9,6 +9,7 @@
 
 #include <cstdarg>
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdlib>
@@ -396,6 +397,65 @@ int dmrgx_matrix_download(const dmrgx_matrix *A, int64_t *row, int32_t *col, dou
 int dmrgx_spmv_device(const dmrgx_matrix *A, const double *x_dev, double *y_dev) {
   if (!A) return DMRGX_ERR_ARG;
   return run_status([&] { launch_spmv(op_of(*A), x_dev, y_dev, ctx().stream); });
+}
+
+int dmrgx_matrix_set_schedule(dmrgx_matrix *A, int32_t lanes, int32_t grid_cap) {
+  if (!A || lanes < 1 || lanes > 64 || (lanes & (lanes - 1)) || grid_cap < 0) return DMRGX_ERR_ARG;
+  std::lock_guard<std::recursive_mutex> lk(api_mutex());
+  A->lanes = lanes;
+  A->grid = grid_cap;
+  return DMRGX_OK;
+}
+
+int dmrgx_matrix_get_schedule(const dmrgx_matrix *A, int32_t *lanes, int32_t *grid_cap) {
+  if (!A) return DMRGX_ERR_ARG;
+  if (lanes) *lanes = A->lanes;
+  if (grid_cap) *grid_cap = A->grid;
+  return DMRGX_OK;
+}
+
+// SpMV schedule autotuner: times y = A x (HIP events on the library stream) for each
+// power-of-two lanes-per-row count around the mean row length and each workgroup cap, and
+// keeps the fastest pair.
+int dmrgx_matrix_autotune(dmrgx_matrix *A, int32_t *lanes_out, int32_t *grid_out, double *best_us_out) {
+  if (!A) return DMRGX_ERR_ARG;
+  return run_status([&] {
+    hipStream_t st = ctx().stream;
+    DevBuf<double> x(A->n), y(A->n);
+    launch_start_vector(A->n, 7, x.p, st);
+    const int base = pick_lanes(A->n, A->nnz);
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    int best_g = A->lanes, best_c = A->grid;
+    float best_ms = 1e30f;
+    const int caps[] = {1024, 2048, 4096, 8192, 16384, 0};
+    for (int g = std::max(1, base / 4); g <= std::min(64, base * 2); g *= 2) {
+      for (int cap : caps) {
+        A->lanes = g;
+        A->grid = cap;
+        launch_spmv(op_of(*A), x.p, y.p, st);  // warm
+        HIPCHK(hipEventRecord(e0, st));
+        for (int r = 0; r < 3; r++) launch_spmv(op_of(*A), x.p, y.p, st);
+        HIPCHK(hipEventRecord(e1, st));
+        HIPCHK(hipEventSynchronize(e1));
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+        if (ms < best_ms) {
+          best_ms = ms;
+          best_g = g;
+          best_c = cap;
+        }
+      }
+    }
+    HIPCHK(hipEventDestroy(e0));
+    HIPCHK(hipEventDestroy(e1));
+    A->lanes = best_g;
+    A->grid = best_c;
+    if (lanes_out) *lanes_out = best_g;
+    if (grid_out) *grid_out = best_c;
+    if (best_us_out) *best_us_out = best_ms * 1000.0 / 3.0;
+  });
 }
 
 int dmrgx_spmv_host(const dmrgx_matrix *A, const double *x, double *y) {

@@ -118,27 +118,30 @@ __global__ __launch_bounds__(kBlock) void k_seg_spmv(const SegDesc *desc, const 
   constexpr int RPB = kBlock / G;
   const int g = threadIdx.x / G;
   const int lane = threadIdx.x % G;
-  const int32_t r = (blockIdx.x - d.blk_lo) * RPB + g;
-  double s = 0.0;
-  if (r < d.n) {
-    if (d.fp32) s = csr_lane_sum<G>(d.row, d.col, d.val32, r, lane, d.x);
-    else s = csr_lane_sum<G>(d.row, d.col, d.val, r, lane, d.x);
-  }
-#pragma unroll
-  for (int off = G / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, G);
+  const double scale = (MODE == LANCZOS_A) ? 1.0 / sqrt(*d.s1) : 0.0;
   double c = 0.0;
-  if (lane == 0 && r < d.n) {
-    if (MODE == LANCZOS_A) {
-      const double scale = 1.0 / sqrt(*d.s1);
-      const double xs = scale * d.x[r];
-      d.o1[r] = xs;
-      const double yv = scale * s + d.shift * xs;
-      d.o2[r] = yv;
-      c = yv * xs;
-    } else {
-      const double yv = s + d.shift * d.x[r];
-      d.o1[r] = yv;
-      c = d.x[r] * yv;
+  // row groups round-robin over the slot's blocks, as k_apply deals them
+  for (int32_t r0 = (blockIdx.x - d.blk_lo) * RPB; r0 < d.n; r0 += d.nblk * RPB) {
+    const int32_t r = r0 + g;
+    double s = 0.0;
+    if (r < d.n) {
+      if (d.fp32) s = csr_lane_sum<G>(d.row, d.col, d.val32, r, lane, d.x);
+      else s = csr_lane_sum<G>(d.row, d.col, d.val, r, lane, d.x);
+    }
+#pragma unroll
+    for (int off = G / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, G);
+    if (lane == 0 && r < d.n) {
+      if (MODE == LANCZOS_A) {
+        const double xs = scale * d.x[r];
+        d.o1[r] = xs;
+        const double yv = scale * s + d.shift * xs;
+        d.o2[r] = yv;
+        c += yv * xs;
+      } else {
+        const double yv = s + d.shift * d.x[r];
+        d.o1[r] = yv;
+        c += d.x[r] * yv;
+      }
     }
   }
   block_partial(c, partial);
@@ -548,7 +551,7 @@ void run_chunk(dmrgx_engine &E, int ticks) {
         a.s1 = s.nrm2();
         a.s2 = s.alpha() + k;
         segA.push_back(a);
-        nbA.push_back(spmv_blocks(s.n, kv.first));
+        nbA.push_back(spmv_blocks(s.n, kv.first, s.A->grid));
         SegDesc b{};
         b.n = s.n;
         b.k = k;
@@ -586,7 +589,7 @@ void run_chunk(dmrgx_engine &E, int ticks) {
         a.s1 = s.pap() + it;
         a.status = s.cgst.p;
         sp.push_back(a);
-        nbp.push_back(spmv_blocks(s.n, kv.first));
+        nbp.push_back(spmv_blocks(s.n, kv.first, s.A->grid));
         SegDesc u{};
         u.n = s.n;
         u.k = it;

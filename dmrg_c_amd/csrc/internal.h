@@ -17,6 +17,7 @@ namespace dmrgx {
 
 constexpr int kBlock = 256;          // threads per workgroup (4 wave64)
 constexpr int kVecGrid = 1024;       // fixed grid of the streaming vector kernels
+constexpr int kGridCap = 4096;       // default cap on SpMV workgroups (row groups round-robin)
 
 // Error carried from the HIP layer up to the C entry points.
 struct Failure : std::runtime_error {
@@ -102,6 +103,7 @@ struct dmrgx_matrix {
   dmrgx::DevBuf<double> val;
   dmrgx::DevBuf<float> val32;   // FP32 copy of val for the mixed-precision Lanczos
   int lanes = 8;                // lanes per row of the SpMV kernel
+  int grid = dmrgx::kGridCap;   // cap on SpMV workgroups (0: one per row group)
 };
 
 namespace dmrgx {
@@ -188,32 +190,35 @@ struct SlotRows {
 };
 
 // This lane's share of sum_j H_rj x_j over CSR row r: entries b+lane, b+lane+G, ... summed in
-// that order.  Four entries are loaded before their x gathers so each lane keeps several
-// loads in flight; the matrix streams past the caches (non-temporal loads) so that x stays
-// resident in L2.  The order of the additions is that of the plain loop.
+// that order.  Each pass issues the lane's next four (value, column) loads and their four
+// x gathers together; positions past the row end load the row's last entry again and are
+// masked out of the sum, so there is no branch between the loads.  The matrix streams past
+// the caches (non-temporal loads) so that x stays resident in L2.  The order of the
+// additions is that of the plain loop.
 template <int G, class VT>
 __device__ __forceinline__ double csr_lane_sum(const int64_t *__restrict__ row, const int32_t *__restrict__ col,
                                                const VT *__restrict__ val, int32_t r, int lane,
                                                const double *__restrict__ x) {
   double s = 0.0;
   const int64_t e = row[r + 1];
-  int64_t j = row[r] + lane;
-  for (; j + 3 * G < e; j += 4 * G) {
+  const int64_t last = e - 1;
+  for (int64_t j = row[r] + lane; j < e; j += 4 * G) {
+    const bool p1 = j + G < e, p2 = j + 2 * G < e, p3 = j + 3 * G < e;
+    const int64_t j1 = p1 ? j + G : last, j2 = p2 ? j + 2 * G : last, j3 = p3 ? j + 3 * G : last;
     const VT v0 = __builtin_nontemporal_load(val + j);
-    const VT v1 = __builtin_nontemporal_load(val + j + G);
-    const VT v2 = __builtin_nontemporal_load(val + j + 2 * G);
-    const VT v3 = __builtin_nontemporal_load(val + j + 3 * G);
+    const VT v1 = __builtin_nontemporal_load(val + j1);
+    const VT v2 = __builtin_nontemporal_load(val + j2);
+    const VT v3 = __builtin_nontemporal_load(val + j3);
     const int32_t c0 = __builtin_nontemporal_load(col + j);
-    const int32_t c1 = __builtin_nontemporal_load(col + j + G);
-    const int32_t c2 = __builtin_nontemporal_load(col + j + 2 * G);
-    const int32_t c3 = __builtin_nontemporal_load(col + j + 3 * G);
+    const int32_t c1 = __builtin_nontemporal_load(col + j1);
+    const int32_t c2 = __builtin_nontemporal_load(col + j2);
+    const int32_t c3 = __builtin_nontemporal_load(col + j3);
     const double x0 = x[c0], x1 = x[c1], x2 = x[c2], x3 = x[c3];
     s += (double)v0 * x0;
-    s += (double)v1 * x1;
-    s += (double)v2 * x2;
-    s += (double)v3 * x3;
+    if (p1) s += (double)v1 * x1;
+    if (p2) s += (double)v2 * x2;
+    if (p3) s += (double)v3 * x3;
   }
-  for (; j < e; j += G) s += (double)__builtin_nontemporal_load(val + j) * x[__builtin_nontemporal_load(col + j)];
   return s;
 }
 
@@ -287,11 +292,12 @@ struct ObcDev {
 struct Operator {
   int32_t n = 0;
   int lanes = 8;
+  int grid = kGridCap;
   const dmrgx_matrix *A = nullptr;
   const ObcDev *obc = nullptr;
 };
-inline Operator op_of(const dmrgx_matrix &A) { return Operator{A.n, A.lanes, &A, nullptr}; }
-inline Operator op_of(const ObcDev &O) { return Operator{O.n, O.lanes, nullptr, &O}; }
+inline Operator op_of(const dmrgx_matrix &A) { return Operator{A.n, A.lanes, A.grid, &A, nullptr}; }
+inline Operator op_of(const ObcDev &O) { return Operator{O.n, O.lanes, kGridCap, nullptr, &O}; }
 
 // ---- kernel launchers (kernels.hip)
 struct Reduce {              // deterministic two-kernel reduction workspace
@@ -300,7 +306,7 @@ struct Reduce {              // deterministic two-kernel reduction workspace
 };
 
 int pick_lanes(int32_t n, int64_t nnz);
-int spmv_blocks(int32_t n, int lanes);
+int spmv_blocks(int32_t n, int lanes, int grid_cap);
 
 void launch_spmv(const Operator &op, const double *x, double *y, hipStream_t s);
 void launch_expand_lower(int32_t n, int64_t nnz_tri, const int64_t *trow, const int32_t *tcol,

@@ -78,6 +78,9 @@ struct CsrRows {
   }
 };
 
+// Row groups of kBlock/G rows are dealt to the blocks round-robin (the grid is capped, so a
+// block takes several groups); each thread's epilogue contributions are summed in the order
+// of its groups, then per block.
 template <int G, class Rows, class Epi>
 __global__ __launch_bounds__(kBlock) void k_apply(int32_t n, Rows rows, const double *__restrict__ x, Epi epi,
                                                   double *partial) {
@@ -85,14 +88,16 @@ __global__ __launch_bounds__(kBlock) void k_apply(int32_t n, Rows rows, const do
   constexpr int RPB = kBlock / G;
   const int g = threadIdx.x / G;
   const int lane = threadIdx.x % G;
-  const int32_t r = blockIdx.x * RPB + g;
-  double s = 0.0;
-  if (r < n) s = rows.template lane_sum<G>(r, lane, x);
-#pragma unroll
-  for (int off = G / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, G);
   epi.begin();
   double c = 0.0;
-  if (lane == 0 && r < n) c = epi.row(r, s);
+  for (int32_t r0 = blockIdx.x * RPB; r0 < n; r0 += gridDim.x * RPB) {
+    const int32_t r = r0 + g;
+    double s = 0.0;
+    if (r < n) s = rows.template lane_sum<G>(r, lane, x);
+#pragma unroll
+    for (int off = G / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, G);
+    if (lane == 0 && r < n) c += epi.row(r, s);
+  }
   if constexpr (Epi::kReduce) block_partial(c, partial);
 }
 
@@ -209,9 +214,9 @@ struct EpiStoreDot {
 };
 
 template <class Rows, class Epi>
-void apply_rows(int32_t n, int lanes, const Rows &rows, const double *x, const Epi &epi, Reduce *R,
+void apply_rows(int32_t n, int lanes, int grid_cap, const Rows &rows, const double *x, const Epi &epi, Reduce *R,
                 hipStream_t s) {
-  const int blocks = spmv_blocks(n, lanes);
+  const int blocks = spmv_blocks(n, lanes, grid_cap);
   if (blocks == 0) return;
   double *part = nullptr;
   if (Epi::kReduce) {
@@ -243,12 +248,14 @@ void apply_rows(int32_t n, int lanes, const Rows &rows, const double *x, const E
 template <class Epi>
 void apply_op(const Operator &op, bool fp32, const double *x, const Epi &epi, Reduce *R, hipStream_t s) {
   if (op.obc) {
-    apply_rows(op.n, op.lanes, op.obc->rows(), x, epi, R, s);
+    apply_rows(op.n, op.lanes, op.grid, op.obc->rows(), x, epi, R, s);
   } else if (fp32) {
     if (!op.A->val32.p) throw Failure(DMRGX_ERR_ARG, "FP32 values not prepared");
-    apply_rows(op.n, op.lanes, CsrRows<float>{op.A->row.p, op.A->col.p, op.A->val32.p, op.A->nnz}, x, epi, R, s);
+    apply_rows(op.n, op.lanes, op.grid, CsrRows<float>{op.A->row.p, op.A->col.p, op.A->val32.p, op.A->nnz}, x, epi,
+               R, s);
   } else {
-    apply_rows(op.n, op.lanes, CsrRows<double>{op.A->row.p, op.A->col.p, op.A->val.p, op.A->nnz}, x, epi, R, s);
+    apply_rows(op.n, op.lanes, op.grid, CsrRows<double>{op.A->row.p, op.A->col.p, op.A->val.p, op.A->nnz}, x, epi,
+               R, s);
   }
 }
 
@@ -589,9 +596,10 @@ int pick_lanes(int32_t n, int64_t nnz) {
   return g;
 }
 
-int spmv_blocks(int32_t n, int lanes) {
+int spmv_blocks(int32_t n, int lanes, int grid_cap) {
   const int rpb = kBlock / lanes;
-  return (int)((n + rpb - 1) / rpb);
+  const int64_t groups = ((int64_t)n + rpb - 1) / rpb;
+  return (int)(grid_cap > 0 && groups > grid_cap ? grid_cap : groups);
 }
 
 void launch_spmv(const Operator &op, const double *x, double *y, hipStream_t s) {
@@ -600,13 +608,13 @@ void launch_spmv(const Operator &op, const double *x, double *y, hipStream_t s) 
 
 void launch_slot_apply(int32_t n, const SlotRows &rows, const double *x, double *y, const double *dot_with,
                        double *dot_out, Reduce &R, hipStream_t s) {
-  apply_rows(n, 8, rows, x, EpiStoreDot{y, dot_with, dot_out}, &R, s);
+  apply_rows(n, 8, kGridCap, rows, x, EpiStoreDot{y, dot_with, dot_out}, &R, s);
 }
 
 void launch_ed_apply(int32_t n, const EdRows &rows, const double *x, double *y, const double *dot_with,
                      double *dot_out, Reduce &R, hipStream_t s) {
   // an on-site row has at most d entries: one lane per row
-  apply_rows(n, 1, rows, x, EpiStoreDot{y, dot_with, dot_out}, &R, s);
+  apply_rows(n, 1, kGridCap, rows, x, EpiStoreDot{y, dot_with, dot_out}, &R, s);
 }
 
 void launch_norm2(int64_t n, const double *x, Reduce &R, double *out, hipStream_t s) {
