@@ -52,17 +52,18 @@ def gpu_solve(D, sb, params, sched=None):
 
 
 def autotune(D, sb):
-    """SpMV schedule (lanes per row, workgroup cap) for this workload, picked on the device by
-    dmrgx_matrix_autotune; the superblocks of one DMRG run share their row statistics."""
+    """SpMV schedule (lanes per row, workgroup cap, non-temporal matrix loads) for this
+    workload, picked on the device by dmrgx_matrix_autotune; the superblocks of one DMRG run
+    share their row statistics."""
     L = D.lib()
-    L.dmrgx_matrix_autotune.argtypes = [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
-                                        C.POINTER(C.c_double)]
-    L.dmrgx_matrix_set_schedule.argtypes = [C.c_void_p, C.c_int32, C.c_int32]
+    i32p = C.POINTER(C.c_int32)
+    L.dmrgx_matrix_autotune.argtypes = [C.c_void_p, i32p, i32p, i32p, C.POINTER(C.c_double)]
+    L.dmrgx_matrix_set_schedule.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_int32]
     A = sb.assemble(full=True)
-    g, cap, us = C.c_int32(), C.c_int32(), C.c_double()
-    D.check(L.dmrgx_matrix_autotune(A.handle, C.byref(g), C.byref(cap), C.byref(us)))
+    g, cap, nt, us = C.c_int32(), C.c_int32(), C.c_int32(), C.c_double()
+    D.check(L.dmrgx_matrix_autotune(A.handle, C.byref(g), C.byref(cap), C.byref(nt), C.byref(us)))
     A.free()
-    return (g.value, cap.value), us.value
+    return (g.value, cap.value, nt.value), us.value
 
 
 def cpu_baseline(snap, e_ref, budget_s):
@@ -227,6 +228,80 @@ def run_ed_batch(D, count, precision_forms=(0, 1)):
     return res
 
 
+# ------------------------------------------------------------------ served stream
+SERVE_SNAPSHOTS = ["xxz_L12_m24_jz05_s13.npz", "hubbard_L8_m32_s07.npz", "klm_L6_m32_s04.npz",
+                   "klm_L6_m24_ixy_s04.npz", "hubbard_L64_m400_s31.npz"]
+
+
+def run_serve(D, rate, duration, seed=0):
+    """Poisson arrivals (rate/s, seeded) for `duration` s: 70 % ED (XXZ L 10..20, Hubbard
+    L 6..10, random J_z, h), 30 % superblock snapshots of the reference's runs; each request
+    is assembled on the device when it arrives and joins the running engine (continuous
+    batching).  Latency = completion - scheduled arrival."""
+    from dmrg_c_amd import ed, engine, snapshot
+    rng = np.random.default_rng(seed)
+    gaps = rng.exponential(1.0 / rate, size=int(rate * duration * 2) + 10)
+    arrivals = np.cumsum(gaps)
+    arrivals = arrivals[arrivals < duration]
+    sbs, refs = [], []
+    for name in SERVE_SNAPSHOTS:
+        snap = snapshot.load(os.path.join(REPO, "tests", "golden", name))
+        sbs.append(D.Superblock(snap))
+        refs.append(float(snap.meta["gs_energy"][0]))
+    reqs = []
+    for _ in arrivals:
+        if rng.random() < 0.7:
+            if rng.random() < 0.5:
+                L = int(rng.integers(10, 21))
+                reqs.append(("ed", ed.Model("xxz", L=L, sz=L % 2, J_z=float(rng.uniform(0.5, 1.5)),
+                                            h_z=float(rng.uniform(0, 0.5)))))
+            else:
+                L = int(rng.integers(6, 11))
+                reqs.append(("ed", ed.Model("hubbard", L=L, N=L, sz=L % 2, t=-1.0, U=4.0,
+                                            h_z=float(rng.uniform(0, 0.5)))))
+        else:
+            reqs.append(("sb", int(rng.integers(0, len(sbs)))))
+    E = engine.Engine(chunk_ticks=8)
+    p = engine.default_params()
+    t0 = time.perf_counter()
+    i, done = 0, {}
+    errs = []
+    while len(done) < len(reqs):
+        now = time.perf_counter() - t0
+        while i < len(reqs) and arrivals[i] <= now:
+            kind, what = reqs[i]
+            if kind == "ed":
+                P = ed.EdProblem.from_model(what)
+                A = P.assemble()
+                P.free()
+            else:
+                A = sbs[what].assemble(full=True)
+            E.submit(A, p, tag=i, own=True)
+            i += 1
+        if E.pending > 0:
+            E.run(1)
+        elif i < len(reqs):
+            time.sleep(max(0.0, arrivals[i] - (time.perf_counter() - t0)))
+        for r in E.poll(want_vec=False):
+            done[r.tag] = r.t_done - t0 - arrivals[r.tag]
+            kind, what = reqs[r.tag]
+            if kind == "sb":
+                errs.append(abs(r.energy - refs[what]) / abs(refs[what]))
+    wall = time.perf_counter() - t0
+    E.free()
+    for sb in sbs:
+        sb.free()
+    lat = np.array([done[k] for k in sorted(done)])
+    n_sb = sum(1 for k, _ in reqs if k == "sb")
+    return {"requests": len(reqs), "rate_per_s": rate, "duration_s": duration, "ed_requests": len(reqs) - n_sb,
+            "superblock_requests": n_sb, "solves_per_s": round(len(reqs) / wall, 2),
+            "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 2),
+            "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 2),
+            "max_rel_energy_error_superblocks": float(max(errs)) if errs else None,
+            "note": "superblock requests drawn from the reference-run snapshots in tests/golden "
+                    "(n up to 144542), assembled on the device at arrival"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -237,9 +312,12 @@ def main():
     ap.add_argument("--traffic-probe", action="store_true",
                     help="run 5 plain SpMV launches on the workload and one copy kernel of 2^26 doubles "
                          "(traffic calibration under rocprofv3 --pmc), then exit")
-    ap.add_argument("--workload", choices=["superblock", "ed-batch"], default="superblock",
+    ap.add_argument("--workload", choices=["superblock", "ed-batch", "serve"], default="superblock",
                     help="superblock: the headline line; ed-batch: 256 seeded ED solves one by one and "
-                         "through the batched engine (a separate JSON line)")
+                         "through the batched engine; serve: Poisson request stream through the engine "
+                         "(each a separate JSON line)")
+    ap.add_argument("--rate", type=float, default=200.0)
+    ap.add_argument("--duration", type=float, default=10.0)
     ap.add_argument("--ed-count", type=int, default=256)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -256,6 +334,10 @@ def main():
         print(json.dumps({"metric": "batched ED ground-state solves/s", "workload": "ed-batch",
                           "result": run_ed_batch(D, args.ed_count)}), flush=True)
         return
+    if args.workload == "serve":
+        print(json.dumps({"metric": "served solves/s and p50/p99 latency", "workload": "serve",
+                          "result": run_serve(D, args.rate, args.duration)}), flush=True)
+        return
     snap = snapshot.load(WORKLOAD)
     e_ref = float(snap.meta["gs_energy"][0])
     params = dict(diag_acc=1e-14, diag_max_step=800, cg_acc=1e-7, cg_max_step=2000, ii_acc=1e-9,
@@ -266,7 +348,7 @@ def main():
         return
     log(f"workload: Hubbard L=64 m=400 superblock n={snap.n}")
     lanes, tuned_us = autotune(D, sb)
-    log(f"autotune: (lanes per row, workgroup cap) = {lanes}, {tuned_us:.1f} us per SpMV")
+    log(f"autotune: (lanes per row, workgroup cap, nt loads) = {lanes}, {tuned_us:.1f} us per SpMV + dot")
     for i in range(args.warmup):
         E, st, nnz = gpu_solve(D, sb, params, lanes)
         log(f"warmup {i}: E={E:.15f} lanczos={st.lanczos_steps} cg={st.cg_steps}")
@@ -326,7 +408,8 @@ def main():
             "max_rel_energy_error": float(max(errs)),
             "lanczos_steps": int(np.median(lan)),
             "cg_steps": int(np.median(cg)),
-            "spmv_schedule": {"lanes_per_row": lanes[0], "workgroup_cap": lanes[1]},
+            "spmv_schedule": {"lanes_per_row": lanes[0], "workgroup_cap": lanes[1],
+                              "nontemporal_matrix_loads": bool(lanes[2])},
         },
         "forms": {
             "fp32_refined": {"solves_per_s": round(len(mlat) / sum(mlat), 4),

@@ -133,6 +133,9 @@ def lib() -> C.CDLL:
     L.dmrgx_spmv_host.argtypes = [vp, vp, vp]
     L.dmrgx_default_params.argtypes = [C.POINTER(SolveParams)]
     L.dmrgx_ground_state.argtypes = [vp, C.POINTER(SolveParams), vp, C.POINTER(SolveStats)]
+    L.dmrgx_matrix_set_schedule.argtypes = [vp, i32, i32, i32]
+    L.dmrgx_matrix_get_schedule.argtypes = [vp, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32)]
+    L.dmrgx_matrix_autotune.argtypes = [vp, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32), C.POINTER(f64)]
     L.MATRIX_VECTOR_PRODUCT_SYM.argtypes = [C.POINTER(CRS1), vp, vp, vp, C.c_int]
     L.MATRIX_VECTOR_PRODUCT.argtypes = [C.POINTER(CRS1), vp, vp, C.c_int]
     for nm in ("LANCZOS_SLOW_SYM", "LANCZOS_SYM", "LANCZOS_SLOW", "LANCZOS"):
@@ -213,6 +216,23 @@ class DeviceMatrix:
         y = np.empty(self.n)
         check(lib().dmrgx_spmv_host(self.handle, x.ctypes.data, y.ctypes.data))
         return y
+
+    @property
+    def schedule(self) -> Tuple[int, int, int]:
+        """(lanes per row, workgroup cap, non-temporal matrix loads) of this matrix's SpMV."""
+        g, cap, nt = C.c_int32(), C.c_int32(), C.c_int32()
+        check(lib().dmrgx_matrix_get_schedule(self.handle, C.byref(g), C.byref(cap), C.byref(nt)))
+        return g.value, cap.value, nt.value
+
+    def set_schedule(self, lanes: int, grid_cap: int, nt_loads: int = 0) -> None:
+        check(lib().dmrgx_matrix_set_schedule(self.handle, lanes, grid_cap, nt_loads))
+
+    def autotune(self) -> Tuple[Tuple[int, int, int], float]:
+        """Time the schedule candidates on the device, keep the fastest; returns it and its
+        time per product (microseconds)."""
+        g, cap, nt, us = C.c_int32(), C.c_int32(), C.c_int32(), C.c_double()
+        check(lib().dmrgx_matrix_autotune(self.handle, C.byref(g), C.byref(cap), C.byref(nt), C.byref(us)))
+        return (g.value, cap.value, nt.value), us.value
 
     def ground_state(self, start: Optional[np.ndarray] = None, **kw):
         p = SolveParams()

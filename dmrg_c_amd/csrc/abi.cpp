@@ -399,25 +399,29 @@ int dmrgx_spmv_device(const dmrgx_matrix *A, const double *x_dev, double *y_dev)
   return run_status([&] { launch_spmv(op_of(*A), x_dev, y_dev, ctx().stream); });
 }
 
-int dmrgx_matrix_set_schedule(dmrgx_matrix *A, int32_t lanes, int32_t grid_cap) {
-  if (!A || lanes < 1 || lanes > 64 || (lanes & (lanes - 1)) || grid_cap < 0) return DMRGX_ERR_ARG;
+int dmrgx_matrix_set_schedule(dmrgx_matrix *A, int32_t lanes, int32_t grid_cap, int32_t nt_loads) {
+  if (!A || lanes < 1 || lanes > 64 || (lanes & (lanes - 1)) || grid_cap < 0 || nt_loads < 0 || nt_loads > 1)
+    return DMRGX_ERR_ARG;
   std::lock_guard<std::recursive_mutex> lk(api_mutex());
   A->lanes = lanes;
   A->grid = grid_cap;
+  A->nt = nt_loads;
   return DMRGX_OK;
 }
 
-int dmrgx_matrix_get_schedule(const dmrgx_matrix *A, int32_t *lanes, int32_t *grid_cap) {
+int dmrgx_matrix_get_schedule(const dmrgx_matrix *A, int32_t *lanes, int32_t *grid_cap, int32_t *nt_loads) {
   if (!A) return DMRGX_ERR_ARG;
   if (lanes) *lanes = A->lanes;
   if (grid_cap) *grid_cap = A->grid;
+  if (nt_loads) *nt_loads = A->nt;
   return DMRGX_OK;
 }
 
-// SpMV schedule autotuner: times y = A x (HIP events on the library stream) for each
-// power-of-two lanes-per-row count around the mean row length and each workgroup cap, and
-// keeps the fastest pair.
-int dmrgx_matrix_autotune(dmrgx_matrix *A, int32_t *lanes_out, int32_t *grid_out, double *best_us_out) {
+// SpMV schedule autotuner: times y = A x with a fused dot product (HIP events on the library
+// stream) for each power-of-two lanes-per-row count around the mean row length, each
+// workgroup cap and both load policies, and keeps the fastest.
+int dmrgx_matrix_autotune(dmrgx_matrix *A, int32_t *lanes_out, int32_t *grid_out, int32_t *nt_out,
+                          double *best_us_out) {
   if (!A) return DMRGX_ERR_ARG;
   return run_status([&] {
     hipStream_t st = ctx().stream;
@@ -427,24 +431,31 @@ int dmrgx_matrix_autotune(dmrgx_matrix *A, int32_t *lanes_out, int32_t *grid_out
     hipEvent_t e0, e1;
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
-    int best_g = A->lanes, best_c = A->grid;
+    int best_g = A->lanes, best_c = A->grid, best_nt = A->nt;
     float best_ms = 1e30f;
     const int caps[] = {1024, 2048, 4096, 8192, 16384, 0};
-    for (int g = std::max(1, base / 4); g <= std::min(64, base * 2); g *= 2) {
-      for (int cap : caps) {
-        A->lanes = g;
-        A->grid = cap;
-        launch_spmv(op_of(*A), x.p, y.p, st);  // warm
-        HIPCHK(hipEventRecord(e0, st));
-        for (int r = 0; r < 3; r++) launch_spmv(op_of(*A), x.p, y.p, st);
-        HIPCHK(hipEventRecord(e1, st));
-        HIPCHK(hipEventSynchronize(e1));
-        float ms = 0;
-        HIPCHK(hipEventElapsedTime(&ms, e0, e1));
-        if (ms < best_ms) {
-          best_ms = ms;
-          best_g = g;
-          best_c = cap;
+    Reduce R;
+    DevBuf<double> dot(1);
+    // timed as the solvers use it: product, fused epilogue, block partials and finish kernel
+    for (int nt = 0; nt < 2; nt++) {
+      for (int g = std::max(1, base / 4); g <= std::min(64, base * 2); g *= 2) {
+        for (int cap : caps) {
+          A->lanes = g;
+          A->grid = cap;
+          A->nt = nt;
+          launch_spmv_dot(op_of(*A), x.p, y.p, dot.p, R, st);  // warm
+          HIPCHK(hipEventRecord(e0, st));
+          for (int r = 0; r < 3; r++) launch_spmv_dot(op_of(*A), x.p, y.p, dot.p, R, st);
+          HIPCHK(hipEventRecord(e1, st));
+          HIPCHK(hipEventSynchronize(e1));
+          float ms = 0;
+          HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+          if (ms < best_ms) {
+            best_ms = ms;
+            best_g = g;
+            best_c = cap;
+            best_nt = nt;
+          }
         }
       }
     }
@@ -452,8 +463,10 @@ int dmrgx_matrix_autotune(dmrgx_matrix *A, int32_t *lanes_out, int32_t *grid_out
     HIPCHK(hipEventDestroy(e1));
     A->lanes = best_g;
     A->grid = best_c;
+    A->nt = best_nt;
     if (lanes_out) *lanes_out = best_g;
     if (grid_out) *grid_out = best_c;
+    if (nt_out) *nt_out = best_nt;
     if (best_us_out) *best_us_out = best_ms * 1000.0 / 3.0;
   });
 }

@@ -132,7 +132,8 @@ __device__ __forceinline__ unsigned lanes_below(uint64_t mask) {
 
 // One wavefront (one 64-thread workgroup) per superblock row.  CAP: LDS list capacity.
 // FILL = false: counts[r] = kept entries; rows whose list exceeds CAP are appended to
-// overflow (count pass) and redone by the larger variant.
+// overflow (count pass) and redone by the next larger variant; FILL = true writes the rows
+// whose lists fit and skips the others.
 template <int CAP, bool FILL>
 __global__ __launch_bounds__(64) void k_assemble(SbView sb, const int32_t *row_list, int32_t n_list,
                                                  int64_t *counts, const int64_t *row_ptr, int32_t *col_out,
@@ -268,8 +269,12 @@ __global__ __launch_bounds__(256) void k_inv_scatter(SbView sb, int32_t *inv) {
   inv[sb_index(sb.LL[r], sb.LR[r], sb.RR[r], sb.RL[r], sb.dim_RR, sb.d)] = r;
 }
 
-constexpr int kCapSmall = 1024;  // 16 KiB of LDS per row
-constexpr int kCapLarge = 4096;  // 64 KiB of LDS per row
+// List capacities of the three k_assemble variants: most rows fit the small one, whose 4 KiB
+// of LDS lets a CU hold as many rows as it has wave slots; longer rows are redone by the
+// larger variants.
+constexpr int kCapSmall = 256;   // 4 KiB of LDS per row
+constexpr int kCapMid = 1024;    // 16 KiB
+constexpr int kCapLarge = 4096;  // 64 KiB
 
 // ------------------------------------------------------------------ host
 int op_role_dim(int slot, const dmrgx_superblock &S) {
@@ -422,21 +427,32 @@ static int64_t assemble_lower(dmrgx_superblock &S) {
   const int32_t n = S.n;
   const SbView v = S.view();
   DevBuf<int64_t> counts(n);
-  DevBuf<int32_t> over(n), n_over(2);
-  HIPCHK(hipMemsetAsync(n_over.p, 0, 2 * sizeof(int32_t), st));
-  k_assemble<kCapSmall, false><<<n, 64, 0, st>>>(v, nullptr, n, counts.p, nullptr, nullptr, nullptr, over.p, n_over.p);
+  DevBuf<int32_t> over1(n), over2(n), n_over(3);
+  HIPCHK(hipMemsetAsync(n_over.p, 0, 3 * sizeof(int32_t), st));
+  // count pass: small lists for every row, then the rows that overflowed with larger lists
+  k_assemble<kCapSmall, false><<<n, 64, 0, st>>>(v, nullptr, n, counts.p, nullptr, nullptr, nullptr, over1.p,
+                                                 n_over.p);
   HIPCHK(hipGetLastError());
-  int h[2];
+  int h[3] = {0, 0, 0};
   HIPCHK(hipMemcpyAsync(&h[0], n_over.p, sizeof(int), hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
-  const int n_overflow = h[0];
-  if (n_overflow > 0) {
-    k_assemble<kCapLarge, false><<<n_overflow, 64, 0, st>>>(v, over.p, n_overflow, counts.p, nullptr, nullptr,
-                                                            nullptr, nullptr, n_over.p + 1);
+  const int n_mid = h[0];
+  int n_large = 0;
+  if (n_mid > 0) {
+    k_assemble<kCapMid, false><<<n_mid, 64, 0, st>>>(v, over1.p, n_mid, counts.p, nullptr, nullptr, nullptr,
+                                                     over2.p, n_over.p + 1);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(&h[1], n_over.p + 1, sizeof(int), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
-    if (h[1] > 0)
+    n_large = h[1];
+  }
+  if (n_large > 0) {
+    k_assemble<kCapLarge, false><<<n_large, 64, 0, st>>>(v, over2.p, n_large, counts.p, nullptr, nullptr,
+                                                         nullptr, nullptr, n_over.p + 2);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(&h[2], n_over.p + 2, sizeof(int), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    if (h[2] > 0)
       throw Failure(DMRGX_ERR_ARG, "a superblock row has more than 4096 lower-triangle contributions");
   }
   S.lrow.resize((size_t)n + 1);
@@ -446,10 +462,14 @@ static int64_t assemble_lower(dmrgx_superblock &S) {
   HIPCHK(hipStreamSynchronize(st));
   S.lcol.resize(std::max<int64_t>(nnz, 1));
   S.lval.resize(std::max<int64_t>(nnz, 1));
+  // fill pass: each variant writes the rows whose lists fit it and skips the others
   k_assemble<kCapSmall, true><<<n, 64, 0, st>>>(v, nullptr, n, nullptr, S.lrow.p, S.lcol.p, S.lval.p, nullptr, nullptr);
-  if (n_overflow > 0)
-    k_assemble<kCapLarge, true><<<n_overflow, 64, 0, st>>>(v, over.p, n_overflow, nullptr, S.lrow.p, S.lcol.p,
-                                                           S.lval.p, nullptr, nullptr);
+  if (n_mid > 0)
+    k_assemble<kCapMid, true><<<n_mid, 64, 0, st>>>(v, over1.p, n_mid, nullptr, S.lrow.p, S.lcol.p, S.lval.p,
+                                                    nullptr, nullptr);
+  if (n_large > 0)
+    k_assemble<kCapLarge, true><<<n_large, 64, 0, st>>>(v, over2.p, n_large, nullptr, S.lrow.p, S.lcol.p,
+                                                        S.lval.p, nullptr, nullptr);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(st));
   S.lnnz = nnz;

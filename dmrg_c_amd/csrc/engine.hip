@@ -34,6 +34,7 @@
 #include <map>
 
 #include "entry.h"
+#include "reduce.h"
 
 namespace dmrgx {
 namespace {
@@ -48,7 +49,7 @@ struct SegDesc {
   const float *val32;
   int32_t n, fp32;
   int32_t blk_lo, nblk;   // this slot's blocks in the launch
-  int32_t k, pad;         // step (Lanczos) or iteration (CG)
+  int32_t k, nt;          // step (Lanczos) or iteration (CG); non-temporal matrix loads
   double shift, acc;
   const double *x;        // LANCZOS_A: w      CG_AP: p
   double *o1, *o2;        // LANCZOS_A: v_k, y CG_AP: y
@@ -59,31 +60,6 @@ struct SegDesc {
   int *status;            // CG: {done, last iteration}
 };
 
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-  return v;
-}
-
-__device__ __forceinline__ double block_sum(double v) {
-  __shared__ double lds[kBlock / 64];
-  v = wave_sum(v);
-  const int w = threadIdx.x >> 6;
-  if ((threadIdx.x & 63) == 0) lds[w] = v;
-  __syncthreads();
-  double t = 0.0;
-  if (threadIdx.x == 0) {
-#pragma unroll
-    for (int i = 0; i < kBlock / 64; i++) t += lds[i];
-  }
-  return t;
-}
-
-__device__ __forceinline__ void block_partial(double v, double *partial) {
-  const double t = block_sum(v);
-  if (threadIdx.x == 0) partial[blockIdx.x] = t;
-}
-
 enum FinKind { FIN_LANCZOS_A = 0, FIN_CG_AP = 1, FIN_LANCZOS_B = 2, FIN_CG_UPDATE = 3 };
 
 // One block per segment: its partials summed in block order (the single-solve k_finish
@@ -92,9 +68,7 @@ template <int KIND>
 __global__ __launch_bounds__(kBlock) void k_seg_finish(const SegDesc *desc, const double *partial) {
   const SegDesc d = desc[blockIdx.x];
   if ((KIND == FIN_CG_AP || KIND == FIN_CG_UPDATE) && d.status[0] != 0) return;
-  double s = 0.0;
-  for (int i = threadIdx.x; i < d.nblk; i += kBlock) s += partial[d.blk_lo + i];
-  const double t = block_sum(s);
+  const double t = finish_sum(partial + d.blk_lo, d.nblk);
   if (threadIdx.x != 0) return;
   if (KIND == FIN_LANCZOS_A) {
     *d.s2 = t;                 // alpha_k
@@ -125,8 +99,13 @@ __global__ __launch_bounds__(kBlock) void k_seg_spmv(const SegDesc *desc, const 
     const int32_t r = r0 + g;
     double s = 0.0;
     if (r < d.n) {
-      if (d.fp32) s = csr_lane_sum<G>(d.row, d.col, d.val32, r, lane, d.x);
-      else s = csr_lane_sum<G>(d.row, d.col, d.val, r, lane, d.x);
+      if (d.nt) {
+        if (d.fp32) s = csr_lane_sum<G, true>(d.row, d.col, d.val32, r, lane, d.x);
+        else s = csr_lane_sum<G, true>(d.row, d.col, d.val, r, lane, d.x);
+      } else {
+        if (d.fp32) s = csr_lane_sum<G, false>(d.row, d.col, d.val32, r, lane, d.x);
+        else s = csr_lane_sum<G, false>(d.row, d.col, d.val, r, lane, d.x);
+      }
     }
 #pragma unroll
     for (int off = G / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, G);
@@ -543,6 +522,7 @@ void run_chunk(dmrgx_engine &E, int ticks) {
         a.val32 = s.A->val32.p;
         a.n = s.n;
         a.fp32 = s.mixed ? 1 : 0;
+        a.nt = s.A->nt;
         a.k = k;
         a.shift = 0.0;
         a.x = s.w.p;
@@ -583,6 +563,7 @@ void run_chunk(dmrgx_engine &E, int ticks) {
         a.val = s.A->val.p;
         a.n = s.n;
         a.k = it;
+        a.nt = s.A->nt;
         a.shift = s.p.ii_diag_add - s.eig;
         a.x = s.pv.p;
         a.o1 = s.cy.p;

@@ -104,6 +104,7 @@ struct dmrgx_matrix {
   dmrgx::DevBuf<float> val32;   // FP32 copy of val for the mixed-precision Lanczos
   int lanes = 8;                // lanes per row of the SpMV kernel
   int grid = dmrgx::kGridCap;   // cap on SpMV workgroups (0: one per row group)
+  int nt = 0;                   // 1: non-temporal loads of the matrix arrays
 };
 
 namespace dmrgx {
@@ -189,13 +190,21 @@ struct SlotRows {
   }
 };
 
+// Load of the streamed matrix arrays: NT = true marks them non-temporal (evict first), so
+// that x has the L2 to itself; which policy moves fewer bytes depends on the matrix, so the
+// schedule autotuner picks it.
+template <bool NT, class T>
+__device__ __forceinline__ T stream_load(const T *p) {
+  if constexpr (NT) return __builtin_nontemporal_load(p);
+  else return *p;
+}
+
 // This lane's share of sum_j H_rj x_j over CSR row r: entries b+lane, b+lane+G, ... summed in
 // that order.  Each pass issues the lane's next four (value, column) loads and their four
 // x gathers together; positions past the row end load the row's last entry again and are
-// masked out of the sum, so there is no branch between the loads.  The matrix streams past
-// the caches (non-temporal loads) so that x stays resident in L2.  The order of the
+// masked out of the sum, so there is no branch between the loads.  The order of the
 // additions is that of the plain loop.
-template <int G, class VT>
+template <int G, bool NT, class VT>
 __device__ __forceinline__ double csr_lane_sum(const int64_t *__restrict__ row, const int32_t *__restrict__ col,
                                                const VT *__restrict__ val, int32_t r, int lane,
                                                const double *__restrict__ x) {
@@ -205,14 +214,14 @@ __device__ __forceinline__ double csr_lane_sum(const int64_t *__restrict__ row, 
   for (int64_t j = row[r] + lane; j < e; j += 4 * G) {
     const bool p1 = j + G < e, p2 = j + 2 * G < e, p3 = j + 3 * G < e;
     const int64_t j1 = p1 ? j + G : last, j2 = p2 ? j + 2 * G : last, j3 = p3 ? j + 3 * G : last;
-    const VT v0 = __builtin_nontemporal_load(val + j);
-    const VT v1 = __builtin_nontemporal_load(val + j1);
-    const VT v2 = __builtin_nontemporal_load(val + j2);
-    const VT v3 = __builtin_nontemporal_load(val + j3);
-    const int32_t c0 = __builtin_nontemporal_load(col + j);
-    const int32_t c1 = __builtin_nontemporal_load(col + j1);
-    const int32_t c2 = __builtin_nontemporal_load(col + j2);
-    const int32_t c3 = __builtin_nontemporal_load(col + j3);
+    const VT v0 = stream_load<NT>(val + j);
+    const VT v1 = stream_load<NT>(val + j1);
+    const VT v2 = stream_load<NT>(val + j2);
+    const VT v3 = stream_load<NT>(val + j3);
+    const int32_t c0 = stream_load<NT>(col + j);
+    const int32_t c1 = stream_load<NT>(col + j1);
+    const int32_t c2 = stream_load<NT>(col + j2);
+    const int32_t c3 = stream_load<NT>(col + j3);
     const double x0 = x[c0], x1 = x[c1], x2 = x[c2], x3 = x[c3];
     s += (double)v0 * x0;
     if (p1) s += (double)v1 * x1;
@@ -293,11 +302,12 @@ struct Operator {
   int32_t n = 0;
   int lanes = 8;
   int grid = kGridCap;
+  int nt = 0;
   const dmrgx_matrix *A = nullptr;
   const ObcDev *obc = nullptr;
 };
-inline Operator op_of(const dmrgx_matrix &A) { return Operator{A.n, A.lanes, A.grid, &A, nullptr}; }
-inline Operator op_of(const ObcDev &O) { return Operator{O.n, O.lanes, kGridCap, nullptr, &O}; }
+inline Operator op_of(const dmrgx_matrix &A) { return Operator{A.n, A.lanes, A.grid, A.nt, &A, nullptr}; }
+inline Operator op_of(const ObcDev &O) { return Operator{O.n, O.lanes, kGridCap, 0, nullptr, &O}; }
 
 // ---- kernel launchers (kernels.hip)
 struct Reduce {              // deterministic two-kernel reduction workspace
@@ -309,6 +319,8 @@ int pick_lanes(int32_t n, int64_t nnz);
 int spmv_blocks(int32_t n, int lanes, int grid_cap);
 
 void launch_spmv(const Operator &op, const double *x, double *y, hipStream_t s);
+// y = A x and *dot = <x, y>: the shape of the solver's products (reduction + finish kernel)
+void launch_spmv_dot(const Operator &op, const double *x, double *y, double *dot, Reduce &R, hipStream_t s);
 void launch_expand_lower(int32_t n, int64_t nnz_tri, const int64_t *trow, const int32_t *tcol,
                          const double *tval, dmrgx_matrix &A, hipStream_t s);
 void launch_to_fp32(const dmrgx_matrix &A, hipStream_t s);

@@ -17,51 +17,22 @@
 #include <cmath>
 
 #include "internal.h"
+#include "reduce.h"
 
 namespace dmrgx {
 namespace {
-
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-  return v;
-}
-
-// Sum over the workgroup; the result is valid in thread 0.  Uses LDS, so callers that
-// reduce twice in one kernel must separate the calls with __syncthreads().
-__device__ __forceinline__ double block_sum(double v) {
-  __shared__ double lds[kBlock / 64];
-  v = wave_sum(v);
-  const int w = threadIdx.x >> 6;
-  if ((threadIdx.x & 63) == 0) lds[w] = v;
-  __syncthreads();
-  double t = 0.0;
-  if (threadIdx.x == 0) {
-#pragma unroll
-    for (int i = 0; i < kBlock / 64; i++) t += lds[i];
-  }
-  return t;
-}
-
-// This block's share of a reduction; k_finish sums the shares in block order.
-__device__ __forceinline__ void block_partial(double v, double *partial) {
-  const double t = block_sum(v);
-  if (threadIdx.x == 0) partial[blockIdx.x] = t;
-}
 
 // One block: sum of the nblk partials in index order, handed to f.finish().
 template <class Fin>
 __global__ __launch_bounds__(kBlock) void k_finish(Fin f, const double *partial, int nblk) {
   if (f.skip()) return;
-  double s = 0.0;
-  for (int i = threadIdx.x; i < nblk; i += kBlock) s += partial[i];
-  const double t = block_sum(s);
+  const double t = finish_sum(partial, nblk);
   if (threadIdx.x == 0) f.finish(t);
 }
 
 // ------------------------------------------------------------------ SpMV + epilogue
 // Row sources: each returns this lane's share of sum_j H_rj x_j for row r.
-template <class VT>
+template <class VT, bool NT = false>
 struct CsrRows {
   const int64_t *__restrict__ row;
   const int32_t *__restrict__ col;
@@ -74,7 +45,7 @@ struct CsrRows {
   }
   template <int G>
   __device__ __forceinline__ double lane_sum(int32_t r, int lane, const double *__restrict__ x) const {
-    return csr_lane_sum<G>(row, col, val, r, lane, x);
+    return csr_lane_sum<G, NT>(row, col, val, r, lane, x);
   }
 };
 
@@ -251,11 +222,19 @@ void apply_op(const Operator &op, bool fp32, const double *x, const Epi &epi, Re
     apply_rows(op.n, op.lanes, op.grid, op.obc->rows(), x, epi, R, s);
   } else if (fp32) {
     if (!op.A->val32.p) throw Failure(DMRGX_ERR_ARG, "FP32 values not prepared");
-    apply_rows(op.n, op.lanes, op.grid, CsrRows<float>{op.A->row.p, op.A->col.p, op.A->val32.p, op.A->nnz}, x, epi,
-               R, s);
+    if (op.nt)
+      apply_rows(op.n, op.lanes, op.grid, CsrRows<float, true>{op.A->row.p, op.A->col.p, op.A->val32.p, op.A->nnz},
+                 x, epi, R, s);
+    else
+      apply_rows(op.n, op.lanes, op.grid, CsrRows<float, false>{op.A->row.p, op.A->col.p, op.A->val32.p, op.A->nnz},
+                 x, epi, R, s);
   } else {
-    apply_rows(op.n, op.lanes, op.grid, CsrRows<double>{op.A->row.p, op.A->col.p, op.A->val.p, op.A->nnz}, x, epi,
-               R, s);
+    if (op.nt)
+      apply_rows(op.n, op.lanes, op.grid, CsrRows<double, true>{op.A->row.p, op.A->col.p, op.A->val.p, op.A->nnz},
+                 x, epi, R, s);
+    else
+      apply_rows(op.n, op.lanes, op.grid, CsrRows<double, false>{op.A->row.p, op.A->col.p, op.A->val.p, op.A->nnz},
+                 x, epi, R, s);
   }
 }
 
@@ -604,6 +583,10 @@ int spmv_blocks(int32_t n, int lanes, int grid_cap) {
 
 void launch_spmv(const Operator &op, const double *x, double *y, hipStream_t s) {
   apply_op(op, false, x, EpiPlain{y}, nullptr, s);
+}
+
+void launch_spmv_dot(const Operator &op, const double *x, double *y, double *dot, Reduce &R, hipStream_t s) {
+  apply_op(op, false, x, EpiStoreDot{y, x, dot}, &R, s);
 }
 
 void launch_slot_apply(int32_t n, const SlotRows &rows, const double *x, double *y, const double *dot_with,

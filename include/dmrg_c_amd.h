@@ -460,13 +460,16 @@ int dmrgx_matrix_download(const dmrgx_matrix *A, int64_t *row, int32_t *col, dou
 int dmrgx_spmv_device(const dmrgx_matrix *A, const double *x_dev, double *y_dev);
 /* y = A x with host pointers. */
 int dmrgx_spmv_host(const dmrgx_matrix *A, const double *x, double *y);
-/* SpMV schedule of this matrix: lanes (threads) per row, a power of two 1..64, and the cap on
- * workgroups (row groups are dealt round-robin; 0 = one workgroup per row group).  Defaults
- * come from the mean row length; dmrgx_matrix_autotune times the candidates on the device
- * and keeps the fastest.  Single and batched solves use the matrix's schedule. */
-int dmrgx_matrix_set_schedule(dmrgx_matrix *A, int32_t lanes, int32_t grid_cap);
-int dmrgx_matrix_get_schedule(const dmrgx_matrix *A, int32_t *lanes, int32_t *grid_cap);
-int dmrgx_matrix_autotune(dmrgx_matrix *A, int32_t *lanes_out, int32_t *grid_out, double *best_us_out);
+/* SpMV schedule of this matrix: lanes (threads) per row, a power of two 1..64; the cap on
+ * workgroups (row groups are dealt round-robin; 0 = one workgroup per row group); and the
+ * load policy of the matrix arrays (1 = non-temporal).  Defaults come from the mean row
+ * length; dmrgx_matrix_autotune times the candidates on the device (product + reduction, as
+ * the solvers run it) and keeps the fastest.  Single and batched solves use the matrix's
+ * schedule. */
+int dmrgx_matrix_set_schedule(dmrgx_matrix *A, int32_t lanes, int32_t grid_cap, int32_t nt_loads);
+int dmrgx_matrix_get_schedule(const dmrgx_matrix *A, int32_t *lanes, int32_t *grid_cap, int32_t *nt_loads);
+int dmrgx_matrix_autotune(dmrgx_matrix *A, int32_t *lanes_out, int32_t *grid_out, int32_t *nt_out,
+                          double *best_us_out);
 
 /* ---- ground-state solve (Lanczos -> inverse iteration with CG), all on the device */
 typedef struct {
