@@ -283,6 +283,8 @@ def run_serve(D, rate, duration, seed=0):
         elif i < len(reqs):
             time.sleep(max(0.0, arrivals[i] - (time.perf_counter() - t0)))
         for r in E.poll(want_vec=False):
+            if len(done) % 200 == 0:
+                log(f"serve: {len(done)} of {len(reqs)} done, {i} arrived, t={time.perf_counter() - t0:.1f} s")
             done[r.tag] = r.t_done - t0 - arrivals[r.tag]
             kind, what = reqs[r.tag]
             if kind == "sb":
