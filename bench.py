@@ -52,9 +52,9 @@ def gpu_solve(D, sb, params, sched=None):
 
 
 def autotune(D, sb):
-    """SpMV schedule (lanes per row, workgroup cap, non-temporal matrix loads) for this
-    workload, picked on the device by dmrgx_matrix_autotune; the superblocks of one DMRG run
-    share their row statistics."""
+    """SpMV schedule (lanes per row, workgroup cap, load variant) for this workload, picked on
+    the device by dmrgx_matrix_autotune; the superblocks of one DMRG run share their row
+    statistics."""
     L = D.lib()
     i32p = C.POINTER(C.c_int32)
     L.dmrgx_matrix_autotune.argtypes = [C.c_void_p, i32p, i32p, i32p, C.POINTER(C.c_double)]
@@ -350,7 +350,7 @@ def main():
         return
     log(f"workload: Hubbard L=64 m=400 superblock n={snap.n}")
     lanes, tuned_us = autotune(D, sb)
-    log(f"autotune: (lanes per row, workgroup cap, nt loads) = {lanes}, {tuned_us:.1f} us per SpMV + dot")
+    log(f"autotune: (lanes per row, workgroup cap, load variant) = {lanes}, {tuned_us:.1f} us per SpMV + dot")
     for i in range(args.warmup):
         E, st, nnz = gpu_solve(D, sb, params, lanes)
         log(f"warmup {i}: E={E:.15f} lanczos={st.lanczos_steps} cg={st.cg_steps}")
@@ -411,7 +411,9 @@ def main():
             "lanczos_steps": int(np.median(lan)),
             "cg_steps": int(np.median(cg)),
             "spmv_schedule": {"lanes_per_row": lanes[0], "workgroup_cap": lanes[1],
-                              "nontemporal_matrix_loads": bool(lanes[2])},
+                              "nontemporal_matrix_loads": bool(lanes[2] & 1),
+                              "loads_in_flight_per_lane": 8 if lanes[2] & 2 else 4,
+                              "xcd_contiguous_rows": bool(lanes[2] & 4)},
         },
         "forms": {
             "fp32_refined": {"solves_per_s": round(len(mlat) / sum(mlat), 4),

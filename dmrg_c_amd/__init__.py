@@ -219,13 +219,15 @@ class DeviceMatrix:
 
     @property
     def schedule(self) -> Tuple[int, int, int]:
-        """(lanes per row, workgroup cap, non-temporal matrix loads) of this matrix's SpMV."""
+        """(lanes per row, workgroup cap, load variant) of this matrix's SpMV; load variant
+        bit 0 = non-temporal matrix loads, bit 1 = eight loads in flight per lane, bit 2 =
+        contiguous row ranges per workgroup and per XCD."""
         g, cap, nt = C.c_int32(), C.c_int32(), C.c_int32()
         check(lib().dmrgx_matrix_get_schedule(self.handle, C.byref(g), C.byref(cap), C.byref(nt)))
         return g.value, cap.value, nt.value
 
-    def set_schedule(self, lanes: int, grid_cap: int, nt_loads: int = 0) -> None:
-        check(lib().dmrgx_matrix_set_schedule(self.handle, lanes, grid_cap, nt_loads))
+    def set_schedule(self, lanes: int, grid_cap: int, load_variant: int = 0) -> None:
+        check(lib().dmrgx_matrix_set_schedule(self.handle, lanes, grid_cap, load_variant))
 
     def autotune(self) -> Tuple[Tuple[int, int, int], float]:
         """Time the schedule candidates on the device, keep the fastest; returns it and its

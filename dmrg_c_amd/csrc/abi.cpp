@@ -365,6 +365,14 @@ void dmrgx_start_vector(int64_t n, uint64_t seed, double *out) {
   }
 }
 
+int dmrgx_tridiag_lowest(int32_t k, const double *a, const double *b, double *lam_out, double *vec_out) {
+  if (k < 1 || !a || (k > 1 && !b) || !lam_out) return DMRGX_ERR_ARG;
+  RitzTrack track;
+  for (int32_t j = 0; j < k; j++) lam_out[j] = track.next(j + 1, a, b);
+  if (vec_out) tridiag_vector(k, a, b, lam_out[k - 1], vec_out);
+  return DMRGX_OK;
+}
+
 int dmrgx_matrix_from_crs1(const CRS1 *M, int sym, dmrgx_matrix **out) {
   if (!M || !out) return DMRGX_ERR_ARG;
   return dmrgx_matrix_from_arrays(M->row_dim, M->Row, M->Col, M->Val, sym, out);
@@ -399,28 +407,29 @@ int dmrgx_spmv_device(const dmrgx_matrix *A, const double *x_dev, double *y_dev)
   return run_status([&] { launch_spmv(op_of(*A), x_dev, y_dev, ctx().stream); });
 }
 
-int dmrgx_matrix_set_schedule(dmrgx_matrix *A, int32_t lanes, int32_t grid_cap, int32_t nt_loads) {
-  if (!A || lanes < 1 || lanes > 64 || (lanes & (lanes - 1)) || grid_cap < 0 || nt_loads < 0 || nt_loads > 1)
+int dmrgx_matrix_set_schedule(dmrgx_matrix *A, int32_t lanes, int32_t grid_cap, int32_t load_variant) {
+  if (!A || lanes < 1 || lanes > 64 || (lanes & (lanes - 1)) || grid_cap < 0 || load_variant < 0 ||
+      load_variant >= kLoadVariants)
     return DMRGX_ERR_ARG;
   std::lock_guard<std::recursive_mutex> lk(api_mutex());
   A->lanes = lanes;
   A->grid = grid_cap;
-  A->nt = nt_loads;
+  A->load = load_variant;
   return DMRGX_OK;
 }
 
-int dmrgx_matrix_get_schedule(const dmrgx_matrix *A, int32_t *lanes, int32_t *grid_cap, int32_t *nt_loads) {
+int dmrgx_matrix_get_schedule(const dmrgx_matrix *A, int32_t *lanes, int32_t *grid_cap, int32_t *load_variant) {
   if (!A) return DMRGX_ERR_ARG;
   if (lanes) *lanes = A->lanes;
   if (grid_cap) *grid_cap = A->grid;
-  if (nt_loads) *nt_loads = A->nt;
+  if (load_variant) *load_variant = A->load;
   return DMRGX_OK;
 }
 
 // SpMV schedule autotuner: times y = A x with a fused dot product (HIP events on the library
 // stream) for each power-of-two lanes-per-row count around the mean row length, each
-// workgroup cap and both load policies, and keeps the fastest.
-int dmrgx_matrix_autotune(dmrgx_matrix *A, int32_t *lanes_out, int32_t *grid_out, int32_t *nt_out,
+// workgroup cap and each load variant, and keeps the fastest.
+int dmrgx_matrix_autotune(dmrgx_matrix *A, int32_t *lanes_out, int32_t *grid_out, int32_t *load_out,
                           double *best_us_out) {
   if (!A) return DMRGX_ERR_ARG;
   return run_status([&] {
@@ -431,18 +440,18 @@ int dmrgx_matrix_autotune(dmrgx_matrix *A, int32_t *lanes_out, int32_t *grid_out
     hipEvent_t e0, e1;
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
-    int best_g = A->lanes, best_c = A->grid, best_nt = A->nt;
+    int best_g = A->lanes, best_c = A->grid, best_v = A->load;
     float best_ms = 1e30f;
     const int caps[] = {1024, 2048, 4096, 8192, 16384, 0};
     Reduce R;
     DevBuf<double> dot(1);
     // timed as the solvers use it: product, fused epilogue, block partials and finish kernel
-    for (int nt = 0; nt < 2; nt++) {
+    for (int v = 0; v < kLoadVariants; v++) {
       for (int g = std::max(1, base / 4); g <= std::min(64, base * 2); g *= 2) {
         for (int cap : caps) {
           A->lanes = g;
           A->grid = cap;
-          A->nt = nt;
+          A->load = v;
           launch_spmv_dot(op_of(*A), x.p, y.p, dot.p, R, st);  // warm
           HIPCHK(hipEventRecord(e0, st));
           for (int r = 0; r < 3; r++) launch_spmv_dot(op_of(*A), x.p, y.p, dot.p, R, st);
@@ -454,7 +463,7 @@ int dmrgx_matrix_autotune(dmrgx_matrix *A, int32_t *lanes_out, int32_t *grid_out
             best_ms = ms;
             best_g = g;
             best_c = cap;
-            best_nt = nt;
+            best_v = v;
           }
         }
       }
@@ -463,10 +472,10 @@ int dmrgx_matrix_autotune(dmrgx_matrix *A, int32_t *lanes_out, int32_t *grid_out
     HIPCHK(hipEventDestroy(e1));
     A->lanes = best_g;
     A->grid = best_c;
-    A->nt = best_nt;
+    A->load = best_v;
     if (lanes_out) *lanes_out = best_g;
     if (grid_out) *grid_out = best_c;
-    if (nt_out) *nt_out = best_nt;
+    if (load_out) *load_out = best_v;
     if (best_us_out) *best_us_out = best_ms * 1000.0 / 3.0;
   });
 }

@@ -49,7 +49,7 @@ struct SegDesc {
   const float *val32;
   int32_t n, fp32;
   int32_t blk_lo, nblk;   // this slot's blocks in the launch
-  int32_t k, nt;          // step (Lanczos) or iteration (CG); non-temporal matrix loads
+  int32_t k, load;        // step (Lanczos) or iteration (CG); SpMV load variant
   double shift, acc;
   const double *x;        // LANCZOS_A: w      CG_AP: p
   double *o1, *o2;        // LANCZOS_A: v_k, y CG_AP: y
@@ -94,17 +94,22 @@ __global__ __launch_bounds__(kBlock) void k_seg_spmv(const SegDesc *desc, const 
   const int lane = threadIdx.x % G;
   const double scale = (MODE == LANCZOS_A) ? 1.0 / sqrt(*d.s1) : 0.0;
   double c = 0.0;
-  // row groups round-robin over the slot's blocks, as k_apply deals them
-  for (int32_t r0 = (blockIdx.x - d.blk_lo) * RPB; r0 < d.n; r0 += d.nblk * RPB) {
-    const int32_t r = r0 + g;
+  // row groups over the slot's logical blocks, as k_apply deals them (the partial of
+  // logical block lb is at blk_lo + lb, where the finish kernel sums it in lb order)
+  const int32_t lb = blockIdx.x - d.blk_lo;
+  const GroupRange gr = group_range(lb, d.nblk, (d.n + RPB - 1) / RPB, (d.load & kLoadXcd) != 0);
+  for (int32_t gi = gr.start; gi < gr.end; gi += gr.step) {
+    const int32_t r = gi * RPB + g;
     double s = 0.0;
     if (r < d.n) {
-      if (d.nt) {
-        if (d.fp32) s = csr_lane_sum<G, true>(d.row, d.col, d.val32, r, lane, d.x);
-        else s = csr_lane_sum<G, true>(d.row, d.col, d.val, r, lane, d.x);
-      } else {
-        if (d.fp32) s = csr_lane_sum<G, false>(d.row, d.col, d.val32, r, lane, d.x);
-        else s = csr_lane_sum<G, false>(d.row, d.col, d.val, r, lane, d.x);
+      switch (d.load & (kLoadXcd - 1)) {
+#define V_CASE(V)                                                         \
+  case V:                                                                 \
+    if (d.fp32) s = csr_lane_sum_v<G, V>(d.row, d.col, d.val32, r, lane, d.x); \
+    else s = csr_lane_sum_v<G, V>(d.row, d.col, d.val, r, lane, d.x);       \
+    break;
+        V_CASE(0) V_CASE(1) V_CASE(2) V_CASE(3)
+#undef V_CASE
       }
     }
 #pragma unroll
@@ -220,6 +225,7 @@ struct Slot {
   int k_next = 0;        // next Lanczos step to compute
   int k_read = 0;        // steps whose scalars the host has checked
   double E = 1e15, lam = 0.0;
+  RitzTrack track;       // lowest Ritz values of the growing tridiagonal
   std::vector<double> ha, hb, s;
   // II / CG state
   double eig = 0.0, err = 1000.0;
@@ -415,7 +421,7 @@ void lanczos_check(Slot &s, const double *alpha_h, const double *beta_h) {
     if (k == 0) continue;  // step 0 is the start of the recurrence (solver.hip: block step 0)
     const bool check = small || k >= s.p.diag_min_step;
     if (!check) continue;
-    s.lam = tridiag_lowest_value(k + 1, s.ha.data(), s.hb.data() + 1, std::numeric_limits<double>::infinity());
+    s.lam = s.track.next(k + 1, s.ha.data(), s.hb.data() + 1);
     bool conv;
     const double bnext = s.hb[k + 1];
     const double tscale = std::max(1.0, std::fabs(s.lam));
@@ -522,7 +528,7 @@ void run_chunk(dmrgx_engine &E, int ticks) {
         a.val32 = s.A->val32.p;
         a.n = s.n;
         a.fp32 = s.mixed ? 1 : 0;
-        a.nt = s.A->nt;
+        a.load = s.A->load;
         a.k = k;
         a.shift = 0.0;
         a.x = s.w.p;
@@ -563,7 +569,7 @@ void run_chunk(dmrgx_engine &E, int ticks) {
         a.val = s.A->val.p;
         a.n = s.n;
         a.k = it;
-        a.nt = s.A->nt;
+        a.load = s.A->load;
         a.shift = s.p.ii_diag_add - s.eig;
         a.x = s.pv.p;
         a.o1 = s.cy.p;

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <limits>
 #include <cstdio>
 #include <memory>
 #include <stdexcept>
@@ -104,7 +105,7 @@ struct dmrgx_matrix {
   dmrgx::DevBuf<float> val32;   // FP32 copy of val for the mixed-precision Lanczos
   int lanes = 8;                // lanes per row of the SpMV kernel
   int grid = dmrgx::kGridCap;   // cap on SpMV workgroups (0: one per row group)
-  int nt = 0;                   // 1: non-temporal loads of the matrix arrays
+  int load = 0;                 // load variant (kLoadNT | kLoadDeep bits)
 };
 
 namespace dmrgx {
@@ -200,35 +201,75 @@ __device__ __forceinline__ T stream_load(const T *p) {
 }
 
 // This lane's share of sum_j H_rj x_j over CSR row r: entries b+lane, b+lane+G, ... summed in
-// that order.  Each pass issues the lane's next four (value, column) loads and their four
-// x gathers together; positions past the row end load the row's last entry again and are
+// that order.  Each pass issues the lane's next U (value, column) loads and their U x
+// gathers together; positions past the row end load the row's last entry again and are
 // masked out of the sum, so there is no branch between the loads.  The order of the
-// additions is that of the plain loop.
-template <int G, bool NT, class VT>
+// additions is that of the plain loop whatever U and the load policy are, so the schedule
+// never changes the result.
+template <int G, int U, bool NT, class VT>
 __device__ __forceinline__ double csr_lane_sum(const int64_t *__restrict__ row, const int32_t *__restrict__ col,
                                                const VT *__restrict__ val, int32_t r, int lane,
                                                const double *__restrict__ x) {
   double s = 0.0;
   const int64_t e = row[r + 1];
   const int64_t last = e - 1;
-  for (int64_t j = row[r] + lane; j < e; j += 4 * G) {
-    const bool p1 = j + G < e, p2 = j + 2 * G < e, p3 = j + 3 * G < e;
-    const int64_t j1 = p1 ? j + G : last, j2 = p2 ? j + 2 * G : last, j3 = p3 ? j + 3 * G : last;
-    const VT v0 = stream_load<NT>(val + j);
-    const VT v1 = stream_load<NT>(val + j1);
-    const VT v2 = stream_load<NT>(val + j2);
-    const VT v3 = stream_load<NT>(val + j3);
-    const int32_t c0 = stream_load<NT>(col + j);
-    const int32_t c1 = stream_load<NT>(col + j1);
-    const int32_t c2 = stream_load<NT>(col + j2);
-    const int32_t c3 = stream_load<NT>(col + j3);
-    const double x0 = x[c0], x1 = x[c1], x2 = x[c2], x3 = x[c3];
-    s += (double)v0 * x0;
-    if (p1) s += (double)v1 * x1;
-    if (p2) s += (double)v2 * x2;
-    if (p3) s += (double)v3 * x3;
+  for (int64_t j = row[r] + lane; j < e; j += U * G) {
+    VT v[U];
+    int32_t c[U];
+    bool p[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      p[u] = j + u * G < e;
+      const int64_t ju = p[u] ? j + u * G : last;
+      v[u] = stream_load<NT>(val + ju);
+      c[u] = stream_load<NT>(col + ju);
+    }
+    double xv[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) xv[u] = x[c[u]];
+#pragma unroll
+    for (int u = 0; u < U; u++)
+      if (p[u]) s += (double)v[u] * xv[u];
   }
   return s;
+}
+
+// Variants of the CSR product.  Bit 0: non-temporal matrix loads; bit 1: eight loads in
+// flight per lane instead of four (neither changes a result).  Bit 2: row groups in
+// contiguous ranges per block, blocks mapped so that each XCD takes a contiguous share of
+// the rows (with a capped grid this changes the order of the fused dot products).
+constexpr int kLoadNT = 1;
+constexpr int kLoadDeep = 2;
+constexpr int kLoadXcd = 4;
+constexpr int kLoadVariants = 8;
+
+// Row groups [start, end) with stride step that logical block lb of nblk takes out of
+// ngroups: round-robin (lb, lb + nblk, ...) or one contiguous range.
+struct GroupRange {
+  int32_t start, end, step;
+};
+__device__ __forceinline__ GroupRange group_range(int32_t lb, int32_t nblk, int32_t ngroups, bool contiguous) {
+  if (!contiguous) return GroupRange{lb, ngroups, nblk};
+  const int32_t per = ngroups / nblk, rem = ngroups % nblk;
+  const int32_t s = lb * per + min(lb, rem);
+  return GroupRange{s, s + per + (lb < rem ? 1 : 0), 1};
+}
+
+// Logical id of block i of nblk when the hardware deals blocks round-robin over the 8 XCDs
+// (block i runs on XCD i % 8): XCD j gets the contiguous ids [j q + min(j, r), ...), q =
+// nblk / 8, r = nblk % 8, so with contiguous group ranges each XCD's L2 serves one band of
+// rows and the x columns near it.
+constexpr int kXcds = 8;
+__device__ __forceinline__ int32_t xcd_logical(int32_t i, int32_t nblk) {
+  const int32_t q = nblk / kXcds, r = nblk % kXcds, j = i % kXcds, t = i / kXcds;
+  return j * q + min(j, r) + t;
+}
+
+template <int G, int V, class VT>
+__device__ __forceinline__ double csr_lane_sum_v(const int64_t *__restrict__ row, const int32_t *__restrict__ col,
+                                                 const VT *__restrict__ val, int32_t r, int lane,
+                                                 const double *__restrict__ x) {
+  return csr_lane_sum<G, (V & kLoadDeep) ? 8 : 4, (V & kLoadNT) != 0>(row, col, val, r, lane, x);  // V < 4
 }
 
 // Binary search of an ascending int64 array: index of target or -1
@@ -302,11 +343,11 @@ struct Operator {
   int32_t n = 0;
   int lanes = 8;
   int grid = kGridCap;
-  int nt = 0;
+  int load = 0;
   const dmrgx_matrix *A = nullptr;
   const ObcDev *obc = nullptr;
 };
-inline Operator op_of(const dmrgx_matrix &A) { return Operator{A.n, A.lanes, A.grid, A.nt, &A, nullptr}; }
+inline Operator op_of(const dmrgx_matrix &A) { return Operator{A.n, A.lanes, A.grid, A.load, &A, nullptr}; }
 inline Operator op_of(const ObcDev &O) { return Operator{O.n, O.lanes, kGridCap, 0, nullptr, &O}; }
 
 // ---- kernel launchers (kernels.hip)
@@ -371,8 +412,18 @@ void launch_ed_apply(int32_t n, const EdRows &rows, const double *x, double *y, 
                      double *dot_out, Reduce &R, hipStream_t s);
 
 // ---- host tridiagonal solver (tridiag.cpp)
-// lowest eigenvalue of T(a[0..k-1], b[0..k-2]); lam_hint: an upper bound or +inf
-double tridiag_lowest_value(int k, const double *a, const double *b, double upper_hint);
+// lowest eigenvalue of T(a[0..k-1], b[0..k-2]); upper_hint: an upper bound or +inf;
+// lower_guess: a value probably below it, or NaN
+double tridiag_lowest_value(int k, const double *a, const double *b, double upper_hint,
+                            double lower_guess = std::numeric_limits<double>::quiet_NaN());
+// Lowest eigenvalues of the successive Lanczos tridiagonals T_1, T_2, ... (T_k is the
+// leading block of T_{k+1}, so by interlacing each value is at most the previous one: that
+// bounds the search from above, and twice the last drop gives a guess below it).
+struct RitzTrack {
+  double last = std::numeric_limits<double>::infinity();
+  double drop = std::numeric_limits<double>::quiet_NaN();
+  double next(int k, const double *a, const double *b);
+};
 // eigenvector of T for eigenvalue lam, normalised
 void tridiag_vector(int k, const double *a, const double *b, double lam, double *v);
 

@@ -32,7 +32,7 @@ __global__ __launch_bounds__(kBlock) void k_finish(Fin f, const double *partial,
 
 // ------------------------------------------------------------------ SpMV + epilogue
 // Row sources: each returns this lane's share of sum_j H_rj x_j for row r.
-template <class VT, bool NT = false>
+template <class VT, int V = 0>
 struct CsrRows {
   const int64_t *__restrict__ row;
   const int32_t *__restrict__ col;
@@ -45,31 +45,34 @@ struct CsrRows {
   }
   template <int G>
   __device__ __forceinline__ double lane_sum(int32_t r, int lane, const double *__restrict__ x) const {
-    return csr_lane_sum<G, NT>(row, col, val, r, lane, x);
+    return csr_lane_sum_v<G, V>(row, col, val, r, lane, x);
   }
 };
 
-// Row groups of kBlock/G rows are dealt to the blocks round-robin (the grid is capped, so a
-// block takes several groups); each thread's epilogue contributions are summed in the order
-// of its groups, then per block.
+// Row groups of kBlock/G rows are dealt to the blocks (the grid may be capped, so a block
+// takes several groups) round-robin, or with xcd set in contiguous ranges over XCD-mapped
+// logical blocks; each thread's epilogue contributions are summed in the order of its
+// groups, then per block, and the block's partial goes to its logical slot.
 template <int G, class Rows, class Epi>
 __global__ __launch_bounds__(kBlock) void k_apply(int32_t n, Rows rows, const double *__restrict__ x, Epi epi,
-                                                  double *partial) {
+                                                  double *partial, int xcd) {
   if (epi.skip()) return;
   constexpr int RPB = kBlock / G;
   const int g = threadIdx.x / G;
   const int lane = threadIdx.x % G;
+  const int32_t lb = xcd ? xcd_logical(blockIdx.x, gridDim.x) : (int32_t)blockIdx.x;
+  const GroupRange gr = group_range(lb, gridDim.x, (n + RPB - 1) / RPB, xcd != 0);
   epi.begin();
   double c = 0.0;
-  for (int32_t r0 = blockIdx.x * RPB; r0 < n; r0 += gridDim.x * RPB) {
-    const int32_t r = r0 + g;
+  for (int32_t gi = gr.start; gi < gr.end; gi += gr.step) {
+    const int32_t r = gi * RPB + g;
     double s = 0.0;
     if (r < n) s = rows.template lane_sum<G>(r, lane, x);
 #pragma unroll
     for (int off = G / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, G);
     if (lane == 0 && r < n) c += epi.row(r, s);
   }
-  if constexpr (Epi::kReduce) block_partial(c, partial);
+  if constexpr (Epi::kReduce) block_partial_to(c, partial + lb);
 }
 
 struct EpiPlain {
@@ -186,7 +189,7 @@ struct EpiStoreDot {
 
 template <class Rows, class Epi>
 void apply_rows(int32_t n, int lanes, int grid_cap, const Rows &rows, const double *x, const Epi &epi, Reduce *R,
-                hipStream_t s) {
+                hipStream_t s, int xcd = 0) {
   const int blocks = spmv_blocks(n, lanes, grid_cap);
   if (blocks == 0) return;
   double *part = nullptr;
@@ -203,7 +206,7 @@ void apply_rows(int32_t n, int lanes, int grid_cap, const Rows &rows, const doub
   }
   switch (lanes) {
 #define L_CASE(G) \
-  case G: k_apply<G, Rows, Epi><<<blocks, kBlock, 0, s>>>(n, rows, x, epi, part); break;
+  case G: k_apply<G, Rows, Epi><<<blocks, kBlock, 0, s>>>(n, rows, x, epi, part, xcd); break;
     L_CASE(1) L_CASE(2) L_CASE(4) L_CASE(8) L_CASE(16) L_CASE(32) L_CASE(64)
 #undef L_CASE
     default: throw Failure(DMRGX_ERR_ARG, "bad lanes per row");
@@ -216,25 +219,30 @@ void apply_rows(int32_t n, int lanes, int grid_cap, const Rows &rows, const doub
   }
 }
 
+template <class VT, class Epi>
+void apply_csr(const Operator &op, const VT *val, const double *x, const Epi &epi, Reduce *R, hipStream_t s) {
+  const dmrgx_matrix &A = *op.A;
+  const int xcd = (op.load & kLoadXcd) ? 1 : 0;
+  switch (op.load & (kLoadXcd - 1)) {
+#define V_CASE(V)                                                                                              \
+  case V:                                                                                                      \
+    apply_rows(op.n, op.lanes, op.grid, CsrRows<VT, V>{A.row.p, A.col.p, val, A.nnz}, x, epi, R, s, xcd); \
+    break;
+    V_CASE(0) V_CASE(1) V_CASE(2) V_CASE(3)
+#undef V_CASE
+    default: throw Failure(DMRGX_ERR_ARG, "bad SpMV load variant");
+  }
+}
+
 template <class Epi>
 void apply_op(const Operator &op, bool fp32, const double *x, const Epi &epi, Reduce *R, hipStream_t s) {
   if (op.obc) {
     apply_rows(op.n, op.lanes, op.grid, op.obc->rows(), x, epi, R, s);
   } else if (fp32) {
     if (!op.A->val32.p) throw Failure(DMRGX_ERR_ARG, "FP32 values not prepared");
-    if (op.nt)
-      apply_rows(op.n, op.lanes, op.grid, CsrRows<float, true>{op.A->row.p, op.A->col.p, op.A->val32.p, op.A->nnz},
-                 x, epi, R, s);
-    else
-      apply_rows(op.n, op.lanes, op.grid, CsrRows<float, false>{op.A->row.p, op.A->col.p, op.A->val32.p, op.A->nnz},
-                 x, epi, R, s);
+    apply_csr<float>(op, op.A->val32.p, x, epi, R, s);
   } else {
-    if (op.nt)
-      apply_rows(op.n, op.lanes, op.grid, CsrRows<double, true>{op.A->row.p, op.A->col.p, op.A->val.p, op.A->nnz},
-                 x, epi, R, s);
-    else
-      apply_rows(op.n, op.lanes, op.grid, CsrRows<double, false>{op.A->row.p, op.A->col.p, op.A->val.p, op.A->nnz},
-                 x, epi, R, s);
+    apply_csr<double>(op, op.A->val.p, x, epi, R, s);
   }
 }
 

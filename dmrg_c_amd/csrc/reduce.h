@@ -32,11 +32,12 @@ __device__ __forceinline__ double block_sum(double v) {
   return t;
 }
 
-// This block's share of a reduction, stored with a plain store at partial[blockIdx.x].
-__device__ __forceinline__ void block_partial(double v, double *partial) {
+// This block's share of a reduction, stored with a plain store at *slot.
+__device__ __forceinline__ void block_partial_to(double v, double *slot) {
   const double t = block_sum(v);
-  if (threadIdx.x == 0) partial[blockIdx.x] = t;
+  if (threadIdx.x == 0) *slot = t;
 }
+__device__ __forceinline__ void block_partial(double v, double *partial) { block_partial_to(v, partial + blockIdx.x); }
 
 // Sum of partial[0..nblk) by one workgroup (valid in thread 0): thread t adds
 // partial[t], partial[t + kBlock], ... in index order, with eight loads in flight ahead of

@@ -88,6 +88,7 @@ void lanczos(const Operator &A, const LanczosOpts &o, double *vec, SolveOut &out
     out.spmv++;
     int next = 1, kconv = -1, chunk = small ? 16 : 8;
     double lam = E;
+    RitzTrack track;  // T of this block
     bool block_done = false;
     while (next < block && total_step < max_step) {
       const int c = std::min({chunk, block - next, max_step - total_step});
@@ -107,8 +108,7 @@ void lanczos(const Operator &A, const LanczosOpts &o, double *vec, SolveOut &out
         const bool check = small || k >= o.min_step || (restarts && k == block - 1);
         if (!check) continue;
         // T_{k+1}: diag ha[0..k], off hb[1..k]
-        lam = tridiag_lowest_value(k + 1, ha.data(), hb.data() + 1,
-                                   std::numeric_limits<double>::infinity());
+        lam = track.next(k + 1, ha.data(), hb.data() + 1);
         bool conv;
         const double bnext = hb[k + 1];
         const double tscale = std::max(1.0, std::fabs(lam));

@@ -1,54 +1,141 @@
 // Lowest eigenpair of a symmetric tridiagonal matrix, the part of dstev_ the Lanczos
 // drivers use (sml/LANCZOS_SLOW_SYM.c:127-137 take Lap_D[0] and the first eigenvector).
 //
-// Eigenvalue: bisection on the Sturm count of T - x I (number of negative pivots of its
-// LDL^T factorisation), started from the Gershgorin interval, run until the interval
-// stops shrinking (full double precision).  Eigenvector: inverse iteration with the
+// Eigenvalue: Newton's method on det(T - x I) from below, bracketed by the Sturm count of
+// T - x I (number of negative pivots of its LDL^T factorisation) and safeguarded by
+// bisection, run until it stops moving (full double precision).  Eigenvector: inverse iteration with the
 // tridiagonal LU factorisation of T - lam I using partial pivoting (row interchanges as
 // in LAPACK's dgttrf), three sweeps from a constant vector.
 #include <algorithm>
 #include <cmath>
+#include <limits>
 #include <vector>
 
 #include "internal.h"
 
 namespace dmrgx {
 
+
 namespace {
-int count_below(int k, const double *a, const double *b, double x) {
-  int c = 0;
-  double d = a[0] - x;
-  if (d < 0) c++;
-  for (int i = 1; i < k; i++) {
-    if (d == 0.0) d = -1e-300;  // perturb an exact zero pivot
-    d = (a[i] - x) - (b[i - 1] / d) * b[i - 1];
-    if (d < 0) c++;
+// Pivots of the LDL^T factorisation of T - x I.  Returns the number of pivots that are not
+// positive (an exact zero pivot counts and is replaced by a tiny negative one to go on):
+// zero means x is below every eigenvalue.  Then *ratio = p'(x)/p(x) for p(x) = det(T - x I),
+// from d'_i = -1 + (b_{i-1}/d_{i-1})^2 d'_{i-1} and p'/p = sum_i d'_i/d_i.
+int pivots(int k, const double *a, const double *b, double x, double *ratio) {
+  double d = a[0] - x, dp = -1.0;
+  int neg = 0;
+  double r = 0.0;
+  for (int i = 0;;) {
+    if (d <= 0.0) {
+      neg++;
+      if (d == 0.0) d = -1e-300;
+    }
+    r += dp / d;
+    if (++i == k) break;
+    const double q = b[i - 1] / d;
+    d = (a[i] - x) - q * b[i - 1];
+    dp = -1.0 + q * q * dp;
   }
-  return c;
+  *ratio = r;
+  return neg;
 }
 }  // namespace
 
-double tridiag_lowest_value(int k, const double *a, const double *b, double upper_hint) {
+// Lowest eigenvalue by Newton's method on det(T - x I) from below, safeguarded by
+// bisection.  Left of the lowest eigenvalue every pivot is positive and the Newton step
+// x - p/p' stays left of it and approaches it monotonically (quadratically once close, the
+// lowest eigenvalue being simple); a bisection midpoint that tests negative-free becomes the
+// new Newton point, one that does not becomes the new upper end.  upper_hint (an upper
+// bound, e.g. the previous Lanczos step's value) and lower_guess (a guess below the value)
+// shorten the search; pass +inf / NaN without them.
+double tridiag_lowest_value(int k, const double *a, const double *b, double upper_hint, double lower_guess) {
   if (k == 1) return a[0];
-  double lo = a[0], hi = a[0];
+  double glo = a[0], ghi = a[0];
   for (int i = 0; i < k; i++) {
     const double rad = (i > 0 ? std::fabs(b[i - 1]) : 0.0) + (i + 1 < k ? std::fabs(b[i]) : 0.0);
-    lo = std::min(lo, a[i] - rad);
-    hi = std::max(hi, a[i] + rad);
+    glo = std::min(glo, a[i] - rad);
+    ghi = std::max(ghi, a[i] + rad);
   }
-  const double pad = 1e-13 * (hi - lo) + 1e-300;
-  lo -= pad;
-  hi += pad;
-  if (std::isfinite(upper_hint) && upper_hint > lo && upper_hint < hi &&
-      count_below(k, a, b, upper_hint) >= 1)
-    hi = upper_hint;
-  for (int it = 0; it < 256; it++) {
-    const double mid = lo + 0.5 * (hi - lo);
-    if (!(mid > lo && mid < hi)) break;
-    if (count_below(k, a, b, mid) >= 1) hi = mid;
-    else lo = mid;
+  const double pad = 1e-13 * (ghi - glo) + 1e-300;
+  glo -= pad;
+  ghi += pad;
+  double hi = ghi, lo = glo, r = 0.0;
+  bool have_lo = false;
+  if (std::isfinite(upper_hint) && upper_hint > glo && upper_hint < ghi) {
+    if (pivots(k, a, b, upper_hint, &r) == 0) {
+      // the hint is below the value (equal to it to rounding): it is a lower end
+      lo = upper_hint;
+      have_lo = true;
+    } else {
+      hi = upper_hint;
+    }
   }
-  return lo + 0.5 * (hi - lo);
+  // lower end: the guess if it is below the value, else walk down from hi in growing steps
+  if (!have_lo && std::isfinite(lower_guess) && lower_guess > glo && lower_guess < hi &&
+      pivots(k, a, b, lower_guess, &r) == 0) {
+    lo = lower_guess;
+    have_lo = true;
+  }
+  if (!have_lo) {
+    double step = std::max(1e-12 * std::max(std::fabs(hi), 1.0), (hi - glo) * 1e-6);
+    while (true) {
+      const double x = hi - step;
+      if (!(x > glo)) {
+        lo = glo;
+        pivots(k, a, b, lo, &r);
+        break;
+      }
+      if (pivots(k, a, b, x, &r) == 0) {
+        lo = x;
+        break;
+      }
+      hi = x;
+      step *= 16.0;
+    }
+  }
+  // r = p'/p at lo (< 0); Newton moves lo up, bisection shrinks hi when Newton is slow
+  double prev_gain = std::numeric_limits<double>::infinity();
+  for (int it = 0; it < 300; it++) {
+    double x = lo - 1.0 / r;
+    if (!(x > lo)) break;                      // no progress: lo is the eigenvalue to rounding
+    if (!(x < hi)) x = lo + 0.5 * (hi - lo);   // left the bracket (rounding): bisect
+    if (!(x > lo && x < hi)) break;            // bracket at rounding width
+    double rx;
+    if (pivots(k, a, b, x, &rx) == 0) {
+      const double gain = x - lo, width = hi - lo;
+      lo = x;
+      r = rx;
+      const bool slow = gain > 0.25 * prev_gain && gain < 0.125 * width;
+      prev_gain = gain;
+      if (slow) {
+        // linear, not quadratic, progress (far left, or a close second eigenvalue): also
+        // test the midpoint
+        const double m = lo + 0.5 * (hi - lo);
+        double rm;
+        if (m > lo && m < hi) {
+          if (pivots(k, a, b, m, &rm) == 0) {
+            lo = m;
+            r = rm;
+          } else {
+            hi = m;
+          }
+        }
+      }
+    } else {
+      hi = x;
+    }
+  }
+  return lo;
+}
+
+double RitzTrack::next(int k, const double *a, const double *b) {
+  double guess = std::numeric_limits<double>::quiet_NaN();
+  if (std::isfinite(last) && std::isfinite(drop))
+    guess = last - 2.0 * std::max(drop, 0.0) - 1e-12 * std::max(1.0, std::fabs(last));
+  const double lam = tridiag_lowest_value(k, a, b, last, guess);
+  if (std::isfinite(last)) drop = last - lam;
+  last = lam;
+  return lam;
 }
 
 void tridiag_vector(int k, const double *a, const double *b, double lam, double *v) {

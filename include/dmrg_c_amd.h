@@ -443,6 +443,13 @@ void dmrgx_set_seed(uint64_t seed);
 /* Same splitmix64 counter hash as the Guess == "No" start vector: values are integers
  * in [-5e7, 5e7), the range of rand()%100000000 - 50000000 (sml/LANCZOS_SLOW_SYM.c:82). */
 void dmrgx_start_vector(int64_t n, uint64_t seed, double *out);
+/* Host routine of the Lanczos convergence checks, in place of dstev_ on the k x k
+ * tridiagonal (sml/LANCZOS_SLOW_SYM.c:137, dmrg/DMRG_LANCZOS_SLOW_OBC.c:110): lam_out[j]
+ * = lowest eigenvalue of the leading (j+1) x (j+1) block of T(a[0..k-1], b[0..k-2]) for
+ * j = 0..k-1, computed step by step as the solvers do (each step bounded by the last);
+ * vec_out (may be NULL): normalised eigenvector of the full T for lam_out[k-1].  Host
+ * only: needs no GPU. */
+int dmrgx_tridiag_lowest(int32_t k, const double *a, const double *b, double *lam_out, double *vec_out);
 
 /* ---- device matrix (full CSR, columns ascending, resident in HBM) */
 typedef struct dmrgx_matrix dmrgx_matrix;
@@ -461,14 +468,18 @@ int dmrgx_spmv_device(const dmrgx_matrix *A, const double *x_dev, double *y_dev)
 /* y = A x with host pointers. */
 int dmrgx_spmv_host(const dmrgx_matrix *A, const double *x, double *y);
 /* SpMV schedule of this matrix: lanes (threads) per row, a power of two 1..64; the cap on
- * workgroups (row groups are dealt round-robin; 0 = one workgroup per row group); and the
- * load policy of the matrix arrays (1 = non-temporal).  Defaults come from the mean row
- * length; dmrgx_matrix_autotune times the candidates on the device (product + reduction, as
- * the solvers run it) and keeps the fastest.  Single and batched solves use the matrix's
+ * workgroups (0 = one workgroup per row group); and the load variant 0..7 (bit 0:
+ * non-temporal loads of the matrix arrays; bit 1: eight loads in flight per lane instead of
+ * four; bit 2: each workgroup takes a contiguous range of row groups and each XCD a
+ * contiguous share of the rows, instead of round-robin groups).  Bits 0 and 1 never change
+ * results; lanes change the summation order within a row, and cap and bit 2 the order of
+ * the dot products fused into the solvers' products.  Defaults come from the mean row length;
+ * dmrgx_matrix_autotune times the candidates on the device (product + reduction, as the
+ * solvers run it) and keeps the fastest.  Single and batched solves use the matrix's
  * schedule. */
-int dmrgx_matrix_set_schedule(dmrgx_matrix *A, int32_t lanes, int32_t grid_cap, int32_t nt_loads);
-int dmrgx_matrix_get_schedule(const dmrgx_matrix *A, int32_t *lanes, int32_t *grid_cap, int32_t *nt_loads);
-int dmrgx_matrix_autotune(dmrgx_matrix *A, int32_t *lanes_out, int32_t *grid_out, int32_t *nt_out,
+int dmrgx_matrix_set_schedule(dmrgx_matrix *A, int32_t lanes, int32_t grid_cap, int32_t load_variant);
+int dmrgx_matrix_get_schedule(const dmrgx_matrix *A, int32_t *lanes, int32_t *grid_cap, int32_t *load_variant);
+int dmrgx_matrix_autotune(dmrgx_matrix *A, int32_t *lanes_out, int32_t *grid_out, int32_t *load_out,
                           double *best_us_out);
 
 /* ---- ground-state solve (Lanczos -> inverse iteration with CG), all on the device */
