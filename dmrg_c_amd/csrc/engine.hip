@@ -409,11 +409,13 @@ void lanczos_finished(Slot &s, int kconv) {
 }
 
 // Host checks of the steps computed since the last chunk (solver.hip:lanczos, no restarts).
-void lanczos_check(Slot &s, const double *alpha_h, const double *beta_h) {
+// alpha_new[j] = alpha_{k_read + j}, beta_new[j] = beta_{k_read + 1 + j} for the steps
+// k_read .. k_next - 1 run since the last check.
+void lanczos_check(Slot &s, const double *alpha_new, const double *beta_new) {
   const bool small = s.n < 1000;
   for (int k = s.k_read; k < s.k_next; k++) {
-    s.ha[k] = alpha_h[k];
-    s.hb[k + 1] = beta_h[k + 1];
+    s.ha[k] = alpha_new[k - s.k_read];
+    s.hb[k + 1] = beta_new[k - s.k_read];
   }
   for (int k = s.k_read; k < s.k_next; k++) {
     s.k_read = k + 1;
@@ -638,19 +640,33 @@ void run_chunk(dmrgx_engine &E, int ticks) {
   HIPCHK(hipGetLastError());
   for (size_t i = 0; i < L.size(); i++) L[i]->k_next = lan_k[i];
   for (size_t i = 0; i < Cg.size(); i++) Cg[i]->cg_it = cg_it[i];
-  HIPCHK(hipStreamSynchronize(st));  // host descriptor vectors go out of scope; scalars read next
-  // host side: Lanczos scalars and CG status
-  for (Slot *s : L) {
-    double *h = ctx().host_stage(2 * (size_t)s->KS);
-    HIPCHK(hipMemcpyAsync(h, s->alpha(), sizeof(double) * 2 * s->KS, hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
-    std::vector<double> a(h, h + s->KS), b(h + s->KS, h + 2 * s->KS);
-    lanczos_check(*s, a.data(), b.data());
+  // host side: the new Lanczos scalars (alpha_k and beta_{k+1} of this chunk's steps) and
+  // the CG status of every slot, copied into one staging area before one synchronisation
+  std::vector<size_t> off(L.size() + Cg.size() + 1, 0);
+  for (size_t i = 0; i < L.size(); i++) off[i + 1] = off[i] + 2 * (size_t)(L[i]->k_next - L[i]->k_read);
+  for (size_t i = 0; i < Cg.size(); i++) off[L.size() + i + 1] = off[L.size() + i] + 1;  // {done, it} in 8 bytes
+  double *h = ctx().host_stage(std::max<size_t>(off.back(), 1));
+  for (size_t i = 0; i < L.size(); i++) {
+    const Slot &s = *L[i];
+    const size_t c = (size_t)(s.k_next - s.k_read);
+    if (c == 0) continue;
+    HIPCHK(hipMemcpyAsync(h + off[i], s.scal.p + s.k_read, sizeof(double) * c, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(h + off[i] + c, s.scal.p + s.KS + s.k_read + 1, sizeof(double) * c,
+                          hipMemcpyDeviceToHost, st));
   }
-  for (Slot *s : Cg) {
-    int *hs = reinterpret_cast<int *>(ctx().host_stage(1));
-    HIPCHK(hipMemcpyAsync(hs, s->cgst.p, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
+  for (size_t i = 0; i < Cg.size(); i++)
+    HIPCHK(hipMemcpyAsync(h + off[L.size() + i], Cg[i]->cgst.p, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));  // host descriptor vectors go out of scope; scalars read next
+  // the staging area is shared: keep a copy before the phase changes below reuse it
+  const std::vector<double> stage(h, h + off.back());
+  for (size_t i = 0; i < L.size(); i++) {
+    const size_t c = off[i + 1] - off[i];
+    lanczos_check(*L[i], stage.data() + off[i], stage.data() + off[i] + c / 2);
+  }
+  for (size_t i = 0; i < Cg.size(); i++) {
+    Slot *s = Cg[i];
+    int hs[2];
+    std::memcpy(hs, stage.data() + off[L.size() + i], sizeof(hs));
     if (hs[0]) cg_finished(*s, hs[1]);
     else if (s->cg_it >= s->p.cg_max_step) cg_finished(*s, -1);
   }
