@@ -142,6 +142,25 @@ def cpu_baseline(snap, e_ref, budget_s):
     }
 
 
+def spmv_sweep(D, sb, snap):
+    """Every schedule the autotuner considers, timed with 20 launches each: lanes per row x
+    workgroup cap x load variant -> us per product + fused dot, and algorithmic GB/s."""
+    A = sb.assemble(full=True)
+    n = snap.n
+    alg = A.nnz * 12 + (n + 1) * 8 + 2 * n * 8
+    rows = []
+    for v in range(8):
+        for g in (4, 8, 16, 32, 64):
+            for cap in (1024, 2048, 4096, 8192, 16384, 0):
+                us = A.time_schedule(g, cap, v, 20)
+                rows.append({"lanes": g, "cap": cap, "variant": v, "us": round(us, 2),
+                             "alg_GBs": round(alg / us / 1e3, 1)})
+        log(f"sweep: variant {v} done")
+    A.free()
+    rows.sort(key=lambda r: r["us"])
+    print(json.dumps({"spmv_sweep": rows, "n": n, "alg_bytes": alg}))
+
+
 def traffic_probe(D, L, sb, snap):
     L.dmrgx_device_alloc.argtypes = [C.c_int64, C.POINTER(C.c_void_p)]
     L.dmrgx_device_free.argtypes = [C.c_void_p]
@@ -314,6 +333,8 @@ def main():
     ap.add_argument("--traffic-probe", action="store_true",
                     help="run 5 plain SpMV launches on the workload and one copy kernel of 2^26 doubles "
                          "(traffic calibration under rocprofv3 --pmc), then exit")
+    ap.add_argument("--spmv-sweep", action="store_true",
+                    help="time every SpMV schedule on the workload superblock (JSON table), then exit")
     ap.add_argument("--workload", choices=["superblock", "ed-batch", "serve"], default="superblock",
                     help="superblock: the headline line; ed-batch: 256 seeded ED solves one by one and "
                          "through the batched engine; serve: Poisson request stream through the engine "
@@ -347,6 +368,9 @@ def main():
     sb = D.Superblock(snap)  # inputs resident in HBM from here on
     if args.traffic_probe:
         traffic_probe(D, L, sb, snap)
+        return
+    if args.spmv_sweep:
+        spmv_sweep(D, sb, snap)
         return
     log(f"workload: Hubbard L=64 m=400 superblock n={snap.n}")
     lanes, tuned_us = autotune(D, sb)

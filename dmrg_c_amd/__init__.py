@@ -136,6 +136,7 @@ def lib() -> C.CDLL:
     L.dmrgx_matrix_set_schedule.argtypes = [vp, i32, i32, i32]
     L.dmrgx_matrix_get_schedule.argtypes = [vp, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32)]
     L.dmrgx_matrix_autotune.argtypes = [vp, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32), C.POINTER(f64)]
+    L.dmrgx_matrix_time_schedule.argtypes = [vp, i32, i32, i32, i32, C.POINTER(f64)]
     L.MATRIX_VECTOR_PRODUCT_SYM.argtypes = [C.POINTER(CRS1), vp, vp, vp, C.c_int]
     L.MATRIX_VECTOR_PRODUCT.argtypes = [C.POINTER(CRS1), vp, vp, C.c_int]
     for nm in ("LANCZOS_SLOW_SYM", "LANCZOS_SYM", "LANCZOS_SLOW", "LANCZOS"):
@@ -235,6 +236,12 @@ class DeviceMatrix:
         g, cap, nt, us = C.c_int32(), C.c_int32(), C.c_int32(), C.c_double()
         check(lib().dmrgx_matrix_autotune(self.handle, C.byref(g), C.byref(cap), C.byref(nt), C.byref(us)))
         return (g.value, cap.value, nt.value), us.value
+
+    def time_schedule(self, lanes: int, grid_cap: int, load_variant: int = 0, reps: int = 10) -> float:
+        """Mean microseconds of one product + fused dot under that schedule."""
+        us = C.c_double()
+        check(lib().dmrgx_matrix_time_schedule(self.handle, lanes, grid_cap, load_variant, reps, C.byref(us)))
+        return us.value
 
     def ground_state(self, start: Optional[np.ndarray] = None, **kw):
         p = SolveParams()

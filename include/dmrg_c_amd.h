@@ -481,6 +481,10 @@ int dmrgx_matrix_set_schedule(dmrgx_matrix *A, int32_t lanes, int32_t grid_cap, 
 int dmrgx_matrix_get_schedule(const dmrgx_matrix *A, int32_t *lanes, int32_t *grid_cap, int32_t *load_variant);
 int dmrgx_matrix_autotune(dmrgx_matrix *A, int32_t *lanes_out, int32_t *grid_out, int32_t *load_out,
                           double *best_us_out);
+/* Mean microseconds of one product + fused dot under the given schedule (reps launches
+ * after one warm-up), leaving A's schedule unchanged. */
+int dmrgx_matrix_time_schedule(dmrgx_matrix *A, int32_t lanes, int32_t grid_cap, int32_t load_variant,
+                               int32_t reps, double *us_out);
 
 /* ---- ground-state solve (Lanczos -> inverse iteration with CG), all on the device */
 typedef struct {

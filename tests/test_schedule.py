@@ -96,3 +96,14 @@ def test_gpu_autotune_sets_schedule():
         assert abs(E - e_ref) <= 1e-10 * abs(e_ref)
     finally:
         A.free()
+
+
+@pytest.mark.gpu
+def test_gpu_time_schedule_leaves_schedule():
+    A, _ = _matrix()
+    try:
+        A.set_schedule(8, 2048, 1)
+        us = A.time_schedule(32, 0, 6, 5)
+        assert us > 0 and A.schedule == (8, 2048, 1)
+    finally:
+        A.free()
