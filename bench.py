@@ -149,7 +149,7 @@ def spmv_sweep(D, sb, snap):
     n = snap.n
     alg = A.nnz * 12 + (n + 1) * 8 + 2 * n * 8
     rows = []
-    for v in range(8):
+    for v in range(16):
         for g in (4, 8, 16, 32, 64):
             for cap in (1024, 2048, 4096, 8192, 16384, 0):
                 us = A.time_schedule(g, cap, v, 20)
@@ -437,7 +437,8 @@ def main():
             "spmv_schedule": {"lanes_per_row": lanes[0], "workgroup_cap": lanes[1],
                               "nontemporal_matrix_loads": bool(lanes[2] & 1),
                               "loads_in_flight_per_lane": 8 if lanes[2] & 2 else 4,
-                              "xcd_contiguous_rows": bool(lanes[2] & 4)},
+                              "xcd_contiguous_rows": bool(lanes[2] & 4),
+                              "value_dictionary": bool(lanes[2] & 8)},
         },
         "forms": {
             "fp32_refined": {"solves_per_s": round(len(mlat) / sum(mlat), 4),

@@ -276,7 +276,7 @@ std::unique_ptr<dmrgx_matrix> upload_csr(int32_t n, const int64_t *row, const in
     HIPCHK(hipMemcpyAsync(A->val.p, val, sizeof(double) * nnz, hipMemcpyHostToDevice, st));
     HIPCHK(hipStreamSynchronize(st));
   }
-  A->lanes = pick_lanes(A->n, A->nnz);
+  default_schedule(*A);
   return A;
 }
 }  // namespace dmrgx
@@ -428,8 +428,8 @@ int dmrgx_matrix_get_schedule(const dmrgx_matrix *A, int32_t *lanes, int32_t *gr
 
 // SpMV schedule autotuner: times y = A x with a fused dot product (HIP events on the library
 // stream) for each power-of-two lanes-per-row count around the mean row length, each
-// workgroup cap and each load variant, re-times the six fastest with more launches, and
-// keeps the fastest.
+// workgroup cap and each load variant (the dictionary ones if A has a value dictionary),
+// re-times the six fastest with more launches, and keeps the fastest.
 namespace {
 // Mean time of y = A x with a fused dot product under schedule (g, cap, v), as the solvers
 // run their products (product, block partials, finish kernel), over reps launches after
@@ -485,6 +485,15 @@ int dmrgx_matrix_time_schedule(dmrgx_matrix *A, int32_t lanes, int32_t grid_cap,
   });
 }
 
+int dmrgx_matrix_dict_values(dmrgx_matrix *A, int64_t *n_values) {
+  if (!A || !n_values) return DMRGX_ERR_ARG;
+  std::lock_guard<std::recursive_mutex> lk(api_mutex());
+  return run_status([&] {
+    ensure_dict(*A);
+    *n_values = std::max<int64_t>(A->n_values, 0);
+  });
+}
+
 int dmrgx_matrix_autotune(dmrgx_matrix *A, int32_t *lanes_out, int32_t *grid_out, int32_t *load_out,
                           double *best_us_out) {
   if (!A) return DMRGX_ERR_ARG;
@@ -498,9 +507,11 @@ int dmrgx_matrix_autotune(dmrgx_matrix *A, int32_t *lanes_out, int32_t *grid_out
       double us;
     };
     std::vector<Cand> cands;
+    const bool dict = ensure_dict(*A);
     for (int v = 0; v < kLoadVariants; v++)
-      for (int g = std::max(1, base / 4); g <= std::min(64, base * 2); g *= 2)
-        for (int cap : caps) cands.push_back({g, cap, v, env.time(A, g, cap, v, 3)});
+      if (dict || !(v & kLoadDict))
+        for (int g = std::max(1, base / 4); g <= std::min(64, base * 2); g *= 2)
+          for (int cap : caps) cands.push_back({g, cap, v, env.time(A, g, cap, v, 3)});
     // the leaders again with more launches, so that noise does not pick the schedule
     std::sort(cands.begin(), cands.end(), [](const Cand &a, const Cand &b) { return a.us < b.us; });
     cands.resize(std::min<size_t>(cands.size(), 6));

@@ -479,7 +479,7 @@ static int64_t assemble_lower(dmrgx_superblock &S) {
 static dmrgx_matrix *expand_last(dmrgx_superblock &S) {
   auto A = std::make_unique<dmrgx_matrix>();
   launch_expand_lower(S.n, S.lnnz, S.lrow.p, S.lcol.p, S.lval.p, *A, ctx().stream);
-  A->lanes = pick_lanes(A->n, A->nnz);
+  default_schedule(*A);
   return A.release();
 }
 

@@ -411,7 +411,7 @@ std::unique_ptr<dmrgx_matrix> ed_assemble(const dmrgx_ed &E) {
                                                         nullptr, nullptr);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(st));
-  A->lanes = pick_lanes(A->n, A->nnz);
+  default_schedule(*A);
   return A;
 }
 

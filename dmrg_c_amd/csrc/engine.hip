@@ -47,6 +47,8 @@ struct SegDesc {
   const int32_t *col;
   const double *val;
   const float *val32;
+  const uint64_t *ent;    // value dictionary form (load & kLoadDict, FP64 products)
+  const double *table;
   int32_t n, fp32;
   int32_t blk_lo, nblk;   // this slot's blocks in the launch
   int32_t k, load;        // step (Lanczos) or iteration (CG); SpMV load variant
@@ -103,10 +105,11 @@ __global__ __launch_bounds__(kBlock) void k_seg_spmv(const SegDesc *desc, const 
     double s = 0.0;
     if (r < d.n) {
       switch (d.load & (kLoadXcd - 1)) {
-#define V_CASE(V)                                                         \
-  case V:                                                                 \
-    if (d.fp32) s = csr_lane_sum_v<G, V>(d.row, d.col, d.val32, r, lane, d.x); \
-    else s = csr_lane_sum_v<G, V>(d.row, d.col, d.val, r, lane, d.x);       \
+#define V_CASE(V)                                                              \
+  case V:                                                                      \
+    if (d.fp32) s = csr_lane_sum_v<G, V>(d.row, d.col, d.val32, r, lane, d.x);   \
+    else if (d.load & kLoadDict) s = dict_lane_sum_v<G, V>(d.row, d.ent, d.table, r, lane, d.x); \
+    else s = csr_lane_sum_v<G, V>(d.row, d.col, d.val, r, lane, d.x);         \
     break;
         V_CASE(0) V_CASE(1) V_CASE(2) V_CASE(3)
 #undef V_CASE
@@ -267,6 +270,11 @@ size_t slot_bytes(const dmrgx_engine &E, const Slot &s) {
   return b;
 }
 
+// The slot matrix's load variant, without the dictionary bit when it has no dictionary.
+int slot_load(const Slot &s) {
+  return (s.A->load & kLoadDict) && s.A->n_values <= 0 ? (s.A->load & ~kLoadDict) : s.A->load;
+}
+
 // Start a slot: start vector and |w|^2 (solver.hip:lanczos before its first step).
 void start_slot(Slot &s) {
   hipStream_t st = ctx().stream;
@@ -283,6 +291,7 @@ void start_slot(Slot &s) {
   s.hb.assign(s.KS, 0.0);
   s.s.assign(s.KS, 0.0);
   if (s.mixed) ensure_fp32(*s.A);
+  if (s.A->load & kLoadDict) ensure_dict(*s.A);
   launch_start_vector(n, s.p.seed, s.vec.p, st);
   launch_copy(n, s.vec.p, s.w.p, st);
   Reduce R;
@@ -530,7 +539,9 @@ void run_chunk(dmrgx_engine &E, int ticks) {
         a.val32 = s.A->val32.p;
         a.n = s.n;
         a.fp32 = s.mixed ? 1 : 0;
-        a.load = s.A->load;
+        a.load = slot_load(s);
+        a.ent = s.A->ent.p;
+        a.table = s.A->table.p;
         a.k = k;
         a.shift = 0.0;
         a.x = s.w.p;
@@ -571,7 +582,9 @@ void run_chunk(dmrgx_engine &E, int ticks) {
         a.val = s.A->val.p;
         a.n = s.n;
         a.k = it;
-        a.load = s.A->load;
+        a.load = slot_load(s);
+        a.ent = s.A->ent.p;
+        a.table = s.A->table.p;
         a.shift = s.p.ii_diag_add - s.eig;
         a.x = s.pv.p;
         a.o1 = s.cy.p;

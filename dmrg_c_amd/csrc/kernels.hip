@@ -49,6 +49,23 @@ struct CsrRows {
   }
 };
 
+template <int V>
+struct DictRows {
+  const int64_t *__restrict__ row;
+  const uint64_t *__restrict__ ent;
+  const double *__restrict__ table;
+  int64_t nnz, n_values;
+  static constexpr bool kTimed = true;
+  // algorithmic bytes of one y = H x: packed entries + table once + row pointers + x + y
+  double alg_bytes(int32_t n) const {
+    return (double)nnz * 8 + (double)n_values * 8 + (double)(n + 1) * 8 + 2.0 * n * 8;
+  }
+  template <int G>
+  __device__ __forceinline__ double lane_sum(int32_t r, int lane, const double *__restrict__ x) const {
+    return dict_lane_sum_v<G, V>(row, ent, table, r, lane, x);
+  }
+};
+
 // Row groups of kBlock/G rows are dealt to the blocks (the grid may be capped, so a block
 // takes several groups) round-robin, or with xcd set in contiguous ranges over XCD-mapped
 // logical blocks; each thread's epilogue contributions are summed in the order of its
@@ -219,6 +236,21 @@ void apply_rows(int32_t n, int lanes, int grid_cap, const Rows &rows, const doub
   }
 }
 
+template <class Epi>
+void apply_dict(const Operator &op, const double *x, const Epi &epi, Reduce *R, hipStream_t s) {
+  const dmrgx_matrix &A = *op.A;
+  const int xcd = (op.load & kLoadXcd) ? 1 : 0;
+  switch (op.load & (kLoadXcd - 1)) {
+#define V_CASE(V)                                                                                           \
+  case V:                                                                                                   \
+    apply_rows(op.n, op.lanes, op.grid, DictRows<V>{A.row.p, A.ent.p, A.table.p, A.nnz, A.n_values}, x, epi, \
+               R, s, xcd);                                                                                  \
+    break;
+    V_CASE(0) V_CASE(1) V_CASE(2) V_CASE(3)
+#undef V_CASE
+  }
+}
+
 template <class VT, class Epi>
 void apply_csr(const Operator &op, const VT *val, const double *x, const Epi &epi, Reduce *R, hipStream_t s) {
   const dmrgx_matrix &A = *op.A;
@@ -241,6 +273,8 @@ void apply_op(const Operator &op, bool fp32, const double *x, const Epi &epi, Re
   } else if (fp32) {
     if (!op.A->val32.p) throw Failure(DMRGX_ERR_ARG, "FP32 values not prepared");
     apply_csr<float>(op, op.A->val32.p, x, epi, R, s);
+  } else if ((op.load & kLoadDict) && ensure_dict(*op.A)) {
+    apply_dict(op, x, epi, R, s);
   } else {
     apply_csr<double>(op, op.A->val.p, x, epi, R, s);
   }
@@ -583,6 +617,13 @@ int pick_lanes(int32_t n, int64_t nnz) {
   return g;
 }
 
+void default_schedule(dmrgx_matrix &A) {
+  A.lanes = pick_lanes(A.n, A.nnz);
+  A.grid = kGridCap;
+  // large matrices read the value dictionary (built at the first product) when it pays
+  A.load = A.nnz >= (int64_t(1) << 20) ? kLoadDict : 0;
+}
+
 int spmv_blocks(int32_t n, int lanes, int grid_cap) {
   const int rpb = kBlock / lanes;
   const int64_t groups = ((int64_t)n + rpb - 1) / rpb;
@@ -781,6 +822,120 @@ void launch_exclusive_scan(int32_t n, const int64_t *len, int64_t *out, hipStrea
 void launch_start_vector(int64_t n, uint64_t seed, double *out, hipStream_t s) {
   k_start_vector<<<vec_grid(n), kBlock, 0, s>>>(n, seed, out);
   HIPCHK(hipGetLastError());
+}
+
+// ---------------------------------------------------------------- value dictionary
+// Open-addressing hash set of the values' bit patterns (linear probing, at most kProbe
+// probes; a full neighbourhood sets *overflow and the build is retried with a larger
+// table).  Inserts are vector atomicCAS on distinct slots; a slot that already holds the
+// key is found with a plain load.
+constexpr uint64_t kEmptyKey = ~0ull;   // a NaN pattern: NaN values make the build fail
+constexpr int kProbe = 64;
+
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+__global__ __launch_bounds__(kBlock) void k_fill_u64(int64_t n, uint64_t v, uint64_t *out) {
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) out[i] = v;
+}
+
+__global__ __launch_bounds__(kBlock) void k_dict_insert(int64_t nnz, const double *__restrict__ val,
+                                                        unsigned long long *keys, uint64_t mask, int *overflow) {
+  for (int64_t j = blockIdx.x * (int64_t)kBlock + threadIdx.x; j < nnz; j += (int64_t)gridDim.x * kBlock) {
+    const uint64_t key = (uint64_t)__double_as_longlong(val[j]);
+    if (key == kEmptyKey) {
+      atomicOr(overflow, 2);   // a value with the empty pattern: no dictionary
+      continue;
+    }
+    uint64_t h = mix64(key) & mask;
+    bool placed = false;
+    for (int p = 0; p < kProbe && !placed; p++) {
+      unsigned long long cur = __atomic_load_n(keys + h, __ATOMIC_RELAXED);
+      if (cur == kEmptyKey) cur = atomicCAS(keys + h, (unsigned long long)kEmptyKey, (unsigned long long)key);
+      if (cur == kEmptyKey || cur == key) placed = true;
+      else h = (h + 1) & mask;
+    }
+    if (!placed) atomicOr(overflow, 1);   // neighbourhood full: retry with a larger table
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_dict_occupied(int64_t cap, const unsigned long long *keys, int64_t *occ) {
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < cap; i += (int64_t)gridDim.x * kBlock)
+    occ[i] = keys[i] != kEmptyKey ? 1 : 0;
+}
+
+__global__ __launch_bounds__(kBlock) void k_dict_table(int64_t cap, const unsigned long long *keys,
+                                                       const int64_t *pos, double *table) {
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < cap; i += (int64_t)gridDim.x * kBlock)
+    if (keys[i] != kEmptyKey) table[pos[i]] = __longlong_as_double((long long)keys[i]);
+}
+
+__global__ __launch_bounds__(kBlock) void k_dict_pack(int64_t nnz, const int32_t *__restrict__ col,
+                                                      const double *__restrict__ val,
+                                                      const unsigned long long *__restrict__ keys, uint64_t mask,
+                                                      const int64_t *__restrict__ pos, uint64_t *__restrict__ ent,
+                                                      int *overflow) {
+  for (int64_t j = blockIdx.x * (int64_t)kBlock + threadIdx.x; j < nnz; j += (int64_t)gridDim.x * kBlock) {
+    const uint64_t key = (uint64_t)__double_as_longlong(val[j]);
+    uint64_t h = mix64(key) & mask;
+    int p = 0;
+    while (p < kProbe && keys[h] != key) {
+      h = (h + 1) & mask;
+      p++;
+    }
+    if (p == kProbe) {
+      *overflow = 1;
+      ent[j] = 0;
+      continue;
+    }
+    ent[j] = (uint64_t)(uint32_t)col[j] | ((uint64_t)pos[h] << 32);
+  }
+}
+
+// Returns the number of distinct values, or 0 if a table of up to max_cap slots could not
+// hold them at half load (the matrix then keeps the plain form).
+int64_t build_value_dict(dmrgx_matrix &A, hipStream_t s) {
+  const int64_t nnz = A.nnz;
+  if (nnz == 0) return 0;
+  constexpr int64_t kMaxCap = int64_t(1) << 24;  // 128 MiB of keys at most
+  int64_t cap = int64_t(1) << 16;
+  while (cap < kMaxCap && cap < 2 * std::min<int64_t>(nnz, int64_t(1) << 20)) cap <<= 1;
+  DevBuf<int> flag(1);
+  for (;; cap <<= 1) {
+    if (cap > kMaxCap) return 0;
+    DevBuf<unsigned long long> keys(cap);
+    k_fill_u64<<<vec_grid(cap), kBlock, 0, s>>>(cap, kEmptyKey, reinterpret_cast<uint64_t *>(keys.p));
+    HIPCHK(hipMemsetAsync(flag.p, 0, sizeof(int), s));
+    k_dict_insert<<<vec_grid(nnz), kBlock, 0, s>>>(nnz, A.val.p, keys.p, (uint64_t)(cap - 1), flag.p);
+    HIPCHK(hipGetLastError());
+    int ovf = 0;
+    HIPCHK(hipMemcpyAsync(&ovf, flag.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (ovf & 2) return 0;
+    if (ovf) continue;
+    DevBuf<int64_t> occ(cap), pos(cap + 1);
+    k_dict_occupied<<<vec_grid(cap), kBlock, 0, s>>>(cap, keys.p, occ.p);
+    HIPCHK(hipGetLastError());
+    launch_exclusive_scan((int32_t)cap, occ.p, pos.p, s);
+    int64_t nv = 0;
+    HIPCHK(hipMemcpyAsync(&nv, pos.p + cap, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (2 * nv > cap) continue;            // keep probe runs short: at most half full
+    if (nv > nnz / 4) return 0;            // too many distinct values to be worth a table
+    A.table.resize(nv);
+    A.ent.resize(nnz);
+    k_dict_table<<<vec_grid(cap), kBlock, 0, s>>>(cap, keys.p, pos.p, A.table.p);
+    k_dict_pack<<<vec_grid(nnz), kBlock, 0, s>>>(nnz, A.col.p, A.val.p, keys.p, (uint64_t)(cap - 1), pos.p,
+                                                 A.ent.p, flag.p);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(&ovf, flag.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (ovf) throw Failure(DMRGX_ERR_HIP, "value dictionary lookup failed after a complete build");
+    return nv;
+  }
 }
 
 void launch_to_fp32(const dmrgx_matrix &A, hipStream_t s) {

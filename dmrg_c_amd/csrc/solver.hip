@@ -189,6 +189,19 @@ void ensure_fp32(const dmrgx_matrix &A) {
   HIPCHK(hipStreamSynchronize(ctx().stream));
 }
 
+bool ensure_dict(const dmrgx_matrix &A) {
+  if (A.n_values < 0) {
+    // the dictionary is a cache of the FP64 values; the matrix itself is not changed
+    dmrgx_matrix &M = const_cast<dmrgx_matrix &>(A);
+    M.n_values = build_value_dict(M, ctx().stream);
+    if (M.n_values == 0) {
+      M.ent.reset();
+      M.table.reset();
+    }
+  }
+  return A.n_values > 0;
+}
+
 int cg_solve(const Operator &A, double shift, const double *b, double *x, double acc, int max_step,
              bool *converged, SolveOut &out) {
   Context &C = ctx();

@@ -468,12 +468,16 @@ int dmrgx_spmv_device(const dmrgx_matrix *A, const double *x_dev, double *y_dev)
 /* y = A x with host pointers. */
 int dmrgx_spmv_host(const dmrgx_matrix *A, const double *x, double *y);
 /* SpMV schedule of this matrix: lanes (threads) per row, a power of two 1..64; the cap on
- * workgroups (0 = one workgroup per row group); and the load variant 0..7 (bit 0:
+ * workgroups (0 = one workgroup per row group); and the load variant 0..15 (bit 0:
  * non-temporal loads of the matrix arrays; bit 1: eight loads in flight per lane instead of
  * four; bit 2: each workgroup takes a contiguous range of row groups and each XCD a
- * contiguous share of the rows, instead of round-robin groups).  Bits 0 and 1 never change
- * results; lanes change the summation order within a row, and cap and bit 2 the order of
- * the dot products fused into the solvers' products.  Defaults come from the mean row length;
+ * contiguous share of the rows, instead of round-robin groups; bit 3: FP64 products read
+ * the value dictionary -- 8-byte entries (column, index of the value in a table of the
+ * matrix's distinct values) instead of 4-byte columns and 8-byte values -- built at the
+ * first such product when the matrix has few distinct values, else ignored).  Bits 0, 1
+ * and 3 never change results; lanes change the summation order within a row, and cap and
+ * bit 2 the order of the dot products fused into the solvers' products.  Defaults come
+ * from the mean row length (with bit 3 for matrices of 2^20 entries or more);
  * dmrgx_matrix_autotune times the candidates on the device (product + reduction, as the
  * solvers run it) and keeps the fastest.  Single and batched solves use the matrix's
  * schedule. */
@@ -485,6 +489,9 @@ int dmrgx_matrix_autotune(dmrgx_matrix *A, int32_t *lanes_out, int32_t *grid_out
  * after one warm-up), leaving A's schedule unchanged. */
 int dmrgx_matrix_time_schedule(dmrgx_matrix *A, int32_t lanes, int32_t grid_cap, int32_t load_variant,
                                int32_t reps, double *us_out);
+/* Distinct values in A's value dictionary (built now if it has not been tried); 0 if A
+ * keeps the plain form. */
+int dmrgx_matrix_dict_values(dmrgx_matrix *A, int64_t *n_values);
 
 /* ---- ground-state solve (Lanczos -> inverse iteration with CG), all on the device */
 typedef struct {

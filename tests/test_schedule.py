@@ -1,7 +1,7 @@
 """SpMV schedules (dmrgx_matrix_set_schedule / _get_schedule / _autotune).
 
-The load bits of the variant (non-temporal loads, loads in flight per lane) change no
-arithmetic, so the solve is bit-identical under them; lanes per row change the summation
+The load bits of the variant (non-temporal loads, loads in flight per lane, the value
+dictionary) change no arithmetic, so the solve is bit-identical under them; lanes per row change the summation
 order of each row, and the workgroup cap and the row-range bit the order of the fused dot
 products, so energies agree with the reference's to the FP64 bound under every schedule."""
 import os
@@ -27,11 +27,11 @@ def test_gpu_schedule_argument_checks():
     import dmrg_c_amd as D
     A, _ = _matrix()
     try:
-        for bad in ((0, 0, 0), (3, 0, 0), (128, 0, 0), (8, -1, 0), (8, 0, 8), (8, 0, -1)):
+        for bad in ((0, 0, 0), (3, 0, 0), (128, 0, 0), (8, -1, 0), (8, 0, 16), (8, 0, -1)):
             with pytest.raises(D.DmrgxError):
                 A.set_schedule(*bad)
-        A.set_schedule(16, 2048, 7)
-        assert A.schedule == (16, 2048, 7)
+        A.set_schedule(16, 2048, 15)
+        assert A.schedule == (16, 2048, 15)
     finally:
         A.free()
 
@@ -56,19 +56,19 @@ def test_gpu_load_variants_bit_identical():
         # capped grid: the load bits give the same bits within each row mapping
         for base in (0, 4):
             ref = _solves(A, (8, 4096, base), x)
-            for v in (1, 2, 3):
+            for v in (1, 2, 3, 8, 9, 10, 11):
                 _same(ref, _solves(A, (8, 4096, base | v), x))
             assert abs(ref[1][0] - e_ref) <= 1e-10 * abs(e_ref)
         # one workgroup per row group: the row mapping changes nothing either
         ref = _solves(A, (16, 0, 0), x)
-        for v in range(1, 8):
+        for v in range(1, 16):
             _same(ref, _solves(A, (16, 0, v), x))
     finally:
         A.free()
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("sched", [(1, 0, 0), (4, 1024, 5), (16, 0, 2), (32, 8192, 7), (64, 2048, 4), (8, 16384, 6)])
+@pytest.mark.parametrize("sched", [(1, 0, 0), (4, 1024, 5), (16, 0, 10), (32, 8192, 15), (64, 2048, 12), (8, 16384, 6)])
 def test_gpu_schedules_agree(sched):
     A, e_ref = _matrix()
     try:
@@ -91,7 +91,7 @@ def test_gpu_autotune_sets_schedule():
         sched, us = A.autotune()
         assert A.schedule == sched and us > 0
         g, cap, v = sched
-        assert g & (g - 1) == 0 and 1 <= g <= 64 and cap >= 0 and v in range(8)
+        assert g & (g - 1) == 0 and 1 <= g <= 64 and cap >= 0 and v in range(16)
         E, _, _ = A.ground_state()
         assert abs(E - e_ref) <= 1e-10 * abs(e_ref)
     finally:
@@ -107,3 +107,55 @@ def test_gpu_time_schedule_leaves_schedule():
         assert us > 0 and A.schedule == (8, 2048, 1)
     finally:
         A.free()
+
+
+@pytest.mark.gpu
+def test_gpu_value_dictionary():
+    """The benchmark superblock's 26.8 M entries hold ~78 k distinct values: the dictionary
+    has exactly the distinct values of the matrix, is on by default, and its products are
+    the plain products bit for bit."""
+    import ctypes as C
+    import dmrg_c_amd as D
+    A, _ = _matrix()
+    try:
+        assert A.schedule[2] & 8
+        L = D.lib()
+        L.dmrgx_matrix_dict_values.argtypes = [C.c_void_p, C.POINTER(C.c_int64)]
+        nv = C.c_int64()
+        D.check(L.dmrgx_matrix_dict_values(A.handle, C.byref(nv)))
+        _, _, val = A.download()
+        assert nv.value == np.unique(val.view(np.uint64)).size
+        x = np.random.default_rng(11).standard_normal(A.n)
+        A.set_schedule(16, 8192, 2)
+        y0 = A.matvec(x)
+        A.set_schedule(16, 8192, 10)
+        assert np.array_equal(A.matvec(x), y0)
+    finally:
+        A.free()
+
+
+@pytest.mark.gpu
+def test_gpu_value_dictionary_ed_and_engine():
+    """ED matrices (a handful of distinct values) through the dictionary, single and batched,
+    bit-identical to the plain form."""
+    from dmrg_c_amd import ed, engine
+    mats = []
+    for m in (ed.Model("xxz", L=16, sz=0, J_z=0.7, h_z=0.2), ed.Model("hubbard", L=8, N=8, sz=0, U=3.0)):
+        P = ed.EdProblem.from_model(m)
+        mats.append(P.assemble())
+        P.free()
+    try:
+        plain = []
+        for A in mats:
+            g, cap, _ = A.schedule
+            A.set_schedule(g, cap, 0)
+            plain.append(A.ground_state())
+            A.set_schedule(g, cap, 8)
+        dict_single = [A.ground_state() for A in mats]
+        batch = engine.ground_state_batch(mats)
+        for p, d, b in zip(plain, dict_single, batch):
+            assert p[0] == d[0] == b[0]
+            assert np.array_equal(p[1], d[1]) and np.array_equal(p[1], b[1])
+    finally:
+        for A in mats:
+            A.free()
