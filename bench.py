@@ -161,6 +161,19 @@ def spmv_sweep(D, sb, snap):
     print(json.dumps({"spmv_sweep": rows, "n": n, "alg_bytes": alg}))
 
 
+def spmv_alg_bytes(D, A, sched, n):
+    """Algorithmic bytes of one product in the format the schedule reads: CSR (8-byte
+    values, 4-byte columns) or the value dictionary (8-byte packed entries + the table)."""
+    if sched[2] & 8:
+        L = D.lib()
+        L.dmrgx_matrix_dict_values.argtypes = [C.c_void_p, C.POINTER(C.c_int64)]
+        nv = C.c_int64()
+        D.check(L.dmrgx_matrix_dict_values(A.handle, C.byref(nv)))
+        if nv.value > 0:
+            return A.nnz * 8 + nv.value * 8 + (n + 1) * 8 + 2 * n * 8, f"value dictionary ({nv.value} values)"
+    return A.nnz * 12 + (n + 1) * 8 + 2 * n * 8, "CSR"
+
+
 def traffic_probe(D, L, sb, snap):
     L.dmrgx_device_alloc.argtypes = [C.c_int64, C.POINTER(C.c_void_p)]
     L.dmrgx_device_free.argtypes = [C.c_void_p]
@@ -179,8 +192,9 @@ def traffic_probe(D, L, sb, snap):
     for _ in range(5):
         D.check(L.dmrgx_spmv_device(A.handle, x, y))
     D.check(L.dmrgx_copy_device(src, dst, ncopy))
-    alg = A.nnz * 12 + (n + 1) * 8 + 2 * n * 8
-    print(json.dumps({"spmv_launches": 5, "n": n, "nnz": A.nnz, "spmv_alg_bytes": alg, "schedule": sched,
+    alg, fmt = spmv_alg_bytes(D, A, sched, n)
+    print(json.dumps({"spmv_launches": 5, "n": n, "nnz": A.nnz, "spmv_alg_bytes": alg, "format": fmt,
+                      "schedule": sched,
                       "copy_doubles": ncopy, "copy_bytes_read": 8 * ncopy, "copy_bytes_written": 8 * ncopy}))
     for p in bufs:
         L.dmrgx_device_free(p)
@@ -449,7 +463,8 @@ def main():
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_apply<CsrRows<double>> (CSR SpMV + fused Lanczos/CG epilogue)",
+            "kernel": ("k_apply<DictRows> (value-dictionary CRS SpMV + fused Lanczos/CG epilogue)" if lanes[2] & 8
+                       else "k_apply<CsrRows<double>> (CSR SpMV + fused Lanczos/CG epilogue)"),
             "achieved": round(achieved, 1),
             "peak": PEAK_HBM_GBS,
             "unit": "GB/s",
