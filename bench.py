@@ -158,6 +158,11 @@ def spmv_sweep(D, sb, snap):
         log(f"sweep: variant {v} done")
     A.free()
     rows.sort(key=lambda r: r["us"])
+    best = {}
+    for r in rows:
+        best.setdefault(r["variant"], r)
+    for v in sorted(best):
+        log(f"sweep: best of variant {v:2d}: {best[v]}")
     print(json.dumps({"spmv_sweep": rows, "n": n, "alg_bytes": alg}))
 
 
@@ -218,7 +223,7 @@ def ed_batch_models(count=256, seed=0):
     return out
 
 
-def run_ed_batch(D, count, precision_forms=(0, 1)):
+def run_ed_batch(D, count, precision_forms=(0, 1), single=True):
     from dmrg_c_amd import ed, engine
     models = ed_batch_models(count)
     t0 = time.perf_counter()
@@ -239,19 +244,23 @@ def run_ed_batch(D, count, precision_forms=(0, 1)):
     for prec in precision_forms:
         p = dict(params, precision=prec)
         t0 = time.perf_counter()
-        es = [A.ground_state(**p)[0] for A in mats]
+        es = [A.ground_state(**p)[0] for A in mats] if single else None
         t_single = time.perf_counter() - t0
         t0 = time.perf_counter()
         out = engine.ground_state_batch(mats, engine.default_params(**p), want_vec=False)
         t_batch = time.perf_counter() - t0
         eb = [o[0] for o in out]
+        tag = "fp64" if prec == 0 else "fp32_refined"
+        if not single:
+            res[tag] = {"batched_solves_per_s": round(count / t_batch, 2)}
+            log(f"ed-batch {tag}: batched {count / t_batch:.1f}/s")
+            continue
         same = all(a == b for a, b in zip(es, eb))
         if prec == 0:
             single_e = es
             err = 0.0
         else:
             err = max(abs(a - b) / abs(b) for a, b in zip(eb, single_e))
-        tag = "fp64" if prec == 0 else "fp32_refined"
         res[tag] = {"single_solves_per_s": round(count / t_single, 2), "batched_solves_per_s": round(count / t_batch, 2),
                     "batched_over_single": round(t_single / t_batch, 2), "batched_equals_single": same,
                     "max_rel_energy_diff_vs_fp64": err}
@@ -356,6 +365,8 @@ def main():
     ap.add_argument("--rate", type=float, default=200.0)
     ap.add_argument("--duration", type=float, default=10.0)
     ap.add_argument("--ed-count", type=int, default=256)
+    ap.add_argument("--ed-batched-only", action="store_true",
+                    help="ed-batch: skip the one-by-one solves (for profiling the engine alone)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the superblock eigensolve runs on a single MI355X "
@@ -369,7 +380,7 @@ def main():
     D.check(L.dmrgx_init(0))
     if args.workload == "ed-batch":
         print(json.dumps({"metric": "batched ED ground-state solves/s", "workload": "ed-batch",
-                          "result": run_ed_batch(D, args.ed_count)}), flush=True)
+                          "result": run_ed_batch(D, args.ed_count, single=not args.ed_batched_only)}), flush=True)
         return
     if args.workload == "serve":
         print(json.dumps({"metric": "served solves/s and p50/p99 latency", "workload": "serve",
