@@ -502,6 +502,7 @@ int dmrgx_matrix_autotune(dmrgx_matrix *A, int32_t *lanes_out, int32_t *grid_out
     TuneEnv env(A->n);
     const int base = pick_lanes(A->n, A->nnz);
     const int caps[] = {1024, 2048, 4096, 8192, 16384, 0};
+    const int g_lo = std::max(1, base / 16), g_hi = std::min(64, base * 2);
     struct Cand {
       int g, cap, v;
       double us;
@@ -510,7 +511,7 @@ int dmrgx_matrix_autotune(dmrgx_matrix *A, int32_t *lanes_out, int32_t *grid_out
     const bool dict = ensure_dict(*A);
     for (int v = 0; v < kLoadVariants; v++)
       if (dict || !(v & kLoadDict))
-        for (int g = std::max(1, base / 4); g <= std::min(64, base * 2); g *= 2)
+        for (int g = g_lo; g <= g_hi; g *= 2)
           for (int cap : caps) cands.push_back({g, cap, v, env.time(A, g, cap, v, 3)});
     // the leaders again with more launches, so that noise does not pick the schedule
     std::sort(cands.begin(), cands.end(), [](const Cand &a, const Cand &b) { return a.us < b.us; });

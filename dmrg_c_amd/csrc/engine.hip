@@ -171,6 +171,18 @@ __global__ __launch_bounds__(kBlock) void k_seg_cg_update(const SegDesc *desc, c
   block_partial(s, partial);
 }
 
+// One block per piece: copies count 8-byte words from src to dst + off, so that the host
+// reads every slot's new scalars with one device-to-host copy.
+struct GatherDesc {
+  const double *src;
+  int64_t off;
+  int32_t count, pad;
+};
+__global__ __launch_bounds__(kBlock) void k_gather_words(const GatherDesc *desc, double *dst) {
+  const GatherDesc d = desc[blockIdx.x];
+  for (int i = threadIdx.x; i < d.count; i += kBlock) dst[d.off + i] = d.src[i];
+}
+
 __global__ __launch_bounds__(kBlock) void k_seg_cg_dir(const SegDesc *desc, const int32_t *blk_seg) {
   const int seg = blk_seg[blockIdx.x];
   const SegDesc d = desc[seg];
@@ -256,6 +268,8 @@ struct dmrgx_engine {
   dmrgx::DevBuf<dmrgx::SegDesc> ddesc;
   dmrgx::DevBuf<int32_t> dblk;
   dmrgx::DevBuf<double> partial;
+  dmrgx::DevBuf<dmrgx::GatherDesc> dgather;  // per-chunk scalar read-back
+  dmrgx::DevBuf<double> dstage;
 };
 
 namespace dmrgx {
@@ -658,17 +672,26 @@ void run_chunk(dmrgx_engine &E, int ticks) {
   std::vector<size_t> off(L.size() + Cg.size() + 1, 0);
   for (size_t i = 0; i < L.size(); i++) off[i + 1] = off[i] + 2 * (size_t)(L[i]->k_next - L[i]->k_read);
   for (size_t i = 0; i < Cg.size(); i++) off[L.size() + i + 1] = off[L.size() + i] + 1;  // {done, it} in 8 bytes
-  double *h = ctx().host_stage(std::max<size_t>(off.back(), 1));
+  std::vector<GatherDesc> gd;
   for (size_t i = 0; i < L.size(); i++) {
     const Slot &s = *L[i];
-    const size_t c = (size_t)(s.k_next - s.k_read);
+    const int32_t c = s.k_next - s.k_read;
     if (c == 0) continue;
-    HIPCHK(hipMemcpyAsync(h + off[i], s.scal.p + s.k_read, sizeof(double) * c, hipMemcpyDeviceToHost, st));
-    HIPCHK(hipMemcpyAsync(h + off[i] + c, s.scal.p + s.KS + s.k_read + 1, sizeof(double) * c,
-                          hipMemcpyDeviceToHost, st));
+    gd.push_back({s.scal.p + s.k_read, (int64_t)off[i], c, 0});
+    gd.push_back({s.scal.p + s.KS + s.k_read + 1, (int64_t)(off[i] + c), c, 0});
   }
   for (size_t i = 0; i < Cg.size(); i++)
-    HIPCHK(hipMemcpyAsync(h + off[L.size() + i], Cg[i]->cgst.p, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
+    gd.push_back({reinterpret_cast<const double *>(Cg[i]->cgst.p), (int64_t)off[L.size() + i], 1, 0});
+  const size_t words = std::max<size_t>(off.back(), 1);
+  double *h = ctx().host_stage(words);
+  if (!gd.empty()) {
+    if (E.dgather.count < gd.size()) E.dgather.resize(std::max(gd.size(), 2 * E.dgather.count));
+    if (E.dstage.count < words) E.dstage.resize(std::max(words, 2 * E.dstage.count));
+    HIPCHK(hipMemcpyAsync(E.dgather.p, gd.data(), sizeof(GatherDesc) * gd.size(), hipMemcpyHostToDevice, st));
+    k_gather_words<<<(int)gd.size(), kBlock, 0, st>>>(E.dgather.p, E.dstage.p);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(h, E.dstage.p, sizeof(double) * off.back(), hipMemcpyDeviceToHost, st));
+  }
   HIPCHK(hipStreamSynchronize(st));  // host descriptor vectors go out of scope; scalars read next
   // the staging area is shared: keep a copy before the phase changes below reuse it
   const std::vector<double> stage(h, h + off.back());

@@ -620,8 +620,10 @@ int pick_lanes(int32_t n, int64_t nnz) {
 void default_schedule(dmrgx_matrix &A) {
   A.lanes = pick_lanes(A.n, A.nnz);
   A.grid = kGridCap;
-  // large matrices read the value dictionary (built at the first product) when it pays
-  A.load = A.nnz >= (int64_t(1) << 20) ? kLoadDict : 0;
+  // eight loads in flight per lane; the value dictionary only where the tuner finds it
+  // faster (its table lookups cost more than the 4 bytes per entry they save on MI355X for
+  // the superblocks measured)
+  A.load = kLoadDeep;
 }
 
 int spmv_blocks(int32_t n, int lanes, int grid_cap) {

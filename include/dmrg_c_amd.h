@@ -476,8 +476,8 @@ int dmrgx_spmv_host(const dmrgx_matrix *A, const double *x, double *y);
  * matrix's distinct values) instead of 4-byte columns and 8-byte values -- built at the
  * first such product when the matrix has few distinct values, else ignored).  Bits 0, 1
  * and 3 never change results; lanes change the summation order within a row, and cap and
- * bit 2 the order of the dot products fused into the solvers' products.  Defaults come
- * from the mean row length (with bit 3 for matrices of 2^20 entries or more);
+ * bit 2 the order of the dot products fused into the solvers' products.  Defaults: lanes
+ * from the mean row length, cap 4096, variant 2;
  * dmrgx_matrix_autotune times the candidates on the device (product + reduction, as the
  * solvers run it) and keeps the fastest.  Single and batched solves use the matrix's
  * schedule. */
