@@ -149,7 +149,7 @@ def spmv_sweep(D, sb, snap):
     n = snap.n
     alg = A.nnz * 12 + (n + 1) * 8 + 2 * n * 8
     rows = []
-    for v in range(16):
+    for v in range(24):
         for g in (4, 8, 16, 32, 64):
             for cap in (1024, 2048, 4096, 8192, 16384, 0):
                 us = A.time_schedule(g, cap, v, 20)
@@ -169,7 +169,10 @@ def spmv_sweep(D, sb, snap):
 def spmv_alg_bytes(D, A, sched, n):
     """Algorithmic bytes of one product in the format the schedule reads: CSR (8-byte
     values, 4-byte columns) or the value dictionary (8-byte packed entries + the table)."""
-    if sched[2] & 8:
+    if (sched[2] >> 3) == 2:
+        return (A.nnz * 10 + (n + 1) * 8 + n * 4 + 2 * n * 8,
+                "delta columns (8-byte values, 2-byte column differences, 4-byte first columns)")
+    if (sched[2] >> 3) == 1:
         L = D.lib()
         L.dmrgx_matrix_dict_values.argtypes = [C.c_void_p, C.POINTER(C.c_int64)]
         nv = C.c_int64()
@@ -233,13 +236,27 @@ def run_ed_batch(D, count, precision_forms=(0, 1), single=True):
         mats.append(P.assemble())
         P.free()
     t_asm = time.perf_counter() - t0
+    # SpMV schedules: the problems of one model and length share their row statistics, so
+    # the first large one of each is autotuned and the others take its schedule (setup, as
+    # a caller sweeping couplings would do; not timed)
+    tuned = {}
+    for m, A in zip(models, mats):
+        key = (m.model, m.L)
+        if A.n < 50000:
+            continue
+        if key not in tuned:
+            tuned[key] = A.autotune()[0]
+            log(f"ed-batch: schedule for {key} (n={A.n}): {tuned[key]}")
+        else:
+            A.set_schedule(*tuned[key])
     dims = [A.n for A in mats]
     log(f"ed-batch: {count} problems, n from {min(dims)} to {max(dims)}, total rows {sum(dims)}, "
         f"device assembly {t_asm:.2f} s")
     params = dict(diag_acc=1e-14, diag_max_step=800, cg_acc=1e-7, cg_max_step=2000, ii_acc=1e-9,
                   ii_diag_add=1e-11, ii_max_step=3, seed=1)
     res = {"problems": count, "n_min": min(dims), "n_max": max(dims), "rows_total": int(sum(dims)),
-           "device_assembly_s": round(t_asm, 3)}
+           "device_assembly_s": round(t_asm, 3),
+           "schedules": {f"{k[0]} L={k[1]}": list(v) for k, v in tuned.items()}}
     single_e = None
     for prec in precision_forms:
         p = dict(params, precision=prec)
@@ -463,7 +480,7 @@ def main():
                               "nontemporal_matrix_loads": bool(lanes[2] & 1),
                               "loads_in_flight_per_lane": 8 if lanes[2] & 2 else 4,
                               "xcd_contiguous_rows": bool(lanes[2] & 4),
-                              "value_dictionary": bool(lanes[2] & 8)},
+                              "format": ("CSR", "value dictionary", "delta columns")[lanes[2] >> 3]},
         },
         "forms": {
             "fp32_refined": {"solves_per_s": round(len(mlat) / sum(mlat), 4),
@@ -474,8 +491,9 @@ def main():
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": ("k_apply<DictRows> (value-dictionary CRS SpMV + fused Lanczos/CG epilogue)" if lanes[2] & 8
-                       else "k_apply<CsrRows<double>> (CSR SpMV + fused Lanczos/CG epilogue)"),
+            "kernel": ("k_apply<CsrRows<double>> (CSR SpMV + fused Lanczos/CG epilogue)",
+                       "k_apply<DictRows> (value-dictionary CRS SpMV + fused Lanczos/CG epilogue)",
+                       "k_apply<DeltaRows<double>> (delta-column CRS SpMV + fused Lanczos/CG epilogue)")[lanes[2] >> 3],
             "achieved": round(achieved, 1),
             "peak": PEAK_HBM_GBS,
             "unit": "GB/s",

@@ -428,7 +428,7 @@ int dmrgx_matrix_get_schedule(const dmrgx_matrix *A, int32_t *lanes, int32_t *gr
 
 // SpMV schedule autotuner: times y = A x with a fused dot product (HIP events on the library
 // stream) for each power-of-two lanes-per-row count around the mean row length, each
-// workgroup cap and each load variant (the dictionary ones if A has a value dictionary),
+// workgroup cap and each load variant (in the formats A can have),
 // re-times the six fastest with more launches, and keeps the fastest.
 namespace {
 // Mean time of y = A x with a fused dot product under schedule (g, cap, v), as the solvers
@@ -508,9 +508,9 @@ int dmrgx_matrix_autotune(dmrgx_matrix *A, int32_t *lanes_out, int32_t *grid_out
       double us;
     };
     std::vector<Cand> cands;
-    const bool dict = ensure_dict(*A);
+    const bool has[3] = {true, ensure_dict(*A), ensure_delta(*A)};
     for (int v = 0; v < kLoadVariants; v++)
-      if (dict || !(v & kLoadDict))
+      if (has[load_format(v)])
         for (int g = g_lo; g <= g_hi; g *= 2)
           for (int cap : caps) cands.push_back({g, cap, v, env.time(A, g, cap, v, 3)});
     // the leaders again with more launches, so that noise does not pick the schedule

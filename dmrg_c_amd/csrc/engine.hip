@@ -47,8 +47,10 @@ struct SegDesc {
   const int32_t *col;
   const double *val;
   const float *val32;
-  const uint64_t *ent;    // value dictionary form (load & kLoadDict, FP64 products)
+  const uint64_t *ent;    // value dictionary form (FP64 products)
   const double *table;
+  const int32_t *first;   // delta form
+  const uint16_t *dcol;
   int32_t n, fp32;
   int32_t blk_lo, nblk;   // this slot's blocks in the launch
   int32_t k, load;        // step (Lanczos) or iteration (CG); SpMV load variant
@@ -100,16 +102,24 @@ __global__ __launch_bounds__(kBlock) void k_seg_spmv(const SegDesc *desc, const 
   // logical block lb is at blk_lo + lb, where the finish kernel sums it in lb order)
   const int32_t lb = blockIdx.x - d.blk_lo;
   const GroupRange gr = group_range(lb, d.nblk, (d.n + RPB - 1) / RPB, (d.load & kLoadXcd) != 0);
+  const int fmt = load_format(d.load);
   for (int32_t gi = gr.start; gi < gr.end; gi += gr.step) {
     const int32_t r = gi * RPB + g;
     double s = 0.0;
     if (r < d.n) {
       switch (d.load & (kLoadXcd - 1)) {
-#define V_CASE(V)                                                              \
-  case V:                                                                      \
-    if (d.fp32) s = csr_lane_sum_v<G, V>(d.row, d.col, d.val32, r, lane, d.x);   \
-    else if (d.load & kLoadDict) s = dict_lane_sum_v<G, V>(d.row, d.ent, d.table, r, lane, d.x); \
-    else s = csr_lane_sum_v<G, V>(d.row, d.col, d.val, r, lane, d.x);         \
+#define V_CASE(V)                                                                              \
+  case V:                                                                                      \
+    if (fmt == kFmtDelta) {                                                                    \
+      if (d.fp32) s = delta_lane_sum_v<G, V>(d.row, d.first, d.dcol, d.val32, r, lane, d.x);   \
+      else s = delta_lane_sum_v<G, V>(d.row, d.first, d.dcol, d.val, r, lane, d.x);          \
+    } else if (d.fp32) {                                                                       \
+      s = csr_lane_sum_v<G, V>(d.row, d.col, d.val32, r, lane, d.x);                           \
+    } else if (fmt == kFmtDict) {                                                              \
+      s = dict_lane_sum_v<G, V>(d.row, d.ent, d.table, r, lane, d.x);                          \
+    } else {                                                                                   \
+      s = csr_lane_sum_v<G, V>(d.row, d.col, d.val, r, lane, d.x);                             \
+    }                                                                                          \
     break;
         V_CASE(0) V_CASE(1) V_CASE(2) V_CASE(3)
 #undef V_CASE
@@ -284,10 +294,8 @@ size_t slot_bytes(const dmrgx_engine &E, const Slot &s) {
   return b;
 }
 
-// The slot matrix's load variant, without the dictionary bit when it has no dictionary.
-int slot_load(const Slot &s) {
-  return (s.A->load & kLoadDict) && s.A->n_values <= 0 ? (s.A->load & ~kLoadDict) : s.A->load;
-}
+// The slot matrix's load variant, restricted to the formats the matrix has.
+int slot_load(const Slot &s) { return effective_load(*s.A); }
 
 // Start a slot: start vector and |w|^2 (solver.hip:lanczos before its first step).
 void start_slot(Slot &s) {
@@ -305,7 +313,8 @@ void start_slot(Slot &s) {
   s.hb.assign(s.KS, 0.0);
   s.s.assign(s.KS, 0.0);
   if (s.mixed) ensure_fp32(*s.A);
-  if (s.A->load & kLoadDict) ensure_dict(*s.A);
+  if (load_format(s.A->load) == kFmtDict) ensure_dict(*s.A);
+  if (load_format(s.A->load) == kFmtDelta) ensure_delta(*s.A);
   launch_start_vector(n, s.p.seed, s.vec.p, st);
   launch_copy(n, s.vec.p, s.w.p, st);
   Reduce R;
@@ -556,6 +565,8 @@ void run_chunk(dmrgx_engine &E, int ticks) {
         a.load = slot_load(s);
         a.ent = s.A->ent.p;
         a.table = s.A->table.p;
+        a.first = s.A->first.p;
+        a.dcol = s.A->dcol.p;
         a.k = k;
         a.shift = 0.0;
         a.x = s.w.p;
@@ -599,6 +610,8 @@ void run_chunk(dmrgx_engine &E, int ticks) {
         a.load = slot_load(s);
         a.ent = s.A->ent.p;
         a.table = s.A->table.p;
+        a.first = s.A->first.p;
+        a.dcol = s.A->dcol.p;
         a.shift = s.p.ii_diag_add - s.eig;
         a.x = s.pv.p;
         a.o1 = s.cy.p;

@@ -108,6 +108,11 @@ struct dmrgx_matrix {
   dmrgx::DevBuf<uint64_t> ent;
   dmrgx::DevBuf<double> table;
   int64_t n_values = -1;        // distinct values; -1 not built yet, 0 not worth building
+  // Delta form (built on first use): first[r] = first column of row r, dcol[j] = col[j] -
+  // col[j-1] within a row (0 at a row start); only when every difference fits 16 bits.
+  dmrgx::DevBuf<int32_t> first;
+  dmrgx::DevBuf<uint16_t> dcol;
+  int delta = -1;               // -1 not built yet, 0 not available, 1 built
   int lanes = 8;                // lanes per row of the SpMV kernel
   int grid = dmrgx::kGridCap;   // cap on SpMV workgroups (0: one per row group)
   int load = 0;                 // load variant (kLoad* bits)
@@ -246,8 +251,17 @@ __device__ __forceinline__ double csr_lane_sum(const int64_t *__restrict__ row, 
 constexpr int kLoadNT = 1;
 constexpr int kLoadDeep = 2;
 constexpr int kLoadXcd = 4;
-constexpr int kLoadDict = 8;   // FP64 products read the value dictionary (same values, same order)
-constexpr int kLoadVariants = 16;
+// Bits 3-4: storage format of the product.  CSR: 4-byte columns; dictionary (FP64 products
+// only): 8-byte (column, value index) entries over a table of the distinct values; delta:
+// 2-byte column differences within each row plus each row's first column.  Every format
+// gives the CSR products bit for bit (same values, same order).
+constexpr int kFmtShift = 3;
+constexpr int kFmtCsr = 0, kFmtDict = 1, kFmtDelta = 2;
+constexpr int kLoadDict = kFmtDict << kFmtShift;
+constexpr int kLoadDelta = kFmtDelta << kFmtShift;
+constexpr int kLoadFmtMask = 3 << kFmtShift;
+constexpr int kLoadVariants = 3 << kFmtShift;   // 0 .. 23
+__host__ __device__ inline int load_format(int load) { return (load & kLoadFmtMask) >> kFmtShift; }
 
 // Row groups [start, end) with stride step that logical block lb of nblk takes out of
 // ngroups: round-robin (lb, lb + nblk, ...) or one contiguous range.
@@ -300,6 +314,59 @@ __device__ __forceinline__ double dict_lane_sum(const int64_t *__restrict__ row,
       if (p[u]) s += v[u] * xv[u];
   }
   return s;
+}
+
+// The same sum over the delta form: per pass each lane loads its U 2-byte column differences
+// and values, the G lanes of the row turn the differences into columns with a prefix sum
+// (carried from pass to pass from the row's first column), then gather x.  All G lanes of
+// a row run the same passes (the shuffles need them); the additions are in the order of
+// csr_lane_sum, so the result equals it bit for bit.
+template <int G, int U, bool NT, class VT>
+__device__ __forceinline__ double delta_lane_sum(const int64_t *__restrict__ row, const int32_t *__restrict__ first,
+                                                 const uint16_t *__restrict__ dcol, const VT *__restrict__ val,
+                                                 int32_t r, int lane, const double *__restrict__ x) {
+  double s = 0.0;
+  const int64_t b = row[r], e = row[r + 1];
+  if (b >= e) return s;
+  const int64_t last = e - 1;
+  int32_t carry = first[r];
+  for (int64_t j0 = b; j0 < e; j0 += U * G) {
+    int32_t c[U];
+    VT v[U];
+    bool p[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int64_t j = j0 + u * G + lane;
+      p[u] = j < e;
+      const int64_t jj = p[u] ? j : last;
+      c[u] = p[u] ? (int32_t)stream_load<NT>(dcol + jj) : 0;
+      v[u] = stream_load<NT>(val + jj);
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+#pragma unroll
+      for (int off = 1; off < G; off <<= 1) {
+        const int32_t t = __shfl_up(c[u], off, G);
+        if (lane >= off) c[u] += t;
+      }
+      c[u] += carry;
+      carry = __shfl(c[u], G - 1, G);
+    }
+    double xv[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) xv[u] = x[c[u]];
+#pragma unroll
+    for (int u = 0; u < U; u++)
+      if (p[u]) s += (double)v[u] * xv[u];
+  }
+  return s;
+}
+
+template <int G, int V, class VT>
+__device__ __forceinline__ double delta_lane_sum_v(const int64_t *__restrict__ row, const int32_t *__restrict__ first,
+                                                   const uint16_t *__restrict__ dcol, const VT *__restrict__ val,
+                                                   int32_t r, int lane, const double *__restrict__ x) {
+  return delta_lane_sum<G, (V & kLoadDeep) ? 8 : 4, (V & kLoadNT) != 0>(row, first, dcol, val, r, lane, x);  // V < 4
 }
 
 template <int G, int V>
@@ -413,6 +480,8 @@ void launch_expand_lower(int32_t n, int64_t nnz_tri, const int64_t *trow, const 
 void launch_to_fp32(const dmrgx_matrix &A, hipStream_t s);
 // value dictionary of A (ent, table); returns the number of distinct values, 0 if none built
 int64_t build_value_dict(dmrgx_matrix &A, hipStream_t s);
+// delta form of A (first, dcol); returns false if a column difference needs more than 16 bits
+bool build_delta(dmrgx_matrix &A, hipStream_t s);
 void launch_start_vector(int64_t n, uint64_t seed, double *out, hipStream_t s);
 
 // Lanczos step kernels.  Scalars live in device arrays indexed by step.
@@ -508,6 +577,10 @@ void ensure_fp32(const dmrgx_matrix &A);
 // Builds A's value dictionary if it has not been tried; true if A has one (a matrix with
 // more distinct values than a table worth caching keeps the plain form).
 bool ensure_dict(const dmrgx_matrix &A);
+// Builds A's delta form if it has not been tried; true if A has one.
+bool ensure_delta(const dmrgx_matrix &A);
+// A's load variant restricted to the formats it has (a format it cannot have reads CSR).
+int effective_load(const dmrgx_matrix &A);
 
 // device matrix handed over by dmrgx_get_ham_lllrrrrl for the CRS1 it returned (assemble.hip)
 dmrgx_matrix *take_pending(const CRS1 *M);

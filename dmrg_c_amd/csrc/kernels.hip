@@ -66,6 +66,25 @@ struct DictRows {
   }
 };
 
+template <class VT, int V>
+struct DeltaRows {
+  const int64_t *__restrict__ row;
+  const int32_t *__restrict__ first;
+  const uint16_t *__restrict__ dcol;
+  const VT *__restrict__ val;
+  int64_t nnz;
+  static constexpr bool kTimed = true;
+  // algorithmic bytes of one y = H x: values + 2-byte differences + row pointers + first
+  // columns + x + y
+  double alg_bytes(int32_t n) const {
+    return (double)nnz * (sizeof(VT) + 2) + (double)(n + 1) * 8 + (double)n * 4 + 2.0 * n * 8;
+  }
+  template <int G>
+  __device__ __forceinline__ double lane_sum(int32_t r, int lane, const double *__restrict__ x) const {
+    return delta_lane_sum_v<G, V>(row, first, dcol, val, r, lane, x);
+  }
+};
+
 // Row groups of kBlock/G rows are dealt to the blocks (the grid may be capped, so a block
 // takes several groups) round-robin, or with xcd set in contiguous ranges over XCD-mapped
 // logical blocks; each thread's epilogue contributions are summed in the order of its
@@ -252,6 +271,21 @@ void apply_dict(const Operator &op, const double *x, const Epi &epi, Reduce *R, 
 }
 
 template <class VT, class Epi>
+void apply_delta(const Operator &op, const VT *val, const double *x, const Epi &epi, Reduce *R, hipStream_t s) {
+  const dmrgx_matrix &A = *op.A;
+  const int xcd = (op.load & kLoadXcd) ? 1 : 0;
+  switch (op.load & (kLoadXcd - 1)) {
+#define V_CASE(V)                                                                                        \
+  case V:                                                                                                \
+    apply_rows(op.n, op.lanes, op.grid, DeltaRows<VT, V>{A.row.p, A.first.p, A.dcol.p, val, A.nnz}, x, epi, R, \
+               s, xcd);                                                                                  \
+    break;
+    V_CASE(0) V_CASE(1) V_CASE(2) V_CASE(3)
+#undef V_CASE
+  }
+}
+
+template <class VT, class Epi>
 void apply_csr(const Operator &op, const VT *val, const double *x, const Epi &epi, Reduce *R, hipStream_t s) {
   const dmrgx_matrix &A = *op.A;
   const int xcd = (op.load & kLoadXcd) ? 1 : 0;
@@ -270,13 +304,20 @@ template <class Epi>
 void apply_op(const Operator &op, bool fp32, const double *x, const Epi &epi, Reduce *R, hipStream_t s) {
   if (op.obc) {
     apply_rows(op.n, op.lanes, op.grid, op.obc->rows(), x, epi, R, s);
-  } else if (fp32) {
-    if (!op.A->val32.p) throw Failure(DMRGX_ERR_ARG, "FP32 values not prepared");
-    apply_csr<float>(op, op.A->val32.p, x, epi, R, s);
-  } else if ((op.load & kLoadDict) && ensure_dict(*op.A)) {
-    apply_dict(op, x, epi, R, s);
   } else {
-    apply_csr<double>(op, op.A->val.p, x, epi, R, s);
+    const int fmt = load_format(op.load);
+    const bool delta = fmt == kFmtDelta && ensure_delta(*op.A);
+    if (fp32) {
+      if (!op.A->val32.p) throw Failure(DMRGX_ERR_ARG, "FP32 values not prepared");
+      if (delta) apply_delta<float>(op, op.A->val32.p, x, epi, R, s);
+      else apply_csr<float>(op, op.A->val32.p, x, epi, R, s);
+    } else if (delta) {
+      apply_delta<double>(op, op.A->val.p, x, epi, R, s);
+    } else if (fmt == kFmtDict && ensure_dict(*op.A)) {
+      apply_dict(op, x, epi, R, s);
+    } else {
+      apply_csr<double>(op, op.A->val.p, x, epi, R, s);
+    }
   }
 }
 
@@ -938,6 +979,39 @@ int64_t build_value_dict(dmrgx_matrix &A, hipStream_t s) {
     if (ovf) throw Failure(DMRGX_ERR_HIP, "value dictionary lookup failed after a complete build");
     return nv;
   }
+}
+
+// ---------------------------------------------------------------- delta form
+// One wavefront per row: first column, then the differences of consecutive columns.
+__global__ __launch_bounds__(kBlock) void k_delta_build(int32_t n, const int64_t *__restrict__ row,
+                                                        const int32_t *__restrict__ col, int32_t *__restrict__ first,
+                                                        uint16_t *__restrict__ dcol, int *too_wide) {
+  const int32_t r = blockIdx.x * (kBlock / 64) + threadIdx.x / 64;
+  const int lane = threadIdx.x % 64;
+  if (r >= n) return;
+  const int64_t b = row[r], e = row[r + 1];
+  if (lane == 0) first[r] = b < e ? col[b] : 0;
+  for (int64_t j = b + lane; j < e; j += 64) {
+    const int32_t d = j == b ? 0 : col[j] - col[j - 1];
+    if (d < 0 || d > 65535) *too_wide = 1;
+    dcol[j] = (uint16_t)(d < 0 || d > 65535 ? 0 : d);
+  }
+}
+
+bool build_delta(dmrgx_matrix &A, hipStream_t s) {
+  if (A.n == 0) return false;
+  A.first.resize(A.n);
+  A.dcol.resize(std::max<int64_t>(A.nnz, 1));
+  DevBuf<int> flag(1);
+  HIPCHK(hipMemsetAsync(flag.p, 0, sizeof(int), s));
+  const int rows_per_block = kBlock / 64;
+  k_delta_build<<<(A.n + rows_per_block - 1) / rows_per_block, kBlock, 0, s>>>(A.n, A.row.p, A.col.p, A.first.p,
+                                                                               A.dcol.p, flag.p);
+  HIPCHK(hipGetLastError());
+  int wide = 0;
+  HIPCHK(hipMemcpyAsync(&wide, flag.p, sizeof(int), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  return wide == 0;
 }
 
 void launch_to_fp32(const dmrgx_matrix &A, hipStream_t s) {

@@ -202,6 +202,25 @@ bool ensure_dict(const dmrgx_matrix &A) {
   return A.n_values > 0;
 }
 
+bool ensure_delta(const dmrgx_matrix &A) {
+  if (A.delta < 0) {
+    // like the dictionary, a cache derived from the columns; the matrix is not changed
+    dmrgx_matrix &M = const_cast<dmrgx_matrix &>(A);
+    M.delta = build_delta(M, ctx().stream) ? 1 : 0;
+    if (!M.delta) {
+      M.first.reset();
+      M.dcol.reset();
+    }
+  }
+  return A.delta > 0;
+}
+
+int effective_load(const dmrgx_matrix &A) {
+  const int fmt = load_format(A.load);
+  const bool ok = fmt == kFmtCsr || (fmt == kFmtDict && A.n_values > 0) || (fmt == kFmtDelta && A.delta > 0);
+  return ok ? A.load : (A.load & ~kLoadFmtMask);
+}
+
 int cg_solve(const Operator &A, double shift, const double *b, double *x, double acc, int max_step,
              bool *converged, SolveOut &out) {
   Context &C = ctx();

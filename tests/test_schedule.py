@@ -1,7 +1,8 @@
 """SpMV schedules (dmrgx_matrix_set_schedule / _get_schedule / _autotune).
 
-The load bits of the variant (non-temporal loads, loads in flight per lane, the value
-dictionary) change no arithmetic, so the solve is bit-identical under them; lanes per row change the summation
+The load bits of the variant (non-temporal loads, loads in flight per lane) and its storage
+format (CSR, value dictionary, 16-bit column differences) change no arithmetic, so the
+solve is bit-identical under them; lanes per row change the summation
 order of each row, and the workgroup cap and the row-range bit the order of the fused dot
 products, so energies agree with the reference's to the FP64 bound under every schedule."""
 import os
@@ -27,11 +28,11 @@ def test_gpu_schedule_argument_checks():
     import dmrg_c_amd as D
     A, _ = _matrix()
     try:
-        for bad in ((0, 0, 0), (3, 0, 0), (128, 0, 0), (8, -1, 0), (8, 0, 16), (8, 0, -1)):
+        for bad in ((0, 0, 0), (3, 0, 0), (128, 0, 0), (8, -1, 0), (8, 0, 24), (8, 0, -1)):
             with pytest.raises(D.DmrgxError):
                 A.set_schedule(*bad)
-        A.set_schedule(16, 2048, 15)
-        assert A.schedule == (16, 2048, 15)
+        A.set_schedule(16, 2048, 23)
+        assert A.schedule == (16, 2048, 23)
     finally:
         A.free()
 
@@ -56,19 +57,19 @@ def test_gpu_load_variants_bit_identical():
         # capped grid: the load bits give the same bits within each row mapping
         for base in (0, 4):
             ref = _solves(A, (8, 4096, base), x)
-            for v in (1, 2, 3, 8, 9, 10, 11):
+            for v in (1, 2, 3, 8, 9, 10, 11, 16, 17, 18, 19):
                 _same(ref, _solves(A, (8, 4096, base | v), x))
             assert abs(ref[1][0] - e_ref) <= 1e-10 * abs(e_ref)
         # one workgroup per row group: the row mapping changes nothing either
         ref = _solves(A, (16, 0, 0), x)
-        for v in range(1, 16):
+        for v in range(1, 24):
             _same(ref, _solves(A, (16, 0, v), x))
     finally:
         A.free()
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("sched", [(1, 0, 0), (4, 1024, 5), (16, 0, 10), (32, 8192, 15), (64, 2048, 12), (8, 16384, 6)])
+@pytest.mark.parametrize("sched", [(1, 0, 0), (4, 1024, 5), (16, 0, 10), (32, 8192, 15), (64, 2048, 12), (8, 16384, 6), (2, 0, 18), (8, 2048, 23)])
 def test_gpu_schedules_agree(sched):
     A, e_ref = _matrix()
     try:
@@ -91,7 +92,7 @@ def test_gpu_autotune_sets_schedule():
         sched, us = A.autotune()
         assert A.schedule == sched and us > 0
         g, cap, v = sched
-        assert g & (g - 1) == 0 and 1 <= g <= 64 and cap >= 0 and v in range(16)
+        assert g & (g - 1) == 0 and 1 <= g <= 64 and cap >= 0 and v in range(24)
         E, _, _ = A.ground_state()
         assert abs(E - e_ref) <= 1e-10 * abs(e_ref)
     finally:
@@ -134,9 +135,10 @@ def test_gpu_value_dictionary():
 
 
 @pytest.mark.gpu
-def test_gpu_value_dictionary_ed_and_engine():
-    """ED matrices (a handful of distinct values) through the dictionary, single and batched,
-    bit-identical to the plain form."""
+@pytest.mark.parametrize("fmt", [8, 16], ids=["dictionary", "delta"])
+def test_gpu_formats_ed_and_engine(fmt):
+    """ED matrices through the value dictionary (a handful of distinct values) and the delta
+    form, single and batched, bit-identical to the plain form."""
     from dmrg_c_amd import ed, engine
     mats = []
     for m in (ed.Model("xxz", L=16, sz=0, J_z=0.7, h_z=0.2), ed.Model("hubbard", L=8, N=8, sz=0, U=3.0)):
@@ -149,12 +151,16 @@ def test_gpu_value_dictionary_ed_and_engine():
             g, cap, _ = A.schedule
             A.set_schedule(g, cap, 0)
             plain.append(A.ground_state())
-            A.set_schedule(g, cap, 8)
-        dict_single = [A.ground_state() for A in mats]
+            A.set_schedule(g, cap, fmt)
+        single = [A.ground_state() for A in mats]
         batch = engine.ground_state_batch(mats)
-        for p, d, b in zip(plain, dict_single, batch):
+        for p, d, b in zip(plain, single, batch):
             assert p[0] == d[0] == b[0]
             assert np.array_equal(p[1], d[1]) and np.array_equal(p[1], b[1])
+        m32 = [A.ground_state(precision=1) for A in mats]
+        b32 = engine.ground_state_batch(mats, engine.default_params(precision=1))
+        for d, b in zip(m32, b32):
+            assert d[0] == b[0] and np.array_equal(d[1], b[1])
     finally:
         for A in mats:
             A.free()
