@@ -64,6 +64,16 @@ struct SegDesc {
   int *status;            // CG: {done, last iteration}
 };
 
+// Segment (slot) of block b of a launch: map[b / kMapStride] is the segment of block
+// kMapStride * (b / kMapStride); the later ones start at desc[s].blk_lo, scanned forward.
+// The map costs one entry per kMapStride blocks instead of one per block.
+constexpr int kMapStride = 64;
+__device__ __forceinline__ int block_segment(const SegDesc *desc, const int32_t *map, int nseg, int b) {
+  int s = map[b / kMapStride];
+  while (s + 1 < nseg && desc[s + 1].blk_lo <= b) s++;
+  return s;
+}
+
 enum FinKind { FIN_LANCZOS_A = 0, FIN_CG_AP = 1, FIN_LANCZOS_B = 2, FIN_CG_UPDATE = 3 };
 
 // One block per segment: its partials summed in block order (the single-solve k_finish
@@ -89,8 +99,9 @@ __global__ __launch_bounds__(kBlock) void k_seg_finish(const SegDesc *desc, cons
 }
 
 template <int G, int MODE>
-__global__ __launch_bounds__(kBlock) void k_seg_spmv(const SegDesc *desc, const int32_t *blk_seg, double *partial) {
-  const int seg = blk_seg[blockIdx.x];
+__global__ __launch_bounds__(kBlock) void k_seg_spmv(const SegDesc *desc, const int32_t *map, int nseg,
+                                                    double *partial) {
+  const int seg = block_segment(desc, map, nseg, blockIdx.x);
   const SegDesc d = desc[seg];
   if (MODE == CG_AP && d.status[0] != 0) return;
   constexpr int RPB = kBlock / G;
@@ -145,9 +156,9 @@ __global__ __launch_bounds__(kBlock) void k_seg_spmv(const SegDesc *desc, const 
 }
 
 // Vector kernels: a slot gets vec_grid(n) blocks and the single-solve grid-stride pattern.
-__global__ __launch_bounds__(kBlock) void k_seg_lanczos_b(const SegDesc *desc, const int32_t *blk_seg,
+__global__ __launch_bounds__(kBlock) void k_seg_lanczos_b(const SegDesc *desc, const int32_t *map, int nseg,
                                                           double *partial) {
-  const int seg = blk_seg[blockIdx.x];
+  const int seg = block_segment(desc, map, nseg, blockIdx.x);
   const SegDesc d = desc[seg];
   const int lb = blockIdx.x - d.blk_lo;
   const double a = *d.s1;
@@ -163,9 +174,9 @@ __global__ __launch_bounds__(kBlock) void k_seg_lanczos_b(const SegDesc *desc, c
   block_partial(s, partial);
 }
 
-__global__ __launch_bounds__(kBlock) void k_seg_cg_update(const SegDesc *desc, const int32_t *blk_seg,
+__global__ __launch_bounds__(kBlock) void k_seg_cg_update(const SegDesc *desc, const int32_t *map, int nseg,
                                                           double *partial) {
-  const int seg = blk_seg[blockIdx.x];
+  const int seg = block_segment(desc, map, nseg, blockIdx.x);
   const SegDesc d = desc[seg];
   if (d.status[0] != 0) return;
   const int lb = blockIdx.x - d.blk_lo;
@@ -193,8 +204,8 @@ __global__ __launch_bounds__(kBlock) void k_gather_words(const GatherDesc *desc,
   for (int i = threadIdx.x; i < d.count; i += kBlock) dst[d.off + i] = d.src[i];
 }
 
-__global__ __launch_bounds__(kBlock) void k_seg_cg_dir(const SegDesc *desc, const int32_t *blk_seg) {
-  const int seg = blk_seg[blockIdx.x];
+__global__ __launch_bounds__(kBlock) void k_seg_cg_dir(const SegDesc *desc, const int32_t *map, int nseg) {
+  const int seg = block_segment(desc, map, nseg, blockIdx.x);
   const SegDesc d = desc[seg];
   if (d.status[0] != 0) return;
   const int lb = blockIdx.x - d.blk_lo;
@@ -226,7 +237,11 @@ struct SlotBasis {
 
 }  // namespace
 
-enum Phase { P_LANCZOS, P_CG, P_DONE };
+// P_WAIT: the slot's next step depends on a scalar its last launches compute (the L1
+// residual of an inverse-iteration round, or the mixed form's Rayleigh quotient); the
+// engine reads those scalars for all waiting slots together.
+enum Phase { P_LANCZOS, P_CG, P_WAIT, P_DONE };
+enum Wait { W_RESID = 1, W_RQ = 2 };
 
 struct Slot {
   int64_t id = 0;
@@ -236,6 +251,7 @@ struct Slot {
   dmrgx_solve_params p{};
   bool mixed = false;
   Phase phase = P_LANCZOS;
+  Wait wait = W_RESID;
   int status = DMRGX_OK;
   int32_t n = 0;
   int lanes = 8;
@@ -263,6 +279,8 @@ struct Slot {
   double *rr() { return scal.p + 2 * KS + 1; }
   double *pap() { return scal.p + 2 * KS + 1 + CS; }
   double *resid() { return scal.p + 2 * KS + 1 + 2 * CS; }
+  double *rq() { return scal.p + 2 * KS + 2 + 2 * CS; }   // <v, H v>, <v, v>
+  std::vector<const double *> ritz_ptrs;                   // kept until the slot ends
 };
 
 }  // namespace dmrgx
@@ -280,6 +298,17 @@ struct dmrgx_engine {
   dmrgx::DevBuf<double> partial;
   dmrgx::DevBuf<dmrgx::GatherDesc> dgather;  // per-chunk scalar read-back
   dmrgx::DevBuf<double> dstage;
+  // host seconds: building + uploading descriptors, launching, waiting for the chunk,
+  // Lanczos checks (with the Ritz/inverse-iteration starts they trigger), CG ends and
+  // inverse-iteration rounds, admission; then the number of chunks
+  double host_t[6] = {0, 0, 0, 0, 0, 0};
+  int64_t chunks = 0;
+  // pinned staging of the descriptor upload (reused once the chunk has synchronised)
+  void *pinned = nullptr;
+  size_t pinned_bytes = 0;
+  ~dmrgx_engine() {
+    if (pinned) (void)hipHostFree(pinned);
+  }
 };
 
 namespace dmrgx {
@@ -307,7 +336,7 @@ void start_slot(Slot &s) {
   s.V.n = n;
   s.KS = s.p.diag_max_step + 2;
   s.CS = std::max(s.p.cg_max_step, 1) + 1;
-  s.scal.resize(2 * (size_t)s.KS + 2 * (size_t)s.CS + 2);
+  s.scal.resize(2 * (size_t)s.KS + 2 * (size_t)s.CS + 4);
   s.cgst.resize(2);
   s.ha.assign(s.KS, 0.0);
   s.hb.assign(s.KS, 0.0);
@@ -354,39 +383,56 @@ void start_cg(Slot &s) {
   s.phase = P_CG;
 }
 
-// Inverse-iteration step after Lanczos or after a CG solve: residual check, then either
-// done or the next CG solve (solver.hip:inverse_iteration).  Synchronises.
-void ii_advance(Slot &s) {
-  hipStream_t st = ctx().stream;
-  const int64_t n = s.n;
-  Reduce R;
-  while (!s.p.skip_ii) {
-    if (s.ii_iter > s.p.ii_max_step) break;
-    launch_residual_l1(op_of(*s.A), s.vec.p, s.eig, s.resid(), R, st);
-    double *h = ctx().host_stage(1);
-    HIPCHK(hipMemcpyAsync(h, s.resid(), sizeof(double), hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
-    s.err = h[0];
-    s.out.spmv++;
-    if (s.err < s.p.ii_acc) {
-      s.out.ii_converged = true;
-      break;
-    }
-    start_cg(s);
-    return;
-  }
+// End of the inverse iteration (solver.hip:inverse_iteration after its loop): counters, and
+// for the mixed form the FP64 Rayleigh quotient, read back with the other waiting slots.
+void ii_finalize(Slot &s) {
   if (!s.p.skip_ii) {
     s.out.ii_steps = s.ii_iter;
     s.out.err = s.err;
   }
   if (s.mixed && s.n > 1) {
-    s.eig = rayleigh_quotient(op_of(*s.A), s.vec.p);
-    s.out.spmv++;
+    Reduce R;
+    launch_rayleigh(op_of(*s.A), s.vec.p, s.rq(), R, ctx().stream);
+    launch_norm2(s.n, s.vec.p, R, s.rq() + 1, ctx().stream);
+    s.phase = P_WAIT;
+    s.wait = W_RQ;
+    return;
   }
   s.out.eig = s.eig;
-  HIPCHK(hipStreamSynchronize(st));
   finish_slot(s, DMRGX_OK);
-  (void)n;
+}
+
+// Inverse-iteration step after Lanczos or after a CG solve (solver.hip:inverse_iteration):
+// launches the residual check and waits for it, or ends the iteration.
+void ii_advance(Slot &s) {
+  if (!s.p.skip_ii && s.ii_iter <= s.p.ii_max_step) {
+    Reduce R;
+    launch_residual_l1(op_of(*s.A), s.vec.p, s.eig, s.resid(), R, ctx().stream);
+    s.phase = P_WAIT;
+    s.wait = W_RESID;
+    return;
+  }
+  ii_finalize(s);
+}
+
+// A waiting slot's scalars arrived: residual -> done or the next CG solve; Rayleigh
+// quotient -> done.
+void wait_resolved(Slot &s, const double *h) {
+  if (s.wait == W_RESID) {
+    s.err = h[0];
+    s.out.spmv++;
+    if (s.err < s.p.ii_acc) {
+      s.out.ii_converged = true;
+      ii_finalize(s);
+    } else {
+      start_cg(s);
+    }
+  } else {
+    s.eig = h[0] / h[1];
+    s.out.spmv++;
+    s.out.eig = s.eig;
+    finish_slot(s, DMRGX_OK);
+  }
 }
 
 // A CG solve ended (converged or out of steps): x -> un, normalise, vec = un, next II round.
@@ -411,7 +457,8 @@ void lanczos_finished(Slot &s, int kconv) {
   hipStream_t st = ctx().stream;
   const int64_t n = s.n;
   tridiag_vector(kconv + 1, s.ha.data(), s.hb.data() + 1, s.lam, s.s.data());
-  std::vector<const double *> ptrs(kconv + 1);
+  std::vector<const double *> &ptrs = s.ritz_ptrs;   // host source of an asynchronous copy
+  ptrs.resize(kconv + 1);
   for (int j = 0; j <= kconv; j++) ptrs[j] = s.V.vec(j);
   DevBuf<const double *> dptr(kconv + 1);
   DevBuf<double> dcoef(kconv + 1);
@@ -421,7 +468,8 @@ void lanczos_finished(Slot &s, int kconv) {
   Reduce R;
   launch_norm2(n, s.vec.p, R, s.nrm2(), st);
   launch_scale_by_rnorm(n, s.nrm2(), s.vec.p, s.vec.p, st);
-  HIPCHK(hipStreamSynchronize(st));
+  // no synchronisation: the basis memory goes back to the allocator and is reused only by
+  // work queued after the Ritz kernel on the same stream
   s.V.chunks.clear();
   s.eig = s.lam;
   s.out.lan_steps = kconv;   // steps after the first, as solver.hip:lanczos counts them
@@ -489,17 +537,19 @@ struct LaunchPlan {
   std::vector<int32_t> blk;
 };
 
-void ensure_scratch(dmrgx_engine &E, size_t n_desc, size_t n_blk) {
+// n_map: block maps of all launches; max_blocks: the largest launch (one partial per block)
+void ensure_scratch(dmrgx_engine &E, size_t n_desc, size_t n_map, size_t max_blocks) {
   if (E.ddesc.count < n_desc) E.ddesc.resize(std::max<size_t>(n_desc, 2 * E.ddesc.count));
-  if (E.dblk.count < n_blk) E.dblk.resize(std::max<size_t>(n_blk, 2 * E.dblk.count));
-  if (E.partial.count < n_blk) E.partial.resize(std::max<size_t>(n_blk, 2 * E.partial.count));
+  if (E.dblk.count < n_map) E.dblk.resize(std::max<size_t>(n_map, 2 * E.dblk.count));
+  if (E.partial.count < max_blocks) E.partial.resize(std::max<size_t>(max_blocks, 2 * E.partial.count));
 }
 
 template <int MODE>
-void launch_seg_spmv(int G, int nblk, const SegDesc *desc, const int32_t *blk, double *partial, hipStream_t st) {
+void launch_seg_spmv(int G, int nblk, const SegDesc *desc, const int32_t *map, int nseg, double *partial,
+                     hipStream_t st) {
   switch (G) {
 #define G_CASE(g) \
-  case g: k_seg_spmv<g, MODE><<<nblk, kBlock, 0, st>>>(desc, blk, partial); break;
+  case g: k_seg_spmv<g, MODE><<<nblk, kBlock, 0, st>>>(desc, map, nseg, partial); break;
     G_CASE(1) G_CASE(2) G_CASE(4) G_CASE(8) G_CASE(16) G_CASE(32) G_CASE(64)
 #undef G_CASE
     default: throw Failure(DMRGX_ERR_ARG, "bad lanes per row");
@@ -507,8 +557,50 @@ void launch_seg_spmv(int G, int nblk, const SegDesc *desc, const int32_t *blk, d
 }
 
 // One chunk of ticks over all active slots.
+// Reads the scalars that every waiting slot needs -- one gather kernel, one copy, one
+// synchronisation for all of them -- and advances those slots; repeats while a decision
+// leaves a slot waiting again (residual converged -> Rayleigh quotient of the mixed form).
+void resolve_waits(dmrgx_engine &E) {
+  hipStream_t st = ctx().stream;
+  for (int round = 0; round < 8; round++) {
+    std::vector<Slot *> W;
+    for (auto &sp : E.active)
+      if (sp->phase == P_WAIT) W.push_back(sp.get());
+    if (W.empty()) return;
+    std::vector<GatherDesc> gd(W.size());
+    std::vector<size_t> off(W.size() + 1, 0);
+    for (size_t i = 0; i < W.size(); i++) {
+      const bool rq = W[i]->wait == W_RQ;
+      gd[i] = GatherDesc{rq ? W[i]->rq() : W[i]->resid(), (int64_t)off[i], rq ? 2 : 1, 0};
+      off[i + 1] = off[i] + (rq ? 2 : 1);
+    }
+    if (E.dgather.count < gd.size()) E.dgather.resize(std::max(gd.size(), 2 * E.dgather.count));
+    if (E.dstage.count < off.back()) E.dstage.resize(std::max(off.back(), 2 * E.dstage.count));
+    HIPCHK(hipMemcpyAsync(E.dgather.p, gd.data(), sizeof(GatherDesc) * gd.size(), hipMemcpyHostToDevice, st));
+    k_gather_words<<<(int)gd.size(), kBlock, 0, st>>>(E.dgather.p, E.dstage.p);
+    HIPCHK(hipGetLastError());
+    double *h = ctx().host_stage(off.back());
+    HIPCHK(hipMemcpyAsync(h, E.dstage.p, sizeof(double) * off.back(), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    const std::vector<double> stage(h, h + off.back());
+    for (size_t i = 0; i < W.size(); i++) wait_resolved(*W[i], stage.data() + off[i]);
+  }
+}
+
 void run_chunk(dmrgx_engine &E, int ticks) {
   hipStream_t st = ctx().stream;
+  {
+    const double t0 = now_sec();
+    resolve_waits(E);
+    E.host_t[4] += now_sec() - t0;
+  }
+  double tm = now_sec();
+  auto lap = [&](int i) {
+    const double t = now_sec();
+    E.host_t[i] += t - tm;
+    tm = t;
+  };
+  E.chunks++;
   std::vector<Slot *> L, Cg;
   for (auto &sp : E.active) {
     if (sp->phase == P_LANCZOS) L.push_back(sp.get());
@@ -536,10 +628,16 @@ void run_chunk(dmrgx_engine &E, int ticks) {
       d.blk_lo = lo;
       d.nblk = nb[i];
       hdesc.push_back(d);
-      for (int b = 0; b < nb[i]; b++) hblk.push_back((int32_t)i);
       lo += nb[i];
     }
     l.nblk = lo;
+    // coarse block -> segment map, one entry per kMapStride blocks
+    size_t seg = 0;
+    int seg_end = segs.empty() ? 0 : nb[0];
+    for (int b = 0; b < lo; b += kMapStride) {
+      while (b >= seg_end) seg_end += nb[++seg];
+      hblk.push_back((int32_t)seg);
+    }
     if (lo > 0) launches.push_back(l);
   };
   std::vector<int> lan_k(L.size()), cg_it(Cg.size());
@@ -650,31 +748,46 @@ void run_chunk(dmrgx_engine &E, int ticks) {
     if (!any) break;
   }
   if (!launches.empty()) {
-    ensure_scratch(E, hdesc.size(), hblk.size());
-    HIPCHK(hipMemcpyAsync(E.ddesc.p, hdesc.data(), sizeof(SegDesc) * hdesc.size(), hipMemcpyHostToDevice, st));
-    HIPCHK(hipMemcpyAsync(E.dblk.p, hblk.data(), sizeof(int32_t) * hblk.size(), hipMemcpyHostToDevice, st));
+    size_t max_blocks = 0;
+    for (const Launch &l : launches) max_blocks = std::max(max_blocks, (size_t)l.nblk);
+    ensure_scratch(E, hdesc.size(), hblk.size(), max_blocks);
+    const size_t bd = sizeof(SegDesc) * hdesc.size(), bm = sizeof(int32_t) * hblk.size();
+    if (E.pinned_bytes < bd + bm) {
+      const size_t want = std::max(bd + bm, 2 * E.pinned_bytes);
+      if (E.pinned) HIPCHK(hipHostFree(E.pinned));
+      E.pinned = nullptr;
+      E.pinned_bytes = 0;
+      HIPCHK(hipHostMalloc(&E.pinned, want, hipHostMallocDefault));
+      E.pinned_bytes = want;
+    }
+    char *pin = static_cast<char *>(E.pinned);
+    std::memcpy(pin, hdesc.data(), bd);
+    std::memcpy(pin + bd, hblk.data(), bm);
+    HIPCHK(hipMemcpyAsync(E.ddesc.p, pin, bd, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(E.dblk.p, pin + bd, bm, hipMemcpyHostToDevice, st));
   }
+  lap(0);
   for (const Launch &l : launches) {
     const SegDesc *dd = E.ddesc.p + l.desc_off;
     const int32_t *bb = E.dblk.p + l.blk_off;
     switch (l.kind) {
       case 0:
-        launch_seg_spmv<LANCZOS_A>(l.G, l.nblk, dd, bb, E.partial.p, st);
+        launch_seg_spmv<LANCZOS_A>(l.G, l.nblk, dd, bb, l.nseg, E.partial.p, st);
         k_seg_finish<FIN_LANCZOS_A><<<l.nseg, kBlock, 0, st>>>(dd, E.partial.p);
         break;
       case 1:
-        k_seg_lanczos_b<<<l.nblk, kBlock, 0, st>>>(dd, bb, E.partial.p);
+        k_seg_lanczos_b<<<l.nblk, kBlock, 0, st>>>(dd, bb, l.nseg, E.partial.p);
         k_seg_finish<FIN_LANCZOS_B><<<l.nseg, kBlock, 0, st>>>(dd, E.partial.p);
         break;
       case 2:
-        launch_seg_spmv<CG_AP>(l.G, l.nblk, dd, bb, E.partial.p, st);
+        launch_seg_spmv<CG_AP>(l.G, l.nblk, dd, bb, l.nseg, E.partial.p, st);
         k_seg_finish<FIN_CG_AP><<<l.nseg, kBlock, 0, st>>>(dd, E.partial.p);
         break;
       case 3:
-        k_seg_cg_update<<<l.nblk, kBlock, 0, st>>>(dd, bb, E.partial.p);
+        k_seg_cg_update<<<l.nblk, kBlock, 0, st>>>(dd, bb, l.nseg, E.partial.p);
         k_seg_finish<FIN_CG_UPDATE><<<l.nseg, kBlock, 0, st>>>(dd, E.partial.p);
         break;
-      case 4: k_seg_cg_dir<<<l.nblk, kBlock, 0, st>>>(dd, bb); break;
+      case 4: k_seg_cg_dir<<<l.nblk, kBlock, 0, st>>>(dd, bb, l.nseg); break;
     }
   }
   HIPCHK(hipGetLastError());
@@ -705,13 +818,16 @@ void run_chunk(dmrgx_engine &E, int ticks) {
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(h, E.dstage.p, sizeof(double) * off.back(), hipMemcpyDeviceToHost, st));
   }
+  lap(1);
   HIPCHK(hipStreamSynchronize(st));  // host descriptor vectors go out of scope; scalars read next
+  lap(2);
   // the staging area is shared: keep a copy before the phase changes below reuse it
   const std::vector<double> stage(h, h + off.back());
   for (size_t i = 0; i < L.size(); i++) {
     const size_t c = off[i + 1] - off[i];
     lanczos_check(*L[i], stage.data() + off[i], stage.data() + off[i] + c / 2);
   }
+  lap(3);
   for (size_t i = 0; i < Cg.size(); i++) {
     Slot *s = Cg[i];
     int hs[2];
@@ -719,6 +835,8 @@ void run_chunk(dmrgx_engine &E, int ticks) {
     if (hs[0]) cg_finished(*s, hs[1]);
     else if (s->cg_it >= s->p.cg_max_step) cg_finished(*s, -1);
   }
+  resolve_waits(E);
+  lap(4);
 }
 
 void admit(dmrgx_engine &E) {
@@ -815,7 +933,9 @@ int dmrgx_engine_run(dmrgx_engine *E, int32_t max_chunks) {
   if (!E) return DMRGX_ERR_ARG;
   int rc = run_status([&] {
     for (int c = 0; c < max_chunks; c++) {
+      const double t0 = now_sec();
       admit(*E);
+      E->host_t[5] += now_sec() - t0;
       if (E->active.empty()) break;
       run_chunk(*E, E->opts.chunk_ticks);
       retire(*E);
@@ -823,6 +943,14 @@ int dmrgx_engine_run(dmrgx_engine *E, int32_t max_chunks) {
     admit(*E);
   });
   return rc;
+}
+
+int dmrgx_engine_host_times(const dmrgx_engine *E, double *seconds6, int64_t *chunks) {
+  if (!E) return DMRGX_ERR_ARG;
+  if (seconds6)
+    for (int i = 0; i < 6; i++) seconds6[i] = E->host_t[i];
+  if (chunks) *chunks = E->chunks;
+  return DMRGX_OK;
 }
 
 int32_t dmrgx_engine_pending(const dmrgx_engine *E) {

@@ -42,6 +42,7 @@ def lib() -> C.CDLL:
         L.dmrgx_engine_poll.argtypes = [vp, C.POINTER(EngineResult), vp]
         L.dmrgx_engine_free.argtypes = [vp]
         L.dmrgx_ground_state_batch.argtypes = [C.c_int32, vp, C.POINTER(SolveParams), vp, C.POINTER(SolveStats)]
+        L.dmrgx_engine_host_times.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_int64)]
         _bound = True
     return L
 
@@ -129,6 +130,16 @@ class Engine:
             res += self.poll(want_vec)
         res += self.poll(want_vec)
         return res
+
+    def host_times(self) -> dict:
+        """Host seconds by activity (dmrgx_engine_host_times) and the number of chunks run."""
+        t = (C.c_double * 6)()
+        c = C.c_int64()
+        check(lib().dmrgx_engine_host_times(self.handle, t, C.byref(c)))
+        keys = ("descriptors", "launch", "wait", "lanczos_checks", "ii_cg_phase_changes", "admission")
+        out = {k: round(t[i], 4) for i, k in enumerate(keys)}
+        out["chunks"] = c.value
+        return out
 
     def free(self):
         if getattr(self, "handle", None):

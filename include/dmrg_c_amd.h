@@ -667,6 +667,11 @@ int dmrgx_engine_peek(const dmrgx_engine *E, uint64_t *tag, int32_t *n);
 /* Oldest finished request: returns 1 and fills *res (and vec_out[0..n) if given), 0 if none. */
 int dmrgx_engine_poll(dmrgx_engine *E, dmrgx_engine_result *res, double *vec_out);
 void dmrgx_engine_free(dmrgx_engine *E);
+/* Host-side seconds spent by the engine so far: [0] building and uploading the chunks'
+ * launch descriptors, [1] launching, [2] waiting for each chunk, [3] Lanczos convergence
+ * checks (with the Ritz vectors and first residual checks they start), [4] CG ends and
+ * inverse-iteration rounds, [5] admitting requests; *chunks = chunks run. */
+int dmrgx_engine_host_times(const dmrgx_engine *E, double *seconds6, int64_t *chunks);
 /* count solves through one engine; vec_out[i] (host, n_i doubles) and stats[i] may be NULL. */
 int dmrgx_ground_state_batch(int32_t count, dmrgx_matrix *const *A, const dmrgx_solve_params *p,
                              double *const *vec_out, dmrgx_solve_stats *stats);
