@@ -19,9 +19,13 @@
 //   k_seg_cg_update           x += a p, r -= a y, |r|^2, done flag
 //   k_seg_cg_dir              p = r + b p
 // The per-tick descriptors of all slots are built on the host and copied once per chunk of
-// ticks; the host reads the recurrence scalars once per chunk, runs the tridiagonal checks,
-// and moves slots between phases.  New requests are admitted at chunk boundaries while the
-// HBM budget allows (continuous batching).
+// ticks; the host reads the recurrence scalars once per chunk (one gather kernel, one copy),
+// runs the tridiagonal checks, and moves slots between phases.  Chunk c + 1 is launched
+// before chunk c's read-back is processed, so that host work overlaps the device: a slot
+// that converged in chunk c has run unused steps in chunk c + 1 (CG kernels skip them on
+// the device; Lanczos slots close to convergence get only the steps their convergence rate
+// predicts), which never changes a result.  New requests are admitted at chunk boundaries
+// while the HBM budget allows (continuous batching).
 
 #include <hip/hip_runtime.h>
 
@@ -251,7 +255,10 @@ struct Slot {
   dmrgx_solve_params p{};
   bool mixed = false;
   Phase phase = P_LANCZOS;
+  int epoch = 0;          // bumped at every phase change (stale read-backs are ignored)
   Wait wait = W_RESID;
+  bool wait_gathered = false;   // the awaited scalars ride on a launched chunk's read-back
+  double dE_prev = INFINITY, dE_last = INFINITY;   // last two |E_k - E_{k-1}| of the checks
   int status = DMRGX_OK;
   int32_t n = 0;
   int lanes = 8;
@@ -285,6 +292,19 @@ struct Slot {
 
 }  // namespace dmrgx
 
+namespace dmrgx {
+// A launched chunk's read-back: the slots whose scalars it carries (with their phase epoch
+// at launch), the Lanczos steps and CG iterations it ran, and where each slot's words are
+// in the pinned read-back area.
+struct Inflight {
+  std::vector<Slot *> L, Cg, W;
+  std::vector<int> epL, epC, epW;
+  std::vector<int> k_start, k_end, cg_end;
+  std::vector<size_t> off;   // L: 2 (k_end - k_start) words, then Cg: 1 word, then W: 1 or 2
+  int area = 0;
+};
+}  // namespace dmrgx
+
 struct dmrgx_engine {
   dmrgx_engine_opts opts{};
   int64_t next_id = 1;
@@ -303,11 +323,22 @@ struct dmrgx_engine {
   // inverse-iteration rounds, admission; then the number of chunks
   double host_t[6] = {0, 0, 0, 0, 0, 0};
   int64_t chunks = 0;
-  // pinned staging of the descriptor upload (reused once the chunk has synchronised)
-  void *pinned = nullptr;
-  size_t pinned_bytes = 0;
+  // Two pinned areas, used by consecutive chunks in turn (a chunk is launched before the
+  // previous one's read-back is processed): uploads, read-backs, and the read-back events.
+  void *up_pin[2] = {nullptr, nullptr};
+  size_t up_bytes[2] = {0, 0};
+  void *rb_pin[2] = {nullptr, nullptr};
+  size_t rb_bytes[2] = {0, 0};
+  hipEvent_t rb_ev[2] = {nullptr, nullptr};
+  int area = 0;
+  std::unique_ptr<dmrgx::Inflight> fl;   // launched, read-back not processed yet
   ~dmrgx_engine() {
-    if (pinned) (void)hipHostFree(pinned);
+    (void)hipStreamSynchronize(dmrgx::ctx().stream);
+    for (int i = 0; i < 2; i++) {
+      if (up_pin[i]) (void)hipHostFree(up_pin[i]);
+      if (rb_pin[i]) (void)hipHostFree(rb_pin[i]);
+      if (rb_ev[i]) (void)hipEventDestroy(rb_ev[i]);
+    }
   }
 };
 
@@ -354,6 +385,7 @@ void start_slot(Slot &s) {
 
 void finish_slot(Slot &s, int status) {
   s.phase = P_DONE;
+  s.epoch++;
   s.status = status;
   s.t_done = now_sec();
   // release the work vectors; vec (the result) stays until polled
@@ -381,6 +413,7 @@ void start_cg(Slot &s) {
   s.out.spmv++;
   s.cg_it = 1;
   s.phase = P_CG;
+  s.epoch++;
 }
 
 // End of the inverse iteration (solver.hip:inverse_iteration after its loop): counters, and
@@ -395,6 +428,7 @@ void ii_finalize(Slot &s) {
     launch_rayleigh(op_of(*s.A), s.vec.p, s.rq(), R, ctx().stream);
     launch_norm2(s.n, s.vec.p, R, s.rq() + 1, ctx().stream);
     s.phase = P_WAIT;
+    s.epoch++;
     s.wait = W_RQ;
     return;
   }
@@ -409,6 +443,7 @@ void ii_advance(Slot &s) {
     Reduce R;
     launch_residual_l1(op_of(*s.A), s.vec.p, s.eig, s.resid(), R, ctx().stream);
     s.phase = P_WAIT;
+    s.epoch++;
     s.wait = W_RESID;
     return;
   }
@@ -490,14 +525,14 @@ void lanczos_finished(Slot &s, int kconv) {
 
 // Host checks of the steps computed since the last chunk (solver.hip:lanczos, no restarts).
 // alpha_new[j] = alpha_{k_read + j}, beta_new[j] = beta_{k_read + 1 + j} for the steps
-// k_read .. k_next - 1 run since the last check.
-void lanczos_check(Slot &s, const double *alpha_new, const double *beta_new) {
+// k_read .. k_to - 1, read back since the last check.
+void lanczos_check(Slot &s, int k_to, const double *alpha_new, const double *beta_new) {
   const bool small = s.n < 1000;
-  for (int k = s.k_read; k < s.k_next; k++) {
+  for (int k = s.k_read; k < k_to; k++) {
     s.ha[k] = alpha_new[k - s.k_read];
     s.hb[k + 1] = beta_new[k - s.k_read];
   }
-  for (int k = s.k_read; k < s.k_next; k++) {
+  for (int k = s.k_read; k < k_to; k++) {
     s.k_read = k + 1;
     s.out.spmv++;
     if (k == 0) continue;  // step 0 is the start of the recurrence (solver.hip: block step 0)
@@ -522,10 +557,12 @@ void lanczos_check(Slot &s, const double *alpha_new, const double *beta_new) {
       lanczos_finished(s, k);
       return;
     }
+    s.dE_prev = s.dE_last;
+    s.dE_last = std::fabs(s.E - s.lam);
     s.E = s.lam;
   }
-  if (s.k_next >= s.p.diag_max_step) {
-    s.out.lan_steps = s.k_next - 1;
+  if (k_to >= s.p.diag_max_step) {
+    s.out.lan_steps = k_to - 1;
     s.out.eig = s.lam;
     finish_slot(s, DMRGX_ERR_NOCONV);
   }
@@ -557,56 +594,53 @@ void launch_seg_spmv(int G, int nblk, const SegDesc *desc, const int32_t *map, i
 }
 
 // One chunk of ticks over all active slots.
-// Reads the scalars that every waiting slot needs -- one gather kernel, one copy, one
-// synchronisation for all of them -- and advances those slots; repeats while a decision
-// leaves a slot waiting again (residual converged -> Rayleigh quotient of the mixed form).
-void resolve_waits(dmrgx_engine &E) {
-  hipStream_t st = ctx().stream;
-  for (int round = 0; round < 8; round++) {
-    std::vector<Slot *> W;
-    for (auto &sp : E.active)
-      if (sp->phase == P_WAIT) W.push_back(sp.get());
-    if (W.empty()) return;
-    std::vector<GatherDesc> gd(W.size());
-    std::vector<size_t> off(W.size() + 1, 0);
-    for (size_t i = 0; i < W.size(); i++) {
-      const bool rq = W[i]->wait == W_RQ;
-      gd[i] = GatherDesc{rq ? W[i]->rq() : W[i]->resid(), (int64_t)off[i], rq ? 2 : 1, 0};
-      off[i + 1] = off[i] + (rq ? 2 : 1);
-    }
-    if (E.dgather.count < gd.size()) E.dgather.resize(std::max(gd.size(), 2 * E.dgather.count));
-    if (E.dstage.count < off.back()) E.dstage.resize(std::max(off.back(), 2 * E.dstage.count));
-    HIPCHK(hipMemcpyAsync(E.dgather.p, gd.data(), sizeof(GatherDesc) * gd.size(), hipMemcpyHostToDevice, st));
-    k_gather_words<<<(int)gd.size(), kBlock, 0, st>>>(E.dgather.p, E.dstage.p);
-    HIPCHK(hipGetLastError());
-    double *h = ctx().host_stage(off.back());
-    HIPCHK(hipMemcpyAsync(h, E.dstage.p, sizeof(double) * off.back(), hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
-    const std::vector<double> stage(h, h + off.back());
-    for (size_t i = 0; i < W.size(); i++) wait_resolved(*W[i], stage.data() + off[i]);
+void *grow_pinned(void *&p, size_t &have, size_t want) {
+  if (have < want) {
+    const size_t nb = std::max(want, 2 * have);
+    if (p) HIPCHK(hipHostFree(p));
+    p = nullptr;
+    have = 0;
+    HIPCHK(hipHostMalloc(&p, nb, hipHostMallocDefault));
+    have = nb;
   }
+  return p;
 }
 
-void run_chunk(dmrgx_engine &E, int ticks) {
+// Lanczos steps to launch for a slot in the next chunk.  The chunk is launched before the
+// previous read-back is checked, so a slot about to converge would run steps that are
+// never used; when the last two energy changes predict convergence (geometric decrease)
+// the slot gets only the predicted steps plus two.  The prediction never changes a
+// result, only how many unused steps run.
+int lan_budget_of(const Slot &s, int ticks) {
+  if (s.n < 1000 || s.p.lan_type == 1) return ticks;
+  if (!(std::isfinite(s.dE_last) && std::isfinite(s.dE_prev)) || !(s.dE_last > 0.0) || !(s.dE_last < s.dE_prev))
+    return ticks;
+  const double need = std::ceil(std::log(s.p.diag_acc / s.dE_last) / std::log(s.dE_last / s.dE_prev));
+  const double unchecked = (double)(s.k_next - s.k_read);
+  return (int)std::min<double>(ticks, std::max(0.0, need + 2.0 - unchecked));
+}
+
+// Launches one chunk of ticks for every Lanczos and CG slot and the read-back of their new
+// scalars (and of every waiting slot's) into a pinned area, without waiting for it.
+std::unique_ptr<Inflight> launch_chunk(dmrgx_engine &E, int ticks) {
   hipStream_t st = ctx().stream;
-  {
-    const double t0 = now_sec();
-    resolve_waits(E);
-    E.host_t[4] += now_sec() - t0;
-  }
   double tm = now_sec();
   auto lap = [&](int i) {
     const double t = now_sec();
     E.host_t[i] += t - tm;
     tm = t;
   };
-  E.chunks++;
-  std::vector<Slot *> L, Cg;
+  auto F = std::make_unique<Inflight>();
+  std::vector<Slot *> &L = F->L, &Cg = F->Cg, &W = F->W;
   for (auto &sp : E.active) {
     if (sp->phase == P_LANCZOS) L.push_back(sp.get());
     else if (sp->phase == P_CG) Cg.push_back(sp.get());
+    else if (sp->phase == P_WAIT && !sp->wait_gathered) W.push_back(sp.get());
   }
-  if (L.empty() && Cg.empty()) return;
+  if (L.empty() && Cg.empty() && W.empty()) return nullptr;
+  E.chunks++;
+  std::vector<int> lan_budget(L.size());
+  for (size_t i = 0; i < L.size(); i++) lan_budget[i] = lan_budget_of(*L[i], ticks);
   // lanes bins (each bin one SpMV launch per tick)
   std::map<int, std::vector<int>> binL, binC;
   for (size_t i = 0; i < L.size(); i++) binL[L[i]->lanes].push_back((int)i);
@@ -652,7 +686,7 @@ void run_chunk(dmrgx_engine &E, int ticks) {
       for (int i : kv.second) {
         Slot &s = *L[i];
         const int k = lan_k[i];
-        if (k >= s.p.diag_max_step) continue;
+        if (k >= s.p.diag_max_step || k - s.k_next >= lan_budget[i]) continue;
         SegDesc a{};
         a.row = s.A->row.p;
         a.col = s.A->col.p;
@@ -747,25 +781,64 @@ void run_chunk(dmrgx_engine &E, int ticks) {
     }
     if (!any) break;
   }
-  if (!launches.empty()) {
-    size_t max_blocks = 0;
-    for (const Launch &l : launches) max_blocks = std::max(max_blocks, (size_t)l.nblk);
-    ensure_scratch(E, hdesc.size(), hblk.size(), max_blocks);
-    const size_t bd = sizeof(SegDesc) * hdesc.size(), bm = sizeof(int32_t) * hblk.size();
-    if (E.pinned_bytes < bd + bm) {
-      const size_t want = std::max(bd + bm, 2 * E.pinned_bytes);
-      if (E.pinned) HIPCHK(hipHostFree(E.pinned));
-      E.pinned = nullptr;
-      E.pinned_bytes = 0;
-      HIPCHK(hipHostMalloc(&E.pinned, want, hipHostMallocDefault));
-      E.pinned_bytes = want;
-    }
-    char *pin = static_cast<char *>(E.pinned);
-    std::memcpy(pin, hdesc.data(), bd);
-    std::memcpy(pin + bd, hblk.data(), bm);
-    HIPCHK(hipMemcpyAsync(E.ddesc.p, pin, bd, hipMemcpyHostToDevice, st));
-    HIPCHK(hipMemcpyAsync(E.dblk.p, pin + bd, bm, hipMemcpyHostToDevice, st));
+  // upload: descriptors, block maps and the read-back gather list through this chunk's area
+  for (size_t i = 0; i < L.size(); i++) {
+    F->epL.push_back(L[i]->epoch);
+    F->k_start.push_back(L[i]->k_next);
+    F->k_end.push_back(lan_k[i]);
+    L[i]->k_next = lan_k[i];
   }
+  for (size_t i = 0; i < Cg.size(); i++) {
+    F->epC.push_back(Cg[i]->epoch);
+    F->cg_end.push_back(cg_it[i]);
+    Cg[i]->cg_it = cg_it[i];
+  }
+  for (Slot *w : W) {
+    F->epW.push_back(w->epoch);
+    w->wait_gathered = true;
+  }
+  std::vector<size_t> &off = F->off;
+  off.assign(L.size() + Cg.size() + W.size() + 1, 0);
+  std::vector<GatherDesc> gd;
+  for (size_t i = 0; i < L.size(); i++) {
+    const Slot &sl = *L[i];
+    const int32_t c = F->k_end[i] - F->k_start[i];
+    off[i + 1] = off[i] + 2 * (size_t)c;
+    if (c == 0) continue;
+    gd.push_back({sl.scal.p + F->k_start[i], (int64_t)off[i], c, 0});
+    gd.push_back({sl.scal.p + sl.KS + F->k_start[i] + 1, (int64_t)(off[i] + c), c, 0});
+  }
+  for (size_t i = 0; i < Cg.size(); i++) {
+    const size_t o = off[L.size() + i];
+    off[L.size() + i + 1] = o + 1;   // {done, last iteration} in 8 bytes
+    gd.push_back({reinterpret_cast<const double *>(Cg[i]->cgst.p), (int64_t)o, 1, 0});
+  }
+  for (size_t i = 0; i < W.size(); i++) {
+    const size_t o = off[L.size() + Cg.size() + i];
+    const bool rq = W[i]->wait == W_RQ;
+    off[L.size() + Cg.size() + i + 1] = o + (rq ? 2 : 1);
+    gd.push_back({rq ? W[i]->rq() : W[i]->resid(), (int64_t)o, rq ? 2 : 1, 0});
+  }
+  const int area = E.area;
+  E.area ^= 1;
+  F->area = area;
+  size_t max_blocks = 0;
+  for (const Launch &l : launches) max_blocks = std::max(max_blocks, (size_t)l.nblk);
+  ensure_scratch(E, hdesc.size(), hblk.size(), max_blocks);
+  if (E.dgather.count < gd.size()) E.dgather.resize(std::max(gd.size(), 2 * E.dgather.count));
+  const size_t words = std::max<size_t>(off.back(), 1);
+  if (E.dstage.count < words) E.dstage.resize(std::max(words, 2 * E.dstage.count));
+  const size_t bd = sizeof(SegDesc) * hdesc.size(), bm = sizeof(int32_t) * hblk.size();
+  const size_t bg = sizeof(GatherDesc) * gd.size();
+  char *pin = static_cast<char *>(grow_pinned(E.up_pin[area], E.up_bytes[area], bd + bm + bg + 64));
+  const size_t og = (bd + bm + 15) / 16 * 16;   // GatherDesc holds 8-byte fields
+  if (og + bg > E.up_bytes[area]) pin = static_cast<char *>(grow_pinned(E.up_pin[area], E.up_bytes[area], og + bg));
+  std::memcpy(pin, hdesc.data(), bd);
+  std::memcpy(pin + bd, hblk.data(), bm);
+  std::memcpy(pin + og, gd.data(), bg);
+  if (bd) HIPCHK(hipMemcpyAsync(E.ddesc.p, pin, bd, hipMemcpyHostToDevice, st));
+  if (bm) HIPCHK(hipMemcpyAsync(E.dblk.p, pin + bd, bm, hipMemcpyHostToDevice, st));
+  if (bg) HIPCHK(hipMemcpyAsync(E.dgather.p, pin + og, bg, hipMemcpyHostToDevice, st));
   lap(0);
   for (const Launch &l : launches) {
     const SegDesc *dd = E.ddesc.p + l.desc_off;
@@ -791,51 +864,54 @@ void run_chunk(dmrgx_engine &E, int ticks) {
     }
   }
   HIPCHK(hipGetLastError());
-  for (size_t i = 0; i < L.size(); i++) L[i]->k_next = lan_k[i];
-  for (size_t i = 0; i < Cg.size(); i++) Cg[i]->cg_it = cg_it[i];
-  // host side: the new Lanczos scalars (alpha_k and beta_{k+1} of this chunk's steps) and
-  // the CG status of every slot, copied into one staging area before one synchronisation
-  std::vector<size_t> off(L.size() + Cg.size() + 1, 0);
-  for (size_t i = 0; i < L.size(); i++) off[i + 1] = off[i] + 2 * (size_t)(L[i]->k_next - L[i]->k_read);
-  for (size_t i = 0; i < Cg.size(); i++) off[L.size() + i + 1] = off[L.size() + i] + 1;  // {done, it} in 8 bytes
-  std::vector<GatherDesc> gd;
-  for (size_t i = 0; i < L.size(); i++) {
-    const Slot &s = *L[i];
-    const int32_t c = s.k_next - s.k_read;
-    if (c == 0) continue;
-    gd.push_back({s.scal.p + s.k_read, (int64_t)off[i], c, 0});
-    gd.push_back({s.scal.p + s.KS + s.k_read + 1, (int64_t)(off[i] + c), c, 0});
-  }
-  for (size_t i = 0; i < Cg.size(); i++)
-    gd.push_back({reinterpret_cast<const double *>(Cg[i]->cgst.p), (int64_t)off[L.size() + i], 1, 0});
-  const size_t words = std::max<size_t>(off.back(), 1);
-  double *h = ctx().host_stage(words);
+  double *h = static_cast<double *>(grow_pinned(E.rb_pin[area], E.rb_bytes[area], sizeof(double) * words));
   if (!gd.empty()) {
-    if (E.dgather.count < gd.size()) E.dgather.resize(std::max(gd.size(), 2 * E.dgather.count));
-    if (E.dstage.count < words) E.dstage.resize(std::max(words, 2 * E.dstage.count));
-    HIPCHK(hipMemcpyAsync(E.dgather.p, gd.data(), sizeof(GatherDesc) * gd.size(), hipMemcpyHostToDevice, st));
     k_gather_words<<<(int)gd.size(), kBlock, 0, st>>>(E.dgather.p, E.dstage.p);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(h, E.dstage.p, sizeof(double) * off.back(), hipMemcpyDeviceToHost, st));
   }
+  if (!E.rb_ev[area]) HIPCHK(hipEventCreateWithFlags(&E.rb_ev[area], hipEventDisableTiming));
+  HIPCHK(hipEventRecord(E.rb_ev[area], st));
   lap(1);
-  HIPCHK(hipStreamSynchronize(st));  // host descriptor vectors go out of scope; scalars read next
+  return F;
+}
+
+// Waits for a launched chunk's read-back and advances its slots: Lanczos checks, CG ends,
+// waiting slots' residuals and Rayleigh quotients.  Slots whose phase changed since the
+// launch (a later chunk already saw them converge) skip their now unused scalars.
+void process_chunk(dmrgx_engine &E, const Inflight &F) {
+  double tm = now_sec();
+  auto lap = [&](int i) {
+    const double t = now_sec();
+    E.host_t[i] += t - tm;
+    tm = t;
+  };
+  HIPCHK(hipEventSynchronize(E.rb_ev[F.area]));
   lap(2);
-  // the staging area is shared: keep a copy before the phase changes below reuse it
-  const std::vector<double> stage(h, h + off.back());
-  for (size_t i = 0; i < L.size(); i++) {
-    const size_t c = off[i + 1] - off[i];
-    lanczos_check(*L[i], stage.data() + off[i], stage.data() + off[i] + c / 2);
+  const double *h = static_cast<const double *>(E.rb_pin[F.area]);
+  const std::vector<double> stage(h, h + F.off.back());   // the phase changes below launch work
+  const size_t nl = F.L.size(), nc = F.Cg.size();
+  for (size_t i = 0; i < nl; i++) {
+    Slot &sl = *F.L[i];
+    if (sl.phase != P_LANCZOS || sl.epoch != F.epL[i] || sl.k_read != F.k_start[i]) continue;
+    const size_t c = (size_t)(F.k_end[i] - F.k_start[i]);
+    lanczos_check(sl, F.k_end[i], stage.data() + F.off[i], stage.data() + F.off[i] + c);
   }
   lap(3);
-  for (size_t i = 0; i < Cg.size(); i++) {
-    Slot *s = Cg[i];
+  for (size_t i = 0; i < nc; i++) {
+    Slot &sl = *F.Cg[i];
+    if (sl.phase != P_CG || sl.epoch != F.epC[i]) continue;
     int hs[2];
-    std::memcpy(hs, stage.data() + off[L.size() + i], sizeof(hs));
-    if (hs[0]) cg_finished(*s, hs[1]);
-    else if (s->cg_it >= s->p.cg_max_step) cg_finished(*s, -1);
+    std::memcpy(hs, stage.data() + F.off[nl + i], sizeof(hs));
+    if (hs[0]) cg_finished(sl, hs[1]);
+    else if (F.cg_end[i] >= sl.p.cg_max_step) cg_finished(sl, -1);
   }
-  resolve_waits(E);
+  for (size_t i = 0; i < F.W.size(); i++) {
+    Slot &sl = *F.W[i];
+    sl.wait_gathered = false;
+    if (sl.phase != P_WAIT || sl.epoch != F.epW[i]) continue;
+    wait_resolved(sl, stage.data() + F.off[nl + nc + i]);
+  }
   lap(4);
 }
 
@@ -932,12 +1008,22 @@ int dmrgx_engine_submit(dmrgx_engine *E, dmrgx_matrix *A, int take_ownership, co
 int dmrgx_engine_run(dmrgx_engine *E, int32_t max_chunks) {
   if (!E) return DMRGX_ERR_ARG;
   int rc = run_status([&] {
+    // chunk c + 1 is launched before chunk c's read-back is processed, so the host's checks
+    // and phase changes overlap the device's next chunk
     for (int c = 0; c < max_chunks; c++) {
       const double t0 = now_sec();
       admit(*E);
       E->host_t[5] += now_sec() - t0;
-      if (E->active.empty()) break;
-      run_chunk(*E, E->opts.chunk_ticks);
+      if (E->active.empty() && !E->fl) break;
+      std::unique_ptr<Inflight> next = launch_chunk(*E, E->opts.chunk_ticks);
+      if (!next && !E->fl) break;
+      if (E->fl) process_chunk(*E, *E->fl);
+      E->fl = std::move(next);
+      retire(*E);
+    }
+    if (E->fl) {
+      process_chunk(*E, *E->fl);
+      E->fl.reset();
       retire(*E);
     }
     admit(*E);
