@@ -36,6 +36,7 @@
 #include <deque>
 #include <limits>
 #include <map>
+#include <unordered_map>
 
 #include "entry.h"
 #include "reduce.h"
@@ -297,7 +298,8 @@ namespace dmrgx {
 // at launch), the Lanczos steps and CG iterations it ran, and where each slot's words are
 // in the pinned read-back area.
 struct Inflight {
-  std::vector<Slot *> L, Cg, W;
+  std::vector<Slot *> L, Cg, W;   // valid only while launching (slots may retire later)
+  std::vector<int64_t> idL, idC, idW;
   std::vector<int> epL, epC, epW;
   std::vector<int> k_start, k_end, cg_end;
   std::vector<size_t> off;   // L: 2 (k_end - k_start) words, then Cg: 1 word, then W: 1 or 2
@@ -331,7 +333,7 @@ struct dmrgx_engine {
   size_t rb_bytes[2] = {0, 0};
   hipEvent_t rb_ev[2] = {nullptr, nullptr};
   int area = 0;
-  std::unique_ptr<dmrgx::Inflight> fl;   // launched, read-back not processed yet
+  std::unique_ptr<dmrgx::Inflight> fl;   // launched, read-back not processed yet (slots by id)
   ~dmrgx_engine() {
     (void)hipStreamSynchronize(dmrgx::ctx().stream);
     for (int i = 0; i < 2; i++) {
@@ -783,17 +785,20 @@ std::unique_ptr<Inflight> launch_chunk(dmrgx_engine &E, int ticks) {
   }
   // upload: descriptors, block maps and the read-back gather list through this chunk's area
   for (size_t i = 0; i < L.size(); i++) {
+    F->idL.push_back(L[i]->id);
     F->epL.push_back(L[i]->epoch);
     F->k_start.push_back(L[i]->k_next);
     F->k_end.push_back(lan_k[i]);
     L[i]->k_next = lan_k[i];
   }
   for (size_t i = 0; i < Cg.size(); i++) {
+    F->idC.push_back(Cg[i]->id);
     F->epC.push_back(Cg[i]->epoch);
     F->cg_end.push_back(cg_it[i]);
     Cg[i]->cg_it = cg_it[i];
   }
   for (Slot *w : W) {
+    F->idW.push_back(w->id);
     F->epW.push_back(w->epoch);
     w->wait_gathered = true;
   }
@@ -890,24 +895,38 @@ void process_chunk(dmrgx_engine &E, const Inflight &F) {
   lap(2);
   const double *h = static_cast<const double *>(E.rb_pin[F.area]);
   const std::vector<double> stage(h, h + F.off.back());   // the phase changes below launch work
-  const size_t nl = F.L.size(), nc = F.Cg.size();
+  // slots are looked up by id among the active ones: a slot that finished after the launch
+  // may have been retired and polled since
+  std::unordered_map<int64_t, Slot *> live;
+  for (auto &sp : E.active) live[sp->id] = sp.get();
+  auto find = [&](int64_t id) -> Slot * {
+    auto it = live.find(id);
+    return it == live.end() ? nullptr : it->second;
+  };
+  const size_t nl = F.idL.size(), nc = F.idC.size();
   for (size_t i = 0; i < nl; i++) {
-    Slot &sl = *F.L[i];
+    Slot *sp = find(F.idL[i]);
+    if (!sp) continue;
+    Slot &sl = *sp;
     if (sl.phase != P_LANCZOS || sl.epoch != F.epL[i] || sl.k_read != F.k_start[i]) continue;
     const size_t c = (size_t)(F.k_end[i] - F.k_start[i]);
     lanczos_check(sl, F.k_end[i], stage.data() + F.off[i], stage.data() + F.off[i] + c);
   }
   lap(3);
   for (size_t i = 0; i < nc; i++) {
-    Slot &sl = *F.Cg[i];
+    Slot *sp = find(F.idC[i]);
+    if (!sp) continue;
+    Slot &sl = *sp;
     if (sl.phase != P_CG || sl.epoch != F.epC[i]) continue;
     int hs[2];
     std::memcpy(hs, stage.data() + F.off[nl + i], sizeof(hs));
     if (hs[0]) cg_finished(sl, hs[1]);
     else if (F.cg_end[i] >= sl.p.cg_max_step) cg_finished(sl, -1);
   }
-  for (size_t i = 0; i < F.W.size(); i++) {
-    Slot &sl = *F.W[i];
+  for (size_t i = 0; i < F.idW.size(); i++) {
+    Slot *sp = find(F.idW[i]);
+    if (!sp) continue;
+    Slot &sl = *sp;
     sl.wait_gathered = false;
     if (sl.phase != P_WAIT || sl.epoch != F.epW[i]) continue;
     wait_resolved(sl, stage.data() + F.off[nl + nc + i]);
@@ -1021,11 +1040,7 @@ int dmrgx_engine_run(dmrgx_engine *E, int32_t max_chunks) {
       E->fl = std::move(next);
       retire(*E);
     }
-    if (E->fl) {
-      process_chunk(*E, *E->fl);
-      E->fl.reset();
-      retire(*E);
-    }
+    // the last chunk stays in flight; the next call (or the engine's end) processes it
     admit(*E);
   });
   return rc;
