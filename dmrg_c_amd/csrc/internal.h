@@ -519,6 +519,22 @@ void launch_cg_update(int64_t n, int it, double acc, const double *p, const doub
                       hipStream_t s);
 void launch_cg_dir(int64_t n, int it, const double *r, const double *rr, double *p, const int *status,
                    hipStream_t s);
+// The same steps without their finish kernels: each consumer sums the producer's block
+// partials itself, in the finish kernel's order (bit-identical scalars); block 0 stores the
+// scalar the finish kernel would have stored.  Return the number of partials written.
+int launch_lanczos_a_part(const Operator &op, bool fp32, int k, const double *w, double *nrm2, double *beta,
+                          const double *pre, int pre_n, double *vk, double *y, double *alpha, Reduce &R,
+                          hipStream_t s);
+int launch_lanczos_b_part(int64_t n, int k, const double *y, const double *vk, const double *vkm1, double *alpha,
+                          const double *beta, const double *pre, int pre_n, double *w, Reduce &R, hipStream_t s);
+void launch_lanczos_b_finish(int k, double *beta, double *nrm2, const Reduce &R, int nblk, hipStream_t s);
+int launch_cg_ap_part(const Operator &op, double shift, const double *p, double *y, const int *done, Reduce &R,
+                      hipStream_t s);
+int launch_cg_update_part(int64_t n, int it, const double *p, const double *y, double *x, double *r, double *pap,
+                          const double *rr, const int *status, const double *pre, int pre_n, Reduce &R,
+                          hipStream_t s);
+void launch_cg_dir_part(int64_t n, int it, double acc, const double *r, double *rr, double *p, int *status,
+                        const double *pre, int pre_n, hipStream_t s);
 
 // single-slot operator products: y = O x (y may be null) and, if dot_with != null,
 // *dot_out = <dot_with, O x> (deterministic reduction)

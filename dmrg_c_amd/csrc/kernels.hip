@@ -124,14 +124,36 @@ struct EpiPlain {
 // (sml/LANCZOS_SLOW_SYM.c:114-122 with NORMALIZE folded into the product).
 struct EpiLanczosA {
   const double *w;
-  const double *nrm2;
+  double *nrm2;
   double *vk, *y, *alpha;
   int k;
   double shift;   // operator H + shift I (the minimum-residual solve of inverse iteration)
   double scale;
+  // With pre set, |w|^2 is the sum of the previous k_lanczos_b's pre_n block partials, taken
+  // in every block exactly as its finish kernel would (block 0 also stores *nrm2 and beta[k]);
+  // without, it is read from *nrm2.
+  const double *pre = nullptr;
+  int pre_n = 0;
+  double *beta = nullptr;
   static constexpr bool kReduce = true;
   __device__ bool skip() const { return false; }
-  __device__ void begin() { scale = 1.0 / sqrt(*nrm2); }
+  __device__ void begin() {
+    if (!pre) {
+      scale = 1.0 / sqrt(*nrm2);
+      return;
+    }
+    __shared__ double bc;
+    const double t = finish_sum(pre, pre_n);
+    if (threadIdx.x == 0) {
+      bc = t;
+      if (blockIdx.x == 0) {
+        *nrm2 = t;
+        beta[k] = sqrt(t);
+      }
+    }
+    __syncthreads();
+    scale = 1.0 / sqrt(bc);
+  }
   __device__ double row(int32_t r, double s) {
     const double xs = scale * w[r];
     vk[r] = xs;
@@ -223,11 +245,14 @@ struct EpiStoreDot {
   }
 };
 
+// Launches the product with its epilogue; with fin the reduction's finish kernel follows,
+// without it the caller's next kernel sums the partials.  Returns the number of blocks
+// (= partials).
 template <class Rows, class Epi>
-void apply_rows(int32_t n, int lanes, int grid_cap, const Rows &rows, const double *x, const Epi &epi, Reduce *R,
-                hipStream_t s, int xcd = 0) {
+int apply_rows(int32_t n, int lanes, int grid_cap, const Rows &rows, const double *x, const Epi &epi, Reduce *R,
+               hipStream_t s, int xcd = 0, bool fin = true) {
   const int blocks = spmv_blocks(n, lanes, grid_cap);
-  if (blocks == 0) return;
+  if (blocks == 0) return 0;
   double *part = nullptr;
   if (Epi::kReduce) {
     R->ensure(blocks);
@@ -249,51 +274,54 @@ void apply_rows(int32_t n, int lanes, int grid_cap, const Rows &rows, const doub
   }
   HIPCHK(hipGetLastError());
   if (ev) HIPCHK(hipEventRecord(ev->second, s));
-  if (Epi::kReduce) {
+  if (Epi::kReduce && fin) {
     k_finish<Epi><<<1, kBlock, 0, s>>>(epi, part, blocks);
     HIPCHK(hipGetLastError());
   }
+  return blocks;
 }
 
 template <class Epi>
-void apply_dict(const Operator &op, const double *x, const Epi &epi, Reduce *R, hipStream_t s) {
+int apply_dict(const Operator &op, const double *x, const Epi &epi, Reduce *R, hipStream_t s, bool fin) {
   const dmrgx_matrix &A = *op.A;
   const int xcd = (op.load & kLoadXcd) ? 1 : 0;
   switch (op.load & (kLoadXcd - 1)) {
 #define V_CASE(V)                                                                                           \
   case V:                                                                                                   \
-    apply_rows(op.n, op.lanes, op.grid, DictRows<V>{A.row.p, A.ent.p, A.table.p, A.nnz, A.n_values}, x, epi, \
-               R, s, xcd);                                                                                  \
-    break;
+    return apply_rows(op.n, op.lanes, op.grid, DictRows<V>{A.row.p, A.ent.p, A.table.p, A.nnz, A.n_values}, x, \
+                      epi, R, s, xcd, fin);
     V_CASE(0) V_CASE(1) V_CASE(2) V_CASE(3)
 #undef V_CASE
+    default: throw Failure(DMRGX_ERR_ARG, "bad SpMV load variant");
   }
 }
 
 template <class VT, class Epi>
-void apply_delta(const Operator &op, const VT *val, const double *x, const Epi &epi, Reduce *R, hipStream_t s) {
+int apply_delta(const Operator &op, const VT *val, const double *x, const Epi &epi, Reduce *R, hipStream_t s,
+                bool fin) {
   const dmrgx_matrix &A = *op.A;
   const int xcd = (op.load & kLoadXcd) ? 1 : 0;
   switch (op.load & (kLoadXcd - 1)) {
 #define V_CASE(V)                                                                                        \
   case V:                                                                                                \
-    apply_rows(op.n, op.lanes, op.grid, DeltaRows<VT, V>{A.row.p, A.first.p, A.dcol.p, val, A.nnz}, x, epi, R, \
-               s, xcd);                                                                                  \
-    break;
+    return apply_rows(op.n, op.lanes, op.grid, DeltaRows<VT, V>{A.row.p, A.first.p, A.dcol.p, val, A.nnz}, x,  \
+                      epi, R, s, xcd, fin);
     V_CASE(0) V_CASE(1) V_CASE(2) V_CASE(3)
 #undef V_CASE
+    default: throw Failure(DMRGX_ERR_ARG, "bad SpMV load variant");
   }
 }
 
 template <class VT, class Epi>
-void apply_csr(const Operator &op, const VT *val, const double *x, const Epi &epi, Reduce *R, hipStream_t s) {
+int apply_csr(const Operator &op, const VT *val, const double *x, const Epi &epi, Reduce *R, hipStream_t s,
+              bool fin) {
   const dmrgx_matrix &A = *op.A;
   const int xcd = (op.load & kLoadXcd) ? 1 : 0;
   switch (op.load & (kLoadXcd - 1)) {
 #define V_CASE(V)                                                                                              \
   case V:                                                                                                      \
-    apply_rows(op.n, op.lanes, op.grid, CsrRows<VT, V>{A.row.p, A.col.p, val, A.nnz}, x, epi, R, s, xcd); \
-    break;
+    return apply_rows(op.n, op.lanes, op.grid, CsrRows<VT, V>{A.row.p, A.col.p, val, A.nnz}, x, epi, R, s, xcd, \
+                      fin);
     V_CASE(0) V_CASE(1) V_CASE(2) V_CASE(3)
 #undef V_CASE
     default: throw Failure(DMRGX_ERR_ARG, "bad SpMV load variant");
@@ -301,24 +329,19 @@ void apply_csr(const Operator &op, const VT *val, const double *x, const Epi &ep
 }
 
 template <class Epi>
-void apply_op(const Operator &op, bool fp32, const double *x, const Epi &epi, Reduce *R, hipStream_t s) {
-  if (op.obc) {
-    apply_rows(op.n, op.lanes, op.grid, op.obc->rows(), x, epi, R, s);
-  } else {
-    const int fmt = load_format(op.load);
-    const bool delta = fmt == kFmtDelta && ensure_delta(*op.A);
-    if (fp32) {
-      if (!op.A->val32.p) throw Failure(DMRGX_ERR_ARG, "FP32 values not prepared");
-      if (delta) apply_delta<float>(op, op.A->val32.p, x, epi, R, s);
-      else apply_csr<float>(op, op.A->val32.p, x, epi, R, s);
-    } else if (delta) {
-      apply_delta<double>(op, op.A->val.p, x, epi, R, s);
-    } else if (fmt == kFmtDict && ensure_dict(*op.A)) {
-      apply_dict(op, x, epi, R, s);
-    } else {
-      apply_csr<double>(op, op.A->val.p, x, epi, R, s);
-    }
+int apply_op(const Operator &op, bool fp32, const double *x, const Epi &epi, Reduce *R, hipStream_t s,
+             bool fin = true) {
+  if (op.obc) return apply_rows(op.n, op.lanes, op.grid, op.obc->rows(), x, epi, R, s, 0, fin);
+  const int fmt = load_format(op.load);
+  const bool delta = fmt == kFmtDelta && ensure_delta(*op.A);
+  if (fp32) {
+    if (!op.A->val32.p) throw Failure(DMRGX_ERR_ARG, "FP32 values not prepared");
+    if (delta) return apply_delta<float>(op, op.A->val32.p, x, epi, R, s, fin);
+    return apply_csr<float>(op, op.A->val32.p, x, epi, R, s, fin);
   }
+  if (delta) return apply_delta<double>(op, op.A->val.p, x, epi, R, s, fin);
+  if (fmt == kFmtDict && ensure_dict(*op.A)) return apply_dict(op, x, epi, R, s, fin);
+  return apply_csr<double>(op, op.A->val.p, x, epi, R, s, fin);
 }
 
 // -------------------------------------------------------------- vector kernels
@@ -337,12 +360,27 @@ struct FinStore {
 
 // K_b of a Lanczos step (sml/LANCZOS_SLOW_SYM.c:183-186 and :116 of the next step):
 // w = y - alpha_k v_k - beta_k v_{k-1}; nrm2 = <w,w>; beta_{k+1} = sqrt(nrm2)
+// With pre set, alpha_k is the sum of the product's pre_n block partials, taken in every
+// block exactly as its finish kernel would (block 0 stores alpha[k]).
 __global__ __launch_bounds__(kBlock) void k_lanczos_b(int64_t n, int k, const double *__restrict__ y,
                                                       const double *__restrict__ vk,
                                                       const double *__restrict__ vkm1,
-                                                      const double *alpha, const double *beta,
-                                                      double *__restrict__ w, double *partial) {
-  const double a = alpha[k];
+                                                      double *alpha, const double *beta,
+                                                      double *__restrict__ w, double *partial,
+                                                      const double *pre, int pre_n) {
+  double a;
+  if (pre) {
+    __shared__ double bc;
+    const double t = finish_sum(pre, pre_n);
+    if (threadIdx.x == 0) {
+      bc = t;
+      if (blockIdx.x == 0) alpha[k] = t;
+    }
+    __syncthreads();
+    a = bc;
+  } else {
+    a = alpha[k];
+  }
   const double b = (k > 0) ? beta[k] : 0.0;
   double s = 0.0;
   for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
@@ -390,14 +428,29 @@ __global__ __launch_bounds__(kBlock) void k_copy(int64_t n, const double *__rest
 
 // CG update (CONJUGATE_GRADIENT_SYM.c:216-236): alpha = rr/pAp, x += alpha p, r -= alpha y,
 // rr_new = <r,r>; the last block records convergence in status[0..1].
+// With pre set, pAp is the sum of the product's pre_n block partials, taken in every block
+// exactly as its finish kernel would (block 0 stores pap[it]).
 __global__ __launch_bounds__(kBlock) void k_cg_update(int64_t n, int it, double acc,
                                                       const double *__restrict__ p,
                                                       const double *__restrict__ y,
                                                       double *__restrict__ x, double *__restrict__ r,
-                                                      const double *pap, const double *rr, const int *status,
-                                                      double *partial) {
+                                                      double *pap, const double *rr, const int *status,
+                                                      double *partial, const double *pre, int pre_n) {
   if (status[0] != 0) return;
-  const double alpha = rr[it - 1] / pap[it];
+  double pAp;
+  if (pre) {
+    __shared__ double bc;
+    const double t = finish_sum(pre, pre_n);
+    if (threadIdx.x == 0) {
+      bc = t;
+      if (blockIdx.x == 0) pap[it] = t;
+    }
+    __syncthreads();
+    pAp = bc;
+  } else {
+    pAp = pap[it];
+  }
+  const double alpha = rr[it - 1] / pAp;
   double s = 0.0;
   for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
     x[i] = x[i] + alpha * p[i];
@@ -421,10 +474,37 @@ struct FinCgUpdate {
   }
 };
 
-// CG direction (CONJUGATE_GRADIENT_SYM.c:257-262): p = r + (rr_it / rr_{it-1}) p
+// CG direction (CONJUGATE_GRADIENT_SYM.c:257-262): p = r + (rr_it / rr_{it-1}) p.  With pre
+// set, rr_it is the sum of k_cg_update's pre_n block partials, taken in every block exactly
+// as its finish kernel would; block 0 stores rr[it] and the status (FinCgUpdate's work), and
+// a converged iteration leaves p alone.
 __global__ __launch_bounds__(kBlock) void k_cg_dir(int64_t n, int it, const double *__restrict__ r,
-                                                   const double *rr, double *__restrict__ p,
-                                                   const int *status) {
+                                                   double *rr, double *__restrict__ p, int *status,
+                                                   const double *pre, int pre_n, double acc) {
+  if (pre) {
+    // block 0 of this very kernel may set status[0]: read it once per block, so that all
+    // threads of a block take the same branch (the reduction below has barriers)
+    __shared__ int done0;
+    __shared__ double bc;
+    if (threadIdx.x == 0) done0 = status[0];
+    __syncthreads();
+    if (done0 != 0) return;
+    const double t = finish_sum(pre, pre_n);
+    if (threadIdx.x == 0) {
+      bc = t;
+      if (blockIdx.x == 0) {
+        rr[it] = t;
+        status[1] = it;
+        if (t < acc) status[0] = 1;
+      }
+    }
+    __syncthreads();
+    if (bc < acc) return;
+    const double beta = bc / rr[it - 1];
+    for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
+      p[i] = r[i] + beta * p[i];
+    return;
+  }
   if (status[0] != 0) return;
   const double beta = rr[it] / rr[it - 1];
   for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
@@ -702,7 +782,7 @@ void launch_norm2(int64_t n, const double *x, Reduce &R, double *out, hipStream_
 
 void launch_lanczos_a(const Operator &op, bool fp32, int k, const double *w, const double *nrm2, double shift,
                       double *vk, double *y, double *alpha, Reduce &R, hipStream_t s) {
-  apply_op(op, fp32, w, EpiLanczosA{w, nrm2, vk, y, alpha, k, shift, 0.0}, &R, s);
+  apply_op(op, fp32, w, EpiLanczosA{w, const_cast<double *>(nrm2), vk, y, alpha, k, shift, 0.0}, &R, s);
 }
 
 void launch_lanczos_b(int64_t n, int k, const double *y, const double *vk, const double *vkm1,
@@ -710,8 +790,33 @@ void launch_lanczos_b(int64_t n, int k, const double *y, const double *vk, const
                       hipStream_t s) {
   const int g = vec_grid(n);
   R.ensure(g);
-  k_lanczos_b<<<g, kBlock, 0, s>>>(n, k, y, vk, vkm1 ? vkm1 : vk, alpha, beta, w, R.partial.p);
+  k_lanczos_b<<<g, kBlock, 0, s>>>(n, k, y, vk, vkm1 ? vkm1 : vk, const_cast<double *>(alpha), beta, w,
+                                   R.partial.p, nullptr, 0);
   k_finish<FinLanczosB><<<1, kBlock, 0, s>>>(FinLanczosB{nrm2, beta, k}, R.partial.p, g);
+  HIPCHK(hipGetLastError());
+}
+
+int launch_lanczos_a_part(const Operator &op, bool fp32, int k, const double *w, double *nrm2, double *beta,
+                          const double *pre, int pre_n, double *vk, double *y, double *alpha, Reduce &R,
+                          hipStream_t s) {
+  EpiLanczosA e{w, nrm2, vk, y, alpha, k, 0.0, 0.0};
+  e.pre = pre;
+  e.pre_n = pre_n;
+  e.beta = beta;
+  return apply_op(op, fp32, w, e, &R, s, false);
+}
+
+int launch_lanczos_b_part(int64_t n, int k, const double *y, const double *vk, const double *vkm1, double *alpha,
+                          const double *beta, const double *pre, int pre_n, double *w, Reduce &R, hipStream_t s) {
+  const int g = vec_grid(n);
+  R.ensure(g);
+  k_lanczos_b<<<g, kBlock, 0, s>>>(n, k, y, vk, vkm1 ? vkm1 : vk, alpha, beta, w, R.partial.p, pre, pre_n);
+  HIPCHK(hipGetLastError());
+  return g;
+}
+
+void launch_lanczos_b_finish(int k, double *beta, double *nrm2, const Reduce &R, int nblk, hipStream_t s) {
+  k_finish<FinLanczosB><<<1, kBlock, 0, s>>>(FinLanczosB{nrm2, beta, k}, R.partial.p, nblk);
   HIPCHK(hipGetLastError());
 }
 
@@ -756,14 +861,37 @@ void launch_cg_update(int64_t n, int it, double acc, const double *p, const doub
                       hipStream_t s) {
   const int g = vec_grid(n);
   R.ensure(g);
-  k_cg_update<<<g, kBlock, 0, s>>>(n, it, acc, p, y, x, r, pap, rr, status, R.partial.p);
+  k_cg_update<<<g, kBlock, 0, s>>>(n, it, acc, p, y, x, r, const_cast<double *>(pap), rr, status, R.partial.p,
+                                   nullptr, 0);
   k_finish<FinCgUpdate><<<1, kBlock, 0, s>>>(FinCgUpdate{rr, status, it, acc}, R.partial.p, g);
   HIPCHK(hipGetLastError());
 }
 
 void launch_cg_dir(int64_t n, int it, const double *r, const double *rr, double *p, const int *status,
                    hipStream_t s) {
-  k_cg_dir<<<vec_grid(n), kBlock, 0, s>>>(n, it, r, rr, p, status);
+  k_cg_dir<<<vec_grid(n), kBlock, 0, s>>>(n, it, r, const_cast<double *>(rr), p, const_cast<int *>(status),
+                                          nullptr, 0, 0.0);
+  HIPCHK(hipGetLastError());
+}
+
+int launch_cg_ap_part(const Operator &op, double shift, const double *p, double *y, const int *done, Reduce &R,
+                      hipStream_t s) {
+  return apply_op(op, false, p, EpiCgAp{p, y, nullptr, done, shift}, &R, s, false);
+}
+
+int launch_cg_update_part(int64_t n, int it, const double *p, const double *y, double *x, double *r, double *pap,
+                          const double *rr, const int *status, const double *pre, int pre_n, Reduce &R,
+                          hipStream_t s) {
+  const int g = vec_grid(n);
+  R.ensure(g);
+  k_cg_update<<<g, kBlock, 0, s>>>(n, it, 0.0, p, y, x, r, pap, rr, status, R.partial.p, pre, pre_n);
+  HIPCHK(hipGetLastError());
+  return g;
+}
+
+void launch_cg_dir_part(int64_t n, int it, double acc, const double *r, double *rr, double *p, int *status,
+                        const double *pre, int pre_n, hipStream_t s) {
+  k_cg_dir<<<vec_grid(n), kBlock, 0, s>>>(n, it, r, rr, p, status, pre, pre_n, acc);
   HIPCHK(hipGetLastError());
 }
 

@@ -74,6 +74,7 @@ void lanczos(const Operator &A, const LanczosOpts &o, double *vec, SolveOut &out
   double *alpha = sc.p, *beta = sc.p + block + 2, *nrm2 = sc.p + 2 * (block + 2);
   Reduce R;
   Basis V(n);
+  Reduce RA, RB;   // product partials, vector-update partials (alternating consumers)
   std::vector<double> ha(block + 2), hb(block + 2), s(block + 2);
 
   double E = 1e15;  // sml/LANCZOS_SYM.c:77
@@ -92,9 +93,16 @@ void lanczos(const Operator &A, const LanczosOpts &o, double *vec, SolveOut &out
     bool block_done = false;
     while (next < block && total_step < max_step) {
       const int c = std::min({chunk, block - next, max_step - total_step});
+      // two kernels per step: each sums the other's block partials itself (the scalar the
+      // finish kernel would store, same order); the chunk's last step gets the finish kernel
+      // so that the host reads beta_{k+1}
+      int nb = 0;
       for (int k = next; k < next + c; k++) {
-        launch_lanczos_a(A, o.fp32, k, w.p, nrm2, 0.0, V.vec(k), y.p, alpha, R, st);
-        launch_lanczos_b(n, k, y.p, V.vec(k), V.vec(k - 1), alpha, beta, w.p, nrm2, R, st);
+        const bool first = k == next;
+        const int na = launch_lanczos_a_part(A, o.fp32, k, w.p, nrm2, beta, first ? nullptr : RB.partial.p,
+                                             first ? 0 : nb, V.vec(k), y.p, alpha, RA, st);
+        nb = launch_lanczos_b_part(n, k, y.p, V.vec(k), V.vec(k - 1), alpha, beta, RA.partial.p, na, w.p, RB, st);
+        if (k == next + c - 1) launch_lanczos_b_finish(k, beta, nrm2, RB, nb, st);
       }
       double *h = C.host_stage(2 * ((size_t)block + 2));
       HIPCHK(hipMemcpyAsync(h, alpha, sizeof(double) * (next + c), hipMemcpyDeviceToHost, st));
@@ -232,7 +240,7 @@ int cg_solve(const Operator &A, double shift, const double *b, double *x, double
   double *rr = sc.p, *pap = sc.p + slots, *nrm2 = sc.p + 2 * slots;
   DevBuf<int> status(2);
   HIPCHK(hipMemsetAsync(status.p, 0, 2 * sizeof(int), st));
-  Reduce R;
+  Reduce R, RA, RB;
   launch_norm2(n, x, R, nrm2, st);
   launch_cg_init(A, shift, nrm2, x, xw.p, b, r.p, p.p, rr, R, st);
   out.spmv++;
@@ -240,10 +248,12 @@ int cg_solve(const Operator &A, double shift, const double *b, double *x, double
   int *hs = reinterpret_cast<int *>(C.host_stage(1));
   while (it < max_step) {
     const int c = std::min(chunk, max_step - it);
+    // three kernels per iteration, each summing the previous one's block partials itself
+    // (pAp in the update, |r|^2 and the convergence flag in the direction kernel)
     for (int i = it; i < it + c; i++) {
-      launch_cg_ap(A, shift, p.p, y.p, pap + i, status.p, R, st);
-      launch_cg_update(n, i, acc, p.p, y.p, xw.p, r.p, pap, rr, status.p, R, st);
-      launch_cg_dir(n, i, r.p, rr, p.p, status.p, st);
+      const int na = launch_cg_ap_part(A, shift, p.p, y.p, status.p, RA, st);
+      const int nb = launch_cg_update_part(n, i, p.p, y.p, xw.p, r.p, pap, rr, status.p, RA.partial.p, na, RB, st);
+      launch_cg_dir_part(n, i, acc, r.p, rr, p.p, status.p, RB.partial.p, nb, st);
     }
     HIPCHK(hipMemcpyAsync(hs, status.p, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
