@@ -170,11 +170,25 @@ __global__ __launch_bounds__(kBlock) void k_seg_lanczos_b(const SegDesc *desc, c
   const double bb = (d.k > 0) ? *d.s2 : 0.0;
   const double *vkm1 = d.o3 ? d.o3 : d.o1;
   double s = 0.0;
-  for (int64_t i = lb * (int64_t)kBlock + threadIdx.x; i < d.n; i += (int64_t)d.nblk * kBlock) {
-    double v = d.o2[i] - a * d.o1[i];
-    if (d.k > 0) v -= bb * vkm1[i];
-    const_cast<double *>(d.x)[i] = v;
-    s += v * v;
+  const int64_t S = (int64_t)d.nblk * kBlock;
+  for (int64_t i0 = lb * (int64_t)kBlock + threadIdx.x; i0 < d.n; i0 += 4 * S) {
+    double v[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int64_t i = i0 + u * S;
+      if (i < d.n) {
+        v[u] = d.o2[i] - a * d.o1[i];
+        if (d.k > 0) v[u] -= bb * vkm1[i];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int64_t i = i0 + u * S;
+      if (i < d.n) {
+        const_cast<double *>(d.x)[i] = v[u];
+        s += v[u] * v[u];
+      }
+    }
   }
   block_partial(s, partial);
 }
@@ -188,11 +202,21 @@ __global__ __launch_bounds__(kBlock) void k_seg_cg_update(const SegDesc *desc, c
   const int it = d.k;
   const double alpha = d.s2[it - 1] / d.s1[it];
   double s = 0.0;
-  for (int64_t i = lb * (int64_t)kBlock + threadIdx.x; i < d.n; i += (int64_t)d.nblk * kBlock) {
-    d.v1[i] = d.v1[i] + alpha * d.o3[i];
-    const double rv = d.v2[i] - alpha * d.o1[i];
-    d.v2[i] = rv;
-    s += rv * rv;
+  const int64_t S = (int64_t)d.nblk * kBlock;
+  for (int64_t i0 = lb * (int64_t)kBlock + threadIdx.x; i0 < d.n; i0 += 4 * S) {
+    double rv[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int64_t i = i0 + u * S;
+      if (i < d.n) {
+        d.v1[i] = d.v1[i] + alpha * d.o3[i];
+        rv[u] = d.v2[i] - alpha * d.o1[i];
+        d.v2[i] = rv[u];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+      if (i0 + u * S < d.n) s += rv[u] * rv[u];
   }
   block_partial(s, partial);
 }
@@ -220,10 +244,8 @@ __global__ __launch_bounds__(kBlock) void k_seg_cg_dir(const SegDesc *desc, cons
     d.v1[i] = d.v2[i] + beta * d.v1[i];
 }
 
-int vec_blocks(int64_t n) {
-  int64_t b = (n + kBlock - 1) / kBlock;
-  return (int)(b < kVecGrid ? (b > 0 ? b : 1) : kVecGrid);
-}
+// the vector kernels' blocks per slot: the single-solve count (same partials, same scalars)
+int vec_blocks(int64_t n) { return reduce_grid(n); }
 
 double now_sec() {
   return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();

@@ -18,6 +18,13 @@ namespace dmrgx {
 
 constexpr int kBlock = 256;          // threads per workgroup (4 wave64)
 constexpr int kVecGrid = 1024;       // fixed grid of the streaming vector kernels
+// Blocks of the vector kernels whose block partials the next kernel sums (Lanczos update,
+// CG update): about four elements per thread, at most kVecGrid blocks.  Single and batched
+// solves use the same count, so their partials (and scalars) are the same.
+inline int reduce_grid(int64_t n) {
+  const int64_t b = (n + 4 * kBlock - 1) / (4 * kBlock);
+  return (int)(b < 1 ? 1 : (b > kVecGrid ? kVecGrid : b));
+}
 constexpr int kGridCap = 4096;       // default cap on SpMV workgroups (row groups round-robin)
 
 // Error carried from the HIP layer up to the C entry points.

@@ -383,11 +383,26 @@ __global__ __launch_bounds__(kBlock) void k_lanczos_b(int64_t n, int k, const do
   }
   const double b = (k > 0) ? beta[k] : 0.0;
   double s = 0.0;
-  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
-    double v = y[i] - a * vk[i];
-    if (k > 0) v -= b * vkm1[i];
-    w[i] = v;
-    s += v * v;
+  // four elements per pass with their loads issued together; sums in index order
+  const int64_t S = (int64_t)gridDim.x * kBlock;
+  for (int64_t i0 = blockIdx.x * (int64_t)kBlock + threadIdx.x; i0 < n; i0 += 4 * S) {
+    double v[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int64_t i = i0 + u * S;
+      if (i < n) {
+        v[u] = y[i] - a * vk[i];
+        if (k > 0) v[u] -= b * vkm1[i];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int64_t i = i0 + u * S;
+      if (i < n) {
+        w[i] = v[u];
+        s += v[u] * v[u];
+      }
+    }
   }
   block_partial(s, partial);
 }
@@ -452,11 +467,21 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(int64_t n, int it, double 
   }
   const double alpha = rr[it - 1] / pAp;
   double s = 0.0;
-  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
-    x[i] = x[i] + alpha * p[i];
-    const double rv = r[i] - alpha * y[i];
-    r[i] = rv;
-    s += rv * rv;
+  const int64_t S = (int64_t)gridDim.x * kBlock;
+  for (int64_t i0 = blockIdx.x * (int64_t)kBlock + threadIdx.x; i0 < n; i0 += 4 * S) {
+    double rv[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int64_t i = i0 + u * S;
+      if (i < n) {
+        x[i] = x[i] + alpha * p[i];
+        rv[u] = r[i] - alpha * y[i];
+        r[i] = rv[u];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+      if (i0 + u * S < n) s += rv[u] * rv[u];
   }
   block_partial(s, partial);
 }
@@ -788,7 +813,7 @@ void launch_lanczos_a(const Operator &op, bool fp32, int k, const double *w, con
 void launch_lanczos_b(int64_t n, int k, const double *y, const double *vk, const double *vkm1,
                       const double *alpha, double *beta, double *w, double *nrm2, Reduce &R,
                       hipStream_t s) {
-  const int g = vec_grid(n);
+  const int g = reduce_grid(n);
   R.ensure(g);
   k_lanczos_b<<<g, kBlock, 0, s>>>(n, k, y, vk, vkm1 ? vkm1 : vk, const_cast<double *>(alpha), beta, w,
                                    R.partial.p, nullptr, 0);
@@ -808,7 +833,7 @@ int launch_lanczos_a_part(const Operator &op, bool fp32, int k, const double *w,
 
 int launch_lanczos_b_part(int64_t n, int k, const double *y, const double *vk, const double *vkm1, double *alpha,
                           const double *beta, const double *pre, int pre_n, double *w, Reduce &R, hipStream_t s) {
-  const int g = vec_grid(n);
+  const int g = reduce_grid(n);
   R.ensure(g);
   k_lanczos_b<<<g, kBlock, 0, s>>>(n, k, y, vk, vkm1 ? vkm1 : vk, alpha, beta, w, R.partial.p, pre, pre_n);
   HIPCHK(hipGetLastError());
@@ -859,7 +884,7 @@ void launch_cg_ap(const Operator &op, double shift, const double *p, double *y, 
 void launch_cg_update(int64_t n, int it, double acc, const double *p, const double *y, double *x,
                       double *r, const double *pap, double *rr, int *status, Reduce &R,
                       hipStream_t s) {
-  const int g = vec_grid(n);
+  const int g = reduce_grid(n);
   R.ensure(g);
   k_cg_update<<<g, kBlock, 0, s>>>(n, it, acc, p, y, x, r, const_cast<double *>(pap), rr, status, R.partial.p,
                                    nullptr, 0);
@@ -882,7 +907,7 @@ int launch_cg_ap_part(const Operator &op, double shift, const double *p, double 
 int launch_cg_update_part(int64_t n, int it, const double *p, const double *y, double *x, double *r, double *pap,
                           const double *rr, const int *status, const double *pre, int pre_n, Reduce &R,
                           hipStream_t s) {
-  const int g = vec_grid(n);
+  const int g = reduce_grid(n);
   R.ensure(g);
   k_cg_update<<<g, kBlock, 0, s>>>(n, it, 0.0, p, y, x, r, pap, rr, status, R.partial.p, pre, pre_n);
   HIPCHK(hipGetLastError());
