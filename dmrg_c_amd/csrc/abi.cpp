@@ -256,6 +256,13 @@ std::unique_ptr<dmrgx_matrix> upload_csr(int32_t n, const int64_t *row, const in
   hipStream_t st = C.stream;
   const int64_t nnz = row[n] - row[0];
   if (row[0] != 0 || nnz < 0) throw Failure(DMRGX_ERR_ARG, "CRS Row[0] must be 0 and Row non-decreasing");
+  // host checks before anything reaches the device: the kernels index x and the output
+  // rows with these arrays
+  for (int32_t r = 0; r < n; r++)
+    if (row[r + 1] < row[r]) throw Failure(DMRGX_ERR_ARG, "CRS Row must be non-decreasing");
+  if (nnz > 0 && (!col || !val)) throw Failure(DMRGX_ERR_ARG, "CRS Col/Val missing");
+  for (int64_t j = 0; j < nnz; j++)
+    if (col[j] < 0 || col[j] >= n) throw Failure(DMRGX_ERR_ARG, "CRS column index out of range");
   auto A = std::make_unique<dmrgx_matrix>();
   if (sym) {
     DevBuf<int64_t> tr((size_t)n + 1);
