@@ -182,11 +182,12 @@ def spmv_alg_bytes(D, A, sched, n):
     return A.nnz * 12 + (n + 1) * 8 + 2 * n * 8, "CSR"
 
 
-def traffic_probe(D, L, sb, snap):
+def traffic_probe(D, L, sb, snap, fixed=None):
     L.dmrgx_device_alloc.argtypes = [C.c_int64, C.POINTER(C.c_void_p)]
     L.dmrgx_device_free.argtypes = [C.c_void_p]
     L.dmrgx_copy_device.argtypes = [C.c_void_p, C.c_void_p, C.c_int64]
-    sched, _ = autotune(D, sb)
+    sched = fixed if fixed else autotune(D, sb)[0]
+    L.dmrgx_matrix_set_schedule.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_int32]
     A = sb.assemble(full=True)
     D.check(L.dmrgx_matrix_set_schedule(A.handle, *sched))
     n, ncopy = snap.n, 1 << 26
@@ -380,6 +381,8 @@ def main():
     ap.add_argument("--traffic-probe", action="store_true",
                     help="run 5 plain SpMV launches on the workload and one copy kernel of 2^26 doubles "
                          "(traffic calibration under rocprofv3 --pmc), then exit")
+    ap.add_argument("--schedule", default="",
+                    help="SpMV schedule 'lanes,cap,variant' instead of autotuning (bench and --traffic-probe)")
     ap.add_argument("--spmv-sweep", action="store_true",
                     help="time every SpMV schedule on the workload superblock (JSON table), then exit")
     ap.add_argument("--workload", choices=["superblock", "ed-batch", "serve"], default="superblock",
@@ -415,15 +418,20 @@ def main():
     params = dict(diag_acc=1e-14, diag_max_step=800, cg_acc=1e-7, cg_max_step=2000, ii_acc=1e-9,
                   ii_diag_add=1e-11, ii_max_step=3, seed=1)
     sb = D.Superblock(snap)  # inputs resident in HBM from here on
+    fixed = tuple(int(v) for v in args.schedule.split(",")) if args.schedule else None
     if args.traffic_probe:
-        traffic_probe(D, L, sb, snap)
+        traffic_probe(D, L, sb, snap, fixed)
         return
     if args.spmv_sweep:
         spmv_sweep(D, sb, snap)
         return
     log(f"workload: Hubbard L=64 m=400 superblock n={snap.n}")
-    lanes, tuned_us = autotune(D, sb)
-    log(f"autotune: (lanes per row, workgroup cap, load variant) = {lanes}, {tuned_us:.1f} us per SpMV + dot")
+    if fixed:
+        lanes, tuned_us = fixed, float("nan")
+        log(f"schedule given: (lanes per row, workgroup cap, load variant) = {lanes}")
+    else:
+        lanes, tuned_us = autotune(D, sb)
+        log(f"autotune: (lanes per row, workgroup cap, load variant) = {lanes}, {tuned_us:.1f} us per SpMV + dot")
     for i in range(args.warmup):
         E, st, nnz = gpu_solve(D, sb, params, lanes)
         log(f"warmup {i}: E={E:.15f} lanczos={st.lanczos_steps} cg={st.cg_steps}")

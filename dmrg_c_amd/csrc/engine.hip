@@ -1055,6 +1055,7 @@ int dmrgx_engine_run(dmrgx_engine *E, int32_t max_chunks) {
       const double t0 = now_sec();
       admit(*E);
       E->host_t[5] += now_sec() - t0;
+      retire(*E);   // requests finished at admission (1 x 1) leave the active set here
       if (E->active.empty() && !E->fl) break;
       std::unique_ptr<Inflight> next = launch_chunk(*E, E->opts.chunk_ticks);
       if (!next && !E->fl) break;
@@ -1064,6 +1065,7 @@ int dmrgx_engine_run(dmrgx_engine *E, int32_t max_chunks) {
     }
     // the last chunk stays in flight; the next call (or the engine's end) processes it
     admit(*E);
+    retire(*E);
   });
   return rc;
 }
