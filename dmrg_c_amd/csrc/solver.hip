@@ -88,6 +88,8 @@ void lanczos(const Operator &A, const LanczosOpts &o, double *vec, SolveOut &out
     launch_lanczos_b(n, 0, y.p, V.vec(0), nullptr, alpha, beta, w.p, nrm2, R, st);
     out.spmv++;
     int next = 1, kconv = -1, chunk = small ? 16 : 8;
+    double dE_first = 0.0, dE_last = 0.0;   // first and last |E_k - E_{k-1}| of the chunk
+    int n_dE = 0;
     double lam = E;
     RitzTrack track;  // T of this block
     bool block_done = false;
@@ -140,11 +142,25 @@ void lanczos(const Operator &A, const LanczosOpts &o, double *vec, SolveOut &out
           block_done = true;
           break;
         }
+        if (E < 1e14) {   // a real previous value (E starts at 1e15)
+          if (n_dE++ == 0) dE_first = std::fabs(E - lam);
+          dE_last = std::fabs(E - lam);
+        }
         E = lam;
       }
       if (kconv >= 0 || block_done) break;
       next += c;
-      chunk = std::min(chunk * 2, 32);
+      // Next chunk: the steps the last two energy changes predict ("Normal" convergence is
+      // geometric) plus two -- steps past convergence are computed for nothing -- or
+      // doubling while no prediction holds.
+      if (!small && o.type != 1 && n_dE >= 3 && dE_last > 0.0 && dE_last < dE_first) {
+        const double q = std::pow(dE_last / dE_first, 1.0 / (n_dE - 1));   // mean ratio per step
+        const double need = std::ceil(std::log(o.acc / dE_last) / std::log(q));
+        chunk = (int)std::min(32.0, std::max(8.0, need + 2.0));
+      } else {
+        chunk = std::min(chunk * 2, 32);
+      }
+      n_dE = 0;
     }
     if (kconv >= 0) {
       tridiag_vector(kconv + 1, ha.data(), hb.data() + 1, lam, s.data());
@@ -245,7 +261,10 @@ int cg_solve(const Operator &A, double shift, const double *b, double *x, double
   launch_cg_init(A, shift, nrm2, x, xw.p, b, r.p, p.p, rr, R, st);
   out.spmv++;
   int it = 1, chunk = 4, done_it = -1;
-  int *hs = reinterpret_cast<int *>(C.host_stage(1));
+  // read-back: {done, last iteration} and |r|^2 of the chunk's last iteration
+  double *hst = C.host_stage(2);
+  int *hs = reinterpret_cast<int *>(hst);
+  double rr_prev = -1.0;   // |r|^2 at the end of the previous chunk (rr[0] for the first)
   while (it < max_step) {
     const int c = std::min(chunk, max_step - it);
     // three kernels per iteration, each summing the previous one's block partials itself
@@ -256,13 +275,26 @@ int cg_solve(const Operator &A, double shift, const double *b, double *x, double
       launch_cg_dir_part(n, i, acc, r.p, rr, p.p, status.p, RB.partial.p, nb, st);
     }
     HIPCHK(hipMemcpyAsync(hs, status.p, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(hst + 1, rr + it + c - 1, sizeof(double), hipMemcpyDeviceToHost, st));
+    if (rr_prev < 0.0) HIPCHK(hipMemcpyAsync(&rr_prev, rr, sizeof(double), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
     if (hs[0]) {
       done_it = hs[1];
       break;
     }
+    // Next chunk: the iterations the chunk's mean decrease of |r|^2 predicts to reach acc,
+    // plus two (iterations beyond convergence are skipped on the device, but still cost
+    // their launches); doubling while |r|^2 does not decrease.
+    const double rr_last = hst[1];
+    const double q = (rr_prev > 0.0 && rr_last > 0.0) ? std::pow(rr_last / rr_prev, 1.0 / c) : 1.0;
+    if (q > 0.0 && q < 0.999 && rr_last > acc) {
+      const double need = std::ceil(std::log(acc / rr_last) / std::log(q));
+      chunk = (int)std::min(64.0, std::max(4.0, need + 2.0));
+    } else {
+      chunk = std::min(chunk * 2, 64);
+    }
+    rr_prev = rr_last;
     it += c;
-    chunk = std::min(chunk * 2, 64);
   }
   launch_copy(n, xw.p, x, st);
   HIPCHK(hipStreamSynchronize(st));
