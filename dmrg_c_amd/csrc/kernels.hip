@@ -539,19 +539,21 @@ __global__ __launch_bounds__(kBlock) void k_cg_dir(int64_t n, int it, const doub
 // ------------------------------------------------- lower triangle -> full CSR
 // bad[0] != 0 if a row is empty, its last entry is not the diagonal, or a column is out
 // of [0, row).  tcnt[c] counts the strictly-lower entries in column c.
+// One wavefront per row (coalesced column reads); four rows per workgroup.
 __global__ __launch_bounds__(kBlock) void k_tri_count(int32_t n, const int64_t *__restrict__ trow,
                                                       const int32_t *__restrict__ tcol,
                                                       int64_t *__restrict__ own,
                                                       unsigned *__restrict__ tcnt, int *bad) {
-  const int32_t r = blockIdx.x * kBlock + threadIdx.x;
+  const int32_t r = blockIdx.x * (kBlock / 64) + threadIdx.x / 64;
+  const int lane = threadIdx.x % 64;
   if (r >= n) return;
   const int64_t b = trow[r], e = trow[r + 1];
-  own[r] = e - b;
+  if (lane == 0) own[r] = e - b;
   if (e <= b || tcol[e - 1] != r) {
-    atomicOr(bad, 1);
+    if (lane == 0) atomicOr(bad, 1);
     return;
   }
-  for (int64_t j = b; j < e - 1; j++) {
+  for (int64_t j = b + lane; j < e - 1; j += 64) {
     const int32_t c = tcol[j];
     if (c < 0 || c >= r) {
       atomicOr(bad, 2);
@@ -1182,7 +1184,7 @@ void launch_expand_lower(int32_t n, int64_t nnz_tri, const int64_t *trow, const 
   HIPCHK(hipMemsetAsync(tcnt.p, 0, sizeof(unsigned) * n, s));
   HIPCHK(hipMemsetAsync(tfill.p, 0, sizeof(unsigned) * n, s));
   HIPCHK(hipMemsetAsync(bad.p, 0, sizeof(int), s));
-  k_tri_count<<<nb, kBlock, 0, s>>>(n, trow, tcol, own.p, tcnt.p, bad.p);
+  k_tri_count<<<(n + kBlock / 64 - 1) / (kBlock / 64), kBlock, 0, s>>>(n, trow, tcol, own.p, tcnt.p, bad.p);
   HIPCHK(hipGetLastError());
   int hbad = 0;
   HIPCHK(hipMemcpyAsync(&hbad, bad.p, sizeof(int), hipMemcpyDeviceToHost, s));
