@@ -492,8 +492,8 @@ def main():
             "lanczos_steps": int(np.median(lan)),
             "cg_steps": int(np.median(cg)),
             "spmv_schedule": {"lanes_per_row": lanes[0], "workgroup_cap": lanes[1],
-                              "nontemporal_matrix_loads": bool(lanes[2] & 1),
-                              "loads_in_flight_per_lane": 8 if lanes[2] & 2 else 4,
+                              "paired_entry_loads": bool(lanes[2] & 1) and (lanes[2] >> 3) == 0,
+                              "entries_in_flight_per_lane": 8 if lanes[2] & 2 else 4,
                               "xcd_contiguous_rows": bool(lanes[2] & 4),
                               "format": ("CSR", "value dictionary", "delta columns")[lanes[2] >> 3]},
         },

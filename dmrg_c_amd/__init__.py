@@ -221,9 +221,9 @@ class DeviceMatrix:
     @property
     def schedule(self) -> Tuple[int, int, int]:
         """(lanes per row, workgroup cap, load variant) of this matrix's SpMV; load variant
-        bit 0 = non-temporal matrix loads, bit 1 = eight loads in flight per lane, bit 2 =
-        contiguous row ranges per workgroup and per XCD, bits 3-4 = storage format (0 CSR,
-        1 value dictionary, 2 delta columns)."""
+        bit 0 = two entries per lane per load (CSR), bit 1 = eight entries in flight per lane,
+        bit 2 = contiguous row ranges per workgroup and per XCD, bits 3-4 = storage format
+        (0 CSR, 1 value dictionary, 2 delta columns)."""
         g, cap, nt = C.c_int32(), C.c_int32(), C.c_int32()
         check(lib().dmrgx_matrix_get_schedule(self.handle, C.byref(g), C.byref(cap), C.byref(nt)))
         return g.value, cap.value, nt.value

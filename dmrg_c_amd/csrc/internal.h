@@ -251,11 +251,13 @@ __device__ __forceinline__ double csr_lane_sum(const int64_t *__restrict__ row, 
   return s;
 }
 
-// Variants of the CSR product.  Bit 0: non-temporal matrix loads; bit 1: eight loads in
-// flight per lane instead of four (neither changes a result).  Bit 2: row groups in
-// contiguous ranges per block, blocks mapped so that each XCD takes a contiguous share of
-// the rows (with a capped grid this changes the order of the fused dot products).
-constexpr int kLoadNT = 1;
+// Variants of the CSR product.  Bit 0 (CSR only): two entries per lane per load -- a
+// 16-byte value pair and an 8-byte column pair -- which changes the summation order within
+// a row; bit 1: eight entries in flight per lane instead of four (never changes a result).
+// Bit 2: row groups in contiguous ranges per block, blocks mapped so that each XCD takes
+// a contiguous share of the rows (with a capped grid this changes the order of the fused
+// dot products).
+constexpr int kLoadPair = 1;   // CSR: two entries per lane per load (changes the order)
 constexpr int kLoadDeep = 2;
 constexpr int kLoadXcd = 4;
 // Bits 3-4: storage format of the product.  CSR: 4-byte columns; dictionary (FP64 products
@@ -373,21 +375,74 @@ template <int G, int V, class VT>
 __device__ __forceinline__ double delta_lane_sum_v(const int64_t *__restrict__ row, const int32_t *__restrict__ first,
                                                    const uint16_t *__restrict__ dcol, const VT *__restrict__ val,
                                                    int32_t r, int lane, const double *__restrict__ x) {
-  return delta_lane_sum<G, (V & kLoadDeep) ? 8 : 4, (V & kLoadNT) != 0>(row, first, dcol, val, r, lane, x);  // V < 4
+  return delta_lane_sum<G, (V & kLoadDeep) ? 8 : 4, false>(row, first, dcol, val, r, lane, x);  // V < 4
 }
 
 template <int G, int V>
 __device__ __forceinline__ double dict_lane_sum_v(const int64_t *__restrict__ row, const uint64_t *__restrict__ ent,
                                                   const double *__restrict__ table, int32_t r, int lane,
                                                   const double *__restrict__ x) {
-  return dict_lane_sum<G, (V & kLoadDeep) ? 8 : 4, (V & kLoadNT) != 0>(row, ent, table, r, lane, x);  // V < 4
+  return dict_lane_sum<G, (V & kLoadDeep) ? 8 : 4, false>(row, ent, table, r, lane, x);  // V < 4
+}
+
+// CSR sum with two entries per lane per load: pair q = entries (2q, 2q+1), loaded as one
+// 16-byte value pair and one 8-byte column pair (half the memory instructions of
+// csr_lane_sum for the matrix arrays).  Lane l takes the row's full pairs q_lo + l,
+// q_lo + l + G, ...; lane 0 adds the unpaired first entry (odd row start) before its pairs
+// and the unpaired last entry (odd row end) after them.  A fixed order, different from
+// csr_lane_sum's.
+template <class VT> struct PairOf;
+template <> struct PairOf<double> { using type = double2; };
+template <> struct PairOf<float> { using type = float2; };
+
+template <int G, int U, class VT>
+__device__ __forceinline__ double csr_pair_lane_sum(const int64_t *__restrict__ row, const int32_t *__restrict__ col,
+                                                    const VT *__restrict__ val, int32_t r, int lane,
+                                                    const double *__restrict__ x) {
+  using V2 = typename PairOf<VT>::type;
+  double s = 0.0;
+  const int64_t b = row[r], e = row[r + 1];
+  if (b >= e) return s;
+  const int64_t q_lo = (b + 1) >> 1, q_hi = e >> 1;   // full pairs [q_lo, q_hi)
+  if ((b & 1) && lane == 0) s += (double)val[b] * x[col[b]];
+  const V2 *__restrict__ v2 = reinterpret_cast<const V2 *>(val);
+  const int2 *__restrict__ c2 = reinterpret_cast<const int2 *>(col);
+  for (int64_t q0 = q_lo + lane; q0 < q_hi; q0 += U * G) {
+    V2 v[U];
+    int2 c[U];
+    bool p[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      p[u] = q0 + u * G < q_hi;
+      const int64_t q = p[u] ? q0 + u * G : q_hi - 1;
+      v[u] = v2[q];
+      c[u] = c2[q];
+    }
+    double x0[U], x1[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      x0[u] = x[c[u].x];
+      x1[u] = x[c[u].y];
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++)
+      if (p[u]) {
+        s += (double)v[u].x * x0[u];
+        s += (double)v[u].y * x1[u];
+      }
+  }
+  if ((e & 1) && e - 1 >= 2 * q_lo && lane == 0) s += (double)val[e - 1] * x[col[e - 1]];
+  return s;
 }
 
 template <int G, int V, class VT>
 __device__ __forceinline__ double csr_lane_sum_v(const int64_t *__restrict__ row, const int32_t *__restrict__ col,
                                                  const VT *__restrict__ val, int32_t r, int lane,
                                                  const double *__restrict__ x) {
-  return csr_lane_sum<G, (V & kLoadDeep) ? 8 : 4, (V & kLoadNT) != 0>(row, col, val, r, lane, x);  // V < 4
+  if constexpr ((V & kLoadPair) != 0)
+    return csr_pair_lane_sum<G, (V & kLoadDeep) ? 4 : 2, VT>(row, col, val, r, lane, x);  // V < 4
+  else
+    return csr_lane_sum<G, (V & kLoadDeep) ? 8 : 4, false>(row, col, val, r, lane, x);
 }
 
 // Binary search of an ascending int64 array: index of target or -1

@@ -468,17 +468,18 @@ int dmrgx_spmv_device(const dmrgx_matrix *A, const double *x_dev, double *y_dev)
 /* y = A x with host pointers. */
 int dmrgx_spmv_host(const dmrgx_matrix *A, const double *x, double *y);
 /* SpMV schedule of this matrix: lanes (threads) per row, a power of two 1..64; the cap on
- * workgroups (0 = one workgroup per row group); and the load variant 0..23 (bit 0:
- * non-temporal loads of the matrix arrays; bit 1: eight loads in flight per lane instead of
- * four; bit 2: each workgroup takes a contiguous range of row groups and each XCD a
- * contiguous share of the rows, instead of round-robin groups; bits 3-4, the storage format
+ * workgroups (0 = one workgroup per row group); and the load variant 0..23 (bit 0, CSR
+ * format only: two entries per lane per load, a 16-byte value pair and an 8-byte column
+ * pair; bit 1: eight entries in flight per lane instead of four; bit 2: each workgroup
+ * takes a contiguous range of row groups and each XCD a contiguous share of the rows,
+ * instead of round-robin groups; bits 3-4, the storage format
  * the products read: 0 CSR, 1 value dictionary -- 8-byte entries (column, index into a
  * table of the matrix's distinct values), FP64 products only, when the matrix has few
  * distinct values -- 2 delta columns -- 2-byte differences of consecutive columns in a row
  * plus the row's first column, when every difference fits; a format the matrix cannot have
- * reads CSR; formats are built on the device at their first use).  Bits 0, 1, 3 and 4 never
- * change results; lanes change the summation order within a row, and cap and bit 2 the
- * order of the dot products fused into the solvers' products.  Defaults: lanes
+ * reads CSR; formats are built on the device at their first use).  Bits 1, 3 and 4 never
+ * change results; lanes and bit 0 change the summation order within a row, and cap and
+ * bit 2 the order of the dot products fused into the solvers' products.  Defaults: lanes
  * from the mean row length, cap 4096, variant 2;
  * dmrgx_matrix_autotune times the candidates on the device (product + reduction, as the
  * solvers run it) and keeps the fastest.  Single and batched solves use the matrix's

@@ -31,15 +31,19 @@ def test_gpu_ragged_rows_every_schedule(distinct):
         x = np.random.default_rng(1).standard_normal(n)
         ref = sp.csr_matrix((val, col, row), shape=(n, n)) @ x
         for g in (1, 2, 4, 8, 16, 32, 64):
-            base = None
-            for v in (0, 2, 5, 8, 10, 16, 18, 23):
+            base, pair = None, None
+            for v in (0, 1, 2, 3, 5, 8, 10, 16, 18, 23):
                 A.set_schedule(g, 2048 if v & 4 else 0, v)
                 y = A.matvec(x)
                 assert np.abs(y - ref).max() <= 1e-12 * (np.abs(ref).max() + 1)
-                if v & 4 == 0:
-                    if base is None:
-                        base = y
-                    assert np.array_equal(y, base), (g, v)   # formats and load bits: same bits
+                if v & 4:
+                    continue
+                if v in (1, 3):                              # paired CSR loads: their own order
+                    pair = y if pair is None else pair
+                    assert np.array_equal(y, pair), (g, v)
+                else:
+                    base = y if base is None else base
+                    assert np.array_equal(y, base), (g, v)   # formats and in-flight depth: same bits
         assert np.all(A.matvec(np.ones(n))[np.diff(row) == 0] == 0.0)   # empty rows give 0
     finally:
         A.free()
