@@ -264,20 +264,30 @@ def run_ed_batch(D, count, precision_forms=(0, 1), single=True):
         t0 = time.perf_counter()
         es = [A.ground_state(**p)[0] for A in mats] if single else None
         t_single = time.perf_counter() - t0
-        t0 = time.perf_counter()
-        E = engine.Engine()
         ep = engine.default_params(**p)
-        for i, A in enumerate(mats):
-            E.submit(A, ep, tag=i)
-        out = sorted(E.run_until_done(want_vec=False), key=lambda r: r.tag)
-        t_batch = time.perf_counter() - t0
-        host = E.host_times()
-        E.free()
-        log(f"ed-batch precision {prec}: engine host seconds {host}")
+
+        def batch(use_graphs):
+            t0 = time.perf_counter()
+            E = engine.Engine(use_graphs=use_graphs)
+            for i, A in enumerate(mats):
+                E.submit(A, ep, tag=i)
+            out = sorted(E.run_until_done(want_vec=False), key=lambda r: r.tag)
+            t = time.perf_counter() - t0
+            host, graphs = E.host_times(), E.graph_stats()
+            E.free()
+            return out, t, host, graphs
+
+        out_eager, t_eager, _, _ = batch(False)
+        out, t_batch, host, graphs = batch(True)
+        same_eager = all(a.energy == b.energy for a, b in zip(out, out_eager))
+        host["graphs"] = graphs
+        log(f"ed-batch precision {prec}: engine host seconds {host}; without graphs {count / t_eager:.1f}/s, "
+            f"graph results identical to eager: {same_eager}")
         eb = [o.energy for o in out]
         tag = "fp64" if prec == 0 else "fp32_refined"
+        g = {"batched_solves_per_s_without_graphs": round(count / t_eager, 2), "graphs_equal_eager": same_eager}
         if not single:
-            res[tag] = {"batched_solves_per_s": round(count / t_batch, 2), "engine_host_s": host}
+            res[tag] = {"batched_solves_per_s": round(count / t_batch, 2), "engine_host_s": host, **g}
             log(f"ed-batch {tag}: batched {count / t_batch:.1f}/s")
             continue
         same = all(a == b for a, b in zip(es, eb))
@@ -288,7 +298,7 @@ def run_ed_batch(D, count, precision_forms=(0, 1), single=True):
             err = max(abs(a - b) / abs(b) for a, b in zip(eb, single_e))
         res[tag] = {"single_solves_per_s": round(count / t_single, 2), "batched_solves_per_s": round(count / t_batch, 2),
                     "batched_over_single": round(t_single / t_batch, 2), "batched_equals_single": same,
-                    "max_rel_energy_diff_vs_fp64": err, "batched_wall_s": round(t_batch, 3), "engine_host_s": host}
+                    "max_rel_energy_diff_vs_fp64": err, "batched_wall_s": round(t_batch, 3), "engine_host_s": host, **g}
         log(f"ed-batch {tag}: single {count / t_single:.1f}/s, batched {count / t_batch:.1f}/s, identical={same}")
     for A in mats:
         A.free()
@@ -357,6 +367,7 @@ def run_serve(D, rate, duration, seed=0):
             if kind == "sb":
                 errs.append(abs(r.energy - refs[what]) / abs(refs[what]))
     wall = time.perf_counter() - t0
+    graphs = E.graph_stats()
     E.free()
     for sb in sbs:
         sb.free()
@@ -367,6 +378,7 @@ def run_serve(D, rate, duration, seed=0):
             "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 2),
             "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 2),
             "max_rel_energy_error_superblocks": float(max(errs)) if errs else None,
+            "engine_chunks": graphs,
             "note": "superblock requests drawn from the reference-run snapshots in tests/golden "
                     "(n up to 144542), assembled on the device at arrival"}
 

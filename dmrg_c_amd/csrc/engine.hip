@@ -129,8 +129,10 @@ __global__ __launch_bounds__(kBlock) void k_seg_spmv(const SegDesc *desc, const 
     if (fmt == kFmtDelta) {                                                                    \
       if (d.fp32) s = delta_lane_sum_v<G, V>(d.row, d.first, d.dcol, d.val32, r, lane, d.x);   \
       else s = delta_lane_sum_v<G, V>(d.row, d.first, d.dcol, d.val, r, lane, d.x);          \
-    } else if (d.fp32) {                                                                       \
+    } else if (d.fp32 && fmt == kFmtCsr) {                                                     \
       s = csr_lane_sum_v<G, V>(d.row, d.col, d.val32, r, lane, d.x);                           \
+    } else if (d.fp32) { /* FP32 values of a dictionary matrix: CSR without paired loads */    \
+      s = csr_lane_sum_v<G, (V & ~kLoadPair)>(d.row, d.col, d.val32, r, lane, d.x);            \
     } else if (fmt == kFmtDict) {                                                              \
       s = dict_lane_sum_v<G, V>(d.row, d.ent, d.table, r, lane, d.x);                          \
     } else {                                                                                   \
@@ -356,8 +358,14 @@ struct dmrgx_engine {
   hipEvent_t rb_ev[2] = {nullptr, nullptr};
   int area = 0;
   std::unique_ptr<dmrgx::Inflight> fl;   // launched, read-back not processed yet (slots by id)
+  // hipGraph of the last chunk launch sequence seen twice in a row (kernels only: the
+  // descriptor uploads and the read-back stay outside), replayed while it repeats
+  std::vector<int64_t> prev_sig, graph_sig;
+  hipGraphExec_t graph = nullptr;
+  int64_t n_eager = 0, n_captures = 0, n_replays = 0;
   ~dmrgx_engine() {
     (void)hipStreamSynchronize(dmrgx::ctx().stream);
+    if (graph) (void)hipGraphExecDestroy(graph);
     for (int i = 0; i < 2; i++) {
       if (up_pin[i]) (void)hipHostFree(up_pin[i]);
       if (rb_pin[i]) (void)hipHostFree(rb_pin[i]);
@@ -867,29 +875,74 @@ std::unique_ptr<Inflight> launch_chunk(dmrgx_engine &E, int ticks) {
   if (bm) HIPCHK(hipMemcpyAsync(E.dblk.p, pin + bd, bm, hipMemcpyHostToDevice, st));
   if (bg) HIPCHK(hipMemcpyAsync(E.dgather.p, pin + og, bg, hipMemcpyHostToDevice, st));
   lap(0);
-  for (const Launch &l : launches) {
-    const SegDesc *dd = E.ddesc.p + l.desc_off;
-    const int32_t *bb = E.dblk.p + l.blk_off;
-    switch (l.kind) {
-      case 0:
-        launch_seg_spmv<LANCZOS_A>(l.G, l.nblk, dd, bb, l.nseg, E.partial.p, st);
-        k_seg_finish<FIN_LANCZOS_A><<<l.nseg, kBlock, 0, st>>>(dd, E.partial.p);
-        break;
-      case 1:
-        k_seg_lanczos_b<<<l.nblk, kBlock, 0, st>>>(dd, bb, l.nseg, E.partial.p);
-        k_seg_finish<FIN_LANCZOS_B><<<l.nseg, kBlock, 0, st>>>(dd, E.partial.p);
-        break;
-      case 2:
-        launch_seg_spmv<CG_AP>(l.G, l.nblk, dd, bb, l.nseg, E.partial.p, st);
-        k_seg_finish<FIN_CG_AP><<<l.nseg, kBlock, 0, st>>>(dd, E.partial.p);
-        break;
-      case 3:
-        k_seg_cg_update<<<l.nblk, kBlock, 0, st>>>(dd, bb, l.nseg, E.partial.p);
-        k_seg_finish<FIN_CG_UPDATE><<<l.nseg, kBlock, 0, st>>>(dd, E.partial.p);
-        break;
-      case 4: k_seg_cg_dir<<<l.nblk, kBlock, 0, st>>>(dd, bb, l.nseg); break;
+  auto run_launches = [&]() {
+    for (const Launch &l : launches) {
+      const SegDesc *dd = E.ddesc.p + l.desc_off;
+      const int32_t *bb = E.dblk.p + l.blk_off;
+      switch (l.kind) {
+        case 0:
+          launch_seg_spmv<LANCZOS_A>(l.G, l.nblk, dd, bb, l.nseg, E.partial.p, st);
+          k_seg_finish<FIN_LANCZOS_A><<<l.nseg, kBlock, 0, st>>>(dd, E.partial.p);
+          break;
+        case 1:
+          k_seg_lanczos_b<<<l.nblk, kBlock, 0, st>>>(dd, bb, l.nseg, E.partial.p);
+          k_seg_finish<FIN_LANCZOS_B><<<l.nseg, kBlock, 0, st>>>(dd, E.partial.p);
+          break;
+        case 2:
+          launch_seg_spmv<CG_AP>(l.G, l.nblk, dd, bb, l.nseg, E.partial.p, st);
+          k_seg_finish<FIN_CG_AP><<<l.nseg, kBlock, 0, st>>>(dd, E.partial.p);
+          break;
+        case 3:
+          k_seg_cg_update<<<l.nblk, kBlock, 0, st>>>(dd, bb, l.nseg, E.partial.p);
+          k_seg_finish<FIN_CG_UPDATE><<<l.nseg, kBlock, 0, st>>>(dd, E.partial.p);
+          break;
+        case 4: k_seg_cg_dir<<<l.nblk, kBlock, 0, st>>>(dd, bb, l.nseg); break;
+      }
     }
+  };
+  // the launch sequence: kernels, grids, descriptor offsets and the scratch buffers they use
+  std::vector<int64_t> sig;
+  sig.reserve(6 * launches.size() + 3);
+  for (const Launch &l : launches) {
+    sig.push_back(l.kind);
+    sig.push_back(l.G);
+    sig.push_back(l.nblk);
+    sig.push_back(l.nseg);
+    sig.push_back((int64_t)l.desc_off);
+    sig.push_back((int64_t)l.blk_off);
   }
+  sig.push_back((int64_t)(intptr_t)E.ddesc.p);
+  sig.push_back((int64_t)(intptr_t)E.dblk.p);
+  sig.push_back((int64_t)(intptr_t)E.partial.p);
+  if (E.opts.use_graphs && !launches.empty() && E.graph && sig == E.graph_sig) {
+    HIPCHK(hipGraphLaunch(E.graph, st));
+    E.n_replays++;
+  } else if (E.opts.use_graphs && !launches.empty() && sig == E.prev_sig) {
+    // the same sequence twice in a row: capture it (kernel launches only) and replay it
+    hipGraph_t g = nullptr;
+    HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    try {
+      run_launches();
+    } catch (...) {
+      (void)hipStreamEndCapture(st, &g);
+      if (g) (void)hipGraphDestroy(g);
+      throw;
+    }
+    HIPCHK(hipStreamEndCapture(st, &g));
+    hipGraphExec_t ex = nullptr;
+    const hipError_t ie = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    HIPCHK(ie);
+    if (E.graph) HIPCHK(hipGraphExecDestroy(E.graph));
+    E.graph = ex;
+    E.graph_sig = sig;
+    HIPCHK(hipGraphLaunch(E.graph, st));
+    E.n_captures++;
+  } else {
+    run_launches();
+    E.n_eager++;
+  }
+  E.prev_sig.swap(sig);
   HIPCHK(hipGetLastError());
   double *h = static_cast<double *>(grow_pinned(E.rb_pin[area], E.rb_bytes[area], sizeof(double) * words));
   if (!gd.empty()) {
@@ -1007,6 +1060,7 @@ void dmrgx_engine_default_opts(dmrgx_engine_opts *o) {
   o->max_active = 1024;
   o->chunk_ticks = 16;
   o->basis_steps_estimate = 200;
+  o->use_graphs = 1;
 }
 
 int dmrgx_engine_create(const dmrgx_engine_opts *o, dmrgx_engine **out) {
@@ -1075,6 +1129,14 @@ int dmrgx_engine_host_times(const dmrgx_engine *E, double *seconds6, int64_t *ch
   if (seconds6)
     for (int i = 0; i < 6; i++) seconds6[i] = E->host_t[i];
   if (chunks) *chunks = E->chunks;
+  return DMRGX_OK;
+}
+
+int dmrgx_engine_graph_stats(const dmrgx_engine *E, int64_t *eager, int64_t *captures, int64_t *replays) {
+  if (!E) return DMRGX_ERR_ARG;
+  if (eager) *eager = E->n_eager;
+  if (captures) *captures = E->n_captures;
+  if (replays) *replays = E->n_replays;
   return DMRGX_OK;
 }
 

@@ -337,7 +337,11 @@ int apply_op(const Operator &op, bool fp32, const double *x, const Epi &epi, Red
   if (fp32) {
     if (!op.A->val32.p) throw Failure(DMRGX_ERR_ARG, "FP32 values not prepared");
     if (delta) return apply_delta<float>(op, op.A->val32.p, x, epi, R, s, fin);
-    return apply_csr<float>(op, op.A->val32.p, x, epi, R, s, fin);
+    if (fmt == kFmtCsr) return apply_csr<float>(op, op.A->val32.p, x, epi, R, s, fin);
+    // FP32 values of a dictionary-format matrix are read as CSR, without paired-entry loads
+    Operator csr = op;
+    csr.load &= ~kLoadPair;
+    return apply_csr<float>(csr, op.A->val32.p, x, epi, R, s, fin);
   }
   if (delta) return apply_delta<double>(op, op.A->val.p, x, epi, R, s, fin);
   if (fmt == kFmtDict && ensure_dict(*op.A)) return apply_dict(op, x, epi, R, s, fin);

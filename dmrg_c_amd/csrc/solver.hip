@@ -242,7 +242,8 @@ bool ensure_delta(const dmrgx_matrix &A) {
 int effective_load(const dmrgx_matrix &A) {
   const int fmt = load_format(A.load);
   const bool ok = fmt == kFmtCsr || (fmt == kFmtDict && A.n_values > 0) || (fmt == kFmtDelta && A.delta > 0);
-  return ok ? A.load : (A.load & ~kLoadFmtMask);
+  // paired-entry loads are a choice of the CSR format: its fallback for another format drops them
+  return ok ? A.load : (A.load & ~(kLoadFmtMask | kLoadPair));
 }
 
 int cg_solve(const Operator &A, double shift, const double *b, double *x, double acc, int max_step,

@@ -12,7 +12,7 @@ from . import DeviceMatrix, SolveParams, SolveStats, check, lib as _lib
 
 class EngineOpts(C.Structure):
     _fields_ = [("hbm_budget_bytes", C.c_int64), ("max_active", C.c_int32), ("chunk_ticks", C.c_int32),
-                ("basis_steps_estimate", C.c_int32), ("reserved", C.c_int32)]
+                ("basis_steps_estimate", C.c_int32), ("use_graphs", C.c_int32)]
 
 
 class EngineResult(C.Structure):
@@ -43,6 +43,8 @@ def lib() -> C.CDLL:
         L.dmrgx_engine_free.argtypes = [vp]
         L.dmrgx_ground_state_batch.argtypes = [C.c_int32, vp, C.POINTER(SolveParams), vp, C.POINTER(SolveStats)]
         L.dmrgx_engine_host_times.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_int64)]
+        i64p = C.POINTER(C.c_int64)
+        L.dmrgx_engine_graph_stats.argtypes = [vp, i64p, i64p, i64p]
         _bound = True
     return L
 
@@ -77,7 +79,8 @@ class Result:
 
 class Engine:
     def __init__(self, hbm_budget_bytes: Optional[int] = None, max_active: Optional[int] = None,
-                 chunk_ticks: Optional[int] = None, basis_steps_estimate: Optional[int] = None):
+                 chunk_ticks: Optional[int] = None, basis_steps_estimate: Optional[int] = None,
+                 use_graphs: Optional[bool] = None):
         L = lib()
         o = EngineOpts()
         L.dmrgx_engine_default_opts(C.byref(o))
@@ -89,6 +92,8 @@ class Engine:
             o.chunk_ticks = chunk_ticks
         if basis_steps_estimate is not None:
             o.basis_steps_estimate = basis_steps_estimate
+        if use_graphs is not None:
+            o.use_graphs = 1 if use_graphs else 0
         h = C.c_void_p()
         check(L.dmrgx_engine_create(C.byref(o), C.byref(h)))
         self.handle = h
@@ -140,6 +145,12 @@ class Engine:
         out = {k: round(t[i], 4) for i, k in enumerate(keys)}
         out["chunks"] = c.value
         return out
+
+    def graph_stats(self) -> dict:
+        """Chunks launched eagerly, hipGraphs captured, graph replays."""
+        e, c, r = C.c_int64(), C.c_int64(), C.c_int64()
+        check(lib().dmrgx_engine_graph_stats(self.handle, C.byref(e), C.byref(c), C.byref(r)))
+        return {"eager": e.value, "captures": c.value, "replays": r.value}
 
     def free(self):
         if getattr(self, "handle", None):

@@ -642,7 +642,8 @@ typedef struct {
    int32_t max_active;         /* requests advancing together                           */
    int32_t chunk_ticks;        /* steps between host checks                             */
    int32_t basis_steps_estimate; /* Lanczos vectors assumed per request for admission   */
-   int32_t reserved;
+   int32_t use_graphs;         /* 1: a chunk whose launch sequence repeats the previous
+                                  one's is captured once as a hipGraph and replayed        */
 } dmrgx_engine_opts;
 typedef struct {
    int64_t request_id;
@@ -673,6 +674,8 @@ void dmrgx_engine_free(dmrgx_engine *E);
  * checks (with the Ritz vectors and first residual checks they start), [4] CG ends and
  * inverse-iteration rounds, [5] admitting requests; *chunks = chunks run. */
 int dmrgx_engine_host_times(const dmrgx_engine *E, double *seconds6, int64_t *chunks);
+/* Chunks launched eagerly, graphs captured, graph replays. */
+int dmrgx_engine_graph_stats(const dmrgx_engine *E, int64_t *eager, int64_t *captures, int64_t *replays);
 /* count solves through one engine; vec_out[i] (host, n_i doubles) and stats[i] may be NULL. */
 int dmrgx_ground_state_batch(int32_t count, dmrgx_matrix *const *A, const dmrgx_solve_params *p,
                              double *const *vec_out, dmrgx_solve_stats *stats);

@@ -132,3 +132,40 @@ def test_gpu_engine_owned_matrices_and_many_small():
     assert len(res) == 64
     for r in res:
         assert r.energy == expect[r.tag]
+
+
+@pytest.mark.gpu
+def test_gpu_engine_graph_replays_bit_identical_to_eager():
+    """Chunks whose launch sequence repeats run as a replayed hipGraph; results match the
+    eager engine bit for bit, over many replays, and over two engines in a row."""
+    from dmrg_c_amd import ed, engine
+    mats = []
+    for i in range(6):
+        P = ed.EdProblem.from_model(ed.Model("xxz", L=14, sz=0, J_z=0.6 + 0.1 * i, h_z=0.05 * i))
+        mats.append(P.assemble())
+        P.free()
+    mats += _ed_matrices()
+
+    def run(use_graphs):
+        E = engine.Engine(chunk_ticks=4, use_graphs=use_graphs)
+        for i, A in enumerate(mats):
+            E.submit(A, tag=i)
+        res = {r.tag: r for r in E.run_until_done()}
+        st = E.graph_stats()
+        E.free()
+        return res, st
+
+    eager, st0 = run(False)
+    assert st0["captures"] == 0 and st0["replays"] == 0 and st0["eager"] > 0
+    for _ in range(2):
+        got, st1 = run(True)
+        assert st1["captures"] >= 1 and st1["replays"] >= 1, st1
+        assert sorted(got) == sorted(eager)
+        for i in eager:
+            assert got[i].status == 0
+            assert got[i].energy == eager[i].energy
+            assert np.array_equal(got[i].vec, eager[i].vec)
+            assert got[i].lanczos_steps == eager[i].lanczos_steps
+            assert got[i].cg_steps == eager[i].cg_steps
+    for A in mats:
+        A.free()
