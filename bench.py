@@ -34,6 +34,7 @@ REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, REPO)
 WORKLOAD = os.path.join(REPO, "tests", "golden", "hubbard_L64_m400_s31.npz")
 TRAFFIC = os.path.join(REPO, "profiles", "spmv_hubbard_L64_m400_traffic.json")
+PROFILE_EVERY = 16
 PEAK_HBM_GBS = 8000.0
 
 
@@ -448,7 +449,9 @@ def main():
         E, st, nnz = gpu_solve(D, sb, params, lanes)
         log(f"warmup {i}: E={E:.15f} lanczos={st.lanczos_steps} cg={st.cg_steps}")
 
-    L.dmrgx_profile_begin()
+    # events bracket one SpMV launch in PROFILE_EVERY: every bracketed launch costs the
+    # stream a few microseconds of idle, which would slow the timed solves themselves
+    D.check(L.dmrgx_profile_begin_sampled(PROFILE_EVERY))
     lat, errs, lan, cg = [], [], [], []
     t0 = time.perf_counter()
     for i in range(args.steps):
@@ -529,6 +532,7 @@ def main():
             "alg_bytes_per_launch": int(bytes_per_launch),
             "avg_launch_us": round(avg_launch_s * 1e6, 2),
             "launches": int(launches.value),
+            "launches_sampled": f"1 in {PROFILE_EVERY} SpMV launches of the timed solves, HIP events on the library stream",
         },
         "cpu_baseline": None,
     }

@@ -616,15 +616,20 @@ int dmrgx_ground_state(const dmrgx_matrix *A, const dmrgx_solve_params *p, doubl
   });
 }
 
-int dmrgx_profile_begin(void) {
+int dmrgx_profile_begin_sampled(int32_t every) {
+  if (every < 1) return DMRGX_ERR_ARG;
   return run_status([&] {
     Context &C = ctx();
     HIPCHK(hipStreamSynchronize(C.stream));
     C.prof = true;
     C.prof_used = 0;
     C.prof_bytes = 0.0;
+    C.prof_every = every;
+    C.prof_seen = 0;
   });
 }
+
+int dmrgx_profile_begin(void) { return dmrgx_profile_begin_sampled(1); }
 
 int dmrgx_profile_end(double *spmv_ms, int64_t *spmv_launches, double *spmv_alg_bytes) {
   return run_status([&] {

@@ -61,7 +61,10 @@ struct Context {
   std::vector<std::pair<hipEvent_t, hipEvent_t>> prof_ev;
   size_t prof_used = 0;
   double prof_bytes = 0.0;
+  int prof_every = 1;        // bracket one launch in every prof_every
+  int64_t prof_seen = 0;     // launches seen since begin
   std::pair<hipEvent_t, hipEvent_t> *prof_slot();
+  bool prof_take() { return prof && (prof_seen++ % prof_every) == 0; }
   void sizes_note(void *p, size_t bytes) { live[p] = bytes; }
   size_t sizes_take(void *p) {
     auto it = live.find(p);

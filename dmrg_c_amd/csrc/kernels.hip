@@ -15,6 +15,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <type_traits>
 
 #include "internal.h"
 #include "reduce.h"
@@ -89,6 +90,13 @@ struct DeltaRows {
 // takes several groups) round-robin, or with xcd set in contiguous ranges over XCD-mapped
 // logical blocks; each thread's epilogue contributions are summed in the order of its
 // groups, then per block, and the block's partial goes to its logical slot.
+// Epilogues with kLateBegin load their scalar's inputs in prefetch() and finish it in
+// begin() only after the block's first row group has been gathered.
+template <class E, class = void>
+struct LateBegin : std::false_type {};
+template <class E>
+struct LateBegin<E, std::void_t<decltype(E::kLateBegin)>> : std::bool_constant<E::kLateBegin> {};
+
 template <int G, class Rows, class Epi>
 __global__ __launch_bounds__(kBlock) void k_apply(int32_t n, Rows rows, const double *__restrict__ x, Epi epi,
                                                   double *partial, int xcd) {
@@ -98,9 +106,25 @@ __global__ __launch_bounds__(kBlock) void k_apply(int32_t n, Rows rows, const do
   const int lane = threadIdx.x % G;
   const int32_t lb = xcd ? xcd_logical(blockIdx.x, gridDim.x) : (int32_t)blockIdx.x;
   const GroupRange gr = group_range(lb, gridDim.x, (n + RPB - 1) / RPB, xcd != 0);
-  epi.begin();
   double c = 0.0;
-  for (int32_t gi = gr.start; gi < gr.end; gi += gr.step) {
+  int32_t gi = gr.start;
+  if constexpr (LateBegin<Epi>::value) {
+    epi.prefetch();
+    const bool have = gi < gr.end;   // the same for the whole block
+    const int32_t r = gi * RPB + g;
+    double s = 0.0;
+    if (have && r < n) s = rows.template lane_sum<G>(r, lane, x);
+#pragma unroll
+    for (int off = G / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, G);
+    epi.begin();   // workgroup-wide
+    if (have) {
+      if (lane == 0 && r < n) c += epi.row(r, s);
+      gi += gr.step;
+    }
+  } else {
+    epi.begin();
+  }
+  for (; gi < gr.end; gi += gr.step) {
     const int32_t r = gi * RPB + g;
     double s = 0.0;
     if (r < n) s = rows.template lane_sum<G>(r, lane, x);
@@ -135,15 +159,21 @@ struct EpiLanczosA {
   const double *pre = nullptr;
   int pre_n = 0;
   double *beta = nullptr;
+  PartialSum<1> ps;   // one partial per thread ahead of the gathers (more are read after them)
   static constexpr bool kReduce = true;
+  static constexpr bool kLateBegin = true;
   __device__ bool skip() const { return false; }
+  __device__ void prefetch() {
+    if (pre) ps.load(pre, pre_n);
+    else scale = *nrm2;
+  }
   __device__ void begin() {
     if (!pre) {
-      scale = 1.0 / sqrt(*nrm2);
+      scale = 1.0 / sqrt(scale);
       return;
     }
     __shared__ double bc;
-    const double t = finish_sum(pre, pre_n);
+    const double t = ps.sum(pre, pre_n);
     if (threadIdx.x == 0) {
       bc = t;
       if (blockIdx.x == 0) {
@@ -260,7 +290,7 @@ int apply_rows(int32_t n, int lanes, int grid_cap, const Rows &rows, const doubl
   }
   Context &C = ctx();
   std::pair<hipEvent_t, hipEvent_t> *ev = nullptr;
-  if (C.prof && Rows::kTimed) {
+  if (Rows::kTimed && C.prof_take()) {
     ev = C.prof_slot();
     HIPCHK(hipEventRecord(ev->first, s));
     C.prof_bytes += rows.alg_bytes(n);
@@ -372,6 +402,20 @@ __global__ __launch_bounds__(kBlock) void k_lanczos_b(int64_t n, int k, const do
                                                       double *alpha, const double *beta,
                                                       double *__restrict__ w, double *partial,
                                                       const double *pre, int pre_n) {
+  // four elements per pass with their loads issued together; sums in index order.  The
+  // first pass's operands are loaded before alpha is known, so that their latency overlaps
+  // the partial sum.
+  const int64_t S = (int64_t)gridDim.x * kBlock;
+  const int64_t first = blockIdx.x * (int64_t)kBlock + threadIdx.x;
+  double fy[4], fk[4], fm[4];
+#pragma unroll
+  for (int u = 0; u < 4; u++) {
+    const int64_t i = first + u * S;
+    const bool in = i < n;
+    fy[u] = in ? y[i] : 0.0;
+    fk[u] = in ? vk[i] : 0.0;
+    fm[u] = (in && k > 0) ? vkm1[i] : 0.0;
+  }
   double a;
   if (pre) {
     __shared__ double bc;
@@ -387,16 +431,15 @@ __global__ __launch_bounds__(kBlock) void k_lanczos_b(int64_t n, int k, const do
   }
   const double b = (k > 0) ? beta[k] : 0.0;
   double s = 0.0;
-  // four elements per pass with their loads issued together; sums in index order
-  const int64_t S = (int64_t)gridDim.x * kBlock;
-  for (int64_t i0 = blockIdx.x * (int64_t)kBlock + threadIdx.x; i0 < n; i0 += 4 * S) {
+  for (int64_t i0 = first; i0 < n; i0 += 4 * S) {
     double v[4];
 #pragma unroll
     for (int u = 0; u < 4; u++) {
       const int64_t i = i0 + u * S;
       if (i < n) {
-        v[u] = y[i] - a * vk[i];
-        if (k > 0) v[u] -= b * vkm1[i];
+        const bool pf = i0 == first;
+        v[u] = (pf ? fy[u] : y[i]) - a * (pf ? fk[u] : vk[i]);
+        if (k > 0) v[u] -= b * (pf ? fm[u] : vkm1[i]);
       }
     }
 #pragma unroll
@@ -456,6 +499,19 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(int64_t n, int it, double 
                                                       double *pap, const double *rr, const int *status,
                                                       double *partial, const double *pre, int pre_n) {
   if (status[0] != 0) return;
+  // the first pass's operands are loaded before pAp is known (see k_lanczos_b)
+  const int64_t S = (int64_t)gridDim.x * kBlock;
+  const int64_t first = blockIdx.x * (int64_t)kBlock + threadIdx.x;
+  double fx[4], fp[4], fr[4], fy[4];
+#pragma unroll
+  for (int u = 0; u < 4; u++) {
+    const int64_t i = first + u * S;
+    const bool in = i < n;
+    fx[u] = in ? x[i] : 0.0;
+    fp[u] = in ? p[i] : 0.0;
+    fr[u] = in ? r[i] : 0.0;
+    fy[u] = in ? y[i] : 0.0;
+  }
   double pAp;
   if (pre) {
     __shared__ double bc;
@@ -471,15 +527,15 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(int64_t n, int it, double 
   }
   const double alpha = rr[it - 1] / pAp;
   double s = 0.0;
-  const int64_t S = (int64_t)gridDim.x * kBlock;
-  for (int64_t i0 = blockIdx.x * (int64_t)kBlock + threadIdx.x; i0 < n; i0 += 4 * S) {
+  for (int64_t i0 = first; i0 < n; i0 += 4 * S) {
     double rv[4];
 #pragma unroll
     for (int u = 0; u < 4; u++) {
       const int64_t i = i0 + u * S;
       if (i < n) {
-        x[i] = x[i] + alpha * p[i];
-        rv[u] = r[i] - alpha * y[i];
+        const bool pf = i0 == first;
+        x[i] = (pf ? fx[u] : x[i]) + alpha * (pf ? fp[u] : p[i]);
+        rv[u] = (pf ? fr[u] : r[i]) - alpha * (pf ? fy[u] : y[i]);
         r[i] = rv[u];
       }
     }
@@ -518,6 +574,9 @@ __global__ __launch_bounds__(kBlock) void k_cg_dir(int64_t n, int it, const doub
     if (threadIdx.x == 0) done0 = status[0];
     __syncthreads();
     if (done0 != 0) return;
+    // this thread's first element, loaded before the partial sum
+    const int64_t i1 = blockIdx.x * (int64_t)kBlock + threadIdx.x;
+    const double r1 = i1 < n ? r[i1] : 0.0, p1 = i1 < n ? p[i1] : 0.0;
     const double t = finish_sum(pre, pre_n);
     if (threadIdx.x == 0) {
       bc = t;
@@ -530,7 +589,8 @@ __global__ __launch_bounds__(kBlock) void k_cg_dir(int64_t n, int it, const doub
     __syncthreads();
     if (bc < acc) return;
     const double beta = bc / rr[it - 1];
-    for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
+    if (i1 < n) p[i1] = r1 + beta * p1;
+    for (int64_t i = i1 + (int64_t)gridDim.x * kBlock; i < n; i += (int64_t)gridDim.x * kBlock)
       p[i] = r[i] + beta * p[i];
     return;
   }

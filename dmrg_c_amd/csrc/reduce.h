@@ -40,20 +40,46 @@ __device__ __forceinline__ void block_partial_to(double v, double *slot) {
 __device__ __forceinline__ void block_partial(double v, double *partial) { block_partial_to(v, partial + blockIdx.x); }
 
 // Sum of partial[0..nblk) by one workgroup (valid in thread 0): thread t adds
-// partial[t], partial[t + kBlock], ... in index order, with eight loads in flight ahead of
-// the additions, then the workgroup sum.
-__device__ __forceinline__ double finish_sum(const double *__restrict__ partial, int nblk) {
-  double s = 0.0;
-  int i = threadIdx.x;
-  for (; i + 7 * kBlock < nblk; i += 8 * kBlock) {
-    double p[8];
+// partial[t], partial[t + kBlock], ... in index order, then the workgroup sum.  The loads
+// go out in batches of D per thread (the partials were just written by other workgroups,
+// often on other XCDs, so each batch costs a full memory round trip); the additions stay
+// in index order, so the batch size does not change the result.  load() issues the first
+// batch, sum() adds it (and any further batches) and reduces, so that a kernel can overlap
+// the first round trip with other work.
+template <int D>
+struct PartialSum {
+  double p[D];
+  __device__ __forceinline__ void load(const double *__restrict__ partial, int nblk) {
+    const int i = threadIdx.x;
 #pragma unroll
-    for (int u = 0; u < 8; u++) p[u] = partial[i + u * kBlock];
-#pragma unroll
-    for (int u = 0; u < 8; u++) s += p[u];
+    for (int u = 0; u < D; u++) p[u] = (i + u * kBlock < nblk) ? partial[i + u * kBlock] : 0.0;
   }
-  for (; i < nblk; i += kBlock) s += partial[i];
-  return block_sum(s);
+  __device__ __forceinline__ double sum(const double *__restrict__ partial, int nblk) {
+    double s = 0.0;
+    int i = threadIdx.x;
+#pragma unroll
+    for (int u = 0; u < D; u++)
+      if (i + u * kBlock < nblk) s += p[u];
+    for (i += D * kBlock; i < nblk; i += D * kBlock) {
+      load_at(partial, nblk, i);
+#pragma unroll
+      for (int u = 0; u < D; u++)
+        if (i + u * kBlock < nblk) s += p[u];
+    }
+    return block_sum(s);
+  }
+
+ private:
+  __device__ __forceinline__ void load_at(const double *__restrict__ partial, int nblk, int i) {
+#pragma unroll
+    for (int u = 0; u < D; u++) p[u] = (i + u * kBlock < nblk) ? partial[i + u * kBlock] : 0.0;
+  }
+};
+
+__device__ __forceinline__ double finish_sum(const double *__restrict__ partial, int nblk) {
+  PartialSum<16> ps;
+  ps.load(partial, nblk);
+  return ps.sum(partial, nblk);
 }
 
 }  // namespace dmrgx

@@ -17,26 +17,43 @@ namespace dmrgx {
 
 
 namespace {
-// Pivots of the LDL^T factorisation of T - x I.  Returns the number of pivots that are not
-// positive (an exact zero pivot counts and is replaced by a tiny negative one to go on):
-// zero means x is below every eigenvalue.  Then *ratio = p'(x)/p(x) for p(x) = det(T - x I),
-// from d'_i = -1 + (b_{i-1}/d_{i-1})^2 d'_{i-1} and p'/p = sum_i d'_i/d_i.
+// Sturm count and Newton ratio of T - x I.  Returns the number of pivots d_i of its LDL^T
+// factorisation that are not positive (an exact zero pivot counts and is replaced by a tiny
+// negative one to go on): zero means x is below every eigenvalue.  Then *ratio = p'(x)/p(x)
+// for p(x) = det(T - x I).
+// Computed from the leading principal minors p_i = (a_i - x) p_{i-1} - b_{i-1}^2 p_{i-2}
+// and their x-derivatives (d_i = p_i / p_{i-1}, so d_i <= 0 when p_i is zero or differs in
+// sign from p_{i-1}): one multiply-add per step on the dependent chain instead of a
+// division.  The minors are rescaled by powers of two (exact) to stay in range.
 int pivots(int k, const double *a, const double *b, double x, double *ratio) {
-  double d = a[0] - x, dp = -1.0;
+  double pm = 1.0, qm = 0.0;        // p_{i-1}, p'_{i-1}   (p_{-1} = 1)
+  double p = a[0] - x, q = -1.0;    // p_0, p'_0
   int neg = 0;
-  double r = 0.0;
   for (int i = 0;;) {
-    if (d <= 0.0) {
+    if (p == 0.0) {
+      p = -1e-300 * pm;
       neg++;
-      if (d == 0.0) d = -1e-300;
+    } else if (std::signbit(p) != std::signbit(pm)) {
+      neg++;
     }
-    r += dp / d;
     if (++i == k) break;
-    const double q = b[i - 1] / d;
-    d = (a[i] - x) - q * b[i - 1];
-    dp = -1.0 + q * q * dp;
+    const double c = a[i] - x, b2 = b[i - 1] * b[i - 1];
+    const double pn = c * p - b2 * pm;
+    const double qn = c * q - b2 * qm - p;
+    pm = p;
+    qm = q;
+    p = pn;
+    q = qn;
+    const double mag = std::fabs(p);
+    if (mag > 0x1p+400 || (mag < 0x1p-400 && mag > 0.0)) {
+      const int e = std::ilogb(mag);
+      p = std::scalbn(p, -e);
+      q = std::scalbn(q, -e);
+      pm = std::scalbn(pm, -e);
+      qm = std::scalbn(qm, -e);
+    }
   }
-  *ratio = r;
+  *ratio = q / p;
   return neg;
 }
 }  // namespace

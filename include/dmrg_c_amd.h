@@ -685,6 +685,10 @@ int dmrgx_ground_state_batch(int32_t count, dmrgx_matrix *const *A, const dmrgx_
  * on the library's stream; end returns the summed kernel time, the launch count and the
  * algorithmic bytes (nnz*(sizeof(val)+4) + (n+1)*8 + 2*n*8 per launch). */
 int dmrgx_profile_begin(void);
+/* As dmrgx_profile_begin, bracketing only the first of every `every` SpMV launches: each
+ * event pair costs the stream a few microseconds between kernels, so sampling keeps the
+ * timed solves close to their unprofiled speed.  end then reports the sampled launches. */
+int dmrgx_profile_begin_sampled(int32_t every);
 int dmrgx_profile_end(double *spmv_ms, int64_t *spmv_launches, double *spmv_alg_bytes);
 /* Plain streaming copy of count doubles (device pointers): traffic calibration for
  * rocprofv3 FETCH_SIZE / WRITE_SIZE. */
