@@ -444,7 +444,7 @@ def main():
         log(f"schedule given: (lanes per row, workgroup cap, load variant) = {lanes}")
     else:
         lanes, tuned_us = autotune(D, sb)
-        log(f"autotune: (lanes per row, workgroup cap, load variant) = {lanes}, {tuned_us:.1f} us per SpMV + dot")
+        log(f"autotune: (lanes per row, workgroup cap, load variant) = {lanes}, {tuned_us:.1f} us per Lanczos step + CG iteration")
     for i in range(args.warmup):
         E, st, nnz = gpu_solve(D, sb, params, lanes)
         log(f"warmup {i}: E={E:.15f} lanczos={st.lanczos_steps} cg={st.cg_steps}")

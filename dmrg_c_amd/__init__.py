@@ -233,7 +233,7 @@ class DeviceMatrix:
 
     def autotune(self) -> Tuple[Tuple[int, int, int], float]:
         """Time the schedule candidates on the device, keep the fastest; returns it and its
-        time per product (microseconds)."""
+        time per Lanczos step + CG iteration (microseconds)."""
         g, cap, nt, us = C.c_int32(), C.c_int32(), C.c_int32(), C.c_double()
         check(lib().dmrgx_matrix_autotune(self.handle, C.byref(g), C.byref(cap), C.byref(nt), C.byref(us)))
         return (g.value, cap.value, nt.value), us.value

@@ -268,9 +268,9 @@ std::unique_ptr<dmrgx_matrix> upload_csr(int32_t n, const int64_t *row, const in
     DevBuf<int64_t> tr((size_t)n + 1);
     DevBuf<int32_t> tc(nnz);
     DevBuf<double> tv(nnz);
-    HIPCHK(hipMemcpyAsync(tr.p, row, sizeof(int64_t) * ((size_t)n + 1), hipMemcpyHostToDevice, st));
-    HIPCHK(hipMemcpyAsync(tc.p, col, sizeof(int32_t) * nnz, hipMemcpyHostToDevice, st));
-    HIPCHK(hipMemcpyAsync(tv.p, val, sizeof(double) * nnz, hipMemcpyHostToDevice, st));
+    ctx().copy_in(tr.p, row, sizeof(int64_t) * ((size_t)n + 1));
+    ctx().copy_in(tc.p, col, sizeof(int32_t) * nnz);
+    ctx().copy_in(tv.p, val, sizeof(double) * nnz);
     launch_expand_lower(n, nnz, tr.p, tc.p, tv.p, *A, st);
   } else {
     A->n = n;
@@ -278,9 +278,9 @@ std::unique_ptr<dmrgx_matrix> upload_csr(int32_t n, const int64_t *row, const in
     A->row.resize((size_t)n + 1);
     A->col.resize(nnz);
     A->val.resize(nnz);
-    HIPCHK(hipMemcpyAsync(A->row.p, row, sizeof(int64_t) * ((size_t)n + 1), hipMemcpyHostToDevice, st));
-    HIPCHK(hipMemcpyAsync(A->col.p, col, sizeof(int32_t) * nnz, hipMemcpyHostToDevice, st));
-    HIPCHK(hipMemcpyAsync(A->val.p, val, sizeof(double) * nnz, hipMemcpyHostToDevice, st));
+    ctx().copy_in(A->row.p, row, sizeof(int64_t) * ((size_t)n + 1));
+    ctx().copy_in(A->col.p, col, sizeof(int32_t) * nnz);
+    ctx().copy_in(A->val.p, val, sizeof(double) * nnz);
     HIPCHK(hipStreamSynchronize(st));
   }
   default_schedule(*A);
@@ -402,9 +402,9 @@ int dmrgx_matrix_download(const dmrgx_matrix *A, int64_t *row, int32_t *col, dou
   if (!A) return DMRGX_ERR_ARG;
   return run_status([&] {
     hipStream_t st = ctx().stream;
-    HIPCHK(hipMemcpyAsync(row, A->row.p, sizeof(int64_t) * ((size_t)A->n + 1), hipMemcpyDeviceToHost, st));
-    HIPCHK(hipMemcpyAsync(col, A->col.p, sizeof(int32_t) * A->nnz, hipMemcpyDeviceToHost, st));
-    HIPCHK(hipMemcpyAsync(val, A->val.p, sizeof(double) * A->nnz, hipMemcpyDeviceToHost, st));
+    ctx().copy_out(row, A->row.p, sizeof(int64_t) * ((size_t)A->n + 1));
+    ctx().copy_out(col, A->col.p, sizeof(int32_t) * A->nnz);
+    ctx().copy_out(val, A->val.p, sizeof(double) * A->nnz);
     HIPCHK(hipStreamSynchronize(st));
   });
 }
@@ -462,10 +462,11 @@ double time_schedule(dmrgx_matrix *A, int g, int cap, int v, int reps, const dou
 }
 
 struct TuneEnv {
+  int32_t n;
   DevBuf<double> x, y, dot;
   Reduce R;
   hipEvent_t e0 = nullptr, e1 = nullptr;
-  explicit TuneEnv(int32_t n) : x(n), y(n), dot(1) {
+  explicit TuneEnv(int32_t n_) : n(n_), x(n_), y(n_), dot(1) {
     launch_start_vector(n, 7, x.p, ctx().stream);
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
@@ -477,6 +478,55 @@ struct TuneEnv {
   double time(dmrgx_matrix *A, int g, int cap, int v, int reps) {
     return time_schedule(A, g, cap, v, reps, x.p, y.p, dot.p, R, e0, e1);
   }
+  // Microseconds per (Lanczos step + CG iteration) run with the solver's fused kernels:
+  // the product with its epilogue and the vector kernels that sum its block partials,
+  // whose cost grows with the product's block count.  The numbers computed do not matter
+  // (the CG runs on the unshifted operator with a zero tolerance, so it never stops).
+  double time_steps(dmrgx_matrix *A, int g, int cap, int v, int reps) {
+    hipStream_t st = ctx().stream;
+    if (w.count < (size_t)n) {
+      for (DevBuf<double> *b : {&w, &va, &vb, &ly, &p, &q, &xw, &r}) b->resize(n);
+      status.resize(2);
+    }
+    const size_t ns = (size_t)reps + 2;
+    if (sc.count < 4 * ns + 1) sc.resize(4 * ns + 1);
+    double *alpha = sc.p, *beta = sc.p + ns, *rr = sc.p + 2 * ns, *pap = sc.p + 3 * ns, *nrm2 = sc.p + 4 * ns;
+    const int g0 = A->lanes, c0 = A->grid, v0 = A->load;
+    A->lanes = g;
+    A->grid = cap;
+    A->load = v;
+    const Operator op = op_of(*A);
+    launch_copy(n, x.p, w.p, st);
+    launch_copy(n, x.p, p.p, st);
+    launch_copy(n, x.p, r.p, st);
+    launch_norm2(n, w.p, R, nrm2, st);
+    launch_norm2(n, r.p, R, rr, st);
+    HIPCHK(hipMemsetAsync(status.p, 0, 2 * sizeof(int), st));
+    HIPCHK(hipEventRecord(e0, st));
+    int nbl = 0;
+    for (int k = 1; k <= reps; k++) {
+      double *vk = (k & 1) ? va.p : vb.p, *vkm1 = (k & 1) ? vb.p : va.p;
+      const int na = launch_lanczos_a_part(op, false, k, w.p, nrm2, beta, k == 1 ? nullptr : LB.partial.p,
+                                           k == 1 ? 0 : nbl, vk, ly.p, alpha, LA, st);
+      nbl = launch_lanczos_b_part(n, k, ly.p, vk, vkm1, alpha, beta, LA.partial.p, na, w.p, LB, st);
+      const int nc = launch_cg_ap_part(op, 0.0, p.p, q.p, status.p, CA, st);
+      const int nu = launch_cg_update_part(n, k, p.p, q.p, xw.p, r.p, pap, rr, status.p, CA.partial.p, nc, CB, st);
+      launch_cg_dir_part(n, k, 0.0, r.p, rr, p.p, status.p, CB.partial.p, nu, st);
+    }
+    HIPCHK(hipEventRecord(e1, st));
+    HIPCHK(hipEventSynchronize(e1));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    A->lanes = g0;
+    A->grid = c0;
+    A->load = v0;
+    return ms * 1000.0 / reps;
+  }
+
+ private:
+  DevBuf<double> w, va, vb, ly, p, q, xw, r, sc;
+  DevBuf<int> status;
+  Reduce LA, LB, CA, CB;
 };
 }  // namespace
 
@@ -520,10 +570,11 @@ int dmrgx_matrix_autotune(dmrgx_matrix *A, int32_t *lanes_out, int32_t *grid_out
       if (has[load_format(v)])
         for (int g = g_lo; g <= g_hi; g *= 2)
           for (int cap : caps) cands.push_back({g, cap, v, env.time(A, g, cap, v, 3)});
-    // the leaders again with more launches, so that noise does not pick the schedule
+    // the leaders again, timed as the solver runs them: a Lanczos step and a CG iteration
+    // with their fused vector kernels, several times over
     std::sort(cands.begin(), cands.end(), [](const Cand &a, const Cand &b) { return a.us < b.us; });
-    cands.resize(std::min<size_t>(cands.size(), 6));
-    for (Cand &c : cands) c.us = env.time(A, c.g, c.cap, c.v, 12);
+    cands.resize(std::min<size_t>(cands.size(), 8));
+    for (Cand &c : cands) c.us = env.time_steps(A, c.g, c.cap, c.v, 12);
     const Cand best = *std::min_element(cands.begin(), cands.end(),
                                         [](const Cand &a, const Cand &b) { return a.us < b.us; });
     A->lanes = best.g;

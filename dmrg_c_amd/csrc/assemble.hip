@@ -133,11 +133,15 @@ __device__ __forceinline__ unsigned lanes_below(uint64_t mask) {
 // One wavefront (one 64-thread workgroup) per superblock row.  CAP: LDS list capacity.
 // FILL = false: counts[r] = kept entries; rows whose list exceeds CAP are appended to
 // overflow (count pass) and redone by the next larger variant; FILL = true writes the rows
-// whose lists fit and skips the others.
+// whose lists fit and skips the others.  A count pass given scratch (stride CAP + 1 per
+// row) also writes each fitting row's kept entries there and sets fitted[r], so that the
+// fill pass for those rows is a copy (k_assemble_compact) instead of a second enumeration.
 template <int CAP, bool FILL>
 __global__ __launch_bounds__(64) void k_assemble(SbView sb, const int32_t *row_list, int32_t n_list,
                                                  int64_t *counts, const int64_t *row_ptr, int32_t *col_out,
-                                                 double *val_out, int32_t *overflow, int32_t *n_overflow) {
+                                                 double *val_out, int32_t *overflow, int32_t *n_overflow,
+                                                 int32_t *scol = nullptr, double *sval = nullptr,
+                                                 uint8_t *fitted = nullptr) {
   __shared__ uint64_t key[CAP];
   __shared__ double val[CAP];
   const int lane = threadIdx.x;
@@ -198,9 +202,11 @@ __global__ __launch_bounds__(64) void k_assemble(SbView sb, const int32_t *row_l
     if (!FILL && lane == 0) {
       if (overflow) overflow[atomicAdd(n_overflow, 1)] = r;
       else atomicAdd(n_overflow, 1);
+      if (fitted) fitted[r] = 0;
     }
     return;
   }
+  const int64_t sbase = (int64_t)r * (CAP + 1);   // this row's scratch slot
   // bitonic sort of key[0..P) with P = next power of two >= m (padding sorts last)
   int P = 1;
   while (P < m) P <<= 1;
@@ -247,6 +253,11 @@ __global__ __launch_bounds__(64) void k_assemble(SbView sb, const int32_t *row_l
       col_out[dst] = c;
       val_out[dst] = s;
     }
+    if (!FILL && scol && keep) {
+      const int64_t dst = sbase + out + (int64_t)lanes_below(km);
+      scol[dst] = c;
+      sval[dst] = s;
+    }
     out += __popcll(km);
   }
   if (!diag_seen && sb.has_zero) {
@@ -254,9 +265,34 @@ __global__ __launch_bounds__(64) void k_assemble(SbView sb, const int32_t *row_l
       col_out[row_ptr[r] + out] = r;
       val_out[row_ptr[r] + out] = 0.0;
     }
+    if (!FILL && scol && lane == 0) {
+      scol[sbase + out] = r;
+      sval[sbase + out] = 0.0;
+    }
     out++;
   }
-  if (!FILL && lane == 0) counts[r] = out;
+  if (!FILL && lane == 0) {
+    counts[r] = out;
+    if (fitted) fitted[r] = 1;
+  }
+}
+
+// Fill pass of the rows whose kept entries a count pass left in scratch: one wavefront
+// per row copies them to their place in the CRS.
+__global__ __launch_bounds__(64) void k_assemble_compact(int32_t n, const uint8_t *__restrict__ fitted,
+                                                         const int64_t *__restrict__ row_ptr, int64_t stride,
+                                                         const int32_t *__restrict__ scol,
+                                                         const double *__restrict__ sval,
+                                                         int32_t *__restrict__ col_out,
+                                                         double *__restrict__ val_out) {
+  const int32_t r = blockIdx.x;
+  if (r >= n || !fitted[r]) return;
+  const int64_t b = row_ptr[r], len = row_ptr[r + 1] - b;
+  const int64_t src = (int64_t)r * stride;
+  for (int64_t i = threadIdx.x; i < len; i += 64) {
+    col_out[b + i] = scol[src + i];
+    val_out[b + i] = sval[src + i];
+  }
 }
 
 __global__ __launch_bounds__(256) void k_fill_int(int64_t n, int32_t v, int32_t *out) {
@@ -275,6 +311,9 @@ __global__ __launch_bounds__(256) void k_inv_scatter(SbView sb, int32_t *inv) {
 constexpr int kCapSmall = 256;   // 4 KiB of LDS per row
 constexpr int kCapMid = 1024;    // 16 KiB
 constexpr int kCapLarge = 4096;  // 64 KiB
+// Device scratch the count pass may use to keep its rows' entries (n * (kCapSmall + 1) * 12
+// bytes: 0.45 GB at n = 144542).
+constexpr int64_t kScratchBytes = (int64_t)4 << 30;
 
 // ------------------------------------------------------------------ host
 int op_role_dim(int slot, const dmrgx_superblock &S) {
@@ -358,7 +397,7 @@ static dmrgx_superblock *create_superblock(int32_t n, int32_t dim_LL, int32_t di
   hipStream_t st = ctx().stream;
   auto up16 = [&](DevBuf<int16_t> &b, const int16_t *h) {
     b.resize(n);
-    HIPCHK(hipMemcpyAsync(b.p, h, sizeof(int16_t) * n, hipMemcpyHostToDevice, st));
+    ctx().copy_in(b.p, h, sizeof(int16_t) * n);
   };
   up16(S->LL, LL);
   up16(S->LR, LR);
@@ -374,10 +413,10 @@ static dmrgx_superblock *create_superblock(int32_t n, int32_t dim_LL, int32_t di
     S->op_row[i].resize((size_t)a.row_dim + 1);
     S->op_col[i].resize(std::max<int64_t>(nnz, 1));
     S->op_val[i].resize(std::max<int64_t>(nnz, 1));
-    HIPCHK(hipMemcpyAsync(S->op_row[i].p, a.row, sizeof(int64_t) * ((size_t)a.row_dim + 1), hipMemcpyHostToDevice, st));
+    ctx().copy_in(S->op_row[i].p, a.row, sizeof(int64_t) * ((size_t)a.row_dim + 1));
     if (nnz) {
-      HIPCHK(hipMemcpyAsync(S->op_col[i].p, a.col, sizeof(int32_t) * nnz, hipMemcpyHostToDevice, st));
-      HIPCHK(hipMemcpyAsync(S->op_val[i].p, a.val, sizeof(double) * nnz, hipMemcpyHostToDevice, st));
+      ctx().copy_in(S->op_col[i].p, a.col, sizeof(int32_t) * nnz);
+      ctx().copy_in(S->op_val[i].p, a.val, sizeof(double) * nnz);
     }
     hops[i] = DOp{S->op_row[i].p, S->op_col[i].p, S->op_val[i].p};
   }
@@ -385,16 +424,16 @@ static dmrgx_superblock *create_superblock(int32_t n, int32_t dim_LL, int32_t di
   S->ele.resize(eles.size());
   for (size_t i = 0; i < eles.size(); i++) {
     S->ele[i].resize(std::max(eles[i].second, 1));
-    HIPCHK(hipMemcpyAsync(S->ele[i].p, eles[i].first, sizeof(int32_t) * eles[i].second, hipMemcpyHostToDevice, st));
+    ctx().copy_in(S->ele[i].p, eles[i].first, sizeof(int32_t) * eles[i].second);
     heles[i] = S->ele[i].p;
   }
   S->ops.resize(hops.size());
-  HIPCHK(hipMemcpyAsync(S->ops.p, hops.data(), sizeof(DOp) * hops.size(), hipMemcpyHostToDevice, st));
+  ctx().copy_in(S->ops.p, hops.data(), sizeof(DOp) * hops.size());
   S->terms.resize(std::max<size_t>(terms.size(), 1));
   if (!terms.empty())
-    HIPCHK(hipMemcpyAsync(S->terms.p, terms.data(), sizeof(DTerm) * terms.size(), hipMemcpyHostToDevice, st));
+    ctx().copy_in(S->terms.p, terms.data(), sizeof(DTerm) * terms.size());
   S->ele_ptrs.resize(heles.size());
-  HIPCHK(hipMemcpyAsync(S->ele_ptrs.p, heles.data(), sizeof(const int32_t *) * heles.size(), hipMemcpyHostToDevice, st));
+  ctx().copy_in(S->ele_ptrs.p, heles.data(), sizeof(const int32_t *) * heles.size());
   // dense superblock inverse, -1 where absent (GET_BASIS_SUPERBLOCK.c Inv fill)
   const int64_t dim_whole = (int64_t)dim_LL * d * dim_RR * d;
   if (dim_whole > (int64_t)1 << 34) throw Failure(DMRGX_ERR_ARG, "superblock product space too large for the dense inverse");
@@ -429,20 +468,35 @@ static int64_t assemble_lower(dmrgx_superblock &S) {
   DevBuf<int64_t> counts(n);
   DevBuf<int32_t> over1(n), over2(n), n_over(3);
   HIPCHK(hipMemsetAsync(n_over.p, 0, 3 * sizeof(int32_t), st));
+  // The small-list count pass keeps its rows' entries in scratch (one slot of
+  // kCapSmall + 1 entries per row) when that fits kScratchBytes; the fill pass of those rows
+  // is then a copy.
+  const int64_t stride = kCapSmall + 1;
+  const bool keep = (int64_t)n * stride * (int64_t)(sizeof(int32_t) + sizeof(double)) <= kScratchBytes;
+  DevBuf<int32_t> scol(keep ? (size_t)n * stride : 0);
+  DevBuf<double> sval(keep ? (size_t)n * stride : 0);
+  DevBuf<uint8_t> fitted(keep ? (size_t)n : 0);
   // count pass: small lists for every row, then the rows that overflowed with larger lists
   k_assemble<kCapSmall, false><<<n, 64, 0, st>>>(v, nullptr, n, counts.p, nullptr, nullptr, nullptr, over1.p,
-                                                 n_over.p);
+                                                 n_over.p, scol.p, sval.p, fitted.p);
   HIPCHK(hipGetLastError());
-  int h[3] = {0, 0, 0};
-  HIPCHK(hipMemcpyAsync(&h[0], n_over.p, sizeof(int), hipMemcpyDeviceToHost, st));
+  // row pointers right away: they hold unless some row overflowed (then they are redone),
+  // so the common case reads the overflow count and nnz with one round trip
+  S.lrow.resize((size_t)n + 1);
+  launch_exclusive_scan(n, counts.p, S.lrow.p, st);
+  int64_t *hb = reinterpret_cast<int64_t *>(ctx().host_stage(4));
+  int *h = reinterpret_cast<int *>(hb + 1);
+  HIPCHK(hipMemcpyAsync(hb, S.lrow.p + n, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipMemcpyAsync(h, n_over.p, sizeof(int), hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
   const int n_mid = h[0];
+  int64_t nnz = hb[0];
   int n_large = 0;
   if (n_mid > 0) {
     k_assemble<kCapMid, false><<<n_mid, 64, 0, st>>>(v, over1.p, n_mid, counts.p, nullptr, nullptr, nullptr,
                                                      over2.p, n_over.p + 1);
     HIPCHK(hipGetLastError());
-    HIPCHK(hipMemcpyAsync(&h[1], n_over.p + 1, sizeof(int), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(h + 1, n_over.p + 1, sizeof(int), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
     n_large = h[1];
   }
@@ -450,20 +504,25 @@ static int64_t assemble_lower(dmrgx_superblock &S) {
     k_assemble<kCapLarge, false><<<n_large, 64, 0, st>>>(v, over2.p, n_large, counts.p, nullptr, nullptr,
                                                          nullptr, nullptr, n_over.p + 2);
     HIPCHK(hipGetLastError());
-    HIPCHK(hipMemcpyAsync(&h[2], n_over.p + 2, sizeof(int), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(h + 2, n_over.p + 2, sizeof(int), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
     if (h[2] > 0)
       throw Failure(DMRGX_ERR_ARG, "a superblock row has more than 4096 lower-triangle contributions");
   }
-  S.lrow.resize((size_t)n + 1);
-  launch_exclusive_scan(n, counts.p, S.lrow.p, st);
-  int64_t nnz = 0;
-  HIPCHK(hipMemcpyAsync(&nnz, S.lrow.p + n, sizeof(int64_t), hipMemcpyDeviceToHost, st));
-  HIPCHK(hipStreamSynchronize(st));
+  if (n_mid > 0) {
+    launch_exclusive_scan(n, counts.p, S.lrow.p, st);
+    HIPCHK(hipMemcpyAsync(hb, S.lrow.p + n, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    nnz = hb[0];
+  }
   S.lcol.resize(std::max<int64_t>(nnz, 1));
   S.lval.resize(std::max<int64_t>(nnz, 1));
   // fill pass: each variant writes the rows whose lists fit it and skips the others
-  k_assemble<kCapSmall, true><<<n, 64, 0, st>>>(v, nullptr, n, nullptr, S.lrow.p, S.lcol.p, S.lval.p, nullptr, nullptr);
+  if (keep)
+    k_assemble_compact<<<n, 64, 0, st>>>(n, fitted.p, S.lrow.p, stride, scol.p, sval.p, S.lcol.p, S.lval.p);
+  else
+    k_assemble<kCapSmall, true><<<n, 64, 0, st>>>(v, nullptr, n, nullptr, S.lrow.p, S.lcol.p, S.lval.p, nullptr,
+                                                  nullptr);
   if (n_mid > 0)
     k_assemble<kCapMid, true><<<n_mid, 64, 0, st>>>(v, over1.p, n_mid, nullptr, S.lrow.p, S.lcol.p, S.lval.p,
                                                     nullptr, nullptr);
@@ -471,7 +530,8 @@ static int64_t assemble_lower(dmrgx_superblock &S) {
     k_assemble<kCapLarge, true><<<n_large, 64, 0, st>>>(v, over2.p, n_large, nullptr, S.lrow.p, S.lcol.p,
                                                         S.lval.p, nullptr, nullptr);
   HIPCHK(hipGetLastError());
-  HIPCHK(hipStreamSynchronize(st));
+  // no wait: whatever reads the triangle next is queued behind the fill on this stream
+  // (and the scratch released here is only reused in stream order)
   S.lnnz = nnz;
   return nnz;
 }
@@ -790,10 +850,10 @@ int dmrgx_superblock_download_lower(const dmrgx_superblock *S, int64_t *row, int
   if (!S || S->lnnz < 0) return DMRGX_ERR_ARG;
   return run_status([&] {
     hipStream_t st = ctx().stream;
-    HIPCHK(hipMemcpyAsync(row, S->lrow.p, sizeof(int64_t) * ((size_t)S->n + 1), hipMemcpyDeviceToHost, st));
+    ctx().copy_out(row, S->lrow.p, sizeof(int64_t) * ((size_t)S->n + 1));
     if (S->lnnz) {
-      HIPCHK(hipMemcpyAsync(col, S->lcol.p, sizeof(int32_t) * S->lnnz, hipMemcpyDeviceToHost, st));
-      HIPCHK(hipMemcpyAsync(val, S->lval.p, sizeof(double) * S->lnnz, hipMemcpyDeviceToHost, st));
+      ctx().copy_out(col, S->lcol.p, sizeof(int32_t) * S->lnnz);
+      ctx().copy_out(val, S->lval.p, sizeof(double) * S->lnnz);
     }
     HIPCHK(hipStreamSynchronize(st));
   });

@@ -1,8 +1,10 @@
 // Process-wide device context: one non-blocking stream at default priority, pinned
 // staging for scalars, and a cache of device blocks so repeated solves do not pay
 // hipMalloc/hipFree on the critical path.
+#include <algorithm>
 #include <cstdarg>
 #include <cstdlib>
+#include <cstring>
 #include <mutex>
 
 #include "internal.h"
@@ -102,6 +104,63 @@ std::pair<hipEvent_t, hipEvent_t> *Context::prof_slot() {
     prof_ev.push_back(p);
   }
   return &prof_ev[prof_used++];
+}
+
+namespace {
+constexpr size_t kXferChunk = (size_t)1 << 20;   // bytes per pinned chunk buffer
+}
+
+static void xfer_init(Context &C) {
+  if (C.xfer_pin[0]) return;
+  for (int i = 0; i < 2; i++) {
+    HIPCHK(hipHostMalloc(reinterpret_cast<void **>(&C.xfer_pin[i]), kXferChunk, hipHostMallocDefault));
+    HIPCHK(hipEventCreateWithFlags(&C.xfer_ev[i], hipEventDisableTiming));
+  }
+}
+
+void Context::copy_out(void *dst, const void *dev_src, size_t bytes) {
+  if (bytes == 0) return;
+  xfer_init(*this);
+  char *out = static_cast<char *>(dst);
+  const char *src = static_cast<const char *>(dev_src);
+  const size_t chunks = (bytes + kXferChunk - 1) / kXferChunk;
+  auto len_of = [&](size_t k) { return std::min(kXferChunk, bytes - k * kXferChunk); };
+  for (size_t k = 0; k <= chunks; k++) {
+    if (k < chunks) {
+      // chunk k's buffer was last used by chunk k - 2, whose host copy is done
+      HIPCHK(hipMemcpyAsync(xfer_pin[k & 1], src + k * kXferChunk, len_of(k), hipMemcpyDeviceToHost, stream));
+      HIPCHK(hipEventRecord(xfer_ev[k & 1], stream));
+    }
+    if (k > 0) {
+      const size_t j = k - 1;
+      HIPCHK(hipEventSynchronize(xfer_ev[j & 1]));
+      std::memcpy(out + j * kXferChunk, xfer_pin[j & 1], len_of(j));
+    }
+  }
+}
+
+void Context::copy_in(void *dev_dst, const void *src, size_t bytes) {
+  if (bytes == 0) return;
+  xfer_init(*this);
+  char *dst = static_cast<char *>(dev_dst);
+  const char *in = static_cast<const char *>(src);
+  const size_t chunks = (bytes + kXferChunk - 1) / kXferChunk;
+  for (size_t k = 0; k < chunks; k++) {
+    const size_t len = std::min(kXferChunk, bytes - k * kXferChunk);
+    if (k >= 2) HIPCHK(hipEventSynchronize(xfer_ev[k & 1]));   // chunk k - 2's DMA has read the buffer
+    std::memcpy(xfer_pin[k & 1], in + k * kXferChunk, len);
+    HIPCHK(hipMemcpyAsync(dst + k * kXferChunk, xfer_pin[k & 1], len, hipMemcpyHostToDevice, stream));
+    HIPCHK(hipEventRecord(xfer_ev[k & 1], stream));
+  }
+  // the buffers are free again (and the data on the device) once the last DMAs are done
+  HIPCHK(hipEventSynchronize(xfer_ev[(chunks - 1) & 1]));
+  if (chunks > 1) HIPCHK(hipEventSynchronize(xfer_ev[chunks & 1]));
+}
+
+hipEvent_t *Context::step_events() {
+  if (!step_ev[0])
+    for (int i = 0; i < 2; i++) HIPCHK(hipEventCreateWithFlags(&step_ev[i], hipEventDisableTiming));
+  return step_ev;
 }
 
 double *Context::host_stage(size_t count) {

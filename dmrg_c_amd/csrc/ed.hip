@@ -339,7 +339,7 @@ void ed_set_basis(dmrgx_ed &E, int64_t n, const int64_t *basis) {
   }
   E.n = n;
   E.basis.resize(n);
-  HIPCHK(hipMemcpyAsync(E.basis.p, basis, sizeof(int64_t) * n, hipMemcpyHostToDevice, ctx().stream));
+  ctx().copy_in(E.basis.p, basis, sizeof(int64_t) * n);
   HIPCHK(hipStreamSynchronize(ctx().stream));
 }
 
@@ -470,7 +470,7 @@ int64_t dmrgx_ed_dim(const dmrgx_ed *E) { return E ? E->n : -1; }
 int dmrgx_ed_download_basis(const dmrgx_ed *E, int64_t *basis) {
   if (!E || !basis) return DMRGX_ERR_ARG;
   return run_status([&] {
-    HIPCHK(hipMemcpyAsync(basis, E->basis.p, sizeof(int64_t) * E->n, hipMemcpyDeviceToHost, ctx().stream));
+    ctx().copy_out(basis, E->basis.p, sizeof(int64_t) * E->n);
     HIPCHK(hipStreamSynchronize(ctx().stream));
   });
 }

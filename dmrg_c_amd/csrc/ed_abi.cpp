@@ -32,21 +32,20 @@ struct DevOnsite {
 void upload_onsite(const char *fn, const CRS1 *M, int d, DevOnsite &O) {
   if (!M || M->row_dim < d || !M->Row) fatal(fn, "M_On->row_dim=%d is smaller than dim_onsite=%d\n", M ? M->row_dim : -1, d);
   const int64_t nnz = M->Row[d];
-  hipStream_t st = ctx().stream;
   O.row.resize(d + 1);
   O.col.resize(std::max<int64_t>(nnz, 1));
   O.val.resize(std::max<int64_t>(nnz, 1));
-  HIPCHK(hipMemcpyAsync(O.row.p, M->Row, sizeof(int64_t) * (d + 1), hipMemcpyHostToDevice, st));
+  ctx().copy_in(O.row.p, M->Row, sizeof(int64_t) * (d + 1));
   if (nnz > 0) {
-    HIPCHK(hipMemcpyAsync(O.col.p, M->Col, sizeof(int32_t) * nnz, hipMemcpyHostToDevice, st));
-    HIPCHK(hipMemcpyAsync(O.val.p, M->Val, sizeof(double) * nnz, hipMemcpyHostToDevice, st));
+    ctx().copy_in(O.col.p, M->Col, sizeof(int32_t) * nnz);
+    ctx().copy_in(O.val.p, M->Val, sizeof(double) * nnz);
   }
 }
 
 void upload_codes(const long *B, int n, DevBuf<int64_t> &out) {
   out.resize(std::max(n, 1));
   if (n > 0)
-    HIPCHK(hipMemcpyAsync(out.p, B, sizeof(int64_t) * n, hipMemcpyHostToDevice, ctx().stream));
+    ctx().copy_in(out.p, B, sizeof(int64_t) * n);
 }
 
 int64_t site_power(int d, int site) {
@@ -72,7 +71,7 @@ void v_m(const char *fn, const CRS1 *M, int d, int site, const long *basis_out, 
       bin = bi.p;
     }
     DevBuf<int32_t> ne(d);
-    if (sign) HIPCHK(hipMemcpyAsync(ne.p, N_Ele, sizeof(int32_t) * d, hipMemcpyHostToDevice, st));
+    if (sign) ctx().copy_in(ne.p, N_Ele, sizeof(int32_t) * d);
     DevVec x(std::max(n_in, 1)), y(n_out);
     if (n_in > 0) x.up(Vec);
     const EdRows rows{d, site, sign ? 1 : 0, site_power(d, site), n_in, bo.p, bin, O.row.p, O.col.p, O.val.p, ne.p};
@@ -472,7 +471,7 @@ void EXACT_EXPECTATION_ONSITE(CRS1 *M_On, double *Out, double *Vec, double *T_Ve
       const EdRows rows{dim_onsite, site, 0, site_power(dim_onsite, site), n, b.p, b.p, O.row.p, O.col.p, O.val.p, ne.p};
       launch_ed_apply(n, rows, v.d.p, t.d.p, v.d.p, res.p + site, R, st);
     }
-    HIPCHK(hipMemcpyAsync(Out, res.p, sizeof(double) * tot_site, hipMemcpyDeviceToHost, st));
+    ctx().copy_out(Out, res.p, sizeof(double) * tot_site);
     if (T_Vec) t.down(T_Vec);
     HIPCHK(hipStreamSynchronize(st));
   });
@@ -505,7 +504,7 @@ void EXACT_EXPECTATION_INTERSITE_Q0(CRS1 *M_O, CRS1 *M_R, int start, int end, do
       const EdRows rr{dim_onsite, site, 0, site_power(dim_onsite, site), n, b.p, b.p, Rop.row.p, Rop.col.p, Rop.val.p, ne.p};
       launch_ed_apply(n, rr, v.d.p, t2.d.p, t1.d.p, res.p + (site - start), R, st);
     }
-    if (len > 0) HIPCHK(hipMemcpyAsync(Out, res.p, sizeof(double) * len, hipMemcpyDeviceToHost, st));
+    if (len > 0) ctx().copy_out(Out, res.p, sizeof(double) * len);
     if (T_Vec1) t1.down(T_Vec1);
     if (T_Vec2 && len > 0) t2.down(T_Vec2);
     HIPCHK(hipStreamSynchronize(st));

@@ -1177,7 +1177,7 @@ int dmrgx_engine_poll(dmrgx_engine *E, dmrgx_engine_result *res, double *vec_out
     res->t_start = s->t_start;
     res->t_done = s->t_done;
     if (vec_out && s->vec.p) {
-      HIPCHK(hipMemcpyAsync(vec_out, s->vec.p, sizeof(double) * s->n, hipMemcpyDeviceToHost, ctx().stream));
+      ctx().copy_out(vec_out, s->vec.p, sizeof(double) * s->n);
       HIPCHK(hipStreamSynchronize(ctx().stream));
     }
     if (s->owned) dmrgx_matrix_free(s->owned);

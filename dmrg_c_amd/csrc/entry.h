@@ -49,11 +49,8 @@ struct DevVec {
   DevBuf<double> d;
   int64_t n;
   explicit DevVec(int64_t n_) : d(n_), n(n_) {}
-  void up(const double *h) { HIPCHK(hipMemcpyAsync(d.p, h, sizeof(double) * n, hipMemcpyHostToDevice, ctx().stream)); }
-  void down(double *h) {
-    HIPCHK(hipMemcpyAsync(h, d.p, sizeof(double) * n, hipMemcpyDeviceToHost, ctx().stream));
-    HIPCHK(hipStreamSynchronize(ctx().stream));
-  }
+  void up(const double *h) { ctx().copy_in(d.p, h, sizeof(double) * n); }
+  void down(double *h) { ctx().copy_out(h, d.p, sizeof(double) * n); }
 };
 
 }  // namespace dmrgx

@@ -43,7 +43,7 @@ std::unique_ptr<DevBasis> upload_basis(const DMRG_BASIS *B, int32_t n) {
   DevBuf<int16_t> *bufs[4] = {&D->LL, &D->LR, &D->RR, &D->RL};
   for (int k = 0; k < 4; k++) {
     bufs[k]->resize(n);
-    HIPCHK(hipMemcpyAsync(bufs[k]->p, Q[k], sizeof(int16_t) * n, hipMemcpyHostToDevice, st));
+    ctx().copy_in(bufs[k]->p, Q[k], sizeof(int16_t) * n);
   }
   const int64_t dim_whole = (int64_t)D->dim_LL * D->d * D->dim_RR * D->d;
   D->inv.resize(dim_whole);
@@ -63,14 +63,13 @@ void upload_op(const CRS1 *M, int dim, DevOp &O) {
   const int64_t nnz = M->Row[dim];
   for (int64_t j = 0; j < nnz; j++)
     if (M->Col[j] < 0 || M->Col[j] >= dim) throw Failure(DMRGX_ERR_ARG, "operator column outside its block");
-  hipStream_t st = ctx().stream;
   O.row.resize((size_t)dim + 1);
   O.col.resize(nnz > 0 ? nnz : 1);
   O.val.resize(nnz > 0 ? nnz : 1);
-  HIPCHK(hipMemcpyAsync(O.row.p, M->Row, sizeof(int64_t) * ((size_t)dim + 1), hipMemcpyHostToDevice, st));
+  ctx().copy_in(O.row.p, M->Row, sizeof(int64_t) * ((size_t)dim + 1));
   if (nnz) {
-    HIPCHK(hipMemcpyAsync(O.col.p, M->Col, sizeof(int32_t) * nnz, hipMemcpyHostToDevice, st));
-    HIPCHK(hipMemcpyAsync(O.val.p, M->Val, sizeof(double) * nnz, hipMemcpyHostToDevice, st));
+    ctx().copy_in(O.col.p, M->Col, sizeof(int32_t) * nnz);
+    ctx().copy_in(O.val.p, M->Val, sizeof(double) * nnz);
   }
 }
 
@@ -87,10 +86,10 @@ void v_m(const char *fn, int slot, CRS1 *M, double *V, double *Out, int dim, DMR
     upload_op(M, slot_dim(*D, slot), O);
     hipStream_t st = ctx().stream;
     DevBuf<double> x(dim), y(dim);
-    HIPCHK(hipMemcpyAsync(x.p, V, sizeof(double) * dim, hipMemcpyHostToDevice, st));
+    ctx().copy_in(x.p, V, sizeof(double) * dim);
     Reduce R;
     launch_slot_apply(dim, rows_of(*D, slot, O), x.p, y.p, nullptr, nullptr, R, st);
-    HIPCHK(hipMemcpyAsync(Out, y.p, sizeof(double) * dim, hipMemcpyDeviceToHost, st));
+    ctx().copy_out(Out, y.p, sizeof(double) * dim);
     HIPCHK(hipStreamSynchronize(st));
   });
 }
@@ -126,7 +125,7 @@ void DMRG_EXPECTATION_ONSITE(CRS1 **M_LL, CRS1 *M_On, CRS1 **M_RR, double *Out, 
     auto D = upload_basis(B, n);
     hipStream_t st = ctx().stream;
     DevBuf<double> v(n), t(n), res(nout);
-    HIPCHK(hipMemcpyAsync(v.p, Vec, sizeof(double) * n, hipMemcpyHostToDevice, st));
+    ctx().copy_in(v.p, Vec, sizeof(double) * n);
     Reduce R;
     DevOp O;
     // LL block sites (:11-14), LR (:17-18), RL (:21-22), RR block sites (:24-27)
@@ -141,8 +140,8 @@ void DMRG_EXPECTATION_ONSITE(CRS1 **M_LL, CRS1 *M_On, CRS1 **M_RR, double *Out, 
       upload_op(M_RR[site], D->dim_RR, O);
       launch_slot_apply(n, rows_of(*D, 2, O), v.p, t.p, v.p, res.p + RR_site + LL_site + 3 - site, R, st);
     }
-    HIPCHK(hipMemcpyAsync(Out, res.p, sizeof(double) * nout, hipMemcpyDeviceToHost, st));
-    if (Temp_V) HIPCHK(hipMemcpyAsync(Temp_V, t.p, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+    ctx().copy_out(Out, res.p, sizeof(double) * nout);
+    if (Temp_V) ctx().copy_out(Temp_V, t.p, sizeof(double) * n);
     HIPCHK(hipStreamSynchronize(st));
   });
 }
@@ -159,7 +158,7 @@ void DMRG_EXPECTATION_INTERSITE_Q0(CRS1 **M_CF, CRS1 **M_LL, CRS1 *M_On, CRS1 **
     auto D = upload_basis(B, n);
     hipStream_t st = ctx().stream;
     DevBuf<double> v(n), t1(n), t2(n), res(nout > 0 ? nout : 1);
-    HIPCHK(hipMemcpyAsync(v.p, Vec, sizeof(double) * n, hipMemcpyHostToDevice, st));
+    ctx().copy_in(v.p, Vec, sizeof(double) * n);
     Reduce R;
     DevOp O;
     // T1 = O_origin v (:14-19)
@@ -185,9 +184,9 @@ void DMRG_EXPECTATION_INTERSITE_Q0(CRS1 **M_CF, CRS1 **M_LL, CRS1 *M_On, CRS1 **
       upload_op(M_RR[site], D->dim_RR, O);
       launch_slot_apply(n, rows_of(*D, 2, O), v.p, t2.p, t1.p, res.p + r, R, st);
     }
-    HIPCHK(hipMemcpyAsync(Out, res.p, sizeof(double) * nout, hipMemcpyDeviceToHost, st));
-    if (Temp_V1) HIPCHK(hipMemcpyAsync(Temp_V1, t1.p, sizeof(double) * n, hipMemcpyDeviceToHost, st));
-    if (Temp_V2) HIPCHK(hipMemcpyAsync(Temp_V2, t2.p, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+    ctx().copy_out(Out, res.p, sizeof(double) * nout);
+    if (Temp_V1) ctx().copy_out(Temp_V1, t1.p, sizeof(double) * n);
+    if (Temp_V2) ctx().copy_out(Temp_V2, t2.p, sizeof(double) * n);
     HIPCHK(hipStreamSynchronize(st));
   });
 }

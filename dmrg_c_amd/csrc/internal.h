@@ -56,6 +56,17 @@ struct Context {
   void *alloc(size_t bytes);
   void release(void *p, size_t bytes);
   double *host_stage(size_t count);
+  // Copies between device memory and caller (pageable) host memory through two pinned
+  // chunk buffers: the DMA of one chunk overlaps the host copy of the other.  A plain
+  // hipMemcpy from pageable memory runs at a few GB/s.  Both return after the data has
+  // landed (copy_in: on the device, in stream order).
+  void copy_out(void *dst, const void *dev_src, size_t bytes);
+  void copy_in(void *dev_dst, const void *src, size_t bytes);
+  char *xfer_pin[2] = {nullptr, nullptr};
+  hipEvent_t xfer_ev[2] = {nullptr, nullptr};
+  // two events for the solvers' pipelined chunk read-backs
+  hipEvent_t *step_events();
+  hipEvent_t step_ev[2] = {nullptr, nullptr};
   // live kernel timing of the SpMV launches (dmrgx_profile_begin/end)
   bool prof = false;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> prof_ev;

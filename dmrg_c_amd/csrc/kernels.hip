@@ -698,17 +698,31 @@ __global__ __launch_bounds__(kBlock) void k_scan_apply(int32_t n, const int64_t 
   if (blockIdx.x == gridDim.x - 1 && threadIdx.x == kBlock - 1) out[n] = ex;
 }
 
-// One wavefront per row: copy the row's own (lower) entries, then append each of them,
-// transposed, to the tail of row c (position by an atomic counter per row; the tails are
-// put in column order by k_sort_tail).
+// A transposed entry on its way to the tail of its new row: column and value in one
+// 16-byte record, so that the scattered append touches one cache line, not two.
+struct alignas(16) TailRec {
+  int32_t col;
+  int32_t pad;
+  double val;
+};
+
+// Start of row c's tail in the record array: the strictly-lower entries of the columns
+// before c (frow counts own + tail entries per row, trow only the own ones).
+__device__ __forceinline__ int64_t tail_base(const int64_t *frow, const int64_t *trow, int32_t c) {
+  return frow[c] - (trow[c] - trow[0]);
+}
+
+// One wavefront per row: copy the row's own (lower) entries to the full CSR, and append
+// each of them, transposed, to the tail records of row c (position by an atomic counter
+// per row; k_sort_tail puts the tails in column order).
 __global__ __launch_bounds__(64) void k_tri_fill(int32_t n, const int64_t *__restrict__ trow,
                                                  const int32_t *__restrict__ tcol,
                                                  const double *__restrict__ tval,
                                                  const int64_t *__restrict__ frow,
-                                                 const int64_t *__restrict__ own,
                                                  unsigned *__restrict__ tfill,
                                                  int32_t *__restrict__ fcol,
-                                                 double *__restrict__ fval) {
+                                                 double *__restrict__ fval,
+                                                 TailRec *__restrict__ tail) {
   const int32_t r = blockIdx.x;
   if (r >= n) return;
   const int64_t b = trow[r], e = trow[r + 1];
@@ -719,17 +733,22 @@ __global__ __launch_bounds__(64) void k_tri_fill(int32_t n, const int64_t *__res
   }
   for (int64_t j = b + threadIdx.x; j < e - 1; j += 64) {
     const int32_t c = tcol[j];
-    const int64_t pos = frow[c] + own[c] + atomicAdd(&tfill[c], 1u);
-    fcol[pos] = r;
-    fval[pos] = tval[j];
+    const int64_t pos = tail_base(frow, trow, c) + atomicAdd(&tfill[c], 1u);
+    TailRec t;
+    t.col = r;
+    t.pad = 0;
+    t.val = tval[j];
+    tail[pos] = t;
   }
 }
 
-// One wavefront per row: sort the transposed tail by column (the columns are distinct).
-// Tails up to kTailCap entries are bitonic-sorted in LDS; longer ones by one lane.
+// One wavefront per row: sort the row's tail records by column (the columns are
+// distinct) and write them after the row's own entries.  Tails up to kTailCap entries are
+// bitonic-sorted in LDS; longer ones by one lane.
 constexpr int kTailCap = 512;
-__global__ __launch_bounds__(64) void k_sort_tail(int32_t n, const int64_t *__restrict__ frow,
-                                                  const int64_t *__restrict__ own,
+__global__ __launch_bounds__(64) void k_sort_tail(int32_t n, const int64_t *__restrict__ trow,
+                                                  const int64_t *__restrict__ frow,
+                                                  const TailRec *__restrict__ tail,
                                                   int32_t *__restrict__ fcol,
                                                   double *__restrict__ fval) {
   __shared__ int32_t kc[kTailCap];
@@ -737,15 +756,17 @@ __global__ __launch_bounds__(64) void k_sort_tail(int32_t n, const int64_t *__re
   const int32_t r = blockIdx.x;
   if (r >= n) return;
   const int lane = threadIdx.x;
-  const int64_t b = frow[r] + own[r], e = frow[r + 1];
+  const int64_t own = trow[r + 1] - trow[r];
+  const int64_t b = frow[r] + own, e = frow[r + 1];
   const int m = (int)(e - b);
-  if (m <= 1) return;
+  const TailRec *src = tail + tail_base(frow, trow, r);
+  if (m <= 0) return;
   if (m > kTailCap) {
     if (lane != 0) return;
-    for (int64_t i = b + 1; i < e; i++) {
-      const int32_t c0 = fcol[i];
-      const double v0 = fval[i];
-      int64_t j = i - 1;
+    for (int64_t i = 0; i < m; i++) {
+      const int32_t c0 = src[i].col;
+      const double v0 = src[i].val;
+      int64_t j = b + i - 1;
       while (j >= b && fcol[j] > c0) {
         fcol[j + 1] = fcol[j];
         fval[j + 1] = fval[j];
@@ -759,8 +780,14 @@ __global__ __launch_bounds__(64) void k_sort_tail(int32_t n, const int64_t *__re
   int P = 1;
   while (P < m) P <<= 1;
   for (int i = lane; i < P; i += 64) {
-    kc[i] = i < m ? fcol[b + i] : INT32_MAX;
-    kv[i] = i < m ? fval[b + i] : 0.0;
+    if (i < m) {
+      const TailRec t = src[i];
+      kc[i] = t.col;
+      kv[i] = t.val;
+    } else {
+      kc[i] = INT32_MAX;
+      kv[i] = 0.0;
+    }
   }
   __syncthreads();
   for (int size = 2; size <= P; size <<= 1) {
@@ -1240,7 +1267,6 @@ void launch_to_fp32(const dmrgx_matrix &A, hipStream_t s) {
 
 void launch_expand_lower(int32_t n, int64_t nnz_tri, const int64_t *trow, const int32_t *tcol,
                          const double *tval, dmrgx_matrix &A, hipStream_t s) {
-  (void)nnz_tri;
   const int nb = (n + kBlock - 1) / kBlock;
   DevBuf<int64_t> own(n), len(n);
   DevBuf<unsigned> tcnt(n), tfill(n);
@@ -1250,11 +1276,7 @@ void launch_expand_lower(int32_t n, int64_t nnz_tri, const int64_t *trow, const 
   HIPCHK(hipMemsetAsync(bad.p, 0, sizeof(int), s));
   k_tri_count<<<(n + kBlock / 64 - 1) / (kBlock / 64), kBlock, 0, s>>>(n, trow, tcol, own.p, tcnt.p, bad.p);
   HIPCHK(hipGetLastError());
-  int hbad = 0;
-  HIPCHK(hipMemcpyAsync(&hbad, bad.p, sizeof(int), hipMemcpyDeviceToHost, s));
-  HIPCHK(hipStreamSynchronize(s));
-  if (hbad & 1) throw Failure(DMRGX_ERR_NODIAG, "a row has no diagonal element as its last entry");
-  if (hbad & 2) throw Failure(DMRGX_ERR_ARG, "lower-triangle CRS has a column outside [0,row)");
+  // row lengths and pointers before the structure check is read: one round trip for both
   k_row_len<<<nb, kBlock, 0, s>>>(n, own.p, tcnt.p, len.p);
   HIPCHK(hipGetLastError());
   A.row.resize((size_t)n + 1);
@@ -1264,17 +1286,25 @@ void launch_expand_lower(int32_t n, int64_t nnz_tri, const int64_t *trow, const 
   k_scan_top<<<1, kBlock, 0, s>>>(sb, bsum.p);
   k_scan_apply<<<sb, kBlock, 0, s>>>(n, len.p, bsum.p, A.row.p);
   HIPCHK(hipGetLastError());
-  int64_t nnz = 0;
-  HIPCHK(hipMemcpyAsync(&nnz, A.row.p + n, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+  int64_t *h = reinterpret_cast<int64_t *>(ctx().host_stage(2));
+  HIPCHK(hipMemcpyAsync(h, A.row.p + n, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipMemcpyAsync(h + 1, bad.p, sizeof(int), hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
+  const int hbad = *reinterpret_cast<const int *>(h + 1);
+  if (hbad & 1) throw Failure(DMRGX_ERR_NODIAG, "a row has no diagonal element as its last entry");
+  if (hbad & 2) throw Failure(DMRGX_ERR_ARG, "lower-triangle CRS has a column outside [0,row)");
+  const int64_t nnz = h[0];
   A.n = n;
   A.nnz = nnz;
   A.col.resize(nnz);
   A.val.resize(nnz);
-  k_tri_fill<<<n, 64, 0, s>>>(n, trow, tcol, tval, A.row.p, own.p, tfill.p, A.col.p, A.val.p);
-  k_sort_tail<<<n, 64, 0, s>>>(n, A.row.p, own.p, A.col.p, A.val.p);
+  // the transposed (strictly lower) entries: all but the triangle's own (trow[0] == 0)
+  DevBuf<TailRec> tail(std::max<int64_t>(nnz - nnz_tri, 1));
+  k_tri_fill<<<n, 64, 0, s>>>(n, trow, tcol, tval, A.row.p, tfill.p, A.col.p, A.val.p, tail.p);
+  k_sort_tail<<<n, 64, 0, s>>>(n, trow, A.row.p, tail.p, A.col.p, A.val.p);
   HIPCHK(hipGetLastError());
-  HIPCHK(hipStreamSynchronize(s));
+  // no wait: the matrix's users are queued behind the fill on this stream (and the scratch
+  // released here is only reused in stream order)
 }
 
 }  // namespace dmrgx

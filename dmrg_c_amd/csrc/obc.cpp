@@ -23,7 +23,7 @@ namespace {
 template <class T>
 void up(DevBuf<T> &b, const T *h, size_t n) {
   b.resize(n ? n : 1);
-  if (n) HIPCHK(hipMemcpyAsync(b.p, h, sizeof(T) * n, hipMemcpyHostToDevice, ctx().stream));
+  if (n) ctx().copy_in(b.p, h, sizeof(T) * n);
 }
 
 void up_crs(const CRS1 *M, int rows_needed, int col_limit, DevBuf<int64_t> &r, DevBuf<int32_t> &c,
@@ -131,9 +131,9 @@ void DMRG_MATRIX_VECTOR_PRODUCT_OBC(CRS1 *M_LLLR, CRS1 *M_LRRL, CRS1 *M_LRRL_Sig
     const int64_t n = O->n;
     DevBuf<double> x(n), y(n);
     hipStream_t st = ctx().stream;
-    HIPCHK(hipMemcpyAsync(x.p, V, sizeof(double) * n, hipMemcpyHostToDevice, st));
+    ctx().copy_in(x.p, V, sizeof(double) * n);
     launch_spmv(op_of(*O), x.p, y.p, st);
-    HIPCHK(hipMemcpyAsync(Out_V, y.p, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+    ctx().copy_out(Out_V, y.p, sizeof(double) * n);
     HIPCHK(hipStreamSynchronize(st));
   });
 }
@@ -147,11 +147,11 @@ double DMRG_CONVERGE_CHECK_OBC(double eig_val, CRS1 *M_LLLR, CRS1 *M_LRRL, CRS1 
     const int64_t n = O->n;
     DevBuf<double> x(n), y(n);
     hipStream_t st = ctx().stream;
-    HIPCHK(hipMemcpyAsync(x.p, Eig_V, sizeof(double) * n, hipMemcpyHostToDevice, st));
+    ctx().copy_in(x.p, Eig_V, sizeof(double) * n);
     r = converge_check(op_of(*O), x.p, eig_val);
     // the reference leaves H v in Temp_V (DMRG_CONVERGE_CHECK_OBC.c:10)
     launch_spmv(op_of(*O), x.p, y.p, st);
-    HIPCHK(hipMemcpyAsync(Temp_V, y.p, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+    ctx().copy_out(Temp_V, y.p, sizeof(double) * n);
     HIPCHK(hipStreamSynchronize(st));
   });
   return r;
@@ -170,7 +170,7 @@ void DMRG_LANCZOS_SLOW_OBC(DMRG_BOX_LAN *Box) {
     const int64_t n = O->n;
     hipStream_t st = ctx().stream;
     DevBuf<double> v(n);
-    if (guess) HIPCHK(hipMemcpyAsync(v.p, Box->Eig_Vec, sizeof(double) * n, hipMemcpyHostToDevice, st));
+    if (guess) ctx().copy_in(v.p, Box->Eig_Vec, sizeof(double) * n);
     else launch_start_vector(n, ctx().seed, v.p, st);
     LanczosOpts o;
     o.acc = Box->acc;
@@ -185,7 +185,7 @@ void DMRG_LANCZOS_SLOW_OBC(DMRG_BOX_LAN *Box) {
       fatal(fn, "Does not converge (max step is %d)\n", Box->max_step);
     }
     Box->eig_val[0] = out.eig;
-    HIPCHK(hipMemcpyAsync(Box->Eig_Vec, v.p, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+    ctx().copy_out(Box->Eig_Vec, v.p, sizeof(double) * n);
     HIPCHK(hipStreamSynchronize(st));
     sml_append("LANCZOS_SLOW_OBC_Step.txt", "%d\n", out.lan_steps);
   });
@@ -198,12 +198,12 @@ void DMRG_CONJUGATE_GRADIENT_OBC(DMRG_BOX_CG *Box) {
     const int64_t n = O->n;
     hipStream_t st = ctx().stream;
     DevBuf<double> b(n), x(n);
-    HIPCHK(hipMemcpyAsync(b.p, Box->Vec, sizeof(double) * n, hipMemcpyHostToDevice, st));
-    HIPCHK(hipMemcpyAsync(x.p, Box->Out_Vec, sizeof(double) * n, hipMemcpyHostToDevice, st));
+    ctx().copy_in(b.p, Box->Vec, sizeof(double) * n);
+    ctx().copy_in(x.p, Box->Out_Vec, sizeof(double) * n);
     SolveOut out;
     bool conv = false;
     const int it = cg_solve(op_of(*O), Box->diag_val, b.p, x.p, Box->acc, Box->max_step, &conv, out);
-    HIPCHK(hipMemcpyAsync(Box->Out_Vec, x.p, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+    ctx().copy_out(Box->Out_Vec, x.p, sizeof(double) * n);
     HIPCHK(hipStreamSynchronize(st));
     if (!conv) {
       printf("Error in %s\nDoes not converge (max step is %d)\nContinue...\n", fn, Box->max_step);
@@ -220,11 +220,11 @@ void DMRG_INVERSE_ITERATION_OBC(DMRG_BOX_II *Box) {
     const int64_t n = O->n;
     hipStream_t st = ctx().stream;
     DevBuf<double> v(n);
-    HIPCHK(hipMemcpyAsync(v.p, Box->Eig_Vec, sizeof(double) * n, hipMemcpyHostToDevice, st));
+    ctx().copy_in(v.p, Box->Eig_Vec, sizeof(double) * n);
     SolveOut out;
     inverse_iteration(op_of(*O), Box->eig_val[0], v.p, Box->ii_acc, Box->ii_diag_add, Box->ii_max_step, Box->cg_acc,
                       Box->cg_max_step, out);
-    HIPCHK(hipMemcpyAsync(Box->Eig_Vec, v.p, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+    ctx().copy_out(Box->Eig_Vec, v.p, sizeof(double) * n);
     HIPCHK(hipStreamSynchronize(st));
     Box->error = out.err;
     if (out.ii_converged) {

@@ -60,6 +60,16 @@ double read_scalar(const double *d, hipStream_t st) {
 
 }  // namespace
 
+// Steps (iterations) queued behind the chunk whose read-back the host examines: enough
+// device work to cover the host's check and next launch, and all that runs past
+// convergence.  A Lanczos step costs a product; a CG iteration past convergence is skipped
+// on the device and costs only its launches.
+constexpr int kLanczosAhead = 2;
+constexpr int kCgAhead = 4;
+// Longest CG chunk: |r|^2 does not fall geometrically, so long predictions overshoot, and
+// every iteration launched past convergence still costs the host its three launches.
+constexpr int kCgChunkMax = 16;
+
 void lanczos(const Operator &A, const LanczosOpts &o, double *vec, SolveOut &out) {
   Context &C = ctx();
   hipStream_t st = C.stream;
@@ -87,29 +97,57 @@ void lanczos(const Operator &A, const LanczosOpts &o, double *vec, SolveOut &out
     launch_lanczos_a(A, o.fp32, 0, w.p, nrm2, 0.0, V.vec(0), y.p, alpha, R, st);
     launch_lanczos_b(n, 0, y.p, V.vec(0), nullptr, alpha, beta, w.p, nrm2, R, st);
     out.spmv++;
-    int next = 1, kconv = -1, chunk = small ? 16 : 8;
+    int kconv = -1, chunk = small ? 16 : 8;
     double dE_first = 0.0, dE_last = 0.0;   // first and last |E_k - E_{k-1}| of the chunk
     int n_dE = 0;
     double lam = E;
     RitzTrack track;  // T of this block
     bool block_done = false;
-    while (next < block && total_step < max_step) {
-      const int c = std::min({chunk, block - next, max_step - total_step});
+    // Chunks of steps are pipelined: the next chunk is launched before the host examines
+    // the previous one's alpha/beta, so the device does not wait for the host.  Steps
+    // launched past convergence compute for nothing; the chunk lengths keep them few.
+    int enext = 1, etotal = total_step;   // next step to launch, steps launched
+    struct Pending {
+      int k0, count, slot;
+    };
+    Pending pend[2];
+    int npend = 0, nslot = 0;
+    const size_t span = 2 * ((size_t)block + 2);   // alpha | beta per read-back slot
+    double *hbase = C.host_stage(2 * span);
+    hipEvent_t *ev = C.step_events();
+    auto room = [&] { return std::min(block - enext, max_step - etotal); };
+    auto enqueue = [&](int c) {
       // two kernels per step: each sums the other's block partials itself (the scalar the
-      // finish kernel would store, same order); the chunk's last step gets the finish kernel
-      // so that the host reads beta_{k+1}
+      // finish kernel would store, same order); the chunk's last step gets the finish
+      // kernel so that the host reads beta_{k+1}
       int nb = 0;
-      for (int k = next; k < next + c; k++) {
-        const bool first = k == next;
+      for (int k = enext; k < enext + c; k++) {
+        const bool first = k == enext;
         const int na = launch_lanczos_a_part(A, o.fp32, k, w.p, nrm2, beta, first ? nullptr : RB.partial.p,
                                              first ? 0 : nb, V.vec(k), y.p, alpha, RA, st);
         nb = launch_lanczos_b_part(n, k, y.p, V.vec(k), V.vec(k - 1), alpha, beta, RA.partial.p, na, w.p, RB, st);
-        if (k == next + c - 1) launch_lanczos_b_finish(k, beta, nrm2, RB, nb, st);
+        if (k == enext + c - 1) launch_lanczos_b_finish(k, beta, nrm2, RB, nb, st);
       }
-      double *h = C.host_stage(2 * ((size_t)block + 2));
-      HIPCHK(hipMemcpyAsync(h, alpha, sizeof(double) * (next + c), hipMemcpyDeviceToHost, st));
-      HIPCHK(hipMemcpyAsync(h + block + 2, beta, sizeof(double) * (next + c + 1), hipMemcpyDeviceToHost, st));
-      HIPCHK(hipStreamSynchronize(st));
+      double *h = hbase + nslot * span;
+      HIPCHK(hipMemcpyAsync(h, alpha, sizeof(double) * (enext + c), hipMemcpyDeviceToHost, st));
+      HIPCHK(hipMemcpyAsync(h + block + 2, beta, sizeof(double) * (enext + c + 1), hipMemcpyDeviceToHost, st));
+      HIPCHK(hipEventRecord(ev[nslot], st));
+      pend[npend++] = Pending{enext, c, nslot};
+      nslot ^= 1;
+      enext += c;
+      etotal += c;
+    };
+    if (room() > 0) enqueue(std::min(chunk, room()));
+    while (npend > 0) {
+      // a short chunk queued behind the one the host waits for keeps the device busy
+      // while the host examines it; it is all that runs past convergence
+      if (npend == 1 && room() > 0) enqueue(std::min(kLanczosAhead, room()));
+      const Pending q = pend[0];
+      pend[0] = pend[1];
+      npend--;
+      HIPCHK(hipEventSynchronize(ev[q.slot]));
+      const double *h = hbase + q.slot * span;
+      const int next = q.k0, c = q.count;
       std::copy(h, h + next + c, ha.begin());
       std::copy(h + block + 2, h + block + 2 + next + c + 1, hb.begin());
       for (int k = next; k < next + c; k++) {
@@ -149,18 +187,21 @@ void lanczos(const Operator &A, const LanczosOpts &o, double *vec, SolveOut &out
         E = lam;
       }
       if (kconv >= 0 || block_done) break;
-      next += c;
-      // Next chunk: the steps the last two energy changes predict ("Normal" convergence is
-      // geometric) plus two -- steps past convergence are computed for nothing -- or
+      // Next chunk: the steps the chunk's energy changes predict ("Normal" convergence is
+      // geometric) plus two, less the steps already queued behind this chunk -- or
       // doubling while no prediction holds.
+      const int queued = (npend > 0) ? pend[0].count : 0;
+      int nc;
       if (!small && o.type != 1 && n_dE >= 3 && dE_last > 0.0 && dE_last < dE_first) {
-        const double q = std::pow(dE_last / dE_first, 1.0 / (n_dE - 1));   // mean ratio per step
-        const double need = std::ceil(std::log(o.acc / dE_last) / std::log(q));
-        chunk = (int)std::min(32.0, std::max(8.0, need + 2.0));
+        const double ratio = std::pow(dE_last / dE_first, 1.0 / (n_dE - 1));   // mean ratio per step
+        const double need = std::ceil(std::log(o.acc / dE_last) / std::log(ratio)) + 2.0;
+        nc = (int)std::min(32.0, need - queued);
       } else {
         chunk = std::min(chunk * 2, 32);
+        nc = chunk;
       }
       n_dE = 0;
+      if (nc > 0 && room() > 0) enqueue(std::min(nc, room()));
     }
     if (kconv >= 0) {
       tridiag_vector(kconv + 1, ha.data(), hb.data() + 1, lam, s.data());
@@ -261,13 +302,23 @@ int cg_solve(const Operator &A, double shift, const double *b, double *x, double
   launch_norm2(n, x, R, nrm2, st);
   launch_cg_init(A, shift, nrm2, x, xw.p, b, r.p, p.p, rr, R, st);
   out.spmv++;
-  int it = 1, chunk = 4, done_it = -1;
-  // read-back: {done, last iteration} and |r|^2 of the chunk's last iteration
-  double *hst = C.host_stage(2);
-  int *hs = reinterpret_cast<int *>(hst);
-  double rr_prev = -1.0;   // |r|^2 at the end of the previous chunk (rr[0] for the first)
-  while (it < max_step) {
-    const int c = std::min(chunk, max_step - it);
+  // Chunks of iterations are pipelined: the next chunk is launched before the previous
+  // one's read-back ({done, last iteration} and the last |r|^2) is examined, so the device
+  // does not wait for the host.  Iterations launched past convergence are skipped on the
+  // device (every kernel checks the flag).
+  int it = 1, done_it = -1;
+  double *hst = C.host_stage(8);
+  hipEvent_t *ev = C.step_events();
+  struct Pending {
+    int first, count;
+    double *h;   // {done, last iteration} | |r|^2 at the chunk's end | |r|^2 before it
+    int slot;
+  };
+  Pending pend[2];
+  int npend = 0, nslot = 0;
+  auto enqueue = [&](int c) {
+    Pending q{it, c, hst + 4 * nslot, nslot};
+    nslot ^= 1;
     // three kernels per iteration, each summing the previous one's block partials itself
     // (pAp in the update, |r|^2 and the convergence flag in the direction kernel)
     for (int i = it; i < it + c; i++) {
@@ -275,27 +326,47 @@ int cg_solve(const Operator &A, double shift, const double *b, double *x, double
       const int nb = launch_cg_update_part(n, i, p.p, y.p, xw.p, r.p, pap, rr, status.p, RA.partial.p, na, RB, st);
       launch_cg_dir_part(n, i, acc, r.p, rr, p.p, status.p, RB.partial.p, nb, st);
     }
-    HIPCHK(hipMemcpyAsync(hs, status.p, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
-    HIPCHK(hipMemcpyAsync(hst + 1, rr + it + c - 1, sizeof(double), hipMemcpyDeviceToHost, st));
-    if (rr_prev < 0.0) HIPCHK(hipMemcpyAsync(&rr_prev, rr, sizeof(double), hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
+    HIPCHK(hipMemcpyAsync(q.h, status.p, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(q.h + 1, rr + it + c - 1, sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(q.h + 2, rr + it - 1, sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipEventRecord(ev[q.slot], st));
+    pend[npend++] = q;
+    it += c;
+  };
+  // iterations the |r|^2 decrease over a finished chunk predicts are still needed, plus two
+  // (launches past convergence cost little, a chunk that stops short costs a round trip)
+  auto predict = [&](const Pending &q) {
+    const double rr_last = q.h[1], rr_prev = q.h[2];
+    const double ratio = (rr_prev > 0.0 && rr_last > 0.0) ? std::pow(rr_last / rr_prev, 1.0 / q.count) : 1.0;
+    if (ratio > 0.0 && ratio < 0.999 && rr_last > acc)
+      return (int)std::ceil(std::log(acc / rr_last) / std::log(ratio)) + 2;
+    return -1;   // no prediction: the chunk length doubles
+  };
+  int chunk = 4;
+  if (it < max_step) enqueue(std::min(chunk, max_step - it));
+  while (npend > 0) {
+    // a short chunk queued behind the one the host waits for keeps the device busy while
+    // the host examines it (at most two are queued)
+    if (npend == 1 && it < max_step) enqueue(std::min(kCgAhead, max_step - it));
+    const Pending q = pend[0];
+    pend[0] = pend[1];
+    npend--;
+    HIPCHK(hipEventSynchronize(ev[q.slot]));
+    const int *hs = reinterpret_cast<const int *>(q.h);
     if (hs[0]) {
       done_it = hs[1];
       break;
     }
-    // Next chunk: the iterations the chunk's mean decrease of |r|^2 predicts to reach acc,
-    // plus two (iterations beyond convergence are skipped on the device, but still cost
-    // their launches); doubling while |r|^2 does not decrease.
-    const double rr_last = hst[1];
-    const double q = (rr_prev > 0.0 && rr_last > 0.0) ? std::pow(rr_last / rr_prev, 1.0 / c) : 1.0;
-    if (q > 0.0 && q < 0.999 && rr_last > acc) {
-      const double need = std::ceil(std::log(acc / rr_last) / std::log(q));
-      chunk = (int)std::min(64.0, std::max(4.0, need + 2.0));
+    const int need = predict(q);
+    const int queued = (npend > 0) ? pend[0].count : 0;
+    int nc;
+    if (need > 0) {
+      nc = std::min(kCgChunkMax, need - queued);
     } else {
-      chunk = std::min(chunk * 2, 64);
+      chunk = std::min(chunk * 2, kCgChunkMax);
+      nc = chunk;
     }
-    rr_prev = rr_last;
-    it += c;
+    if (nc > 0 && it < max_step) enqueue(std::min(nc, max_step - it));
   }
   launch_copy(n, xw.p, x, st);
   HIPCHK(hipStreamSynchronize(st));

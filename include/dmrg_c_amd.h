@@ -481,9 +481,10 @@ int dmrgx_spmv_host(const dmrgx_matrix *A, const double *x, double *y);
  * change results; lanes and bit 0 change the summation order within a row, and cap and
  * bit 2 the order of the dot products fused into the solvers' products.  Defaults: lanes
  * from the mean row length, cap 4096, variant 2;
- * dmrgx_matrix_autotune times the candidates on the device (product + reduction, as the
- * solvers run it) and keeps the fastest.  Single and batched solves use the matrix's
- * schedule. */
+ * dmrgx_matrix_autotune times every candidate on the device (product + fused dot), then
+ * times the eight fastest as the solver runs them -- a Lanczos step and a CG iteration
+ * with their fused vector kernels -- and keeps the fastest; *best_us_out is that
+ * step-pair time.  Single and batched solves use the matrix's schedule. */
 int dmrgx_matrix_set_schedule(dmrgx_matrix *A, int32_t lanes, int32_t grid_cap, int32_t load_variant);
 int dmrgx_matrix_get_schedule(const dmrgx_matrix *A, int32_t *lanes, int32_t *grid_cap, int32_t *load_variant);
 int dmrgx_matrix_autotune(dmrgx_matrix *A, int32_t *lanes_out, int32_t *grid_out, int32_t *load_out,
