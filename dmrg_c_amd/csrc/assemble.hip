@@ -96,6 +96,10 @@ struct dmrgx_superblock {
   dmrgx::DevBuf<int32_t> lcol;
   dmrgx::DevBuf<double> lval;
   int64_t lnnz = -1;
+  // counted by the assembly for the expansion: strictly-lower entries per column, and a
+  // flag (bit 0) for rows without a diagonal entry
+  dmrgx::DevBuf<unsigned> tcnt;
+  dmrgx::DevBuf<int> bad;
   dmrgx::SbView view() const {
     return dmrgx::SbView{n, dim_RR, d, nt, has_zero, LL.p, LR.p, RR.p, RL.p, inv.p, terms.p, ops.p, ele_ptrs.p};
   }
@@ -141,7 +145,8 @@ __global__ __launch_bounds__(64) void k_assemble(SbView sb, const int32_t *row_l
                                                  int64_t *counts, const int64_t *row_ptr, int32_t *col_out,
                                                  double *val_out, int32_t *overflow, int32_t *n_overflow,
                                                  int32_t *scol = nullptr, double *sval = nullptr,
-                                                 uint8_t *fitted = nullptr) {
+                                                 uint8_t *fitted = nullptr, unsigned *tcnt = nullptr,
+                                                 int *bad = nullptr) {
   __shared__ uint64_t key[CAP];
   __shared__ double val[CAP];
   const int lane = threadIdx.x;
@@ -258,6 +263,8 @@ __global__ __launch_bounds__(64) void k_assemble(SbView sb, const int32_t *row_l
       scol[dst] = c;
       sval[dst] = s;
     }
+    // the count pass also counts each column's strictly-lower entries for the expansion
+    if (!FILL && tcnt && keep && c < r) atomicAdd(&tcnt[c], 1u);
     out += __popcll(km);
   }
   if (!diag_seen && sb.has_zero) {
@@ -274,6 +281,7 @@ __global__ __launch_bounds__(64) void k_assemble(SbView sb, const int32_t *row_l
   if (!FILL && lane == 0) {
     counts[r] = out;
     if (fitted) fitted[r] = 1;
+    if (bad && !diag_seen && !sb.has_zero) atomicOr(bad, 1);
   }
 }
 
@@ -476,9 +484,13 @@ static int64_t assemble_lower(dmrgx_superblock &S) {
   DevBuf<int32_t> scol(keep ? (size_t)n * stride : 0);
   DevBuf<double> sval(keep ? (size_t)n * stride : 0);
   DevBuf<uint8_t> fitted(keep ? (size_t)n : 0);
+  S.tcnt.resize(n);
+  S.bad.resize(1);
+  HIPCHK(hipMemsetAsync(S.tcnt.p, 0, sizeof(unsigned) * n, st));
+  HIPCHK(hipMemsetAsync(S.bad.p, 0, sizeof(int), st));
   // count pass: small lists for every row, then the rows that overflowed with larger lists
   k_assemble<kCapSmall, false><<<n, 64, 0, st>>>(v, nullptr, n, counts.p, nullptr, nullptr, nullptr, over1.p,
-                                                 n_over.p, scol.p, sval.p, fitted.p);
+                                                 n_over.p, scol.p, sval.p, fitted.p, S.tcnt.p, S.bad.p);
   HIPCHK(hipGetLastError());
   // row pointers right away: they hold unless some row overflowed (then they are redone),
   // so the common case reads the overflow count and nnz with one round trip
@@ -494,7 +506,8 @@ static int64_t assemble_lower(dmrgx_superblock &S) {
   int n_large = 0;
   if (n_mid > 0) {
     k_assemble<kCapMid, false><<<n_mid, 64, 0, st>>>(v, over1.p, n_mid, counts.p, nullptr, nullptr, nullptr,
-                                                     over2.p, n_over.p + 1);
+                                                     over2.p, n_over.p + 1, nullptr, nullptr, nullptr, S.tcnt.p,
+                                                     S.bad.p);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(h + 1, n_over.p + 1, sizeof(int), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
@@ -502,7 +515,8 @@ static int64_t assemble_lower(dmrgx_superblock &S) {
   }
   if (n_large > 0) {
     k_assemble<kCapLarge, false><<<n_large, 64, 0, st>>>(v, over2.p, n_large, counts.p, nullptr, nullptr,
-                                                         nullptr, nullptr, n_over.p + 2);
+                                                         nullptr, nullptr, n_over.p + 2, nullptr, nullptr, nullptr,
+                                                         S.tcnt.p, S.bad.p);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(h + 2, n_over.p + 2, sizeof(int), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
@@ -538,7 +552,7 @@ static int64_t assemble_lower(dmrgx_superblock &S) {
 
 static dmrgx_matrix *expand_last(dmrgx_superblock &S) {
   auto A = std::make_unique<dmrgx_matrix>();
-  launch_expand_lower(S.n, S.lnnz, S.lrow.p, S.lcol.p, S.lval.p, *A, ctx().stream);
+  launch_expand_lower(S.n, S.lnnz, S.lrow.p, S.lcol.p, S.lval.p, *A, ctx().stream, S.tcnt.p, S.bad.p);
   default_schedule(*A);
   return A.release();
 }

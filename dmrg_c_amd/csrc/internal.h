@@ -551,8 +551,12 @@ int spmv_blocks(int32_t n, int lanes, int grid_cap);
 void launch_spmv(const Operator &op, const double *x, double *y, hipStream_t s);
 // y = A x and *dot = <x, y>: the shape of the solver's products (reduction + finish kernel)
 void launch_spmv_dot(const Operator &op, const double *x, double *y, double *dot, Reduce &R, hipStream_t s);
+// Full CSR from a lower triangle (diagonal last in every row).  A producer that already
+// counted the strictly-lower entries per column (tcnt) and checked the rows' diagonals
+// (bad, nonzero if a row lacks it) passes them; otherwise they are computed here.
 void launch_expand_lower(int32_t n, int64_t nnz_tri, const int64_t *trow, const int32_t *tcol,
-                         const double *tval, dmrgx_matrix &A, hipStream_t s);
+                         const double *tval, dmrgx_matrix &A, hipStream_t s,
+                         const unsigned *tcnt_given = nullptr, const int *bad_given = nullptr);
 void launch_to_fp32(const dmrgx_matrix &A, hipStream_t s);
 // value dictionary of A (ent, table); returns the number of distinct values, 0 if none built
 int64_t build_value_dict(dmrgx_matrix &A, hipStream_t s);

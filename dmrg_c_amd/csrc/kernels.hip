@@ -627,10 +627,10 @@ __global__ __launch_bounds__(kBlock) void k_tri_count(int32_t n, const int64_t *
   }
 }
 
-__global__ __launch_bounds__(kBlock) void k_row_len(int32_t n, const int64_t *own,
+__global__ __launch_bounds__(kBlock) void k_row_len(int32_t n, const int64_t *trow,
                                                     const unsigned *tcnt, int64_t *len) {
   const int32_t r = blockIdx.x * kBlock + threadIdx.x;
-  if (r < n) len[r] = own[r] + tcnt[r];
+  if (r < n) len[r] = (trow[r + 1] - trow[r]) + tcnt[r];
 }
 
 // exclusive scan of len[0..n) into out[0..n], 1024 elements per block
@@ -1266,18 +1266,28 @@ void launch_to_fp32(const dmrgx_matrix &A, hipStream_t s) {
 }
 
 void launch_expand_lower(int32_t n, int64_t nnz_tri, const int64_t *trow, const int32_t *tcol,
-                         const double *tval, dmrgx_matrix &A, hipStream_t s) {
+                         const double *tval, dmrgx_matrix &A, hipStream_t s, const unsigned *tcnt_given,
+                         const int *bad_given) {
   const int nb = (n + kBlock - 1) / kBlock;
-  DevBuf<int64_t> own(n), len(n);
-  DevBuf<unsigned> tcnt(n), tfill(n);
-  DevBuf<int> bad(1);
-  HIPCHK(hipMemsetAsync(tcnt.p, 0, sizeof(unsigned) * n, s));
+  DevBuf<int64_t> own, len(n);
+  DevBuf<unsigned> tcnt, tfill(n);
+  DevBuf<int> bad;
   HIPCHK(hipMemsetAsync(tfill.p, 0, sizeof(unsigned) * n, s));
-  HIPCHK(hipMemsetAsync(bad.p, 0, sizeof(int), s));
-  k_tri_count<<<(n + kBlock / 64 - 1) / (kBlock / 64), kBlock, 0, s>>>(n, trow, tcol, own.p, tcnt.p, bad.p);
-  HIPCHK(hipGetLastError());
+  const unsigned *cnt = tcnt_given;
+  const int *badp = bad_given;
+  if (!cnt) {
+    own.resize(n);
+    tcnt.resize(n);
+    bad.resize(1);
+    HIPCHK(hipMemsetAsync(tcnt.p, 0, sizeof(unsigned) * n, s));
+    HIPCHK(hipMemsetAsync(bad.p, 0, sizeof(int), s));
+    k_tri_count<<<(n + kBlock / 64 - 1) / (kBlock / 64), kBlock, 0, s>>>(n, trow, tcol, own.p, tcnt.p, bad.p);
+    HIPCHK(hipGetLastError());
+    cnt = tcnt.p;
+    badp = bad.p;
+  }
   // row lengths and pointers before the structure check is read: one round trip for both
-  k_row_len<<<nb, kBlock, 0, s>>>(n, own.p, tcnt.p, len.p);
+  k_row_len<<<nb, kBlock, 0, s>>>(n, trow, cnt, len.p);
   HIPCHK(hipGetLastError());
   A.row.resize((size_t)n + 1);
   const int sb = (n + kScanChunk - 1) / kScanChunk;
@@ -1288,7 +1298,7 @@ void launch_expand_lower(int32_t n, int64_t nnz_tri, const int64_t *trow, const 
   HIPCHK(hipGetLastError());
   int64_t *h = reinterpret_cast<int64_t *>(ctx().host_stage(2));
   HIPCHK(hipMemcpyAsync(h, A.row.p + n, sizeof(int64_t), hipMemcpyDeviceToHost, s));
-  HIPCHK(hipMemcpyAsync(h + 1, bad.p, sizeof(int), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipMemcpyAsync(h + 1, badp, sizeof(int), hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
   const int hbad = *reinterpret_cast<const int *>(h + 1);
   if (hbad & 1) throw Failure(DMRGX_ERR_NODIAG, "a row has no diagonal element as its last entry");
