@@ -532,7 +532,9 @@ def main():
             "alg_bytes_per_launch": int(bytes_per_launch),
             "avg_launch_us": round(avg_launch_s * 1e6, 2),
             "launches": int(launches.value),
-            "launches_sampled": f"1 in {PROFILE_EVERY} SpMV launches of the timed solves, HIP events on the library stream",
+            "launches_sampled": f"1 in {PROFILE_EVERY} of the timed solves' SpMV launches that always run in full "
+                                "(Lanczos products, start residuals; CG products can return at once past "
+                                "convergence and are left out), HIP events on the library stream",
         },
         "cpu_baseline": None,
     }

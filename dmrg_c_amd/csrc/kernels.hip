@@ -93,6 +93,11 @@ struct DeltaRows {
 // Epilogues with kLateBegin load their scalar's inputs in prefetch() and finish it in
 // begin() only after the block's first row group has been gathered.
 template <class E, class = void>
+struct MaySkip : std::false_type {};
+template <class E>
+struct MaySkip<E, std::void_t<decltype(E::kMaySkip)>> : std::bool_constant<E::kMaySkip> {};
+
+template <class E, class = void>
 struct LateBegin : std::false_type {};
 template <class E>
 struct LateBegin<E, std::void_t<decltype(E::kLateBegin)>> : std::bool_constant<E::kLateBegin> {};
@@ -109,14 +114,15 @@ __global__ __launch_bounds__(kBlock) void k_apply(int32_t n, Rows rows, const do
   double c = 0.0;
   int32_t gi = gr.start;
   if constexpr (LateBegin<Epi>::value) {
-    epi.prefetch();
+    typename Epi::Pre pf;
+    epi.prefetch(pf);
     const bool have = gi < gr.end;   // the same for the whole block
     const int32_t r = gi * RPB + g;
     double s = 0.0;
     if (have && r < n) s = rows.template lane_sum<G>(r, lane, x);
 #pragma unroll
     for (int off = G / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, G);
-    epi.begin();   // workgroup-wide
+    epi.begin(pf);
     if (have) {
       if (lane == 0 && r < n) c += epi.row(r, s);
       gi += gr.step;
@@ -154,35 +160,32 @@ struct EpiLanczosA {
   double shift;   // operator H + shift I (the minimum-residual solve of inverse iteration)
   double scale;
   // With pre set, |w|^2 is the sum of the previous k_lanczos_b's pre_n block partials, taken
-  // in every block exactly as its finish kernel would (block 0 also stores *nrm2 and beta[k]);
-  // without, it is read from *nrm2.
+  // by every wavefront exactly as the finish kernel would (block 0 also stores *nrm2 and
+  // beta[k]); without, it is read from *nrm2.
   const double *pre = nullptr;
   int pre_n = 0;
   double *beta = nullptr;
-  PartialSum<1> ps;   // one partial per thread ahead of the gathers (more are read after them)
+  // the partials loaded ahead of the gathers (those past four per lane are read after them);
+  // held by the kernel as a local, so that they stay in registers
+  using Pre = WavePartialSum<1>;
   static constexpr bool kReduce = true;
   static constexpr bool kLateBegin = true;
   __device__ bool skip() const { return false; }
-  __device__ void prefetch() {
+  __device__ void prefetch(Pre &ps) {
     if (pre) ps.load(pre, pre_n);
     else scale = *nrm2;
   }
-  __device__ void begin() {
+  __device__ void begin(const Pre &ps) {
     if (!pre) {
       scale = 1.0 / sqrt(scale);
       return;
     }
-    __shared__ double bc;
     const double t = ps.sum(pre, pre_n);
-    if (threadIdx.x == 0) {
-      bc = t;
-      if (blockIdx.x == 0) {
-        *nrm2 = t;
-        beta[k] = sqrt(t);
-      }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      *nrm2 = t;
+      beta[k] = sqrt(t);
     }
-    __syncthreads();
-    scale = 1.0 / sqrt(bc);
+    scale = 1.0 / sqrt(t);
   }
   __device__ double row(int32_t r, double s) {
     const double xs = scale * w[r];
@@ -236,6 +239,7 @@ struct EpiCgAp {
   const int *status;
   double shift;
   static constexpr bool kReduce = true;
+  static constexpr bool kMaySkip = true;   // launches past convergence return at once
   __device__ bool skip() const { return status[0] != 0; }
   __device__ void begin() {}
   __device__ double row(int32_t r, double s) {
@@ -290,7 +294,9 @@ int apply_rows(int32_t n, int lanes, int grid_cap, const Rows &rows, const doubl
   }
   Context &C = ctx();
   std::pair<hipEvent_t, hipEvent_t> *ev = nullptr;
-  if (Rows::kTimed && C.prof_take()) {
+  // live timing brackets only products that always run in full (a skipped launch would
+  // count its bytes over a few microseconds)
+  if (Rows::kTimed && !MaySkip<Epi>::value && C.prof_take()) {
     ev = C.prof_slot();
     HIPCHK(hipEventRecord(ev->first, s));
     C.prof_bytes += rows.alg_bytes(n);

@@ -76,6 +76,41 @@ struct PartialSum {
   }
 };
 
+// finish_sum's result computed by each wavefront on its own (no LDS, no barrier): lane l
+// plays the finishing workgroup's threads l, l + 64, ... in turn, each adding its partials
+// in index order, and their wavefront sums are added in wavefront order -- the same
+// operations in the same order as finish_sum.  load() issues D partials per played thread.
+template <int D>
+struct WavePartialSum {
+  static constexpr int W = kBlock / 64;
+  double p[W][D];
+  __device__ __forceinline__ void load(const double *__restrict__ partial, int nblk) {
+    const int l = threadIdx.x & 63;
+#pragma unroll
+    for (int w = 0; w < W; w++)
+#pragma unroll
+      for (int u = 0; u < D; u++) {
+        const int i = 64 * w + l + u * kBlock;
+        p[w][u] = (i < nblk) ? partial[i] : 0.0;
+      }
+  }
+  __device__ __forceinline__ double sum(const double *__restrict__ partial, int nblk) const {
+    const int l = threadIdx.x & 63;
+    double tot = 0.0;
+#pragma unroll
+    for (int w = 0; w < W; w++) {
+      const int t = 64 * w + l;
+      double s = 0.0;
+#pragma unroll
+      for (int u = 0; u < D; u++)
+        if (t + u * kBlock < nblk) s += p[w][u];
+      for (int i = t + D * kBlock; i < nblk; i += kBlock) s += partial[i];
+      tot += __shfl(wave_sum(s), 0, 64);
+    }
+    return tot;
+  }
+};
+
 __device__ __forceinline__ double finish_sum(const double *__restrict__ partial, int nblk) {
   PartialSum<16> ps;
   ps.load(partial, nblk);

@@ -682,9 +682,11 @@ int dmrgx_ground_state_batch(int32_t count, dmrgx_matrix *const *A, const dmrgx_
                              double *const *vec_out, dmrgx_solve_stats *stats);
 
 /* ---- measurement
- * Between begin and end, every CSR SpMV launch of the library is bracketed by HIP events
- * on the library's stream; end returns the summed kernel time, the launch count and the
- * algorithmic bytes (nnz*(sizeof(val)+4) + (n+1)*8 + 2*n*8 per launch). */
+ * Between begin and end, every CSR SpMV launch of the library that always runs in full is
+ * bracketed by HIP events on the library's stream (the CG products are not: a launch past
+ * convergence returns at once on the device); end returns the summed kernel time, the
+ * launch count and the algorithmic bytes (nnz*(sizeof(val)+4) + (n+1)*8 + 2*n*8 per
+ * launch). */
 int dmrgx_profile_begin(void);
 /* As dmrgx_profile_begin, bracketing only the first of every `every` SpMV launches: each
  * event pair costs the stream a few microseconds between kernels, so sampling keeps the
