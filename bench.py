@@ -306,6 +306,45 @@ def run_ed_batch(D, count, precision_forms=(0, 1), single=True):
     return res
 
 
+# ------------------------------------------------------------------ basis construction
+BASIS_QN = os.path.join(REPO, "tests", "golden", "qn_hubbard_L64_m400_s31.npz")
+
+
+def run_basis(D, reps: int, cpu: bool):
+    """Superblock basis of the Hubbard L=64 m=400 step (the bench superblock, n=144542) from
+    the blocks' quantum numbers the reference run had: dmrgx_superblock_basis (sectors on the
+    host, rows on the device, arrays back in host memory) against the reference's own arrays,
+    and the oracle's restatement of the reference's enumerate-and-sort on one host core."""
+    z = np.load(BASIS_QN)
+    n_qn, dim_LL, dim_RR, d, t_sz, t_ele = (int(v) for v in z["header"])
+    q = dict(n_qn=n_qn, target_sz=t_sz, target_ele=t_ele,
+             **{k: z[k] for k in ("sz_LL", "ele_LL", "sz_RR", "ele_RR", "sz_site", "ele_site")})
+    ref = tuple(z[k] for k in ("LL", "LR", "RR", "RL"))
+    got = D.superblock_basis(q)   # warm-up
+    same = all(np.array_equal(a, b) for a, b in zip(got, ref))
+    ts = []
+    for _ in range(max(reps, 5)):
+        t0 = time.perf_counter()
+        D.superblock_basis(q)
+        ts.append(time.perf_counter() - t0)
+    res = {"n": len(ref[0]), "dim_LL": dim_LL, "dim_RR": dim_RR, "dim_onsite": d,
+           "product_space": dim_LL * d * dim_RR * d, "builds_per_s": round(1.0 / float(np.median(ts)), 2),
+           "ms_per_build": round(1e3 * float(np.median(ts)), 3), "equals_reference_basis": same,
+           "note": "quantum numbers dumped from the reference's own run (tests/golden/qn_hubbard_L64_m400_s31.npz)"}
+    if cpu:
+        sys.path.insert(0, os.path.join(REPO, "tests"))
+        import oracle_py as O
+        t0 = time.perf_counter()
+        k = 0
+        while k < 3 or time.perf_counter() - t0 < 2.0:
+            O.superblock_basis(q)
+            k += 1
+        res["cpu_baseline"] = {"builds_per_s": round(k / (time.perf_counter() - t0), 2), "cores": 1,
+                               "kind": "port", "sample": f"{k} builds: the reference's enumeration of the "
+                                                         "product space and quicksort order, restated in C"}
+    return res
+
+
 # ------------------------------------------------------------------ served stream
 SERVE_SNAPSHOTS = ["xxz_L12_m24_jz05_s13.npz", "hubbard_L8_m32_s07.npz", "klm_L6_m32_s04.npz",
                    "klm_L6_m24_ixy_s04.npz", "hubbard_L64_m400_s31.npz"]
@@ -398,7 +437,7 @@ def main():
                     help="SpMV schedule 'lanes,cap,variant' instead of autotuning (bench and --traffic-probe)")
     ap.add_argument("--spmv-sweep", action="store_true",
                     help="time every SpMV schedule on the workload superblock (JSON table), then exit")
-    ap.add_argument("--workload", choices=["superblock", "ed-batch", "serve"], default="superblock",
+    ap.add_argument("--workload", choices=["superblock", "ed-batch", "serve", "basis"], default="superblock",
                     help="superblock: the headline line; ed-batch: 256 seeded ED solves one by one and "
                          "through the batched engine; serve: Poisson request stream through the engine "
                          "(each a separate JSON line)")
@@ -425,6 +464,10 @@ def main():
     if args.workload == "serve":
         print(json.dumps({"metric": "served solves/s and p50/p99 latency", "workload": "serve",
                           "result": run_serve(D, args.rate, args.duration)}), flush=True)
+        return
+    if args.workload == "basis":
+        print(json.dumps({"metric": "superblock basis builds/s", "workload": "basis",
+                          "result": run_basis(D, args.steps, not args.no_cpu_baseline)}), flush=True)
         return
     snap = snapshot.load(WORKLOAD)
     e_ref = float(snap.meta["gs_energy"][0])

@@ -378,6 +378,9 @@ def _bind_superblock(L):
     L.dmrgx_superblock_create_arrays.argtypes = [C.c_int32] * 4 + [vp] * 4 + [C.c_int32, vp, C.c_int32, vp, vp,
                                                                               C.c_int32, vp, C.POINTER(vp)]
     L.dmrgx_superblock_create.argtypes = [C.POINTER(DMRG_BASIS), vp, C.POINTER(vp)]
+    L.dmrgx_superblock_basis.argtypes = [vp, C.POINTER(C.c_int64), vp, vp, vp, vp]
+    L.dmrgx_superblock_create_qn.argtypes = [vp, C.c_int32, vp, C.c_int32, vp, vp, C.c_int32, vp, C.POINTER(vp)]
+    L.dmrgx_superblock_n.argtypes = [vp, C.POINTER(C.c_int32)]
     L.dmrgx_superblock_assemble.argtypes = [vp, C.POINTER(C.c_int64), C.POINTER(vp)]
     L.dmrgx_superblock_download_lower.argtypes = [vp, vp, vp, vp]
     L.dmrgx_superblock_free.argtypes = [vp]
@@ -418,10 +421,40 @@ def lib() -> C.CDLL:  # noqa: F811  (extends the loader with the superblock bind
     return L
 
 
-class Superblock:
-    """Device-resident superblock inputs (basis, operators, term list) from a Snapshot."""
+class BasisQN(C.Structure):
+    """dmrgx_basis_qn: the blocks' quantum numbers a superblock basis is built from."""
+    _fields_ = [(k, C.c_int32) for k in ("n_qn", "dim_LL", "dim_RR", "dim_onsite", "target_sz", "target_ele")] + \
+               [(k, C.c_void_p) for k in ("sz_LL", "ele_LL", "sz_RR", "ele_RR", "sz_site", "ele_site")]
 
-    def __init__(self, snap):
+
+def _basis_qn(q) -> Tuple[BasisQN, list]:
+    """BasisQN from a dict (n_qn, sz_LL, ele_LL, sz_RR, ele_RR, sz_site, ele_site, target_sz,
+    target_ele); returns the struct and the arrays it points to (keep them alive)."""
+    arrs = {k: np.ascontiguousarray(q[k], np.int32) for k in ("sz_LL", "ele_LL", "sz_RR", "ele_RR", "sz_site",
+                                                            "ele_site")}
+    b = BasisQN(int(q["n_qn"]), len(arrs["sz_LL"]), len(arrs["sz_RR"]), len(arrs["sz_site"]), int(q["target_sz"]),
+                int(q["target_ele"]), *[arrs[k].ctypes.data for k in ("sz_LL", "ele_LL", "sz_RR", "ele_RR",
+                                                                         "sz_site", "ele_site")])
+    return b, list(arrs.values())
+
+
+def superblock_basis(q) -> Tuple[np.ndarray, np.ndarray, np.ndarray, np.ndarray]:
+    """dmrgx_superblock_basis: (LL, LR, RR, RL) of the target sector, built on the device."""
+    L = lib()
+    b, keep = _basis_qn(q)
+    n = C.c_int64()
+    check(L.dmrgx_superblock_basis(C.byref(b), C.byref(n), None, None, None, None))
+    out = [np.empty(n.value, np.int16) for _ in range(4)]
+    check(L.dmrgx_superblock_basis(C.byref(b), C.byref(n), *[o.ctypes.data for o in out]))
+    return tuple(out)
+
+
+class Superblock:
+    """Device-resident superblock inputs (basis, operators, term list) from a Snapshot; with
+    qn given, the basis is built on the device from the blocks' quantum numbers instead of
+    taken from the snapshot."""
+
+    def __init__(self, snap, qn=None):
         L = lib()
         self.snap = snap
         self._ops = [HostCRS(o.row, o.col, o.val, o.row_dim) for o in snap.ops]
@@ -435,15 +468,24 @@ class Superblock:
         for i, t in enumerate(snap.terms):
             terms[i] = DMRG_TERM(int(t["kind"]), int(t["op1"]), int(t["op2"]), int(t["ele0"]), int(t["ele1"]),
                                  int(t["ele2"]), int(t["sign_mode"]), 0, float(t["coeff"]))
-        self._basis = [np.ascontiguousarray(a, np.int16) for a in (snap.LL, snap.LR, snap.RR, snap.RL)]
         h = C.c_void_p()
-        check(L.dmrgx_superblock_create_arrays(snap.n, snap.dim_LL, snap.dim_RR, snap.dim_onsite,
-                                               *[a.ctypes.data for a in self._basis], len(self._ops),
-                                               C.cast(ops, C.c_void_p), len(self._eles), C.cast(eptr, C.c_void_p),
-                                               elen.ctypes.data, len(snap.terms), C.cast(terms, C.c_void_p),
-                                               C.byref(h)))
+        if qn is None:
+            self._basis = [np.ascontiguousarray(a, np.int16) for a in (snap.LL, snap.LR, snap.RR, snap.RL)]
+            check(L.dmrgx_superblock_create_arrays(snap.n, snap.dim_LL, snap.dim_RR, snap.dim_onsite,
+                                                   *[a.ctypes.data for a in self._basis], len(self._ops),
+                                                   C.cast(ops, C.c_void_p), len(self._eles),
+                                                   C.cast(eptr, C.c_void_p), elen.ctypes.data, len(snap.terms),
+                                                   C.cast(terms, C.c_void_p), C.byref(h)))
+            self.n = snap.n
+        else:
+            b, self._qn_keep = _basis_qn(qn)
+            check(L.dmrgx_superblock_create_qn(C.byref(b), len(self._ops), C.cast(ops, C.c_void_p), len(self._eles),
+                                               C.cast(eptr, C.c_void_p), elen.ctypes.data, len(snap.terms),
+                                               C.cast(terms, C.c_void_p), C.byref(h)))
+            n = C.c_int32()
+            check(L.dmrgx_superblock_n(h, C.byref(n)))
+            self.n = n.value
         self.handle = h
-        self.n = snap.n
         self.nnz_lower = -1
 
     def assemble(self, full: bool = True) -> Optional[DeviceMatrix]:

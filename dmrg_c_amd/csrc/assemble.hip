@@ -339,19 +339,32 @@ struct ArrOp {
   const double *val;
 };
 
+int64_t build_basis_device(const dmrgx_basis_qn &q, DevBuf<int16_t> &LL, DevBuf<int16_t> &LR, DevBuf<int16_t> &RR,
+                           DevBuf<int16_t> &RL);
+
+// The basis comes from the caller's arrays (LL..RL, host) or, with qn set, is built on the
+// device from the blocks' quantum numbers (n, dim_LL, dim_RR, d are then taken from qn).
 static dmrgx_superblock *create_superblock(int32_t n, int32_t dim_LL, int32_t dim_RR, int32_t d,
                                            const int16_t *LL, const int16_t *LR, const int16_t *RR,
                                            const int16_t *RL, const std::vector<ArrOp> &ops,
                                            const std::vector<std::pair<const int32_t *, int32_t>> &eles,
-                                           const std::vector<DTerm> &terms_in) {
+                                           const std::vector<DTerm> &terms_in, const dmrgx_basis_qn *qn = nullptr) {
+  auto S = std::make_unique<dmrgx_superblock>();
+  if (qn) {
+    dim_LL = qn->dim_LL;
+    dim_RR = qn->dim_RR;
+    d = qn->dim_onsite;
+    n = (int32_t)build_basis_device(*qn, S->LL, S->LR, S->RR, S->RL);
+  }
   if (n <= 0 || dim_LL <= 0 || dim_RR <= 0 || d <= 0) throw Failure(DMRGX_ERR_ARG, "superblock dimensions must be positive");
   if ((int)terms_in.size() > kMaxTerms) throw Failure(DMRGX_ERR_ARG, "too many terms");
-  const int dims[4] = {dim_LL, d, dim_RR, d};
-  const int16_t *Q[4] = {LL, LR, RR, RL};
-  for (int k = 0; k < 4; k++)
-    for (int32_t i = 0; i < n; i++)
-      if (Q[k][i] < 0 || Q[k][i] >= dims[k]) throw Failure(DMRGX_ERR_ARG, "basis index outside its block dimension");
-  auto S = std::make_unique<dmrgx_superblock>();
+  if (!qn) {
+    const int dims[4] = {dim_LL, d, dim_RR, d};
+    const int16_t *Q[4] = {LL, LR, RR, RL};
+    for (int k = 0; k < 4; k++)
+      for (int32_t i = 0; i < n; i++)
+        if (Q[k][i] < 0 || Q[k][i] >= dims[k]) throw Failure(DMRGX_ERR_ARG, "basis index outside its block dimension");
+  }
   S->n = n;
   S->dim_LL = dim_LL;
   S->dim_RR = dim_RR;
@@ -407,10 +420,12 @@ static dmrgx_superblock *create_superblock(int32_t n, int32_t dim_LL, int32_t di
     b.resize(n);
     ctx().copy_in(b.p, h, sizeof(int16_t) * n);
   };
-  up16(S->LL, LL);
-  up16(S->LR, LR);
-  up16(S->RR, RR);
-  up16(S->RL, RL);
+  if (!qn) {
+    up16(S->LL, LL);
+    up16(S->LR, LR);
+    up16(S->RR, RR);
+    up16(S->RL, RL);
+  }
   std::vector<DOp> hops(std::max<size_t>(ops.size(), 1));
   S->op_row.resize(ops.size());
   S->op_col.resize(ops.size());
@@ -798,6 +813,30 @@ int dmrgx_superblock_create_arrays(int32_t n, int32_t dim_LL, int32_t dim_RR, in
                    terms[i].ele2, 0, terms[i].coeff};
     *out = create_superblock(n, dim_LL, dim_RR, d, LL, LR, RR, RL, o, e, t);
   });
+}
+
+int dmrgx_superblock_create_qn(const dmrgx_basis_qn *q, int32_t n_ops, const CRS1 *ops, int32_t n_eles,
+                               const int32_t *const *eles, const int32_t *ele_len, int32_t n_terms,
+                               const dmrgx_term *terms, dmrgx_superblock **out) {
+  if (!q || !out || n_ops < 0 || n_eles < 0 || n_terms < 0) return DMRGX_ERR_ARG;
+  return run_status([&] {
+    std::vector<ArrOp> o(n_ops);
+    for (int i = 0; i < n_ops; i++)
+      o[i] = ArrOp{ops[i].row_dim, ops[i].col_dim, reinterpret_cast<const int64_t *>(ops[i].Row), ops[i].Col, ops[i].Val};
+    std::vector<std::pair<const int32_t *, int32_t>> e(n_eles);
+    for (int i = 0; i < n_eles; i++) e[i] = {eles[i], ele_len[i]};
+    std::vector<DTerm> t(n_terms);
+    for (int i = 0; i < n_terms; i++)
+      t[i] = DTerm{terms[i].kind, terms[i].op1, terms[i].op2, -1, -1, terms[i].sign_mode, terms[i].ele0, terms[i].ele1,
+                   terms[i].ele2, 0, terms[i].coeff};
+    *out = create_superblock(0, 0, 0, 0, nullptr, nullptr, nullptr, nullptr, o, e, t, q);
+  });
+}
+
+int dmrgx_superblock_n(const dmrgx_superblock *S, int32_t *n) {
+  if (!S || !n) return DMRGX_ERR_ARG;
+  *n = S->n;
+  return DMRGX_OK;
 }
 
 int dmrgx_superblock_create(const DMRG_BASIS *B, const dmrgx_terms *T, dmrgx_superblock **out) {

@@ -560,6 +560,29 @@ int dmrgx_superblock_create_arrays(int32_t n, int32_t dim_LL, int32_t dim_RR, in
                                    int32_t n_ops, const CRS1 *ops, int32_t n_eles, const int32_t *const *eles,
                                    const int32_t *ele_len, int32_t n_terms, const dmrgx_term *terms,
                                    dmrgx_superblock **out);
+/* Superblock basis from the blocks' quantum numbers, as GET_BASIS_SUPERBLOCK of each DMRG
+ * program builds it (main/dmrg/hubbard_vf/src/allocations/GET_BASIS_SUPERBLOCK.c:63-97,
+ * xxz_vf/...:36-64, klm_vf/...:66-97, ordered as dmrg/DMRG_QUICK_SORT_BASIS_Q1/Q2.c): the
+ * states (LL, LR, RR, RL) whose S^z (n_qn == 1) or S^z and electron numbers (n_qn == 2)
+ * add up to the target, ordered by electrons of LL+LR, S^z of LL+LR, LL, LR, RR, RL.
+ * sz_LL / ele_LL: Tot_Sz / Tot_Ele of the LL block's states (System->Tot_Sz[LL_site]),
+ * likewise RR; sz_site / ele_site: of the on-site states (FIND_SITE_SZ,
+ * ONSITE_FIND_SITE_SZ/ELE_SZBASIS_*); ele_* are ignored when n_qn == 1.  The target is the
+ * program's (GET_BASIS_SUPERBLOCK.c:24-42 adjusts it per step; pass the adjusted value). */
+typedef struct {
+   int32_t n_qn, dim_LL, dim_RR, dim_onsite, target_sz, target_ele;
+   const int32_t *sz_LL, *ele_LL, *sz_RR, *ele_RR, *sz_site, *ele_site;
+} dmrgx_basis_qn;
+/* *n_out = the sector dimension; with LL..RL given (n_out entries each) also the basis,
+ * built on the device.  No counterpart of the whole-space enumeration and quicksort: the
+ * order is produced directly, sector by sector. */
+int dmrgx_superblock_basis(const dmrgx_basis_qn *q, int64_t *n_out, int16_t *LL, int16_t *LR, int16_t *RR,
+                           int16_t *RL);
+/* dmrgx_superblock_create_arrays with the basis built on the device from q. */
+int dmrgx_superblock_create_qn(const dmrgx_basis_qn *q, int32_t n_ops, const CRS1 *ops, int32_t n_eles,
+                               const int32_t *const *eles, const int32_t *ele_len, int32_t n_terms,
+                               const dmrgx_term *terms, dmrgx_superblock **out);
+int dmrgx_superblock_n(const dmrgx_superblock *S, int32_t *n);
 /* Assemble on the device: the lower-triangle CRS of GET_HAM_LLLRRRRL (kept in the handle,
  * nnz in *nnz_lower) and, if full_out != NULL, its full symmetric expansion ready to solve. */
 int dmrgx_superblock_assemble(dmrgx_superblock *S, int64_t *nnz_lower, dmrgx_matrix **full_out);

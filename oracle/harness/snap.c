@@ -41,6 +41,8 @@ static int g_dump_steps[64];
 static int g_n_dump_steps = 0;
 static int g_dump_ham = 1;          /* include the assembled CRS in the snapshot          */
 static int g_dump_vec = 1;          /* write the GS vector                                */
+static int g_dump_qn = 0;           /* write the basis quantum numbers (qn_<step>.bin)    */
+static int g_dump_snap = 1;         /* write the superblock snapshot (snap_<step>.bin)    */
 static int g_stop_after = -1;       /* exit(0) after this step has been dumped            */
 static long g_time_value = 12345;   /* value returned by time()                           */
 static FILE *g_energy_log = NULL;
@@ -177,12 +179,81 @@ static int find_ptr(void **list, int n, void *p) {
    return -1;
 }
 
+/* --------------------------------------------- superblock basis inputs (QN) */
+/* Quantum numbers GET_BASIS_SUPERBLOCK (main/dmrg/<model>_vf/src/allocations/
+ * GET_BASIS_SUPERBLOCK.c) builds the basis from, and the basis it built:
+ *   "DMRGQN01", n_qn (1: S^z only, XXZ; 2: S^z and electrons), dim_LL, dim_RR, d,
+ *   target_sz, target_ele, Sz_LL[dim_LL], Ele_LL[dim_LL], Sz_RR[dim_RR], Ele_RR[dim_RR],
+ *   Sz_site[d], Ele_site[d] (int32; Ele_* zero when n_qn == 1), n, LL[n], LR[n], RR[n],
+ *   RL[n] (int16). */
+static void dump_qn(int step, DMRG_BASIS *B, BLOCK *System, BLOCK *Enviro, MODEL_T *Model, DMRG_STATUS *St) {
+   char path[4096];
+   snprintf(path, sizeof(path), "%s/qn_%04d.bin", g_outdir, step);
+   FILE *f = fopen(path, "wb");
+   if (!f) { perror(path); exit(2); }
+   const int LLs = St->LL_site, RRs = St->RR_site, d = B->dim_onsite;
+#if defined(SNAP_XXZ)
+   const int n_qn = 1;
+#else
+   const int n_qn = 2;
+#endif
+   fwrite("DMRGQN01", 1, 8, f);
+   w_i32(f, n_qn);
+   w_i32(f, B->dim_LL);
+   w_i32(f, B->dim_RR);
+   w_i32(f, d);
+   w_i32(f, B->tot_sz_LLLRRRRL);
+   w_i32(f, n_qn == 2 ? B->tot_ele_LLLRRRRL : 0);
+   for (int i = 0; i < B->dim_LL; i++) w_i32(f, System->Tot_Sz[LLs][i]);
+   for (int i = 0; i < B->dim_LL; i++) {
+#if defined(SNAP_XXZ)
+      w_i32(f, 0);
+#else
+      w_i32(f, System->Tot_Ele[LLs][i]);
+#endif
+   }
+   for (int i = 0; i < B->dim_RR; i++) w_i32(f, Enviro->Tot_Sz[RRs][i]);
+   for (int i = 0; i < B->dim_RR; i++) {
+#if defined(SNAP_XXZ)
+      w_i32(f, 0);
+#else
+      w_i32(f, Enviro->Tot_Ele[RRs][i]);
+#endif
+   }
+   for (int s = 0; s < d; s++) {
+#if defined(SNAP_XXZ)
+      w_i32(f, FIND_SITE_SZ(s, Model->spin));
+#elif defined(SNAP_HUBBARD)
+      w_i32(f, ONSITE_FIND_SITE_SZ_SZBASIS_HUBBARD(s));
+#else
+      w_i32(f, ONSITE_FIND_SITE_SZ_SZBASIS_KLM(s, Model->spin_loc));
+#endif
+   }
+   for (int s = 0; s < d; s++) {
+#if defined(SNAP_XXZ)
+      w_i32(f, 0);
+#elif defined(SNAP_HUBBARD)
+      w_i32(f, ONSITE_FIND_SITE_ELE_SZBASIS_HUBBARD(s));
+#else
+      w_i32(f, ONSITE_FIND_SITE_ELE_SZBASIS_KLM(s, Model->spin_loc));
+#endif
+   }
+   const int n = B->dim_LLLRRRRL;
+   w_i32(f, n);
+   fwrite(B->LL_LLLRRRRL, 2, (size_t)n, f);
+   fwrite(B->LR_LLLRRRRL, 2, (size_t)n, f);
+   fwrite(B->RR_LLLRRRRL, 2, (size_t)n, f);
+   fwrite(B->RL_LLLRRRRL, 2, (size_t)n, f);
+   fclose(f);
+}
+
 /* ------------------------------------------------------------ GET_HAM hook */
 CRS1 *__real_GET_HAM_LLLRRRRL(DMRG_BASIS *, BLOCK *, BLOCK *, MODEL_T *, DMRG_STATUS *, DMRG_TIME *);
 CRS1 *__wrap_GET_HAM_LLLRRRRL(DMRG_BASIS *Dmrg_Basis, BLOCK *System, BLOCK *Enviro, MODEL_T *Model, DMRG_STATUS *Dmrg_Status, DMRG_TIME *Dmrg_Time) {
    int step = Dmrg_Status->tot_iter_now;
    FILE *f = NULL;
-   if (step_selected(step)) {
+   if (step_selected(step) && g_dump_qn) dump_qn(step, Dmrg_Basis, System, Enviro, Model, Dmrg_Status);
+   if (step_selected(step) && g_dump_snap) {
       char path[4096];
       snprintf(path, sizeof(path), "%s/snap_%04d.bin", g_outdir, step);
       f = fopen(path, "wb");
@@ -399,6 +470,8 @@ int main(int argc, char **argv) {
    g_time_value = (long)argd(argc, argv, "seed", 12345);
    g_dump_ham = (int)argd(argc, argv, "dump_ham", 1);
    g_dump_vec = (int)argd(argc, argv, "dump_vec", 1);
+   g_dump_qn = (int)argd(argc, argv, "dump_qn", 0);
+   g_dump_snap = (int)argd(argc, argv, "dump_snap", 1);
    g_stop_after = (int)argd(argc, argv, "stop_after", -1);
    const char *steps = args(argc, argv, "dump", "");
    if (strcmp(steps, "all") == 0) g_dump_every = 1;

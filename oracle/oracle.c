@@ -768,3 +768,66 @@ void or_ed_v_m(int32_t d, int32_t site, const or_crs *op, int32_t n_out, const i
       y[i] = v * sign;
    }
 }
+
+/* ------------------------------------------------------------ superblock basis */
+/* main/dmrg/hubbard_vf/src/allocations/GET_BASIS_SUPERBLOCK.c:63-97: enumerate the product
+ * space in (LL, LR, RR, RL) order, keep the target sector, then sort with the key order of
+ * dmrg/DMRG_QUICK_SORT_BASIS_Q2.c:29-46 (Q1 for S^z only).  The keys are unique per state,
+ * so any correct sort gives the reference's order. */
+typedef struct {
+   int32_t ele, sz;
+   int16_t LL, LR, RR, RL;
+} or_bstate;
+
+static int or_bstate_cmp(const void *pa, const void *pb) {
+   const or_bstate *a = (const or_bstate *)pa, *b = (const or_bstate *)pb;
+   if (a->ele != b->ele) return a->ele < b->ele ? -1 : 1;
+   if (a->sz != b->sz) return a->sz < b->sz ? -1 : 1;
+   if (a->LL != b->LL) return a->LL < b->LL ? -1 : 1;
+   if (a->LR != b->LR) return a->LR < b->LR ? -1 : 1;
+   if (a->RR != b->RR) return a->RR < b->RR ? -1 : 1;
+   if (a->RL != b->RL) return a->RL < b->RL ? -1 : 1;
+   return 0;
+}
+
+int64_t or_superblock_basis(int32_t n_qn, int32_t dim_LL, const int32_t *sz_LL, const int32_t *ele_LL,
+                            int32_t dim_RR, const int32_t *sz_RR, const int32_t *ele_RR, int32_t d,
+                            const int32_t *sz_site, const int32_t *ele_site, int32_t target_sz,
+                            int32_t target_ele, int16_t *LL, int16_t *LR, int16_t *RR, int16_t *RL) {
+   int64_t n = 0;
+   for (int32_t ll = 0; ll < dim_LL; ll++)
+      for (int32_t lr = 0; lr < d; lr++)
+         for (int32_t rr = 0; rr < dim_RR; rr++)
+            for (int32_t rl = 0; rl < d; rl++) {
+               const int32_t sz = sz_LL[ll] + sz_site[lr] + sz_RR[rr] + sz_site[rl];
+               const int32_t ele = n_qn == 2 ? ele_LL[ll] + ele_site[lr] + ele_RR[rr] + ele_site[rl] : 0;
+               if (sz == target_sz && (n_qn != 2 || ele == target_ele)) n++;
+            }
+   if (!LL || n == 0) return n;
+   or_bstate *st = (or_bstate *)malloc(sizeof(or_bstate) * (size_t)n);
+   int64_t k = 0;
+   for (int32_t ll = 0; ll < dim_LL; ll++)
+      for (int32_t lr = 0; lr < d; lr++)
+         for (int32_t rr = 0; rr < dim_RR; rr++)
+            for (int32_t rl = 0; rl < d; rl++) {
+               const int32_t sz = sz_LL[ll] + sz_site[lr] + sz_RR[rr] + sz_site[rl];
+               const int32_t ele = n_qn == 2 ? ele_LL[ll] + ele_site[lr] + ele_RR[rr] + ele_site[rl] : 0;
+               if (sz != target_sz || (n_qn == 2 && ele != target_ele)) continue;
+               or_bstate *b = &st[k++];
+               b->ele = n_qn == 2 ? ele_LL[ll] + ele_site[lr] : 0;   /* Tot_Ele_LLLR */
+               b->sz = sz_LL[ll] + sz_site[lr];                       /* Tot_Sz_LLLR  */
+               b->LL = (int16_t)ll;
+               b->LR = (int16_t)lr;
+               b->RR = (int16_t)rr;
+               b->RL = (int16_t)rl;
+            }
+   qsort(st, (size_t)n, sizeof(or_bstate), or_bstate_cmp);
+   for (int64_t i = 0; i < n; i++) {
+      LL[i] = st[i].LL;
+      LR[i] = st[i].LR;
+      RR[i] = st[i].RR;
+      RL[i] = st[i].RL;
+   }
+   free(st);
+   return n;
+}

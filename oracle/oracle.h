@@ -161,6 +161,18 @@ void or_ed_v_m(int32_t d, int32_t site, const or_crs *op, int32_t n_out, const i
                int32_t n_in, const int64_t *basis_in, const double *x, int32_t sign_flag,
                const int32_t *n_ele, double *y);
 
+/* Superblock basis (GET_BASIS_SUPERBLOCK, main/dmrg/hubbard_vf/src/allocations/
+ * GET_BASIS_SUPERBLOCK.c:63-97; xxz_vf/...:36-64; klm_vf/...:66-97): every (LL, LR, RR, RL)
+ * whose S^z (and, with n_qn == 2, electron number) add up to the target, enumerated and then
+ * sorted as DMRG_QUICK_SORT_BASIS_Q1/Q2 (dmrg/DMRG_QUICK_SORT_BASIS_Q2.c:29-46) order them:
+ * electrons of LL+LR, S^z of LL+LR, LL, LR, RR, RL, ascending, most significant first.
+ * Returns n; writes the arrays when LL != NULL.  Pinned: the reference's own basis arrays
+ * (tests/golden/qn_*.npz, dumped by oracle/harness/snap.c) and a numpy lexsort. */
+int64_t or_superblock_basis(int32_t n_qn, int32_t dim_LL, const int32_t *sz_LL, const int32_t *ele_LL,
+                            int32_t dim_RR, const int32_t *sz_RR, const int32_t *ele_RR, int32_t d,
+                            const int32_t *sz_site, const int32_t *ele_site, int32_t target_sz,
+                            int32_t target_ele, int16_t *LL, int16_t *LR, int16_t *RR, int16_t *RL);
+
 #ifdef __cplusplus
 }
 #endif

@@ -18,7 +18,7 @@ def pytest_configure(config):
 def golden_snapshots(max_n=None, pattern="*_s*.npz", exclude_big=True):
     out = []
     for p in sorted(glob.glob(os.path.join(GOLDEN, pattern))):
-        if "_expect_" in p or os.path.basename(p).startswith("ed_"):
+        if "_expect_" in p or os.path.basename(p).startswith(("ed_", "qn_")):
             continue
         if exclude_big and "L64" in p:
             continue

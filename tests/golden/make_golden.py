@@ -67,6 +67,15 @@ ED_CASES = {
     "ed_hubbard_L7": "model=hubbard L=7 N=7 sz=1 t=-1 U=4 h_z=0.1",
     "ed_hubbard_L6_ssd": "model=hubbard L=6 N=6 sz=0 t=-1 U=4 BC=SSD",
 }
+# superblock basis construction (GET_BASIS_SUPERBLOCK): the quantum numbers it starts from
+# and the basis it builds, at a few steps per model (qn_<name>_s<step>.npz)
+QN_CASES = {
+    "xxz_L12_m24": ("snap_xxz", "L=12 m=24 sweep=1 J_xy=1.0 J_z=0.5 h_z=0.1", [1, 4, 9, 13]),
+    "xxz_L16_s1_m32": ("snap_xxz", "L=16 m=32 sweep=1 spin=2 tot_sz=2 J_z=0.8", [3, 8, 14]),
+    "hubbard_L8_m32": ("snap_hubbard", "L=8 N=8 m=32 sweep=1 t1=-1 U=4 h_z=0.1", [1, 3, 5, 7]),
+    "hubbard_L12_N10_m48": ("snap_hubbard", "L=12 N=10 tot_sz=0 m=48 sweep=1 t1=-1 U=2", [2, 6, 11]),
+    "klm_L6_m32": ("snap_klm", "L=6 N=4 m=32 sweep=1 t=-1 J=2 h_z=0.1", [1, 3, 5]),
+}
 # bench input (not a parity fixture): Hubbard L=64 N=64 m=400 U=4 h=0.1, last infinite step
 BIG = {
     "hubbard_L64_m400": ("snap_hubbard", "L=64 N=64 m=400 sweep=3 t1=-1 U=4 h_z=0.1", 31),
@@ -107,6 +116,27 @@ def convert(workdir: str, name: str, steps, keep_vec: bool) -> None:
         print("wrote", os.path.relpath(out, REPO), "kind =", e.kind)
 
 
+def convert_qn(workdir: str, name: str, steps) -> None:
+    """qn_<step>.bin (oracle/harness/snap.c: dump_qn) -> qn_<name>_s<step>.npz."""
+    for path in sorted(glob.glob(os.path.join(workdir, "qn_*.bin"))):
+        step = int(os.path.basename(path)[3:7])
+        if step not in steps:
+            continue
+        with open(path, "rb") as f:
+            assert f.read(8) == b"DMRGQN01"
+            n_qn, dim_LL, dim_RR, d, t_sz, t_ele = np.frombuffer(f.read(24), "<i4")
+            rd = lambda k, dt="<i4": np.frombuffer(f.read(k * np.dtype(dt).itemsize), dt).copy()
+            out = dict(header=np.array([n_qn, dim_LL, dim_RR, d, t_sz, t_ele], np.int64),
+                       sz_LL=rd(dim_LL), ele_LL=rd(dim_LL), sz_RR=rd(dim_RR), ele_RR=rd(dim_RR),
+                       sz_site=rd(d), ele_site=rd(d))
+            n = int(rd(1)[0])
+            for k in ("LL", "LR", "RR", "RL"):
+                out[k] = rd(n, "<i2")
+        dst = os.path.join(HERE, f"qn_{name}_s{step:02d}.npz")
+        np.savez_compressed(dst, **out)
+        print("wrote", os.path.relpath(dst, REPO), "n =", n)
+
+
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--only", default=None)
@@ -120,6 +150,13 @@ def main() -> None:
         run(prog, args, wd, f"dump={dump}")
         convert(wd, name, steps, keep_vec)
         shutil.copy(os.path.join(wd, "energies.txt"), os.path.join(HERE, f"{name}_energies.txt"))
+        shutil.rmtree(wd)
+    for name, (prog, args, steps) in QN_CASES.items():
+        if a.only and a.only != "qn_" + name:
+            continue
+        wd = tempfile.mkdtemp(prefix="golden_")
+        run(prog, args, wd, "dump=" + ",".join(str(s) for s in steps) + " dump_qn=1 dump_snap=0 dump_vec=0")
+        convert_qn(wd, name, steps)
         shutil.rmtree(wd)
     for name, (prog, args) in ENERGY_RUNS.items():
         if a.only and a.only != name:
@@ -144,7 +181,8 @@ def main() -> None:
     if a.big:
         for name, (prog, args, step) in BIG.items():
             wd = tempfile.mkdtemp(prefix="golden_")
-            run(prog, args, wd, f"dump={step} dump_ham=0 stop_after={step}")
+            run(prog, args, wd, f"dump={step} dump_ham=0 dump_qn=1 stop_after={step}")
+            convert_qn(wd, name, [step])   # qn_<name>_s<step>.npz: the same step's basis inputs
             snap = snapshot.load_bin(os.path.join(wd, f"snap_{step:04d}.bin"))
             e, err, _ = snapshot.load_gs_bin(os.path.join(wd, f"gs_{step:04d}.bin"))
             out = os.path.join(HERE, f"{name}_s{step:02d}.npz")

@@ -366,3 +366,24 @@ def minimum_residual(op, b, x0, acc=1e-10, max_step=2000):
     x = np.ascontiguousarray(x0, np.float64).copy()
     it = lib().or_minimum_residual(op.n, op.fn, op.ctx, np.ascontiguousarray(b, np.float64), x, acc, max_step)
     return it, x
+
+
+def superblock_basis(q):
+    """Oracle superblock basis (or_superblock_basis) from a dict of quantum numbers:
+    n_qn, sz_LL, ele_LL, sz_RR, ele_RR, sz_site, ele_site, target_sz, target_ele
+    -> (LL, LR, RR, RL) int16 arrays."""
+    L = lib()
+    i32 = lambda a: np.ascontiguousarray(a, np.int32)
+    a = [i32(q[k]) for k in ("sz_LL", "ele_LL", "sz_RR", "ele_RR", "sz_site", "ele_site")]
+    fn = L.or_superblock_basis
+    fn.restype = C.c_int64
+    fn.argtypes = [C.c_int32, C.c_int32, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_int32,
+                   C.c_void_p, C.c_void_p, C.c_int32, C.c_int32] + [C.c_void_p] * 4
+    args = [int(q["n_qn"]), len(a[0]), a[0].ctypes.data, a[1].ctypes.data, len(a[2]), a[2].ctypes.data,
+            a[3].ctypes.data, len(a[4]), a[4].ctypes.data, a[5].ctypes.data, int(q["target_sz"]),
+            int(q["target_ele"])]
+    n = fn(*args, None, None, None, None)
+    out = [np.empty(n, np.int16) for _ in range(4)]
+    if n:
+        fn(*args, *[o.ctypes.data for o in out])
+    return tuple(out)
