@@ -106,6 +106,27 @@ def test_lanczos_slow_same_start_as_oracle(path):
     assert abs(steps - k_o) <= 3
 
 
+@pytest.mark.parametrize("block,lan_type", [(20, b"Normal"), (37, b"Normal"), (800, b"Tight")],
+                         ids=["restart20", "restart37", "tight"])
+@pytest.mark.parametrize("path", [p for p in SNAPS if snapshot.load(p).n >= 1000][:3],
+                         ids=golden_ids([p for p in SNAPS if snapshot.load(p).n >= 1000][:3]))
+def test_lanczos_sym_restarts_and_tight(path, block, lan_type):
+    """LANCZOS_SYM restarting from the Ritz vector every `block` steps (sml/LANCZOS_SYM.c:248-255)
+    and the "Tight" criterion reach the reference's energy; the chunks the driver queues ahead
+    never cross a restart."""
+    s = snapshot.load(path)
+    H = D.HostCRS(s.ham.row, s.ham.col, s.ham.val)
+    vec = O.start_vector(s.n, 5)
+    ev = C.c_double()
+    box = D.BOX_LAN(H.ptr, vec.ctypes.data_as(C.POINTER(C.c_double)), C.pointer(ev), 1e-12, 1, 800, block, 1,
+                    b"Yes", lan_type)
+    D.lib().LANCZOS_SYM(C.byref(box))
+    e_ref = float(s.meta["gs_energy"][0])
+    assert abs(ev.value - e_ref) <= 1e-10 * abs(e_ref)
+    H2 = O.spmv_sym(s.ham.row, s.ham.col, s.ham.val, vec)
+    assert np.abs(H2 - ev.value * vec).max() <= 1e-4 * max(1.0, abs(e_ref))
+
+
 def test_start_vector_matches_oracle():
     n = 1000
     assert np.array_equal(D.start_vector(n, 5), O.start_vector(n, 5))
