@@ -383,8 +383,11 @@ def run_serve(D, rate, duration, seed=0):
     t0 = time.perf_counter()
     i, done = 0, {}
     errs = []
+    host = {"assemble": 0.0, "engine_run": 0.0, "poll": 0.0}
+    wait, service = {}, {}
     while len(done) < len(reqs):
         now = time.perf_counter() - t0
+        ta = time.perf_counter()
         while i < len(reqs) and arrivals[i] <= now:
             kind, what = reqs[i]
             if kind == "ed":
@@ -395,17 +398,25 @@ def run_serve(D, rate, duration, seed=0):
                 A = sbs[what].assemble(full=True)
             E.submit(A, p, tag=i, own=True)
             i += 1
+        tb = time.perf_counter()
         if E.pending > 0:
             E.run(1)
         elif i < len(reqs):
             time.sleep(max(0.0, arrivals[i] - (time.perf_counter() - t0)))
+        tc = time.perf_counter()
         for r in E.poll(want_vec=False):
             if len(done) % 200 == 0:
                 log(f"serve: {len(done)} of {len(reqs)} done, {i} arrived, t={time.perf_counter() - t0:.1f} s")
             done[r.tag] = r.t_done - t0 - arrivals[r.tag]
+            wait[r.tag] = r.t_start - t0 - arrivals[r.tag]
+            service[r.tag] = r.t_done - r.t_start
             kind, what = reqs[r.tag]
             if kind == "sb":
                 errs.append(abs(r.energy - refs[what]) / abs(refs[what]))
+        td = time.perf_counter()
+        host["assemble"] += tb - ta
+        host["engine_run"] += tc - tb
+        host["poll"] += td - tc
     wall = time.perf_counter() - t0
     graphs = E.graph_stats()
     E.free()
@@ -419,6 +430,13 @@ def run_serve(D, rate, duration, seed=0):
             "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 2),
             "max_rel_energy_error_superblocks": float(max(errs)) if errs else None,
             "engine_chunks": graphs,
+            "host_s": {k: round(v, 3) for k, v in host.items()},
+            "latency_split_ms": {
+                part: {kind: {"p50": round(float(np.percentile(v, 50)) * 1e3, 2),
+                              "p99": round(float(np.percentile(v, 99)) * 1e3, 2)}
+                       for kind, v in (("ed", [d[k] for k in d if reqs[k][0] == "ed"]),
+                                       ("superblock", [d[k] for k in d if reqs[k][0] == "sb"])) if v}
+                for part, d in (("until_admitted", wait), ("in_engine", service))},
             "note": "superblock requests drawn from the reference-run snapshots in tests/golden "
                     "(n up to 144542), assembled on the device at arrival"}
 
