@@ -15,6 +15,8 @@ struct dmrgx_ed {
   std::vector<dmrgx::DevBuf<double>> op_val;
   std::vector<dmrgx_ed_term> hterms;
   std::vector<int32_t> hn_ele;
+  std::vector<int64_t> hpw;     // host sources of the uploads, kept until the problem is freed
+  std::vector<char> hops;
   dmrgx::DevBuf<dmrgx_ed_term> terms;
   dmrgx::DevBuf<int32_t> n_ele;
   struct DOp {

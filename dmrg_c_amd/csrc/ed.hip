@@ -297,8 +297,8 @@ std::unique_ptr<dmrgx_ed> ed_setup(int32_t L, int32_t d, int32_t n_ops, const CR
   if (need_parity && !n_ele) throw Failure(DMRGX_ERR_ARG, "parity range given without n_ele");
   if (n_ele) E->hn_ele.assign(n_ele, n_ele + d);
   hipStream_t st = ctx().stream;
-  std::vector<int64_t> pw(L + 1);
-  pw[0] = 1;
+  std::vector<int64_t> &pw = E->hpw;
+  pw.assign(L + 1, 1);
   for (int s = 1; s <= L; s++) pw[s] = pw[s - 1] * d;
   E->pw.resize(L + 1);
   HIPCHK(hipMemcpyAsync(E->pw.p, pw.data(), sizeof(int64_t) * (L + 1), hipMemcpyHostToDevice, st));
@@ -319,14 +319,16 @@ std::unique_ptr<dmrgx_ed> ed_setup(int32_t L, int32_t d, int32_t n_ops, const CR
     hops[i] = DOp{E->op_row[i].p, E->op_col[i].p, E->op_val[i].p};
   }
   E->ops.resize(hops.size());
-  HIPCHK(hipMemcpyAsync(E->ops.p, hops.data(), sizeof(DOp) * hops.size(), hipMemcpyHostToDevice, st));
+  E->hops.assign(reinterpret_cast<const char *>(hops.data()), reinterpret_cast<const char *>(hops.data() + hops.size()));
+  HIPCHK(hipMemcpyAsync(E->ops.p, E->hops.data(), sizeof(DOp) * hops.size(), hipMemcpyHostToDevice, st));
   E->terms.resize(std::max<size_t>(E->hterms.size(), 1));
   if (!E->hterms.empty())
     HIPCHK(hipMemcpyAsync(E->terms.p, E->hterms.data(), sizeof(dmrgx_ed_term) * E->hterms.size(),
                           hipMemcpyHostToDevice, st));
   E->n_ele.resize(d);
   if (n_ele) HIPCHK(hipMemcpyAsync(E->n_ele.p, E->hn_ele.data(), sizeof(int32_t) * d, hipMemcpyHostToDevice, st));
-  HIPCHK(hipStreamSynchronize(st));
+  // no wait: the host arrays copied from live in E until it is freed, and every user of the
+  // device copies is queued behind the copies on this stream
   return E;
 }
 
@@ -371,7 +373,7 @@ void ed_enumerate(dmrgx_ed &E, const int32_t *qn1, int32_t t1, const int32_t *qn
   E.basis.resize(n);
   k_sector_fill<<<(unsigned)tiles, kTileThreads, 0, st>>>(q, off.p, E.basis.p);
   HIPCHK(hipGetLastError());
-  HIPCHK(hipStreamSynchronize(st));
+  // no wait: the basis's users are queued behind the fill on this stream
 }
 
 std::unique_ptr<dmrgx_matrix> ed_assemble(const dmrgx_ed &E) {
@@ -385,32 +387,40 @@ std::unique_ptr<dmrgx_matrix> ed_assemble(const dmrgx_ed &E) {
   k_ed_assemble<kCapSmall, false><<<n, 64, 0, st>>>(v, nullptr, n, counts.p, nullptr, nullptr, nullptr, over.p,
                                                      n_over.p);
   HIPCHK(hipGetLastError());
-  int h[2] = {0, 0};
-  HIPCHK(hipMemcpyAsync(&h[0], n_over.p, sizeof(int), hipMemcpyDeviceToHost, st));
-  HIPCHK(hipStreamSynchronize(st));
-  if (h[0] > 0) {
-    k_ed_assemble<kCapLarge, false><<<h[0], 64, 0, st>>>(v, over.p, h[0], counts.p, nullptr, nullptr, nullptr,
-                                                         nullptr, n_over.p + 1);
-    HIPCHK(hipGetLastError());
-    HIPCHK(hipMemcpyAsync(&h[1], n_over.p + 1, sizeof(int), hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
-    if (h[1] > 0) throw Failure(DMRGX_ERR_ARG, "an ED row has more than 4096 contributions");
-  }
+  // row pointers right away: they hold unless a row overflowed (then they are redone), so
+  // the common case reads the overflow count and nnz in one round trip
   auto A = std::make_unique<dmrgx_matrix>();
   A->n = n;
   A->row.resize((size_t)n + 1);
   launch_exclusive_scan(n, counts.p, A->row.p, st);
-  HIPCHK(hipMemcpyAsync(&A->nnz, A->row.p + n, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+  int64_t *hb = reinterpret_cast<int64_t *>(ctx().host_stage(4));
+  int *h = reinterpret_cast<int *>(hb + 1);
+  HIPCHK(hipMemcpyAsync(hb, A->row.p + n, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipMemcpyAsync(h, n_over.p, sizeof(int), hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
+  A->nnz = hb[0];
+  const int n_large = h[0];
+  if (n_large > 0) {
+    k_ed_assemble<kCapLarge, false><<<n_large, 64, 0, st>>>(v, over.p, n_large, counts.p, nullptr, nullptr, nullptr,
+                                                            nullptr, n_over.p + 1);
+    HIPCHK(hipGetLastError());
+    launch_exclusive_scan(n, counts.p, A->row.p, st);
+    HIPCHK(hipMemcpyAsync(h + 1, n_over.p + 1, sizeof(int), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(hb, A->row.p + n, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    if (h[1] > 0) throw Failure(DMRGX_ERR_ARG, "an ED row has more than 4096 contributions");
+    A->nnz = hb[0];
+  }
   A->col.resize(std::max<int64_t>(A->nnz, 1));
   A->val.resize(std::max<int64_t>(A->nnz, 1));
   k_ed_assemble<kCapSmall, true><<<n, 64, 0, st>>>(v, nullptr, n, nullptr, A->row.p, A->col.p, A->val.p, nullptr,
                                                     nullptr);
-  if (h[0] > 0)
-    k_ed_assemble<kCapLarge, true><<<h[0], 64, 0, st>>>(v, over.p, h[0], nullptr, A->row.p, A->col.p, A->val.p,
-                                                        nullptr, nullptr);
+  if (n_large > 0)
+    k_ed_assemble<kCapLarge, true><<<n_large, 64, 0, st>>>(v, over.p, n_large, nullptr, A->row.p, A->col.p,
+                                                           A->val.p, nullptr, nullptr);
   HIPCHK(hipGetLastError());
-  HIPCHK(hipStreamSynchronize(st));
+  // no wait: the matrix's users are queued behind the fill on this stream (and the scratch
+  // released here is only reused in stream order)
   default_schedule(*A);
   return A;
 }
