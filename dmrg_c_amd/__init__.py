@@ -9,7 +9,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
-from typing import Optional, Tuple
+from typing import List, Optional, Tuple
 
 import numpy as np
 
@@ -447,6 +447,71 @@ def superblock_basis(q) -> Tuple[np.ndarray, np.ndarray, np.ndarray, np.ndarray]
     out = [np.empty(n.value, np.int16) for _ in range(4)]
     check(L.dmrgx_superblock_basis(C.byref(b), C.byref(n), *[o.ctypes.data for o in out]))
     return tuple(out)
+
+
+def _host_crs1(row, col, val) -> Tuple[CRS1, list]:
+    """CRS1 over numpy arrays (kept alive by the returned list)."""
+    r = np.ascontiguousarray(row, np.int64)
+    c = np.ascontiguousarray(col, np.int32)
+    v = np.ascontiguousarray(val, np.float64)
+    m = CRS1(len(v), len(r) - 1, len(r) - 1, len(r) - 1, c.ctypes.data_as(C.POINTER(C.c_int)),
+             r.ctypes.data_as(C.POINTER(C.c_long)), v.ctypes.data_as(C.POINTER(C.c_double)))
+    return m, [r, c, v]
+
+
+class WholeBasis:
+    """dmrgx_whole_basis: the whole-space inverse table of DMRG_WHOLE_BASIS_Q1..Q4 on the
+    device plus the sectors registered with add_sector(); v_m() is DMRG_V_M_*_Q1..Q4."""
+
+    def __init__(self, dim_LL: int, dim_RR: int, dim_onsite: int, inv: np.ndarray):
+        L = lib()
+        if not getattr(L, "_wb_bound", False):
+            vp = C.c_void_p
+            L.dmrgx_whole_basis_create.argtypes = [C.c_int32, C.c_int32, C.c_int32, vp, C.POINTER(vp)]
+            L.dmrgx_whole_basis_add_sector.argtypes = [vp, C.c_int32, vp, vp, vp, vp, C.POINTER(C.c_int32)]
+            L.dmrgx_v_m_sector.argtypes = [vp, C.c_int32, C.POINTER(CRS1), C.c_int32, C.c_int32, vp, vp, vp, vp,
+                                           C.c_int32, vp]
+            L.dmrgx_whole_basis_free.argtypes = [vp]
+            L._wb_bound = True
+        inv = np.ascontiguousarray(inv, np.int32)
+        if inv.size != dim_LL * dim_onsite * dim_RR * dim_onsite:
+            raise ValueError("inverse table size is not dim_LL * d * dim_RR * d")
+        h = C.c_void_p()
+        check(L.dmrgx_whole_basis_create(dim_LL, dim_RR, dim_onsite, inv.ctypes.data, C.byref(h)))
+        self.handle = h
+        self.dims = (dim_LL, dim_onsite, dim_RR, dim_onsite)
+        self.sector_dims: List[int] = []
+
+    def add_sector(self, LL, LR, RR, RL) -> int:
+        arrs = [np.ascontiguousarray(a, np.int16) for a in (LL, LR, RR, RL)]
+        sid = C.c_int32()
+        check(lib().dmrgx_whole_basis_add_sector(self.handle, len(arrs[0]), *[a.ctypes.data for a in arrs],
+                                                 C.byref(sid)))
+        self.sector_dims.append(len(arrs[0]))
+        return sid.value
+
+    def v_m(self, slot: int, op, sector_out: int, x: np.ndarray, ele_LL=None, ele_LR=None, ele_RR=None,
+            sign: bool = False) -> np.ndarray:
+        """op = (row, col, val) of the slot operator; x: the input vector (its sector's length)."""
+        m, keep = _host_crs1(*op)
+        x = np.ascontiguousarray(x, np.float64)
+        eles = [None if e is None else np.ascontiguousarray(e, np.int32) for e in (ele_LL, ele_LR, ele_RR)]
+        out = np.empty(self.sector_dims[sector_out] if 0 <= sector_out < len(self.sector_dims) else 0)
+        check(lib().dmrgx_v_m_sector(self.handle, slot, C.byref(m), sector_out, len(x), x.ctypes.data,
+                                     *[None if e is None else e.ctypes.data for e in eles], 1 if sign else 0,
+                                     out.ctypes.data))
+        return out
+
+    def free(self):
+        if getattr(self, "handle", None):
+            lib().dmrgx_whole_basis_free(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
 
 
 class Superblock:

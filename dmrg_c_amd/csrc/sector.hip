@@ -1,0 +1,184 @@
+// Single-slot products from the state's sector into another sector:
+// DMRG_V_M_{LL,LR,RR,RL}_Q1..Q4 (include/dmrg.h:369-382; dmrg/DMRG_V_M_LL_Q1.c:8-42,
+// DMRG_V_M_LR_Q2.c:8-55, DMRG_V_M_RR_Q2.c:8-55, DMRG_V_M_RL_Q2.c:8-55 and the Q1/Q3/Q4
+// siblings, which differ only in how the output sector is indexed).
+//
+//   Out[b] = sign_b * sum_{j in row q_slot(b) of M} V[Inv(q(b) with q_slot = col_j)] * M_j
+//
+// over the output sector's states b = (LL, LR, RR, RL), entries whose Inv is -1 skipped;
+// sign_b = (-1)^(electrons of the slots before the acted one, in the order LL, LR, RR, RL)
+// when the sign flag is set.  The reference's sector arrays and the whole-space inverse
+// table carry neither the table's length nor the input vector's, so the table and the
+// sectors are handed over once, with their sizes, to a dmrgx_whole_basis, and each product
+// names its output sector by the id it got.  One thread per output state adds its row's
+// entries in the reference's order, so the result is bit-identical to the reference loop.
+#include <hip/hip_runtime.h>
+
+#include <memory>
+#include <vector>
+
+#include "entry.h"
+
+namespace dmrgx {
+namespace {
+
+struct SectorDev {
+  int32_t n;
+  DevBuf<int16_t> LL, LR, RR, RL;
+};
+
+__global__ __launch_bounds__(256) void k_v_m_sector(int32_t n_out, int32_t slot, int32_t dim_RR, int32_t d,
+                                                    const int16_t *__restrict__ LL, const int16_t *__restrict__ LR,
+                                                    const int16_t *__restrict__ RR, const int16_t *__restrict__ RL,
+                                                    const int32_t *__restrict__ inv, const int64_t *__restrict__ row,
+                                                    const int32_t *__restrict__ col, const double *__restrict__ val,
+                                                    const int32_t *__restrict__ ele_LL,
+                                                    const int32_t *__restrict__ ele_LR,
+                                                    const int32_t *__restrict__ ele_RR, int32_t n_in,
+                                                    const double *__restrict__ x, double *__restrict__ y,
+                                                    int32_t *__restrict__ bad) {
+  const int32_t b = blockIdx.x * 256 + threadIdx.x;
+  if (b >= n_out) return;
+  int q[4] = {LL[b], LR[b], RR[b], RL[b]};
+  double sgn = 1.0;
+  if (ele_LL) {
+    int32_t e = 0;
+    if (slot >= 1) e += ele_LL[q[0]];
+    if (slot >= 2) e += ele_LR[q[1]];
+    if (slot >= 3) e += ele_RR[q[2]];
+    if (e % 2 != 0) sgn = -1.0;
+  }
+  const int own = q[slot];
+  double s = 0.0;
+  bool out_of_range = false;
+  for (int64_t j = row[own]; j < row[own + 1]; j++) {
+    q[slot] = col[j];
+    const int32_t iv = inv[(((int64_t)q[0] * d + q[1]) * dim_RR + q[2]) * d + q[3]];
+    if (iv >= n_in) {
+      out_of_range = true;
+    } else if (iv >= 0) {
+      s += sgn * x[iv] * val[j];
+    }
+  }
+  y[b] = s;
+  if (out_of_range) *bad = 1;
+}
+
+}  // namespace
+}  // namespace dmrgx
+
+using namespace dmrgx;
+
+struct dmrgx_whole_basis {
+  int32_t dim_LL, dim_RR, d;
+  int64_t dim_whole;
+  DevBuf<int32_t> inv;
+  std::vector<std::unique_ptr<SectorDev>> sectors;
+};
+
+extern "C" {
+
+int dmrgx_whole_basis_create(int32_t dim_LL, int32_t dim_RR, int32_t dim_onsite, const int32_t *inv,
+                             dmrgx_whole_basis **out) {
+  if (!out || !inv) return DMRGX_ERR_ARG;
+  *out = nullptr;
+  return run_status([&] {
+    if (dim_LL <= 0 || dim_RR <= 0 || dim_onsite <= 0 || dim_LL > 32767 || dim_RR > 32767 || dim_onsite > 32767)
+      throw Failure(DMRGX_ERR_ARG, "block dimensions must lie in 1..32767");
+    auto W = std::make_unique<dmrgx_whole_basis>();
+    W->dim_LL = dim_LL;
+    W->dim_RR = dim_RR;
+    W->d = dim_onsite;
+    W->dim_whole = (int64_t)dim_LL * dim_onsite * dim_RR * dim_onsite;
+    for (int64_t i = 0; i < W->dim_whole; i++)
+      if (inv[i] < -1) throw Failure(DMRGX_ERR_ARG, "inverse table entry below -1");
+    W->inv.resize(W->dim_whole);
+    ctx().copy_in(W->inv.p, inv, sizeof(int32_t) * W->dim_whole);
+    *out = W.release();
+  });
+}
+
+int dmrgx_whole_basis_add_sector(dmrgx_whole_basis *W, int32_t dim, const int16_t *LL, const int16_t *LR,
+                                 const int16_t *RR, const int16_t *RL, int32_t *id) {
+  if (!W || !id || dim < 0 || (dim > 0 && !(LL && LR && RR && RL))) return DMRGX_ERR_ARG;
+  return run_status([&] {
+    const int32_t dims[4] = {W->dim_LL, W->d, W->dim_RR, W->d};
+    const int16_t *Q[4] = {LL, LR, RR, RL};
+    for (int k = 0; k < 4; k++)
+      for (int32_t i = 0; i < dim; i++)
+        if (Q[k][i] < 0 || Q[k][i] >= dims[k]) throw Failure(DMRGX_ERR_ARG, "sector state outside its block dimension");
+    auto S = std::make_unique<SectorDev>();
+    S->n = dim;
+    DevBuf<int16_t> *bufs[4] = {&S->LL, &S->LR, &S->RR, &S->RL};
+    for (int k = 0; k < 4; k++) {
+      bufs[k]->resize(dim > 0 ? dim : 1);
+      if (dim > 0) ctx().copy_in(bufs[k]->p, Q[k], sizeof(int16_t) * dim);
+    }
+    *id = (int32_t)W->sectors.size();
+    W->sectors.push_back(std::move(S));
+  });
+}
+
+int dmrgx_v_m_sector(dmrgx_whole_basis *W, int32_t slot, const CRS1 *M, int32_t sector_out, int32_t dim_in,
+                     const double *V, const int32_t *ele_LL, const int32_t *ele_LR, const int32_t *ele_RR,
+                     int32_t sign_flag, double *Out_V) {
+  if (!W || !M || slot < 0 || slot > 3 || dim_in < 0 || (dim_in > 0 && !V)) return DMRGX_ERR_ARG;
+  return run_status([&] {
+    if (sector_out < 0 || sector_out >= (int32_t)W->sectors.size())
+      throw Failure(DMRGX_ERR_ARG, "no such output sector");
+    const SectorDev &S = *W->sectors[sector_out];
+    if (S.n > 0 && !Out_V) throw Failure(DMRGX_ERR_ARG, "output vector missing");
+    const int32_t slot_dim = slot == 0 ? W->dim_LL : slot == 2 ? W->dim_RR : W->d;
+    if (!M->Row || M->row_dim < slot_dim) throw Failure(DMRGX_ERR_ARG, "operator has fewer rows than its block");
+    const int64_t nnz = M->Row[slot_dim];
+    if (nnz < 0) throw Failure(DMRGX_ERR_ARG, "operator row pointers negative");
+    for (int32_t r = 0; r < slot_dim; r++)
+      if (M->Row[r] < 0 || M->Row[r] > M->Row[r + 1]) throw Failure(DMRGX_ERR_ARG, "operator row pointers not ascending");
+    for (int64_t j = 0; j < nnz; j++)
+      if (M->Col[j] < 0 || M->Col[j] >= slot_dim) throw Failure(DMRGX_ERR_ARG, "operator column outside its block");
+    const bool sign = sign_flag != 0 && slot >= 1;
+    if (sign && (!ele_LL || (slot >= 2 && !ele_LR) || (slot >= 3 && !ele_RR)))
+      throw Failure(DMRGX_ERR_ARG, "electron numbers missing for the sign");
+    if (S.n == 0) return;
+    Context &C = ctx();
+    hipStream_t st = C.stream;
+    DevBuf<int64_t> row(slot_dim + 1);
+    DevBuf<int32_t> col(nnz > 0 ? nnz : 1);
+    DevBuf<double> val(nnz > 0 ? nnz : 1), x(dim_in > 0 ? dim_in : 1), y(S.n);
+    C.copy_in(row.p, M->Row, sizeof(int64_t) * (slot_dim + 1));
+    if (nnz > 0) {
+      C.copy_in(col.p, M->Col, sizeof(int32_t) * nnz);
+      C.copy_in(val.p, M->Val, sizeof(double) * nnz);
+    }
+    if (dim_in > 0) C.copy_in(x.p, V, sizeof(double) * dim_in);
+    DevBuf<int32_t> eL, eLR, eR, bad(1);
+    if (sign) {
+      eL.resize(W->dim_LL);
+      C.copy_in(eL.p, ele_LL, sizeof(int32_t) * W->dim_LL);
+      if (slot >= 2) {
+        eLR.resize(W->d);
+        C.copy_in(eLR.p, ele_LR, sizeof(int32_t) * W->d);
+      }
+      if (slot >= 3) {
+        eR.resize(W->dim_RR);
+        C.copy_in(eR.p, ele_RR, sizeof(int32_t) * W->dim_RR);
+      }
+    }
+    HIPCHK(hipMemsetAsync(bad.p, 0, sizeof(int32_t), st));
+    k_v_m_sector<<<(unsigned)((S.n + 255) / 256), 256, 0, st>>>(
+        S.n, slot, W->dim_RR, W->d, S.LL.p, S.LR.p, S.RR.p, S.RL.p, W->inv.p, row.p, col.p, val.p,
+        sign ? eL.p : nullptr, eLR.p, eR.p, dim_in, x.p, y.p, bad.p);
+    HIPCHK(hipGetLastError());
+    int32_t hbad = 0;
+    C.copy_out(&hbad, bad.p, sizeof(int32_t));
+    if (hbad) throw Failure(DMRGX_ERR_ARG, "the operator reaches a state whose index lies outside the input vector");
+    C.copy_out(Out_V, y.p, sizeof(double) * S.n);
+  });
+}
+
+void dmrgx_whole_basis_free(dmrgx_whole_basis *W) {
+  std::lock_guard<std::recursive_mutex> lk(api_mutex());
+  delete W;
+}
+
+}  // extern "C"

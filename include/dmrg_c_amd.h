@@ -594,6 +594,33 @@ void dmrgx_superblock_free(dmrgx_superblock *S);
  * DMRG_DIAGONALIZE_SUPERBLOCK_SYM call on that CRS1 (no re-upload).  exit(1) on error. */
 CRS1 *dmrgx_get_ham_lllrrrrl(const DMRG_BASIS *B, const dmrgx_terms *T);
 
+/* ---- single-slot products between sectors: DMRG_V_M_{LL,LR,RR,RL}_Q1..Q4
+ * (include/dmrg.h:369-382, dmrg/DMRG_V_M_*_Q1..Q4.c).  The reference passes a
+ * DMRG_WHOLE_BASIS_Q1..Q4 (include/dmrg.h:140-200): per sector the states' LL/LR/RR/RL
+ * arrays and one whole-space inverse table Inv_LLLRRRRL (dim_LL*d*dim_RR*d entries, each
+ * -1 or the state's index within its own sector).  Neither the table's length nor the
+ * input vector's is in those structs, so here the table goes to the device once, with its
+ * block sizes, the sectors are registered one by one (the caller keeps the map from its
+ * quantum-number indices qn1_out.. to the ids), and each product names its output sector
+ * by id and gives the input vector's length.  Results are bit-identical to the reference's
+ * loop.  An operator that reaches a state whose Inv lies at or past dim_in is an error
+ * (the reference would read past V). */
+typedef struct dmrgx_whole_basis dmrgx_whole_basis;
+int dmrgx_whole_basis_create(int32_t dim_LL, int32_t dim_RR, int32_t dim_onsite, const int32_t *inv,
+                             dmrgx_whole_basis **out);
+/* One sector's states (LL_LLLRRRRL[qn..] etc., dim entries each); its id in *id (0, 1, ...). */
+int dmrgx_whole_basis_add_sector(dmrgx_whole_basis *W, int32_t dim, const int16_t *LL, const int16_t *LR,
+                                 const int16_t *RR, const int16_t *RL, int32_t *id);
+/* Out_V (the output sector's dimension) = M acting on slot (0 LL, 1 LR, 2 RR, 3 RL) of V
+ * (dim_in entries).  sign_flag != 0 is Sign_Flag "Yes": (-1)^(Ele_LL[LL]) for LR,
+ * (-1)^(Ele_LL[LL] + Ele_LR[LR]) for RR, (-1)^(Ele_LL[LL] + Ele_LR[LR] + Ele_RR[RR]) for
+ * RL (ele_LL: dim_LL entries, ele_LR: d, ele_RR: dim_RR; the ones a slot does not use may
+ * be NULL); the LL product has no sign, as DMRG_V_M_LL_Q* has no Sign_Flag. */
+int dmrgx_v_m_sector(dmrgx_whole_basis *W, int32_t slot, const CRS1 *M, int32_t sector_out, int32_t dim_in,
+                     const double *V, const int32_t *ele_LL, const int32_t *ele_LR, const int32_t *ele_RR,
+                     int32_t sign_flag, double *Out_V);
+void dmrgx_whole_basis_free(dmrgx_whole_basis *W);
+
 /* ---- exact diagonalization on the device
  * A problem is a sorted basis of base-d codes (site s is digit s, site 0 least
  * significant: exact/EXACT_FIND_SITE_STATE.c), on-site operators (d x d CRS) and a term

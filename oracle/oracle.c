@@ -612,6 +612,30 @@ void or_v_m_q0(int slot, int32_t n, int32_t dim_RR, int32_t d, const int16_t *LL
    }
 }
 
+void or_v_m_sector(int slot, int32_t n_out, int32_t dim_RR, int32_t d, const int16_t *LL, const int16_t *LR,
+                   const int16_t *RR, const int16_t *RL, const int32_t *inv_whole, const int64_t *row,
+                   const int32_t *col, const double *val, const int32_t *ele_LL, const int32_t *ele_LR,
+                   const int32_t *ele_RR, int32_t sign_flag, const double *x, double *y) {
+   for (int32_t b = 0; b < n_out; b++) {
+      int q[4] = {LL[b], LR[b], RR[b], RL[b]};
+      int sign = 1;
+      if (sign_flag && slot > 0) {   /* DMRG_V_M_RL_Q2.c:30-39: parity of the earlier slots */
+         int e = ele_LL[q[0]];
+         if (slot >= 2) e += ele_LR[q[1]];
+         if (slot >= 3) e += ele_RR[q[2]];
+         sign = (e % 2 == 0) ? 1 : -1;
+      }
+      const int own = q[slot];
+      double v = 0.0;
+      for (int64_t j = row[own]; j < row[own + 1]; j++) {
+         q[slot] = col[j];
+         int32_t inv = inv_whole[sb_index(q, dim_RR, d)];
+         if (inv >= 0) v = v + sign * x[inv] * val[j];   /* DMRG_V_M_RL_Q2.c:44-49 */
+      }
+      y[b] = v;
+   }
+}
+
 /* ------------------------------------------------------ exact diagonalization */
 /* sml/BINARY_SEARCH_LINT1.c: index of target in the ascending array, -1 when absent */
 static int64_t bsearch_i64(const int64_t *a, int64_t lo, int64_t hi, int64_t target) {

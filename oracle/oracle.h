@@ -128,6 +128,18 @@ void or_v_m_q0(int slot, int32_t n, int32_t dim_RR, int32_t d, const int16_t *LL
                const int16_t *RR, const int16_t *RL, const int32_t *inv_whole, const int64_t *row,
                const int32_t *col, const double *val, const double *x, double *y);
 
+/* DMRG_V_M_{LL,LR,RR,RL}_Q1..Q4 (dmrg/DMRG_V_M_LL_Q1.c:23-40, DMRG_V_M_LR_Q2.c:24-53,
+ * DMRG_V_M_RR_Q2.c:24-53, DMRG_V_M_RL_Q2.c:24-53; Q1/Q3/Q4 differ only in how the output
+ * sector is indexed): y[b] over the n_out states of the output sector, entries whose Inv is
+ * -1 skipped, times (-1)^(electrons of the slots before the acted one) when sign_flag
+ * (never for LL).  Pinned: the reference's own DMRG_V_M_*_Q1 and _Q2 (oracle/_ref) on
+ * the same whole basis, and a dense product on the full product space restricted to the
+ * sectors (tests/test_v_m_sector.py). */
+void or_v_m_sector(int slot, int32_t n_out, int32_t dim_RR, int32_t d, const int16_t *LL, const int16_t *LR,
+                   const int16_t *RR, const int16_t *RL, const int32_t *inv_whole, const int64_t *row,
+                   const int32_t *col, const double *val, const int32_t *ele_LL, const int32_t *ele_LR,
+                   const int32_t *ele_RR, int32_t sign_flag, const double *x, double *y);
+
 /* ------------------------------------------------------------ exact diagonalization */
 /* One term of an ED Hamiltonian row (layout identical to dmrgx_ed_term):
  *   kind 0: coeff * O1 on site1               (exact/EXACT_MAKE_ELEM_ON.c:7-31)
