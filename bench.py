@@ -345,6 +345,81 @@ def run_basis(D, reps: int, cpu: bool):
     return res
 
 
+def run_v_m(D, reps: int, cpu: bool):
+    """Single-slot product into a neighbouring sector (DMRG_V_M_LL_Q2 form: an operator on
+    LL that raises S^z by one unit) on the Hubbard L=64 m=400 step: whole product space
+    396*4*396*4, input the n=144542 target sector, output its S^z+2 sector.  The whole
+    basis (sectors keyed by electrons and S^z, enumeration order, the target in the
+    reference's order) is built from the quantum numbers of the reference's own run; the
+    operator is a seeded random matrix with the S^+ sparsity pattern.  Times
+    dmrgx_v_m_sector per call (operator and vectors from and to host memory, as the
+    reference's signature has them) and the oracle's restatement on one host core."""
+    z = np.load(BASIS_QN)
+    n_qn, dim_LL, dim_RR, d, t_sz, t_ele = (int(v) for v in z["header"])
+    qn = {k: z[k].astype(np.int64) for k in ("sz_LL", "ele_LL", "sz_RR", "ele_RR", "sz_site", "ele_site")}
+    dims = (dim_LL, d, dim_RR, d)
+    LL, LR, RR, RL = (a.ravel() for a in np.meshgrid(*[np.arange(k) for k in dims], indexing="ij"))
+    ele = qn["ele_LL"][LL] + qn["ele_site"][LR] + qn["ele_RR"][RR] + qn["ele_site"][RL]
+    sz = qn["sz_LL"][LL] + qn["sz_site"][LR] + qn["sz_RR"][RR] + qn["sz_site"][RL]
+    inv = np.full(LL.size, -1, np.int32)
+    sec = {}
+    for key in ((t_ele, t_sz), (t_ele, t_sz + 2)):
+        if key == (t_ele, t_sz):
+            flat = np.ravel_multi_index(tuple(np.asarray(z[k], np.int64) for k in ("LL", "LR", "RR", "RL")), dims)
+        else:
+            flat = np.nonzero((ele == key[0]) & (sz == key[1]))[0]
+        inv[flat] = np.arange(flat.size, dtype=np.int32)
+        sec[key] = [a[flat].astype(np.int16) for a in (LL, LR, RR, RL)]
+    rng = np.random.default_rng(5)
+    e, s = qn["ele_LL"], qn["sz_LL"]
+    mask = (e[:, None] == e[None, :]) & (s[:, None] == s[None, :] + 2) & (rng.random((dim_LL, dim_LL)) < 0.5)
+    dense = np.where(mask, rng.uniform(-1, 1, mask.shape), 0.0)
+    import scipy.sparse as sp
+    m = sp.csr_matrix(dense)
+    op = (m.indptr.astype(np.int64), m.indices.astype(np.int32), m.data.copy())
+    n_in, n_out = sec[(t_ele, t_sz)][0].size, sec[(t_ele, t_sz + 2)][0].size
+    x = rng.uniform(-1, 1, n_in)
+    W = D.WholeBasis(dim_LL, dim_RR, d, inv)
+    try:
+        sid = W.add_sector(*sec[(t_ele, t_sz + 2)])
+        y = W.v_m(0, op, sid, x)
+        ts = []
+        for _ in range(max(reps, 10)):
+            t0 = time.perf_counter()
+            W.v_m(0, op, sid, x)
+            ts.append(time.perf_counter() - t0)
+    finally:
+        W.free()
+    ob = sec[(t_ele, t_sz + 2)]
+    entries = int(sum(op[0][r + 1] - op[0][r] for r in ob[0]))
+    res = {"n_in": n_in, "n_out": n_out, "product_space": int(LL.size), "operator_nnz": len(op[2]),
+           "entries_gathered": entries, "products_per_s": round(1.0 / float(np.median(ts)), 2),
+           "ms_per_product": round(1e3 * float(np.median(ts)), 3),
+           "note": "host-inclusive: operator upload, input vector up, output vector down; "
+                   "quantum numbers and target basis of the reference's own run"}
+    if cpu:
+        sys.path.insert(0, os.path.join(REPO, "tests"))
+        import ctypes as C
+        import oracle_py as O
+        fn = O.lib().or_v_m_sector
+        fn.restype = None
+        fn.argtypes = [C.c_int, C.c_int32, C.c_int32, C.c_int32] + [C.c_void_p] * 11 + [C.c_int32, C.c_void_p,
+                                                                                          C.c_void_p]
+        eles = [np.ascontiguousarray(qn[k], np.int32) for k in ("ele_LL", "ele_site", "ele_RR")]
+        yc = np.empty(n_out)
+        args = [0, n_out, dim_RR, d, *[a.ctypes.data for a in ob], inv.ctypes.data,
+                *[a.ctypes.data for a in op], *[a.ctypes.data for a in eles], 0, x.ctypes.data, yc.ctypes.data]
+        t0 = time.perf_counter()
+        k = 0
+        while k < 3 or time.perf_counter() - t0 < 2.0:
+            fn(*args)
+            k += 1
+        res["equals_oracle"] = bool(np.array_equal(y, yc))
+        res["cpu_baseline"] = {"products_per_s": round(k / (time.perf_counter() - t0), 2), "cores": 1,
+                               "kind": "port", "sample": f"{k} products: the reference's loop restated in C"}
+    return res
+
+
 # ------------------------------------------------------------------ served stream
 SERVE_SNAPSHOTS = ["xxz_L12_m24_jz05_s13.npz", "hubbard_L8_m32_s07.npz", "klm_L6_m32_s04.npz",
                    "klm_L6_m24_ixy_s04.npz", "hubbard_L64_m400_s31.npz"]
@@ -455,10 +530,11 @@ def main():
                     help="SpMV schedule 'lanes,cap,variant' instead of autotuning (bench and --traffic-probe)")
     ap.add_argument("--spmv-sweep", action="store_true",
                     help="time every SpMV schedule on the workload superblock (JSON table), then exit")
-    ap.add_argument("--workload", choices=["superblock", "ed-batch", "serve", "basis"], default="superblock",
+    ap.add_argument("--workload", choices=["superblock", "ed-batch", "serve", "basis", "v-m"], default="superblock",
                     help="superblock: the headline line; ed-batch: 256 seeded ED solves one by one and "
-                         "through the batched engine; serve: Poisson request stream through the engine "
-                         "(each a separate JSON line)")
+                         "through the batched engine; serve: Poisson request stream through the engine; "
+                         "basis: superblock basis from quantum numbers; v-m: sector-changing "
+                         "single-slot product (each a separate JSON line)")
     ap.add_argument("--rate", type=float, default=200.0)
     ap.add_argument("--duration", type=float, default=10.0)
     ap.add_argument("--ed-count", type=int, default=256)
@@ -486,6 +562,10 @@ def main():
     if args.workload == "basis":
         print(json.dumps({"metric": "superblock basis builds/s", "workload": "basis",
                           "result": run_basis(D, args.steps, not args.no_cpu_baseline)}), flush=True)
+        return
+    if args.workload == "v-m":
+        print(json.dumps({"metric": "sector-changing single-slot products/s", "workload": "v-m",
+                          "result": run_v_m(D, args.steps, not args.no_cpu_baseline)}), flush=True)
         return
     snap = snapshot.load(WORKLOAD)
     e_ref = float(snap.meta["gs_energy"][0])
