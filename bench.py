@@ -420,6 +420,69 @@ def run_v_m(D, reps: int, cpu: bool):
     return res
 
 
+def run_density(D, reps: int, cpu: bool):
+    """Reduced density matrices of every LLLR block (DMRG_GET_DENSITY_MATRIX for all blocks)
+    of the Hubbard L=64 m=400 step: the reference's own target-sector basis (n=144542), its
+    LLLR states by first appearance, blocks of equal (electrons, S^z), a seeded unit
+    vector.  dmrgx_density_matrices per call (vector up, matrices down) against the oracle's
+    restatement of the reference's loop on one host core."""
+    import ctypes as C
+    z = np.load(BASIS_QN)
+    n_qn, dim_LL, dim_RR, d, _, _ = (int(v) for v in z["header"])
+    LL, LR, RR, RL = (np.ascontiguousarray(z[k], np.int16) for k in ("LL", "LR", "RR", "RL"))
+    n = LL.size
+    dims = (dim_LL, d, dim_RR, d)
+    inv = np.full(int(np.prod(dims)), -1, np.int32)
+    inv[np.ravel_multi_index(tuple(a.astype(np.int64) for a in (LL, LR, RR, RL)), dims)] = np.arange(n, dtype=np.int32)
+    key = LL.astype(np.int64) * d + LR
+    first = np.r_[True, key[1:] != key[:-1]]
+    LL_LLLR, LR_LLLR = np.ascontiguousarray(LL[first]), np.ascontiguousarray(LR[first])
+    Sum = np.diff(np.r_[np.nonzero(first)[0], n]).astype(np.int32)
+    q = np.stack([z["ele_LL"][LL_LLLR] + z["ele_site"][LR_LLLR], z["sz_LL"][LL_LLLR] + z["sz_site"][LR_LLLR]], 1)
+    starts = np.r_[0, np.nonzero(np.any(q[1:] != q[:-1], axis=1))[0] + 1, LL_LLLR.size]
+    dim_block = np.diff(starts).astype(np.int32)
+    v = np.random.default_rng(3).standard_normal(n)
+    v /= np.linalg.norm(v)
+    B = D.DMRG_BASIS()
+    P = lambda a, t: a.ctypes.data_as(C.POINTER(t))
+    B.LL_LLLRRRRL, B.LR_LLLRRRRL, B.RL_LLLRRRRL, B.RR_LLLRRRRL = (P(a, C.c_short) for a in (LL, LR, RL, RR))
+    B.Inv_LLLRRRRL = P(inv, C.c_int)
+    B.dim_LLLRRRRL, B.dim_LL, B.dim_RR, B.dim_onsite = n, dim_LL, dim_RR, d
+    B.LL_LLLR, B.LR_LLLR, B.dim_LLLR, B.Sum_LLLR = P(LL_LLLR, C.c_short), P(LR_LLLR, C.c_short), LL_LLLR.size, \
+        P(Sum, C.c_int)
+    mats = D.density_matrices(B, dim_block, v)
+    ts = []
+    for _ in range(max(reps, 10)):
+        t0 = time.perf_counter()
+        D.density_matrices(B, dim_block, v)
+        ts.append(time.perf_counter() - t0)
+    flops = float(np.sum(dim_block.astype(np.float64) ** 2 * Sum[starts[:-1]]))
+    res = {"n": n, "dim_LLLR": int(LL_LLLR.size), "blocks": len(dim_block), "max_block": int(dim_block.max()),
+           "matrix_entries": int(np.sum(dim_block.astype(np.int64) ** 2)),
+           "multiply_adds_full_blocks": flops, "calls_per_s": round(1.0 / float(np.median(ts)), 2),
+           "ms_per_call": round(1e3 * float(np.median(ts)), 3),
+           "note": "host-inclusive (vector up, matrices down); basis and quantum numbers of the reference's own run"}
+    if cpu:
+        sys.path.insert(0, os.path.join(REPO, "tests"))
+        import oracle_py as O
+        fn = O.lib().or_density_matrix
+        fn.restype = C.c_int
+        fn.argtypes = [C.c_int32] * 4 + [C.c_void_p] * 7 + [C.c_int32, C.c_int32, C.c_void_p]
+        outs = [np.empty((k, k)) for k in dim_block]
+        t0 = time.perf_counter()
+        k = 0
+        while k < 2 or time.perf_counter() - t0 < 3.0:
+            for b, o in enumerate(outs):
+                fn(n, dim_RR, d, LL_LLLR.size, LL_LLLR.ctypes.data, LR_LLLR.ctypes.data, Sum.ctypes.data,
+                   RR.ctypes.data, RL.ctypes.data, inv.ctypes.data, v.ctypes.data, int(starts[b]), int(dim_block[b]),
+                   o.ctypes.data)
+            k += 1
+        res["equals_oracle"] = all(np.array_equal(a, b) for a, b in zip(mats, outs))
+        res["cpu_baseline"] = {"calls_per_s": round(k / (time.perf_counter() - t0), 2), "cores": 1, "kind": "port",
+                               "sample": f"{k} passes over all blocks: the reference's loop restated in C"}
+    return res
+
+
 # ------------------------------------------------------------------ served stream
 SERVE_SNAPSHOTS = ["xxz_L12_m24_jz05_s13.npz", "hubbard_L8_m32_s07.npz", "klm_L6_m32_s04.npz",
                    "klm_L6_m24_ixy_s04.npz", "hubbard_L64_m400_s31.npz"]
@@ -530,11 +593,12 @@ def main():
                     help="SpMV schedule 'lanes,cap,variant' instead of autotuning (bench and --traffic-probe)")
     ap.add_argument("--spmv-sweep", action="store_true",
                     help="time every SpMV schedule on the workload superblock (JSON table), then exit")
-    ap.add_argument("--workload", choices=["superblock", "ed-batch", "serve", "basis", "v-m"], default="superblock",
+    ap.add_argument("--workload", choices=["superblock", "ed-batch", "serve", "basis", "v-m", "density"], default="superblock",
                     help="superblock: the headline line; ed-batch: 256 seeded ED solves one by one and "
                          "through the batched engine; serve: Poisson request stream through the engine; "
                          "basis: superblock basis from quantum numbers; v-m: sector-changing "
-                         "single-slot product (each a separate JSON line)")
+                         "single-slot product; density: reduced density matrices of the left half "
+                         "(each a separate JSON line)")
     ap.add_argument("--rate", type=float, default=200.0)
     ap.add_argument("--duration", type=float, default=10.0)
     ap.add_argument("--ed-count", type=int, default=256)
@@ -562,6 +626,10 @@ def main():
     if args.workload == "basis":
         print(json.dumps({"metric": "superblock basis builds/s", "workload": "basis",
                           "result": run_basis(D, args.steps, not args.no_cpu_baseline)}), flush=True)
+        return
+    if args.workload == "density":
+        print(json.dumps({"metric": "reduced density matrix sets/s", "workload": "density",
+                          "result": run_density(D, args.steps, not args.no_cpu_baseline)}), flush=True)
         return
     if args.workload == "v-m":
         print(json.dumps({"metric": "sector-changing single-slot products/s", "workload": "v-m",

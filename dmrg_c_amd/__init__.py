@@ -459,6 +459,28 @@ def _host_crs1(row, col, val) -> Tuple[CRS1, list]:
     return m, [r, c, v]
 
 
+def density_matrices(B: "DMRG_BASIS", dim_block, vec: np.ndarray) -> List[np.ndarray]:
+    """dmrgx_density_matrices: the reduced density matrix of every LLLR block of `B`
+    (DMRG_GET_DENSITY_MATRIX for blocks 0, 1, ...), formed on the device."""
+    L = lib()
+    if not getattr(L, "_dm_bound", False):
+        L.dmrgx_density_matrices.argtypes = [C.POINTER(DMRG_BASIS), C.c_int32, C.c_void_p, C.c_void_p,
+                                             C.POINTER(C.c_int64), C.c_void_p]
+        L._dm_bound = True
+    db = np.ascontiguousarray(dim_block, np.int32)
+    v = np.ascontiguousarray(vec, np.float64)
+    n = C.c_int64()
+    check(L.dmrgx_density_matrices(C.byref(B), len(db), db.ctypes.data, None, C.byref(n), None))
+    out = np.empty(n.value)
+    check(L.dmrgx_density_matrices(C.byref(B), len(db), db.ctypes.data, v.ctypes.data, C.byref(n),
+                                   out.ctypes.data))
+    mats, off = [], 0
+    for d in db:
+        mats.append(out[off:off + d * d].reshape(d, d))
+        off += d * d
+    return mats
+
+
 class WholeBasis:
     """dmrgx_whole_basis: the whole-space inverse table of DMRG_WHOLE_BASIS_Q1..Q4 on the
     device plus the sectors registered with add_sector(); v_m() is DMRG_V_M_*_Q1..Q4."""

@@ -621,6 +621,21 @@ int dmrgx_v_m_sector(dmrgx_whole_basis *W, int32_t slot, const CRS1 *M, int32_t 
                      int32_t sign_flag, double *Out_V);
 void dmrgx_whole_basis_free(dmrgx_whole_basis *W);
 
+/* ---- reduced density matrices of the left half (truncation, first step)
+ * dmrg/DMRG_GET_DENSITY_MATRIX.c:5-107, called per block by DMRG_GET_SYSTEM_INFO_Q1..Q3
+ * (dmrg/DMRG_GET_SYSTEM_INFO_Q2.c:22-23).  Block b is the run of Dim_Block[b] LLLR states
+ * (LL_LLLR/LR_LLLR, Sum_LLLR of the basis) after those of the blocks before it; its matrix
+ * is rho[r][c] = sum_k Vec[state (r, k)] Vec[state (c, k)] over the partners k the block's
+ * states share.  Entries are bit-identical to the reference's; the reference's checks
+ * (equal Sum_LLLR and partners within a block, no missing state) are errors here too. */
+/* All blocks at once: *n_out = sum of Dim_Block[b]^2; out (if not NULL) receives the
+ * blocks' row-major matrices one after another. */
+int dmrgx_density_matrices(const DMRG_BASIS *B, int32_t n_blocks, const int32_t *Dim_Block, const double *Vec,
+                           int64_t *n_out, double *out);
+/* include/dmrg.h:383 -- one block, rows malloc'ed as GET_ARRAY_DOUBLE2 does (free with
+ * FREE_ARRAY_DOUBLE2); exit(1) on error. */
+double **DMRG_GET_DENSITY_MATRIX(int block, int *Dim_Block, DMRG_BASIS *Dmrg_Basis, double *Vec, int p_threads);
+
 /* ---- exact diagonalization on the device
  * A problem is a sorted basis of base-d codes (site s is digit s, site 0 least
  * significant: exact/EXACT_FIND_SITE_STATE.c), on-site operators (d x d CRS) and a term

@@ -636,6 +636,37 @@ void or_v_m_sector(int slot, int32_t n_out, int32_t dim_RR, int32_t d, const int
    }
 }
 
+int or_density_matrix(int32_t n, int32_t dim_RR, int32_t d, int32_t dim_LLLR, const int16_t *LL_LLLR,
+                      const int16_t *LR_LLLR, const int32_t *Sum_LLLR, const int16_t *RR, const int16_t *RL,
+                      const int32_t *inv_whole, const double *vec, int32_t s0, int32_t db, double *out) {
+   int64_t *base = calloc((size_t)dim_LLLR + 1, sizeof(int64_t));   /* Base_LLLR, :20-22 */
+   for (int32_t s = 0; s < dim_LLLR; s++) base[s + 1] = base[s] + Sum_LLLR[s];
+   int bad = base[dim_LLLR] != n || s0 < 0 || s0 + db > dim_LLLR;
+   for (int64_t i = 0; i < (int64_t)db * db; i++) out[i] = 0.0;
+   for (int32_t r = 0; r < db && !bad; r++) {
+      const int32_t sr = s0 + r;
+      for (int32_t c = r; c < db && !bad; c++) {
+         const int32_t sc = s0 + c;
+         if (Sum_LLLR[sr] != Sum_LLLR[sc]) { bad = 1; break; }
+         double m = 0.0;
+         for (int32_t k = 0; k < Sum_LLLR[sr]; k++) {   /* :52-83 */
+            const int64_t pr = base[sr] + k, pc = base[sc] + k;
+            if (RR[pr] != RR[pc] || RL[pr] != RL[pc]) { bad = 1; break; }
+            int qr[4] = {LL_LLLR[sr], LR_LLLR[sr], RR[pr], RL[pr]};
+            int qc[4] = {LL_LLLR[sc], LR_LLLR[sc], RR[pc], RL[pc]};
+            const int32_t ir = inv_whole[sb_index(qr, dim_RR, d)], ic = inv_whole[sb_index(qc, dim_RR, d)];
+            if (ir < 0 || ic < 0) { bad = 1; break; }
+            m = m + vec[ir] * vec[ic];
+         }
+         out[(int64_t)r * db + c] = m;
+      }
+   }
+   for (int32_t r = 0; r < db; r++)   /* :97-101 */
+      for (int32_t c = r; c < db; c++) out[(int64_t)c * db + r] = out[(int64_t)r * db + c];
+   free(base);
+   return bad ? -1 : 0;
+}
+
 /* ------------------------------------------------------ exact diagonalization */
 /* sml/BINARY_SEARCH_LINT1.c: index of target in the ascending array, -1 when absent */
 static int64_t bsearch_i64(const int64_t *a, int64_t lo, int64_t hi, int64_t target) {

@@ -140,6 +140,17 @@ void or_v_m_sector(int slot, int32_t n_out, int32_t dim_RR, int32_t d, const int
                    const int32_t *col, const double *val, const int32_t *ele_LL, const int32_t *ele_LR,
                    const int32_t *ele_RR, int32_t sign_flag, const double *x, double *y);
 
+/* DMRG_GET_DENSITY_MATRIX (dmrg/DMRG_GET_DENSITY_MATRIX.c:5-107) for the block of db LLLR
+ * states starting at LLLR state s0: out[r*db + c] = sum over the Sum_LLLR[s0] partners k of
+ * Vec[Inv(LL_LLLR[s0+r], LR_LLLR[s0+r], RR_k, RL_k)] * Vec[Inv(... s0+c ...)], k ascending,
+ * upper triangle computed and mirrored.  Returns 0, or -1 where the reference stops with an
+ * error (partners differ, missing state, sums do not add up).  Pinned: the reference's own
+ * DMRG_GET_DENSITY_MATRIX (oracle/_ref) on the sub-bases of 5 reference runs, and numpy's
+ * Psi Psi^T with its trace equal to the block's share of |Vec|^2 (tests/test_density.py). */
+int or_density_matrix(int32_t n, int32_t dim_RR, int32_t d, int32_t dim_LLLR, const int16_t *LL_LLLR,
+                      const int16_t *LR_LLLR, const int32_t *Sum_LLLR, const int16_t *RR, const int16_t *RL,
+                      const int32_t *inv_whole, const double *vec, int32_t s0, int32_t db, double *out);
+
 /* ------------------------------------------------------------ exact diagonalization */
 /* One term of an ED Hamiltonian row (layout identical to dmrgx_ed_term):
  *   kind 0: coeff * O1 on site1               (exact/EXACT_MAKE_ELEM_ON.c:7-31)
