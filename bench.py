@@ -480,6 +480,51 @@ def run_density(D, reps: int, cpu: bool):
         res["equals_oracle"] = all(np.array_equal(a, b) for a, b in zip(mats, outs))
         res["cpu_baseline"] = {"calls_per_s": round(k / (time.perf_counter() - t0), 2), "cores": 1, "kind": "port",
                                "sample": f"{k} passes over all blocks: the reference's loop restated in C"}
+    # the whole truncation step: DMRG_GET_SYSTEM_INFO_Q2 with max_dim 400 (the run's m)
+    q1 = np.ascontiguousarray(q[:, 0], np.int32)
+    q2 = np.ascontiguousarray(q[:, 1], np.int32)
+    info = D.system_info(B, 2, q1, q2, v, 400)
+    L = D.lib()
+    ts = []
+    for _ in range(max(reps // 2, 5)):
+        out = C.POINTER(D.DMRG_SYSTEM_INFO)()
+        t0 = time.perf_counter()
+        D.check(L.dmrgx_get_system_info(2, q1.ctypes.data, q2.ctypes.data, C.byref(B), v.ctypes.data, 400,
+                                        C.byref(out)))
+        ts.append(time.perf_counter() - t0)
+        L.dmrgx_free_system_info(out)
+    tr = {"max_dim": 400, "dim_renorm": info["dim_renorm"], "tr_error": info["tr_error"],
+          "calls_per_s": round(1.0 / float(np.median(ts)), 2), "ms_per_call": round(1e3 * float(np.median(ts)), 3),
+          "note": "density matrices, rocSOLVER dsyevd per block, selection and transformation matrices; "
+                  "host-inclusive, result back in host memory"}
+    ref_lib = os.path.join(REPO, "oracle", "_ref", "libdmrgref.so")
+    if cpu and os.path.exists(ref_lib):
+        os.environ["MKL_NUM_THREADS"] = "1"   # read by LAPACK at its first call: one core, as reported
+        R = C.CDLL(ref_lib)
+        R.DMRG_GET_SYSTEM_INFO_Q2.restype = C.POINTER(D.DMRG_SYSTEM_INFO)
+        R.DMRG_GET_SYSTEM_INFO_Q2.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(D.DMRG_BASIS), C.c_void_p, C.c_int,
+                                              C.c_int, C.POINTER(D.DMRG_TIME)]
+        R.DMRG_FREE_SYSTEM_INFO.argtypes = [C.POINTER(D.DMRG_SYSTEM_INFO)]
+        t = D.DMRG_TIME()
+        t0 = time.perf_counter()
+        k = 0
+        pr = R.DMRG_GET_SYSTEM_INFO_Q2(q1.ctypes.data, q2.ctypes.data, C.byref(B), v.ctypes.data, 400, 1,
+                                       C.byref(t))
+        ref_info = D.system_info_dict(pr.contents, 2)
+        R.DMRG_FREE_SYSTEM_INFO(pr)
+        tr["dim_renorm_equals_reference"] = ref_info["dim_renorm"] == info["dim_renorm"]
+        tr["max_abs_val_dm_diff_vs_reference"] = float(np.abs(ref_info["val_dist"] - info["val_dist"]).max())
+        t0 = time.perf_counter()
+        k = 0
+        while k < 2 or time.perf_counter() - t0 < 3.0:
+            pr = R.DMRG_GET_SYSTEM_INFO_Q2(q1.ctypes.data, q2.ctypes.data, C.byref(B), v.ctypes.data, 400, 1,
+                                           C.byref(t))
+            R.DMRG_FREE_SYSTEM_INFO(pr)
+            k += 1
+        tr["cpu_baseline"] = {"calls_per_s": round(k / (time.perf_counter() - t0), 2), "cores": 1,
+                              "kind": "reference", "sample": f"{k} calls of the reference's own "
+                              "DMRG_GET_SYSTEM_INFO_Q2 (LAPACK dsyev), p_threads=1"}
+    res["system_info"] = tr
     return res
 
 

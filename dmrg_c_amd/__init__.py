@@ -481,6 +481,60 @@ def density_matrices(B: "DMRG_BASIS", dim_block, vec: np.ndarray) -> List[np.nda
     return mats
 
 
+class CCS1(C.Structure):
+    """include/SML.h:43-54"""
+    _fields_ = [("max_val", C.c_long), ("max_col", C.c_int), ("row_dim", C.c_int), ("col_dim", C.c_int),
+                ("Col", C.POINTER(C.c_long)), ("Row", C.POINTER(C.c_int)), ("Val", C.POINTER(C.c_double))]
+
+
+class DMRG_SYSTEM_INFO(C.Structure):
+    """include/dmrg.h:220-234"""
+    _fields_ = [("Trans_Matrix", C.POINTER(CCS1)), ("Trans_Matrix_Dagger", C.POINTER(CRS1)),
+                ("tr_error", C.c_double), ("Val_DM_Dist", C.POINTER(C.c_double)), ("sum_val_dm", C.c_double),
+                ("dim_LLLR", C.c_int), ("dim_renorm", C.c_int), ("Q_Number1", C.POINTER(C.c_int)),
+                ("Q_Number2", C.POINTER(C.c_int)), ("Q_Number3", C.POINTER(C.c_int))]
+
+
+def system_info_dict(S: DMRG_SYSTEM_INFO, n_qn: int) -> dict:
+    """Host copy of a DMRG_SYSTEM_INFO: the dense transformation matrix T (dim_LLLR x
+    dim_renorm) from its CCS1, and its CRS1 transpose checked against it."""
+    m, k = S.dim_LLLR, S.dim_renorm
+    T = np.zeros((m, k))
+    A = S.Trans_Matrix.contents
+    for c in range(k):
+        for e in range(A.Col[c], A.Col[c + 1]):
+            T[A.Row[e], c] = A.Val[e]
+    Ad = S.Trans_Matrix_Dagger.contents
+    Td = np.zeros((k, m))
+    for r in range(k):
+        for e in range(Ad.Row[r], Ad.Row[r + 1]):
+            Td[r, Ad.Col[e]] = Ad.Val[e]
+    return {"dim_LLLR": m, "dim_renorm": k, "tr_error": S.tr_error, "sum_val_dm": S.sum_val_dm,
+            "val_dist": np.array(S.Val_DM_Dist[:m]), "q1": np.array(S.Q_Number1[:k]),
+            "q2": np.array(S.Q_Number2[:k]) if n_qn == 2 else None, "T": T, "T_dagger_is_transpose":
+            bool(np.array_equal(Td, T.T)), "shapes": (A.row_dim, A.col_dim, Ad.row_dim, Ad.col_dim)}
+
+
+def system_info(B: "DMRG_BASIS", n_qn: int, q1, q2, vec: np.ndarray, max_dim: int) -> dict:
+    """dmrgx_get_system_info (DMRG_GET_SYSTEM_INFO_Q1/Q2 on the device) as a dict."""
+    L = lib()
+    if not getattr(L, "_si_bound", False):
+        L.dmrgx_get_system_info.argtypes = [C.c_int32, C.c_void_p, C.c_void_p, C.POINTER(DMRG_BASIS), C.c_void_p,
+                                            C.c_int32, C.POINTER(C.POINTER(DMRG_SYSTEM_INFO))]
+        L.dmrgx_free_system_info.argtypes = [C.POINTER(DMRG_SYSTEM_INFO)]
+        L._si_bound = True
+    a1 = np.ascontiguousarray(q1, np.int32)
+    a2 = np.ascontiguousarray(q2 if q2 is not None else q1, np.int32)
+    v = np.ascontiguousarray(vec, np.float64)
+    out = C.POINTER(DMRG_SYSTEM_INFO)()
+    check(L.dmrgx_get_system_info(n_qn, a1.ctypes.data, a2.ctypes.data, C.byref(B), v.ctypes.data, max_dim,
+                                  C.byref(out)))
+    try:
+        return system_info_dict(out.contents, n_qn)
+    finally:
+        L.dmrgx_free_system_info(out)
+
+
 class WholeBasis:
     """dmrgx_whole_basis: the whole-space inverse table of DMRG_WHOLE_BASIS_Q1..Q4 on the
     device plus the sectors registered with add_sector(); v_m() is DMRG_V_M_*_Q1..Q4."""

@@ -14,8 +14,17 @@
 // is built without FMA contraction), so the result is the reference's bit for bit.
 #include <hip/hip_runtime.h>
 
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
+#include <algorithm>
+#include <climits>
+#include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <map>
+#include <memory>
+#include <utility>
 #include <vector>
 
 #include "entry.h"
@@ -78,6 +87,24 @@ __global__ __launch_bounds__(256) void k_dm_syrk(const DmTile *__restrict__ tile
   }
 }
 
+// Copies block matrices into a batch of p x p matrices: slot j holds the block whose matrix
+// starts at src_off[j] (dimension dim[j]) in its leading corner, -1 on the rest of the
+// diagonal and 0 elsewhere.  The padded matrix is block-diagonal, so its eigenpairs are the
+// block's plus p - dim[j] pairs of eigenvalue -1 (below every eigenvalue of a density
+// matrix), which the eigensolver lists first.
+__global__ __launch_bounds__(256) void k_dm_pad(int32_t p, const int64_t *__restrict__ src_off,
+                                                const int32_t *__restrict__ dim, const double *__restrict__ src,
+                                                double *__restrict__ dst) {
+  const int32_t j = blockIdx.y;
+  const int32_t e = blockIdx.x * 256 + threadIdx.x;
+  if (e >= p * p) return;
+  const int32_t r = e / p, c = e % p, db = dim[j];
+  double v = 0.0;
+  if (r < db && c < db) v = src[src_off[j] + (int64_t)r * db + c];
+  else if (r == c) v = -1.0;
+  dst[(int64_t)j * p * p + e] = v;
+}
+
 // Checks and the Psi index array for blocks [b0, b1).  Returns the tiles; idx gets the Vec
 // index of every Psi element, packed block after block.
 std::vector<DmTile> dm_plan(const DMRG_BASIS &B, int32_t b0, int32_t b1, const int32_t *Dim_Block,
@@ -126,19 +153,19 @@ std::vector<DmTile> dm_plan(const DMRG_BASIS &B, int32_t b0, int32_t b1, const i
   return tiles;
 }
 
-// Device matrices of blocks [b0, b1) into host memory at out (packed row-major blocks).
-void density_blocks(const DMRG_BASIS &B, int32_t b0, int32_t b1, const int32_t *Dim_Block, const double *Vec,
-                    double *out) {
+// Device matrices of blocks [b0, b1) (packed row-major blocks, block b at out_off[b - b0]).
+void density_device(const DMRG_BASIS &B, int32_t b0, int32_t b1, const int32_t *Dim_Block, const double *Vec,
+                    DevBuf<double> &dout, std::vector<int64_t> &out_off) {
   std::vector<int32_t> idx;
-  std::vector<int64_t> out_off;
   const std::vector<DmTile> tiles = dm_plan(B, b0, b1, Dim_Block, idx, out_off);
   const int64_t n_out = out_off.back(), n_psi = (int64_t)idx.size();
   Context &C = ctx();
   hipStream_t st = C.stream;
   const int32_t n = B.dim_LLLRRRRL;
-  DevBuf<double> vec(n), psi(n_psi > 0 ? n_psi : 1), dout(n_out > 0 ? n_out : 1);
+  DevBuf<double> vec(n), psi(n_psi > 0 ? n_psi : 1);
   DevBuf<int32_t> didx(n_psi > 0 ? n_psi : 1);
   DevBuf<DmTile> dt(tiles.size());
+  dout.resize(n_out > 0 ? n_out : 1);
   C.copy_in(vec.p, Vec, sizeof(double) * n);
   if (n_psi > 0) C.copy_in(didx.p, idx.data(), sizeof(int32_t) * n_psi);
   C.copy_in(dt.p, tiles.data(), sizeof(DmTile) * tiles.size());
@@ -146,7 +173,228 @@ void density_blocks(const DMRG_BASIS &B, int32_t b0, int32_t b1, const int32_t *
   // an empty partner list leaves its block's matrix zero, as the reference's does
   k_dm_syrk<<<(unsigned)tiles.size(), 256, 0, st>>>(dt.p, psi.p, dout.p);
   HIPCHK(hipGetLastError());
-  C.copy_out(out, dout.p, sizeof(double) * n_out);
+  // the temporaries go back to the allocation cache only after the kernels that read them
+  HIPCHK(hipStreamSynchronize(st));
+}
+
+// Host copy of the matrices of blocks [b0, b1) at out.
+void density_blocks(const DMRG_BASIS &B, int32_t b0, int32_t b1, const int32_t *Dim_Block, const double *Vec,
+                    double *out) {
+  DevBuf<double> dout;
+  std::vector<int64_t> out_off;
+  density_device(B, b0, b1, Dim_Block, Vec, dout, out_off);
+  ctx().copy_out(out, dout.p, sizeof(double) * out_off.back());
+}
+
+rocblas_handle solver_handle() {
+  static rocblas_handle h = nullptr;
+  if (!h && rocblas_create_handle(&h) != rocblas_status_success)
+    throw Failure(DMRGX_ERR_HIP, "rocblas_create_handle failed");
+  if (rocblas_set_stream(h, ctx().stream) != rocblas_status_success)
+    throw Failure(DMRGX_ERR_HIP, "rocblas_set_stream failed");
+  return h;
+}
+
+// One eigenpair of a block, as DMRG_GET_SYSTEM_INFO_Q2.c:33-46 lists them.
+struct Pair {
+  double val;
+  int32_t block, basis, q1, q2;
+};
+
+template <class T>
+T *host_array(int64_t n) {   // GET_ARRAY_*: zeroed, freed with free()
+  T *p = static_cast<T *>(std::calloc((size_t)(n > 0 ? n : 1), sizeof(T)));
+  if (!p) throw Failure(DMRGX_ERR_NOMEM, "host allocation failed");
+  return p;
+}
+
+DMRG_SYSTEM_INFO *system_info(int32_t n_qn, const int32_t *Q1, const int32_t *Q2, const DMRG_BASIS &B,
+                              const double *Vec, int32_t max_dim) {
+  const int32_t dim_LLLR = B.dim_LLLR;
+  if (n_qn != 1 && n_qn != 2) throw Failure(DMRGX_ERR_ARG, "n_qn must be 1 or 2");
+  if (dim_LLLR <= 0) throw Failure(DMRGX_ERR_ARG, "dim_LLLR must be positive");
+  if (!Q1 || (n_qn == 2 && !Q2) || !Vec) throw Failure(DMRGX_ERR_ARG, "quantum numbers or vector missing");
+  auto q2 = [&](int32_t s) { return n_qn == 2 ? Q2[s] : 0; };
+  // DMRG_GET_DENSITY_MATRIX_INFO_Q1/Q2: order checks, then the blocks in ascending
+  // (Q_Number1, Q_Number2) order
+  for (int32_t s = 0; s + 1 < dim_LLLR; s++) {
+    const bool bad = n_qn == 1 ? Q1[s] > Q1[s + 1]
+                               : ((Q1[s] > Q1[s + 1] && q2(s) == q2(s + 1)) || (Q1[s] == Q1[s + 1] && q2(s) > q2(s + 1)));
+    if (bad) throw Failure(DMRGX_ERR_ARG, "LLLR quantum numbers out of order");
+  }
+  std::map<std::pair<int32_t, int32_t>, int32_t> count;
+  for (int32_t s = 0; s < dim_LLLR; s++) count[{Q1[s], q2(s)}]++;
+  std::vector<int32_t> dim_block, qb1, qb2;
+  for (const auto &kv : count) {
+    dim_block.push_back(kv.second);
+    qb1.push_back(kv.first.first);
+    qb2.push_back(kv.first.second);
+  }
+  const int32_t nb = (int32_t)dim_block.size();
+  DevBuf<double> dm;
+  std::vector<int64_t> off;
+  density_device(B, 0, nb, dim_block.data(), Vec, dm, off);
+  // eigenpairs of every block in place (LAPACK_DSYEV: ascending eigenvalues; rows of the
+  // row-major result are the eigenvectors, the matrix being symmetric)
+  Context &C = ctx();
+  std::vector<int64_t> row_base(nb + 1, 0);
+  for (int32_t b = 0; b < nb; b++) row_base[b + 1] = row_base[b] + dim_block[b];
+  // eigenpairs by batches of equal padded size (powers of two from 16): one batched solve
+  // per size instead of one solve per block
+  std::map<int32_t, std::vector<int32_t>> buckets;
+  for (int32_t b = 0; b < nb; b++) {
+    int32_t p = 16;
+    while (p < dim_block[b]) p *= 2;
+    buckets[p].push_back(b);
+  }
+  rocblas_handle h = solver_handle();
+  struct Batch {
+    int32_t p;
+    std::vector<int32_t> blocks;
+    DevBuf<double> a, w, e;
+    DevBuf<rocblas_int> info;
+  };
+  std::vector<std::unique_ptr<Batch>> batches;
+  for (const auto &kv : buckets) {
+    auto Bt = std::make_unique<Batch>();
+    Bt->p = kv.first;
+    Bt->blocks = kv.second;
+    const int32_t p = Bt->p, cnt = (int32_t)Bt->blocks.size();
+    std::vector<int64_t> so(cnt);
+    std::vector<int32_t> sd(cnt);
+    for (int32_t j = 0; j < cnt; j++) {
+      so[j] = off[Bt->blocks[j]];
+      sd[j] = dim_block[Bt->blocks[j]];
+    }
+    DevBuf<int64_t> dso(cnt);
+    DevBuf<int32_t> dsd(cnt);
+    C.copy_in(dso.p, so.data(), sizeof(int64_t) * cnt);
+    C.copy_in(dsd.p, sd.data(), sizeof(int32_t) * cnt);
+    Bt->a.resize((int64_t)cnt * p * p);
+    Bt->w.resize((int64_t)cnt * p);
+    Bt->e.resize((int64_t)cnt * p);
+    Bt->info.resize(cnt);
+    k_dm_pad<<<dim3((unsigned)((p * p + 255) / 256), (unsigned)cnt), 256, 0, C.stream>>>(p, dso.p, dsd.p, dm.p,
+                                                                                        Bt->a.p);
+    HIPCHK(hipGetLastError());
+    if (rocsolver_dsyevd_strided_batched(h, rocblas_evect_original, rocblas_fill_upper, p, Bt->a.p, p,
+                                         (rocblas_stride)p * p, Bt->w.p, p, Bt->e.p, p, Bt->info.p,
+                                         cnt) != rocblas_status_success)
+      throw Failure(DMRGX_ERR_HIP, "rocsolver_dsyevd_strided_batched failed");
+    // the offset and size arrays go back to the allocation cache after the padding kernel
+    HIPCHK(hipStreamSynchronize(C.stream));
+    batches.push_back(std::move(Bt));
+  }
+  // per block: eigenvalues ascending (hw, at row_base) and eigenvectors as rows (hv, at off)
+  std::vector<double> hw(dim_LLLR), hv(off.back());
+  for (const auto &Bt : batches) {
+    const int32_t p = Bt->p, cnt = (int32_t)Bt->blocks.size();
+    std::vector<double> ha((int64_t)cnt * p * p), hwp((int64_t)cnt * p);
+    std::vector<rocblas_int> hinfo(cnt);
+    C.copy_out(hinfo.data(), Bt->info.p, sizeof(rocblas_int) * cnt);
+    C.copy_out(hwp.data(), Bt->w.p, sizeof(double) * cnt * p);
+    C.copy_out(ha.data(), Bt->a.p, sizeof(double) * cnt * p * p);
+    for (int32_t j = 0; j < cnt; j++) {
+      if (hinfo[j] != 0) throw Failure(DMRGX_ERR_HIP, "block eigendecomposition did not converge");
+      const int32_t b = Bt->blocks[j], db = dim_block[b], pad = p - db;
+      const double *wj = hwp.data() + (int64_t)j * p, *aj = ha.data() + (int64_t)j * p * p;
+      // the pad eigenvalues (-1) come first; eigenvector i of the result is row i of the
+      // row-major view (column i of the column-major output)
+      for (int32_t i = 0; i < pad; i++)
+        if (wj[i] > -0.5) throw Failure(DMRGX_ERR_HIP, "padded eigenvalues out of place");
+      for (int32_t i = 0; i < db; i++) {
+        hw[row_base[b] + i] = wj[pad + i];
+        std::memcpy(hv.data() + off[b] + (int64_t)i * db, aj + (int64_t)(pad + i) * p, sizeof(double) * db);
+      }
+    }
+  }
+  // DMRG_GET_SYSTEM_INFO_Q2.c:33-86: every eigenpair, stable descending order
+  std::vector<Pair> p;
+  p.reserve(dim_LLLR);
+  for (int32_t b = 0; b < nb; b++)
+    for (int32_t i = 0; i < dim_block[b]; i++) p.push_back(Pair{hw[row_base[b] + i], b, i, qb1[b], qb2[b]});
+  std::stable_sort(p.begin(), p.end(), [](const Pair &a, const Pair &c) { return a.val > c.val; });
+  DMRG_SYSTEM_INFO *S = static_cast<DMRG_SYSTEM_INFO *>(std::calloc(1, sizeof(DMRG_SYSTEM_INFO)));
+  if (!S) throw Failure(DMRGX_ERR_NOMEM, "host allocation failed");
+  std::unique_ptr<DMRG_SYSTEM_INFO, void (*)(DMRG_SYSTEM_INFO *)> guard(S, dmrgx_free_system_info);
+  S->Val_DM_Dist = host_array<double>(dim_LLLR);
+  S->sum_val_dm = 0;
+  for (int32_t i = 0; i < dim_LLLR; i++) {
+    S->Val_DM_Dist[i] = p[i].val;
+    S->sum_val_dm = S->sum_val_dm + p[i].val;
+  }
+  // :91-112: the cut, moved down past (nearly) equal values
+  const double zero = 1e-15;
+  int32_t dim_renorm = 0;
+  if (dim_LLLR <= max_dim) {
+    for (int32_t i = dim_LLLR; i >= 1; i--)
+      if (p[i - 1].val > zero) {
+        dim_renorm = i;
+        break;
+      }
+  } else {
+    for (int32_t i = max_dim; i >= 1; i--)
+      if (std::fabs(p[i - 1].val - p[i].val) > zero && p[i - 1].val > zero) {
+        dim_renorm = i;
+        break;
+      }
+  }
+  if (dim_renorm <= 0) throw Failure(DMRGX_ERR_ARG, "no state kept (dim_renorm <= 0)");
+  double kept = 0;
+  for (int32_t i = 0; i < dim_renorm; i++) kept = kept + p[i].val;
+  S->tr_error = S->sum_val_dm - kept;
+  // :124-158: kept states by Q_Number2, then by Q_Number1, both stable
+  std::stable_sort(p.begin(), p.begin() + dim_renorm, [](const Pair &a, const Pair &c) { return a.q2 < c.q2; });
+  std::stable_sort(p.begin(), p.begin() + dim_renorm, [](const Pair &a, const Pair &c) { return a.q1 < c.q1; });
+  // :159-205: transformation matrix (CCS1) and its transpose (CRS1) from the eigenvectors
+  int64_t nnz = 0;
+  for (int32_t i = 0; i < dim_renorm; i++) nnz += dim_block[p[i].block];
+  CCS1 *T = static_cast<CCS1 *>(std::calloc(1, sizeof(CCS1)));
+  if (!T) throw Failure(DMRGX_ERR_NOMEM, "host allocation failed");
+  S->Trans_Matrix = T;
+  T->max_val = nnz;
+  T->max_col = dim_renorm + 1;
+  T->Val = host_array<double>(nnz);
+  T->Col = host_array<long>(dim_renorm + 1);
+  T->Row = host_array<int>(nnz);
+  CRS1 *Td = static_cast<CRS1 *>(std::calloc(1, sizeof(CRS1)));
+  if (!Td) throw Failure(DMRGX_ERR_NOMEM, "host allocation failed");
+  S->Trans_Matrix_Dagger = Td;
+  Td->max_val = nnz;
+  Td->max_row = dim_renorm + 1;
+  Td->Val = host_array<double>(nnz);
+  Td->Col = host_array<int>(nnz);
+  Td->Row = host_array<long>(dim_renorm + 1);
+  S->Q_Number1 = host_array<int>(dim_renorm);
+  S->Q_Number2 = host_array<int>(n_qn == 2 ? dim_renorm : 1);
+  S->Q_Number3 = host_array<int>(1);
+  S->dim_renorm = dim_renorm;
+  S->dim_LLLR = dim_LLLR;
+  for (int32_t i = 0; i < dim_renorm; i++) {
+    S->Q_Number1[i] = p[i].q1;
+    if (n_qn == 2) S->Q_Number2[i] = p[i].q2;
+  }
+  if (n_qn == 1) S->Q_Number2[0] = INT_MIN;
+  S->Q_Number3[0] = INT_MIN;
+  int64_t k = 0;
+  for (int32_t i = 0; i < dim_renorm; i++) {
+    const int32_t b = p[i].block, db = dim_block[b];
+    const double *vec = hv.data() + off[b] + (int64_t)p[i].basis * db;
+    for (int32_t j = 0; j < db; j++, k++) {
+      T->Val[k] = vec[j];
+      T->Row[k] = (int)(row_base[b] + j);
+      Td->Val[k] = vec[j];
+      Td->Col[k] = (int)(row_base[b] + j);
+    }
+    T->Col[i + 1] = k;
+    Td->Row[i + 1] = k;
+  }
+  T->row_dim = dim_LLLR;
+  T->col_dim = dim_renorm;
+  Td->row_dim = dim_renorm;
+  Td->col_dim = dim_LLLR;
+  guard.release();
+  return S;
 }
 
 }  // namespace
@@ -155,6 +403,60 @@ void density_blocks(const DMRG_BASIS &B, int32_t b0, int32_t b1, const int32_t *
 using namespace dmrgx;
 
 extern "C" {
+
+void dmrgx_free_system_info(DMRG_SYSTEM_INFO *Info) {
+  if (!Info) return;
+  if (Info->Trans_Matrix) {
+    std::free(Info->Trans_Matrix->Val);
+    std::free(Info->Trans_Matrix->Col);
+    std::free(Info->Trans_Matrix->Row);
+    std::free(Info->Trans_Matrix);
+  }
+  if (Info->Trans_Matrix_Dagger) {
+    std::free(Info->Trans_Matrix_Dagger->Val);
+    std::free(Info->Trans_Matrix_Dagger->Col);
+    std::free(Info->Trans_Matrix_Dagger->Row);
+    std::free(Info->Trans_Matrix_Dagger);
+  }
+  std::free(Info->Val_DM_Dist);
+  std::free(Info->Q_Number1);
+  std::free(Info->Q_Number2);
+  std::free(Info->Q_Number3);
+  std::free(Info);
+}
+
+int dmrgx_get_system_info(int32_t n_qn, const int32_t *Q_Number1, const int32_t *Q_Number2, const DMRG_BASIS *B,
+                          const double *Vec, int32_t max_dim, DMRG_SYSTEM_INFO **out) {
+  if (!B || !out) return DMRGX_ERR_ARG;
+  *out = nullptr;
+  return run_status([&] { *out = system_info(n_qn, Q_Number1, Q_Number2, *B, Vec, max_dim); });
+}
+
+DMRG_SYSTEM_INFO *DMRG_GET_SYSTEM_INFO_Q1(int *Q_Number, DMRG_BASIS *Dmrg_Basis, double *Vec, int max_dim,
+                                          int p_threads, DMRG_TIME *Dmrg_Time) {
+  (void)p_threads;
+  DMRG_SYSTEM_INFO *S = nullptr;
+  const double t0 = now_s();
+  run_or_die("DMRG_GET_SYSTEM_INFO_Q1", [&] {
+    if (!Dmrg_Basis) throw Failure(DMRGX_ERR_ARG, "basis missing");
+    S = system_info(1, Q_Number, nullptr, *Dmrg_Basis, Vec, max_dim);
+  });
+  if (Dmrg_Time) Dmrg_Time->make_dens_mat = now_s() - t0;
+  return S;
+}
+
+DMRG_SYSTEM_INFO *DMRG_GET_SYSTEM_INFO_Q2(int *Q_Number1, int *Q_Number2, DMRG_BASIS *Dmrg_Basis, double *Vec,
+                                          int max_dim, int p_threads, DMRG_TIME *Dmrg_Time) {
+  (void)p_threads;
+  DMRG_SYSTEM_INFO *S = nullptr;
+  const double t0 = now_s();
+  run_or_die("DMRG_GET_SYSTEM_INFO_Q2", [&] {
+    if (!Dmrg_Basis) throw Failure(DMRGX_ERR_ARG, "basis missing");
+    S = system_info(2, Q_Number1, Q_Number2, *Dmrg_Basis, Vec, max_dim);
+  });
+  if (Dmrg_Time) Dmrg_Time->make_dens_mat = now_s() - t0;
+  return S;
+}
 
 int dmrgx_density_matrices(const DMRG_BASIS *B, int32_t n_blocks, const int32_t *Dim_Block, const double *Vec,
                            int64_t *n_out, double *out) {

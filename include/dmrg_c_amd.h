@@ -74,6 +74,17 @@ typedef struct {
    char Guess[100];
    char Type[100];
 } BOX_LAN;
+
+/* include/SML.h:43-54 */
+typedef struct {
+   long max_val;
+   int max_col;
+   int row_dim;
+   int col_dim;
+   long *Col;
+   int *Row;
+   double *Val;
+} CCS1;
 #endif /* SML_h */
 
 #ifndef dmrg_h
@@ -251,6 +262,20 @@ typedef struct {
    int *Ele_RR;
    DMRG_BASIS *Dmrg_Basis;
 } DMRG_BOX_II;
+
+/* include/dmrg.h:220-234 */
+typedef struct {
+   CCS1 *Trans_Matrix;
+   CRS1 *Trans_Matrix_Dagger;
+   double tr_error;
+   double *Val_DM_Dist;
+   double sum_val_dm;
+   int dim_LLLR;
+   int dim_renorm;
+   int *Q_Number1;
+   int *Q_Number2;
+   int *Q_Number3;
+} DMRG_SYSTEM_INFO;
 #endif /* dmrg_h */
 
 #ifndef exact_h
@@ -635,6 +660,30 @@ int dmrgx_density_matrices(const DMRG_BASIS *B, int32_t n_blocks, const int32_t 
 /* include/dmrg.h:383 -- one block, rows malloc'ed as GET_ARRAY_DOUBLE2 does (free with
  * FREE_ARRAY_DOUBLE2); exit(1) on error. */
 double **DMRG_GET_DENSITY_MATRIX(int block, int *Dim_Block, DMRG_BASIS *Dmrg_Basis, double *Vec, int p_threads);
+
+/* ---- truncation: density matrices, their block eigendecompositions and the kept states
+ * include/dmrg.h:390-391 (dmrg/DMRG_GET_SYSTEM_INFO_Q1.c, DMRG_GET_SYSTEM_INFO_Q2.c:7-236):
+ * blocks of equal quantum numbers (DMRG_GET_DENSITY_MATRIX_INFO_Q1/Q2 and their order
+ * checks), each block's density matrix and its eigendecomposition on the device
+ * (rocSOLVER dsyevd in place of LAPACK_DSYEV), then the reference's selection: all
+ * eigenvalues sorted descending (stable), the cut at max_dim moved down past values that
+ * differ by 1e-15 or less, the truncation error, the kept states re-sorted by Q_Number2 and
+ * then Q_Number1 (stable), and the transformation matrix (CCS1, columns = kept
+ * eigenvectors over the LLLR states) with its transpose (CRS1).  Allocations are those of
+ * GET_CCS1/GET_CRS1/GET_ARRAY_*: DMRG_FREE_SYSTEM_INFO frees the result.  The eigenvalues
+ * (Val_DM_Dist), dim_renorm, the truncation error and the quantum numbers match the
+ * reference to LAPACK's accuracy; each kept eigenvector may differ from LAPACK's by its
+ * sign, and within a degenerate set by a rotation of the set.  Time in Dmrg_Time->make_dens_mat. */
+DMRG_SYSTEM_INFO *DMRG_GET_SYSTEM_INFO_Q1(int *Q_Number, DMRG_BASIS *Dmrg_Basis, double *Vec, int max_dim,
+                                          int p_threads, DMRG_TIME *Dmrg_Time);
+DMRG_SYSTEM_INFO *DMRG_GET_SYSTEM_INFO_Q2(int *Q_Number1, int *Q_Number2, DMRG_BASIS *Dmrg_Basis, double *Vec,
+                                          int max_dim, int p_threads, DMRG_TIME *Dmrg_Time);
+/* The same with a status code: n_qn 1 (Q_Number2 ignored) or 2; *out = NULL on error. */
+int dmrgx_get_system_info(int32_t n_qn, const int32_t *Q_Number1, const int32_t *Q_Number2, const DMRG_BASIS *B,
+                          const double *Vec, int32_t max_dim, DMRG_SYSTEM_INFO **out);
+/* Frees what the three calls above return, as DMRG_FREE_SYSTEM_INFO (include/dmrg.h:357,
+ * dmrg/DMRG_FREE_SYSTEM_INFO.c) does; the reference's own function frees it as well. */
+void dmrgx_free_system_info(DMRG_SYSTEM_INFO *Info);
 
 /* ---- exact diagonalization on the device
  * A problem is a sorted basis of base-d codes (site s is digit s, site 0 least

@@ -667,6 +667,64 @@ int or_density_matrix(int32_t n, int32_t dim_RR, int32_t d, int32_t dim_LLLR, co
    return bad ? -1 : 0;
 }
 
+static void swap_i32(int32_t *a, int32_t *b) { int32_t t = *a; *a = *b; *b = t; }
+
+int32_t or_system_select(int32_t nb, const int32_t *dim_block, const int32_t *qb1, const int32_t *qb2,
+                         const double *w, int32_t max_dim, double *val_dist, double *sums, int32_t *kept_block,
+                         int32_t *kept_basis) {
+   int32_t dim = 0;
+   for (int32_t b = 0; b < nb; b++) dim += dim_block[b];
+   int32_t *blk = malloc(sizeof(int32_t) * dim), *bas = malloc(sizeof(int32_t) * dim);
+   int32_t *q1 = malloc(sizeof(int32_t) * dim), *q2 = malloc(sizeof(int32_t) * dim);
+   int64_t *first = malloc(sizeof(int64_t) * (nb + 1));
+   first[0] = 0;
+   for (int32_t b = 0; b < nb; b++) first[b + 1] = first[b] + dim_block[b];
+   int32_t c = 0;
+   for (int32_t b = 0; b < nb; b++)   /* :38-46 */
+      for (int32_t i = 0; i < dim_block[b]; i++, c++) { blk[c] = b; bas[c] = i; q1[c] = qb1[b]; q2[c] = qb2[b]; }
+#define VAL(k) (w[first[blk[k]] + bas[k]])
+#define SWAP4(k) do { swap_i32(&blk[k], &blk[k + 1]); swap_i32(&bas[k], &bas[k + 1]); \
+                      swap_i32(&q1[k], &q1[k + 1]); swap_i32(&q2[k], &q2[k + 1]); } while (0)
+   for (int32_t i = 0; i < dim - 1; i++) {   /* :55-72 */
+      int moved = 0;
+      for (int32_t j = 0; j < dim - 1 - i; j++)
+         if (VAL(j) < VAL(j + 1)) { SWAP4(j); moved = 1; }
+      if (!moved) break;
+   }
+   double sum = 0;
+   for (int32_t i = 0; i < dim; i++) { val_dist[i] = VAL(i); sum = sum + VAL(i); }   /* :77-81 */
+   int32_t renorm = 0;
+   const double zero = pow(10, -15);
+   if (dim <= max_dim) {   /* :85-110 */
+      for (int32_t i = dim; i >= 1; i--)
+         if (VAL(i - 1) > zero) { renorm = i; break; }
+   } else {
+      for (int32_t i = max_dim; i >= 1; i--)
+         if (fabs(VAL(i - 1) - VAL(i)) > zero && VAL(i - 1) > zero) { renorm = i; break; }
+   }
+   double kept = 0;
+   for (int32_t i = 0; i < renorm; i++) kept = kept + VAL(i);   /* :116-120 */
+   sums[0] = sum;
+   sums[1] = sum - kept;
+   for (int32_t i = 0; i < renorm - 1; i++) {   /* :124-139, by Q_Number2 */
+      int moved = 0;
+      for (int32_t j = 0; j < renorm - 1 - i; j++)
+         if (q2[j] > q2[j + 1]) { SWAP4(j); moved = 1; }
+      if (!moved) break;
+   }
+   for (int32_t i = 0; i < renorm - 1; i++) {   /* :141-158, by Q_Number1 */
+      int moved = 0;
+      for (int32_t j = 0; j < renorm - 1 - i; j++)
+         if (q1[j] > q1[j + 1]) { SWAP4(j); moved = 1; }
+      if (!moved) break;
+   }
+#undef VAL
+#undef SWAP4
+   for (int32_t i = 0; i < renorm; i++) { kept_block[i] = blk[i]; kept_basis[i] = bas[i]; }
+   free(blk); free(bas); free(q1); free(q2); free(first);
+   return renorm > 0 ? renorm : -1;
+}
+
 /* ------------------------------------------------------ exact diagonalization */
 /* sml/BINARY_SEARCH_LINT1.c: index of target in the ascending array, -1 when absent */
 static int64_t bsearch_i64(const int64_t *a, int64_t lo, int64_t hi, int64_t target) {

@@ -151,6 +151,19 @@ int or_density_matrix(int32_t n, int32_t dim_RR, int32_t d, int32_t dim_LLLR, co
                       const int16_t *LR_LLLR, const int32_t *Sum_LLLR, const int16_t *RR, const int16_t *RL,
                       const int32_t *inv_whole, const double *vec, int32_t s0, int32_t db, double *out);
 
+/* The selection of DMRG_GET_SYSTEM_INFO_Q1/Q2 (dmrg/DMRG_GET_SYSTEM_INFO_Q2.c:33-158) from
+ * the blocks' eigenvalues w (block after block, ascending within a block, as LAPACK_DSYEV
+ * returns them): the bubble sort of all eigenpairs by descending value, Val_DM_Dist and
+ * sum_val_dm, the cut at max_dim moved down past values within 1e-15, the truncation error,
+ * and the bubble sorts of the kept pairs by qb2 and then qb1.  Writes val_dist (dim_LLLR),
+ * sums = {sum_val_dm, tr_error} and the kept pairs' (block, index) in the final order;
+ * returns dim_renorm, or -1 where the reference stops with an error.  Pinned: the
+ * reference's own DMRG_GET_SYSTEM_INFO_Q1/Q2 (oracle/_ref) on the blocks of 5 reference
+ * runs, with numpy's eigh for the eigenpairs (tests/test_truncate.py). */
+int32_t or_system_select(int32_t nb, const int32_t *dim_block, const int32_t *qb1, const int32_t *qb2,
+                         const double *w, int32_t max_dim, double *val_dist, double *sums, int32_t *kept_block,
+                         int32_t *kept_basis);
+
 /* ------------------------------------------------------------ exact diagonalization */
 /* One term of an ED Hamiltonian row (layout identical to dmrgx_ed_term):
  *   kind 0: coeff * O1 on site1               (exact/EXACT_MAKE_ELEM_ON.c:7-31)

@@ -53,6 +53,8 @@ class Sub:
         self.dim_LLLR = self.LL_LLLR.size
         ele = z["ele_LL"][self.LL_LLLR] + z["ele_site"][self.LR_LLLR] if n_qn == 2 else np.zeros(self.dim_LLLR)
         sz = z["sz_LL"][self.LL_LLLR] + z["sz_site"][self.LR_LLLR]
+        self.n_qn = n_qn
+        self.q_ele, self.q_sz = np.ascontiguousarray(ele, np.int32), np.ascontiguousarray(sz, np.int32)
         q = np.stack([ele, sz], 1)
         starts = np.r_[0, np.nonzero(np.any(q[1:] != q[:-1], axis=1))[0] + 1, self.dim_LLLR]
         self.dim_block = np.diff(starts).astype(np.int32)
