@@ -525,6 +525,59 @@ def run_density(D, reps: int, cpu: bool):
                               "kind": "reference", "sample": f"{k} calls of the reference's own "
                               "DMRG_GET_SYSTEM_INFO_Q2 (LAPACK dsyev), p_threads=1"}
     res["system_info"] = tr
+    # renormalisation of one LLLR operator: CRS_CRS_CCS_PRODUCT(T^dagger, O, T) with the kept
+    # transformation matrix and a seeded sparse symmetric O (1 % of the entries)
+    T = info["T"]
+    m, kk = T.shape
+    rng = np.random.default_rng(12)
+    Od = np.where(rng.random((m, m)) < 0.005, rng.uniform(-1, 1, (m, m)), 0.0)
+    Od = Od + Od.T
+    import scipy.sparse as sp
+    Oc, Tt, Tc = sp.csr_matrix(Od), sp.csr_matrix(T.T), sp.csc_matrix(T)
+    mk = lambda a: [np.ascontiguousarray(x) for x in (a.indptr.astype(np.int64), a.indices.astype(np.int32), a.data)]
+    M1, M2, M3 = mk(Tt), mk(Oc), mk(Tc)
+
+    def crs(a, n, cdim):
+        return D.CRS1(len(a[2]), n + 1, n, cdim, a[1].ctypes.data_as(C.POINTER(C.c_int)),
+                      a[0].ctypes.data_as(C.POINTER(C.c_long)), a[2].ctypes.data_as(C.POINTER(C.c_double)))
+    m1, m2 = crs(M1, kk, m), crs(M2, m, m)
+    m3 = D.CCS1(len(M3[2]), kk + 1, m, kk, M3[0].ctypes.data_as(C.POINTER(C.c_long)),
+                M3[1].ctypes.data_as(C.POINTER(C.c_int)), M3[2].ctypes.data_as(C.POINTER(C.c_double)))
+    bufs = [np.zeros(kk + 1, np.int64), np.zeros(kk * kk, np.int32), np.zeros(kk * kk),
+            np.zeros(kk + 1, np.int64), np.zeros(m * kk, np.int32), np.zeros(m * kk), np.zeros(m)]
+    out = D.CRS1(kk * kk, kk + 1, 0, 0, bufs[1].ctypes.data_as(C.POINTER(C.c_int)),
+                 bufs[0].ctypes.data_as(C.POINTER(C.c_long)), bufs[2].ctypes.data_as(C.POINTER(C.c_double)))
+    work = D.CCS1(m * kk, kk + 1, 0, 0, bufs[3].ctypes.data_as(C.POINTER(C.c_long)),
+                  bufs[4].ctypes.data_as(C.POINTER(C.c_int)), bufs[5].ctypes.data_as(C.POINTER(C.c_double)))
+    argt = [C.POINTER(D.CRS1), C.POINTER(D.CRS1), C.POINTER(D.CCS1), C.POINTER(D.CRS1), C.POINTER(D.CCS1),
+            C.c_void_p]
+
+    def run(fn):
+        fn.argtypes = argt
+        fn(C.byref(m1), C.byref(m2), C.byref(m3), C.byref(out), C.byref(work), bufs[6].ctypes.data)
+        n = int(bufs[0][kk])
+        return bufs[0].copy(), bufs[1][:n].copy(), bufs[2][:n].copy()
+    got = run(L.CRS_CRS_CCS_PRODUCT)
+    ts = []
+    for _ in range(max(reps // 2, 5)):
+        t0 = time.perf_counter()
+        run(L.CRS_CRS_CCS_PRODUCT)
+        ts.append(time.perf_counter() - t0)
+    rn = {"dim_LLLR": m, "dim_renorm": kk, "operator_nnz": int(len(M2[2])), "out_nnz": int(got[0][kk]),
+          "calls_per_s": round(1.0 / float(np.median(ts)), 2), "ms_per_call": round(1e3 * float(np.median(ts)), 3),
+          "note": "drop-in CRS_CRS_CCS_PRODUCT, host arrays in and out"}
+    if cpu and os.path.exists(ref_lib):
+        Rf = C.CDLL(ref_lib)
+        want = run(Rf.CRS_CRS_CCS_PRODUCT)
+        rn["equals_reference"] = all(np.array_equal(a, b) for a, b in zip(got, want))
+        t0 = time.perf_counter()
+        k = 0
+        while k < 2 or time.perf_counter() - t0 < 3.0:
+            run(Rf.CRS_CRS_CCS_PRODUCT)
+            k += 1
+        rn["cpu_baseline"] = {"calls_per_s": round(k / (time.perf_counter() - t0), 2), "cores": 1,
+                              "kind": "reference", "sample": f"{k} calls of the reference's own CRS_CRS_CCS_PRODUCT"}
+    res["renormalise"] = rn
     return res
 
 

@@ -725,6 +725,48 @@ int32_t or_system_select(int32_t nb, const int32_t *dim_block, const int32_t *qb
    return renorm > 0 ? renorm : -1;
 }
 
+int64_t or_crs_crs_ccs_product(int32_t n1, const int64_t *r1, const int32_t *c1, const double *v1, int32_t n2,
+                               const int64_t *r2, const int32_t *c2, const double *v2, int32_t n3, int32_t k,
+                               const int64_t *cptr3, const int32_t *ridx3, const double *v3, int64_t *out_row,
+                               int32_t *out_col, double *out_val) {
+   const double zero = pow(10, -15);
+   /* pass 1 (:33-63): W = M2 M3 column by column, kept entries in CCS */
+   int64_t cap = (int64_t)n2 * k + 1, nw = 0;
+   double *wv = malloc(sizeof(double) * cap), *dense = calloc((size_t)(n3 > n2 ? n3 : n2) + 1, sizeof(double));
+   int32_t *wr = malloc(sizeof(int32_t) * cap);
+   int64_t *wc = malloc(sizeof(int64_t) * ((size_t)k + 1));
+   wc[0] = 0;
+   for (int32_t j = 0; j < k; j++) {
+      for (int64_t e = cptr3[j]; e < cptr3[j + 1]; e++) dense[ridx3[e]] = v3[e];
+      for (int32_t i = 0; i < n2; i++) {
+         double val = 0;
+         for (int64_t e = r2[i]; e < r2[i + 1]; e++) val = val + dense[c2[e]] * v2[e];
+         if (fabs(val) > zero) { wv[nw] = val; wr[nw] = i; nw++; }
+      }
+      for (int64_t e = cptr3[j]; e < cptr3[j + 1]; e++) dense[ridx3[e]] = 0.0;
+      wc[j + 1] = nw;
+   }
+   /* pass 2 (:67-121): out = M1 W row by row */
+   for (int32_t i = 0; i <= n2; i++) dense[i] = 0;
+   int64_t nnz = 0;
+   out_row[0] = 0;
+   for (int32_t i = 0; i < n1; i++) {
+      for (int64_t e = r1[i]; e < r1[i + 1]; e++) dense[c1[e]] = v1[e];
+      for (int32_t j = 0; j < k; j++) {
+         double val = 0;
+         for (int64_t e = wc[j]; e < wc[j + 1]; e++) val = val + dense[wr[e]] * wv[e];
+         if (fabs(val) > zero) {
+            if (out_col) { out_col[nnz] = j; out_val[nnz] = val; }
+            nnz++;
+         }
+      }
+      for (int64_t e = r1[i]; e < r1[i + 1]; e++) dense[c1[e]] = 0.0;
+      out_row[i + 1] = nnz;
+   }
+   free(wv); free(dense); free(wr); free(wc);
+   return nnz;
+}
+
 /* ------------------------------------------------------ exact diagonalization */
 /* sml/BINARY_SEARCH_LINT1.c: index of target in the ascending array, -1 when absent */
 static int64_t bsearch_i64(const int64_t *a, int64_t lo, int64_t hi, int64_t target) {

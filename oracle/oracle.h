@@ -164,6 +164,16 @@ int32_t or_system_select(int32_t nb, const int32_t *dim_block, const int32_t *qb
                          const double *w, int32_t max_dim, double *val_dist, double *sums, int32_t *kept_block,
                          int32_t *kept_basis);
 
+/* CRS_CRS_CCS_PRODUCT (sml/CRS_CRS_CCS_PRODUCT.c:6-124): out = M1 (M2 M3), M1 n1 x n2 and
+ * M2 n2 x n3 in CRS, M3 n3 x k in CCS (cptr over columns, ridx), both passes dropping sums of
+ * magnitude <= 1e-15.  Writes out_row (n1 + 1) and, when out_col != NULL, the entries
+ * (columns ascending); returns nnz.  Pinned: the reference's own CRS_CRS_CCS_PRODUCT
+ * (oracle/_ref) bit for bit, and a dense numpy triple product (tests/test_transmat.py). */
+int64_t or_crs_crs_ccs_product(int32_t n1, const int64_t *r1, const int32_t *c1, const double *v1, int32_t n2,
+                               const int64_t *r2, const int32_t *c2, const double *v2, int32_t n3, int32_t k,
+                               const int64_t *cptr3, const int32_t *ridx3, const double *v3, int64_t *out_row,
+                               int32_t *out_col, double *out_val);
+
 /* ------------------------------------------------------------ exact diagonalization */
 /* One term of an ED Hamiltonian row (layout identical to dmrgx_ed_term):
  *   kind 0: coeff * O1 on site1               (exact/EXACT_MAKE_ELEM_ON.c:7-31)
