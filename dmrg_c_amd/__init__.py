@@ -578,6 +578,40 @@ class WholeBasis:
                                      out.ctypes.data))
         return out
 
+    def intersite(self, sector_in: int, sector_out1: int, sector_out2: int, M_CF, M_LL, M_On, M_RR, origin: int,
+                  LL_site: int, RR_site: int, x: np.ndarray) -> np.ndarray:
+        """dmrgx_expectation_intersite_sector; operators as (row, col, val) tuples, M_LL a
+        dict or list indexed by site (only M_LL[origin] is used)."""
+        L = lib()
+        P = C.POINTER(CRS1)
+        L.dmrgx_expectation_intersite_sector.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_void_p,
+                                                         C.c_void_p, P, C.c_void_p, C.c_int32, C.c_int32, C.c_int32,
+                                                         C.c_void_p, C.c_void_p]
+        keep = []
+
+        def arr(ops):
+            ms = []
+            for o in ops:
+                m, k = _host_crs1(*o)
+                keep.append(k)
+                ms.append(m)
+            a = (CRS1 * max(len(ms), 1))(*ms)
+            keep.append(a)
+            return (P * max(len(ms), 1))(*[C.pointer(a[i]) for i in range(len(ms))])
+        cf = arr(M_CF)
+        ll = None
+        if origin <= LL_site:
+            ll = arr([M_LL[origin] if i == origin else M_LL[origin] for i in range(origin + 1)])
+        on, kon = _host_crs1(*M_On)
+        rr = arr(M_RR)
+        x = np.ascontiguousarray(x, np.float64)
+        out = np.zeros(max(RR_site + LL_site + 4 - origin, 1))
+        check(L.dmrgx_expectation_intersite_sector(self.handle, sector_in, sector_out1, sector_out2, C.cast(cf, C.c_void_p),
+                                                   None if ll is None else C.cast(ll, C.c_void_p), C.byref(on),
+                                                   C.cast(rr, C.c_void_p), origin, LL_site, RR_site, x.ctypes.data,
+                                                   out.ctypes.data))
+        return out
+
     def free(self):
         if getattr(self, "handle", None):
             lib().dmrgx_whole_basis_free(self.handle)

@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "entry.h"
+#include "reduce.h"
 
 namespace dmrgx {
 namespace {
@@ -64,6 +65,24 @@ __global__ __launch_bounds__(256) void k_v_m_sector(int32_t n_out, int32_t slot,
   if (out_of_range) *bad = 1;
 }
 
+constexpr int kDotBlocks = 512;
+
+// This block's share of sum_i a_i b_i (grid-stride, then the deterministic block sum).
+__global__ __launch_bounds__(kBlock) void k_dot_partial(int64_t n, const double *__restrict__ a,
+                                                        const double *__restrict__ b, double *__restrict__ partial) {
+  double s = 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
+    s += a[i] * b[i];
+  block_partial(s, partial);
+}
+
+// *out = (accumulate ? *out : 0) + the partials' sum in block order.
+__global__ __launch_bounds__(kBlock) void k_dot_finish(const double *__restrict__ partial, int nblk,
+                                                       double *__restrict__ out, int accumulate) {
+  const double t = finish_sum(partial, nblk);
+  if (threadIdx.x == 0) *out = accumulate ? *out + t : t;
+}
+
 }  // namespace
 }  // namespace dmrgx
 
@@ -75,6 +94,58 @@ struct dmrgx_whole_basis {
   DevBuf<int32_t> inv;
   std::vector<std::unique_ptr<SectorDev>> sectors;
 };
+
+namespace {
+
+// Device copy of a slot operator (validated like dmrgx_v_m_sector's).
+struct DevSlotOp {
+  DevBuf<int64_t> row;
+  DevBuf<int32_t> col;
+  DevBuf<double> val;
+};
+
+void upload_slot_op(const dmrgx_whole_basis &W, int32_t slot, const CRS1 *M, DevSlotOp &O) {
+  const int32_t slot_dim = slot == 0 ? W.dim_LL : slot == 2 ? W.dim_RR : W.d;
+  if (!M || !M->Row || M->row_dim < slot_dim) throw Failure(DMRGX_ERR_ARG, "operator has fewer rows than its block");
+  const int64_t nnz = M->Row[slot_dim];
+  for (int32_t r = 0; r < slot_dim; r++)
+    if (M->Row[r] < 0 || M->Row[r] > M->Row[r + 1]) throw Failure(DMRGX_ERR_ARG, "operator row pointers not ascending");
+  for (int64_t j = 0; j < nnz; j++)
+    if (M->Col[j] < 0 || M->Col[j] >= slot_dim) throw Failure(DMRGX_ERR_ARG, "operator column outside its block");
+  Context &C = ctx();
+  O.row.resize(slot_dim + 1);
+  O.col.resize(nnz > 0 ? nnz : 1);
+  O.val.resize(nnz > 0 ? nnz : 1);
+  C.copy_in(O.row.p, M->Row, sizeof(int64_t) * (slot_dim + 1));
+  if (nnz > 0) {
+    C.copy_in(O.col.p, M->Col, sizeof(int32_t) * nnz);
+    C.copy_in(O.val.p, M->Val, sizeof(double) * nnz);
+  }
+}
+
+// y (the output sector's dimension) = O on slot of x (dim_in entries), no sign.
+void apply_slot(const dmrgx_whole_basis &W, int32_t slot, const CRS1 *M, int32_t sector_out, const double *x,
+                int32_t dim_in, double *y, int32_t *bad) {
+  DevSlotOp O;
+  upload_slot_op(W, slot, M, O);
+  const SectorDev &S = *W.sectors[sector_out];
+  if (S.n == 0) return;
+  k_v_m_sector<<<(unsigned)((S.n + 255) / 256), 256, 0, ctx().stream>>>(
+      S.n, slot, W.dim_RR, W.d, S.LL.p, S.LR.p, S.RR.p, S.RL.p, W.inv.p, O.row.p, O.col.p, O.val.p, nullptr, nullptr,
+      nullptr, dim_in, x, y, bad);
+  HIPCHK(hipGetLastError());
+  // O's buffers go back to the allocation cache here; whatever reuses them is queued on the
+  // same stream, behind this kernel
+}
+
+void dot_into(int64_t n, const double *a, const double *b, double *partial, double *out, bool accumulate) {
+  hipStream_t st = ctx().stream;
+  k_dot_partial<<<kDotBlocks, kBlock, 0, st>>>(n, a, b, partial);
+  k_dot_finish<<<1, kBlock, 0, st>>>(partial, kDotBlocks, out, accumulate ? 1 : 0);
+  HIPCHK(hipGetLastError());
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -173,6 +244,68 @@ int dmrgx_v_m_sector(dmrgx_whole_basis *W, int32_t slot, const CRS1 *M, int32_t 
     C.copy_out(&hbad, bad.p, sizeof(int32_t));
     if (hbad) throw Failure(DMRGX_ERR_ARG, "the operator reaches a state whose index lies outside the input vector");
     C.copy_out(Out_V, y.p, sizeof(double) * S.n);
+  });
+}
+
+int dmrgx_expectation_intersite_sector(dmrgx_whole_basis *W, int32_t sector_in, int32_t sector_out1,
+                                       int32_t sector_out2, const CRS1 *const *M_CF, const CRS1 *const *M_LL,
+                                       const CRS1 *M_On, const CRS1 *const *M_RR, int32_t origin, int32_t LL_site,
+                                       int32_t RR_site, const double *Vec, double *Out) {
+  if (!W || !M_CF || !M_On || !M_RR || !Vec || !Out || origin < 0 || LL_site < 0 || RR_site < 0)
+    return DMRGX_ERR_ARG;
+  return run_status([&] {
+    const int32_t ns = (int32_t)W->sectors.size();
+    if (sector_in < 0 || sector_in >= ns || sector_out1 < 0 || sector_out1 >= ns || sector_out2 < 0 ||
+        sector_out2 >= ns)
+      throw Failure(DMRGX_ERR_ARG, "no such sector");
+    if (origin > LL_site + 1) return;   // EXPECTATION_INTERSITE_Q2.c:27-29
+    if (origin <= LL_site && (!M_LL || !M_LL[origin])) throw Failure(DMRGX_ERR_ARG, "M_LL[origin] missing");
+    const int32_t n_in = W->sectors[sector_in]->n, n1 = W->sectors[sector_out1]->n, n2 = W->sectors[sector_out2]->n;
+    const int32_t n_out = RR_site + LL_site + 4 - origin;
+    Context &C = ctx();
+    hipStream_t st = C.stream;
+    DevBuf<double> x(n_in > 0 ? n_in : 1), t1a(n1 > 0 ? n1 : 1), t1b(n2 > 0 ? n2 : 1), t2a(n1 > 0 ? n1 : 1),
+        t2b(n2 > 0 ? n2 : 1), t2in(n_in > 0 ? n_in : 1), res(n_out), partial(kDotBlocks);
+    DevBuf<int32_t> bad(1);
+    HIPCHK(hipMemsetAsync(bad.p, 0, sizeof(int32_t), st));
+    if (n_in > 0) C.copy_in(x.p, Vec, sizeof(double) * n_in);
+    // :31-37: the origin's operator into both neighbouring sectors
+    if (origin == LL_site + 1) {
+      apply_slot(*W, 1, M_On, sector_out1, x.p, n_in, t1a.p, bad.p);
+      apply_slot(*W, 1, M_On, sector_out2, x.p, n_in, t1b.p, bad.p);
+    } else {
+      apply_slot(*W, 0, M_LL[origin], sector_out1, x.p, n_in, t1a.p, bad.p);
+      apply_slot(*W, 0, M_LL[origin], sector_out2, x.p, n_in, t1b.p, bad.p);
+    }
+    // :39-43: sites origin..LL_site, the correlation operator in the input sector
+    for (int32_t site = origin; site <= LL_site; site++) {
+      const int32_t r = site - origin;
+      if (!M_CF[r]) throw Failure(DMRGX_ERR_ARG, "M_CF[r] missing");
+      apply_slot(*W, 0, M_CF[r], sector_in, x.p, n_in, t2in.p, bad.p);
+      dot_into(n_in, x.p, t2in.p, partial.p, res.p + r, false);
+    }
+    // :45-54: LR and RL sites, both neighbouring sectors
+    const int32_t slots[2] = {1, 3};
+    for (int q = 0; q < 2; q++) {
+      const int32_t r = LL_site + 1 + q - origin;
+      apply_slot(*W, slots[q], M_On, sector_out1, x.p, n_in, t2a.p, bad.p);
+      apply_slot(*W, slots[q], M_On, sector_out2, x.p, n_in, t2b.p, bad.p);
+      dot_into(n1, t1a.p, t2a.p, partial.p, res.p + r, false);
+      dot_into(n2, t1b.p, t2b.p, partial.p, res.p + r, true);
+    }
+    // :56-62: RR sites from RR_site down to 0
+    for (int32_t site = RR_site; site >= 0; site--) {
+      const int32_t r = RR_site + LL_site + 3 - site - origin;
+      if (!M_RR[site]) throw Failure(DMRGX_ERR_ARG, "M_RR[site] missing");
+      apply_slot(*W, 2, M_RR[site], sector_out1, x.p, n_in, t2a.p, bad.p);
+      apply_slot(*W, 2, M_RR[site], sector_out2, x.p, n_in, t2b.p, bad.p);
+      dot_into(n1, t1a.p, t2a.p, partial.p, res.p + r, false);
+      dot_into(n2, t1b.p, t2b.p, partial.p, res.p + r, true);
+    }
+    int32_t hbad = 0;
+    C.copy_out(&hbad, bad.p, sizeof(int32_t));
+    if (hbad) throw Failure(DMRGX_ERR_ARG, "an operator reaches a state whose index lies outside the input vector");
+    C.copy_out(Out, res.p, sizeof(double) * n_out);
   });
 }
 

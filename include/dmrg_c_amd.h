@@ -644,6 +644,18 @@ int dmrgx_whole_basis_add_sector(dmrgx_whole_basis *W, int32_t dim, const int16_
 int dmrgx_v_m_sector(dmrgx_whole_basis *W, int32_t slot, const CRS1 *M, int32_t sector_out, int32_t dim_in,
                      const double *V, const int32_t *ele_LL, const int32_t *ele_LR, const int32_t *ele_RR,
                      int32_t sign_flag, double *Out_V);
+/* EXPECTATION_INTERSITE_Q2 of the XXZ program (main/dmrg/xxz_vf/src/expectations/
+ * EXPECTATION_INTERSITE_Q2.c:11-67, the S^x-type correlation across the S^z-2 and S^z+2
+ * sectors) on the device, its intermediate vectors kept there: Out[r] for
+ * r = 0 .. RR_site + LL_site + 3 - origin, the sectors named by their ids (the input vector
+ * has the input sector's dimension).  M_CF[0 .. LL_site - origin], M_LL[origin] (unless
+ * origin == LL_site + 1), M_RR[0 .. RR_site].  Nothing is written when origin > LL_site + 1,
+ * as in the reference.  Inner products are the library's deterministic reductions, so
+ * entries agree with the reference to rounding. */
+int dmrgx_expectation_intersite_sector(dmrgx_whole_basis *W, int32_t sector_in, int32_t sector_out1,
+                                       int32_t sector_out2, const CRS1 *const *M_CF, const CRS1 *const *M_LL,
+                                       const CRS1 *M_On, const CRS1 *const *M_RR, int32_t origin, int32_t LL_site,
+                                       int32_t RR_site, const double *Vec, double *Out);
 void dmrgx_whole_basis_free(dmrgx_whole_basis *W);
 
 /* ---- reduced density matrices of the left half (truncation, first step)

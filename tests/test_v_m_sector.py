@@ -307,3 +307,61 @@ def test_device_rejects_out_of_range_inverse():
         np.testing.assert_array_equal(ok, np.arange(sectors[big][0].size, dtype=np.float64))
     finally:
         W.free()
+
+
+def _sx_like(q, slot, rng, dsz_abs=2, conserve=False):
+    """Random CRS on the slot's block: S^z-conserving (conserve) or changing S^z by +-2."""
+    e, s = slot_qn(q, slot)
+    diff = s[:, None] - s[None, :]
+    mask = (diff == 0) if conserve else (np.abs(diff) == dsz_abs)
+    mask &= (e[:, None] == e[None, :]) & (rng.random(mask.shape) < 0.6)
+    m = sp.csr_matrix(np.where(mask, rng.uniform(-1, 1, mask.shape), 0.0))
+    m.sort_indices()
+    return m.indptr.astype(np.int64), m.indices.astype(np.int32), m.data.copy()
+
+
+@pytest.mark.gpu
+def test_device_intersite_correlation_matches_composition():
+    """EXPECTATION_INTERSITE_Q2 (main/dmrg/xxz_vf/src/expectations/EXPECTATION_INTERSITE_Q2.c)
+    on a three-sector whole basis (target and target -+ 2, other states -1 in the inverse table,
+    as GET_WHOLE_BASIS_Q2_SUPERBLOCK builds it) against the same steps composed from the
+    oracle's sector products and numpy inner products."""
+    import dmrg_c_amd as D
+    path = next(f for f in SMALL if "xxz_L12" in f)
+    q, tb = _load(path)
+    inv_all, sectors = whole_basis(q, tb)
+    t = q["target"]
+    keys = [t, (t[0], t[1] - 2), (t[0], t[1] + 2)]
+    inv = np.full_like(inv_all, -1)
+    for k in keys:
+        inv[sectors[k][0]] = inv_all[sectors[k][0]]
+    rng = np.random.default_rng(21)
+    LL_site, RR_site = 3, 2
+    x = rng.uniform(-1, 1, sectors[t][0].size)
+    W = D.WholeBasis(q["dim_LL"], q["dim_RR"], q["d"], inv)
+    try:
+        ids = [W.add_sector(*sectors[k][1:]) for k in keys]
+        for origin in (0, 2, LL_site + 1, LL_site + 2):
+            M_CF = [_sx_like(q, 0, rng, conserve=True) for _ in range(max(LL_site - origin + 1, 0))]
+            M_LL = {origin: _sx_like(q, 0, rng)}
+            M_On = _sx_like(q, 1, rng)
+            M_RR = [_sx_like(q, 2, rng) for _ in range(RR_site + 1)]
+            print(f"intersite {os.path.basename(path)} origin {origin}", flush=True)
+            got = W.intersite(*ids, M_CF, M_LL, M_On, M_RR, origin, LL_site, RR_site, x)
+            if origin > LL_site + 1:
+                assert not got.any()
+                continue
+            ov = lambda slot, op, k, v: oracle_v_m(q, slot, op, inv, sectors[k], v, False)
+            first = (1, M_On) if origin == LL_site + 1 else (0, M_LL[origin])
+            t1 = [ov(*first, k, x) for k in keys[1:]]
+            want = np.zeros(RR_site + LL_site + 4 - origin)
+            for site in range(origin, LL_site + 1):
+                want[site - origin] = x @ ov(0, M_CF[site - origin], t, x)
+            for qq, slot in enumerate((1, 3)):
+                want[LL_site + 1 + qq - origin] = sum(t1[i] @ ov(slot, M_On, keys[1 + i], x) for i in range(2))
+            for site in range(RR_site, -1, -1):
+                want[RR_site + LL_site + 3 - site - origin] = sum(t1[i] @ ov(2, M_RR[site], keys[1 + i], x)
+                                                                  for i in range(2))
+            np.testing.assert_allclose(got, want, rtol=1e-12, atol=1e-12 * np.abs(want).max())
+    finally:
+        W.free()
