@@ -50,11 +50,34 @@ __global__ __launch_bounds__(256) void k_v_m_sector(int32_t n_out, int32_t slot,
     if (e % 2 != 0) sgn = -1.0;
   }
   const int own = q[slot];
+  // the flat index with the acted slot's term left out: the gathered state of column c is
+  // base + c * stride
+  const int64_t stride = slot == 0 ? (int64_t)d * dim_RR * d : slot == 1 ? (int64_t)dim_RR * d : slot == 2 ? d : 1;
+  q[slot] = 0;
+  const int64_t base = (((int64_t)q[0] * d + q[1]) * dim_RR + q[2]) * d + q[3];
   double s = 0.0;
   bool out_of_range = false;
-  for (int64_t j = row[own]; j < row[own + 1]; j++) {
-    q[slot] = col[j];
-    const int32_t iv = inv[(((int64_t)q[0] * d + q[1]) * dim_RR + q[2]) * d + q[3]];
+  int64_t j = row[own];
+  const int64_t end = row[own + 1];
+  // four entries' loads in flight at a time; the additions stay in entry order
+  for (; j + 4 <= end; j += 4) {
+    int32_t iv[4];
+    double xv[4], vv[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) iv[u] = inv[base + (int64_t)col[j + u] * stride];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      xv[u] = (iv[u] >= 0 && iv[u] < n_in) ? x[iv[u]] : 0.0;
+      vv[u] = val[j + u];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      if (iv[u] >= n_in) out_of_range = true;
+      else if (iv[u] >= 0) s += sgn * xv[u] * vv[u];
+    }
+  }
+  for (; j < end; j++) {
+    const int32_t iv = inv[base + (int64_t)col[j] * stride];
     if (iv >= n_in) {
       out_of_range = true;
     } else if (iv >= 0) {
