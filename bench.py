@@ -586,7 +586,7 @@ SERVE_SNAPSHOTS = ["xxz_L12_m24_jz05_s13.npz", "hubbard_L8_m32_s07.npz", "klm_L6
                    "klm_L6_m24_ixy_s04.npz", "hubbard_L64_m400_s31.npz"]
 
 
-def run_serve(D, rate, duration, seed=0):
+def run_serve(D, rate, duration, seed=0, chunk_ticks=2):
     """Poisson arrivals (rate/s, seeded) for `duration` s: 70 % ED (XXZ L 10..20, Hubbard
     L 6..10, random J_z, h), 30 % superblock snapshots of the reference's runs; each request
     is assembled on the device when it arrives and joins the running engine (continuous
@@ -614,7 +614,7 @@ def run_serve(D, rate, duration, seed=0):
                                             h_z=float(rng.uniform(0, 0.5)))))
         else:
             reqs.append(("sb", int(rng.integers(0, len(sbs)))))
-    E = engine.Engine(chunk_ticks=8)
+    E = engine.Engine(chunk_ticks=chunk_ticks)
     p = engine.default_params()
     t0 = time.perf_counter()
     i, done = 0, {}
@@ -673,6 +673,7 @@ def run_serve(D, rate, duration, seed=0):
                        for kind, v in (("ed", [d[k] for k in d if reqs[k][0] == "ed"]),
                                        ("superblock", [d[k] for k in d if reqs[k][0] == "sb"])) if v}
                 for part, d in (("until_admitted", wait), ("in_engine", service))},
+            "chunk_ticks": chunk_ticks,
             "note": "superblock requests drawn from the reference-run snapshots in tests/golden "
                     "(n up to 144542), assembled on the device at arrival"}
 
@@ -699,6 +700,8 @@ def main():
                          "(each a separate JSON line)")
     ap.add_argument("--rate", type=float, default=200.0)
     ap.add_argument("--duration", type=float, default=10.0)
+    ap.add_argument("--chunk-ticks", type=int, default=2,
+                    help="serve: engine steps per launched chunk (2: lowest latency of 2/4/8/16 measured)")
     ap.add_argument("--ed-count", type=int, default=256)
     ap.add_argument("--ed-batched-only", action="store_true",
                     help="ed-batch: skip the one-by-one solves (for profiling the engine alone)")
@@ -719,7 +722,7 @@ def main():
         return
     if args.workload == "serve":
         print(json.dumps({"metric": "served solves/s and p50/p99 latency", "workload": "serve",
-                          "result": run_serve(D, args.rate, args.duration)}), flush=True)
+                          "result": run_serve(D, args.rate, args.duration, chunk_ticks=args.chunk_ticks)}), flush=True)
         return
     if args.workload == "basis":
         print(json.dumps({"metric": "superblock basis builds/s", "workload": "basis",
