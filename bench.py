@@ -586,7 +586,7 @@ SERVE_SNAPSHOTS = ["xxz_L12_m24_jz05_s13.npz", "hubbard_L8_m32_s07.npz", "klm_L6
                    "klm_L6_m24_ixy_s04.npz", "hubbard_L64_m400_s31.npz"]
 
 
-def run_serve(D, rate, duration, seed=0, chunk_ticks=2):
+def run_serve(D, rate, duration, seed=0, chunk_ticks=1):
     """Poisson arrivals (rate/s, seeded) for `duration` s: 70 % ED (XXZ L 10..20, Hubbard
     L 6..10, random J_z, h), 30 % superblock snapshots of the reference's runs; each request
     is assembled on the device when it arrives and joins the running engine (continuous
@@ -700,8 +700,8 @@ def main():
                          "(each a separate JSON line)")
     ap.add_argument("--rate", type=float, default=200.0)
     ap.add_argument("--duration", type=float, default=10.0)
-    ap.add_argument("--chunk-ticks", type=int, default=2,
-                    help="serve: engine steps per launched chunk (2: lowest latency of 2/4/8/16 measured)")
+    ap.add_argument("--chunk-ticks", type=int, default=1,
+                    help="serve: engine steps per launched chunk (1: lowest latency of 1/2/4/8/16 measured)")
     ap.add_argument("--ed-count", type=int, default=256)
     ap.add_argument("--ed-batched-only", action="store_true",
                     help="ed-batch: skip the one-by-one solves (for profiling the engine alone)")
