@@ -524,6 +524,16 @@ def run_density(D, reps: int, cpu: bool):
         tr["cpu_baseline"] = {"calls_per_s": round(k / (time.perf_counter() - t0), 2), "cores": 1,
                               "kind": "reference", "sample": f"{k} calls of the reference's own "
                               "DMRG_GET_SYSTEM_INFO_Q2 (LAPACK dsyev), p_threads=1"}
+        t0 = time.perf_counter()
+        k = 0
+        while k < 2 or time.perf_counter() - t0 < 3.0:
+            pr = R.DMRG_GET_SYSTEM_INFO_Q2(q1.ctypes.data, q2.ctypes.data, C.byref(B), v.ctypes.data, 400, 16,
+                                           C.byref(t))
+            R.DMRG_FREE_SYSTEM_INFO(pr)
+            k += 1
+        tr["cpu_baseline_16_threads"] = {"calls_per_s": round(k / (time.perf_counter() - t0), 2), "cores": 16,
+                                         "kind": "reference", "sample": f"{k} calls, p_threads=16 (blocks in "
+                                         "parallel, density matrices and dsyev), LAPACK single-threaded"}
     res["system_info"] = tr
     # renormalisation of one LLLR operator: CRS_CRS_CCS_PRODUCT(T^dagger, O, T) with the kept
     # transformation matrix and a seeded sparse symmetric O (1 % of the entries)

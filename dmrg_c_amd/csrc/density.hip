@@ -186,13 +186,28 @@ void density_blocks(const DMRG_BASIS &B, int32_t b0, int32_t b1, const int32_t *
   ctx().copy_out(out, dout.p, sizeof(double) * out_off.back());
 }
 
-rocblas_handle solver_handle() {
-  static rocblas_handle h = nullptr;
-  if (!h && rocblas_create_handle(&h) != rocblas_status_success)
-    throw Failure(DMRGX_ERR_HIP, "rocblas_create_handle failed");
-  if (rocblas_set_stream(h, ctx().stream) != rocblas_status_success)
-    throw Failure(DMRGX_ERR_HIP, "rocblas_set_stream failed");
-  return h;
+// Side streams for the batched eigensolves (default priority), each with its own rocBLAS
+// handle and workspace, created once per process; the batches of one call run concurrently
+// and the library stream waits for all of them through events.
+constexpr int kSolverStreams = 8;
+struct SolverLane {
+  hipStream_t stream = nullptr;
+  rocblas_handle handle = nullptr;
+  hipEvent_t done = nullptr;
+};
+
+SolverLane &solver_lane(int i) {
+  static SolverLane lanes[kSolverStreams];
+  SolverLane &L = lanes[i];
+  if (!L.stream) {
+    HIPCHK(hipStreamCreateWithFlags(&L.stream, hipStreamNonBlocking));
+    HIPCHK(hipEventCreateWithFlags(&L.done, hipEventDisableTiming));
+    if (rocblas_create_handle(&L.handle) != rocblas_status_success)
+      throw Failure(DMRGX_ERR_HIP, "rocblas_create_handle failed");
+    if (rocblas_set_stream(L.handle, L.stream) != rocblas_status_success)
+      throw Failure(DMRGX_ERR_HIP, "rocblas_set_stream failed");
+  }
+  return L;
 }
 
 // One eigenpair of a block, as DMRG_GET_SYSTEM_INFO_Q2.c:33-46 lists them.
@@ -247,7 +262,6 @@ DMRG_SYSTEM_INFO *system_info(int32_t n_qn, const int32_t *Q1, const int32_t *Q2
     while (p < dim_block[b]) p *= 2;
     buckets[p].push_back(b);
   }
-  rocblas_handle h = solver_handle();
   struct Batch {
     int32_t p;
     std::vector<int32_t> blocks;
@@ -255,35 +269,51 @@ DMRG_SYSTEM_INFO *system_info(int32_t n_qn, const int32_t *Q1, const int32_t *Q2
     DevBuf<rocblas_int> info;
   };
   std::vector<std::unique_ptr<Batch>> batches;
-  for (const auto &kv : buckets) {
-    auto Bt = std::make_unique<Batch>();
-    Bt->p = kv.first;
-    Bt->blocks = kv.second;
-    const int32_t p = Bt->p, cnt = (int32_t)Bt->blocks.size();
-    std::vector<int64_t> so(cnt);
-    std::vector<int32_t> sd(cnt);
-    for (int32_t j = 0; j < cnt; j++) {
-      so[j] = off[Bt->blocks[j]];
-      sd[j] = dim_block[Bt->blocks[j]];
+  // every batch padded on the library stream first
+  {
+    std::vector<std::unique_ptr<DevBuf<int64_t>>> keep_so;
+    std::vector<std::unique_ptr<DevBuf<int32_t>>> keep_sd;
+    for (const auto &kv : buckets) {
+      auto Bt = std::make_unique<Batch>();
+      Bt->p = kv.first;
+      Bt->blocks = kv.second;
+      const int32_t p = Bt->p, cnt = (int32_t)Bt->blocks.size();
+      std::vector<int64_t> so(cnt);
+      std::vector<int32_t> sd(cnt);
+      for (int32_t j = 0; j < cnt; j++) {
+        so[j] = off[Bt->blocks[j]];
+        sd[j] = dim_block[Bt->blocks[j]];
+      }
+      keep_so.push_back(std::make_unique<DevBuf<int64_t>>(cnt));
+      keep_sd.push_back(std::make_unique<DevBuf<int32_t>>(cnt));
+      C.copy_in(keep_so.back()->p, so.data(), sizeof(int64_t) * cnt);
+      C.copy_in(keep_sd.back()->p, sd.data(), sizeof(int32_t) * cnt);
+      Bt->a.resize((int64_t)cnt * p * p);
+      Bt->w.resize((int64_t)cnt * p);
+      Bt->e.resize((int64_t)cnt * p);
+      Bt->info.resize(cnt);
+      k_dm_pad<<<dim3((unsigned)((p * p + 255) / 256), (unsigned)cnt), 256, 0, C.stream>>>(
+          p, keep_so.back()->p, keep_sd.back()->p, dm.p, Bt->a.p);
+      HIPCHK(hipGetLastError());
+      batches.push_back(std::move(Bt));
     }
-    DevBuf<int64_t> dso(cnt);
-    DevBuf<int32_t> dsd(cnt);
-    C.copy_in(dso.p, so.data(), sizeof(int64_t) * cnt);
-    C.copy_in(dsd.p, sd.data(), sizeof(int32_t) * cnt);
-    Bt->a.resize((int64_t)cnt * p * p);
-    Bt->w.resize((int64_t)cnt * p);
-    Bt->e.resize((int64_t)cnt * p);
-    Bt->info.resize(cnt);
-    k_dm_pad<<<dim3((unsigned)((p * p + 255) / 256), (unsigned)cnt), 256, 0, C.stream>>>(p, dso.p, dsd.p, dm.p,
-                                                                                        Bt->a.p);
-    HIPCHK(hipGetLastError());
-    if (rocsolver_dsyevd_strided_batched(h, rocblas_evect_original, rocblas_fill_upper, p, Bt->a.p, p,
-                                         (rocblas_stride)p * p, Bt->w.p, p, Bt->e.p, p, Bt->info.p,
-                                         cnt) != rocblas_status_success)
-      throw Failure(DMRGX_ERR_HIP, "rocsolver_dsyevd_strided_batched failed");
-    // the offset and size arrays go back to the allocation cache after the padding kernel
+    // the offset and size arrays go back to the allocation cache after the padding kernels
     HIPCHK(hipStreamSynchronize(C.stream));
-    batches.push_back(std::move(Bt));
+  }
+  // then the batched eigensolves side by side, batch i on side stream i mod kSolverStreams
+  for (size_t i = 0; i < batches.size(); i++) {
+    Batch &Bt = *batches[i];
+    SolverLane &L = solver_lane((int)(i % kSolverStreams));
+    const int32_t p = Bt.p, cnt = (int32_t)Bt.blocks.size();
+    if (rocsolver_dsyevd_strided_batched(L.handle, rocblas_evect_original, rocblas_fill_upper, p, Bt.a.p, p,
+                                         (rocblas_stride)p * p, Bt.w.p, p, Bt.e.p, p, Bt.info.p,
+                                         cnt) != rocblas_status_success) {
+      // let the solves already queued finish before their buffers are released
+      for (size_t k = 0; k < std::min(i, (size_t)kSolverStreams); k++) (void)hipStreamSynchronize(solver_lane((int)k).stream);
+      throw Failure(DMRGX_ERR_HIP, "rocsolver_dsyevd_strided_batched failed");
+    }
+    HIPCHK(hipEventRecord(L.done, L.stream));
+    HIPCHK(hipStreamWaitEvent(C.stream, L.done, 0));
   }
   // per block: eigenvalues ascending (hw, at row_base) and eigenvectors as rows (hv, at off)
   std::vector<double> hw(dim_LLLR), hv(off.back());
