@@ -696,6 +696,12 @@ int dmrgx_get_system_info(int32_t n_qn, const int32_t *Q_Number1, const int32_t 
 /* Frees what the three calls above return, as DMRG_FREE_SYSTEM_INFO (include/dmrg.h:357,
  * dmrg/DMRG_FREE_SYSTEM_INFO.c) does; the reference's own function frees it as well. */
 void dmrgx_free_system_info(DMRG_SYSTEM_INFO *Info);
+/* include/SML.h:181 (sml/CRS_CRS_CCS_PRODUCT.c:6-124) -- Out = M1 (M2 M3), M3 in CCS, entries
+ * of magnitude <= 1e-15 dropped after each of the two passes, Out's columns ascending; the
+ * renormalisation T^dagger O T of DMRG_TRANS_MAT_*.  Bit-identical to the reference.  Work and
+ * Work_Vec are accepted and not used (only Work->max_col is checked, as the reference does);
+ * exit(1) with the reference's conditions (dimensions that do not chain, Out too small). */
+void CRS_CRS_CCS_PRODUCT(CRS1 *M1, CRS1 *M2, CCS1 *M3, CRS1 *Out, CCS1 *Work, double *Work_Vec);
 
 /* ---- exact diagonalization on the device
  * A problem is a sorted basis of base-d codes (site s is digit s, site 0 least
