@@ -48,7 +48,9 @@ CASES = {
     "klm_L6_m32": ("snap_klm", "L=6 N=4 m=32 sweep=1 t=-1 J=2 h_z=0.1", "all", True),
     "klm_L6_m24_ixy": ("snap_klm", "L=6 N=6 m=24 sweep=1 t=-1 J=1.5 I_xy=0.5 I_z=0.3 h_z=0.05", [2, 4], True),
 }
-# per-step energies only
+# per-step energies (energies.txt) and the program's own result files (result/: expectation
+# values and correlation functions at the final sweeps' centre, entanglement entropy,
+# truncation errors, energies), copied to <name>_result/ -- XXZ L=32 m=64, 4 sweeps
 ENERGY_RUNS = {
     "xxz_L32_m64_sw4": ("snap_xxz", "L=32 m=64 sweep=4 J_xy=1.0 J_z=1.0 h_z=0.0 diag_acc=1e-12"),
 }
@@ -77,8 +79,14 @@ QN_CASES = {
     "klm_L6_m32": ("snap_klm", "L=6 N=4 m=32 sweep=1 t=-1 J=2 h_z=0.1", [1, 3, 5]),
 }
 # bench input (not a parity fixture): Hubbard L=64 N=64 m=400 U=4 h=0.1, last infinite step
+# KLM L=48 N=36 m=800 (t=-1, J=2, h_z drawn from numpy default_rng(0).uniform(0, 0.3)), the
+# last infinite step (48 sites); Lanczos_Slow keeps the reference's memory at 3 vectors
 BIG = {
     "hubbard_L64_m400": ("snap_hubbard", "L=64 N=64 m=400 sweep=3 t1=-1 U=4 h_z=0.1", 31),
+    "klm_L48_m800": ("snap_klm", "L=48 N=36 m=800 sweep=0 t=-1 J=2 h_z=0.19108850619643628 "
+                     "method=Lanczos_Slow", 23),
+    # served-stream superblock at the low end of m=200..1000: XXZ L=64 m=300, last infinite step
+    "xxz_L64_m300": ("snap_xxz", "L=64 m=300 sweep=0 J_xy=1.0 J_z=1.0 h_z=0.0", 31),
 }
 
 
@@ -141,6 +149,7 @@ def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--only", default=None)
     ap.add_argument("--big", action="store_true")
+    ap.add_argument("--big-only", default=None, help="with --big: only this BIG case")
     a = ap.parse_args()
     for name, (prog, args, steps, keep_vec) in CASES.items():
         if a.only and a.only != name:
@@ -162,10 +171,13 @@ def main() -> None:
         if a.only and a.only != name:
             continue
         wd = tempfile.mkdtemp(prefix="golden_")
-        run(prog, args, wd, "dump=")
+        run(prog, args, wd, "dump= dump_snap=0 dump_vec=0")
         shutil.copy(os.path.join(wd, "energies.txt"), os.path.join(HERE, f"{name}_energies.txt"))
+        dst = os.path.join(HERE, f"{name}_result")
+        shutil.rmtree(dst, ignore_errors=True)
+        shutil.copytree(os.path.join(wd, "result"), dst, ignore=shutil.ignore_patterns("log.txt"))
         shutil.rmtree(wd)
-        print("wrote", f"{name}_energies.txt")
+        print("wrote", f"{name}_energies.txt", f"{name}_result/")
     for name, args in ED_CASES.items():
         if a.only and a.only != name:
             continue
@@ -180,6 +192,8 @@ def main() -> None:
         shutil.rmtree(wd)
     if a.big:
         for name, (prog, args, step) in BIG.items():
+            if a.big_only and a.big_only != name:
+                continue
             wd = tempfile.mkdtemp(prefix="golden_")
             run(prog, args, wd, f"dump={step} dump_ham=0 dump_qn=1 stop_after={step}")
             convert_qn(wd, name, [step])   # qn_<name>_s<step>.npz: the same step's basis inputs
