@@ -196,18 +196,51 @@ struct SolverLane {
   hipEvent_t done = nullptr;
 };
 
+// A lane is published (stored in the table) only once its stream, event and handle all
+// exist and the handle is bound to the stream; a failure part way leaves the slot empty
+// and releases what was made.
 SolverLane &solver_lane(int i) {
   static SolverLane lanes[kSolverStreams];
   SolverLane &L = lanes[i];
-  if (!L.stream) {
-    HIPCHK(hipStreamCreateWithFlags(&L.stream, hipStreamNonBlocking));
-    HIPCHK(hipEventCreateWithFlags(&L.done, hipEventDisableTiming));
-    if (rocblas_create_handle(&L.handle) != rocblas_status_success)
-      throw Failure(DMRGX_ERR_HIP, "rocblas_create_handle failed");
-    if (rocblas_set_stream(L.handle, L.stream) != rocblas_status_success)
-      throw Failure(DMRGX_ERR_HIP, "rocblas_set_stream failed");
+  if (L.handle) return L;
+  SolverLane T;
+  auto release = [&T]() {
+    if (T.handle) (void)rocblas_destroy_handle(T.handle);
+    if (T.done) (void)hipEventDestroy(T.done);
+    if (T.stream) (void)hipStreamDestroy(T.stream);
+  };
+  if (hipStreamCreateWithFlags(&T.stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&T.done, hipEventDisableTiming) != hipSuccess) {
+    release();
+    throw Failure(DMRGX_ERR_HIP, "solver stream creation failed");
   }
+  if (rocblas_create_handle(&T.handle) != rocblas_status_success) {
+    T.handle = nullptr;
+    release();
+    throw Failure(DMRGX_ERR_HIP, "rocblas_create_handle failed");
+  }
+  if (rocblas_set_stream(T.handle, T.stream) != rocblas_status_success) {
+    release();
+    throw Failure(DMRGX_ERR_HIP, "rocblas_set_stream failed");
+  }
+  L = T;
   return L;
+}
+
+// Waits for every side stream used so far when it goes out of scope, whatever the exit
+// path, so that no batched solve still runs on buffers the caller's scope releases.
+struct LaneGuard {
+  size_t used = 0;
+  ~LaneGuard() {
+    for (size_t k = 0; k < std::min(used, (size_t)kSolverStreams); k++)
+      (void)hipStreamSynchronize(solver_lane((int)k).stream);
+  }
+};
+
+int32_t padded_block_size(int32_t d) {
+  if (d <= 16) return 16;
+  if (d <= 512) return (d + 31) / 32 * 32;
+  return (d + 63) / 64 * 64;
 }
 
 // One eigenpair of a block, as DMRG_GET_SYSTEM_INFO_Q2.c:33-46 lists them.
@@ -232,13 +265,20 @@ DMRG_SYSTEM_INFO *system_info(int32_t n_qn, const int32_t *Q1, const int32_t *Q2
   auto q2 = [&](int32_t s) { return n_qn == 2 ? Q2[s] : 0; };
   // DMRG_GET_DENSITY_MATRIX_INFO_Q1/Q2: order checks, then the blocks in ascending
   // (Q_Number1, Q_Number2) order
+  // (DMRG_GET_DENSITY_MATRIX_INFO_Q2.c:18-35): pairwise order, and the number of runs of
+  // equal (Q1, Q2) must equal the number of distinct pairs (:69-74), so that each block's
+  // states are contiguous
+  int32_t runs = 1;
   for (int32_t s = 0; s + 1 < dim_LLLR; s++) {
     const bool bad = n_qn == 1 ? Q1[s] > Q1[s + 1]
                                : ((Q1[s] > Q1[s + 1] && q2(s) == q2(s + 1)) || (Q1[s] == Q1[s + 1] && q2(s) > q2(s + 1)));
     if (bad) throw Failure(DMRGX_ERR_ARG, "LLLR quantum numbers out of order");
+    if (Q1[s] != Q1[s + 1] || q2(s) != q2(s + 1)) runs++;
   }
   std::map<std::pair<int32_t, int32_t>, int32_t> count;
   for (int32_t s = 0; s < dim_LLLR; s++) count[{Q1[s], q2(s)}]++;
+  if ((size_t)runs != count.size())
+    throw Failure(DMRGX_ERR_ARG, "LLLR quantum-number blocks not contiguous (runs != distinct pairs)");
   std::vector<int32_t> dim_block, qb1, qb2;
   for (const auto &kv : count) {
     dim_block.push_back(kv.second);
@@ -254,14 +294,12 @@ DMRG_SYSTEM_INFO *system_info(int32_t n_qn, const int32_t *Q1, const int32_t *Q2
   Context &C = ctx();
   std::vector<int64_t> row_base(nb + 1, 0);
   for (int32_t b = 0; b < nb; b++) row_base[b + 1] = row_base[b] + dim_block[b];
-  // eigenpairs by batches of equal padded size (powers of two from 16): one batched solve
-  // per size instead of one solve per block
+  // eigenpairs by batches of equal padded size: one batched solve per size instead of one
+  // solve per block; sizes padded to 16, then to multiples of 32 up to 512 and of 64 above,
+  // so that a block just past a size step costs at most ~25 % more work, not the 8x of a
+  // power-of-two step
   std::map<int32_t, std::vector<int32_t>> buckets;
-  for (int32_t b = 0; b < nb; b++) {
-    int32_t p = 16;
-    while (p < dim_block[b]) p *= 2;
-    buckets[p].push_back(b);
-  }
+  for (int32_t b = 0; b < nb; b++) buckets[padded_block_size(dim_block[b])].push_back(b);
   struct Batch {
     int32_t p;
     std::vector<int32_t> blocks;
@@ -301,17 +339,16 @@ DMRG_SYSTEM_INFO *system_info(int32_t n_qn, const int32_t *Q1, const int32_t *Q2
     HIPCHK(hipStreamSynchronize(C.stream));
   }
   // then the batched eigensolves side by side, batch i on side stream i mod kSolverStreams
+  LaneGuard lanes_used;   // declared after batches: destroyed (streams drained) before them
   for (size_t i = 0; i < batches.size(); i++) {
     Batch &Bt = *batches[i];
     SolverLane &L = solver_lane((int)(i % kSolverStreams));
+    lanes_used.used = i + 1;
     const int32_t p = Bt.p, cnt = (int32_t)Bt.blocks.size();
     if (rocsolver_dsyevd_strided_batched(L.handle, rocblas_evect_original, rocblas_fill_upper, p, Bt.a.p, p,
                                          (rocblas_stride)p * p, Bt.w.p, p, Bt.e.p, p, Bt.info.p,
-                                         cnt) != rocblas_status_success) {
-      // let the solves already queued finish before their buffers are released
-      for (size_t k = 0; k < std::min(i, (size_t)kSolverStreams); k++) (void)hipStreamSynchronize(solver_lane((int)k).stream);
+                                         cnt) != rocblas_status_success)
       throw Failure(DMRGX_ERR_HIP, "rocsolver_dsyevd_strided_batched failed");
-    }
     HIPCHK(hipEventRecord(L.done, L.stream));
     HIPCHK(hipStreamWaitEvent(C.stream, L.done, 0));
   }

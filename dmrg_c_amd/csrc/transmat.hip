@@ -40,24 +40,38 @@ __global__ __launch_bounds__(256) void k_ccs_to_dense(int32_t cols, int32_t rows
 
 // out[i][j] = sum over the entries e of row i of A (stored order) of B[col_e][j] * A_e,
 // dropped to 0 when |sum| <= 1e-15; with skip_zero, terms whose B value is 0 are left out.
+// With kept set, the number of kept entries other than the last one, (rows-1, cols-1), is
+// added to *kept (the count the reference's capacity check sees, CRS_CRS_CCS_PRODUCT.c:56).
 template <bool kSkipZero>
 __global__ __launch_bounds__(256) void k_crs_dense(int32_t rows, int32_t cols, const int64_t *__restrict__ rptr,
                                                    const int32_t *__restrict__ cidx, const double *__restrict__ aval,
-                                                   const double *__restrict__ B, double *__restrict__ out) {
+                                                   const double *__restrict__ B, double *__restrict__ out,
+                                                   unsigned long long *kept) {
   const int32_t j = blockIdx.x * 256 + threadIdx.x;
-  if (j >= cols) return;
-  for (int32_t i = blockIdx.y; i < rows; i += gridDim.y) {
-    double s = 0.0;
-    for (int64_t e = rptr[i]; e < rptr[i + 1]; e++) {
-      const double b = B[(int64_t)cidx[e] * cols + j];
-      if (kSkipZero) {
-        if (b != 0.0) s = s + b * aval[e];
-      } else {
-        s = s + b * aval[e];
+  unsigned int mine = 0;
+  if (j < cols) {
+    for (int32_t i = blockIdx.y; i < rows; i += gridDim.y) {
+      double s = 0.0;
+      for (int64_t e = rptr[i]; e < rptr[i + 1]; e++) {
+        const double b = B[(int64_t)cidx[e] * cols + j];
+        if (kSkipZero) {
+          if (b != 0.0) s = s + b * aval[e];
+        } else {
+          s = s + b * aval[e];
+        }
       }
+      const bool keep = std::fabs(s) > kDropBelow;
+      out[(int64_t)i * cols + j] = keep ? s : 0.0;
+      if (keep && !(i == rows - 1 && j == cols - 1)) mine++;
     }
-    out[(int64_t)i * cols + j] = std::fabs(s) > kDropBelow ? s : 0.0;
   }
+  if (!kept) return;   // uniform over the launch
+  __shared__ unsigned int cnt;
+  if (threadIdx.x == 0) cnt = 0;
+  __syncthreads();
+  if (mine) atomicAdd(&cnt, mine);
+  __syncthreads();
+  if (threadIdx.x == 0 && cnt) atomicAdd(kept, (unsigned long long)cnt);
 }
 
 void check_crs(const CRS1 *M, const char *what) {
@@ -137,19 +151,35 @@ void CRS_CRS_CCS_PRODUCT(CRS1 *M1, CRS1 *M2, CCS1 *M3, CRS1 *Out, CCS1 *Work, do
       C.copy_in(dv3.p, M3->Val, sizeof(double) * nnz3);
     }
     std::vector<double> ho(no > 0 ? no : 1);
+    unsigned long long w_kept = 0;
     if (k > 0 && n1 > 0) {
+      DevBuf<unsigned long long> dkept(1);
+      HIPCHK(hipMemsetAsync(dkept.p, 0, sizeof(unsigned long long), st));
       if (nd3 > 0) HIPCHK(hipMemsetAsync(d3.p, 0, sizeof(double) * nd3, st));
       const unsigned gx = (unsigned)((k + 255) / 256);
       k_ccs_to_dense<<<gx, 256, 0, st>>>(k, n3, p3.p, dr3.p, dv3.p, d3.p);
       // grid y at most 65535; the kernels loop over the rows beyond it
       if (n2 > 0)
         k_crs_dense<false><<<dim3(gx, (unsigned)std::min(n2, 65535)), 256, 0, st>>>(n2, k, r2.p, dc2.p, dv2.p, d3.p,
-                                                                                   w.p);
-      k_crs_dense<true><<<dim3(gx, (unsigned)std::min(n1, 65535)), 256, 0, st>>>(n1, k, r1.p, dc1.p, dv1.p, w.p, o.p);
+                                                                                   w.p, dkept.p);
+      k_crs_dense<true><<<dim3(gx, (unsigned)std::min(n1, 65535)), 256, 0, st>>>(n1, k, r1.p, dc1.p, dv1.p, w.p, o.p,
+                                                                                nullptr);
       HIPCHK(hipGetLastError());
       C.copy_out(ho.data(), o.p, sizeof(double) * no);
+      C.copy_out(&w_kept, dkept.p, sizeof(unsigned long long));
     } else {
       std::fill(ho.begin(), ho.end(), 0.0);
+    }
+    // The reference checks the kept count against the capacity before every entry it
+    // evaluates (CRS_CRS_CCS_PRODUCT.c:56 for Work, :101 for Out), so it exits when the
+    // entries kept before the last evaluated position reach max_val, even if nothing more
+    // is stored after them.
+    if (n2 > 0 && k > 0 && (long long)w_kept >= (long long)Work->max_val)
+      throw Failure(DMRGX_ERR_ARG, "Work has too few elements");
+    if (n1 > 0 && k > 0) {
+      long long out_kept = 0;
+      for (int64_t t = 0; t + 1 < no; t++) out_kept += ho[t] != 0.0;
+      if (out_kept >= (long long)Out->max_val) throw Failure(DMRGX_ERR_ARG, "Out has too few elements");
     }
     // compaction into Out, columns ascending (sml/CRS_CRS_CCS_PRODUCT.c:98-121)
     int64_t e = 0;

@@ -699,8 +699,11 @@ void dmrgx_free_system_info(DMRG_SYSTEM_INFO *Info);
 /* include/SML.h:181 (sml/CRS_CRS_CCS_PRODUCT.c:6-124) -- Out = M1 (M2 M3), M3 in CCS, entries
  * of magnitude <= 1e-15 dropped after each of the two passes, Out's columns ascending; the
  * renormalisation T^dagger O T of DMRG_TRANS_MAT_*.  Bit-identical to the reference.  Work and
- * Work_Vec are accepted and not used (only Work->max_col is checked, as the reference does);
- * exit(1) with the reference's conditions (dimensions that do not chain, Out too small). */
+ * Work_Vec are not written (the intermediate product stays on the device); exit(1) with the
+ * reference's conditions: dimensions that do not chain, Work->max_col <= M3's columns,
+ * Out->max_row <= M1's rows, and the kept entries of M2 M3 (or of Out) before the last
+ * position it evaluates reaching Work->max_val (or Out->max_val) -- so an exactly full
+ * Work or Out whose last position is dropped exits, as in the reference. */
 void CRS_CRS_CCS_PRODUCT(CRS1 *M1, CRS1 *M2, CCS1 *M3, CRS1 *Out, CCS1 *Work, double *Work_Vec);
 
 /* ---- exact diagonalization on the device

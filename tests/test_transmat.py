@@ -9,6 +9,8 @@ Cases: random sparse matrices (rows in random column order, a repeated column, e
 small enough to be dropped), and T^dagger O T with the transformation matrix of a
 reference run's truncation."""
 import ctypes as C
+import subprocess
+import sys
 import os
 
 import numpy as np
@@ -64,8 +66,10 @@ def oracle(M1, M2, M3, n1, n2, n3, k):
     return row, col[:nnz], val[:nnz]
 
 
-def call(fn, M1, M2, M3, n1, n2, n3, k):
-    """The reference's signature: CRS_CRS_CCS_PRODUCT(M1, M2, M3, Out, Work, Work_Vec)."""
+def call(fn, M1, M2, M3, n1, n2, n3, k, out_cap=None, work_cap=None, work_out=None):
+    """The reference's signature: CRS_CRS_CCS_PRODUCT(M1, M2, M3, Out, Work, Work_Vec).
+    out_cap / work_cap: Out->max_val / Work->max_val (default: dense size); work_out, a
+    list, receives Work's (col pointers, rows, values) as the function left them."""
     import dmrg_c_amd as D
 
     def crs(r, c, v, n, m):
@@ -77,15 +81,17 @@ def call(fn, M1, M2, M3, n1, n2, n3, k):
     cap = max(n1 * k, 1)
     orow, ocol, oval = np.zeros(n1 + 1, np.int64), np.zeros(cap, np.int32), np.zeros(cap)
     out = crs(orow, ocol, oval, n1, k)
-    out.max_val = cap
+    out.max_val = cap if out_cap is None else out_cap
     wcap = max(n2 * k, 1)
     wcol, wrow, wval = np.zeros(k + 1, np.int64), np.zeros(wcap, np.int32), np.zeros(wcap)
-    work = D.CCS1(wcap, k + 1, 0, 0, wcol.ctypes.data_as(C.POINTER(C.c_long)),
+    work = D.CCS1(wcap if work_cap is None else work_cap, k + 1, 0, 0, wcol.ctypes.data_as(C.POINTER(C.c_long)),
                   wrow.ctypes.data_as(C.POINTER(C.c_int)), wval.ctypes.data_as(C.POINTER(C.c_double)))
     wvec = np.zeros(max(n2, n3, 1))
     fn.argtypes = [C.POINTER(D.CRS1), C.POINTER(D.CRS1), C.POINTER(D.CCS1), C.POINTER(D.CRS1), C.POINTER(D.CCS1),
                    C.c_void_p]
     fn(C.byref(m1), C.byref(m2), C.byref(m3), C.byref(out), C.byref(work), wvec.ctypes.data)
+    if work_out is not None:
+        work_out[:] = [wcol, wrow, wval]
     assert out.row_dim == n1 and out.col_dim == k
     nnz = int(orow[n1])
     return orow, ocol[:nnz], oval[:nnz]
@@ -166,3 +172,64 @@ def test_device_matches_oracle(name):
     want = oracle(M1, M2, M3, *dims)
     for a, b in zip(got, want):
         np.testing.assert_array_equal(a, b)
+
+
+# ------------------------------------------------------------------ capacity checks
+# The reference checks the kept count against Work->max_val / Out->max_val before every
+# position it evaluates (sml/CRS_CRS_CCS_PRODUCT.c:56,101), so it exits as soon as the
+# entries kept before the last evaluated position reach the capacity -- also when the last
+# position is dropped and the result would have fit exactly.  Each call runs in a child
+# process (the drop-in exits); the capacities at the boundary are taken from the
+# reference's own Work and Out for a case whose last positions are dropped.
+CAP_CASE = "random0"
+
+
+def _cap_boundaries():
+    R = C.CDLL(REF_LIB)
+    M1, M2, M3, (n1, n2, n3, k) = CASES[CAP_CASE]
+    w = []
+    orow, ocol, _ = call(R.CRS_CRS_CCS_PRODUCT, M1, M2, M3, n1, n2, n3, k, work_out=w)
+    wcol, wrow, _ = w
+    w_nnz, o_nnz = int(wcol[k]), int(orow[n1])
+    w_last = w_nnz > wcol[k - 1] and wrow[w_nnz - 1] == n2 - 1     # W(n2-1, k-1) kept
+    o_last = o_nnz > orow[n1 - 1] and ocol[o_nnz - 1] == k - 1     # Out(n1-1, k-1) kept
+    return w_nnz - int(w_last), o_nnz - int(o_last), w_nnz, o_nnz
+
+
+def _run_child(which, out_cap, work_cap):
+    code = (f"import sys; sys.path[:0] = [{os.path.dirname(os.path.abspath(__file__))!r}, {REPO!r}]\n"
+            "import ctypes as C, test_transmat as T\n"
+            f"M1, M2, M3, dims = T.CASES[{CAP_CASE!r}]\n"
+            + ("fn = C.CDLL(T.REF_LIB).CRS_CRS_CCS_PRODUCT\n" if which == "ref"
+               else "import dmrg_c_amd as D; fn = D.lib().CRS_CRS_CCS_PRODUCT\n")
+            + f"T.call(fn, M1, M2, M3, *dims, out_cap={out_cap}, work_cap={work_cap})\n")
+    env = dict(os.environ, DMRGX_SML_OUT="0")
+    return subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, timeout=120).returncode
+
+
+def _cap_cases():
+    w_before, o_before, w_nnz, o_nnz = _cap_boundaries()
+    big = 10 ** 6
+    # (Out->max_val, Work->max_val, exits?)
+    return [(o_before, big, True), (o_before + 1, big, False), (big, w_before, True), (big, w_before + 1, False),
+            (o_nnz, big, o_nnz == o_before), (big, w_nnz, w_nnz == w_before)]
+
+
+def test_reference_capacity_rule():
+    """Pins the rule on the reference itself (and that the case exercises a dropped last
+    position for both passes, where the rule differs from 'capacity >= kept entries')."""
+    if not os.path.exists(REF_LIB):
+        pytest.skip("oracle/_ref/libdmrgref.so not built (make -C oracle ref)")
+    w_before, o_before, w_nnz, o_nnz = _cap_boundaries()
+    assert w_nnz == w_before and o_nnz == o_before, "case should drop the last position of W and Out"
+    for out_cap, work_cap, exits in _cap_cases():
+        assert (_run_child("ref", out_cap, work_cap) != 0) == exits, (out_cap, work_cap, exits)
+
+
+@pytest.mark.gpu
+def test_device_capacity_rule():
+    if not os.path.exists(REF_LIB):
+        pytest.skip("oracle/_ref/libdmrgref.so not built (make -C oracle ref)")
+    for out_cap, work_cap, exits in _cap_cases():
+        print(f"crs_crs_ccs capacity out={out_cap} work={work_cap}", flush=True)
+        assert (_run_child("dev", out_cap, work_cap) != 0) == exits, (out_cap, work_cap, exits)

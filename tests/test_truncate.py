@@ -168,3 +168,21 @@ def test_device_order_error():
     q1, q2 = qns(S)
     with pytest.raises(D.DmrgxError, match="out of order"):
         D.system_info(B, S.n_qn, q1[::-1].copy(), None if q2 is None else q2[::-1].copy(), S.vec, 10)
+
+
+@pytest.mark.gpu
+def test_device_blocks_not_contiguous_error():
+    """(0,0)... (1,1)... (0,0)...: every neighbouring pair is in order, but the (0,0) states
+    form two runs, so the run count (3) differs from the distinct pairs (2); the reference
+    exits at DMRG_GET_DENSITY_MATRIX_INFO_Q2.c:69-74, the library returns an argument error."""
+    import dmrg_c_amd as D
+    S = next(s for s in SUBS if s.n_qn == 2)
+    B = S.basis_struct(D)
+    m = S.dim_LLLR
+    assert m >= 3
+    q1 = np.zeros(m, np.int32)
+    q2 = np.zeros(m, np.int32)
+    q1[m // 3:2 * m // 3] = 1
+    q2[m // 3:2 * m // 3] = 1
+    with pytest.raises(D.DmrgxError, match="not contiguous"):
+        D.system_info(B, 2, q1, q2, S.vec, 10)
