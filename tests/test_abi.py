@@ -35,10 +35,10 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_no_gpu_here_is_reported_not_faked():
-    """Without a GPU the library must refuse (no CPU fallback)."""
-    import torch
-    if torch.cuda.is_available():
-        pytest.skip("GPU present")
+    """Without a GPU the library must refuse (no CPU fallback).  The GPU's absence is read
+    from the ROCm kernel driver's device node, not from the library under test."""
+    if os.path.exists("/dev/kfd"):
+        pytest.skip("ROCm device node present")
     assert D.lib().dmrgx_init(-1) == -6  # DMRGX_ERR_NOGPU
 
 
