@@ -716,4 +716,18 @@ int dmrgx_device_free(void *p) {
   return run_status([&] { HIPCHK(hipFree(p)); });
 }
 
+int dmrgx_copy_to_device(void *dst_dev, const void *src_host, int64_t bytes) {
+  if ((!dst_dev || !src_host) && bytes > 0) return DMRGX_ERR_ARG;
+  return run_status([&] {
+    if (bytes > 0) HIPCHK(hipMemcpy(dst_dev, src_host, (size_t)bytes, hipMemcpyHostToDevice));
+  });
+}
+
+int dmrgx_copy_to_host(void *dst_host, const void *src_dev, int64_t bytes) {
+  if ((!dst_host || !src_dev) && bytes > 0) return DMRGX_ERR_ARG;
+  return run_status([&] {
+    if (bytes > 0) HIPCHK(hipMemcpy(dst_host, src_dev, (size_t)bytes, hipMemcpyDeviceToHost));
+  });
+}
+
 }  // extern "C"

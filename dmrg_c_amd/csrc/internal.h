@@ -544,6 +544,12 @@ struct Reduce {              // deterministic two-kernel reduction workspace
 };
 
 int pick_lanes(int32_t n, int64_t nnz);
+// CONVERGE_CHECK's product over caller-held device CSR arrays (default schedule of a matrix
+// of these dimensions), block partials at partial (residual_l1_default_blocks(n, nnz)
+// doubles), sum_i |eig v_i - (H v)_i| at *out; allocates nothing.
+int residual_l1_default_blocks(int32_t n, int64_t nnz);
+void launch_residual_l1_into(int32_t n, int64_t nnz, const int64_t *row, const int32_t *col, const double *val,
+                             const double *v, double eig, double *out, double *partial, hipStream_t s);
 // lanes from the mean row length, the default workgroup cap, the dictionary for large A
 void default_schedule(dmrgx_matrix &A);
 int spmv_blocks(int32_t n, int lanes, int grid_cap);

@@ -282,21 +282,18 @@ struct EpiStoreDot {
 // Launches the product with its epilogue; with fin the reduction's finish kernel follows,
 // without it the caller's next kernel sums the partials.  Returns the number of blocks
 // (= partials).
+// The launch with the block partials at part (spmv_blocks(n, lanes, grid_cap) doubles when
+// the epilogue reduces); timed = false keeps it out of the live SpMV timing.
 template <class Rows, class Epi>
-int apply_rows(int32_t n, int lanes, int grid_cap, const Rows &rows, const double *x, const Epi &epi, Reduce *R,
-               hipStream_t s, int xcd = 0, bool fin = true) {
+int apply_rows_into(int32_t n, int lanes, int grid_cap, const Rows &rows, const double *x, const Epi &epi,
+                    double *part, hipStream_t s, int xcd, bool fin, bool timed) {
   const int blocks = spmv_blocks(n, lanes, grid_cap);
   if (blocks == 0) return 0;
-  double *part = nullptr;
-  if (Epi::kReduce) {
-    R->ensure(blocks);
-    part = R->partial.p;
-  }
   Context &C = ctx();
   std::pair<hipEvent_t, hipEvent_t> *ev = nullptr;
   // live timing brackets only products that always run in full (a skipped launch would
   // count its bytes over a few microseconds)
-  if (Rows::kTimed && !MaySkip<Epi>::value && C.prof_take()) {
+  if (timed && Rows::kTimed && !MaySkip<Epi>::value && C.prof_take()) {
     ev = C.prof_slot();
     HIPCHK(hipEventRecord(ev->first, s));
     C.prof_bytes += rows.alg_bytes(n);
@@ -315,6 +312,19 @@ int apply_rows(int32_t n, int lanes, int grid_cap, const Rows &rows, const doubl
     HIPCHK(hipGetLastError());
   }
   return blocks;
+}
+
+template <class Rows, class Epi>
+int apply_rows(int32_t n, int lanes, int grid_cap, const Rows &rows, const double *x, const Epi &epi, Reduce *R,
+               hipStream_t s, int xcd = 0, bool fin = true) {
+  const int blocks = spmv_blocks(n, lanes, grid_cap);
+  if (blocks == 0) return 0;
+  double *part = nullptr;
+  if (Epi::kReduce) {
+    R->ensure(blocks);
+    part = R->partial.p;
+  }
+  return apply_rows_into(n, lanes, grid_cap, rows, x, epi, part, s, xcd, fin, true);
 }
 
 template <class Epi>
@@ -967,6 +977,17 @@ void launch_rayleigh(const Operator &op, const double *v, double *out, Reduce &R
 void launch_residual_l1(const Operator &op, const double *v, double eig, double *out, Reduce &R,
                         hipStream_t s) {
   apply_op(op, false, v, EpiResidualL1{v, out, eig}, &R, s);
+}
+
+int residual_l1_default_blocks(int32_t n, int64_t nnz) { return spmv_blocks(n, pick_lanes(n, nnz), kGridCap); }
+
+void launch_residual_l1_into(int32_t n, int64_t nnz, const int64_t *row, const int32_t *col, const double *val,
+                             const double *v, double eig, double *out, double *partial, hipStream_t s) {
+  // default_schedule's choices for a CSR matrix of these dimensions (lanes from the mean row
+  // length, kGridCap workgroups, eight loads in flight, CSR), so that the result equals
+  // launch_residual_l1 on a dmrgx_matrix of the same arrays bit for bit
+  apply_rows_into(n, pick_lanes(n, nnz), kGridCap, CsrRows<double, kLoadDeep>{row, col, val, nnz}, v,
+                  EpiResidualL1{v, out, eig}, partial, s, 0, true, false);
 }
 
 void launch_cg_init(const Operator &op, double shift, const double *nrm2, const double *x_in,

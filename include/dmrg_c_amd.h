@@ -833,6 +833,23 @@ int dmrgx_profile_end(double *spmv_ms, int64_t *spmv_launches, double *spmv_alg_
 int dmrgx_copy_device(const double *src, double *dst, int64_t count);
 int dmrgx_device_alloc(int64_t bytes, void **out);
 int dmrgx_device_free(void *p);
+/* Synchronous copies between caller host memory and device memory (bytes). */
+int dmrgx_copy_to_device(void *dst_dev, const void *src_host, int64_t bytes);
+int dmrgx_copy_to_host(void *dst_host, const void *src_dev, int64_t bytes);
+
+/* ---- allocation-free entry point
+ * No counterpart in the reference: it is for callers that cannot allocate device memory
+ * (code under HIP graph capture, serving loops and harnesses that forbid hipMalloc).  The
+ * computation is CONVERGE_CHECK's (sml/CONVERGE_CHECK.c:7-23), *out = sum_i |eig_val v_i -
+ * (H v)_i|, over a full CRS (int64 row pointers, int32 columns, FP64 values; n rows, nnz
+ * entries) and a vector that are already in device memory; *out is host memory.  It calls
+ * no hipMalloc/hipFree and no allocating library routine: the block partials live in a
+ * fixed file-scope device array of 4128 doubles (33 KB) per GPU, held until the process
+ * ends.  Same kernels and schedule as CONVERGE_CHECK on the same arrays, so the same bits.
+ * Calls that overlap on one GPU share that array and race; the call returns after the
+ * result has reached *out.  DMRGX_ERR_ARG for negative sizes or NULL arrays. */
+int dmrgx_converge_check_noalloc(int32_t n, int64_t nnz, const int64_t *row_dev, const int32_t *col_dev,
+                                 const double *val_dev, const double *vec_dev, double eig_val, double *out);
 
 #ifdef __cplusplus
 }
