@@ -418,6 +418,7 @@ int dmrgx_matrix_set_schedule(dmrgx_matrix *A, int32_t lanes, int32_t grid_cap, 
   if (!A || lanes < 1 || lanes > 64 || (lanes & (lanes - 1)) || grid_cap < 0 || load_variant < 0 ||
       load_variant >= kLoadVariants)
     return DMRGX_ERR_ARG;
+  if (load_format(load_variant) == kFmtSell && lanes > kSellMaxLanes) return DMRGX_ERR_ARG;
   std::lock_guard<std::recursive_mutex> lk(api_mutex());
   A->lanes = lanes;
   A->grid = grid_cap;
@@ -441,7 +442,7 @@ namespace {
 // Mean time of y = A x with a fused dot product under schedule (g, cap, v), as the solvers
 // run their products (product, block partials, finish kernel), over reps launches after
 // one warm-up launch; A's own schedule is restored.
-double time_schedule(dmrgx_matrix *A, int g, int cap, int v, int reps, const double *x, double *y, double *dot,
+static double time_schedule(dmrgx_matrix *A, int g, int cap, int v, int reps, const double *x, double *y, double *dot,
                      Reduce &R, hipEvent_t e0, hipEvent_t e1) {
   hipStream_t st = ctx().stream;
   const int g0 = A->lanes, c0 = A->grid, v0 = A->load;
@@ -566,10 +567,19 @@ int dmrgx_matrix_autotune(dmrgx_matrix *A, int32_t *lanes_out, int32_t *grid_out
     };
     std::vector<Cand> cands;
     const bool has[3] = {true, ensure_dict(*A), ensure_delta(*A)};
-    for (int v = 0; v < kLoadVariants; v++)
+    for (int v = 0; v < kLoadVariants; v++) {
+      if (load_format(v) == kFmtSell) {
+        // sliced form: 1, 2 or 4 lanes per row, plain or XCD-contiguous row ranges (the
+        // other load bits do not apply to it)
+        if ((v & (kLoadPair | kLoadDeep)) != 0) continue;
+        for (int g = 1; g <= kSellMaxLanes; g *= 2)
+          for (int cap : caps) cands.push_back({g, cap, v, env.time(A, g, cap, v, 3)});
+        continue;
+      }
       if (has[load_format(v)])
         for (int g = g_lo; g <= g_hi; g *= 2)
           for (int cap : caps) cands.push_back({g, cap, v, env.time(A, g, cap, v, 3)});
+    }
     // the leaders again, timed as the solver runs them: a Lanczos step and a CG iteration
     // with their fused vector kernels, several times over
     std::sort(cands.begin(), cands.end(), [](const Cand &a, const Cand &b) { return a.us < b.us; });
@@ -580,6 +590,8 @@ int dmrgx_matrix_autotune(dmrgx_matrix *A, int32_t *lanes_out, int32_t *grid_out
     A->lanes = best.g;
     A->grid = best.cap;
     A->load = best.v;
+    // the sliced forms built for the candidates, except the one the schedule reads
+    drop_sell_except(*A, load_format(best.v) == kFmtSell ? best.g : 0);
     if (lanes_out) *lanes_out = best.g;
     if (grid_out) *grid_out = best.cap;
     if (load_out) *load_out = best.v;

@@ -56,6 +56,10 @@ struct SegDesc {
   const double *table;
   const int32_t *first;   // delta form
   const uint16_t *dcol;
+  const int64_t *sptr;    // sliced form (lanes per row 1, 2 or 4)
+  const int32_t *slen, *scol;
+  const double *sval;
+  const float *sval32;
   int32_t n, fp32;
   int32_t blk_lo, nblk;   // this slot's blocks in the launch
   int32_t k, load;        // step (Lanczos) or iteration (CG); SpMV load variant
@@ -122,7 +126,12 @@ __global__ __launch_bounds__(kBlock) void k_seg_spmv(const SegDesc *desc, const 
   for (int32_t gi = gr.start; gi < gr.end; gi += gr.step) {
     const int32_t r = gi * RPB + g;
     double s = 0.0;
-    if (r < d.n) {
+    if (r < d.n && fmt == kFmtSell) {
+      if constexpr (G <= kSellMaxLanes) {
+        if (d.fp32) s = sell_lane_sum<G, float>(d.sptr, d.slen, d.scol, d.sval32, r, lane, d.x);
+        else s = sell_lane_sum<G, double>(d.sptr, d.slen, d.scol, d.sval, r, lane, d.x);
+      }
+    } else if (r < d.n) {
       switch (d.load & (kLoadXcd - 1)) {
 #define V_CASE(V)                                                                              \
   case V:                                                                                      \
@@ -389,6 +398,17 @@ size_t slot_bytes(const dmrgx_engine &E, const Slot &s) {
 // The slot matrix's load variant, restricted to the formats the matrix has.
 int slot_load(const Slot &s) { return effective_load(*s.A); }
 
+// the sliced-form arrays of A for its lanes per row (null when A has none)
+void set_sell(SegDesc &a, const dmrgx_matrix &A) {
+  const int lp = A.lanes == 1 ? 0 : (A.lanes == 2 ? 1 : (A.lanes == 4 ? 2 : -1));
+  if (lp < 0) return;
+  a.sptr = A.sell_ptr[lp].p;
+  a.slen = A.sell_len.p;
+  a.scol = A.sell_col[lp].p;
+  a.sval = A.sell_val[lp].p;
+  a.sval32 = A.sell_val32[lp].p;
+}
+
 // Start a slot: start vector and |w|^2 (solver.hip:lanczos before its first step).
 void start_slot(Slot &s) {
   hipStream_t st = ctx().stream;
@@ -407,6 +427,7 @@ void start_slot(Slot &s) {
   if (s.mixed) ensure_fp32(*s.A);
   if (load_format(s.A->load) == kFmtDict) ensure_dict(*s.A);
   if (load_format(s.A->load) == kFmtDelta) ensure_delta(*s.A);
+  if (load_format(s.A->load) == kFmtSell) ensure_sell(*s.A, s.A->lanes, s.mixed);
   launch_start_vector(n, s.p.seed, s.vec.p, st);
   launch_copy(n, s.vec.p, s.w.p, st);
   Reduce R;
@@ -731,6 +752,7 @@ std::unique_ptr<Inflight> launch_chunk(dmrgx_engine &E, int ticks) {
         a.table = s.A->table.p;
         a.first = s.A->first.p;
         a.dcol = s.A->dcol.p;
+        set_sell(a, *s.A);
         a.k = k;
         a.shift = 0.0;
         a.x = s.w.p;
@@ -776,6 +798,7 @@ std::unique_ptr<Inflight> launch_chunk(dmrgx_engine &E, int ticks) {
         a.table = s.A->table.p;
         a.first = s.A->first.p;
         a.dcol = s.A->dcol.p;
+        set_sell(a, *s.A);
         a.shift = s.p.ii_diag_add - s.eig;
         a.x = s.pv.p;
         a.o1 = s.cy.p;

@@ -134,6 +134,14 @@ struct dmrgx_matrix {
   dmrgx::DevBuf<int32_t> first;
   dmrgx::DevBuf<uint16_t> dcol;
   int delta = -1;               // -1 not built yet, 0 not available, 1 built
+  // Sliced form per lanes-per-row count P = 1, 2, 4 (index log2 P), built on first use:
+  // slice offsets, padded columns and values (FP64, and FP32 once the mixed form asks);
+  // sell_len[r] = entries of row r.
+  dmrgx::DevBuf<int64_t> sell_ptr[3];
+  dmrgx::DevBuf<int32_t> sell_col[3];
+  dmrgx::DevBuf<double> sell_val[3];
+  dmrgx::DevBuf<float> sell_val32[3];
+  dmrgx::DevBuf<int32_t> sell_len;
   int lanes = 8;                // lanes per row of the SpMV kernel
   int grid = dmrgx::kGridCap;   // cap on SpMV workgroups (0: one per row group)
   int load = 0;                 // load variant (kLoad* bits)
@@ -278,12 +286,16 @@ constexpr int kLoadXcd = 4;
 // only): 8-byte (column, value index) entries over a table of the distinct values; delta:
 // 2-byte column differences within each row plus each row's first column.  Every format
 // gives the CSR products bit for bit (same values, same order).
+// Format 3, sliced: see sell_lane_sum; its lanes per row (1, 2 or 4) set the slice height,
+// and its sums are in an order of their own.
 constexpr int kFmtShift = 3;
-constexpr int kFmtCsr = 0, kFmtDict = 1, kFmtDelta = 2;
+constexpr int kFmtCsr = 0, kFmtDict = 1, kFmtDelta = 2, kFmtSell = 3;
 constexpr int kLoadDict = kFmtDict << kFmtShift;
 constexpr int kLoadDelta = kFmtDelta << kFmtShift;
+constexpr int kLoadSell = kFmtSell << kFmtShift;
 constexpr int kLoadFmtMask = 3 << kFmtShift;
-constexpr int kLoadVariants = 3 << kFmtShift;   // 0 .. 23
+constexpr int kLoadVariants = 4 << kFmtShift;   // 0 .. 31
+constexpr int kSellMaxLanes = 4;
 __host__ __device__ inline int load_format(int load) { return (load & kLoadFmtMask) >> kFmtShift; }
 
 // Row groups [start, end) with stride step that logical block lb of nblk takes out of
@@ -457,6 +469,51 @@ __device__ __forceinline__ double csr_lane_sum_v(const int64_t *__restrict__ row
     return csr_pair_lane_sum<G, (V & kLoadDeep) ? 4 : 2, VT>(row, col, val, r, lane, x);  // V < 4
   else
     return csr_lane_sum<G, (V & kLoadDeep) ? 8 : 4, false>(row, col, val, r, lane, x);
+}
+
+// Sliced form (format 3): slices of C = 64/P consecutive rows, one wavefront per slice with P
+// lanes per row.  Pass k of slice s holds entry k P + l of every row of the slice (lane l of
+// the row) at sptr[s] + 64 k + (r mod C) P + l; a row shorter than the slice's widest is
+// padded with its last column and the value 0, masked out of the sum by the row length.
+// A wavefront's matrix loads are then contiguous 512-byte (values) and 256-byte (columns)
+// segments, and one x gather instruction reads the columns of C neighbouring rows at the
+// same position in their rows, which lie close together in superblock and ED matrices (the
+// same terms acting on neighbouring basis states), instead of G consecutive columns of
+// 64/G rows as the CSR forms do.  Lane l sums its entries in eight accumulators (pass k into
+// accumulator k mod 8, eight passes' loads in flight), combined as
+// ((a0 + a4) + (a2 + a6)) + ((a1 + a5) + (a3 + a7)); the P lanes of a row are then combined by
+// the kernel's shuffles.  A fixed order, different from the CSR forms'.
+constexpr int kSellAcc = 8;
+template <int P, class VT>
+__device__ __forceinline__ double sell_lane_sum(const int64_t *__restrict__ sptr, const int32_t *__restrict__ slen,
+                                                const int32_t *__restrict__ scol, const VT *__restrict__ sval,
+                                                int32_t r, int lane, const double *__restrict__ x) {
+  constexpr int C = 64 / P;
+  const int32_t s = r / C;
+  const int64_t b0 = sptr[s];
+  const int64_t base = b0 + (int64_t)(r % C) * P + lane;
+  const int32_t w = (int32_t)((sptr[s + 1] - b0) >> 6);
+  const int32_t len = slen[r];
+  double acc[kSellAcc];
+#pragma unroll
+  for (int t = 0; t < kSellAcc; t++) acc[t] = 0.0;
+  for (int32_t k = 0; k < w; k += kSellAcc) {
+    VT v[kSellAcc];
+    int32_t c[kSellAcc];
+#pragma unroll
+    for (int t = 0; t < kSellAcc; t++) {
+      const int64_t at = base + (int64_t)min(k + t, w - 1) * 64;
+      v[t] = sval[at];
+      c[t] = scol[at];
+    }
+    double xv[kSellAcc];
+#pragma unroll
+    for (int t = 0; t < kSellAcc; t++) xv[t] = x[c[t]];
+#pragma unroll
+    for (int t = 0; t < kSellAcc; t++)
+      if ((k + t) * P + lane < len) acc[t] += (double)v[t] * xv[t];
+  }
+  return ((acc[0] + acc[4]) + (acc[2] + acc[6])) + ((acc[1] + acc[5]) + (acc[3] + acc[7]));
 }
 
 // Binary search of an ascending int64 array: index of target or -1
@@ -681,6 +738,12 @@ void ensure_fp32(const dmrgx_matrix &A);
 bool ensure_dict(const dmrgx_matrix &A);
 // Builds A's delta form if it has not been tried; true if A has one.
 bool ensure_delta(const dmrgx_matrix &A);
+// Builds A's sliced form for P lanes per row (1, 2, 4) if not built (and its FP32 values
+// with fp32); false for any other P.
+bool ensure_sell(const dmrgx_matrix &A, int P, bool fp32 = false);
+void build_sell(dmrgx_matrix &A, int P, bool fp32, hipStream_t s);
+// frees the sliced forms other than P's (all with P = 0)
+void drop_sell_except(const dmrgx_matrix &A, int P);
 // A's load variant restricted to the formats it has (a format it cannot have reads CSR).
 int effective_load(const dmrgx_matrix &A);
 

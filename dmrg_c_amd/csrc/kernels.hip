@@ -86,6 +86,28 @@ struct DeltaRows {
   }
 };
 
+// Sliced form with P lanes per row (sell_lane_sum); only launched with G == P.
+template <class VT, int P>
+struct SellRows {
+  const int64_t *__restrict__ sptr;
+  const int32_t *__restrict__ slen;
+  const int32_t *__restrict__ scol;
+  const VT *__restrict__ sval;
+  int64_t nnz;
+  int32_t nslices;
+  static constexpr bool kTimed = true;
+  // algorithmic bytes of one y = H x: the entries' values and columns (padding not
+  // counted) + row lengths + slice offsets + x once + y once
+  double alg_bytes(int32_t n) const {
+    return (double)nnz * (sizeof(VT) + 4) + (double)n * 4 + (double)(nslices + 1) * 8 + 2.0 * n * 8;
+  }
+  template <int G>
+  __device__ __forceinline__ double lane_sum(int32_t r, int lane, const double *__restrict__ x) const {
+    static_assert(G == P, "sliced rows are read with P lanes per row");
+    return sell_lane_sum<P, VT>(sptr, slen, scol, sval, r, lane, x);
+  }
+};
+
 // Row groups of kBlock/G rows are dealt to the blocks (the grid may be capped, so a block
 // takes several groups) round-robin, or with xcd set in contiguous ranges over XCD-mapped
 // logical blocks; each thread's epilogue contributions are summed in the order of its
@@ -282,12 +304,13 @@ struct EpiStoreDot {
 // Launches the product with its epilogue; with fin the reduction's finish kernel follows,
 // without it the caller's next kernel sums the partials.  Returns the number of blocks
 // (= partials).
-// The launch with the block partials at part (spmv_blocks(n, lanes, grid_cap) doubles when
-// the epilogue reduces); timed = false keeps it out of the live SpMV timing.
-template <class Rows, class Epi>
-int apply_rows_into(int32_t n, int lanes, int grid_cap, const Rows &rows, const double *x, const Epi &epi,
-                    double *part, hipStream_t s, int xcd, bool fin, bool timed) {
-  const int blocks = spmv_blocks(n, lanes, grid_cap);
+// The launch with G lanes per row and the block partials at part (spmv_blocks(n, G,
+// grid_cap) doubles when the epilogue reduces); timed = false keeps it out of the live SpMV
+// timing.
+template <int G, class Rows, class Epi>
+int apply_rows_g(int32_t n, int grid_cap, const Rows &rows, const double *x, const Epi &epi, double *part,
+                 hipStream_t s, int xcd, bool fin, bool timed) {
+  const int blocks = spmv_blocks(n, G, grid_cap);
   if (blocks == 0) return 0;
   Context &C = ctx();
   std::pair<hipEvent_t, hipEvent_t> *ev = nullptr;
@@ -298,13 +321,7 @@ int apply_rows_into(int32_t n, int lanes, int grid_cap, const Rows &rows, const 
     HIPCHK(hipEventRecord(ev->first, s));
     C.prof_bytes += rows.alg_bytes(n);
   }
-  switch (lanes) {
-#define L_CASE(G) \
-  case G: k_apply<G, Rows, Epi><<<blocks, kBlock, 0, s>>>(n, rows, x, epi, part, xcd); break;
-    L_CASE(1) L_CASE(2) L_CASE(4) L_CASE(8) L_CASE(16) L_CASE(32) L_CASE(64)
-#undef L_CASE
-    default: throw Failure(DMRGX_ERR_ARG, "bad lanes per row");
-  }
+  k_apply<G, Rows, Epi><<<blocks, kBlock, 0, s>>>(n, rows, x, epi, part, xcd);
   HIPCHK(hipGetLastError());
   if (ev) HIPCHK(hipEventRecord(ev->second, s));
   if (Epi::kReduce && fin) {
@@ -312,6 +329,18 @@ int apply_rows_into(int32_t n, int lanes, int grid_cap, const Rows &rows, const 
     HIPCHK(hipGetLastError());
   }
   return blocks;
+}
+
+template <class Rows, class Epi>
+int apply_rows_into(int32_t n, int lanes, int grid_cap, const Rows &rows, const double *x, const Epi &epi,
+                    double *part, hipStream_t s, int xcd, bool fin, bool timed) {
+  switch (lanes) {
+#define L_CASE(G) \
+  case G: return apply_rows_g<G>(n, grid_cap, rows, x, epi, part, s, xcd, fin, timed);
+    L_CASE(1) L_CASE(2) L_CASE(4) L_CASE(8) L_CASE(16) L_CASE(32) L_CASE(64)
+#undef L_CASE
+    default: throw Failure(DMRGX_ERR_ARG, "bad lanes per row");
+  }
 }
 
 template <class Rows, class Epi>
@@ -374,11 +403,50 @@ int apply_csr(const Operator &op, const VT *val, const double *x, const Epi &epi
   }
 }
 
+template <class VT, int P, class Epi>
+int apply_sell_p(const Operator &op, const VT *sval, const double *x, const Epi &epi, Reduce *R, hipStream_t s,
+                 bool fin) {
+  const dmrgx_matrix &A = *op.A;
+  constexpr int lp = P == 1 ? 0 : (P == 2 ? 1 : 2);
+  const int32_t nsl = (op.n + 64 / P - 1) / (64 / P);
+  const SellRows<VT, P> rows{A.sell_ptr[lp].p, A.sell_len.p, A.sell_col[lp].p, sval, A.nnz, nsl};
+  const int blocks = spmv_blocks(op.n, P, op.grid);
+  double *part = nullptr;
+  if (Epi::kReduce && blocks > 0) {
+    R->ensure(blocks);
+    part = R->partial.p;
+  }
+  return apply_rows_g<P>(op.n, op.grid, rows, x, epi, part, s, (op.load & kLoadXcd) ? 1 : 0, fin, true);
+}
+
+template <class Epi>
+int apply_sell(const Operator &op, bool fp32, const double *x, const Epi &epi, Reduce *R, hipStream_t s, bool fin) {
+  const dmrgx_matrix &A = *op.A;
+  const int lp = op.lanes == 1 ? 0 : (op.lanes == 2 ? 1 : 2);
+  if (fp32) {
+    const float *v = A.sell_val32[lp].p;
+    switch (op.lanes) {
+      case 1: return apply_sell_p<float, 1>(op, v, x, epi, R, s, fin);
+      case 2: return apply_sell_p<float, 2>(op, v, x, epi, R, s, fin);
+      case 4: return apply_sell_p<float, 4>(op, v, x, epi, R, s, fin);
+    }
+  } else {
+    const double *v = A.sell_val[lp].p;
+    switch (op.lanes) {
+      case 1: return apply_sell_p<double, 1>(op, v, x, epi, R, s, fin);
+      case 2: return apply_sell_p<double, 2>(op, v, x, epi, R, s, fin);
+      case 4: return apply_sell_p<double, 4>(op, v, x, epi, R, s, fin);
+    }
+  }
+  throw Failure(DMRGX_ERR_ARG, "sliced form needs 1, 2 or 4 lanes per row");
+}
+
 template <class Epi>
 int apply_op(const Operator &op, bool fp32, const double *x, const Epi &epi, Reduce *R, hipStream_t s,
              bool fin = true) {
   if (op.obc) return apply_rows(op.n, op.lanes, op.grid, op.obc->rows(), x, epi, R, s, 0, fin);
   const int fmt = load_format(op.load);
+  if (fmt == kFmtSell && ensure_sell(*op.A, op.lanes, fp32)) return apply_sell(op, fp32, x, epi, R, s, fin);
   const bool delta = fmt == kFmtDelta && ensure_delta(*op.A);
   if (fp32) {
     if (!op.A->val32.p) throw Failure(DMRGX_ERR_ARG, "FP32 values not prepared");
@@ -1285,6 +1353,108 @@ bool build_delta(dmrgx_matrix &A, hipStream_t s) {
   HIPCHK(hipMemcpyAsync(&wide, flag.p, sizeof(int), hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
   return wide == 0;
+}
+
+// ---------------------------------------------------------------- sliced form
+// Row lengths, and per slice of C = 64/P rows the slots it takes: 64 per pass, passes =
+// ceil(longest row / P).
+template <int P>
+__global__ __launch_bounds__(kBlock) void k_sell_width(int32_t n, int32_t nsl, const int64_t *__restrict__ row,
+                                                       int32_t *__restrict__ slen, int64_t *__restrict__ slots) {
+  constexpr int C = 64 / P;
+  const int32_t sl = blockIdx.x * kBlock + threadIdx.x;
+  if (sl >= nsl) return;
+  int64_t mx = 0;
+  for (int i = 0; i < C; i++) {
+    const int32_t r = sl * C + i;
+    if (r >= n) break;
+    const int64_t len = row[r + 1] - row[r];
+    slen[r] = (int32_t)len;
+    mx = len > mx ? len : mx;
+  }
+  slots[sl] = (mx + P - 1) / P * 64;
+}
+
+// One thread per row position of every slice (rows past n included): the row's entries
+// at their slots, padding with the row's last column and the value 0 (column 0 for an
+// empty row or a position past n).
+template <int P, class VT>
+__global__ __launch_bounds__(kBlock) void k_sell_fill(int32_t n, int32_t nsl, const int64_t *__restrict__ row,
+                                                      const int32_t *__restrict__ col, const VT *__restrict__ val,
+                                                      const int64_t *__restrict__ sptr, int32_t *__restrict__ scol,
+                                                      VT *__restrict__ sval) {
+  constexpr int C = 64 / P;
+  const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (t >= (int64_t)nsl * C) return;
+  const int32_t sl = (int32_t)(t / C), i = (int32_t)(t % C), r = (int32_t)t;
+  const int64_t b0 = sptr[sl], slots = (sptr[sl + 1] - b0) / 64 * P;   // entries per row position
+  const int64_t b = r < n ? row[r] : 0, len = r < n ? row[r + 1] - b : 0;
+  for (int64_t j = 0; j < slots; j++) {
+    const int64_t at = b0 + (j / P) * 64 + (int64_t)i * P + j % P;
+    if (j < len) {
+      scol[at] = col ? col[b + j] : 0;
+      sval[at] = val[b + j];
+    } else {
+      scol[at] = (len > 0 && col) ? col[b + len - 1] : 0;
+      sval[at] = VT(0);
+    }
+  }
+}
+
+void build_sell(dmrgx_matrix &A, int P, bool fp32, hipStream_t s) {
+  const int lp = P == 1 ? 0 : (P == 2 ? 1 : 2);
+  const int C = 64 / P;
+  const int32_t nsl = (A.n + C - 1) / C;
+  if (!A.sell_ptr[lp].p) {
+    if (!A.sell_len.p) A.sell_len.resize(std::max(A.n, 1));
+    DevBuf<int64_t> slots(std::max(nsl, 1));
+    A.sell_ptr[lp].resize((size_t)nsl + 1);
+    const unsigned gw = (unsigned)((nsl + kBlock - 1) / kBlock);
+    if (nsl > 0) {
+      switch (P) {
+        case 1: k_sell_width<1><<<gw, kBlock, 0, s>>>(A.n, nsl, A.row.p, A.sell_len.p, slots.p); break;
+        case 2: k_sell_width<2><<<gw, kBlock, 0, s>>>(A.n, nsl, A.row.p, A.sell_len.p, slots.p); break;
+        default: k_sell_width<4><<<gw, kBlock, 0, s>>>(A.n, nsl, A.row.p, A.sell_len.p, slots.p); break;
+      }
+      HIPCHK(hipGetLastError());
+    }
+    launch_exclusive_scan(nsl, slots.p, A.sell_ptr[lp].p, s);
+    int64_t total = 0;
+    HIPCHK(hipMemcpyAsync(&total, A.sell_ptr[lp].p + nsl, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    A.sell_col[lp].resize((size_t)std::max<int64_t>(total, 1));
+    A.sell_val[lp].resize((size_t)std::max<int64_t>(total, 1));
+    const unsigned gf = (unsigned)(((int64_t)nsl * C + kBlock - 1) / kBlock);
+    if (nsl > 0) {
+      switch (P) {
+        case 1: k_sell_fill<1, double><<<gf, kBlock, 0, s>>>(A.n, nsl, A.row.p, A.col.p, A.val.p, A.sell_ptr[lp].p,
+                                                            A.sell_col[lp].p, A.sell_val[lp].p); break;
+        case 2: k_sell_fill<2, double><<<gf, kBlock, 0, s>>>(A.n, nsl, A.row.p, A.col.p, A.val.p, A.sell_ptr[lp].p,
+                                                            A.sell_col[lp].p, A.sell_val[lp].p); break;
+        default: k_sell_fill<4, double><<<gf, kBlock, 0, s>>>(A.n, nsl, A.row.p, A.col.p, A.val.p, A.sell_ptr[lp].p,
+                                                             A.sell_col[lp].p, A.sell_val[lp].p); break;
+      }
+      HIPCHK(hipGetLastError());
+    }
+  }
+  if (fp32 && !A.sell_val32[lp].p) {
+    A.sell_val32[lp].resize(A.sell_val[lp].count);
+    const unsigned gf = (unsigned)(((int64_t)nsl * C + kBlock - 1) / kBlock);
+    if (nsl > 0) {
+      // columns are already in place: only the FP32 values are written (col = nullptr
+      // would zero the columns, so the column array is passed again and rewritten alike)
+      switch (P) {
+        case 1: k_sell_fill<1, float><<<gf, kBlock, 0, s>>>(A.n, nsl, A.row.p, A.col.p, A.val32.p, A.sell_ptr[lp].p,
+                                                           A.sell_col[lp].p, A.sell_val32[lp].p); break;
+        case 2: k_sell_fill<2, float><<<gf, kBlock, 0, s>>>(A.n, nsl, A.row.p, A.col.p, A.val32.p, A.sell_ptr[lp].p,
+                                                           A.sell_col[lp].p, A.sell_val32[lp].p); break;
+        default: k_sell_fill<4, float><<<gf, kBlock, 0, s>>>(A.n, nsl, A.row.p, A.col.p, A.val32.p, A.sell_ptr[lp].p,
+                                                            A.sell_col[lp].p, A.sell_val32[lp].p); break;
+      }
+      HIPCHK(hipGetLastError());
+    }
+  }
+  HIPCHK(hipStreamSynchronize(s));
 }
 
 void launch_to_fp32(const dmrgx_matrix &A, hipStream_t s) {

@@ -280,9 +280,33 @@ bool ensure_delta(const dmrgx_matrix &A) {
   return A.delta > 0;
 }
 
+bool ensure_sell(const dmrgx_matrix &A, int P, bool fp32) {
+  if (P != 1 && P != 2 && P != 4) return false;
+  const int lp = P == 1 ? 0 : (P == 2 ? 1 : 2);
+  if (fp32) ensure_fp32(A);
+  if (!A.sell_ptr[lp].p || (fp32 && !A.sell_val32[lp].p)) {
+    // a cache derived from the CSR arrays; the matrix itself is not changed
+    build_sell(const_cast<dmrgx_matrix &>(A), P, fp32, ctx().stream);
+  }
+  return true;
+}
+
+void drop_sell_except(const dmrgx_matrix &A, int P) {
+  dmrgx_matrix &M = const_cast<dmrgx_matrix &>(A);
+  for (int lp = 0; lp < 3; lp++)
+    if ((1 << lp) != P) {
+      M.sell_ptr[lp].reset();
+      M.sell_col[lp].reset();
+      M.sell_val[lp].reset();
+      M.sell_val32[lp].reset();
+    }
+  if (P != 1 && P != 2 && P != 4) M.sell_len.reset();
+}
+
 int effective_load(const dmrgx_matrix &A) {
   const int fmt = load_format(A.load);
-  const bool ok = fmt == kFmtCsr || (fmt == kFmtDict && A.n_values > 0) || (fmt == kFmtDelta && A.delta > 0);
+  const bool ok = fmt == kFmtCsr || (fmt == kFmtDict && A.n_values > 0) || (fmt == kFmtDelta && A.delta > 0) ||
+                  (fmt == kFmtSell && A.lanes <= kSellMaxLanes);
   // paired-entry loads are a choice of the CSR format: its fallback for another format drops them
   return ok ? A.load : (A.load & ~(kLoadFmtMask | kLoadPair));
 }

@@ -1,5 +1,5 @@
 """Edge cases of the device paths: ragged CSR rows (empty, one entry, thousands) under every
-lanes-per-row count and storage format; one-state sectors; non-convergence reported as a
+lanes-per-row count and storage format (the sliced form included); one-state sectors; non-convergence reported as a
 status code by the single and batched solves; malformed input rejected before any launch."""
 import numpy as np
 import pytest
@@ -44,6 +44,18 @@ def test_gpu_ragged_rows_every_schedule(distinct):
                 else:
                     base = y if base is None else base
                     assert np.array_equal(y, base), (g, v)   # formats and in-flight depth: same bits
+        # sliced form: 1, 2, 4 lanes per row; its other load bits and the row mapping leave
+        # the products' bits alone
+        for g in (1, 2, 4):
+            sl = None
+            for v in (24, 25, 26, 27, 28, 31):
+                A.set_schedule(g, 2048 if v & 4 else 0, v)
+                y = A.matvec(x)
+                assert np.abs(y - ref).max() <= 1e-12 * (np.abs(ref).max() + 1), (g, v)
+                sl = y if sl is None else sl
+                assert np.array_equal(y, sl), (g, v)
+            assert np.all(A.matvec(np.ones(n))[np.diff(row) == 0] == 0.0)
+        A.set_schedule(8, 0, 0)
         assert np.all(A.matvec(np.ones(n))[np.diff(row) == 0] == 0.0)   # empty rows give 0
     finally:
         A.free()

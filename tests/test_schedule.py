@@ -4,7 +4,9 @@ Entries in flight per lane and the storage format (CSR, value dictionary, 16-bit
 differences) change no arithmetic, so the solve is bit-identical under them; lanes per row
 and paired-entry loads change the summation order of each row, and the workgroup cap and
 the row-range bit the order of the fused dot products, so energies agree with the
-reference's to the FP64 bound under every schedule."""
+reference's to the FP64 bound under every schedule.  The sliced form (format 3, 1, 2 or 4
+lanes per row) sums each row in an order of its own: its products agree with the CSR ones
+to rounding, and its solves are the same bits single and batched."""
 import os
 
 import numpy as np
@@ -28,7 +30,7 @@ def test_gpu_schedule_argument_checks():
     import dmrg_c_amd as D
     A, _ = _matrix()
     try:
-        for bad in ((0, 0, 0), (3, 0, 0), (128, 0, 0), (8, -1, 0), (8, 0, 24), (8, 0, -1)):
+        for bad in ((0, 0, 0), (3, 0, 0), (128, 0, 0), (8, -1, 0), (8, 0, 24), (8, 0, 32), (8, 0, -1)):
             with pytest.raises(D.DmrgxError):
                 A.set_schedule(*bad)
         A.set_schedule(16, 2048, 23)
@@ -74,7 +76,8 @@ def test_gpu_load_variants_bit_identical():
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("sched", [(1, 0, 0), (4, 1024, 5), (16, 0, 10), (32, 8192, 15), (64, 2048, 12), (8, 16384, 6), (2, 0, 18), (8, 2048, 23)])
+@pytest.mark.parametrize("sched", [(1, 0, 0), (4, 1024, 5), (16, 0, 10), (32, 8192, 15), (64, 2048, 12), (8, 16384, 6), (2, 0, 18), (8, 2048, 23),
+                                   (1, 0, 24), (2, 4096, 24), (4, 8192, 28), (2, 16384, 31)])
 def test_gpu_schedules_agree(sched):
     A, e_ref = _matrix()
     try:
@@ -97,7 +100,8 @@ def test_gpu_autotune_sets_schedule():
         sched, us = A.autotune()
         assert A.schedule == sched and us > 0
         g, cap, v = sched
-        assert g & (g - 1) == 0 and 1 <= g <= 64 and cap >= 0 and v in range(24)
+        assert g & (g - 1) == 0 and 1 <= g <= 64 and cap >= 0 and v in range(32)
+        assert (v >> 3) != 3 or g <= 4
         E, _, _ = A.ground_state()
         assert abs(E - e_ref) <= 1e-10 * abs(e_ref)
     finally:
@@ -169,3 +173,39 @@ def test_gpu_formats_ed_and_engine(fmt):
     finally:
         for A in mats:
             A.free()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("lanes", [1, 2, 4])
+def test_gpu_sliced_ed_and_engine(lanes):
+    """ED matrices and the m=400 superblock in the sliced form: energies within 1e-10 of the
+    CSR form's (FP32-refined within 1e-9), single and batched solves the same bits."""
+    from dmrg_c_amd import ed, engine
+    mats = []
+    for m in (ed.Model("xxz", L=16, sz=0, J_z=0.7, h_z=0.2), ed.Model("hubbard", L=8, N=8, sz=0, U=3.0)):
+        P = ed.EdProblem.from_model(m)
+        mats.append(P.assemble())
+        P.free()
+    A, e_ref = _matrix()
+    mats.append(A)
+    try:
+        plain = []
+        for M in mats:
+            M.set_schedule(8, 4096, 2)
+            plain.append(M.ground_state())
+            M.set_schedule(lanes, 4096, 24 | (4 if lanes == 2 else 0))
+        print(f"sliced lanes={lanes}", flush=True)
+        single = [M.ground_state() for M in mats]
+        batch = engine.ground_state_batch(mats)
+        for p, d, b in zip(plain, single, batch):
+            assert abs(d[0] - p[0]) <= 1e-10 * abs(p[0])
+            assert d[0] == b[0] and np.array_equal(d[1], b[1])
+        assert abs(single[-1][0] - e_ref) <= 1e-10 * abs(e_ref)
+        m32 = [M.ground_state(precision=1) for M in mats]
+        b32 = engine.ground_state_batch(mats, engine.default_params(precision=1))
+        for p, d, b in zip(plain, m32, b32):
+            assert abs(d[0] - p[0]) <= 1e-9 * abs(p[0])
+            assert d[0] == b[0] and np.array_equal(d[1], b[1])
+    finally:
+        for M in mats:
+            M.free()
