@@ -689,9 +689,87 @@ void host_elem(int kind, DMRG_BASIS_ONSITE *B, DMRG_A_BASIS *A, int *Inv, const 
   *elem_num = ne;
 }
 
+// Element makers of the matrix-free path's sub-Hamiltonians (dmrg/DMRG_MAKE_ELEM_LL_LLLR.c,
+// _LR_LLLR, _LLLR_LLLR, _LRRL_LRRL, _RL_RRRL, _RR_RRRL, _RRRL_RRRL): the two-slot sub-basis
+// (sa, sb) of LLLR (LL, LR), LRRL (LR, RL) or RRRL (RR, RL), its 2-D inverse Inv[q_sa][q_sb],
+// one operator acting on slot a1 or two acting on a1 and a2; the same first-insertion /
+// accumulation rule and 1e-15 drops as the superblock makers, products left to right.
+// A state absent from the sub-basis (inverse -1) is skipped before its accumulator entry is
+// read (the reference reads A_Basis->Inv_LLLRRRRL[-1] there and then discards it).
+void host_sub_elem(int sa, int sb, int a1, int a2, DMRG_BASIS_ONSITE *B, DMRG_A_BASIS *A, int **Inv,
+                   const CRS1 *M1, const CRS1 *M2, double coeff, int sign, int *elem_num) {
+  if (std::fabs(coeff) < kZero) return;
+  int q[4] = {B->LL, B->LR, B->RR, B->RL};
+  short *field[4] = {A->LL_LLLRRRRL, A->LR_LLLRRRRL, A->RR_LLLRRRRL, A->RL_LLLRRRRL};
+  int ne = *elem_num;
+  auto put = [&](const int qq[4], double v) {
+    const int inv = Inv[qq[sa]][qq[sb]];
+    if (!(std::fabs(v) > kZero && inv >= 0)) return;
+    const int chk = A->Inv_LLLRRRRL[inv];
+    if (chk == -1) {
+      field[sa][ne] = (short)qq[sa];
+      field[sb][ne] = (short)qq[sb];
+      A->Val_LLLRRRRL[ne] = v;
+      A->Inv_LLLRRRRL[inv] = ne;
+      ne++;
+    } else {
+      A->Val_LLLRRRRL[chk] = A->Val_LLLRRRRL[chk] + v;
+    }
+  };
+  for (long i = M1->Row[q[a1]]; i < M1->Row[q[a1] + 1]; i++) {
+    int qq[4] = {q[0], q[1], q[2], q[3]};
+    qq[a1] = M1->Col[i];
+    if (a2 < 0) {
+      put(qq, coeff * M1->Val[i]);
+    } else {
+      const double v1 = M1->Val[i];
+      for (long j = M2->Row[q[a2]]; j < M2->Row[q[a2] + 1]; j++) {
+        qq[a2] = M2->Col[j];
+        put(qq, (double)sign * coeff * v1 * M2->Val[j]);
+      }
+    }
+  }
+  *elem_num = ne;
+}
+
 }  // namespace
 
 extern "C" {
+
+// ---- sub-Hamiltonians of the matrix-free path (include/dmrg.h:342-350)
+void DMRG_MAKE_ELEM_LL_LLLR(DMRG_BASIS_ONSITE *B, DMRG_A_BASIS *A, int **Inv_LLLR, CRS1 *M, double c, int *en) {
+  host_sub_elem(0, 1, 0, -1, B, A, Inv_LLLR, M, nullptr, c, 1, en);
+}
+void DMRG_MAKE_ELEM_LR_LLLR(DMRG_BASIS_ONSITE *B, DMRG_A_BASIS *A, int **Inv_LLLR, CRS1 *M, double c, int *en) {
+  host_sub_elem(0, 1, 1, -1, B, A, Inv_LLLR, M, nullptr, c, 1, en);
+}
+void DMRG_MAKE_ELEM_LLLR_LLLR(DMRG_BASIS_ONSITE *B, DMRG_A_BASIS *A, int **Inv_LLLR, CRS1 *M_LL, CRS1 *M_LR, double c,
+                              int *LL_Ele, int *en, char Type[], char Sign_Flag[]) {
+  if (std::fabs(c) < kZero) return;
+  const bool yes = Sign_Flag && strcmp(Sign_Flag, "Yes") == 0;
+  const int sg = sign_of("DMRG_MAKE_ELEM_LLLR_LLLR", Sign_Flag, Type, "LL_LR", "LR_LL", yes ? LL_Ele[B->LL] : 0);
+  host_sub_elem(0, 1, 0, 1, B, A, Inv_LLLR, M_LL, M_LR, c, sg, en);
+}
+void DMRG_MAKE_ELEM_LRRL_LRRL(DMRG_BASIS_ONSITE *B, DMRG_A_BASIS *A, int **Inv_LRRL, CRS1 *M_LR, CRS1 *M_RL, double c,
+                              int *Ele_LR, int *en, char Type[], char Sign_Flag[]) {
+  if (std::fabs(c) < kZero) return;
+  const bool yes = Sign_Flag && strcmp(Sign_Flag, "Yes") == 0;
+  const int sg = sign_of("DMRG_MAKE_ELEM_LRRL_LRRL", Sign_Flag, Type, "LR_RL", "RL_LR", yes ? Ele_LR[B->LR] : 0);
+  host_sub_elem(1, 3, 1, 3, B, A, Inv_LRRL, M_LR, M_RL, c, sg, en);
+}
+void DMRG_MAKE_ELEM_RL_RRRL(DMRG_BASIS_ONSITE *B, DMRG_A_BASIS *A, int **Inv_RRRL, CRS1 *M, double c, int *en) {
+  host_sub_elem(2, 3, 3, -1, B, A, Inv_RRRL, M, nullptr, c, 1, en);
+}
+void DMRG_MAKE_ELEM_RR_RRRL(DMRG_BASIS_ONSITE *B, DMRG_A_BASIS *A, int **Inv_RRRL, CRS1 *M, double c, int *en) {
+  host_sub_elem(2, 3, 2, -1, B, A, Inv_RRRL, M, nullptr, c, 1, en);
+}
+void DMRG_MAKE_ELEM_RRRL_RRRL(DMRG_BASIS_ONSITE *B, DMRG_A_BASIS *A, int **Inv_RRRL, CRS1 *M_RR, CRS1 *M_RL, double c,
+                              int *Ele_RR, int *en, char Type[], char Sign_Flag[]) {
+  if (std::fabs(c) < kZero) return;
+  const bool yes = Sign_Flag && strcmp(Sign_Flag, "Yes") == 0;
+  const int sg = sign_of("DMRG_MAKE_ELEM_RRRL_RRRL", Sign_Flag, Type, "RR_RL", "RL_RR", yes ? Ele_RR[B->RR] : 0);
+  host_sub_elem(2, 3, 2, 3, B, A, Inv_RRRL, M_RR, M_RL, c, sg, en);
+}
 
 void DMRG_MAKE_ELEM_LL_LLLRRRRL(DMRG_BASIS_ONSITE *B, DMRG_A_BASIS *A, int *Inv, CRS1 *M, double c, int *en) {
   if (g_rec) return record(K_LL, M, nullptr, c, 0, nullptr, 0, nullptr, 0, nullptr, 0);

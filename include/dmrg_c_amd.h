@@ -428,6 +428,18 @@ void DMRG_MAKE_ELEM_LRRR_LLLRRRRL(DMRG_BASIS_ONSITE *Basis, DMRG_A_BASIS *A_Basi
 void DMRG_MAKE_ELEM_RRRL_LLLRRRRL(DMRG_BASIS_ONSITE *Basis, DMRG_A_BASIS *A_Basis, int *Inv_LLLRRRRL, CRS1 *M_RR, CRS1 *M_RL, double coeef, int *Ele_RR, int *elem_num, char Type[], char Sign_Flag[]);
 void DMRG_MAKE_ELEM_ZERO_LLLRRRRL(DMRG_BASIS_ONSITE *Basis, DMRG_A_BASIS *A_Basis, int *Inv_LLLRRRRL, int *elem_num);
 
+/* include/dmrg.h:342-350 (dmrg/DMRG_MAKE_ELEM_LL_LLLR.c ... DMRG_MAKE_ELEM_RRRL_RRRL.c) --
+ * element makers of the sub-Hamiltonians the matrix-free path multiplies with
+ * (GET_HAM_LLLR / _LRRL / _RRRL of the XXZ program): one row of the LLLR, LRRL or RRRL
+ * sub-basis accumulated into A_Basis on the host, the reference's rule bit for bit. */
+void DMRG_MAKE_ELEM_LL_LLLR(DMRG_BASIS_ONSITE *Basis, DMRG_A_BASIS *A_Basis, int **Inv_LLLR, CRS1 *M, double coeef, int *elem_num);
+void DMRG_MAKE_ELEM_LR_LLLR(DMRG_BASIS_ONSITE *Basis, DMRG_A_BASIS *A_Basis, int **Inv_LLLR, CRS1 *M, double coeef, int *elem_num);
+void DMRG_MAKE_ELEM_LLLR_LLLR(DMRG_BASIS_ONSITE *Basis, DMRG_A_BASIS *A_Basis, int **Inv_LLLR, CRS1 *M_LL, CRS1 *M_LR, double coeef, int *LL_Ele, int *elem_num, char Type[], char Sign_Flag[]);
+void DMRG_MAKE_ELEM_LRRL_LRRL(DMRG_BASIS_ONSITE *Basis, DMRG_A_BASIS *A_Basis, int **Inv_LRRL, CRS1 *M_LR, CRS1 *M_RL, double coeef, int *Ele_LR, int *elem_num, char Type[], char Sign_Flag[]);
+void DMRG_MAKE_ELEM_RL_RRRL(DMRG_BASIS_ONSITE *Basis, DMRG_A_BASIS *A_Basis, int **Inv_RRRL, CRS1 *M, double coeef, int *elem_num);
+void DMRG_MAKE_ELEM_RR_RRRL(DMRG_BASIS_ONSITE *Basis, DMRG_A_BASIS *A_Basis, int **Inv_RRRL, CRS1 *M, double coeef, int *elem_num);
+void DMRG_MAKE_ELEM_RRRL_RRRL(DMRG_BASIS_ONSITE *Basis, DMRG_A_BASIS *A_Basis, int **Inv_RRRL, CRS1 *M_RR, CRS1 *M_RL, double coeef, int *Ele_RR, int *elem_num, char Type[], char Sign_Flag[]);
+
 /* ---- exact diagonalization (include/exact.h:113-125) */
 /* include/exact.h:118 -- Lanczos ("Lanczos" | "Lanczos_Slow"), inverse iteration with CG and
  * the residual check on the full CRS Ham_Info->Ham, on the device.  Fills Vector[0],

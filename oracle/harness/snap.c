@@ -15,6 +15,7 @@
  *   __wrap_GET_HAM_LLLRRRRL              dump basis + block operators + term list,
  *                                        and the assembled lower-triangle CRS
  *   __wrap_DMRG_DIAGONALIZE_SUPERBLOCK_SYM  log E, gs_error per step; dump GS vector
+ *   __wrap_DIAGONALIZE_SUPERBLOCK (XXZ)     the same log line on the matrix-free path
  *
  * Build: oracle/Makefile (outputs into oracle/_ref/ only).  Compile with one of
  * -DSNAP_XXZ, -DSNAP_HUBBARD, -DSNAP_KLM and -I main/dmrg/<model>/include.
@@ -394,6 +395,25 @@ void __wrap_DMRG_DIAGONALIZE_SUPERBLOCK_SYM(CRS1 *Ham, DMRG_TIME *Time, DMRG_PAR
    }
 }
 
+#if defined(SNAP_XXZ)
+/* ------------------------------------------- matrix-free eigensolve hook (XXZ) */
+/* main/dmrg/xxz_vf/src/allocations/DIAGONALIZE_SUPERBLOCK.c: taken instead of
+ * DMRG_DIAGONALIZE_SUPERBLOCK_SYM when the superblock dimension reaches limit=
+ * (GET_GROUND_STATE.c:15-21); logs the same energy line. */
+void __real_DIAGONALIZE_SUPERBLOCK(DMRG_BASIS *, BLOCK *, BLOCK *, MODEL_T *, DMRG_PARAMETER *, DMRG_STATUS *, DMRG_TIME *);
+void __wrap_DIAGONALIZE_SUPERBLOCK(DMRG_BASIS *B, BLOCK *System, BLOCK *Enviro, MODEL_T *Model, DMRG_PARAMETER *P,
+                                   DMRG_STATUS *St, DMRG_TIME *Time) {
+   double t0 = omp_get_wtime();
+   __real_DIAGONALIZE_SUPERBLOCK(B, System, Enviro, Model, P, St, Time);
+   double t1 = omp_get_wtime();
+   if (g_energy_log) {
+      fprintf(g_energy_log, "%d %d %d %d %d %.17g %.6e %.6f\n", St->tot_iter_now, St->sweep_now, St->LL_site, St->RR_site,
+              B->dim_LLLRRRRL, St->gs_val, St->gs_error, t1 - t0);
+      fflush(g_energy_log);
+   }
+}
+#endif
+
 /* ----------------------------------------------------- expectation values */
 /* dmrg/DMRG_EXPECTATION_ONSITE.c and DMRG_EXPECTATION_INTERSITE_Q0.c: dump inputs and
  * the reference's outputs of every call (file expect_<step>_<k>.bin). */
@@ -490,7 +510,9 @@ int main(int argc, char **argv) {
    strcpy(P->Lan_Con, args(argc, argv, "lan_con", "Normal"));
    strcpy(P->II_Type, "CG");
    P->sweep = (int)argd(argc, argv, "sweep", 3);
-   P->dim_LLLRRRRL_limit = INT_MAX;
+   /* superblocks at or above this dimension take the matrix-free path (OBC/SSD),
+    * main/dmrg/xxz_vf/src/allocations/GET_GROUND_STATE.c:15-21 */
+   P->dim_LLLRRRRL_limit = (int)argd(argc, argv, "limit", (double)INT_MAX);
    P->max_dim_system = (int)argd(argc, argv, "m", 64);
    P->sp_LL = argd(argc, argv, "sp_LL", 0.5);
    P->param_iter = 1;

@@ -6,7 +6,14 @@ ahead of the reference archive, exactly as a maintainer relinks a dmrg_C program
 reference symbol the library exports (superblock eigensolve, element makers, expectation
 values) resolves to the library.  ``eig`` keeps the reference's own truncation objects
 (DMRG_GET_DENSITY_MATRIX, DMRG_GET_SYSTEM_INFO_Q1/Q2, CRS_CRS_CCS_PRODUCT) in the program;
-``all`` takes those from the library too.
+``all`` takes those from the library too.  ``obc`` is ``all`` with limit=0: every step takes the
+program's matrix-free path (main/dmrg/xxz_vf/src/allocations/GET_GROUND_STATE.c:20-21 ->
+DIAGONALIZE_SUPERBLOCK.c:11-79): the sub-Hamiltonians GET_HAM_LLLR/LRRL/RRRL built with the
+library's DMRG_MAKE_ELEM_*_{LLLR,LRRL,RRRL}, then the library's DMRG_LANCZOS_SLOW_OBC,
+DMRG_INVERSE_ITERATION_OBC, DMRG_CONJUGATE_GRADIENT_OBC, DMRG_MATRIX_VECTOR_PRODUCT_OBC and
+DMRG_CONVERGE_CHECK_OBC.  The unmodified reference run through its own matrix-free path
+(limit=0) gives the same 127 dimensions and energies within 6e-14 of the CRS path's, so all
+variants are compared with the same reference output.
 
 Workload: XXZ chain, L=32, J_xy=J_z=1, h_z=0, S^z=0, m=64, 4 sweeps, Lanczos_Slow with
 diag_acc=1e-12, inverse iteration CG (main/dmrg/xxz_vf/src/main/DMRG.c:28-53, one superblock
@@ -54,16 +61,21 @@ def _table(path):
     return np.array([[float(x) for x in r] for r in rows])
 
 
-@pytest.fixture(scope="module", params=["eig", "all"])
+# variant -> (program, extra arguments); "obc" takes every step through the matrix-free path
+VARIANTS = {"eig": ("relink_xxz_eig", ""), "all": ("relink_xxz_all", ""), "obc": ("relink_xxz_all", "limit=0")}
+
+
+@pytest.fixture(scope="module", params=sorted(VARIANTS))
 def run(request, tmp_path_factory):
-    exe = os.path.join(REPO, "oracle", "_ref", f"relink_xxz_{request.param}")
+    prog, extra = VARIANTS[request.param]
+    exe = os.path.join(REPO, "oracle", "_ref", prog)
     if not os.path.exists(exe):
         pytest.skip("relinked reference program not built (make -C oracle relink, needs the reference tree)")
     wd = tmp_path_factory.mktemp(f"relink_{request.param}")
     env = dict(os.environ, MKL_THREADING_LAYER="GNU")
-    print(f"relink_xxz_{request.param}: {ARGS}", flush=True)
+    print(f"{prog}: {ARGS} {extra}", flush=True)
     with open(wd / "log.txt", "w") as log:
-        r = subprocess.run([exe] + ARGS.split() + [f"out={wd}"], cwd=wd, env=env, stdout=log,
+        r = subprocess.run([exe] + ARGS.split() + extra.split() + [f"out={wd}"], cwd=wd, env=env, stdout=log,
                            stderr=subprocess.STDOUT, timeout=240)
     assert r.returncode == 0, open(wd / "log.txt", errors="replace").read()[-2000:]
     return request.param, wd
