@@ -657,6 +657,13 @@ int dmrgx_ground_state(const dmrgx_matrix *A, const dmrgx_solve_params *p, doubl
       lanczos(op_of(*A), o, v.d.p, out);
     }
     if (!out.lan_converged) throw Failure(DMRGX_ERR_NOCONV, "Lanczos did not converge within diag_max_step");
+    // mixed precision: the FP32 Lanczos value is off by the FP32 rounding of the matrix; the
+    // inverse iteration's shift and its residual test take the FP64 Rayleigh quotient of the
+    // Ritz vector instead (exact to the square of the vector's error)
+    if (mixed && A->n > 1 && !p->skip_ii) {
+      out.eig = rayleigh_quotient(op_of(*A), v.d.p);
+      out.spmv++;
+    }
     if (!p->skip_ii)
       inverse_iteration(op_of(*A), out.eig, v.d.p, p->ii_acc, p->ii_diag_add, p->ii_max_step, p->cg_acc, p->cg_max_step,
                         out);

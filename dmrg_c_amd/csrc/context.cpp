@@ -15,7 +15,13 @@ namespace {
 Context g_ctx;
 std::mutex g_init_mu;
 thread_local std::string g_last_error;
-constexpr size_t kCacheLimit = size_t(16) << 30;  // bytes kept for reuse
+// Bytes of freed blocks kept for reuse.  A block returned beyond this is freed with
+// hipFree, which waits for the device to go idle; a batch of solves that all finish at
+// different times (the 256-problem ED batch releases ~60 GB of Lanczos bases) would then
+// stall the host at every release and lose the overlap of host checks with the device.
+// An allocation that fails frees the whole cache and retries, so a large cache never
+// keeps a caller from memory it needs.
+constexpr size_t kCacheLimit = size_t(96) << 30;
 }  // namespace
 
 void set_last_error(const std::string &msg) { g_last_error = msg; }

@@ -279,7 +279,7 @@ struct SlotBasis {
 // residual of an inverse-iteration round, or the mixed form's Rayleigh quotient); the
 // engine reads those scalars for all waiting slots together.
 enum Phase { P_LANCZOS, P_CG, P_WAIT, P_DONE };
-enum Wait { W_RESID = 1, W_RQ = 2 };
+enum Wait { W_RESID = 1, W_RQ = 2, W_SHIFT = 3 };   // residual; final Rayleigh quotient; mixed-form shift
 
 struct Slot {
   int64_t id = 0;
@@ -503,8 +503,10 @@ void ii_advance(Slot &s) {
   ii_finalize(s);
 }
 
+void begin_ii(Slot &s);
+
 // A waiting slot's scalars arrived: residual -> done or the next CG solve; Rayleigh
-// quotient -> done.
+// quotient -> the mixed form's shift (inverse iteration starts) or done.
 void wait_resolved(Slot &s, const double *h) {
   if (s.wait == W_RESID) {
     s.err = h[0];
@@ -515,6 +517,10 @@ void wait_resolved(Slot &s, const double *h) {
     } else {
       start_cg(s);
     }
+  } else if (s.wait == W_SHIFT) {
+    s.eig = h[0] / h[1];
+    s.out.spmv++;
+    begin_ii(s);
   } else {
     s.eig = h[0] / h[1];
     s.out.spmv++;
@@ -537,6 +543,13 @@ void cg_finished(Slot &s, int done_it) {
   launch_scale_by_rnorm(n, s.nrm2(), s.un.p, s.un.p, st);
   launch_copy(n, s.un.p, s.vec.p, st);
   s.ii_iter++;
+  ii_advance(s);
+}
+
+// First inverse-iteration round from the Ritz vector in vec.
+void begin_ii(Slot &s) {
+  launch_copy(s.n, s.vec.p, s.un.p, ctx().stream);
+  s.ii_iter = 0;
   ii_advance(s);
 }
 
@@ -571,9 +584,18 @@ void lanczos_finished(Slot &s, int kconv) {
   s.pv.resize(n);
   s.cy.resize(n);
   s.xw.resize(n);
-  launch_copy(n, s.vec.p, s.un.p, st);
-  s.ii_iter = 0;
-  ii_advance(s);
+  if (s.mixed && n > 1) {
+    // the shift of the mixed form's inverse iteration: the FP64 Rayleigh quotient of the
+    // Ritz vector (dmrgx_ground_state does the same), read back with the waiting slots
+    Reduce R;
+    launch_rayleigh(op_of(*s.A), s.vec.p, s.rq(), R, st);
+    launch_norm2(n, s.vec.p, R, s.rq() + 1, st);
+    s.phase = P_WAIT;
+    s.epoch++;
+    s.wait = W_SHIFT;
+    return;
+  }
+  begin_ii(s);
 }
 
 // Host checks of the steps computed since the last chunk (solver.hip:lanczos, no restarts).
@@ -873,7 +895,7 @@ std::unique_ptr<Inflight> launch_chunk(dmrgx_engine &E, int ticks) {
   }
   for (size_t i = 0; i < W.size(); i++) {
     const size_t o = off[L.size() + Cg.size() + i];
-    const bool rq = W[i]->wait == W_RQ;
+    const bool rq = W[i]->wait == W_RQ || W[i]->wait == W_SHIFT;
     off[L.size() + Cg.size() + i + 1] = o + (rq ? 2 : 1);
     gd.push_back({rq ? W[i]->rq() : W[i]->resid(), (int64_t)o, rq ? 2 : 1, 0});
   }
