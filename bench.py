@@ -1,24 +1,30 @@
 #!/usr/bin/env python3
 """Ground-state solves/s of the superblock eigensolve on one MI355X.
 
-Workload: the Hubbard superblock of the reference's own DMRG run at L=64, N=64, S^z=0,
-t=1 (t1=-1), U=4, h=0.1, m=400 (last infinite-system step, 64 sites, n=144542 rows),
-stored as tests/golden/hubbard_L64_m400_s31.npz (basis, block operators, term list and the
-reference's ground-state energy).  Inputs are uploaded to HBM before timing.
+Headline workload: the KLM superblock of the reference's own DMRG run at L=48, N=36, S^z=0,
+t=1, J_K=2, h_z=0.19109 (drawn from seed 0 in [0, 0.3]), m=800 (last infinite-system step,
+48 sites), stored as tests/golden/klm_L48_m800_s23.npz (written by tests/golden/make_golden.py
+--big: block operators, term list, basis, the reference's energy); without that file the
+Hubbard L=64 N=64 U=4 h=0.1 m=400 superblock (tests/golden/hubbard_L64_m400_s31.npz) is the
+headline and otherwise a secondary line.  Inputs are uploaded to HBM before timing.
 
 One step = one ground-state solve as GET_GROUND_STATE does it: device assembly of the
 superblock CRS (bit-identical to GET_HAM_LLLRRRRL), expansion to the full matrix, Lanczos
-(reference defaults of main/dmrg/hubbard_vf/src/main/main.c: "Lanczos", "Normal",
+(reference defaults of main/dmrg/<model>_vf/src/main/main.c: "Lanczos", "Normal",
 diag_acc 1e-14), inverse iteration with CG (cg_acc 1e-7, ii_acc 1e-9), vector to host.
 
 Reported beside it:
 * roofline of the dominant kernel (CSR SpMV inside Lanczos/CG), timed live with HIP events
   on the library's stream during the timed steps; algorithmic bytes per launch
-  nnz*(8+4) + (n+1)*8 + 2*n*8; MI355X HBM peak 8 TB/s;
+  nnz*(8+4) + (n+1)*8 + 2*n*8 (CSR); MI355X HBM peak 8 TB/s;
 * traffic: HBM bytes per SpMV launch from the committed rocprofv3 --pmc summary;
-* cpu_baseline: the reference's DMRG_DIAGONALIZE_SUPERBLOCK_SYM (oracle/_ref/libdmrgref.so,
-  built from the reference sources, OpenMP) on the same superblock, assembled on the host by
-  the oracle; eigensolve only (the reference's assembly time is not counted).
+* cpu_baseline: the reference's own code (oracle/_ref/libdmrgref.so, built from the
+  reference sources, OpenMP, pinned to the process's cores): for the Hubbard superblock its
+  DMRG_DIAGONALIZE_SUPERBLOCK_SYM, for the KLM superblock (one reference solve takes minutes)
+  its MATRIX_VECTOR_PRODUCT_SYM on a bounded sample scaled by the products of one solve;
+* forms: FP32-refined solves of the same superblock; batched (256 seeded ED problems, plain
+  FP64 one by one against FP32-refined through the batched engine); served (Poisson
+  arrivals at 200/s of ED problems and superblock snapshots through the engine).
 """
 import argparse
 import ctypes as C
@@ -32,10 +38,24 @@ import numpy as np
 
 REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, REPO)
-WORKLOAD = os.path.join(REPO, "tests", "golden", "hubbard_L64_m400_s31.npz")
-TRAFFIC = os.path.join(REPO, "profiles", "spmv_hubbard_L64_m400_traffic.json")
+GOLDEN = os.path.join(REPO, "tests", "golden")
+# superblock workloads: snapshot of the reference's own run, description, CPU-baseline form
+SUPERBLOCKS = {
+    "klm_L48_m800": (os.path.join(GOLDEN, "klm_L48_m800_s23.npz"),
+                     "KLM OBC L=48 N=36 Sz=0 t=1 J_K=2 h_z=0.19109 (seed 0) m=800, last infinite step (48 sites)",
+                     "products"),
+    "hubbard_L64_m400": (os.path.join(GOLDEN, "hubbard_L64_m400_s31.npz"),
+                         "Hubbard OBC L=64 N=64 Sz=0 t=1 U=4 h=0.1 m=400, last infinite step (64 sites)",
+                         "solves"),
+}
+WORKLOAD = SUPERBLOCKS["hubbard_L64_m400"][0]
 PROFILE_EVERY = 16
 PEAK_HBM_GBS = 8000.0
+SPMV_FORMATS = [("CSR", "k_apply<CsrRows<double>> (CSR SpMV + fused Lanczos/CG epilogue)"),
+                ("value dictionary", "k_apply<DictRows> (value-dictionary CRS SpMV + fused Lanczos/CG epilogue)"),
+                ("delta columns", "k_apply<DeltaRows<double>> (delta-column CRS SpMV + fused Lanczos/CG epilogue)"),
+                ("sliced", "k_apply<P, SellRows<double, P>> (sliced SpMV: 64/P-row slices, P lanes per row, "
+                           "+ fused Lanczos/CG epilogue)")]
 
 
 def log(msg):
@@ -67,6 +87,73 @@ def autotune(D, sb):
     return (g.value, cap.value, nt.value), us.value
 
 
+def cpu_env():
+    """OpenMP placement for the reference on the host: one thread per core of this process's
+    CPU set (at most 16, the GPU box's share), bound close on cores.  Read by libgomp when it
+    starts, so set before any OpenMP library is loaded."""
+    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    threads = max(1, min(16, cores))
+    os.environ["OMP_NUM_THREADS"] = str(threads)
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    os.environ.setdefault("OMP_PLACES", "cores")
+    os.environ.setdefault("MKL_THREADING_LAYER", "GNU")
+    return threads
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline_products(sb, n, steps_per_solve, budget_s):
+    """The reference's own MATRIX_VECTOR_PRODUCT_SYM (sml/MATRIX_VECTOR_PRODUCT_SYM.c, OpenMP
+    with per-thread scatter) on the superblock's lower triangle as the reference assembles it
+    (the device assembly, equal to GET_HAM_LLLRRRRL's bit for bit), timed for a bounded number
+    of calls and scaled to one solve by the products the reference's solve makes: one per
+    Lanczos step (LANCZOS_SYM, the program's default method) and one per CG iteration, with
+    the Lanczos and CG step counts of the GPU solve of the same superblock."""
+    import dmrg_c_amd as D
+    threads = int(os.environ.get("OMP_NUM_THREADS", "1"))
+    ref = os.path.join(REPO, "oracle", "_ref", "libdmrgref.so")
+    if not os.path.exists(ref):
+        return None
+    row, col, val = sb.download_lower()
+    H = D.HostCRS(row, col, val)
+    R = C.CDLL(ref)
+    R.MATRIX_VECTOR_PRODUCT_SYM.argtypes = [C.POINTER(D.CRS1), C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]
+    temp = np.zeros((threads, n))
+    rows_p = (C.c_void_p * threads)(*[temp[i].ctypes.data for i in range(threads)])
+    x = np.random.default_rng(4).standard_normal(n)
+    y = np.empty(n)
+    R.MATRIX_VECTOR_PRODUCT_SYM(H.ptr, x.ctypes.data, y.ctypes.data, C.cast(rows_p, C.c_void_p), threads)  # warm
+    ts = []
+    t0 = time.perf_counter()
+    while len(ts) < 3 or (time.perf_counter() - t0 < budget_s and len(ts) < 200):
+        ta = time.perf_counter()
+        R.MATRIX_VECTOR_PRODUCT_SYM(H.ptr, x.ctypes.data, y.ctypes.data, C.cast(rows_p, C.c_void_p), threads)
+        ts.append(time.perf_counter() - ta)
+    med = float(np.median(ts))
+    per_solve = med * steps_per_solve
+    return {
+        "value": round(1.0 / per_solve, 5),
+        "unit": "solves/s",
+        "cores": threads,
+        "kind": "reference",
+        "cpu_model": cpu_model(),
+        "omp": {k: os.environ.get(k) for k in ("OMP_NUM_THREADS", "OMP_PROC_BIND", "OMP_PLACES")},
+        "sample": (f"{len(ts)} calls of the reference's MATRIX_VECTOR_PRODUCT_SYM on the n={n} superblock "
+                   f"(median {med * 1e3:.1f} ms), scaled by the {steps_per_solve} products of one solve "
+                   "(Lanczos steps + CG iterations of the GPU solve); the Lanczos tridiagonal solves, vector "
+                   "updates and assembly are not counted, which favours the CPU"),
+    }
+
+
 def cpu_baseline(snap, e_ref, budget_s):
     """The reference's own eigensolve on the host cores (bounded sample)."""
     import dmrg_c_amd as D
@@ -76,6 +163,7 @@ def cpu_baseline(snap, e_ref, budget_s):
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     ref = os.path.join(REPO, "oracle", "_ref", "libdmrgref.so")
     row, col, val = O.assemble(snap)
+    model = cpu_model()
     kind, times, energies = None, [], []
     if os.path.exists(ref):
         os.environ.setdefault("MKL_THREADING_LAYER", "GNU")
@@ -135,6 +223,8 @@ def cpu_baseline(snap, e_ref, budget_s):
         "unit": "solves/s",
         "cores": threads,
         "kind": kind,
+        "cpu_model": model,
+        "omp": {k: os.environ.get(k) for k in ("OMP_NUM_THREADS", "OMP_PROC_BIND", "OMP_PLACES")},
         "sample": (f"{len(times)} solve(s) of the same n={snap.n} superblock after 1 untimed, median {med:.2f} s; "
                    + ("DMRG_DIAGONALIZE_SUPERBLOCK_SYM of the reference (LANCZOS_SYM + INVERSE_ITERATION_SYM, "
                       "-Ofast -fopenmp, MKL dstev_) on the oracle-assembled CRS; assembly not counted"
@@ -150,8 +240,10 @@ def spmv_sweep(D, sb, snap):
     n = snap.n
     alg = A.nnz * 12 + (n + 1) * 8 + 2 * n * 8
     rows = []
-    for v in range(24):
-        for g in (4, 8, 16, 32, 64):
+    for v in range(32):
+        for g in ((1, 2, 4) if (v >> 3) == 3 else (4, 8, 16, 32, 64)):
+            if (v >> 3) == 3 and v & 1:
+                continue
             for cap in (1024, 2048, 4096, 8192, 16384, 0):
                 us = A.time_schedule(g, cap, v, 20)
                 rows.append({"lanes": g, "cap": cap, "variant": v, "us": round(us, 2),
@@ -170,6 +262,10 @@ def spmv_sweep(D, sb, snap):
 def spmv_alg_bytes(D, A, sched, n):
     """Algorithmic bytes of one product in the format the schedule reads: CSR (8-byte
     values, 4-byte columns) or the value dictionary (8-byte packed entries + the table)."""
+    if (sched[2] >> 3) == 3:
+        nsl = (n + 64 // sched[0] - 1) // (64 // sched[0])
+        return (A.nnz * 12 + n * 4 + (nsl + 1) * 8 + 2 * n * 8,
+                f"sliced ({sched[0]} lanes per row; 8-byte values, 4-byte columns, padding not counted)")
     if (sched[2] >> 3) == 2:
         return (A.nnz * 10 + (n + 1) * 8 + n * 4 + 2 * n * 8,
                 "delta columns (8-byte values, 2-byte column differences, 4-byte first columns)")
@@ -592,25 +688,30 @@ def run_density(D, reps: int, cpu: bool):
 
 
 # ------------------------------------------------------------------ served stream
-SERVE_SNAPSHOTS = ["xxz_L12_m24_jz05_s13.npz", "hubbard_L8_m32_s07.npz", "klm_L6_m32_s04.npz",
-                   "klm_L6_m24_ixy_s04.npz", "hubbard_L64_m400_s31.npz"]
+# superblock snapshots of the served stream: the reference's runs at m = 200..1000
+SERVE_SNAPSHOTS = ["xxz_L64_m300_s31.npz", "hubbard_L64_m400_s31.npz", "klm_L48_m800_s23.npz"]
 
 
-def run_serve(D, rate, duration, seed=0, chunk_ticks=1):
+def run_serve(D, rate, duration, seed=0, chunk_ticks=1, precision=1, snapshots=None):
     """Poisson arrivals (rate/s, seeded) for `duration` s: 70 % ED (XXZ L 10..20, Hubbard
-    L 6..10, random J_z, h), 30 % superblock snapshots of the reference's runs; each request
-    is assembled on the device when it arrives and joins the running engine (continuous
-    batching).  Latency = completion - scheduled arrival."""
+    L 6..10, random J_z, h), 30 % superblock snapshots of the reference's runs at m=300..800
+    (those present in tests/golden); each request is assembled on the device when it
+    arrives and joins the running engine (continuous batching).  Latency = completion -
+    scheduled arrival.  precision 1: FP32-refined solves."""
     from dmrg_c_amd import ed, engine, snapshot
     rng = np.random.default_rng(seed)
     gaps = rng.exponential(1.0 / rate, size=int(rate * duration * 2) + 10)
     arrivals = np.cumsum(gaps)
     arrivals = arrivals[arrivals < duration]
-    sbs, refs = [], []
-    for name in SERVE_SNAPSHOTS:
-        snap = snapshot.load(os.path.join(REPO, "tests", "golden", name))
+    sbs, refs, sb_names = [], [], []
+    for name in (snapshots or SERVE_SNAPSHOTS):
+        path = os.path.join(REPO, "tests", "golden", name)
+        if not os.path.exists(path):
+            continue
+        snap = snapshot.load(path)
         sbs.append(D.Superblock(snap))
         refs.append(float(snap.meta["gs_energy"][0]))
+        sb_names.append(f"{name[:-4]} (n={snap.n})")
     reqs = []
     for _ in arrivals:
         if rng.random() < 0.7:
@@ -625,7 +726,7 @@ def run_serve(D, rate, duration, seed=0, chunk_ticks=1):
         else:
             reqs.append(("sb", int(rng.integers(0, len(sbs)))))
     E = engine.Engine(chunk_ticks=chunk_ticks)
-    p = engine.default_params()
+    p = engine.default_params(precision=precision)
     t0 = time.perf_counter()
     i, done = 0, {}
     errs = []
@@ -684,14 +785,168 @@ def run_serve(D, rate, duration, seed=0, chunk_ticks=1):
                                        ("superblock", [d[k] for k in d if reqs[k][0] == "sb"])) if v}
                 for part, d in (("until_admitted", wait), ("in_engine", service))},
             "chunk_ticks": chunk_ticks,
-            "note": "superblock requests drawn from the reference-run snapshots in tests/golden "
-                    "(n up to 144542), assembled on the device at arrival"}
+            "precision": "fp32_refined" if precision else "fp64",
+            "superblocks": sb_names,
+            "note": "superblock requests drawn uniformly from the reference-run snapshots listed, "
+                    "assembled on the device at arrival"}
+
+
+def run_superblock(D, L, name, steps, warmup, fixed, cpu_budget):
+    """Solves/s of one superblock workload: device assembly + Lanczos + inverse iteration/CG +
+    vector to host per solve, p50/p99, the live SpMV roofline (HIP events on the library
+    stream around one in PROFILE_EVERY of the products that always run in full), the
+    FP32-refined form and, with cpu_budget, the CPU baseline."""
+    from dmrg_c_amd import snapshot
+    path, desc, cpu_form = SUPERBLOCKS[name]
+    snap = snapshot.load(path)
+    e_ref = float(snap.meta["gs_energy"][0])
+    params = dict(diag_acc=1e-14, diag_max_step=800, cg_acc=1e-7, cg_max_step=2000, ii_acc=1e-9,
+                  ii_diag_add=1e-11, ii_max_step=3, seed=1)
+    sb = D.Superblock(snap)  # inputs resident in HBM from here on
+    log(f"workload {name}: n={snap.n}")
+    if fixed:
+        lanes, tuned_us = fixed, float("nan")
+        log(f"schedule given: (lanes per row, workgroup cap, load variant) = {lanes}")
+    else:
+        lanes, tuned_us = autotune(D, sb)
+        log(f"autotune: (lanes per row, workgroup cap, load variant) = {lanes}, {tuned_us:.1f} us per Lanczos "
+            "step + CG iteration")
+    for i in range(warmup):
+        E, st, nnz = gpu_solve(D, sb, params, lanes)
+        log(f"warmup {i}: E={E:.15f} lanczos={st.lanczos_steps} cg={st.cg_steps}")
+    # events bracket one SpMV launch in PROFILE_EVERY: every bracketed launch costs the
+    # stream a few microseconds of idle, which would slow the timed solves themselves
+    D.check(L.dmrgx_profile_begin_sampled(PROFILE_EVERY))
+    lat, errs, lan, cg, gerr = [], [], [], [], []
+    t0 = time.perf_counter()
+    for i in range(steps):
+        ts = time.perf_counter()
+        E, st, nnz = gpu_solve(D, sb, params, lanes)
+        lat.append(time.perf_counter() - ts)
+        errs.append(abs(E - e_ref) / abs(e_ref))
+        lan.append(st.lanczos_steps)
+        cg.append(st.cg_steps)
+        gerr.append(st.gs_error)
+        log(f"step {i}: {lat[-1] * 1e3:.2f} ms  E={E:.15f}  dE/E={errs[-1]:.2e}")
+    t1 = time.perf_counter()
+    spmv_ms, launches, alg_bytes = C.c_double(), C.c_int64(), C.c_double()
+    L.dmrgx_profile_end.argtypes = [C.POINTER(C.c_double), C.POINTER(C.c_int64), C.POINTER(C.c_double)]
+    D.check(L.dmrgx_profile_end(C.byref(spmv_ms), C.byref(launches), C.byref(alg_bytes)))
+    total = t1 - t0
+    # the same solves in the mixed-precision form
+    mixed = dict(params, precision=1)
+    mlat, merrs, mgerr = [], [], []
+    for i in range(max(2, steps // 4)):
+        ts = time.perf_counter()
+        E, st, _ = gpu_solve(D, sb, mixed, lanes)
+        mlat.append(time.perf_counter() - ts)
+        merrs.append(abs(E - e_ref) / abs(e_ref))
+        mgerr.append(st.gs_error)
+    avg_launch_s = spmv_ms.value / 1e3 / max(launches.value, 1)
+    bytes_per_launch = alg_bytes.value / max(launches.value, 1)
+    achieved = bytes_per_launch / avg_launch_s / 1e9 if avg_launch_s > 0 else 0.0
+    traffic = None
+    tfile = os.path.join(REPO, "profiles", f"spmv_{name}_traffic.json")
+    if os.path.exists(tfile):
+        with open(tfile) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+    fmt = lanes[2] >> 3
+    res = {
+        "value": round(steps / total, 4),
+        "ms_per_step": round(total / steps * 1e3, 3),
+        "data": f"superblock snapshot from the reference's own DMRG run (tests/golden/{os.path.basename(path)})",
+        "config": {
+            "workload": f"{desc}, superblock n={snap.n}: device assembly + Lanczos + inverse iteration/CG per solve",
+            "n": snap.n,
+            "nnz_full": int(nnz),
+            "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 3),
+            "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 3),
+            "max_rel_energy_error": float(max(errs)),
+            "max_gs_error": float(max(gerr)),
+            "lanczos_steps": int(np.median(lan)),
+            "cg_steps": int(np.median(cg)),
+            "spmv_schedule": {"lanes_per_row": lanes[0], "workgroup_cap": lanes[1],
+                              "paired_entry_loads": bool(lanes[2] & 1) and fmt == 0,
+                              "entries_in_flight_per_lane": 8 if (lanes[2] & 2 or fmt == 3) else 4,
+                              "non_temporal_matrix_loads": bool(lanes[2] & 2) and fmt == 3,
+                              "xcd_contiguous_rows": bool(lanes[2] & 4),
+                              "format": SPMV_FORMATS[fmt][0]},
+        },
+        "fp32_refined": {"solves_per_s": round(len(mlat) / sum(mlat), 4),
+                         "p99_ms": round(float(np.percentile(mlat, 99)) * 1e3, 3),
+                         "max_rel_energy_error": float(max(merrs)),
+                         "max_gs_error": float(max(mgerr)),
+                         "over_fp64": round((len(mlat) / sum(mlat)) / (steps / total), 3),
+                         "note": "FP32 CSR values in the Lanczos SpMV, FP64 recurrence scalars, FP64 "
+                                 "inverse iteration/CG and Rayleigh quotient"},
+        "roofline": {
+            "bound": "hbm",
+            "kernel": SPMV_FORMATS[fmt][1],
+            "achieved": round(achieved, 1),
+            "peak": PEAK_HBM_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / PEAK_HBM_GBS, 4),
+            "traffic": traffic,
+            "alg_bytes_per_launch": int(bytes_per_launch),
+            "avg_launch_us": round(avg_launch_s * 1e6, 2),
+            "launches": int(launches.value),
+            "launches_sampled": f"1 in {PROFILE_EVERY} of the timed solves' SpMV launches that always run in full "
+                                "(Lanczos products, start residuals; CG products can return at once past "
+                                "convergence and are left out), HIP events on the library stream",
+        },
+        "cpu_baseline": None,
+    }
+    if cpu_budget is not None:
+        if cpu_form == "products":
+            res["cpu_baseline"] = cpu_baseline_products(sb, snap.n, int(np.median(lan)) + int(np.median(cg)),
+                                                        cpu_budget)
+        else:
+            res["cpu_baseline"] = cpu_baseline(snap, e_ref, cpu_budget)
+    sb.free()
+    return res
+
+
+def batched_form(D):
+    """The batched form on the 256-problem ED mix: the plain FP64 path one solve at a time
+    (dmrgx_ground_state per problem) against the FP32-refined solves batched through the engine
+    (segmented launches for all active problems, hipGraph replays of repeated chunks)."""
+    r = run_ed_batch(D, 256, precision_forms=(0, 1), single=True)
+    f64, f32 = r["fp64"], r["fp32_refined"]
+    return {"workload": f"256 seeded ED ground states (n {r['n_min']}..{r['n_max']}): XXZ L=12..24 S^z=0 "
+                        "J_z~U[0.5,1.5] h~U[0,0.5]; Hubbard L=6..10 half filling t=1 U=4 h~U[0,0.5]",
+            "plain_fp64_one_by_one_solves_per_s": f64["single_solves_per_s"],
+            "batched_fp64_solves_per_s": f64["batched_solves_per_s"],
+            "batched_fp32_refined_solves_per_s": f32["batched_solves_per_s"],
+            "batched_fp32_refined_over_plain_fp64": round(f32["batched_solves_per_s"] / f64["single_solves_per_s"], 3),
+            "max_rel_energy_error_vs_fp64": f32["max_rel_energy_diff_vs_fp64"],
+            "batched_equals_single_fp64": f64["batched_equals_single"],
+            "graphs": f32["engine_host_s"].get("graphs"),
+            "detail": r}
+
+
+def served_form(D, duration):
+    """The served stream at 200 requests/s: the mix as configured (superblocks of the
+    reference's runs at m=300..800, the KLM m=800 one with n ~ 2e6 included) and, beside it,
+    the same stream without the m=800 superblock, which one GPU can keep up with."""
+    keep = ("requests", "rate_per_s", "duration_s", "ed_requests", "superblock_requests", "solves_per_s",
+            "p50_ms", "p99_ms", "max_rel_energy_error_superblocks", "superblocks", "precision",
+            "engine_chunks", "host_s", "latency_split_ms")
+    out = {}
+    for tag, names, dur in (("c5_mix", SERVE_SNAPSHOTS, min(duration, 1.0)),
+                            ("without_m800", SERVE_SNAPSHOTS[:2], duration)):
+        r = run_serve(D, 200.0, dur, snapshots=names, precision=0)
+        out[tag] = {k: r[k] for k in keep if k in r}
+    out["c5_mix"]["note"] = ("the configured mix: one third of the superblock requests are the KLM m=800 "
+                             "superblock (n=1993680, ~1.4 s of GPU time each), ~3 GPU-seconds of work per "
+                             "second at 200 requests/s, so one GPU falls behind and latency grows with the "
+                             "arrival window (1 s here)")
+    return out
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU baseline work")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -715,12 +970,20 @@ def main():
     ap.add_argument("--ed-count", type=int, default=256)
     ap.add_argument("--ed-batched-only", action="store_true",
                     help="ed-batch: skip the one-by-one solves (for profiling the engine alone)")
+    ap.add_argument("--superblock", choices=sorted(SUPERBLOCKS), default="",
+                    help="headline superblock (default: KLM L=48 m=800 when its snapshot is present, "
+                         "else Hubbard L=64 m=400)")
+    ap.add_argument("--headline-only", action="store_true",
+                    help="only the headline superblock (no secondary superblock, batched or served forms)")
+    ap.add_argument("--serve-duration", type=float, default=2.0,
+                    help="served form of the headline line: seconds of Poisson arrivals at 200/s")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the superblock eigensolve runs on a single MI355X "
               f"(--gpus {args.gpus} is not supported; multi-GPU sharding is a design note only)", file=sys.stderr)
         sys.exit(2)
     os.environ.setdefault("DMRGX_SML_OUT", "0")
+    cpu_env()   # OpenMP placement of the CPU baseline, before any OpenMP library loads
     import dmrg_c_amd as D
     from dmrg_c_amd import snapshot
 
@@ -746,120 +1009,49 @@ def main():
         print(json.dumps({"metric": "sector-changing single-slot products/s", "workload": "v-m",
                           "result": run_v_m(D, args.steps, not args.no_cpu_baseline)}), flush=True)
         return
-    snap = snapshot.load(WORKLOAD)
-    e_ref = float(snap.meta["gs_energy"][0])
-    params = dict(diag_acc=1e-14, diag_max_step=800, cg_acc=1e-7, cg_max_step=2000, ii_acc=1e-9,
-                  ii_diag_add=1e-11, ii_max_step=3, seed=1)
-    sb = D.Superblock(snap)  # inputs resident in HBM from here on
+    wl = args.superblock if args.superblock else ("klm_L48_m800" if os.path.exists(SUPERBLOCKS["klm_L48_m800"][0])
+                                                   else "hubbard_L64_m400")
+    path = SUPERBLOCKS[wl][0]
     fixed = tuple(int(v) for v in args.schedule.split(",")) if args.schedule else None
-    if args.traffic_probe:
-        traffic_probe(D, L, sb, snap, fixed)
+    if args.traffic_probe or args.spmv_sweep:
+        snap = snapshot.load(path)
+        sb = D.Superblock(snap)
+        if args.traffic_probe:
+            traffic_probe(D, L, sb, snap, fixed)
+        else:
+            spmv_sweep(D, sb, snap)
         return
-    if args.spmv_sweep:
-        spmv_sweep(D, sb, snap)
-        return
-    log(f"workload: Hubbard L=64 m=400 superblock n={snap.n}")
-    if fixed:
-        lanes, tuned_us = fixed, float("nan")
-        log(f"schedule given: (lanes per row, workgroup cap, load variant) = {lanes}")
-    else:
-        lanes, tuned_us = autotune(D, sb)
-        log(f"autotune: (lanes per row, workgroup cap, load variant) = {lanes}, {tuned_us:.1f} us per Lanczos step + CG iteration")
-    for i in range(args.warmup):
-        E, st, nnz = gpu_solve(D, sb, params, lanes)
-        log(f"warmup {i}: E={E:.15f} lanczos={st.lanczos_steps} cg={st.cg_steps}")
-
-    # events bracket one SpMV launch in PROFILE_EVERY: every bracketed launch costs the
-    # stream a few microseconds of idle, which would slow the timed solves themselves
-    D.check(L.dmrgx_profile_begin_sampled(PROFILE_EVERY))
-    lat, errs, lan, cg = [], [], [], []
-    t0 = time.perf_counter()
-    for i in range(args.steps):
-        ts = time.perf_counter()
-        E, st, nnz = gpu_solve(D, sb, params, lanes)
-        lat.append(time.perf_counter() - ts)
-        errs.append(abs(E - e_ref) / abs(e_ref))
-        lan.append(st.lanczos_steps)
-        cg.append(st.cg_steps)
-        log(f"step {i}: {lat[-1] * 1e3:.2f} ms  E={E:.15f}  dE/E={errs[-1]:.2e}")
-    t1 = time.perf_counter()
-    spmv_ms, launches, alg_bytes = C.c_double(), C.c_int64(), C.c_double()
-    L.dmrgx_profile_end.argtypes = [C.POINTER(C.c_double), C.POINTER(C.c_int64), C.POINTER(C.c_double)]
-    D.check(L.dmrgx_profile_end(C.byref(spmv_ms), C.byref(launches), C.byref(alg_bytes)))
-
-    total = t1 - t0
-    # the same solves in the mixed-precision form (FP32 SpMV in Lanczos, FP64 refinement)
-    mixed = dict(params, precision=1)
-    mlat, merrs = [], []
-    for i in range(max(2, args.steps // 4)):
-        ts = time.perf_counter()
-        E, st, _ = gpu_solve(D, sb, mixed, lanes)
-        mlat.append(time.perf_counter() - ts)
-        merrs.append(abs(E - e_ref) / abs(e_ref))
-    avg_launch_s = spmv_ms.value / 1e3 / max(launches.value, 1)
-    bytes_per_launch = alg_bytes.value / max(launches.value, 1)
-    achieved = bytes_per_launch / avg_launch_s / 1e9 if avg_launch_s > 0 else 0.0
-    traffic = None
-    if os.path.exists(TRAFFIC):
-        with open(TRAFFIC) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch")
+    head = run_superblock(D, L, wl, args.steps, args.warmup, fixed,
+                          None if args.no_cpu_baseline else args.cpu_budget)
     line = {
         "metric": "ground-state solves/s + p99 latency at |dE/E|<=1e-9; SpMV HBM GB/s vs peak",
-        "value": round(args.steps / total, 4),
+        "value": head["value"],
         "unit": "solves/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(total / args.steps * 1e3, 3),
+        "ms_per_step": head["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "superblock snapshot from the reference's own DMRG run (tests/golden/hubbard_L64_m400_s31.npz)",
-        "config": {
-            "workload": "Hubbard OBC L=64 N=64 Sz=0 t=1 U=4 h=0.1 m=400, superblock n=144542: "
-                        "device assembly + Lanczos + inverse iteration/CG per solve",
-            "n": snap.n,
-            "nnz_full": int(nnz),
-            "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 3),
-            "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 3),
-            "max_rel_energy_error": float(max(errs)),
-            "lanczos_steps": int(np.median(lan)),
-            "cg_steps": int(np.median(cg)),
-            "spmv_schedule": {"lanes_per_row": lanes[0], "workgroup_cap": lanes[1],
-                              "paired_entry_loads": bool(lanes[2] & 1) and (lanes[2] >> 3) == 0,
-                              "entries_in_flight_per_lane": 8 if lanes[2] & 2 else 4,
-                              "xcd_contiguous_rows": bool(lanes[2] & 4),
-                              "format": ("CSR", "value dictionary", "delta columns")[lanes[2] >> 3]},
-        },
-        "forms": {
-            "fp32_refined": {"solves_per_s": round(len(mlat) / sum(mlat), 4),
-                             "p99_ms": round(float(np.percentile(mlat, 99)) * 1e3, 3),
-                             "max_rel_energy_error": float(max(merrs)),
-                             "note": "FP32 CSR values in the Lanczos SpMV, FP64 recurrence scalars, FP64 "
-                                     "inverse iteration/CG and Rayleigh quotient"},
-        },
-        "roofline": {
-            "bound": "hbm",
-            "kernel": ("k_apply<CsrRows<double>> (CSR SpMV + fused Lanczos/CG epilogue)",
-                       "k_apply<DictRows> (value-dictionary CRS SpMV + fused Lanczos/CG epilogue)",
-                       "k_apply<DeltaRows<double>> (delta-column CRS SpMV + fused Lanczos/CG epilogue)")[lanes[2] >> 3],
-            "achieved": round(achieved, 1),
-            "peak": PEAK_HBM_GBS,
-            "unit": "GB/s",
-            "frac": round(achieved / PEAK_HBM_GBS, 4),
-            "traffic": traffic,
-            "alg_bytes_per_launch": int(bytes_per_launch),
-            "avg_launch_us": round(avg_launch_s * 1e6, 2),
-            "launches": int(launches.value),
-            "launches_sampled": f"1 in {PROFILE_EVERY} of the timed solves' SpMV launches that always run in full "
-                                "(Lanczos products, start residuals; CG products can return at once past "
-                                "convergence and are left out), HIP events on the library stream",
-        },
-        "cpu_baseline": None,
+        "data": head["data"],
+        "config": head["config"],
+        "forms": {"fp32_refined": head["fp32_refined"]},
+        "roofline": head["roofline"],
+        "cpu_baseline": head["cpu_baseline"],
     }
-    if not args.no_cpu_baseline:
-        line["cpu_baseline"] = cpu_baseline(snap, e_ref, args.cpu_budget)
+    if not args.headline_only:
+        # the other superblock as a secondary measurement (same solve, its own roofline)
+        for other in SUPERBLOCKS:
+            if other != wl and os.path.exists(SUPERBLOCKS[other][0]):
+                sec = run_superblock(D, L, other, max(4, args.steps // 2), 1, None, None)
+                line.setdefault("secondary_workloads", {})[other] = {
+                    k: sec[k] for k in ("value", "ms_per_step", "config", "fp32_refined", "roofline")}
+        log("forms: batched ED (256 seeded problems)")
+        line["forms"]["batched"] = batched_form(D)
+        log("forms: served stream")
+        line["forms"]["served"] = served_form(D, args.serve_duration)
     print(json.dumps(line), flush=True)
 
 

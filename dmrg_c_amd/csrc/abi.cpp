@@ -569,9 +569,9 @@ int dmrgx_matrix_autotune(dmrgx_matrix *A, int32_t *lanes_out, int32_t *grid_out
     const bool has[3] = {true, ensure_dict(*A), ensure_delta(*A)};
     for (int v = 0; v < kLoadVariants; v++) {
       if (load_format(v) == kFmtSell) {
-        // sliced form: 1, 2 or 4 lanes per row, plain or XCD-contiguous row ranges (the
-        // other load bits do not apply to it)
-        if ((v & (kLoadPair | kLoadDeep)) != 0) continue;
+        // sliced form: 1, 2 or 4 lanes per row, plain or non-temporal matrix loads, plain or
+        // XCD-contiguous row ranges (bit 0 does not apply to it)
+        if ((v & kLoadPair) != 0) continue;
         for (int g = 1; g <= kSellMaxLanes; g *= 2)
           for (int cap : caps) cands.push_back({g, cap, v, env.time(A, g, cap, v, 3)});
         continue;
@@ -666,7 +666,7 @@ int dmrgx_ground_state(const dmrgx_matrix *A, const dmrgx_solve_params *p, doubl
     }
     if (!p->skip_ii)
       inverse_iteration(op_of(*A), out.eig, v.d.p, p->ii_acc, p->ii_diag_add, p->ii_max_step, p->cg_acc, p->cg_max_step,
-                        out);
+                        out, false, mixed);
     // mixed precision: the FP32 Lanczos value is only a shift for the FP64 refinement;
     // the energy reported is the FP64 Rayleigh quotient of the refined vector
     if (mixed && A->n > 1) {

@@ -128,8 +128,11 @@ __global__ __launch_bounds__(kBlock) void k_seg_spmv(const SegDesc *desc, const 
     double s = 0.0;
     if (r < d.n && fmt == kFmtSell) {
       if constexpr (G <= kSellMaxLanes) {
-        if (d.fp32) s = sell_lane_sum<G, float>(d.sptr, d.slen, d.scol, d.sval32, r, lane, d.x);
-        else s = sell_lane_sum<G, double>(d.sptr, d.slen, d.scol, d.sval, r, lane, d.x);
+        const bool nt = (d.load & kLoadDeep) != 0;
+        if (d.fp32) s = nt ? sell_lane_sum<G, float, true>(d.sptr, d.slen, d.scol, d.sval32, r, lane, d.x)
+                           : sell_lane_sum<G, float>(d.sptr, d.slen, d.scol, d.sval32, r, lane, d.x);
+        else s = nt ? sell_lane_sum<G, double, true>(d.sptr, d.slen, d.scol, d.sval, r, lane, d.x)
+                    : sell_lane_sum<G, double>(d.sptr, d.slen, d.scol, d.sval, r, lane, d.x);
       }
     } else if (r < d.n) {
       switch (d.load & (kLoadXcd - 1)) {
@@ -279,7 +282,9 @@ struct SlotBasis {
 // residual of an inverse-iteration round, or the mixed form's Rayleigh quotient); the
 // engine reads those scalars for all waiting slots together.
 enum Phase { P_LANCZOS, P_CG, P_WAIT, P_DONE };
-enum Wait { W_RESID = 1, W_RQ = 2, W_SHIFT = 3 };   // residual; final Rayleigh quotient; mixed-form shift
+// residual; final Rayleigh quotient; mixed-form shift before the first inner solve and
+// after each later one
+enum Wait { W_RESID = 1, W_RQ = 2, W_SHIFT = 3, W_REFRESH = 4 };
 
 struct Slot {
   int64_t id = 0;
@@ -521,6 +526,10 @@ void wait_resolved(Slot &s, const double *h) {
     s.eig = h[0] / h[1];
     s.out.spmv++;
     begin_ii(s);
+  } else if (s.wait == W_REFRESH) {
+    s.eig = h[0] / h[1];
+    s.out.spmv++;
+    ii_advance(s);
   } else {
     s.eig = h[0] / h[1];
     s.out.spmv++;
@@ -543,6 +552,16 @@ void cg_finished(Slot &s, int done_it) {
   launch_scale_by_rnorm(n, s.nrm2(), s.un.p, s.un.p, st);
   launch_copy(n, s.un.p, s.vec.p, st);
   s.ii_iter++;
+  if (s.mixed && s.ii_iter <= s.p.ii_max_step) {
+    // mixed form: the next residual test and shift at the refined vector's FP64 Rayleigh
+    // quotient (solver.hip:inverse_iteration with rq_shift)
+    launch_rayleigh(op_of(*s.A), s.vec.p, s.rq(), R, st);
+    launch_norm2(n, s.vec.p, R, s.rq() + 1, st);
+    s.phase = P_WAIT;
+    s.epoch++;
+    s.wait = W_REFRESH;
+    return;
+  }
   ii_advance(s);
 }
 
@@ -895,7 +914,7 @@ std::unique_ptr<Inflight> launch_chunk(dmrgx_engine &E, int ticks) {
   }
   for (size_t i = 0; i < W.size(); i++) {
     const size_t o = off[L.size() + Cg.size() + i];
-    const bool rq = W[i]->wait == W_RQ || W[i]->wait == W_SHIFT;
+    const bool rq = W[i]->wait == W_RQ || W[i]->wait == W_SHIFT || W[i]->wait == W_REFRESH;
     off[L.size() + Cg.size() + i + 1] = o + (rq ? 2 : 1);
     gd.push_back({rq ? W[i]->rq() : W[i]->resid(), (int64_t)o, rq ? 2 : 1, 0});
   }

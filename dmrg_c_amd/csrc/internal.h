@@ -287,7 +287,8 @@ constexpr int kLoadXcd = 4;
 // 2-byte column differences within each row plus each row's first column.  Every format
 // gives the CSR products bit for bit (same values, same order).
 // Format 3, sliced: see sell_lane_sum; its lanes per row (1, 2 or 4) set the slice height,
-// and its sums are in an order of their own.
+// and its sums are in an order of their own.  For it bit 1 selects non-temporal matrix
+// loads (bit 0 is ignored); neither changes a result.
 constexpr int kFmtShift = 3;
 constexpr int kFmtCsr = 0, kFmtDict = 1, kFmtDelta = 2, kFmtSell = 3;
 constexpr int kLoadDict = kFmtDict << kFmtShift;
@@ -483,8 +484,10 @@ __device__ __forceinline__ double csr_lane_sum_v(const int64_t *__restrict__ row
 // accumulator k mod 8, eight passes' loads in flight), combined as
 // ((a0 + a4) + (a2 + a6)) + ((a1 + a5) + (a3 + a7)); the P lanes of a row are then combined by
 // the kernel's shuffles.  A fixed order, different from the CSR forms'.
+// With NT the matrix loads are non-temporal (streamed past the caches), which leaves L2 and
+// the Infinity Cache to x when the matrix is far larger than they are.
 constexpr int kSellAcc = 8;
-template <int P, class VT>
+template <int P, class VT, bool NT = false>
 __device__ __forceinline__ double sell_lane_sum(const int64_t *__restrict__ sptr, const int32_t *__restrict__ slen,
                                                 const int32_t *__restrict__ scol, const VT *__restrict__ sval,
                                                 int32_t r, int lane, const double *__restrict__ x) {
@@ -503,8 +506,8 @@ __device__ __forceinline__ double sell_lane_sum(const int64_t *__restrict__ sptr
 #pragma unroll
     for (int t = 0; t < kSellAcc; t++) {
       const int64_t at = base + (int64_t)min(k + t, w - 1) * 64;
-      v[t] = sval[at];
-      c[t] = scol[at];
+      v[t] = stream_load<NT>(sval + at);
+      c[t] = stream_load<NT>(scol + at);
     }
     double xv[kSellAcc];
 #pragma unroll
@@ -719,9 +722,12 @@ struct LanczosOpts {
 // vec: device in/out (start on input, Ritz vector on output)
 void lanczos(const Operator &A, const LanczosOpts &o, double *vec, SolveOut &out);
 // Inverse iteration (shift folded into the operator), CG inner solver; vec device in/out.
-// mr: minimum residual ("MR") instead of conjugate gradient ("CG") for the inner solves
+// mr: minimum residual ("MR") instead of conjugate gradient ("CG") for the inner solves;
+// rq_shift (the mixed form): after each inner solve the residual test and the next shift
+// take the refined vector's FP64 Rayleigh quotient instead of the starting eigenvalue
 void inverse_iteration(const Operator &A, double eig, double *vec, double ii_acc, double ii_diag_add,
-                       int ii_max_step, double cg_acc, int cg_max_step, SolveOut &out, bool mr = false);
+                       int ii_max_step, double cg_acc, int cg_max_step, SolveOut &out, bool mr = false,
+                       bool rq_shift = false);
 // CG solve of (A + shift I) x = b with x normalised first; returns iterations, sets converged.
 int cg_solve(const Operator &A, double shift, const double *b, double *x, double acc, int max_step,
              bool *converged, SolveOut &out);

@@ -87,7 +87,7 @@ struct DeltaRows {
 };
 
 // Sliced form with P lanes per row (sell_lane_sum); only launched with G == P.
-template <class VT, int P>
+template <class VT, int P, bool NT = false>
 struct SellRows {
   const int64_t *__restrict__ sptr;
   const int32_t *__restrict__ slen;
@@ -104,7 +104,7 @@ struct SellRows {
   template <int G>
   __device__ __forceinline__ double lane_sum(int32_t r, int lane, const double *__restrict__ x) const {
     static_assert(G == P, "sliced rows are read with P lanes per row");
-    return sell_lane_sum<P, VT>(sptr, slen, scol, sval, r, lane, x);
+    return sell_lane_sum<P, VT, NT>(sptr, slen, scol, sval, r, lane, x);
   }
 };
 
@@ -409,14 +409,19 @@ int apply_sell_p(const Operator &op, const VT *sval, const double *x, const Epi 
   const dmrgx_matrix &A = *op.A;
   constexpr int lp = P == 1 ? 0 : (P == 2 ? 1 : 2);
   const int32_t nsl = (op.n + 64 / P - 1) / (64 / P);
-  const SellRows<VT, P> rows{A.sell_ptr[lp].p, A.sell_len.p, A.sell_col[lp].p, sval, A.nnz, nsl};
   const int blocks = spmv_blocks(op.n, P, op.grid);
   double *part = nullptr;
   if (Epi::kReduce && blocks > 0) {
     R->ensure(blocks);
     part = R->partial.p;
   }
-  return apply_rows_g<P>(op.n, op.grid, rows, x, epi, part, s, (op.load & kLoadXcd) ? 1 : 0, fin, true);
+  const int xcd = (op.load & kLoadXcd) ? 1 : 0;
+  if (op.load & kLoadDeep) {
+    const SellRows<VT, P, true> rows{A.sell_ptr[lp].p, A.sell_len.p, A.sell_col[lp].p, sval, A.nnz, nsl};
+    return apply_rows_g<P>(op.n, op.grid, rows, x, epi, part, s, xcd, fin, true);
+  }
+  const SellRows<VT, P> rows{A.sell_ptr[lp].p, A.sell_len.p, A.sell_col[lp].p, sval, A.nnz, nsl};
+  return apply_rows_g<P>(op.n, op.grid, rows, x, epi, part, s, xcd, fin, true);
 }
 
 template <class Epi>

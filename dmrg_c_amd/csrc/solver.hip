@@ -449,11 +449,11 @@ int mr_solve(const Operator &A, double shift, const double *b, double *x, double
 }
 
 void inverse_iteration(const Operator &A, double eig, double *vec, double ii_acc, double ii_diag_add,
-                       int ii_max_step, double cg_acc, int cg_max_step, SolveOut &out, bool mr) {
+                       int ii_max_step, double cg_acc, int cg_max_step, SolveOut &out, bool mr, bool rq_shift) {
   Context &C = ctx();
   hipStream_t st = C.stream;
   const int64_t n = A.n;
-  const double shift = ii_diag_add - eig;  // sml/INVERSE_ITERATION_SYM.c:24
+  double shift = ii_diag_add - eig;  // sml/INVERSE_ITERATION_SYM.c:24
   DevBuf<double> un(n), nrm2(1);
   Reduce R;
   launch_copy(n, vec, un.p, st);
@@ -474,6 +474,12 @@ void inverse_iteration(const Operator &A, double eig, double *vec, double ii_acc
     launch_norm2(n, un.p, R, nrm2.p, st);
     launch_scale_by_rnorm(n, nrm2.p, un.p, un.p, st);
     launch_copy(n, un.p, vec, st);
+    if (rq_shift && iter < ii_max_step) {
+      // the next residual test and shift at the refined vector's FP64 Rayleigh quotient
+      eig = rayleigh_quotient(A, vec);
+      out.spmv++;
+      shift = ii_diag_add - eig;
+    }
   }
   HIPCHK(hipStreamSynchronize(st));
   out.err = err;

@@ -17,7 +17,6 @@ from sbhelp import BasisBundle, call_terms, host_ops
 pytestmark = pytest.mark.gpu
 
 SNAPS = golden_snapshots()
-BIG = f"{GOLDEN}/hubbard_L64_m400_s31.npz"
 
 
 @pytest.fixture(autouse=True, scope="module")
@@ -225,11 +224,14 @@ def test_cg_and_converge_check_symbols():
     assert np.linalg.norm(res) < 1e-9
 
 
-def test_big_hubbard_superblock_assembly_and_solve():
-    """Hubbard L=64, m=400 superblock (n=144542): device assembly identical to the oracle's
-    (which equals the reference's bit for bit on the golden set) and E within 1e-10 of the
-    reference run's energy."""
-    s = snapshot.load(BIG)
+@pytest.mark.parametrize("name", ["hubbard_L64_m400_s31", "xxz_L64_m300_s31"])
+def test_big_superblock_assembly_and_solve(name):
+    """Hubbard L=64, m=400 (n=144542) and XXZ L=64, m=300 (n=23524) superblocks of the
+    reference's runs (last infinite-system steps; the served stream's superblocks): device
+    assembly identical to the oracle's (which equals the reference's bit for bit on the
+    golden set) and E within 1e-10 of the reference run's energy."""
+    s = snapshot.load(os.path.join(GOLDEN, name + ".npz"))
+    print(f"big superblock {name} n={s.n}", flush=True)
     sb = D.Superblock(s)
     A = sb.assemble(full=True)
     row, col, val = sb.download_lower()
