@@ -71,6 +71,8 @@ struct SegDesc {
                           // CG_AP: pap                   cg_update: pap, rr      cg_dir: rr
   double *v1, *v2;        // cg_update: x, r
   int *status;            // CG: {done, last iteration}
+  const double *eb;       // cg_update, mixed form: b of the iterate test (CgEarly), else null
+  double eadd, eacc;      //   its diag_add and threshold
 };
 
 // Segment (slot) of block b of a launch: map[b / kMapStride] is the segment of block
@@ -87,11 +89,22 @@ enum FinKind { FIN_LANCZOS_A = 0, FIN_CG_AP = 1, FIN_LANCZOS_B = 2, FIN_CG_UPDAT
 
 // One block per segment: its partials summed in block order (the single-solve k_finish
 // over the segment's blocks), then the kernel's scalar update.
+// For FIN_CG_UPDATE with the iterate test, partial2 holds the update kernel's two extra
+// partial arrays (tot = that launch's block count apart), summed as k_cg_dir's block 0 does.
 template <int KIND>
-__global__ __launch_bounds__(kBlock) void k_seg_finish(const SegDesc *desc, const double *partial) {
+__global__ __launch_bounds__(kBlock) void k_seg_finish(const SegDesc *desc, const double *partial,
+                                                       const double *partial2, int tot) {
   const SegDesc d = desc[blockIdx.x];
   if ((KIND == FIN_CG_AP || KIND == FIN_CG_UPDATE) && d.status[0] != 0) return;
   const double t = finish_sum(partial + d.blk_lo, d.nblk);
+  bool early_done = false;
+  if (KIND == FIN_CG_UPDATE && d.eb) {
+    __syncthreads();   // block_sum's LDS is reused
+    const double e1 = finish_sum(partial2 + d.blk_lo, d.nblk);
+    __syncthreads();
+    const double e2 = finish_sum(partial2 + tot + d.blk_lo, d.nblk);
+    early_done = e2 > 0.0 && e1 < d.eacc * sqrt(e2);   // valid in thread 0
+  }
   if (threadIdx.x != 0) return;
   if (KIND == FIN_LANCZOS_A) {
     *d.s2 = t;                 // alpha_k
@@ -103,7 +116,7 @@ __global__ __launch_bounds__(kBlock) void k_seg_finish(const SegDesc *desc, cons
   } else {
     d.s2[d.k] = t;             // rr_it
     d.status[1] = d.k;
-    if (t < d.acc) d.status[0] = 1;
+    if (t < d.acc || early_done) d.status[0] = 1;
   }
 }
 
@@ -207,23 +220,26 @@ __global__ __launch_bounds__(kBlock) void k_seg_lanczos_b(const SegDesc *desc, c
   block_partial(s, partial);
 }
 
+// With d.eb set (the mixed form) also the iterate test's partials (CgEarly, as k_cg_update):
+// sum |b - r - diag_add x| at partial2[b], |x|^2 at partial2[gridDim.x + b].
 __global__ __launch_bounds__(kBlock) void k_seg_cg_update(const SegDesc *desc, const int32_t *map, int nseg,
-                                                          double *partial) {
+                                                          double *partial, double *partial2) {
   const int seg = block_segment(desc, map, nseg, blockIdx.x);
   const SegDesc d = desc[seg];
   if (d.status[0] != 0) return;
   const int lb = blockIdx.x - d.blk_lo;
   const int it = d.k;
   const double alpha = d.s2[it - 1] / d.s1[it];
-  double s = 0.0;
+  double s = 0.0, e1 = 0.0, e2 = 0.0;
   const int64_t S = (int64_t)d.nblk * kBlock;
   for (int64_t i0 = lb * (int64_t)kBlock + threadIdx.x; i0 < d.n; i0 += 4 * S) {
-    double rv[4];
+    double rv[4], xv[4];
 #pragma unroll
     for (int u = 0; u < 4; u++) {
       const int64_t i = i0 + u * S;
       if (i < d.n) {
-        d.v1[i] = d.v1[i] + alpha * d.o3[i];
+        xv[u] = d.v1[i] + alpha * d.o3[i];
+        d.v1[i] = xv[u];
         rv[u] = d.v2[i] - alpha * d.o1[i];
         d.v2[i] = rv[u];
       }
@@ -231,8 +247,24 @@ __global__ __launch_bounds__(kBlock) void k_seg_cg_update(const SegDesc *desc, c
 #pragma unroll
     for (int u = 0; u < 4; u++)
       if (i0 + u * S < d.n) s += rv[u] * rv[u];
+    if (d.eb) {
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const int64_t i = i0 + u * S;
+        if (i < d.n) {
+          e1 += fabs(d.eb[i] - rv[u] - d.eadd * xv[u]);
+          e2 += xv[u] * xv[u];
+        }
+      }
+    }
   }
   block_partial(s, partial);
+  if (d.eb) {   // uniform per block; block_sum's LDS is reused, hence the barriers
+    __syncthreads();
+    block_partial(e1, partial2);
+    __syncthreads();
+    block_partial_to(e2, partial2 + gridDim.x + blockIdx.x);
+  }
 }
 
 // One block per piece: copies count 8-byte words from src to dst + off, so that the host
@@ -356,6 +388,7 @@ struct dmrgx_engine {
   dmrgx::DevBuf<dmrgx::SegDesc> ddesc;
   dmrgx::DevBuf<int32_t> dblk;
   dmrgx::DevBuf<double> partial;
+  dmrgx::DevBuf<double> partial2;   // the mixed form's CG iterate-test partials
   dmrgx::DevBuf<dmrgx::GatherDesc> dgather;  // per-chunk scalar read-back
   dmrgx::DevBuf<double> dstage;
   // host seconds: building + uploading descriptors, launching, waiting for the chunk,
@@ -673,6 +706,7 @@ void ensure_scratch(dmrgx_engine &E, size_t n_desc, size_t n_map, size_t max_blo
   if (E.ddesc.count < n_desc) E.ddesc.resize(std::max<size_t>(n_desc, 2 * E.ddesc.count));
   if (E.dblk.count < n_map) E.dblk.resize(std::max<size_t>(n_map, 2 * E.dblk.count));
   if (E.partial.count < max_blocks) E.partial.resize(std::max<size_t>(max_blocks, 2 * E.partial.count));
+  if (E.partial2.count < 2 * max_blocks) E.partial2.resize(std::max<size_t>(2 * max_blocks, 2 * E.partial2.count));
 }
 
 template <int MODE>
@@ -858,6 +892,11 @@ std::unique_ptr<Inflight> launch_chunk(dmrgx_engine &E, int ticks) {
         u.v1 = s.xw.p;
         u.v2 = s.r.p;
         u.status = s.cgst.p;
+        if (s.mixed) {   // the iterate test of the mixed form's inner solves (CgEarly)
+          u.eb = s.vec.p;
+          u.eadd = s.p.ii_diag_add;
+          u.eacc = kCgEarlyFraction * s.p.ii_acc;
+        }
         su.push_back(u);
         SegDesc dd{};
         dd.n = s.n;
@@ -946,19 +985,19 @@ std::unique_ptr<Inflight> launch_chunk(dmrgx_engine &E, int ticks) {
       switch (l.kind) {
         case 0:
           launch_seg_spmv<LANCZOS_A>(l.G, l.nblk, dd, bb, l.nseg, E.partial.p, st);
-          k_seg_finish<FIN_LANCZOS_A><<<l.nseg, kBlock, 0, st>>>(dd, E.partial.p);
+          k_seg_finish<FIN_LANCZOS_A><<<l.nseg, kBlock, 0, st>>>(dd, E.partial.p, nullptr, 0);
           break;
         case 1:
           k_seg_lanczos_b<<<l.nblk, kBlock, 0, st>>>(dd, bb, l.nseg, E.partial.p);
-          k_seg_finish<FIN_LANCZOS_B><<<l.nseg, kBlock, 0, st>>>(dd, E.partial.p);
+          k_seg_finish<FIN_LANCZOS_B><<<l.nseg, kBlock, 0, st>>>(dd, E.partial.p, nullptr, 0);
           break;
         case 2:
           launch_seg_spmv<CG_AP>(l.G, l.nblk, dd, bb, l.nseg, E.partial.p, st);
-          k_seg_finish<FIN_CG_AP><<<l.nseg, kBlock, 0, st>>>(dd, E.partial.p);
+          k_seg_finish<FIN_CG_AP><<<l.nseg, kBlock, 0, st>>>(dd, E.partial.p, nullptr, 0);
           break;
         case 3:
-          k_seg_cg_update<<<l.nblk, kBlock, 0, st>>>(dd, bb, l.nseg, E.partial.p);
-          k_seg_finish<FIN_CG_UPDATE><<<l.nseg, kBlock, 0, st>>>(dd, E.partial.p);
+          k_seg_cg_update<<<l.nblk, kBlock, 0, st>>>(dd, bb, l.nseg, E.partial.p, E.partial2.p);
+          k_seg_finish<FIN_CG_UPDATE><<<l.nseg, kBlock, 0, st>>>(dd, E.partial.p, E.partial2.p, l.nblk);
           break;
         case 4: k_seg_cg_dir<<<l.nblk, kBlock, 0, st>>>(dd, bb, l.nseg); break;
       }
@@ -978,6 +1017,7 @@ std::unique_ptr<Inflight> launch_chunk(dmrgx_engine &E, int ticks) {
   sig.push_back((int64_t)(intptr_t)E.ddesc.p);
   sig.push_back((int64_t)(intptr_t)E.dblk.p);
   sig.push_back((int64_t)(intptr_t)E.partial.p);
+  sig.push_back((int64_t)(intptr_t)E.partial2.p);
   if (E.opts.use_graphs && !launches.empty() && E.graph && sig == E.graph_sig) {
     HIPCHK(hipGraphLaunch(E.graph, st));
     E.n_replays++;

@@ -676,11 +676,23 @@ int launch_lanczos_b_part(int64_t n, int k, const double *y, const double *vk, c
 void launch_lanczos_b_finish(int k, double *beta, double *nrm2, const Reduce &R, int nblk, hipStream_t s);
 int launch_cg_ap_part(const Operator &op, double shift, const double *p, double *y, const int *done, Reduce &R,
                       hipStream_t s);
+// The mixed form's test of the CG iterate (inverse iteration with an inexact inner solve):
+// with r = b - (H + d - eig) x the recursively updated residual, H u - eig u =
+// (b - r - d x) / |x| for u = x / |x|, so sum_i |b_i - r_i - d x_i| / |x| is the L1 residual
+// that the inverse iteration's convergence test takes of the normalised iterate.  When b is
+// set, the CG update kernel also writes the block partials of sum |b - r - d x| and of |x|^2
+// (partial[0..nblk) and partial[nblk..2 nblk)), and the direction kernel's block 0 ends the
+// solve (status[0] = 1) once their quotient is below acc.
+struct CgEarly {
+  const double *b = nullptr;
+  double diag_add = 0.0, acc = 0.0;
+  double *partial = nullptr;
+};
 int launch_cg_update_part(int64_t n, int it, const double *p, const double *y, double *x, double *r, double *pap,
                           const double *rr, const int *status, const double *pre, int pre_n, Reduce &R,
-                          hipStream_t s);
+                          hipStream_t s, const CgEarly &early = CgEarly{});
 void launch_cg_dir_part(int64_t n, int it, double acc, const double *r, double *rr, double *p, int *status,
-                        const double *pre, int pre_n, hipStream_t s);
+                        const double *pre, int pre_n, hipStream_t s, const CgEarly &early = CgEarly{});
 
 // single-slot operator products: y = O x (y may be null) and, if dot_with != null,
 // *dot_out = <dot_with, O x> (deterministic reduction)
@@ -724,13 +736,18 @@ void lanczos(const Operator &A, const LanczosOpts &o, double *vec, SolveOut &out
 // Inverse iteration (shift folded into the operator), CG inner solver; vec device in/out.
 // mr: minimum residual ("MR") instead of conjugate gradient ("CG") for the inner solves;
 // rq_shift (the mixed form): after each inner solve the residual test and the next shift
-// take the refined vector's FP64 Rayleigh quotient instead of the starting eigenvalue
+// take the refined vector's FP64 Rayleigh quotient instead of the starting eigenvalue, and
+// an inner CG solve also ends as soon as its normalised iterate passes half the residual
+// test (CgEarly); the test after the solve decides as before
 void inverse_iteration(const Operator &A, double eig, double *vec, double ii_acc, double ii_diag_add,
                        int ii_max_step, double cg_acc, int cg_max_step, SolveOut &out, bool mr = false,
                        bool rq_shift = false);
+// the fraction of ii_acc at which the mixed form's inner solve stops early
+constexpr double kCgEarlyFraction = 0.5;
 // CG solve of (A + shift I) x = b with x normalised first; returns iterations, sets converged.
+// early (optional): diag_add and acc of the iterate test above; b and partial are set here.
 int cg_solve(const Operator &A, double shift, const double *b, double *x, double acc, int max_step,
-             bool *converged, SolveOut &out);
+             bool *converged, SolveOut &out, const CgEarly *early = nullptr);
 // Minimum-residual solve of (A + shift I) x = b from x as given (not normalised); x in/out.
 int mr_solve(const Operator &A, double shift, const double *b, double *x, double acc, int max_step,
              bool *converged, SolveOut &out);

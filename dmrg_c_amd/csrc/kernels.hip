@@ -586,7 +586,8 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(int64_t n, int it, double 
                                                       const double *__restrict__ y,
                                                       double *__restrict__ x, double *__restrict__ r,
                                                       double *pap, const double *rr, const int *status,
-                                                      double *partial, const double *pre, int pre_n) {
+                                                      double *partial, const double *pre, int pre_n,
+                                                      CgEarly early) {
   if (status[0] != 0) return;
   // the first pass's operands are loaded before pAp is known (see k_lanczos_b)
   const int64_t S = (int64_t)gridDim.x * kBlock;
@@ -615,15 +616,16 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(int64_t n, int it, double 
     pAp = pap[it];
   }
   const double alpha = rr[it - 1] / pAp;
-  double s = 0.0;
+  double s = 0.0, e1 = 0.0, e2 = 0.0;
   for (int64_t i0 = first; i0 < n; i0 += 4 * S) {
-    double rv[4];
+    double rv[4], xv[4];
 #pragma unroll
     for (int u = 0; u < 4; u++) {
       const int64_t i = i0 + u * S;
       if (i < n) {
         const bool pf = i0 == first;
-        x[i] = (pf ? fx[u] : x[i]) + alpha * (pf ? fp[u] : p[i]);
+        xv[u] = (pf ? fx[u] : x[i]) + alpha * (pf ? fp[u] : p[i]);
+        x[i] = xv[u];
         rv[u] = (pf ? fr[u] : r[i]) - alpha * (pf ? fy[u] : y[i]);
         r[i] = rv[u];
       }
@@ -631,8 +633,24 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(int64_t n, int it, double 
 #pragma unroll
     for (int u = 0; u < 4; u++)
       if (i0 + u * S < n) s += rv[u] * rv[u];
+    if (early.b) {
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const int64_t i = i0 + u * S;
+        if (i < n) {
+          e1 += fabs(early.b[i] - rv[u] - early.diag_add * xv[u]);
+          e2 += xv[u] * xv[u];
+        }
+      }
+    }
   }
   block_partial(s, partial);
+  if (early.b) {   // uniform per launch; block_sum's LDS is reused, hence the barriers
+    __syncthreads();
+    block_partial(e1, early.partial);
+    __syncthreads();
+    block_partial_to(e2, early.partial + gridDim.x + blockIdx.x);
+  }
 }
 
 struct FinCgUpdate {
@@ -654,7 +672,7 @@ struct FinCgUpdate {
 // a converged iteration leaves p alone.
 __global__ __launch_bounds__(kBlock) void k_cg_dir(int64_t n, int it, const double *__restrict__ r,
                                                    double *rr, double *__restrict__ p, int *status,
-                                                   const double *pre, int pre_n, double acc) {
+                                                   const double *pre, int pre_n, double acc, CgEarly early) {
   if (pre) {
     // block 0 of this very kernel may set status[0]: read it once per block, so that all
     // threads of a block take the same branch (the reduction below has barriers)
@@ -667,12 +685,22 @@ __global__ __launch_bounds__(kBlock) void k_cg_dir(int64_t n, int it, const doub
     const int64_t i1 = blockIdx.x * (int64_t)kBlock + threadIdx.x;
     const double r1 = i1 < n ? r[i1] : 0.0, p1 = i1 < n ? p[i1] : 0.0;
     const double t = finish_sum(pre, pre_n);
+    // the iterate test (CgEarly), block 0 only (a uniform branch: every thread of the block
+    // takes part in the two sums)
+    bool early_done = false;
+    if (early.b && blockIdx.x == 0) {
+      __syncthreads();   // block_sum's LDS is reused
+      const double e1 = finish_sum(early.partial, pre_n);
+      __syncthreads();
+      const double e2 = finish_sum(early.partial + pre_n, pre_n);
+      early_done = e2 > 0.0 && e1 < early.acc * sqrt(e2);   // valid in thread 0
+    }
     if (threadIdx.x == 0) {
       bc = t;
       if (blockIdx.x == 0) {
         rr[it] = t;
         status[1] = it;
-        if (t < acc) status[0] = 1;
+        if (t < acc || early_done) status[0] = 1;
       }
     }
     __syncthreads();
@@ -1080,7 +1108,7 @@ void launch_cg_update(int64_t n, int it, double acc, const double *p, const doub
   const int g = reduce_grid(n);
   R.ensure(g);
   k_cg_update<<<g, kBlock, 0, s>>>(n, it, acc, p, y, x, r, const_cast<double *>(pap), rr, status, R.partial.p,
-                                   nullptr, 0);
+                                   nullptr, 0, CgEarly{});
   k_finish<FinCgUpdate><<<1, kBlock, 0, s>>>(FinCgUpdate{rr, status, it, acc}, R.partial.p, g);
   HIPCHK(hipGetLastError());
 }
@@ -1088,7 +1116,7 @@ void launch_cg_update(int64_t n, int it, double acc, const double *p, const doub
 void launch_cg_dir(int64_t n, int it, const double *r, const double *rr, double *p, const int *status,
                    hipStream_t s) {
   k_cg_dir<<<vec_grid(n), kBlock, 0, s>>>(n, it, r, const_cast<double *>(rr), p, const_cast<int *>(status),
-                                          nullptr, 0, 0.0);
+                                          nullptr, 0, 0.0, CgEarly{});
   HIPCHK(hipGetLastError());
 }
 
@@ -1099,17 +1127,17 @@ int launch_cg_ap_part(const Operator &op, double shift, const double *p, double 
 
 int launch_cg_update_part(int64_t n, int it, const double *p, const double *y, double *x, double *r, double *pap,
                           const double *rr, const int *status, const double *pre, int pre_n, Reduce &R,
-                          hipStream_t s) {
+                          hipStream_t s, const CgEarly &early) {
   const int g = reduce_grid(n);
   R.ensure(g);
-  k_cg_update<<<g, kBlock, 0, s>>>(n, it, 0.0, p, y, x, r, pap, rr, status, R.partial.p, pre, pre_n);
+  k_cg_update<<<g, kBlock, 0, s>>>(n, it, 0.0, p, y, x, r, pap, rr, status, R.partial.p, pre, pre_n, early);
   HIPCHK(hipGetLastError());
   return g;
 }
 
 void launch_cg_dir_part(int64_t n, int it, double acc, const double *r, double *rr, double *p, int *status,
-                        const double *pre, int pre_n, hipStream_t s) {
-  k_cg_dir<<<vec_grid(n), kBlock, 0, s>>>(n, it, r, rr, p, status, pre, pre_n, acc);
+                        const double *pre, int pre_n, hipStream_t s, const CgEarly &early) {
+  k_cg_dir<<<vec_grid(n), kBlock, 0, s>>>(n, it, r, rr, p, status, pre, pre_n, acc, early);
   HIPCHK(hipGetLastError());
 }
 

@@ -312,11 +312,17 @@ int effective_load(const dmrgx_matrix &A) {
 }
 
 int cg_solve(const Operator &A, double shift, const double *b, double *x, double acc, int max_step,
-             bool *converged, SolveOut &out) {
+             bool *converged, SolveOut &out, const CgEarly *early) {
   Context &C = ctx();
   hipStream_t st = C.stream;
   const int64_t n = A.n;
   DevBuf<double> r(n), p(n), y(n), xw(n);
+  Reduce RE;   // the iterate test's two block-partial arrays
+  CgEarly ce;
+  if (early) {
+    RE.ensure(2 * (size_t)reduce_grid(n));
+    ce = CgEarly{b, early->diag_add, early->acc, RE.partial.p};
+  }
   const int slots = std::max(max_step, 1) + 1;
   DevBuf<double> sc(2 * (size_t)slots + 1);
   double *rr = sc.p, *pap = sc.p + slots, *nrm2 = sc.p + 2 * slots;
@@ -347,8 +353,9 @@ int cg_solve(const Operator &A, double shift, const double *b, double *x, double
     // (pAp in the update, |r|^2 and the convergence flag in the direction kernel)
     for (int i = it; i < it + c; i++) {
       const int na = launch_cg_ap_part(A, shift, p.p, y.p, status.p, RA, st);
-      const int nb = launch_cg_update_part(n, i, p.p, y.p, xw.p, r.p, pap, rr, status.p, RA.partial.p, na, RB, st);
-      launch_cg_dir_part(n, i, acc, r.p, rr, p.p, status.p, RB.partial.p, nb, st);
+      const int nb =
+          launch_cg_update_part(n, i, p.p, y.p, xw.p, r.p, pap, rr, status.p, RA.partial.p, na, RB, st, ce);
+      launch_cg_dir_part(n, i, acc, r.p, rr, p.p, status.p, RB.partial.p, nb, st, ce);
     }
     HIPCHK(hipMemcpyAsync(q.h, status.p, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
     HIPCHK(hipMemcpyAsync(q.h + 1, rr + it + c - 1, sizeof(double), hipMemcpyDeviceToHost, st));
@@ -468,8 +475,11 @@ void inverse_iteration(const Operator &A, double eig, double *vec, double ii_acc
       break;
     }
     bool conv = false;
+    CgEarly early;
+    early.diag_add = ii_diag_add;
+    early.acc = kCgEarlyFraction * ii_acc;
     out.cg_steps += mr ? mr_solve(A, shift, vec, un.p, cg_acc, cg_max_step, &conv, out)
-                       : cg_solve(A, shift, vec, un.p, cg_acc, cg_max_step, &conv, out);
+                       : cg_solve(A, shift, vec, un.p, cg_acc, cg_max_step, &conv, out, rq_shift ? &early : nullptr);
     if (!conv) out.cg_all_converged = false;
     launch_norm2(n, un.p, R, nrm2.p, st);
     launch_scale_by_rnorm(n, nrm2.p, un.p, un.p, st);
