@@ -56,6 +56,13 @@ SPMV_FORMATS = [("CSR", "k_apply<CsrRows<double>> (CSR SpMV + fused Lanczos/CG e
                 ("delta columns", "k_apply<DeltaRows<double>> (delta-column CRS SpMV + fused Lanczos/CG epilogue)"),
                 ("sliced", "k_apply<P, SellRows<double, P>> (sliced SpMV: 64/P-row slices, P lanes per row, "
                            "+ fused Lanczos/CG epilogue)")]
+SELL16_KERNEL = ("k_apply<P, Sell16Rows<double, P>> (sliced SpMV with 16-bit column differences: 64/P-row "
+                 "slices, P lanes per row, + fused Lanczos/CG epilogue)")
+
+
+def kernel_name(sched):
+    fmt = sched[2] >> 3
+    return SELL16_KERNEL if fmt == 3 and sched[2] & 1 else SPMV_FORMATS[fmt][1]
 
 
 def log(msg):
@@ -242,8 +249,6 @@ def spmv_sweep(D, sb, snap):
     rows = []
     for v in range(32):
         for g in ((1, 2, 4) if (v >> 3) == 3 else (4, 8, 16, 32, 64)):
-            if (v >> 3) == 3 and v & 1:
-                continue
             for cap in (1024, 2048, 4096, 8192, 16384, 0):
                 us = A.time_schedule(g, cap, v, 20)
                 rows.append({"lanes": g, "cap": cap, "variant": v, "us": round(us, 2),
@@ -264,6 +269,11 @@ def spmv_alg_bytes(D, A, sched, n):
     values, 4-byte columns) or the value dictionary (8-byte packed entries + the table)."""
     if (sched[2] >> 3) == 3:
         nsl = (n + 64 // sched[0] - 1) // (64 // sched[0])
+        if sched[2] & 1:
+            exc = A.sell16_exceptions(sched[0])
+            return (A.nnz * 10 + n * 4 + (nsl + 1) * 8 + nsl * 64 * 12 + exc * 4 + 2 * n * 8,
+                    f"sliced, 16-bit columns ({sched[0]} lanes per row; 8-byte values, 2-byte column differences, "
+                    f"{exc} columns kept whole, padding not counted)")
         return (A.nnz * 12 + n * 4 + (nsl + 1) * 8 + 2 * n * 8,
                 f"sliced ({sched[0]} lanes per row; 8-byte values, 4-byte columns, padding not counted)")
     if (sched[2] >> 3) == 2:
@@ -847,11 +857,15 @@ def run_superblock(D, L, name, steps, warmup, fixed, cpu_budget):
     avg_launch_s = spmv_ms.value / 1e3 / max(launches.value, 1)
     bytes_per_launch = alg_bytes.value / max(launches.value, 1)
     achieved = bytes_per_launch / avg_launch_s / 1e9 if avg_launch_s > 0 else 0.0
-    traffic = None
+    traffic, traffic_note = None, None
     tfile = os.path.join(REPO, "profiles", f"spmv_{name}_traffic.json")
     if os.path.exists(tfile):
         with open(tfile) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch")
+            tj = json.load(f)
+        traffic = tj.get("hbm_bytes_per_launch")
+        if list(tj.get("schedule", [])) != list(lanes):
+            traffic_note = (f"rocprof counters of the schedule {tj.get('schedule')} "
+                            f"(this run's autotuner picked {list(lanes)})")
     fmt = lanes[2] >> 3
     res = {
         "value": round(steps / total, 4),
@@ -871,6 +885,7 @@ def run_superblock(D, L, name, steps, warmup, fixed, cpu_budget):
                               "paired_entry_loads": bool(lanes[2] & 1) and fmt == 0,
                               "entries_in_flight_per_lane": 8 if (lanes[2] & 2 or fmt == 3) else 4,
                               "non_temporal_matrix_loads": bool(lanes[2] & 2) and fmt == 3,
+                              "column_bits": 16 if (fmt == 3 and lanes[2] & 1) else 32,
                               "xcd_contiguous_rows": bool(lanes[2] & 4),
                               "format": SPMV_FORMATS[fmt][0]},
         },
@@ -886,12 +901,13 @@ def run_superblock(D, L, name, steps, warmup, fixed, cpu_budget):
                                  "residual test) and Rayleigh quotient"},
         "roofline": {
             "bound": "hbm",
-            "kernel": SPMV_FORMATS[fmt][1],
+            "kernel": kernel_name(lanes),
             "achieved": round(achieved, 1),
             "peak": PEAK_HBM_GBS,
             "unit": "GB/s",
             "frac": round(achieved / PEAK_HBM_GBS, 4),
             "traffic": traffic,
+            **({"traffic_note": traffic_note} if traffic_note else {}),
             "alg_bytes_per_launch": int(bytes_per_launch),
             "avg_launch_us": round(avg_launch_s * 1e6, 2),
             "launches": int(launches.value),

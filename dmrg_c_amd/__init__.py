@@ -223,7 +223,8 @@ class DeviceMatrix:
         """(lanes per row, workgroup cap, load variant) of this matrix's SpMV; load variant
         bit 0 = two entries per lane per load (CSR), bit 1 = eight entries in flight per lane,
         bit 2 = contiguous row ranges per workgroup and per XCD, bits 3-4 = storage format
-        (0 CSR, 1 value dictionary, 2 delta columns)."""
+        (0 CSR, 1 value dictionary, 2 delta columns, 3 sliced; for the sliced form bit 0 =
+        16-bit column differences, bit 1 = non-temporal matrix loads)."""
         g, cap, nt = C.c_int32(), C.c_int32(), C.c_int32()
         check(lib().dmrgx_matrix_get_schedule(self.handle, C.byref(g), C.byref(cap), C.byref(nt)))
         return g.value, cap.value, nt.value
@@ -237,6 +238,14 @@ class DeviceMatrix:
         g, cap, nt, us = C.c_int32(), C.c_int32(), C.c_int32(), C.c_double()
         check(lib().dmrgx_matrix_autotune(self.handle, C.byref(g), C.byref(cap), C.byref(nt), C.byref(us)))
         return (g.value, cap.value, nt.value), us.value
+
+    def sell16_exceptions(self, lanes: int) -> int:
+        """Columns the sliced form's 16-bit column form (lanes per row 1, 2 or 4) keeps whole."""
+        e = C.c_int64()
+        L = lib()
+        L.dmrgx_matrix_sell16_exceptions.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_int64)]
+        check(L.dmrgx_matrix_sell16_exceptions(self.handle, lanes, C.byref(e)))
+        return e.value
 
     def time_schedule(self, lanes: int, grid_cap: int, load_variant: int = 0, reps: int = 10) -> float:
         """Mean microseconds of one product + fused dot under that schedule."""

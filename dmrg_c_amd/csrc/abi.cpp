@@ -552,6 +552,15 @@ int dmrgx_matrix_dict_values(dmrgx_matrix *A, int64_t *n_values) {
   });
 }
 
+int dmrgx_matrix_sell16_exceptions(dmrgx_matrix *A, int32_t lanes, int64_t *exceptions) {
+  if (!A || !exceptions || (lanes != 1 && lanes != 2 && lanes != 4)) return DMRGX_ERR_ARG;
+  std::lock_guard<std::recursive_mutex> lk(api_mutex());
+  return run_status([&] {
+    ensure_sell(*A, lanes);
+    *exceptions = ensure_sell16(*A, lanes);
+  });
+}
+
 int dmrgx_matrix_autotune(dmrgx_matrix *A, int32_t *lanes_out, int32_t *grid_out, int32_t *load_out,
                           double *best_us_out) {
   if (!A) return DMRGX_ERR_ARG;
@@ -569,9 +578,8 @@ int dmrgx_matrix_autotune(dmrgx_matrix *A, int32_t *lanes_out, int32_t *grid_out
     const bool has[3] = {true, ensure_dict(*A), ensure_delta(*A)};
     for (int v = 0; v < kLoadVariants; v++) {
       if (load_format(v) == kFmtSell) {
-        // sliced form: 1, 2 or 4 lanes per row, plain or non-temporal matrix loads, plain or
-        // XCD-contiguous row ranges (bit 0 does not apply to it)
-        if ((v & kLoadPair) != 0) continue;
+        // sliced form: 1, 2 or 4 lanes per row, 32- or 16-bit columns, plain or
+        // non-temporal matrix loads, plain or XCD-contiguous row ranges
         for (int g = 1; g <= kSellMaxLanes; g *= 2)
           for (int cap : caps) cands.push_back({g, cap, v, env.time(A, g, cap, v, 3)});
         continue;

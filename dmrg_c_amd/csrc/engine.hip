@@ -60,6 +60,9 @@ struct SegDesc {
   const int32_t *slen, *scol;
   const double *sval;
   const float *sval32;
+  const uint16_t *sd16;   // sliced form's 16-bit columns (load bit 0)
+  const int32_t *sfirst, *sexc;
+  const int64_t *septr;
   int32_t n, fp32;
   int32_t blk_lo, nblk;   // this slot's blocks in the launch
   int32_t k, load;        // step (Lanczos) or iteration (CG); SpMV load variant
@@ -142,10 +145,24 @@ __global__ __launch_bounds__(kBlock) void k_seg_spmv(const SegDesc *desc, const 
     if (r < d.n && fmt == kFmtSell) {
       if constexpr (G <= kSellMaxLanes) {
         const bool nt = (d.load & kLoadDeep) != 0;
-        if (d.fp32) s = nt ? sell_lane_sum<G, float, true>(d.sptr, d.slen, d.scol, d.sval32, r, lane, d.x)
-                           : sell_lane_sum<G, float>(d.sptr, d.slen, d.scol, d.sval32, r, lane, d.x);
-        else s = nt ? sell_lane_sum<G, double, true>(d.sptr, d.slen, d.scol, d.sval, r, lane, d.x)
-                    : sell_lane_sum<G, double>(d.sptr, d.slen, d.scol, d.sval, r, lane, d.x);
+        if (d.sd16) {
+          if (d.fp32)
+            s = nt ? sell16_lane_sum<G, float, true>(d.sptr, d.slen, d.sd16, d.sfirst, d.septr, d.sexc, d.sval32, r,
+                                                     lane, d.x)
+                   : sell16_lane_sum<G, float>(d.sptr, d.slen, d.sd16, d.sfirst, d.septr, d.sexc, d.sval32, r,
+                                               lane, d.x);
+          else
+            s = nt ? sell16_lane_sum<G, double, true>(d.sptr, d.slen, d.sd16, d.sfirst, d.septr, d.sexc, d.sval, r,
+                                                      lane, d.x)
+                   : sell16_lane_sum<G, double>(d.sptr, d.slen, d.sd16, d.sfirst, d.septr, d.sexc, d.sval, r, lane,
+                                                d.x);
+        } else if (d.fp32) {
+          s = nt ? sell_lane_sum<G, float, true>(d.sptr, d.slen, d.scol, d.sval32, r, lane, d.x)
+                 : sell_lane_sum<G, float>(d.sptr, d.slen, d.scol, d.sval32, r, lane, d.x);
+        } else {
+          s = nt ? sell_lane_sum<G, double, true>(d.sptr, d.slen, d.scol, d.sval, r, lane, d.x)
+                 : sell_lane_sum<G, double>(d.sptr, d.slen, d.scol, d.sval, r, lane, d.x);
+        }
       }
     } else if (r < d.n) {
       switch (d.load & (kLoadXcd - 1)) {
@@ -445,6 +462,12 @@ void set_sell(SegDesc &a, const dmrgx_matrix &A) {
   a.scol = A.sell_col[lp].p;
   a.sval = A.sell_val[lp].p;
   a.sval32 = A.sell_val32[lp].p;
+  if (load_format(A.load) == kFmtSell && (A.load & kLoadPair) && A.sell_nexc[lp] >= 0) {   // 16-bit columns
+    a.sd16 = A.sell_d16[lp].p;
+    a.sfirst = A.sell_first[lp].p;
+    a.septr = A.sell_eptr[lp].p;
+    a.sexc = A.sell_exc[lp].p;
+  }
 }
 
 // Start a slot: start vector and |w|^2 (solver.hip:lanczos before its first step).
@@ -465,7 +488,10 @@ void start_slot(Slot &s) {
   if (s.mixed) ensure_fp32(*s.A);
   if (load_format(s.A->load) == kFmtDict) ensure_dict(*s.A);
   if (load_format(s.A->load) == kFmtDelta) ensure_delta(*s.A);
-  if (load_format(s.A->load) == kFmtSell) ensure_sell(*s.A, s.A->lanes, s.mixed);
+  if (load_format(s.A->load) == kFmtSell) {
+    ensure_sell(*s.A, s.A->lanes, s.mixed);
+    if (s.A->load & kLoadPair) ensure_sell16(*s.A, s.A->lanes);
+  }
   launch_start_vector(n, s.p.seed, s.vec.p, st);
   launch_copy(n, s.vec.p, s.w.p, st);
   Reduce R;

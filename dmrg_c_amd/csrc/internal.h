@@ -142,6 +142,17 @@ struct dmrgx_matrix {
   dmrgx::DevBuf<double> sell_val[3];
   dmrgx::DevBuf<float> sell_val32[3];
   dmrgx::DevBuf<int32_t> sell_len;
+  // 16-bit columns of the sliced form (per P, built on first use).  A sequence is one of
+  // the 64 slot columns of a slice (the slots one lane of one row reads, in order).
+  // sell_d16 holds at every slot the column's difference from the sequence's previous slot,
+  // or 0xFFFF where that does not fit in 0..65534, in which case the column itself is the
+  // sequence's next entry of sell_exc (from sell_eptr[sequence]); sell_first is each
+  // sequence's first column.  sell_nexc: exceptions, -1 while not built.
+  dmrgx::DevBuf<uint16_t> sell_d16[3];
+  dmrgx::DevBuf<int32_t> sell_first[3];
+  dmrgx::DevBuf<int64_t> sell_eptr[3];
+  dmrgx::DevBuf<int32_t> sell_exc[3];
+  int64_t sell_nexc[3] = {-1, -1, -1};
   int lanes = 8;                // lanes per row of the SpMV kernel
   int grid = dmrgx::kGridCap;   // cap on SpMV workgroups (0: one per row group)
   int load = 0;                 // load variant (kLoad* bits)
@@ -287,8 +298,8 @@ constexpr int kLoadXcd = 4;
 // 2-byte column differences within each row plus each row's first column.  Every format
 // gives the CSR products bit for bit (same values, same order).
 // Format 3, sliced: see sell_lane_sum; its lanes per row (1, 2 or 4) set the slice height,
-// and its sums are in an order of their own.  For it bit 1 selects non-temporal matrix
-// loads (bit 0 is ignored); neither changes a result.
+// and its sums are in an order of their own.  For it bit 0 selects the 16-bit column form
+// (sell16_lane_sum) and bit 1 non-temporal matrix loads; neither changes a result.
 constexpr int kFmtShift = 3;
 constexpr int kFmtCsr = 0, kFmtDict = 1, kFmtDelta = 2, kFmtSell = 3;
 constexpr int kLoadDict = kFmtDict << kFmtShift;
@@ -512,6 +523,65 @@ __device__ __forceinline__ double sell_lane_sum(const int64_t *__restrict__ sptr
     double xv[kSellAcc];
 #pragma unroll
     for (int t = 0; t < kSellAcc; t++) xv[t] = x[c[t]];
+#pragma unroll
+    for (int t = 0; t < kSellAcc; t++)
+      if ((k + t) * P + lane < len) acc[t] += (double)v[t] * xv[t];
+  }
+  return ((acc[0] + acc[4]) + (acc[2] + acc[6])) + ((acc[1] + acc[5]) + (acc[3] + acc[7]));
+}
+
+// sell_lane_sum over the 16-bit column form: the same slots, values, order and masks, the
+// columns rebuilt from the sequence's first column and the differences (an 0xFFFF slot
+// takes the sequence's next exception, which is loaded one exception ahead so that its
+// latency hides behind the slots in between).  10 bytes per FP64 entry instead of 12, and
+// the same sums bit for bit.
+template <int P, class VT, bool NT = false>
+__device__ __forceinline__ double sell16_lane_sum(const int64_t *__restrict__ sptr, const int32_t *__restrict__ slen,
+                                                  const uint16_t *__restrict__ sd16,
+                                                  const int32_t *__restrict__ sfirst,
+                                                  const int64_t *__restrict__ septr,
+                                                  const int32_t *__restrict__ sexc, const VT *__restrict__ sval,
+                                                  int32_t r, int lane, const double *__restrict__ x) {
+  constexpr int C = 64 / P;
+  const int32_t s = r / C;
+  const int j = (r % C) * P + lane;
+  const int64_t b0 = sptr[s];
+  const int64_t base = b0 + j;
+  const int32_t w = (int32_t)((sptr[s + 1] - b0) >> 6);
+  const int32_t len = slen[r];
+  const int64_t q = (int64_t)s * 64 + j;
+  int32_t c = sfirst[q];
+  int64_t e = septr[q];
+  int32_t en = sexc[e];   // sell_exc holds one entry past the last exception
+  double acc[kSellAcc];
+#pragma unroll
+  for (int t = 0; t < kSellAcc; t++) acc[t] = 0.0;
+  for (int32_t k = 0; k < w; k += kSellAcc) {
+    VT v[kSellAcc];
+    uint16_t d[kSellAcc];
+#pragma unroll
+    for (int t = 0; t < kSellAcc; t++) {
+      const int64_t at = base + (int64_t)min(k + t, w - 1) * 64;
+      v[t] = stream_load<NT>(sval + at);
+      d[t] = stream_load<NT>(sd16 + at);
+    }
+    int32_t cc[kSellAcc];
+#pragma unroll
+    for (int t = 0; t < kSellAcc; t++) {
+      const int32_t kk = k + t;
+      if (kk > 0 && kk < w) {
+        if (d[t] == 0xFFFF) {
+          c = en;
+          en = sexc[++e];
+        } else {
+          c += d[t];
+        }
+      }
+      cc[t] = c;
+    }
+    double xv[kSellAcc];
+#pragma unroll
+    for (int t = 0; t < kSellAcc; t++) xv[t] = x[cc[t]];
 #pragma unroll
     for (int t = 0; t < kSellAcc; t++)
       if ((k + t) * P + lane < len) acc[t] += (double)v[t] * xv[t];
@@ -765,6 +835,9 @@ bool ensure_delta(const dmrgx_matrix &A);
 // with fp32); false for any other P.
 bool ensure_sell(const dmrgx_matrix &A, int P, bool fp32 = false);
 void build_sell(dmrgx_matrix &A, int P, bool fp32, hipStream_t s);
+// the sliced form's 16-bit columns for P lanes per row (after ensure_sell); exceptions
+int64_t ensure_sell16(const dmrgx_matrix &A, int P);
+int64_t build_sell16(dmrgx_matrix &A, int P, hipStream_t s);
 // frees the sliced forms other than P's (all with P = 0)
 void drop_sell_except(const dmrgx_matrix &A, int P);
 // A's load variant restricted to the formats it has (a format it cannot have reads CSR).

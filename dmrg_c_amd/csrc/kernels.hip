@@ -108,6 +108,33 @@ struct SellRows {
   }
 };
 
+// The sliced form with 16-bit columns (sell16_lane_sum); only launched with G == P.
+template <class VT, int P, bool NT = false>
+struct Sell16Rows {
+  const int64_t *__restrict__ sptr;
+  const int32_t *__restrict__ slen;
+  const uint16_t *__restrict__ sd16;
+  const int32_t *__restrict__ sfirst;
+  const int64_t *__restrict__ septr;
+  const int32_t *__restrict__ sexc;
+  const VT *__restrict__ sval;
+  int64_t nnz, nexc;
+  int32_t nslices;
+  static constexpr bool kTimed = true;
+  // algorithmic bytes of one y = H x: values and 2-byte column differences of the entries
+  // (padding not counted) + row lengths + slice offsets + per-sequence first column and
+  // exception offset + the exceptions + x once + y once
+  double alg_bytes(int32_t n) const {
+    return (double)nnz * (sizeof(VT) + 2) + (double)n * 4 + (double)(nslices + 1) * 8 +
+           (double)nslices * 64 * 12 + (double)nexc * 4 + 2.0 * n * 8;
+  }
+  template <int G>
+  __device__ __forceinline__ double lane_sum(int32_t r, int lane, const double *__restrict__ x) const {
+    static_assert(G == P, "sliced rows are read with P lanes per row");
+    return sell16_lane_sum<P, VT, NT>(sptr, slen, sd16, sfirst, septr, sexc, sval, r, lane, x);
+  }
+};
+
 // Row groups of kBlock/G rows are dealt to the blocks (the grid may be capped, so a block
 // takes several groups) round-robin, or with xcd set in contiguous ranges over XCD-mapped
 // logical blocks; each thread's epilogue contributions are summed in the order of its
@@ -416,6 +443,16 @@ int apply_sell_p(const Operator &op, const VT *sval, const double *x, const Epi 
     part = R->partial.p;
   }
   const int xcd = (op.load & kLoadXcd) ? 1 : 0;
+  if ((op.load & kLoadPair) && A.sell_nexc[lp] >= 0) {   // 16-bit columns (built by apply_op)
+    if (op.load & kLoadDeep) {
+      const Sell16Rows<VT, P, true> rows{A.sell_ptr[lp].p, A.sell_len.p,   A.sell_d16[lp].p, A.sell_first[lp].p,
+                                         A.sell_eptr[lp].p, A.sell_exc[lp].p, sval, A.nnz, A.sell_nexc[lp], nsl};
+      return apply_rows_g<P>(op.n, op.grid, rows, x, epi, part, s, xcd, fin, true);
+    }
+    const Sell16Rows<VT, P> rows{A.sell_ptr[lp].p, A.sell_len.p,   A.sell_d16[lp].p, A.sell_first[lp].p,
+                                 A.sell_eptr[lp].p, A.sell_exc[lp].p, sval, A.nnz, A.sell_nexc[lp], nsl};
+    return apply_rows_g<P>(op.n, op.grid, rows, x, epi, part, s, xcd, fin, true);
+  }
   if (op.load & kLoadDeep) {
     const SellRows<VT, P, true> rows{A.sell_ptr[lp].p, A.sell_len.p, A.sell_col[lp].p, sval, A.nnz, nsl};
     return apply_rows_g<P>(op.n, op.grid, rows, x, epi, part, s, xcd, fin, true);
@@ -451,7 +488,10 @@ int apply_op(const Operator &op, bool fp32, const double *x, const Epi &epi, Red
              bool fin = true) {
   if (op.obc) return apply_rows(op.n, op.lanes, op.grid, op.obc->rows(), x, epi, R, s, 0, fin);
   const int fmt = load_format(op.load);
-  if (fmt == kFmtSell && ensure_sell(*op.A, op.lanes, fp32)) return apply_sell(op, fp32, x, epi, R, s, fin);
+  if (fmt == kFmtSell && ensure_sell(*op.A, op.lanes, fp32)) {
+    if (op.load & kLoadPair) ensure_sell16(*op.A, op.lanes);
+    return apply_sell(op, fp32, x, epi, R, s, fin);
+  }
   const bool delta = fmt == kFmtDelta && ensure_delta(*op.A);
   if (fp32) {
     if (!op.A->val32.p) throw Failure(DMRGX_ERR_ARG, "FP32 values not prepared");
@@ -1432,6 +1472,74 @@ __global__ __launch_bounds__(kBlock) void k_sell_fill(int32_t n, int32_t nsl, co
       sval[at] = VT(0);
     }
   }
+}
+
+// 16-bit columns of the sliced form: one thread per sequence (slot column q = slice * 64 + j
+// of a slice), walking its slots in order.  Count pass: the sequence's first column and how
+// many of its differences do not fit in 0..65534; fill pass (eptr set): the differences and
+// the exceptions.
+__global__ __launch_bounds__(kBlock) void k_sell16_walk(int32_t nsl, const int64_t *__restrict__ sptr,
+                                                        const int32_t *__restrict__ scol, int32_t *__restrict__ first,
+                                                        int64_t *__restrict__ count, const int64_t *__restrict__ eptr,
+                                                        uint16_t *__restrict__ d16, int32_t *__restrict__ exc) {
+  const int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (q >= (int64_t)nsl * 64) return;
+  const int64_t sl = q / 64, j = q % 64;
+  const int64_t b0 = sptr[sl];
+  const int64_t w = (sptr[sl + 1] - b0) / 64;
+  int32_t prev = w > 0 ? scol[b0 + j] : 0;
+  int64_t e = eptr ? eptr[q] : 0, cnt = 0;
+  if (!eptr) first[q] = prev;
+  else if (w > 0) d16[b0 + j] = 0;
+  for (int64_t k = 1; k < w; k++) {
+    const int64_t at = b0 + k * 64 + j;
+    const int32_t c = scol[at];
+    const int64_t d = (int64_t)c - prev;
+    const bool wide = d < 0 || d > 65534;
+    if (!eptr) {
+      cnt += wide ? 1 : 0;
+    } else if (wide) {
+      d16[at] = 0xFFFF;
+      exc[e++] = c;
+    } else {
+      d16[at] = (uint16_t)d;
+    }
+    prev = c;
+  }
+  if (!eptr) count[q] = cnt;
+}
+
+int64_t build_sell16(dmrgx_matrix &A, int P, hipStream_t s) {
+  const int lp = P == 1 ? 0 : (P == 2 ? 1 : 2);
+  const int C = 64 / P;
+  const int32_t nsl = (A.n + C - 1) / C;
+  const int64_t nseq = (int64_t)nsl * 64;
+  if (!A.sell_ptr[lp].p) throw Failure(DMRGX_ERR_ARG, "16-bit sliced columns need the sliced form first");
+  if (nseq > INT32_MAX) throw Failure(DMRGX_ERR_ARG, "too many slices for the 16-bit sliced columns");
+  A.sell_first[lp].resize((size_t)std::max<int64_t>(nseq, 1));
+  A.sell_eptr[lp].resize((size_t)nseq + 1);
+  A.sell_d16[lp].resize(A.sell_col[lp].count);
+  DevBuf<int64_t> count((size_t)std::max<int64_t>(nseq, 1));
+  const unsigned g = (unsigned)((nseq + kBlock - 1) / kBlock);
+  if (nseq > 0) {
+    k_sell16_walk<<<g, kBlock, 0, s>>>(nsl, A.sell_ptr[lp].p, A.sell_col[lp].p, A.sell_first[lp].p, count.p,
+                                       nullptr, nullptr, nullptr);
+    HIPCHK(hipGetLastError());
+  }
+  launch_exclusive_scan((int32_t)nseq, count.p, A.sell_eptr[lp].p, s);
+  int64_t total = 0;
+  HIPCHK(hipMemcpyAsync(&total, A.sell_eptr[lp].p + nseq, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  // one entry past the last exception: the lane sums load an exception one ahead
+  A.sell_exc[lp].resize((size_t)total + 1);
+  HIPCHK(hipMemsetAsync(A.sell_exc[lp].p + total, 0, sizeof(int32_t), s));
+  if (nseq > 0) {
+    k_sell16_walk<<<g, kBlock, 0, s>>>(nsl, A.sell_ptr[lp].p, A.sell_col[lp].p, nullptr, nullptr,
+                                       A.sell_eptr[lp].p, A.sell_d16[lp].p, A.sell_exc[lp].p);
+    HIPCHK(hipGetLastError());
+  }
+  A.sell_nexc[lp] = total;
+  return total;
 }
 
 void build_sell(dmrgx_matrix &A, int P, bool fp32, hipStream_t s) {

@@ -291,6 +291,12 @@ bool ensure_sell(const dmrgx_matrix &A, int P, bool fp32) {
   return true;
 }
 
+int64_t ensure_sell16(const dmrgx_matrix &A, int P) {
+  const int lp = P == 1 ? 0 : (P == 2 ? 1 : 2);
+  if (A.sell_nexc[lp] < 0) build_sell16(const_cast<dmrgx_matrix &>(A), P, ctx().stream);
+  return A.sell_nexc[lp];
+}
+
 void drop_sell_except(const dmrgx_matrix &A, int P) {
   dmrgx_matrix &M = const_cast<dmrgx_matrix &>(A);
   for (int lp = 0; lp < 3; lp++)
@@ -299,6 +305,11 @@ void drop_sell_except(const dmrgx_matrix &A, int P) {
       M.sell_col[lp].reset();
       M.sell_val[lp].reset();
       M.sell_val32[lp].reset();
+      M.sell_d16[lp].reset();
+      M.sell_first[lp].reset();
+      M.sell_eptr[lp].reset();
+      M.sell_exc[lp].reset();
+      M.sell_nexc[lp] = -1;
     }
   if (P != 1 && P != 2 && P != 4) M.sell_len.reset();
 }

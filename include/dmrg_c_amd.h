@@ -505,22 +505,26 @@ int dmrgx_spmv_device(const dmrgx_matrix *A, const double *x_dev, double *y_dev)
 /* y = A x with host pointers. */
 int dmrgx_spmv_host(const dmrgx_matrix *A, const double *x, double *y);
 /* SpMV schedule of this matrix: lanes (threads) per row, a power of two 1..64; the cap on
- * workgroups (0 = one workgroup per row group); and the load variant 0..23 (bit 0, CSR
- * format only: two entries per lane per load, a 16-byte value pair and an 8-byte column
- * pair; bit 1: eight entries in flight per lane instead of four; bit 2: each workgroup
- * takes a contiguous range of row groups and each XCD a contiguous share of the rows,
- * instead of round-robin groups; bits 3-4, the storage format
- * the products read: 0 CSR, 1 value dictionary -- 8-byte entries (column, index into a
- * table of the matrix's distinct values), FP64 products only, when the matrix has few
- * distinct values -- 2 delta columns -- 2-byte differences of consecutive columns in a row
- * plus the row's first column, when every difference fits; a format the matrix cannot have
- * reads CSR; formats are built on the device at their first use).  Bits 1, 3 and 4 never
- * change results; lanes and bit 0 change the summation order within a row, and cap and
- * bit 2 the order of the dot products fused into the solvers' products.  Defaults: lanes
- * from the mean row length, cap 4096, variant 2;
- * dmrgx_matrix_autotune times every candidate on the device (product + fused dot), then
- * times the eight fastest as the solver runs them -- a Lanczos step and a CG iteration
- * with their fused vector kernels -- and keeps the fastest; *best_us_out is that
+ * workgroups (0 = one workgroup per row group); and the load variant 0..31.  Bits 3-4 are
+ * the storage format the products read: 0 CSR, 1 value dictionary -- 8-byte entries
+ * (column, index into a table of the matrix's distinct values), FP64 products only, when
+ * the matrix has few distinct values -- 2 delta columns -- 2-byte differences of
+ * consecutive columns in a row plus the row's first column, when every difference fits --
+ * 3 sliced -- slices of 64/lanes rows read one slot column per lane, entry k*lanes + l of
+ * each row in slot k of lane l, padded to the slice's longest row; lanes 1, 2 or 4 only.
+ * A format the matrix cannot have reads CSR; formats are built on the device at their
+ * first use.  Bit 0: for CSR two entries per lane per load (a 16-byte value pair and an
+ * 8-byte column pair); for the sliced form 16-bit column differences per slot column
+ * (0xFFFF marking a column kept whole in an exception list).  Bit 1: eight entries in
+ * flight per lane instead of four (CSR, dictionary, delta); non-temporal matrix loads
+ * (sliced).  Bit 2: each workgroup takes a contiguous range of row groups and each XCD a
+ * contiguous share of the rows, instead of round-robin groups.  Bits 1, 3 and 4 and the
+ * sliced form's bit 0 never change results; lanes, the sliced form, and CSR's bit 0 change
+ * the summation order within a row, and cap and bit 2 the order of the dot products fused
+ * into the solvers' products.  Defaults: lanes from the mean row length, cap 4096,
+ * variant 2; dmrgx_matrix_autotune times every candidate on the device (product + fused
+ * dot), then times the eight fastest as the solver runs them -- a Lanczos step and a CG
+ * iteration with their fused vector kernels -- and keeps the fastest; *best_us_out is that
  * step-pair time.  Single and batched solves use the matrix's schedule. */
 int dmrgx_matrix_set_schedule(dmrgx_matrix *A, int32_t lanes, int32_t grid_cap, int32_t load_variant);
 int dmrgx_matrix_get_schedule(const dmrgx_matrix *A, int32_t *lanes, int32_t *grid_cap, int32_t *load_variant);
@@ -533,6 +537,9 @@ int dmrgx_matrix_time_schedule(dmrgx_matrix *A, int32_t lanes, int32_t grid_cap,
 /* Distinct values in A's value dictionary (built now if it has not been tried); 0 if A
  * keeps the plain form. */
 int dmrgx_matrix_dict_values(dmrgx_matrix *A, int64_t *n_values);
+/* Columns of A's sliced form with lanes (1, 2 or 4) lanes per row that the 16-bit column
+ * form keeps whole (built now if it has not been); for the products' byte count. */
+int dmrgx_matrix_sell16_exceptions(dmrgx_matrix *A, int32_t lanes, int64_t *exceptions);
 
 /* ---- ground-state solve (Lanczos -> inverse iteration with CG), all on the device */
 typedef struct {

@@ -20,6 +20,57 @@ def _ragged(n=5000, seed=0, distinct=None):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("n", [70000, 300000])
+def test_gpu_sliced_16bit_columns_with_wide_gaps(n):
+    """The sliced form's 16-bit columns on rows whose column gaps exceed 65534 (kept whole in
+    the exception list), next to runs of small gaps and rows of every length: products equal
+    to the 32-bit sliced form's bit for bit and to scipy's within 1e-12, for 1, 2 and 4 lanes
+    per row, plain and non-temporal loads; exceptions counted on the host must match."""
+    import scipy.sparse as sp
+    import dmrg_c_amd as D
+    rng = np.random.default_rng(n)
+    lens = rng.choice([0, 1, 2, 9, 40, 130], size=n, p=[0.05, 0.1, 0.15, 0.4, 0.25, 0.05])
+    cols = []
+    for r, k in enumerate(lens):
+        near = rng.integers(max(0, r - 3000), min(n, r + 3000), size=k // 2)   # small gaps
+        far = rng.integers(0, n, size=k - k // 2)                              # gaps up to n
+        cols.append(np.unique(np.concatenate([near, far])))
+    row = np.zeros(n + 1, np.int64)
+    row[1:] = np.cumsum([len(c) for c in cols])
+    col = np.concatenate(cols).astype(np.int32)
+    val = rng.standard_normal(row[-1])
+    A = D.DeviceMatrix(row, col, val, sym=False)
+    try:
+        x = rng.standard_normal(n)
+        ref = sp.csr_matrix((val, col, row), shape=(n, n)) @ x
+        for g in (1, 2, 4):
+            # exceptions expected: per slot column of each 64/g-row slice, the differences of
+            # consecutive slots (padding repeats the row's last column) outside 0..65534
+            C_ = 64 // g
+            rl = np.diff(row)
+            width = np.repeat([((rl[i:i + C_] + g - 1) // g).max() for i in range(0, n, C_)], C_)[:n]
+            exp = 0
+            for r in range(n):
+                c = cols[r].astype(np.int64)
+                for l in range(min(g, len(c))):
+                    sub = c[l::g]
+                    exp += int(np.count_nonzero(np.diff(sub) > 65534))
+                    if len(sub) < width[r] and c[-1] - sub[-1] > 65534:   # padding: the row's last column
+                        exp += 1
+            print(f"sliced 16-bit n={n} lanes={g} exceptions expected {exp}", flush=True)
+            assert A.sell16_exceptions(g) == exp
+            A.set_schedule(g, 0, 24)
+            y32 = A.matvec(x)
+            assert np.abs(y32 - ref).max() <= 1e-12 * (np.abs(ref).max() + 1)
+            for v in (25, 27, 29, 31):
+                A.set_schedule(g, 2048 if v & 4 else 0, v)
+                assert np.array_equal(A.matvec(x), y32), (g, v)
+        assert exp > 0
+    finally:
+        A.free()
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("distinct", [None, 40], ids=["random_values", "few_values"])
 def test_gpu_ragged_rows_every_schedule(distinct):
     import scipy.sparse as sp

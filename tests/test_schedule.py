@@ -176,10 +176,11 @@ def test_gpu_formats_ed_and_engine(fmt):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("lanes", [1, 2, 4])
-def test_gpu_sliced_ed_and_engine(lanes):
-    """ED matrices and the m=400 superblock in the sliced form: energies within 1e-10 of the
-    CSR form's (FP32-refined within 1e-9), single and batched solves the same bits."""
+@pytest.mark.parametrize("lanes,cols16", [(1, False), (2, False), (4, False), (1, True), (2, True), (4, True)])
+def test_gpu_sliced_ed_and_engine(lanes, cols16):
+    """ED matrices and the m=400 superblock in the sliced form (32- or 16-bit columns):
+    energies within 1e-10 of the CSR form's (FP32-refined within 1e-9), single and batched
+    solves the same bits; with 16-bit columns also the same bits as with 32-bit ones."""
     from dmrg_c_amd import ed, engine
     mats = []
     for m in (ed.Model("xxz", L=16, sz=0, J_z=0.7, h_z=0.2), ed.Model("hubbard", L=8, N=8, sz=0, U=3.0)):
@@ -189,13 +190,18 @@ def test_gpu_sliced_ed_and_engine(lanes):
     A, e_ref = _matrix()
     mats.append(A)
     try:
-        plain = []
+        plain, wide = [], []
         for M in mats:
             M.set_schedule(8, 4096, 2)
             plain.append(M.ground_state())
             M.set_schedule(lanes, 4096, 24 | (4 if lanes == 2 else 0))
-        print(f"sliced lanes={lanes}", flush=True)
+            if cols16:
+                wide.append(M.ground_state())
+                M.set_schedule(lanes, 4096, 25 | (4 if lanes == 2 else 0) | (2 if lanes == 4 else 0))
+        print(f"sliced lanes={lanes} 16-bit columns={cols16}", flush=True)
         single = [M.ground_state() for M in mats]
+        for w, d in zip(wide, single):
+            assert d[0] == w[0] and np.array_equal(d[1], w[1])
         batch = engine.ground_state_batch(mats)
         for p, d, b in zip(plain, single, batch):
             assert abs(d[0] - p[0]) <= 1e-10 * abs(p[0])
