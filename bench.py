@@ -56,6 +56,9 @@ SPMV_FORMATS = [("CSR", "k_apply<CsrRows<double>> (CSR SpMV + fused Lanczos/CG e
                 ("delta columns", "k_apply<DeltaRows<double>> (delta-column CRS SpMV + fused Lanczos/CG epilogue)"),
                 ("sliced", "k_apply<P, SellRows<double, P>> (sliced SpMV: 64/P-row slices, P lanes per row, "
                            "+ fused Lanczos/CG epilogue)")]
+SPMV_FORMATS.append(("sliced, value indices", "k_apply<P, SellxRows<double, P>> (sliced SpMV with 16-bit column "
+                     "differences and 16-bit value indices: 64/P-row slices, P lanes per row, + fused Lanczos/CG "
+                     "epilogue)"))
 SELL16_KERNEL = ("k_apply<P, Sell16Rows<double, P>> (sliced SpMV with 16-bit column differences: 64/P-row "
                  "slices, P lanes per row, + fused Lanczos/CG epilogue)")
 
@@ -247,8 +250,10 @@ def spmv_sweep(D, sb, snap):
     n = snap.n
     alg = A.nnz * 12 + (n + 1) * 8 + 2 * n * 8
     rows = []
-    for v in range(32):
-        for g in ((1, 2, 4) if (v >> 3) == 3 else (4, 8, 16, 32, 64)):
+    for v in range(40):
+        if (v >> 3) == 4 and v & 1:
+            continue
+        for g in ((1, 2, 4) if (v >> 3) >= 3 else (4, 8, 16, 32, 64)):
             for cap in (1024, 2048, 4096, 8192, 16384, 0):
                 us = A.time_schedule(g, cap, v, 20)
                 rows.append({"lanes": g, "cap": cap, "variant": v, "us": round(us, 2),
@@ -267,6 +272,18 @@ def spmv_sweep(D, sb, snap):
 def spmv_alg_bytes(D, A, sched, n):
     """Algorithmic bytes of one product in the format the schedule reads: CSR (8-byte
     values, 4-byte columns) or the value dictionary (8-byte packed entries + the table)."""
+    if (sched[2] >> 3) == 4:
+        nsl = (n + 64 // sched[0] - 1) // (64 // sched[0])
+        exc = A.sell16_exceptions(sched[0])
+        L = D.lib()
+        L.dmrgx_matrix_dict_values.argtypes = [C.c_void_p, C.POINTER(C.c_int64)]
+        nv = C.c_int64()
+        D.check(L.dmrgx_matrix_dict_values(A.handle, C.byref(nv)))
+        if 0 < nv.value <= 65536:
+            return (A.nnz * 4 + n * 4 + (nsl + 1) * 8 + nsl * 64 * 12 + exc * 4 + nv.value * 8 + 2 * n * 8,
+                    f"sliced, 16-bit columns and value indices ({sched[0]} lanes per row; {nv.value} distinct "
+                    f"values, {exc} columns kept whole, padding not counted)")
+        sched = (sched[0], sched[1], (sched[2] & 7) | 24 | 1)   # read as format 3 with 16-bit columns
     if (sched[2] >> 3) == 3:
         nsl = (n + 64 // sched[0] - 1) // (64 // sched[0])
         if sched[2] & 1:
@@ -883,9 +900,9 @@ def run_superblock(D, L, name, steps, warmup, fixed, cpu_budget):
             "cg_steps": int(np.median(cg)),
             "spmv_schedule": {"lanes_per_row": lanes[0], "workgroup_cap": lanes[1],
                               "paired_entry_loads": bool(lanes[2] & 1) and fmt == 0,
-                              "entries_in_flight_per_lane": 8 if (lanes[2] & 2 or fmt == 3) else 4,
-                              "non_temporal_matrix_loads": bool(lanes[2] & 2) and fmt == 3,
-                              "column_bits": 16 if (fmt == 3 and lanes[2] & 1) else 32,
+                              "entries_in_flight_per_lane": 8 if (lanes[2] & 2 or fmt >= 3) else 4,
+                              "non_temporal_matrix_loads": bool(lanes[2] & 2) and fmt >= 3,
+                              "column_bits": 16 if ((fmt == 3 and lanes[2] & 1) or fmt == 4) else 32,
                               "xcd_contiguous_rows": bool(lanes[2] & 4),
                               "format": SPMV_FORMATS[fmt][0]},
         },

@@ -222,9 +222,10 @@ class DeviceMatrix:
     def schedule(self) -> Tuple[int, int, int]:
         """(lanes per row, workgroup cap, load variant) of this matrix's SpMV; load variant
         bit 0 = two entries per lane per load (CSR), bit 1 = eight entries in flight per lane,
-        bit 2 = contiguous row ranges per workgroup and per XCD, bits 3-4 = storage format
-        (0 CSR, 1 value dictionary, 2 delta columns, 3 sliced; for the sliced form bit 0 =
-        16-bit column differences, bit 1 = non-temporal matrix loads)."""
+        bit 2 = contiguous row ranges per workgroup and per XCD, bits 3-5 = storage format
+        (0 CSR, 1 value dictionary, 2 delta columns, 3 sliced, 4 sliced with 16-bit columns
+        and 16-bit value indices; for the sliced forms bit 1 = non-temporal matrix loads,
+        and for format 3 bit 0 = 16-bit column differences)."""
         g, cap, nt = C.c_int32(), C.c_int32(), C.c_int32()
         check(lib().dmrgx_matrix_get_schedule(self.handle, C.byref(g), C.byref(cap), C.byref(nt)))
         return g.value, cap.value, nt.value

@@ -418,7 +418,7 @@ int dmrgx_matrix_set_schedule(dmrgx_matrix *A, int32_t lanes, int32_t grid_cap, 
   if (!A || lanes < 1 || lanes > 64 || (lanes & (lanes - 1)) || grid_cap < 0 || load_variant < 0 ||
       load_variant >= kLoadVariants)
     return DMRGX_ERR_ARG;
-  if (load_format(load_variant) == kFmtSell && lanes > kSellMaxLanes) return DMRGX_ERR_ARG;
+  if (sliced_format(load_format(load_variant)) && lanes > kSellMaxLanes) return DMRGX_ERR_ARG;
   std::lock_guard<std::recursive_mutex> lk(api_mutex());
   A->lanes = lanes;
   A->grid = grid_cap;
@@ -576,10 +576,13 @@ int dmrgx_matrix_autotune(dmrgx_matrix *A, int32_t *lanes_out, int32_t *grid_out
     };
     std::vector<Cand> cands;
     const bool has[3] = {true, ensure_dict(*A), ensure_delta(*A)};
+    const bool has_idx = has[1] && A->n_values <= kSellIdxMaxValues;
     for (int v = 0; v < kLoadVariants; v++) {
-      if (load_format(v) == kFmtSell) {
+      if (sliced_format(load_format(v))) {
         // sliced form: 1, 2 or 4 lanes per row, 32- or 16-bit columns, plain or
-        // non-temporal matrix loads, plain or XCD-contiguous row ranges
+        // non-temporal matrix loads, plain or XCD-contiguous row ranges; with value
+        // indices (bit 0 ignored) where the matrix has a small value table
+        if (load_format(v) == kFmtSellIdx && ((v & kLoadPair) || !has_idx)) continue;
         for (int g = 1; g <= kSellMaxLanes; g *= 2)
           for (int cap : caps) cands.push_back({g, cap, v, env.time(A, g, cap, v, 3)});
         continue;
@@ -599,7 +602,7 @@ int dmrgx_matrix_autotune(dmrgx_matrix *A, int32_t *lanes_out, int32_t *grid_out
     A->grid = best.cap;
     A->load = best.v;
     // the sliced forms built for the candidates, except the one the schedule reads
-    drop_sell_except(*A, load_format(best.v) == kFmtSell ? best.g : 0);
+    drop_sell_except(*A, sliced_format(load_format(best.v)) ? best.g : 0);
     if (lanes_out) *lanes_out = best.g;
     if (grid_out) *grid_out = best.cap;
     if (load_out) *load_out = best.v;

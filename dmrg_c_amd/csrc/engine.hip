@@ -60,9 +60,12 @@ struct SegDesc {
   const int32_t *slen, *scol;
   const double *sval;
   const float *sval32;
-  const uint16_t *sd16;   // sliced form's 16-bit columns (load bit 0)
+  const uint16_t *sd16;   // sliced form's 16-bit columns (load bit 0, format 4)
   const int32_t *sfirst, *sexc;
   const int64_t *septr;
+  const uint16_t *sidx;   // format 4: value indices into stab / stab32
+  const double *stab;
+  const float *stab32;
   int32_t n, fp32;
   int32_t blk_lo, nblk;   // this slot's blocks in the launch
   int32_t k, load;        // step (Lanczos) or iteration (CG); SpMV load variant
@@ -142,10 +145,21 @@ __global__ __launch_bounds__(kBlock) void k_seg_spmv(const SegDesc *desc, const 
   for (int32_t gi = gr.start; gi < gr.end; gi += gr.step) {
     const int32_t r = gi * RPB + g;
     double s = 0.0;
-    if (r < d.n && fmt == kFmtSell) {
+    if (r < d.n && sliced_format(fmt)) {
       if constexpr (G <= kSellMaxLanes) {
         const bool nt = (d.load & kLoadDeep) != 0;
-        if (d.sd16) {
+        if (d.sidx) {
+          if (d.fp32)
+            s = nt ? sellx_lane_sum<G, float, true>(d.sptr, d.slen, d.sd16, d.sfirst, d.septr, d.sexc, d.sidx,
+                                                    d.stab32, r, lane, d.x)
+                   : sellx_lane_sum<G, float>(d.sptr, d.slen, d.sd16, d.sfirst, d.septr, d.sexc, d.sidx, d.stab32, r,
+                                              lane, d.x);
+          else
+            s = nt ? sellx_lane_sum<G, double, true>(d.sptr, d.slen, d.sd16, d.sfirst, d.septr, d.sexc, d.sidx,
+                                                     d.stab, r, lane, d.x)
+                   : sellx_lane_sum<G, double>(d.sptr, d.slen, d.sd16, d.sfirst, d.septr, d.sexc, d.sidx, d.stab, r,
+                                               lane, d.x);
+        } else if (d.sd16) {
           if (d.fp32)
             s = nt ? sell16_lane_sum<G, float, true>(d.sptr, d.slen, d.sd16, d.sfirst, d.septr, d.sexc, d.sval32, r,
                                                      lane, d.x)
@@ -462,11 +476,17 @@ void set_sell(SegDesc &a, const dmrgx_matrix &A) {
   a.scol = A.sell_col[lp].p;
   a.sval = A.sell_val[lp].p;
   a.sval32 = A.sell_val32[lp].p;
-  if (load_format(A.load) == kFmtSell && (A.load & kLoadPair) && A.sell_nexc[lp] >= 0) {   // 16-bit columns
+  const int fmt = load_format(A.load);
+  if ((fmt == kFmtSellIdx || (fmt == kFmtSell && (A.load & kLoadPair))) && A.sell_nexc[lp] >= 0) {   // 16-bit columns
     a.sd16 = A.sell_d16[lp].p;
     a.sfirst = A.sell_first[lp].p;
     a.septr = A.sell_eptr[lp].p;
     a.sexc = A.sell_exc[lp].p;
+  }
+  if (fmt == kFmtSellIdx && A.sell_idx_ok[lp] == 1) {   // value indices (made at slot start)
+    a.sidx = A.sell_idx16[lp].p;
+    a.stab = A.table.p;
+    a.stab32 = A.table32.p;
   }
 }
 
@@ -488,9 +508,15 @@ void start_slot(Slot &s) {
   if (s.mixed) ensure_fp32(*s.A);
   if (load_format(s.A->load) == kFmtDict) ensure_dict(*s.A);
   if (load_format(s.A->load) == kFmtDelta) ensure_delta(*s.A);
-  if (load_format(s.A->load) == kFmtSell) {
+  const int fmt = load_format(s.A->load);
+  if (fmt == kFmtSell) {
     ensure_sell(*s.A, s.A->lanes, s.mixed);
     if (s.A->load & kLoadPair) ensure_sell16(*s.A, s.A->lanes);
+  } else if (fmt == kFmtSellIdx) {
+    // format 4, or format 3 with 16-bit columns where the matrix has no small value table
+    ensure_sell(*s.A, s.A->lanes, false);
+    ensure_sell16(*s.A, s.A->lanes);
+    if (!ensure_sellx(*s.A, s.A->lanes, s.mixed)) ensure_sell(*s.A, s.A->lanes, s.mixed);
   }
   launch_start_vector(n, s.p.seed, s.vec.p, st);
   launch_copy(n, s.vec.p, s.w.p, st);

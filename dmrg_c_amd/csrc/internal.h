@@ -153,6 +153,13 @@ struct dmrgx_matrix {
   dmrgx::DevBuf<int64_t> sell_eptr[3];
   dmrgx::DevBuf<int32_t> sell_exc[3];
   int64_t sell_nexc[3] = {-1, -1, -1};
+  // Value indices of the sliced form (format 4, per P): sell_idx16 = the slot's index into
+  // the value dictionary's table (0 at padding slots); table32 = the table in FP32 for the
+  // mixed form.  sell_idx_ok: -1 not tried, 0 not available (no table of at most 65536
+  // values), 1 built.
+  dmrgx::DevBuf<uint16_t> sell_idx16[3];
+  dmrgx::DevBuf<float> table32;
+  int sell_idx_ok[3] = {-1, -1, -1};
   int lanes = 8;                // lanes per row of the SpMV kernel
   int grid = dmrgx::kGridCap;   // cap on SpMV workgroups (0: one per row group)
   int load = 0;                 // load variant (kLoad* bits)
@@ -293,22 +300,30 @@ __device__ __forceinline__ double csr_lane_sum(const int64_t *__restrict__ row, 
 constexpr int kLoadPair = 1;   // CSR: two entries per lane per load (changes the order)
 constexpr int kLoadDeep = 2;
 constexpr int kLoadXcd = 4;
-// Bits 3-4: storage format of the product.  CSR: 4-byte columns; dictionary (FP64 products
+// Bits 3-5: storage format of the product.  CSR: 4-byte columns; dictionary (FP64 products
 // only): 8-byte (column, value index) entries over a table of the distinct values; delta:
 // 2-byte column differences within each row plus each row's first column.  Every format
 // gives the CSR products bit for bit (same values, same order).
 // Format 3, sliced: see sell_lane_sum; its lanes per row (1, 2 or 4) set the slice height,
 // and its sums are in an order of their own.  For it bit 0 selects the 16-bit column form
 // (sell16_lane_sum) and bit 1 non-temporal matrix loads; neither changes a result.
+// Format 4, sliced with value indices: the sliced form's 16-bit columns and a 16-bit index
+// per slot into the value dictionary's table (sellx_lane_sum; 4 bytes per entry), for a
+// matrix with at most 65536 distinct values (ED Hamiltonians have a few dozen); bit 0 is
+// ignored, bit 1 selects non-temporal loads.  Same values and order as format 3, so the
+// same bits; a matrix without such a table reads format 3 with 16-bit columns.
 constexpr int kFmtShift = 3;
-constexpr int kFmtCsr = 0, kFmtDict = 1, kFmtDelta = 2, kFmtSell = 3;
+constexpr int kFmtCsr = 0, kFmtDict = 1, kFmtDelta = 2, kFmtSell = 3, kFmtSellIdx = 4;
 constexpr int kLoadDict = kFmtDict << kFmtShift;
 constexpr int kLoadDelta = kFmtDelta << kFmtShift;
 constexpr int kLoadSell = kFmtSell << kFmtShift;
-constexpr int kLoadFmtMask = 3 << kFmtShift;
-constexpr int kLoadVariants = 4 << kFmtShift;   // 0 .. 31
+constexpr int kLoadSellIdx = kFmtSellIdx << kFmtShift;
+constexpr int kLoadFmtMask = 7 << kFmtShift;
+constexpr int kLoadVariants = 5 << kFmtShift;   // 0 .. 39
 constexpr int kSellMaxLanes = 4;
+constexpr int64_t kSellIdxMaxValues = 65536;
 __host__ __device__ inline int load_format(int load) { return (load & kLoadFmtMask) >> kFmtShift; }
+__host__ __device__ inline bool sliced_format(int fmt) { return fmt == kFmtSell || fmt == kFmtSellIdx; }
 
 // Row groups [start, end) with stride step that logical block lb of nblk takes out of
 // ngroups: round-robin (lb, lb + nblk, ...) or one contiguous range.
@@ -589,6 +604,68 @@ __device__ __forceinline__ double sell16_lane_sum(const int64_t *__restrict__ sp
   return ((acc[0] + acc[4]) + (acc[2] + acc[6])) + ((acc[1] + acc[5]) + (acc[3] + acc[7]));
 }
 
+// sell16_lane_sum with each slot's value read through its 16-bit index into the value
+// table (format 4): 2 + 2 bytes per slot; table[idx] equals the slot's value bit for bit,
+// so the sums are sell16_lane_sum's.  The table (a few dozen to 65536 values) stays in the
+// caches.
+template <int P, class VT, bool NT = false>
+__device__ __forceinline__ double sellx_lane_sum(const int64_t *__restrict__ sptr, const int32_t *__restrict__ slen,
+                                                 const uint16_t *__restrict__ sd16,
+                                                 const int32_t *__restrict__ sfirst,
+                                                 const int64_t *__restrict__ septr,
+                                                 const int32_t *__restrict__ sexc,
+                                                 const uint16_t *__restrict__ sidx, const VT *__restrict__ table,
+                                                 int32_t r, int lane, const double *__restrict__ x) {
+  constexpr int C = 64 / P;
+  const int32_t s = r / C;
+  const int j = (r % C) * P + lane;
+  const int64_t b0 = sptr[s];
+  const int64_t base = b0 + j;
+  const int32_t w = (int32_t)((sptr[s + 1] - b0) >> 6);
+  const int32_t len = slen[r];
+  const int64_t q = (int64_t)s * 64 + j;
+  int32_t c = sfirst[q];
+  int64_t e = septr[q];
+  int32_t en = sexc[e];
+  double acc[kSellAcc];
+#pragma unroll
+  for (int t = 0; t < kSellAcc; t++) acc[t] = 0.0;
+  for (int32_t k = 0; k < w; k += kSellAcc) {
+    uint16_t ix[kSellAcc], d[kSellAcc];
+#pragma unroll
+    for (int t = 0; t < kSellAcc; t++) {
+      const int64_t at = base + (int64_t)min(k + t, w - 1) * 64;
+      ix[t] = stream_load<NT>(sidx + at);
+      d[t] = stream_load<NT>(sd16 + at);
+    }
+    int32_t cc[kSellAcc];
+#pragma unroll
+    for (int t = 0; t < kSellAcc; t++) {
+      const int32_t kk = k + t;
+      if (kk > 0 && kk < w) {
+        if (d[t] == 0xFFFF) {
+          c = en;
+          en = sexc[++e];
+        } else {
+          c += d[t];
+        }
+      }
+      cc[t] = c;
+    }
+    VT v[kSellAcc];
+    double xv[kSellAcc];
+#pragma unroll
+    for (int t = 0; t < kSellAcc; t++) {
+      v[t] = table[ix[t]];
+      xv[t] = x[cc[t]];
+    }
+#pragma unroll
+    for (int t = 0; t < kSellAcc; t++)
+      if ((k + t) * P + lane < len) acc[t] += (double)v[t] * xv[t];
+  }
+  return ((acc[0] + acc[4]) + (acc[2] + acc[6])) + ((acc[1] + acc[5]) + (acc[3] + acc[7]));
+}
+
 // Binary search of an ascending int64 array: index of target or -1
 // (sml/BINARY_SEARCH_LINT1.c, iterative).
 __device__ __forceinline__ int64_t find_code(const int64_t *__restrict__ a, int64_t n, int64_t target) {
@@ -838,6 +915,10 @@ void build_sell(dmrgx_matrix &A, int P, bool fp32, hipStream_t s);
 // the sliced form's 16-bit columns for P lanes per row (after ensure_sell); exceptions
 int64_t ensure_sell16(const dmrgx_matrix &A, int P);
 int64_t build_sell16(dmrgx_matrix &A, int P, hipStream_t s);
+// format 4's value indices for P lanes per row (after ensure_sell16); false if A has no
+// value table of at most kSellIdxMaxValues values.  fp32: also the FP32 table.
+bool ensure_sellx(const dmrgx_matrix &A, int P, bool fp32);
+void build_sellx(dmrgx_matrix &A, int P, bool fp32, hipStream_t s);
 // frees the sliced forms other than P's (all with P = 0)
 void drop_sell_except(const dmrgx_matrix &A, int P);
 // A's load variant restricted to the formats it has (a format it cannot have reads CSR).

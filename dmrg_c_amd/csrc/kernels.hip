@@ -135,6 +135,35 @@ struct Sell16Rows {
   }
 };
 
+// The sliced form with 16-bit columns and 16-bit value indices (format 4, sellx_lane_sum);
+// only launched with G == P.
+template <class VT, int P, bool NT = false>
+struct SellxRows {
+  const int64_t *__restrict__ sptr;
+  const int32_t *__restrict__ slen;
+  const uint16_t *__restrict__ sd16;
+  const int32_t *__restrict__ sfirst;
+  const int64_t *__restrict__ septr;
+  const int32_t *__restrict__ sexc;
+  const uint16_t *__restrict__ sidx;
+  const VT *__restrict__ table;
+  int64_t nnz, nexc, nvalues;
+  int32_t nslices;
+  static constexpr bool kTimed = true;
+  // algorithmic bytes of one y = H x: 2-byte value index and 2-byte column difference per
+  // entry (padding not counted) + row lengths + slice offsets + per-sequence first column
+  // and exception offset + the exceptions + the value table once + x once + y once
+  double alg_bytes(int32_t n) const {
+    return (double)nnz * 4 + (double)n * 4 + (double)(nslices + 1) * 8 + (double)nslices * 64 * 12 +
+           (double)nexc * 4 + (double)nvalues * sizeof(VT) + 2.0 * n * 8;
+  }
+  template <int G>
+  __device__ __forceinline__ double lane_sum(int32_t r, int lane, const double *__restrict__ x) const {
+    static_assert(G == P, "sliced rows are read with P lanes per row");
+    return sellx_lane_sum<P, VT, NT>(sptr, slen, sd16, sfirst, septr, sexc, sidx, table, r, lane, x);
+  }
+};
+
 // Row groups of kBlock/G rows are dealt to the blocks (the grid may be capped, so a block
 // takes several groups) round-robin, or with xcd set in contiguous ranges over XCD-mapped
 // logical blocks; each thread's epilogue contributions are summed in the order of its
@@ -461,6 +490,51 @@ int apply_sell_p(const Operator &op, const VT *sval, const double *x, const Epi 
   return apply_rows_g<P>(op.n, op.grid, rows, x, epi, part, s, xcd, fin, true);
 }
 
+template <class VT, int P, class Epi>
+int apply_sellx_p(const Operator &op, const VT *table, const double *x, const Epi &epi, Reduce *R, hipStream_t s,
+                  bool fin) {
+  const dmrgx_matrix &A = *op.A;
+  constexpr int lp = P == 1 ? 0 : (P == 2 ? 1 : 2);
+  const int32_t nsl = (op.n + 64 / P - 1) / (64 / P);
+  const int blocks = spmv_blocks(op.n, P, op.grid);
+  double *part = nullptr;
+  if (Epi::kReduce && blocks > 0) {
+    R->ensure(blocks);
+    part = R->partial.p;
+  }
+  const int xcd = (op.load & kLoadXcd) ? 1 : 0;
+  if (op.load & kLoadDeep) {
+    const SellxRows<VT, P, true> rows{A.sell_ptr[lp].p,  A.sell_len.p,     A.sell_d16[lp].p,
+                                      A.sell_first[lp].p, A.sell_eptr[lp].p, A.sell_exc[lp].p,
+                                      A.sell_idx16[lp].p, table,            A.nnz,
+                                      A.sell_nexc[lp],    A.n_values,       nsl};
+    return apply_rows_g<P>(op.n, op.grid, rows, x, epi, part, s, xcd, fin, true);
+  }
+  const SellxRows<VT, P> rows{A.sell_ptr[lp].p,  A.sell_len.p,     A.sell_d16[lp].p, A.sell_first[lp].p,
+                              A.sell_eptr[lp].p, A.sell_exc[lp].p, A.sell_idx16[lp].p, table,
+                              A.nnz,             A.sell_nexc[lp],  A.n_values,       nsl};
+  return apply_rows_g<P>(op.n, op.grid, rows, x, epi, part, s, xcd, fin, true);
+}
+
+template <class Epi>
+int apply_sellx(const Operator &op, bool fp32, const double *x, const Epi &epi, Reduce *R, hipStream_t s, bool fin) {
+  const dmrgx_matrix &A = *op.A;
+  if (fp32) {
+    switch (op.lanes) {
+      case 1: return apply_sellx_p<float, 1>(op, A.table32.p, x, epi, R, s, fin);
+      case 2: return apply_sellx_p<float, 2>(op, A.table32.p, x, epi, R, s, fin);
+      case 4: return apply_sellx_p<float, 4>(op, A.table32.p, x, epi, R, s, fin);
+    }
+  } else {
+    switch (op.lanes) {
+      case 1: return apply_sellx_p<double, 1>(op, A.table.p, x, epi, R, s, fin);
+      case 2: return apply_sellx_p<double, 2>(op, A.table.p, x, epi, R, s, fin);
+      case 4: return apply_sellx_p<double, 4>(op, A.table.p, x, epi, R, s, fin);
+    }
+  }
+  throw Failure(DMRGX_ERR_ARG, "sliced form needs 1, 2 or 4 lanes per row");
+}
+
 template <class Epi>
 int apply_sell(const Operator &op, bool fp32, const double *x, const Epi &epi, Reduce *R, hipStream_t s, bool fin) {
   const dmrgx_matrix &A = *op.A;
@@ -488,6 +562,15 @@ int apply_op(const Operator &op, bool fp32, const double *x, const Epi &epi, Red
              bool fin = true) {
   if (op.obc) return apply_rows(op.n, op.lanes, op.grid, op.obc->rows(), x, epi, R, s, 0, fin);
   const int fmt = load_format(op.load);
+  if (fmt == kFmtSellIdx && ensure_sell(*op.A, op.lanes, false)) {
+    ensure_sell16(*op.A, op.lanes);
+    if (ensure_sellx(*op.A, op.lanes, fp32)) return apply_sellx(op, fp32, x, epi, R, s, fin);
+    // no value table of at most 65536 values: format 3 with 16-bit columns (the same bits)
+    Operator o3 = op;
+    o3.load = (op.load & ~kLoadFmtMask) | kLoadSell | kLoadPair;
+    ensure_sell(*op.A, op.lanes, fp32);
+    return apply_sell(o3, fp32, x, epi, R, s, fin);
+  }
   if (fmt == kFmtSell && ensure_sell(*op.A, op.lanes, fp32)) {
     if (op.load & kLoadPair) ensure_sell16(*op.A, op.lanes);
     return apply_sell(op, fp32, x, epi, R, s, fin);
@@ -1540,6 +1623,55 @@ int64_t build_sell16(dmrgx_matrix &A, int P, hipStream_t s) {
   }
   A.sell_nexc[lp] = total;
   return total;
+}
+
+// Format 4's value indices: one thread per row position of every slice (as k_sell_fill),
+// the index half of each entry's dictionary word at its slot, 0 at padding slots.
+template <int P>
+__global__ __launch_bounds__(kBlock) void k_sellx_fill(int32_t n, int32_t nsl, const int64_t *__restrict__ row,
+                                                       const uint64_t *__restrict__ ent,
+                                                       const int64_t *__restrict__ sptr, uint16_t *__restrict__ sidx) {
+  constexpr int C = 64 / P;
+  const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (t >= (int64_t)nsl * C) return;
+  const int32_t sl = (int32_t)(t / C), i = (int32_t)(t % C), r = (int32_t)t;
+  const int64_t b0 = sptr[sl], slots = (sptr[sl + 1] - b0) / 64 * P;
+  const int64_t b = r < n ? row[r] : 0, len = r < n ? row[r + 1] - b : 0;
+  for (int64_t j = 0; j < slots; j++) {
+    const int64_t at = b0 + (j / P) * 64 + (int64_t)i * P + j % P;
+    sidx[at] = j < len ? (uint16_t)(ent[b + j] >> 32) : (uint16_t)0;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_to_fp32_table(int64_t n, const double *__restrict__ a,
+                                                          float *__restrict__ b) {
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
+    b[i] = (float)a[i];
+}
+
+void build_sellx(dmrgx_matrix &A, int P, bool fp32, hipStream_t s) {
+  const int lp = P == 1 ? 0 : (P == 2 ? 1 : 2);
+  const int C = 64 / P;
+  const int32_t nsl = (A.n + C - 1) / C;
+  if (!A.sell_ptr[lp].p || A.n_values <= 0 || A.n_values > kSellIdxMaxValues || !A.ent.p)
+    throw Failure(DMRGX_ERR_ARG, "sliced value indices need the sliced form and a table of at most 65536 values");
+  if (!A.sell_idx16[lp].p) {
+    A.sell_idx16[lp].resize(A.sell_col[lp].count);
+    const unsigned gf = (unsigned)(((int64_t)nsl * C + kBlock - 1) / kBlock);
+    if (nsl > 0) {
+      switch (P) {
+        case 1: k_sellx_fill<1><<<gf, kBlock, 0, s>>>(A.n, nsl, A.row.p, A.ent.p, A.sell_ptr[lp].p, A.sell_idx16[lp].p); break;
+        case 2: k_sellx_fill<2><<<gf, kBlock, 0, s>>>(A.n, nsl, A.row.p, A.ent.p, A.sell_ptr[lp].p, A.sell_idx16[lp].p); break;
+        default: k_sellx_fill<4><<<gf, kBlock, 0, s>>>(A.n, nsl, A.row.p, A.ent.p, A.sell_ptr[lp].p, A.sell_idx16[lp].p); break;
+      }
+      HIPCHK(hipGetLastError());
+    }
+  }
+  if (fp32 && !A.table32.p) {
+    A.table32.resize((size_t)A.n_values);
+    k_to_fp32_table<<<vec_grid(A.n_values), kBlock, 0, s>>>(A.n_values, A.table.p, A.table32.p);
+    HIPCHK(hipGetLastError());
+  }
 }
 
 void build_sell(dmrgx_matrix &A, int P, bool fp32, hipStream_t s) {

@@ -297,6 +297,22 @@ int64_t ensure_sell16(const dmrgx_matrix &A, int P) {
   return A.sell_nexc[lp];
 }
 
+bool ensure_sellx(const dmrgx_matrix &A, int P, bool fp32) {
+  const int lp = P == 1 ? 0 : (P == 2 ? 1 : 2);
+  if (A.sell_idx_ok[lp] == 0) return false;
+  dmrgx_matrix &M = const_cast<dmrgx_matrix &>(A);
+  if (A.sell_idx_ok[lp] < 0) {
+    if (!ensure_dict(A) || A.n_values > kSellIdxMaxValues) {
+      M.sell_idx_ok[lp] = 0;
+      return false;
+    }
+    build_sellx(M, P, false, ctx().stream);
+    M.sell_idx_ok[lp] = 1;
+  }
+  if (fp32 && !A.table32.p) build_sellx(M, P, true, ctx().stream);
+  return true;
+}
+
 void drop_sell_except(const dmrgx_matrix &A, int P) {
   dmrgx_matrix &M = const_cast<dmrgx_matrix &>(A);
   for (int lp = 0; lp < 3; lp++)
@@ -310,6 +326,8 @@ void drop_sell_except(const dmrgx_matrix &A, int P) {
       M.sell_eptr[lp].reset();
       M.sell_exc[lp].reset();
       M.sell_nexc[lp] = -1;
+      M.sell_idx16[lp].reset();
+      M.sell_idx_ok[lp] = -1;
     }
   if (P != 1 && P != 2 && P != 4) M.sell_len.reset();
 }
@@ -317,7 +335,7 @@ void drop_sell_except(const dmrgx_matrix &A, int P) {
 int effective_load(const dmrgx_matrix &A) {
   const int fmt = load_format(A.load);
   const bool ok = fmt == kFmtCsr || (fmt == kFmtDict && A.n_values > 0) || (fmt == kFmtDelta && A.delta > 0) ||
-                  (fmt == kFmtSell && A.lanes <= kSellMaxLanes);
+                  (sliced_format(fmt) && A.lanes <= kSellMaxLanes);
   // paired-entry loads are a choice of the CSR format: its fallback for another format drops them
   return ok ? A.load : (A.load & ~(kLoadFmtMask | kLoadPair));
 }

@@ -505,21 +505,25 @@ int dmrgx_spmv_device(const dmrgx_matrix *A, const double *x_dev, double *y_dev)
 /* y = A x with host pointers. */
 int dmrgx_spmv_host(const dmrgx_matrix *A, const double *x, double *y);
 /* SpMV schedule of this matrix: lanes (threads) per row, a power of two 1..64; the cap on
- * workgroups (0 = one workgroup per row group); and the load variant 0..31.  Bits 3-4 are
+ * workgroups (0 = one workgroup per row group); and the load variant 0..39.  Bits 3-5 are
  * the storage format the products read: 0 CSR, 1 value dictionary -- 8-byte entries
  * (column, index into a table of the matrix's distinct values), FP64 products only, when
  * the matrix has few distinct values -- 2 delta columns -- 2-byte differences of
  * consecutive columns in a row plus the row's first column, when every difference fits --
  * 3 sliced -- slices of 64/lanes rows read one slot column per lane, entry k*lanes + l of
- * each row in slot k of lane l, padded to the slice's longest row; lanes 1, 2 or 4 only.
- * A format the matrix cannot have reads CSR; formats are built on the device at their
- * first use.  Bit 0: for CSR two entries per lane per load (a 16-byte value pair and an
+ * each row in slot k of lane l, padded to the slice's longest row; lanes 1, 2 or 4 only --
+ * 4 sliced with 16-bit column differences and a 16-bit index per slot into the value
+ * dictionary's table (4 bytes per entry), when the matrix has at most 65536 distinct
+ * values (else it reads format 3 with 16-bit columns, which gives the same bits); lanes 1,
+ * 2 or 4 only.  A format the matrix cannot have reads CSR; formats are built on the
+ * device at their first use.  Bit 0: for CSR two entries per lane per load (a 16-byte value pair and an
  * 8-byte column pair); for the sliced form 16-bit column differences per slot column
  * (0xFFFF marking a column kept whole in an exception list).  Bit 1: eight entries in
  * flight per lane instead of four (CSR, dictionary, delta); non-temporal matrix loads
- * (sliced).  Bit 2: each workgroup takes a contiguous range of row groups and each XCD a
- * contiguous share of the rows, instead of round-robin groups.  Bits 1, 3 and 4 and the
- * sliced form's bit 0 never change results; lanes, the sliced form, and CSR's bit 0 change
+ * (sliced; bit 0 is ignored by format 4).  Bit 2: each workgroup takes a contiguous range
+ * of row groups and each XCD a contiguous share of the rows, instead of round-robin groups.
+ * Bit 1, formats 1-2 against 0, format 4 against 3 and the sliced form's bit 0 never
+ * change results; lanes, the sliced forms against the others, and CSR's bit 0 change
  * the summation order within a row, and cap and bit 2 the order of the dot products fused
  * into the solvers' products.  Defaults: lanes from the mean row length, cap 4096,
  * variant 2; dmrgx_matrix_autotune times every candidate on the device (product + fused

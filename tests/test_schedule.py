@@ -30,7 +30,7 @@ def test_gpu_schedule_argument_checks():
     import dmrg_c_amd as D
     A, _ = _matrix()
     try:
-        for bad in ((0, 0, 0), (3, 0, 0), (128, 0, 0), (8, -1, 0), (8, 0, 24), (8, 0, 32), (8, 0, -1)):
+        for bad in ((0, 0, 0), (3, 0, 0), (128, 0, 0), (8, -1, 0), (8, 0, 24), (8, 0, 32), (2, 0, 40), (8, 0, -1)):
             with pytest.raises(D.DmrgxError):
                 A.set_schedule(*bad)
         A.set_schedule(16, 2048, 23)
@@ -100,8 +100,8 @@ def test_gpu_autotune_sets_schedule():
         sched, us = A.autotune()
         assert A.schedule == sched and us > 0
         g, cap, v = sched
-        assert g & (g - 1) == 0 and 1 <= g <= 64 and cap >= 0 and v in range(32)
-        assert (v >> 3) != 3 or g <= 4
+        assert g & (g - 1) == 0 and 1 <= g <= 64 and cap >= 0 and v in range(40)
+        assert (v >> 3) < 3 or g <= 4
         E, _, _ = A.ground_state()
         assert abs(E - e_ref) <= 1e-10 * abs(e_ref)
     finally:
@@ -176,11 +176,14 @@ def test_gpu_formats_ed_and_engine(fmt):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("lanes,cols16", [(1, False), (2, False), (4, False), (1, True), (2, True), (4, True)])
+@pytest.mark.parametrize("lanes,cols16", [(1, False), (2, False), (4, False), (1, True), (2, True), (4, True),
+                                          (1, "idx"), (2, "idx"), (4, "idx")])
 def test_gpu_sliced_ed_and_engine(lanes, cols16):
-    """ED matrices and the m=400 superblock in the sliced form (32- or 16-bit columns):
-    energies within 1e-10 of the CSR form's (FP32-refined within 1e-9), single and batched
-    solves the same bits; with 16-bit columns also the same bits as with 32-bit ones."""
+    """ED matrices and the m=400 superblock in the sliced form (32- or 16-bit columns, or
+    16-bit columns with 16-bit value indices -- the superblock, with 77,946 distinct values,
+    reads the 16-bit-column form there): energies within 1e-10 of the CSR form's
+    (FP32-refined within 1e-9), single and batched solves the same bits; with 16-bit columns
+    and value indices also the same bits as with 32-bit columns."""
     from dmrg_c_amd import ed, engine
     mats = []
     for m in (ed.Model("xxz", L=16, sz=0, J_z=0.7, h_z=0.2), ed.Model("hubbard", L=8, N=8, sz=0, U=3.0)):
@@ -197,7 +200,8 @@ def test_gpu_sliced_ed_and_engine(lanes, cols16):
             M.set_schedule(lanes, 4096, 24 | (4 if lanes == 2 else 0))
             if cols16:
                 wide.append(M.ground_state())
-                M.set_schedule(lanes, 4096, 25 | (4 if lanes == 2 else 0) | (2 if lanes == 4 else 0))
+                v = 32 if cols16 == "idx" else 25
+                M.set_schedule(lanes, 4096, v | (4 if lanes == 2 else 0) | (2 if lanes == 4 else 0))
         print(f"sliced lanes={lanes} 16-bit columns={cols16}", flush=True)
         single = [M.ground_state() for M in mats]
         for w, d in zip(wide, single):
