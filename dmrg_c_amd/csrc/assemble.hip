@@ -320,8 +320,8 @@ constexpr int kCapSmall = 256;   // 4 KiB of LDS per row
 constexpr int kCapMid = 1024;    // 16 KiB
 constexpr int kCapLarge = 4096;  // 64 KiB
 // Device scratch the count pass may use to keep its rows' entries (n * (kCapSmall + 1) * 12
-// bytes: 0.45 GB at n = 144542).
-constexpr int64_t kScratchBytes = (int64_t)4 << 30;
+// bytes: 0.45 GB at n = 144542, 6.1 GB at n = 1993680; held only during the assembly).
+constexpr int64_t kScratchBytes = (int64_t)16 << 30;
 
 // ------------------------------------------------------------------ host
 int op_role_dim(int slot, const dmrgx_superblock &S) {

@@ -1531,29 +1531,58 @@ __global__ __launch_bounds__(kBlock) void k_sell_width(int32_t n, int32_t nsl, c
   slots[sl] = (mx + P - 1) / P * 64;
 }
 
-// One thread per row position of every slice (rows past n included): the row's entries
-// at their slots, padding with the row's last column and the value 0 (column 0 for an
-// empty row or a position past n).
+// One wavefront (one block) per slice: its rows' entries are copied through LDS in tiles of
+// kFillTile slot columns, so that the reads take each row's run of entries in one piece and
+// the writes fill whole 64-slot columns.  Slot (k, i P + l) of a slice holds entry k P + l of
+// its row i; a slot past the row's end holds the row's last column (0 for an empty row or
+// a row past n) and the value 0.  scol may be null (values only).
+constexpr int kFillTile = 16;
 template <int P, class VT>
-__global__ __launch_bounds__(kBlock) void k_sell_fill(int32_t n, int32_t nsl, const int64_t *__restrict__ row,
-                                                      const int32_t *__restrict__ col, const VT *__restrict__ val,
-                                                      const int64_t *__restrict__ sptr, int32_t *__restrict__ scol,
-                                                      VT *__restrict__ sval) {
-  constexpr int C = 64 / P;
-  const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (t >= (int64_t)nsl * C) return;
-  const int32_t sl = (int32_t)(t / C), i = (int32_t)(t % C), r = (int32_t)t;
-  const int64_t b0 = sptr[sl], slots = (sptr[sl + 1] - b0) / 64 * P;   // entries per row position
-  const int64_t b = r < n ? row[r] : 0, len = r < n ? row[r + 1] - b : 0;
-  for (int64_t j = 0; j < slots; j++) {
-    const int64_t at = b0 + (j / P) * 64 + (int64_t)i * P + j % P;
-    if (j < len) {
-      scol[at] = col ? col[b + j] : 0;
-      sval[at] = val[b + j];
-    } else {
-      scol[at] = (len > 0 && col) ? col[b + len - 1] : 0;
-      sval[at] = VT(0);
+__global__ __launch_bounds__(64) void k_sell_fill(int32_t n, int32_t nsl, const int64_t *__restrict__ row,
+                                                  const int32_t *__restrict__ col, const VT *__restrict__ val,
+                                                  const int64_t *__restrict__ sptr, int32_t *__restrict__ scol,
+                                                  VT *__restrict__ sval) {
+  constexpr int C = 64 / P, E = kFillTile * P;   // rows per slice, a row's entries per tile
+  __shared__ VT tv[kFillTile][64];
+  __shared__ int32_t tc[kFillTile][64];
+  const int lane = threadIdx.x;
+  const int32_t sl = blockIdx.x;
+  const int64_t b0 = sptr[sl];
+  const int32_t w = (int32_t)((sptr[sl + 1] - b0) >> 6);
+  // lane i < C: row i's first entry, length and last column
+  int64_t rb = 0;
+  int32_t rl = 0, rlast = 0;
+  const int32_t r = sl * C + lane;
+  if (lane < C && r < n) {
+    rb = row[r];
+    rl = (int32_t)(row[r + 1] - rb);
+    rlast = rl > 0 ? col[rb + rl - 1] : 0;
+  }
+  for (int32_t k0 = 0; k0 < w; k0 += kFillTile) {
+#pragma unroll 4
+    for (int u = 0; u < kFillTile; u++) {   // C rows x E entries = 64 x kFillTile elements
+      const int e = u * 64 + lane;
+      const int i = e / E, jj = e % E;
+      const int64_t ib = __shfl(rb, i, 64);
+      const int32_t il = __shfl(rl, i, 64), ilast = __shfl(rlast, i, 64);
+      const int32_t j = k0 * P + jj;
+      VT v = VT(0);
+      int32_t c = ilast;
+      if (j < il) {
+        v = val[ib + j];
+        c = col[ib + j];
+      }
+      tv[jj / P][i * P + jj % P] = v;
+      tc[jj / P][i * P + jj % P] = c;
     }
+    __syncthreads();
+    const int kn = min(kFillTile, w - k0);
+    for (int k = 0; k < kn; k++) {
+      const int64_t at = b0 + (int64_t)(k0 + k) * 64 + lane;
+      sval[at] = tv[k][lane];
+      if (scol) scol[at] = tc[k][lane];
+    }
+    __syncthreads();
   }
 }
 
@@ -1697,32 +1726,29 @@ void build_sell(dmrgx_matrix &A, int P, bool fp32, hipStream_t s) {
     HIPCHK(hipStreamSynchronize(s));
     A.sell_col[lp].resize((size_t)std::max<int64_t>(total, 1));
     A.sell_val[lp].resize((size_t)std::max<int64_t>(total, 1));
-    const unsigned gf = (unsigned)(((int64_t)nsl * C + kBlock - 1) / kBlock);
     if (nsl > 0) {
       switch (P) {
-        case 1: k_sell_fill<1, double><<<gf, kBlock, 0, s>>>(A.n, nsl, A.row.p, A.col.p, A.val.p, A.sell_ptr[lp].p,
-                                                            A.sell_col[lp].p, A.sell_val[lp].p); break;
-        case 2: k_sell_fill<2, double><<<gf, kBlock, 0, s>>>(A.n, nsl, A.row.p, A.col.p, A.val.p, A.sell_ptr[lp].p,
-                                                            A.sell_col[lp].p, A.sell_val[lp].p); break;
-        default: k_sell_fill<4, double><<<gf, kBlock, 0, s>>>(A.n, nsl, A.row.p, A.col.p, A.val.p, A.sell_ptr[lp].p,
-                                                             A.sell_col[lp].p, A.sell_val[lp].p); break;
+        case 1: k_sell_fill<1, double><<<nsl, 64, 0, s>>>(A.n, nsl, A.row.p, A.col.p, A.val.p, A.sell_ptr[lp].p,
+                                                         A.sell_col[lp].p, A.sell_val[lp].p); break;
+        case 2: k_sell_fill<2, double><<<nsl, 64, 0, s>>>(A.n, nsl, A.row.p, A.col.p, A.val.p, A.sell_ptr[lp].p,
+                                                         A.sell_col[lp].p, A.sell_val[lp].p); break;
+        default: k_sell_fill<4, double><<<nsl, 64, 0, s>>>(A.n, nsl, A.row.p, A.col.p, A.val.p, A.sell_ptr[lp].p,
+                                                          A.sell_col[lp].p, A.sell_val[lp].p); break;
       }
       HIPCHK(hipGetLastError());
     }
   }
   if (fp32 && !A.sell_val32[lp].p) {
     A.sell_val32[lp].resize(A.sell_val[lp].count);
-    const unsigned gf = (unsigned)(((int64_t)nsl * C + kBlock - 1) / kBlock);
     if (nsl > 0) {
-      // columns are already in place: only the FP32 values are written (col = nullptr
-      // would zero the columns, so the column array is passed again and rewritten alike)
+      // the columns are in place already: only the FP32 values are written
       switch (P) {
-        case 1: k_sell_fill<1, float><<<gf, kBlock, 0, s>>>(A.n, nsl, A.row.p, A.col.p, A.val32.p, A.sell_ptr[lp].p,
-                                                           A.sell_col[lp].p, A.sell_val32[lp].p); break;
-        case 2: k_sell_fill<2, float><<<gf, kBlock, 0, s>>>(A.n, nsl, A.row.p, A.col.p, A.val32.p, A.sell_ptr[lp].p,
-                                                           A.sell_col[lp].p, A.sell_val32[lp].p); break;
-        default: k_sell_fill<4, float><<<gf, kBlock, 0, s>>>(A.n, nsl, A.row.p, A.col.p, A.val32.p, A.sell_ptr[lp].p,
-                                                            A.sell_col[lp].p, A.sell_val32[lp].p); break;
+        case 1: k_sell_fill<1, float><<<nsl, 64, 0, s>>>(A.n, nsl, A.row.p, A.col.p, A.val32.p, A.sell_ptr[lp].p,
+                                                        nullptr, A.sell_val32[lp].p); break;
+        case 2: k_sell_fill<2, float><<<nsl, 64, 0, s>>>(A.n, nsl, A.row.p, A.col.p, A.val32.p, A.sell_ptr[lp].p,
+                                                        nullptr, A.sell_val32[lp].p); break;
+        default: k_sell_fill<4, float><<<nsl, 64, 0, s>>>(A.n, nsl, A.row.p, A.col.p, A.val32.p, A.sell_ptr[lp].p,
+                                                         nullptr, A.sell_val32[lp].p); break;
       }
       HIPCHK(hipGetLastError());
     }
