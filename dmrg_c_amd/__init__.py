@@ -95,13 +95,13 @@ class SolveParams(C.Structure):
                 ("lan_type", C.c_int), ("guess", C.c_int), ("seed", C.c_uint64),
                 ("ii_acc", C.c_double), ("ii_diag_add", C.c_double), ("ii_max_step", C.c_int),
                 ("cg_acc", C.c_double), ("cg_max_step", C.c_int), ("skip_ii", C.c_int),
-                ("precision", C.c_int)]
+                ("precision", C.c_int), ("mixed_switch", C.c_double)]
 
 
 class SolveStats(C.Structure):
     _fields_ = [("eig_val", C.c_double), ("gs_error", C.c_double), ("lanczos_steps", C.c_int),
                 ("ii_steps", C.c_int), ("cg_steps", C.c_int), ("spmv_calls", C.c_int),
-                ("seconds", C.c_double)]
+                ("seconds", C.c_double), ("lanczos_fp32_steps", C.c_int)]
 
 
 def _dp(a: np.ndarray):

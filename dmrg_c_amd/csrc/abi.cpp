@@ -652,11 +652,15 @@ int dmrgx_ground_state(const dmrgx_matrix *A, const dmrgx_solve_params *p, doubl
     o.max_step = p->diag_max_step;
     o.type = p->lan_type;
     const bool mixed = p->precision == 1;
-    if (p->precision != 0 && !mixed) throw Failure(DMRGX_ERR_ARG, "precision must be 0 (FP64) or 1 (FP32 + FP64 refinement)");
-    if (mixed) {
+    const bool two_phase = p->precision == 2;
+    if (p->precision != 0 && !mixed && !two_phase)
+      throw Failure(DMRGX_ERR_ARG,
+                    "precision must be 0 (FP64), 1 (FP32 + FP64 refinement) or 2 (FP32 then FP64 Lanczos)");
+    if (mixed || two_phase) {
       ensure_fp32(*A);
       o.fp32 = true;
     }
+    if (two_phase) o.res_tol = p->mixed_switch > 0.0 ? p->mixed_switch : 1e-3;
     if (A->n == 1) {
       double h;
       HIPCHK(hipMemcpy(&h, A->val.p, sizeof(double), hipMemcpyDeviceToHost));
@@ -668,6 +672,20 @@ int dmrgx_ground_state(const dmrgx_matrix *A, const dmrgx_solve_params *p, doubl
       lanczos(op_of(*A), o, v.d.p, out);
     }
     if (!out.lan_converged) throw Failure(DMRGX_ERR_NOCONV, "Lanczos did not converge within diag_max_step");
+    int fp32_steps = (mixed || two_phase) ? out.lan_steps : 0;
+    if (two_phase && A->n > 1) {
+      // FP64 Lanczos from the FP32 phase's Ritz vector, to a residual bound at which the
+      // L1 residual test (at most sqrt(n) times the 2-norm) passes with room to spare
+      LanczosOpts o2 = o;
+      o2.fp32 = false;
+      o2.res_tol = 0.5 * p->ii_acc / std::sqrt((double)A->n) / std::max(1.0, std::fabs(out.eig));
+      SolveOut out2;
+      lanczos(op_of(*A), o2, v.d.p, out2);
+      if (!out2.lan_converged) throw Failure(DMRGX_ERR_NOCONV, "Lanczos did not converge within diag_max_step");
+      out.eig = out2.eig;
+      out.lan_steps += out2.lan_steps;
+      out.spmv += out2.spmv;
+    }
     // mixed precision: the FP32 Lanczos value is off by the FP32 rounding of the matrix; the
     // inverse iteration's shift and its residual test take the FP64 Rayleigh quotient of the
     // Ritz vector instead (exact to the square of the vector's error)
@@ -677,10 +695,11 @@ int dmrgx_ground_state(const dmrgx_matrix *A, const dmrgx_solve_params *p, doubl
     }
     if (!p->skip_ii)
       inverse_iteration(op_of(*A), out.eig, v.d.p, p->ii_acc, p->ii_diag_add, p->ii_max_step, p->cg_acc, p->cg_max_step,
-                        out, false, mixed);
+                        out, false, mixed || two_phase);
     // mixed precision: the FP32 Lanczos value is only a shift for the FP64 refinement;
-    // the energy reported is the FP64 Rayleigh quotient of the refined vector
-    if (mixed && A->n > 1) {
+    // the energy reported is the FP64 Rayleigh quotient of the refined vector (in the
+    // two-phase form only if inverse iteration changed the FP64 Lanczos vector)
+    if ((mixed || (two_phase && out.cg_steps > 0)) && A->n > 1) {
       out.eig = rayleigh_quotient(op_of(*A), v.d.p);
       out.spmv++;
     }
@@ -693,6 +712,7 @@ int dmrgx_ground_state(const dmrgx_matrix *A, const dmrgx_solve_params *p, doubl
       stats->cg_steps = out.cg_steps;
       stats->spmv_calls = out.spmv;
       stats->seconds = now_s() - t0;
+      stats->lanczos_fp32_steps = fp32_steps;
     }
   });
 }

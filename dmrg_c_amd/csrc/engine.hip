@@ -1384,6 +1384,7 @@ int dmrgx_ground_state_batch(int32_t count, dmrgx_matrix *const *A, const dmrgx_
       stats[i].cg_steps = r.cg_steps;
       stats[i].spmv_calls = r.spmv_calls;
       stats[i].seconds = r.t_done - t0;
+      stats[i].lanczos_fp32_steps = p && p->precision == 1 ? r.lanczos_steps : 0;
     }
   }
   dmrgx_engine_free(E);

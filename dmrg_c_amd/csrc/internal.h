@@ -877,6 +877,9 @@ struct LanczosOpts {
   int type = 0;        // 0 Normal, 1 Tight
   bool fp32 = false;
   bool dense_small = true;  // n < 1000: residual-bound stop standing in for the dense dsyev_ branch
+  // > 0: stop instead when the Ritz pair's residual bound beta_{k+1}|s_k| falls to
+  // res_tol * max(1, |E_k|) (type and acc are then not used)
+  double res_tol = 0.0;
 };
 // vec: device in/out (start on input, Ritz vector on output)
 void lanczos(const Operator &A, const LanczosOpts &o, double *vec, SolveOut &out);

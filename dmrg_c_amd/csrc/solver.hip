@@ -98,7 +98,9 @@ void lanczos(const Operator &A, const LanczosOpts &o, double *vec, SolveOut &out
     launch_lanczos_b(n, 0, y.p, V.vec(0), nullptr, alpha, beta, w.p, nrm2, R, st);
     out.spmv++;
     int kconv = -1, chunk = small ? 16 : 8;
-    double dE_first = 0.0, dE_last = 0.0;   // first and last |E_k - E_{k-1}| of the chunk
+    // first and last |E_k - E_{k-1}| of the chunk (with res_tol: the residual bounds) and
+    // the value the prediction aims at
+    double dE_first = 0.0, dE_last = 0.0, res_target = o.acc;
     int n_dE = 0;
     double lam = E;
     RitzTrack track;  // T of this block
@@ -166,6 +168,15 @@ void lanczos(const Operator &A, const LanczosOpts &o, double *vec, SolveOut &out
           conv = !(bnext > 1e-12 * tscale) || bnext * std::fabs(s[k]) <= 1e-12 * tscale;
         } else if (!(bnext > 1e-14 * tscale)) {
           conv = true;  // invariant subspace: the Ritz values are exact
+        } else if (o.res_tol > 0.0) {
+          tridiag_vector(k + 1, ha.data(), hb.data() + 1, lam, s.data());
+          const double res = bnext * std::fabs(s[k]);
+          conv = res <= o.res_tol * tscale;
+          if (!conv) {   // progress measure of the chunk prediction below
+            if (n_dE++ == 0) dE_first = res;
+            dE_last = res;
+            res_target = o.res_tol * tscale;
+          }
         } else if (o.type == 1) {
           tridiag_vector(k + 1, ha.data(), hb.data() + 1, lam, s.data());
           conv = std::fabs(s[k]) < o.acc;     // "Tight": sml/LANCZOS_SLOW_SYM.c:140-146
@@ -180,7 +191,7 @@ void lanczos(const Operator &A, const LanczosOpts &o, double *vec, SolveOut &out
           block_done = true;
           break;
         }
-        if (E < 1e14) {   // a real previous value (E starts at 1e15)
+        if (o.res_tol <= 0.0 && E < 1e14) {   // a real previous value (E starts at 1e15)
           if (n_dE++ == 0) dE_first = std::fabs(E - lam);
           dE_last = std::fabs(E - lam);
         }
@@ -192,9 +203,9 @@ void lanczos(const Operator &A, const LanczosOpts &o, double *vec, SolveOut &out
       // doubling while no prediction holds.
       const int queued = (npend > 0) ? pend[0].count : 0;
       int nc;
-      if (!small && o.type != 1 && n_dE >= 3 && dE_last > 0.0 && dE_last < dE_first) {
+      if (!small && (o.res_tol > 0.0 || o.type != 1) && n_dE >= 3 && dE_last > 0.0 && dE_last < dE_first) {
         const double ratio = std::pow(dE_last / dE_first, 1.0 / (n_dE - 1));   // mean ratio per step
-        const double need = std::ceil(std::log(o.acc / dE_last) / std::log(ratio)) + 2.0;
+        const double need = std::ceil(std::log(res_target / dE_last) / std::log(ratio)) + 2.0;
         nc = (int)std::min(32.0, need - queued);
       } else {
         chunk = std::min(chunk * 2, 32);

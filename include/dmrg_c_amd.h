@@ -560,7 +560,13 @@ typedef struct {
    int cg_max_step;
    int skip_ii;             /* 1: Lanczos only                                    */
    int precision;           /* 0: FP64; 1: FP32 CRS values + FP32 SpMV in Lanczos,
-                                FP64 recurrence scalars and FP64 II/CG refinement  */
+                                FP64 recurrence scalars and FP64 II/CG refinement;
+                                2 (single solves): FP32 Lanczos until its Ritz residual
+                                bound falls to mixed_switch*max(1,|E|), then FP64
+                                Lanczos from that Ritz vector until the bound is below
+                                ii_acc/(2 sqrt(n)), then the FP64 residual test (and
+                                FP64 II/CG if it fails)                              */
+   double mixed_switch;     /* precision 2: hand-over residual (<= 0: 1e-3)          */
 } dmrgx_solve_params;
 
 typedef struct {
@@ -571,6 +577,7 @@ typedef struct {
    int cg_steps;
    int spmv_calls;
    double seconds;
+   int lanczos_fp32_steps;  /* of lanczos_steps, those with FP32 values (precision 1, 2) */
 } dmrgx_solve_stats;
 
 void dmrgx_default_params(dmrgx_solve_params *p);

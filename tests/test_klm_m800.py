@@ -66,7 +66,8 @@ def test_device_assembly_bit_exact(snap, sb):
     assert np.array_equal(val.view(np.int64), oval.view(np.int64))
 
 
-@pytest.mark.parametrize("precision,tol", [(0, 1e-10), (1, 1e-9)], ids=["fp64", "fp32_refined"])
+@pytest.mark.parametrize("precision,tol", [(0, 1e-10), (1, 1e-9), (2, 1e-9)],
+                         ids=["fp64", "fp32_refined", "fp32_two_phase"])
 def test_ground_state(snap, sb, precision, tol):
     e_ref = float(snap.meta["gs_energy"][0])
     A = sb.assemble(full=True)
