@@ -64,7 +64,7 @@ SELL16_KERNEL = ("k_apply<P, Sell16Rows<double, P>> (sliced SpMV with 16-bit col
 
 
 def kernel_name(sched):
-    fmt = sched[2] >> 3
+    fmt = (sched[2] >> 3) & 7
     return SELL16_KERNEL if fmt == 3 and sched[2] & 1 else SPMV_FORMATS[fmt][1]
 
 
@@ -272,7 +272,7 @@ def spmv_sweep(D, sb, snap):
 def spmv_alg_bytes(D, A, sched, n):
     """Algorithmic bytes of one product in the format the schedule reads: CSR (8-byte
     values, 4-byte columns) or the value dictionary (8-byte packed entries + the table)."""
-    if (sched[2] >> 3) == 4:
+    if ((sched[2] >> 3) & 7) == 4:
         nsl = (n + 64 // sched[0] - 1) // (64 // sched[0])
         exc = A.sell16_exceptions(sched[0])
         L = D.lib()
@@ -284,7 +284,7 @@ def spmv_alg_bytes(D, A, sched, n):
                     f"sliced, 16-bit columns and value indices ({sched[0]} lanes per row; {nv.value} distinct "
                     f"values, {exc} columns kept whole, padding not counted)")
         sched = (sched[0], sched[1], (sched[2] & 7) | 24 | 1)   # read as format 3 with 16-bit columns
-    if (sched[2] >> 3) == 3:
+    if ((sched[2] >> 3) & 7) == 3:
         nsl = (n + 64 // sched[0] - 1) // (64 // sched[0])
         if sched[2] & 1:
             exc = A.sell16_exceptions(sched[0])
@@ -293,10 +293,10 @@ def spmv_alg_bytes(D, A, sched, n):
                     f"{exc} columns kept whole, padding not counted)")
         return (A.nnz * 12 + n * 4 + (nsl + 1) * 8 + 2 * n * 8,
                 f"sliced ({sched[0]} lanes per row; 8-byte values, 4-byte columns, padding not counted)")
-    if (sched[2] >> 3) == 2:
+    if ((sched[2] >> 3) & 7) == 2:
         return (A.nnz * 10 + (n + 1) * 8 + n * 4 + 2 * n * 8,
                 "delta columns (8-byte values, 2-byte column differences, 4-byte first columns)")
-    if (sched[2] >> 3) == 1:
+    if ((sched[2] >> 3) & 7) == 1:
         L = D.lib()
         L.dmrgx_matrix_dict_values.argtypes = [C.c_void_p, C.POINTER(C.c_int64)]
         nv = C.c_int64()
@@ -903,7 +903,7 @@ def run_superblock(D, L, name, steps, warmup, fixed, cpu_budget):
         if list(tj.get("schedule", [])) != list(lanes):
             traffic_note = (f"rocprof counters of the schedule {tj.get('schedule')} "
                             f"(this run's autotuner picked {list(lanes)})")
-    fmt = lanes[2] >> 3
+    fmt = (lanes[2] >> 3) & 7
     res = {
         "value": round(steps / total, 4),
         "ms_per_step": round(total / steps * 1e3, 3),
@@ -922,6 +922,7 @@ def run_superblock(D, L, name, steps, warmup, fixed, cpu_budget):
                               "paired_entry_loads": bool(lanes[2] & 1) and fmt == 0,
                               "entries_in_flight_per_lane": 8 if (lanes[2] & 2 or fmt >= 3) else 4,
                               "non_temporal_matrix_loads": bool(lanes[2] & 2) and fmt >= 3,
+                              "matrix_loads_one_group_ahead": bool(lanes[2] & 64),
                               "column_bits": 16 if ((fmt == 3 and lanes[2] & 1) or fmt == 4) else 32,
                               "xcd_contiguous_rows": bool(lanes[2] & 4),
                               "format": SPMV_FORMATS[fmt][0]},

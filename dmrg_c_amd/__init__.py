@@ -680,6 +680,11 @@ class Superblock:
         self.nnz_lower = nnz.value
         return DeviceMatrix._wrap(A, self.n) if full else None
 
+    def kron(self) -> "KronOperator":
+        """The Kronecker form of this superblock's Hamiltonian (dmrgx_superblock_kron); the
+        superblock must outlive it."""
+        return KronOperator(self)
+
     def download_lower(self):
         row = np.empty(self.n + 1, np.int64)
         col = np.empty(max(self.nnz_lower, 1), np.int32)
@@ -690,6 +695,59 @@ class Superblock:
     def free(self):
         if getattr(self, "handle", None):
             lib().dmrgx_superblock_free(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class KronOperator:
+    """dmrgx_kron: the superblock Hamiltonian applied per sector as M_L X + X M_R^T plus the
+    coupling terms, without an assembled CRS."""
+
+    def __init__(self, sb: "Superblock"):
+        L = lib()
+        vp = C.c_void_p
+        L.dmrgx_superblock_kron.argtypes = [vp, C.POINTER(vp)]
+        L.dmrgx_kron_info.argtypes = [vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                      C.POINTER(C.c_double)]
+        L.dmrgx_kron_spmv.argtypes = [vp, vp, vp]
+        L.dmrgx_kron_ground_state.argtypes = [vp, C.POINTER(SolveParams), vp, C.POINTER(SolveStats)]
+        L.dmrgx_kron_free.argtypes = [vp]
+        L.dmrgx_kron_free.restype = None
+        self._sb = sb   # the superblock outlives the form
+        h = vp()
+        check(L.dmrgx_superblock_kron(sb.handle, C.byref(h)))
+        self.handle = h
+        n, ns, nc, fl = C.c_int32(), C.c_int32(), C.c_int32(), C.c_double()
+        check(L.dmrgx_kron_info(h, C.byref(n), C.byref(ns), C.byref(nc), C.byref(fl)))
+        self.n, self.sectors, self.coupling_terms, self.flops = n.value, ns.value, nc.value, fl.value
+
+    def matvec(self, x: np.ndarray) -> np.ndarray:
+        xx = np.ascontiguousarray(x, np.float64)
+        y = np.empty(self.n)
+        check(lib().dmrgx_kron_spmv(self.handle, xx.ctypes.data, y.ctypes.data))
+        return y
+
+    def ground_state(self, start: Optional[np.ndarray] = None, **kw):
+        p = SolveParams()
+        lib().dmrgx_default_params(C.byref(p))
+        for k, v in kw.items():
+            setattr(p, k, v)
+        vec = np.zeros(self.n)
+        if start is not None:
+            p.guess = 1
+            vec[:] = start
+        st = SolveStats()
+        check(lib().dmrgx_kron_ground_state(self.handle, C.byref(p), vec.ctypes.data, C.byref(st)))
+        return st.eig_val, vec, st
+
+    def free(self):
+        if getattr(self, "handle", None):
+            lib().dmrgx_kron_free(self.handle)
             self.handle = None
 
     def __del__(self):

@@ -415,8 +415,7 @@ int dmrgx_spmv_device(const dmrgx_matrix *A, const double *x_dev, double *y_dev)
 }
 
 int dmrgx_matrix_set_schedule(dmrgx_matrix *A, int32_t lanes, int32_t grid_cap, int32_t load_variant) {
-  if (!A || lanes < 1 || lanes > 64 || (lanes & (lanes - 1)) || grid_cap < 0 || load_variant < 0 ||
-      load_variant >= kLoadVariants)
+  if (!A || lanes < 1 || lanes > 64 || (lanes & (lanes - 1)) || grid_cap < 0 || !valid_load(load_variant))
     return DMRGX_ERR_ARG;
   if (sliced_format(load_format(load_variant)) && lanes > kSellMaxLanes) return DMRGX_ERR_ARG;
   std::lock_guard<std::recursive_mutex> lk(api_mutex());
@@ -533,8 +532,8 @@ struct TuneEnv {
 
 int dmrgx_matrix_time_schedule(dmrgx_matrix *A, int32_t lanes, int32_t grid_cap, int32_t load_variant,
                                int32_t reps, double *us_out) {
-  if (!A || !us_out || lanes < 1 || lanes > 64 || (lanes & (lanes - 1)) || grid_cap < 0 || load_variant < 0 ||
-      load_variant >= kLoadVariants || reps < 1)
+  if (!A || !us_out || lanes < 1 || lanes > 64 || (lanes & (lanes - 1)) || grid_cap < 0 ||
+      !valid_load(load_variant) || reps < 1)
     return DMRGX_ERR_ARG;
   std::lock_guard<std::recursive_mutex> lk(api_mutex());
   return run_status([&] {
@@ -584,7 +583,12 @@ int dmrgx_matrix_autotune(dmrgx_matrix *A, int32_t *lanes_out, int32_t *grid_out
         // indices (bit 0 ignored) where the matrix has a small value table
         if (load_format(v) == kFmtSellIdx && ((v & kLoadPair) || !has_idx)) continue;
         for (int g = 1; g <= kSellMaxLanes; g *= 2)
-          for (int cap : caps) cands.push_back({g, cap, v, env.time(A, g, cap, v, 3)});
+          for (int cap : caps) {
+            cands.push_back({g, cap, v, env.time(A, g, cap, v, 3)});
+            // 16-bit columns: also with the matrix loads one pass group ahead
+            if (load_format(v) == kFmtSell && (v & kLoadPair))
+              cands.push_back({g, cap, v | kLoadPipe, env.time(A, g, cap, v | kLoadPipe, 3)});
+          }
         continue;
       }
       if (has[load_format(v)])
@@ -635,16 +639,21 @@ void dmrgx_default_params(dmrgx_solve_params *p) {
   p->cg_max_step = 2000;
 }
 
-int dmrgx_ground_state(const dmrgx_matrix *A, const dmrgx_solve_params *p, double *vec_inout,
-                       dmrgx_solve_stats *stats) {
-  if (!A || !p || !vec_inout) return DMRGX_ERR_ARG;
+}  // extern "C"
+
+namespace {
+// The ground-state solve on an operator: an assembled matrix A (op = op_of(*A)) or the
+// Kronecker form (A null: FP64 only).
+int ground_state_op(const Operator &op, const dmrgx_matrix *A, const dmrgx_solve_params *p, double *vec_inout,
+                    dmrgx_solve_stats *stats) {
+  if (!p || !vec_inout) return DMRGX_ERR_ARG;
   if (p->diag_max_step <= p->diag_min_step) return DMRGX_ERR_ARG;
   return run_status([&] {
     const double t0 = now_s();
     hipStream_t st = ctx().stream;
-    DevVec v(A->n);
+    DevVec v(op.n);
     if (p->guess) v.up(vec_inout);
-    else launch_start_vector(A->n, p->seed, v.d.p, st);
+    else launch_start_vector(op.n, p->seed, v.d.p, st);
     SolveOut out;
     LanczosOpts o;
     o.acc = p->diag_acc;
@@ -656,31 +665,39 @@ int dmrgx_ground_state(const dmrgx_matrix *A, const dmrgx_solve_params *p, doubl
     if (p->precision != 0 && !mixed && !two_phase)
       throw Failure(DMRGX_ERR_ARG,
                     "precision must be 0 (FP64), 1 (FP32 + FP64 refinement) or 2 (FP32 then FP64 Lanczos)");
+    if ((mixed || two_phase) && !A) throw Failure(DMRGX_ERR_ARG, "the Kronecker form solves in FP64 only (precision 0)");
     if (mixed || two_phase) {
       ensure_fp32(*A);
       o.fp32 = true;
     }
     if (two_phase) o.res_tol = p->mixed_switch > 0.0 ? p->mixed_switch : 1e-3;
-    if (A->n == 1) {
+    if (op.n == 1) {
       double h;
-      HIPCHK(hipMemcpy(&h, A->val.p, sizeof(double), hipMemcpyDeviceToHost));
+      if (A) {
+        HIPCHK(hipMemcpy(&h, A->val.p, sizeof(double), hipMemcpyDeviceToHost));
+      } else {   // the 1 x 1 operator applied to 1
+        const double one = 1.0;
+        v.up(&one);
+        op.kron->product(v.d.p, st);
+        HIPCHK(hipMemcpy(&h, op.kron->z.p, sizeof(double), hipMemcpyDeviceToHost));
+      }
       out.eig = h;
       const double one = 1.0;
       v.up(&one);
       out.lan_converged = true;
     } else {
-      lanczos(op_of(*A), o, v.d.p, out);
+      lanczos(op, o, v.d.p, out);
     }
     if (!out.lan_converged) throw Failure(DMRGX_ERR_NOCONV, "Lanczos did not converge within diag_max_step");
     int fp32_steps = (mixed || two_phase) ? out.lan_steps : 0;
-    if (two_phase && A->n > 1) {
+    if (two_phase && op.n > 1) {
       // FP64 Lanczos from the FP32 phase's Ritz vector, to a residual bound at which the
       // L1 residual test (at most sqrt(n) times the 2-norm) passes with room to spare
       LanczosOpts o2 = o;
       o2.fp32 = false;
-      o2.res_tol = 0.5 * p->ii_acc / std::sqrt((double)A->n) / std::max(1.0, std::fabs(out.eig));
+      o2.res_tol = 0.5 * p->ii_acc / std::sqrt((double)op.n) / std::max(1.0, std::fabs(out.eig));
       SolveOut out2;
-      lanczos(op_of(*A), o2, v.d.p, out2);
+      lanczos(op, o2, v.d.p, out2);
       if (!out2.lan_converged) throw Failure(DMRGX_ERR_NOCONV, "Lanczos did not converge within diag_max_step");
       out.eig = out2.eig;
       out.lan_steps += out2.lan_steps;
@@ -689,18 +706,18 @@ int dmrgx_ground_state(const dmrgx_matrix *A, const dmrgx_solve_params *p, doubl
     // mixed precision: the FP32 Lanczos value is off by the FP32 rounding of the matrix; the
     // inverse iteration's shift and its residual test take the FP64 Rayleigh quotient of the
     // Ritz vector instead (exact to the square of the vector's error)
-    if (mixed && A->n > 1 && !p->skip_ii) {
-      out.eig = rayleigh_quotient(op_of(*A), v.d.p);
+    if (mixed && op.n > 1 && !p->skip_ii) {
+      out.eig = rayleigh_quotient(op, v.d.p);
       out.spmv++;
     }
     if (!p->skip_ii)
-      inverse_iteration(op_of(*A), out.eig, v.d.p, p->ii_acc, p->ii_diag_add, p->ii_max_step, p->cg_acc, p->cg_max_step,
+      inverse_iteration(op, out.eig, v.d.p, p->ii_acc, p->ii_diag_add, p->ii_max_step, p->cg_acc, p->cg_max_step,
                         out, false, mixed || two_phase);
     // mixed precision: the FP32 Lanczos value is only a shift for the FP64 refinement;
     // the energy reported is the FP64 Rayleigh quotient of the refined vector (in the
     // two-phase form only if inverse iteration changed the FP64 Lanczos vector)
-    if ((mixed || (two_phase && out.cg_steps > 0)) && A->n > 1) {
-      out.eig = rayleigh_quotient(op_of(*A), v.d.p);
+    if ((mixed || (two_phase && out.cg_steps > 0)) && op.n > 1) {
+      out.eig = rayleigh_quotient(op, v.d.p);
       out.spmv++;
     }
     v.down(vec_inout);
@@ -715,6 +732,22 @@ int dmrgx_ground_state(const dmrgx_matrix *A, const dmrgx_solve_params *p, doubl
       stats->lanczos_fp32_steps = fp32_steps;
     }
   });
+}
+
+}  // namespace
+
+extern "C" {
+
+int dmrgx_ground_state(const dmrgx_matrix *A, const dmrgx_solve_params *p, double *vec_inout,
+                       dmrgx_solve_stats *stats) {
+  if (!A) return DMRGX_ERR_ARG;
+  return ground_state_op(op_of(*A), A, p, vec_inout, stats);
+}
+
+int dmrgx_kron_ground_state(const dmrgx_kron *K, const dmrgx_solve_params *p, double *vec_inout,
+                            dmrgx_solve_stats *stats) {
+  if (!K) return DMRGX_ERR_ARG;
+  return ground_state_op(op_of(kron_dev(K)), nullptr, p, vec_inout, stats);
 }
 
 int dmrgx_profile_begin_sampled(int32_t every) {

@@ -300,6 +300,9 @@ __device__ __forceinline__ double csr_lane_sum(const int64_t *__restrict__ row, 
 constexpr int kLoadPair = 1;   // CSR: two entries per lane per load (changes the order)
 constexpr int kLoadDeep = 2;
 constexpr int kLoadXcd = 4;
+// Bit 6, sliced form with 16-bit columns only: matrix loads one group of eight passes
+// ahead of the x gathers (sell16_lane_sum_pipe); never changes a result.
+constexpr int kLoadPipe = 64;
 // Bits 3-5: storage format of the product.  CSR: 4-byte columns; dictionary (FP64 products
 // only): 8-byte (column, value index) entries over a table of the distinct values; delta:
 // 2-byte column differences within each row plus each row's first column.  Every format
@@ -323,6 +326,11 @@ constexpr int kLoadVariants = 5 << kFmtShift;   // 0 .. 39
 constexpr int kSellMaxLanes = 4;
 constexpr int64_t kSellIdxMaxValues = 65536;
 __host__ __device__ inline int load_format(int load) { return (load & kLoadFmtMask) >> kFmtShift; }
+// a load variant the products accept: bits 0-5 below kLoadVariants, bit 6 with format 3
+__host__ __device__ inline bool valid_load(int load) {
+  const int base = load & ~kLoadPipe;
+  return load >= 0 && base < kLoadVariants && (!(load & kLoadPipe) || load_format(base) == kFmtSell);
+}
 __host__ __device__ inline bool sliced_format(int fmt) { return fmt == kFmtSell || fmt == kFmtSellIdx; }
 
 // Row groups [start, end) with stride step that logical block lb of nblk takes out of
@@ -604,6 +612,88 @@ __device__ __forceinline__ double sell16_lane_sum(const int64_t *__restrict__ sp
   return ((acc[0] + acc[4]) + (acc[2] + acc[6])) + ((acc[1] + acc[5]) + (acc[3] + acc[7]));
 }
 
+// sell16_lane_sum with the matrix loads one pass group ahead: the values and column
+// differences of slots k+8 .. k+15 are requested before the x gathers of slots k .. k+7,
+// so that a wavefront's next matrix loads are in flight while it waits for its gathers.
+// Same slots, values, columns, order and masks: the same sums bit for bit.  The group
+// ahead is loaded only while there is one (w is uniform over a wavefront, which holds one
+// slice), so every load stays inside the slice.
+template <int P, class VT, bool NT = false>
+__device__ __forceinline__ double sell16_lane_sum_pipe(const int64_t *__restrict__ sptr,
+                                                       const int32_t *__restrict__ slen,
+                                                       const uint16_t *__restrict__ sd16,
+                                                       const int32_t *__restrict__ sfirst,
+                                                       const int64_t *__restrict__ septr,
+                                                       const int32_t *__restrict__ sexc,
+                                                       const VT *__restrict__ sval, int32_t r, int lane,
+                                                       const double *__restrict__ x) {
+  constexpr int C = 64 / P;
+  const int32_t s = r / C;
+  const int j = (r % C) * P + lane;
+  const int64_t b0 = sptr[s];
+  const int64_t base = b0 + j;
+  const int32_t w = (int32_t)((sptr[s + 1] - b0) >> 6);
+  const int32_t len = slen[r];
+  const int64_t q = (int64_t)s * 64 + j;
+  int32_t c = sfirst[q];
+  int64_t e = septr[q];
+  int32_t en = sexc[e];
+  double acc[kSellAcc];
+#pragma unroll
+  for (int t = 0; t < kSellAcc; t++) acc[t] = 0.0;
+  VT v[kSellAcc];
+  uint16_t d[kSellAcc];
+  if (w > 0) {   // an empty slice has no slots to load
+#pragma unroll
+    for (int t = 0; t < kSellAcc; t++) {
+      const int64_t at = base + (int64_t)min(t, w - 1) * 64;
+      v[t] = stream_load<NT>(sval + at);
+      d[t] = stream_load<NT>(sd16 + at);
+    }
+  }
+  for (int32_t k = 0; k < w; k += kSellAcc) {
+    int32_t cc[kSellAcc];
+#pragma unroll
+    for (int t = 0; t < kSellAcc; t++) {
+      const int32_t kk = k + t;
+      if (kk > 0 && kk < w) {
+        if (d[t] == 0xFFFF) {
+          c = en;
+          en = sexc[++e];
+        } else {
+          c += d[t];
+        }
+      }
+      cc[t] = c;
+    }
+    VT vn[kSellAcc];
+    uint16_t dn[kSellAcc];
+    const bool ahead = k + kSellAcc < w;
+    if (ahead) {
+#pragma unroll
+      for (int t = 0; t < kSellAcc; t++) {
+        const int64_t at = base + (int64_t)min(k + kSellAcc + t, w - 1) * 64;
+        vn[t] = stream_load<NT>(sval + at);
+        dn[t] = stream_load<NT>(sd16 + at);
+      }
+    }
+    double xv[kSellAcc];
+#pragma unroll
+    for (int t = 0; t < kSellAcc; t++) xv[t] = x[cc[t]];
+#pragma unroll
+    for (int t = 0; t < kSellAcc; t++)
+      if ((k + t) * P + lane < len) acc[t] += (double)v[t] * xv[t];
+    if (ahead) {
+#pragma unroll
+      for (int t = 0; t < kSellAcc; t++) {
+        v[t] = vn[t];
+        d[t] = dn[t];
+      }
+    }
+  }
+  return ((acc[0] + acc[4]) + (acc[2] + acc[6])) + ((acc[1] + acc[5]) + (acc[3] + acc[7]));
+}
+
 // sell16_lane_sum with each slot's value read through its 16-bit index into the value
 // table (format 4): 2 + 2 bytes per slot; table[idx] equals the slot's value bit for bit,
 // so the sums are sell16_lane_sum's.  The table (a few dozen to 65536 values) stays in the
@@ -732,7 +822,33 @@ struct ObcDev {
   }
 };
 
-// The operator a solver applies: an assembled full CSR, or the matrix-free OBC form.
+// Kronecker form of a superblock Hamiltonian (kron.hip): per sector of the superblock basis
+// (rows with one LLLR quantum number: a dense da x db block of the vector, LLLR states by
+// RRRL states) the LLLR-only terms as a dense da x da matrix M_L and the RRRL-only terms as
+// a dense db x db matrix M_R, so that their part of H x is M_L X + X M_R^T per sector; the
+// terms that couple the two halves are applied row by row from the term list.
+struct KronSector {
+  int64_t off;        // first superblock row
+  int32_t da, db;     // LLLR states, RRRL states
+  int64_t ml, mr;     // offsets of the sector's M_L and M_R in KronDev::ML / MR
+  int32_t a0, b0;     // first of its LLLR / RRRL states in the flat state lists
+};
+struct KronDev {
+  int32_t n = 0, nsec = 0, ntiles = 0, ncross = 0, na = 0, nb = 0;
+  const dmrgx_superblock *sb = nullptr;   // borrowed: basis, inverse, operators, term list
+  std::vector<KronSector> hsec;
+  DevBuf<KronSector> sec;
+  DevBuf<double> ML, MR;                  // dense sector blocks, row-major
+  DevBuf<int32_t> tiles;                  // (sector, tile row, tile column) of the product
+  DevBuf<int32_t> cross;                  // indices of the coupling terms in the term list
+  mutable DevBuf<double> z;               // H x of the last product
+  double flops = 0.0;                     // of the two dense products per H x
+  // z = H x on stream s
+  void product(const double *x, hipStream_t s) const;
+};
+
+// The operator a solver applies: an assembled full CSR, the matrix-free OBC form, or the
+// Kronecker form.
 struct Operator {
   int32_t n = 0;
   int lanes = 8;
@@ -740,9 +856,12 @@ struct Operator {
   int load = 0;
   const dmrgx_matrix *A = nullptr;
   const ObcDev *obc = nullptr;
+  const KronDev *kron = nullptr;
 };
-inline Operator op_of(const dmrgx_matrix &A) { return Operator{A.n, A.lanes, A.grid, A.load, &A, nullptr}; }
-inline Operator op_of(const ObcDev &O) { return Operator{O.n, O.lanes, kGridCap, 0, nullptr, &O}; }
+inline Operator op_of(const dmrgx_matrix &A) { return Operator{A.n, A.lanes, A.grid, A.load, &A, nullptr, nullptr}; }
+inline Operator op_of(const ObcDev &O) { return Operator{O.n, O.lanes, kGridCap, 0, nullptr, &O, nullptr}; }
+inline Operator op_of(const KronDev &K) { return Operator{K.n, 1, kGridCap, 0, nullptr, nullptr, &K}; }
+const KronDev &kron_dev(const dmrgx_kron *K);
 
 // ---- kernel launchers (kernels.hip)
 struct Reduce {              // deterministic two-kernel reduction workspace

@@ -108,8 +108,9 @@ struct SellRows {
   }
 };
 
-// The sliced form with 16-bit columns (sell16_lane_sum); only launched with G == P.
-template <class VT, int P, bool NT = false>
+// The sliced form with 16-bit columns (sell16_lane_sum, or sell16_lane_sum_pipe with PIPE);
+// only launched with G == P.
+template <class VT, int P, bool NT = false, bool PIPE = false>
 struct Sell16Rows {
   const int64_t *__restrict__ sptr;
   const int32_t *__restrict__ slen;
@@ -131,7 +132,10 @@ struct Sell16Rows {
   template <int G>
   __device__ __forceinline__ double lane_sum(int32_t r, int lane, const double *__restrict__ x) const {
     static_assert(G == P, "sliced rows are read with P lanes per row");
-    return sell16_lane_sum<P, VT, NT>(sptr, slen, sd16, sfirst, septr, sexc, sval, r, lane, x);
+    if constexpr (PIPE)
+      return sell16_lane_sum_pipe<P, VT, NT>(sptr, slen, sd16, sfirst, septr, sexc, sval, r, lane, x);
+    else
+      return sell16_lane_sum<P, VT, NT>(sptr, slen, sd16, sfirst, septr, sexc, sval, r, lane, x);
   }
 };
 
@@ -473,6 +477,20 @@ int apply_sell_p(const Operator &op, const VT *sval, const double *x, const Epi 
   }
   const int xcd = (op.load & kLoadXcd) ? 1 : 0;
   if ((op.load & kLoadPair) && A.sell_nexc[lp] >= 0) {   // 16-bit columns (built by apply_op)
+    if (op.load & kLoadPipe) {
+      if (op.load & kLoadDeep) {
+        const Sell16Rows<VT, P, true, true> rows{A.sell_ptr[lp].p, A.sell_len.p,      A.sell_d16[lp].p,
+                                                 A.sell_first[lp].p, A.sell_eptr[lp].p, A.sell_exc[lp].p,
+                                                 sval,              A.nnz,             A.sell_nexc[lp],
+                                                 nsl};
+        return apply_rows_g<P>(op.n, op.grid, rows, x, epi, part, s, xcd, fin, true);
+      }
+      const Sell16Rows<VT, P, false, true> rows{A.sell_ptr[lp].p, A.sell_len.p,      A.sell_d16[lp].p,
+                                                A.sell_first[lp].p, A.sell_eptr[lp].p, A.sell_exc[lp].p,
+                                                sval,              A.nnz,             A.sell_nexc[lp],
+                                                nsl};
+      return apply_rows_g<P>(op.n, op.grid, rows, x, epi, part, s, xcd, fin, true);
+    }
     if (op.load & kLoadDeep) {
       const Sell16Rows<VT, P, true> rows{A.sell_ptr[lp].p, A.sell_len.p,   A.sell_d16[lp].p, A.sell_first[lp].p,
                                          A.sell_eptr[lp].p, A.sell_exc[lp].p, sval, A.nnz, A.sell_nexc[lp], nsl};
@@ -557,10 +575,27 @@ int apply_sell(const Operator &op, bool fp32, const double *x, const Epi &epi, R
   throw Failure(DMRGX_ERR_ARG, "sliced form needs 1, 2 or 4 lanes per row");
 }
 
+// Rows of a product computed beforehand (the Kronecker form writes H x into z): one lane per
+// row returns z[r], so that the solvers' fused epilogues run on it unchanged.
+struct PreRows {
+  const double *z;
+  static constexpr bool kTimed = false;
+  double alg_bytes(int32_t) const { return 0.0; }
+  template <int G>
+  __device__ __forceinline__ double lane_sum(int32_t r, int lane, const double *__restrict__) const {
+    return lane == 0 ? z[r] : 0.0;
+  }
+};
+
 template <class Epi>
 int apply_op(const Operator &op, bool fp32, const double *x, const Epi &epi, Reduce *R, hipStream_t s,
              bool fin = true) {
   if (op.obc) return apply_rows(op.n, op.lanes, op.grid, op.obc->rows(), x, epi, R, s, 0, fin);
+  if (op.kron) {
+    if (fp32) throw Failure(DMRGX_ERR_ARG, "the Kronecker form has no FP32 values");
+    op.kron->product(x, s);
+    return apply_rows(op.n, 1, op.grid, PreRows{op.kron->z.p}, x, epi, R, s, 0, fin);
+  }
   const int fmt = load_format(op.load);
   if (fmt == kFmtSellIdx && ensure_sell(*op.A, op.lanes, false)) {
     ensure_sell16(*op.A, op.lanes);

@@ -1,0 +1,422 @@
+// Kronecker form of the superblock Hamiltonian: the operator of the assembled CRS
+// (GET_HAM_LLLRRRRL from the recorded term list) applied without assembling it.
+//
+// The superblock basis (GET_BASIS_SUPERBLOCK: states ordered by the LLLR quantum numbers,
+// then LL, LR, RR, RL) falls into sectors of one LLLR quantum number, and a sector holds
+// every (LL, LR) state of that number combined with every (RR, RL) state of the
+// complementary one, LLLR-major.  A sector's part of a vector is therefore a dense row-major
+// da x db matrix X (da LLLR states, db RRRL states).  A term of the list that acts only on
+// LL and LR (kinds LL, LR, LLLR) keeps the sector and acts on the rows of X; one that acts
+// only on RR and RL (RR, RL, RRRL) acts on its columns.  Summed per sector they are dense
+// matrices M_L (da x da) and M_R (db x db), built once from the term list with the element
+// makers' rules (operator rows of the row state, sign rules of the row's parities,
+// contributions with |value| <= 1e-15 or outside the superblock basis dropped), and
+//     (H x)_sector = M_L X + X M_R^T + (coupling terms)
+// where the coupling terms (LLRL, LLRR, LRRL, LRRR: one operator on each half) are applied
+// row by row from the term list through the superblock inverse, as the matrix-free product of
+// dmrg/DMRG_MATRIX_VECTOR_PRODUCT_OBC.c:18-83 applies its M_LRRL.
+//
+// A product is two kernels: the coupling terms write z row by row, then one LDS-tiled FP64
+// kernel per 64 x 64 tile of every sector adds M_L X + X M_R^T.  The sums are in an order of
+// their own (the CRS sums every term's contributions to an entry first), so results agree
+// with the CRS product to rounding.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "superblock.h"
+
+#pragma clang fp contract(off)
+
+namespace dmrgx {
+namespace {
+
+constexpr int kTile = 64;   // output tile of the dense products
+constexpr int kKt = 16;     // k-slab staged in LDS
+
+struct KronBuild {
+  int32_t d, nstate_a, nstate_b;
+  const int16_t *aLL, *aLR, *bRR, *bRL;   // flat state lists (sector order)
+  const int32_t *asec, *bsec;             // sector of each flat state
+  const int32_t *aidx, *bidx;             // (LL*d+LR) -> flat a-state, (RR*d+RL) -> flat b-state, -1 if none
+  const KronSector *sec;
+};
+
+// One thread per LLLR state: its row of M_L, from the terms acting on LL and LR only, in
+// term order (op1 row outer, op2 row inner, as DMRG_MAKE_ELEM_LLLR_LLLRRRRL.c:54-79).
+__global__ __launch_bounds__(256) void k_kron_left(KronBuild kb, const DTerm *terms, const int32_t *ids, int nt,
+                                                   const DOp *ops, const int32_t *const *eles, double *ML) {
+  const int32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= kb.nstate_a) return;
+  const int s = kb.asec[i];
+  const KronSector S = kb.sec[s];
+  const int32_t ap = i - S.a0;
+  double *row = ML + S.ml + (int64_t)ap * S.da;
+  const int q[4] = {kb.aLL[i], kb.aLR[i], 0, 0};
+  for (int u = 0; u < nt; u++) {
+    const DTerm t = terms[ids[u]];
+    const DOp a = ops[t.op1];
+    const double sc = (double)term_sign(t, eles, q) * t.coeff;
+    const int64_t b1 = a.row[q[t.s1]], e1 = a.row[q[t.s1] + 1];
+    for (int64_t i1 = b1; i1 < e1; i1++) {
+      int qq[2] = {q[0], q[1]};
+      qq[t.s1] = a.col[i1];
+      const double p1 = sc * a.val[i1];
+      if (t.s2 < 0) {
+        const int32_t c = kb.aidx[qq[0] * kb.d + qq[1]];
+        if (c >= 0 && kb.asec[c] == s && fabs(p1) > 1.0e-15) row[c - S.a0] += p1;
+        continue;
+      }
+      const DOp b = ops[t.op2];
+      const int64_t b2 = b.row[q[t.s2]], e2 = b.row[q[t.s2] + 1];
+      for (int64_t i2 = b2; i2 < e2; i2++) {
+        qq[t.s2] = b.col[i2];
+        const double p = p1 * b.val[i2];
+        const int32_t c = kb.aidx[qq[0] * kb.d + qq[1]];
+        if (c >= 0 && kb.asec[c] == s && fabs(p) > 1.0e-15) row[c - S.a0] += p;
+      }
+    }
+  }
+}
+
+// One thread per RRRL state: its row of M_R, from the terms acting on RR and RL only.
+__global__ __launch_bounds__(256) void k_kron_right(KronBuild kb, const DTerm *terms, const int32_t *ids, int nt,
+                                                    const DOp *ops, const int32_t *const *eles, double *MR) {
+  const int32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= kb.nstate_b) return;
+  const int s = kb.bsec[i];
+  const KronSector S = kb.sec[s];
+  const int32_t bp = i - S.b0;
+  double *row = MR + S.mr + (int64_t)bp * S.db;
+  const int q[4] = {0, 0, kb.bRR[i], kb.bRL[i]};
+  for (int u = 0; u < nt; u++) {
+    const DTerm t = terms[ids[u]];
+    const DOp a = ops[t.op1];
+    const double sc = (double)term_sign(t, eles, q) * t.coeff;
+    const int64_t b1 = a.row[q[t.s1]], e1 = a.row[q[t.s1] + 1];
+    for (int64_t i1 = b1; i1 < e1; i1++) {
+      int qq[4] = {q[0], q[1], q[2], q[3]};
+      qq[t.s1] = a.col[i1];
+      const double p1 = sc * a.val[i1];
+      if (t.s2 < 0) {
+        const int32_t c = kb.bidx[qq[2] * kb.d + qq[3]];
+        if (c >= 0 && kb.bsec[c] == s && fabs(p1) > 1.0e-15) row[c - S.b0] += p1;
+        continue;
+      }
+      const DOp b = ops[t.op2];
+      const int64_t b2 = b.row[q[t.s2]], e2 = b.row[q[t.s2] + 1];
+      for (int64_t i2 = b2; i2 < e2; i2++) {
+        qq[t.s2] = b.col[i2];
+        const double p = p1 * b.val[i2];
+        const int32_t c = kb.bidx[qq[2] * kb.d + qq[3]];
+        if (c >= 0 && kb.bsec[c] == s && fabs(p) > 1.0e-15) row[c - S.b0] += p;
+      }
+    }
+  }
+}
+
+// z = coupling terms applied to x, one thread per superblock row (the element makers'
+// enumeration, the superblock inverse for the column, x gathered).
+__global__ __launch_bounds__(256) void k_kron_cross(SbView sb, const int32_t *ids, int nt, const double *__restrict__ x,
+                                                    double *__restrict__ z) {
+  const int32_t r = blockIdx.x * 256 + threadIdx.x;
+  if (r >= sb.n) return;
+  const int q[4] = {sb.LL[r], sb.LR[r], sb.RR[r], sb.RL[r]};
+  double s = 0.0;
+  for (int u = 0; u < nt; u++) {
+    const DTerm t = sb.terms[ids[u]];
+    const DOp a = sb.ops[t.op1], b = sb.ops[t.op2];
+    const double sc = (double)term_sign(t, sb.eles, q) * t.coeff;
+    const int64_t b1 = a.row[q[t.s1]], e1 = a.row[q[t.s1] + 1];
+    const int64_t b2 = b.row[q[t.s2]], e2 = b.row[q[t.s2] + 1];
+    for (int64_t i1 = b1; i1 < e1; i1++) {
+      int qq[4] = {q[0], q[1], q[2], q[3]};
+      qq[t.s1] = a.col[i1];
+      const double p1 = sc * a.val[i1];
+      for (int64_t i2 = b2; i2 < e2; i2++) {
+        qq[t.s2] = b.col[i2];
+        const double p = p1 * b.val[i2];
+        const int32_t c = sb.inv[sb_index(qq[0], qq[1], qq[2], qq[3], sb.dim_RR, sb.d)];
+        if (c >= 0 && fabs(p) > 1.0e-15) s += p * x[c];
+      }
+    }
+  }
+  z[r] = s;
+}
+
+// One 64 x 64 tile of a sector's Z = M_L X + X M_R^T, added to z (which holds the coupling
+// terms).  256 threads, each a 4 x 4 block of outputs (rows ty + 16 i, columns tx + 16 j);
+// k-slabs of 16 staged in LDS, zero-padded past the sector's edges.
+__global__ __launch_bounds__(256) void k_kron_tile(const KronSector *sec, const int32_t *tiles,
+                                                   const double *__restrict__ ML, const double *__restrict__ MR,
+                                                   const double *__restrict__ x, double *__restrict__ z) {
+  __shared__ double As[kKt][kTile + 1];   // As[k][row]
+  __shared__ double Bs[kKt][kTile + 1];   // Bs[k][col]
+  const int32_t *tl = tiles + 3 * (int64_t)blockIdx.x;
+  const KronSector S = sec[tl[0]];
+  const int32_t r0 = tl[1] * kTile, c0 = tl[2] * kTile;
+  const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;
+  const double *X = x + S.off;
+  const double *L = ML + S.ml;
+  const double *Rm = MR + S.mr;
+  double acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+#pragma unroll
+    for (int j = 0; j < 4; j++) acc[i][j] = 0.0;
+  // M_L X: As[k][i] = M_L[r0 + i][k0 + k], Bs[k][j] = X[k0 + k][c0 + j]
+  for (int32_t k0 = 0; k0 < S.da; k0 += kKt) {
+    for (int e = threadIdx.x; e < kKt * kTile; e += 256) {
+      const int i = e / kKt, k = e % kKt;   // consecutive threads: consecutive k of one row of M_L
+      const int32_t rr = r0 + i, kk = k0 + k;
+      As[k][i] = (rr < S.da && kk < S.da) ? L[(int64_t)rr * S.da + kk] : 0.0;
+      const int k2 = e / kTile, j = e % kTile;   // consecutive threads: consecutive columns of X
+      const int32_t kk2 = k0 + k2, cc = c0 + j;
+      Bs[k2][j] = (kk2 < S.da && cc < S.db) ? X[(int64_t)kk2 * S.db + cc] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kKt; k++) {
+      double a[4], b[4];
+#pragma unroll
+      for (int i = 0; i < 4; i++) a[i] = As[k][ty + 16 * i];
+#pragma unroll
+      for (int j = 0; j < 4; j++) b[j] = Bs[k][tx + 16 * j];
+#pragma unroll
+      for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) acc[i][j] = fma(a[i], b[j], acc[i][j]);
+    }
+    __syncthreads();
+  }
+  // X M_R^T: As[k][i] = X[r0 + i][k0 + k], Bs[k][j] = M_R[c0 + j][k0 + k]
+  for (int32_t k0 = 0; k0 < S.db; k0 += kKt) {
+    for (int e = threadIdx.x; e < kKt * kTile; e += 256) {
+      const int i = e / kKt, k = e % kKt;
+      const int32_t rr = r0 + i, kk = k0 + k, cc = c0 + i;
+      As[k][i] = (rr < S.da && kk < S.db) ? X[(int64_t)rr * S.db + kk] : 0.0;
+      Bs[k][i] = (cc < S.db && kk < S.db) ? Rm[(int64_t)cc * S.db + kk] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kKt; k++) {
+      double a[4], b[4];
+#pragma unroll
+      for (int i = 0; i < 4; i++) a[i] = As[k][ty + 16 * i];
+#pragma unroll
+      for (int j = 0; j < 4; j++) b[j] = Bs[k][tx + 16 * j];
+#pragma unroll
+      for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) acc[i][j] = fma(a[i], b[j], acc[i][j]);
+    }
+    __syncthreads();
+  }
+  double *Z = z + S.off;
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    const int32_t rr = r0 + ty + 16 * i;
+    if (rr >= S.da) continue;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const int32_t cc = c0 + tx + 16 * j;
+      if (cc < S.db) {
+        const int64_t at = (int64_t)rr * S.db + cc;
+        Z[at] = Z[at] + acc[i][j];
+      }
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_zero(int64_t n, double *p) {
+  for (int64_t i = blockIdx.x * 256ll + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) p[i] = 0.0;
+}
+
+bool left_kind(int k) { return k == K_LL || k == K_LR || k == K_LLLR; }
+bool right_kind(int k) { return k == K_RR || k == K_RL || k == K_RRRL; }
+
+}  // namespace
+
+void KronDev::product(const double *x, hipStream_t s) const {
+  if (z.count < (size_t)n) const_cast<DevBuf<double> &>(z).resize(n);
+  const SbView v = sb->view();
+  k_kron_cross<<<(n + 255) / 256, 256, 0, s>>>(v, cross.p, ncross, x, z.p);
+  if (ntiles > 0) k_kron_tile<<<ntiles, 256, 0, s>>>(sec.p, tiles.p, ML.p, MR.p, x, z.p);
+  HIPCHK(hipGetLastError());
+}
+
+}  // namespace dmrgx
+
+using namespace dmrgx;
+
+struct dmrgx_kron {
+  KronDev K;
+};
+
+namespace dmrgx {
+const KronDev &kron_dev(const dmrgx_kron *K) { return K->K; }
+}  // namespace dmrgx
+
+extern "C" {
+
+int dmrgx_superblock_kron(const dmrgx_superblock *S, dmrgx_kron **out) {
+  if (!S || !out) return DMRGX_ERR_ARG;
+  *out = nullptr;
+  return run_status([&] {
+    auto H = std::make_unique<dmrgx_kron>();
+    KronDev &K = H->K;
+    const int32_t n = S->n, d = S->d;
+    K.n = n;
+    K.sb = S;
+    hipStream_t st = ctx().stream;
+    // the basis (host copy) and its sectors
+    std::vector<int16_t> LL(n), LR(n), RR(n), RL(n);
+    ctx().copy_out(LL.data(), S->LL.p, sizeof(int16_t) * n);
+    ctx().copy_out(LR.data(), S->LR.p, sizeof(int16_t) * n);
+    ctx().copy_out(RR.data(), S->RR.p, sizeof(int16_t) * n);
+    ctx().copy_out(RL.data(), S->RL.p, sizeof(int16_t) * n);
+    std::vector<int16_t> aLL, aLR, bRR, bRL;
+    std::vector<int32_t> asec, bsec;
+    int64_t r = 0;
+    while (r < n) {
+      KronSector sc{};
+      sc.off = r;
+      // db: rows until (LL, LR) changes; da: runs of db rows with the same RRRL sequence
+      int64_t e = r;
+      while (e < n && LL[e] == LL[r] && LR[e] == LR[r]) e++;
+      sc.db = (int32_t)(e - r);
+      int64_t a = r;
+      while (a < n && a + sc.db <= n && RR[a] == RR[r] && RL[a] == RL[r]) a += sc.db;
+      sc.da = (int32_t)((a - r) / sc.db);
+      // every row of the sector must be (LLLR state i, RRRL state j) at off + i db + j
+      for (int32_t i = 0; i < sc.da; i++)
+        for (int32_t j = 0; j < sc.db; j++) {
+          const int64_t q = r + (int64_t)i * sc.db + j;
+          if (LL[q] != LL[r + (int64_t)i * sc.db] || LR[q] != LR[r + (int64_t)i * sc.db] || RR[q] != RR[r + j] ||
+              RL[q] != RL[r + j])
+            throw Failure(DMRGX_ERR_ARG, "superblock basis is not a product of LLLR and RRRL states per sector");
+        }
+      sc.a0 = (int32_t)aLL.size();
+      sc.b0 = (int32_t)bRR.size();
+      for (int32_t i = 0; i < sc.da; i++) {
+        aLL.push_back(LL[r + (int64_t)i * sc.db]);
+        aLR.push_back(LR[r + (int64_t)i * sc.db]);
+        asec.push_back(K.nsec);
+      }
+      for (int32_t j = 0; j < sc.db; j++) {
+        bRR.push_back(RR[r + j]);
+        bRL.push_back(RL[r + j]);
+        bsec.push_back(K.nsec);
+      }
+      K.hsec.push_back(sc);
+      K.nsec++;
+      r += (int64_t)sc.da * sc.db;
+    }
+    // state -> flat index maps (each LLLR / RRRL state belongs to one sector)
+    std::vector<int32_t> aidx((size_t)S->dim_LL * d, -1), bidx((size_t)S->dim_RR * d, -1);
+    for (size_t i = 0; i < aLL.size(); i++) {
+      int32_t &m = aidx[(size_t)aLL[i] * d + aLR[i]];
+      if (m >= 0) throw Failure(DMRGX_ERR_ARG, "an LLLR state appears in two sectors");
+      m = (int32_t)i;
+    }
+    for (size_t i = 0; i < bRR.size(); i++) {
+      int32_t &m = bidx[(size_t)bRR[i] * d + bRL[i]];
+      if (m >= 0) throw Failure(DMRGX_ERR_ARG, "an RRRL state appears in two sectors");
+      m = (int32_t)i;
+    }
+    K.na = (int32_t)aLL.size();
+    K.nb = (int32_t)bRR.size();
+    int64_t ml = 0, mr = 0;
+    std::vector<int32_t> tiles;
+    for (int s = 0; s < K.nsec; s++) {
+      KronSector &sc = K.hsec[s];
+      sc.ml = ml;
+      sc.mr = mr;
+      ml += (int64_t)sc.da * sc.da;
+      mr += (int64_t)sc.db * sc.db;
+      K.flops += 2.0 * sc.da * sc.db * (double)(sc.da + sc.db);
+      for (int32_t ti = 0; ti * kTile < sc.da; ti++)
+        for (int32_t tj = 0; tj * kTile < sc.db; tj++) {
+          tiles.push_back(s);
+          tiles.push_back(ti);
+          tiles.push_back(tj);
+        }
+    }
+    K.ntiles = (int32_t)(tiles.size() / 3);
+    // the term list split by the halves it acts on
+    std::vector<int32_t> left, right, cross;
+    for (size_t i = 0; i < S->hterms.size(); i++) {
+      const int k = S->hterms[i].kind;
+      if (left_kind(k)) left.push_back((int32_t)i);
+      else if (right_kind(k)) right.push_back((int32_t)i);
+      else cross.push_back((int32_t)i);
+    }
+    K.ncross = (int32_t)cross.size();
+    auto up = [](auto &buf, const auto &h) {
+      using T = typename std::decay_t<decltype(h)>::value_type;
+      buf.resize(std::max<size_t>(h.size(), 1));
+      if (!h.empty()) ctx().copy_in(buf.p, h.data(), sizeof(T) * h.size());
+    };
+    up(K.sec, K.hsec);
+    up(K.tiles, tiles);
+    up(K.cross, cross);
+    DevBuf<int16_t> daLL, daLR, dbRR, dbRL;
+    DevBuf<int32_t> dasec, dbsec, daidx, dbidx, dleft, dright;
+    up(daLL, aLL);
+    up(daLR, aLR);
+    up(dbRR, bRR);
+    up(dbRL, bRL);
+    up(dasec, asec);
+    up(dbsec, bsec);
+    up(daidx, aidx);
+    up(dbidx, bidx);
+    up(dleft, left);
+    up(dright, right);
+    K.ML.resize(std::max<int64_t>(ml, 1));
+    K.MR.resize(std::max<int64_t>(mr, 1));
+    k_zero<<<1024, 256, 0, st>>>(ml, K.ML.p);
+    k_zero<<<1024, 256, 0, st>>>(mr, K.MR.p);
+    const KronBuild kb{d, K.na, K.nb, daLL.p, daLR.p, dbRR.p, dbRL.p, dasec.p, dbsec.p, daidx.p, dbidx.p, K.sec.p};
+    if (K.na > 0)
+      k_kron_left<<<(K.na + 255) / 256, 256, 0, st>>>(kb, S->terms.p, dleft.p, (int)left.size(), S->ops.p,
+                                                      S->ele_ptrs.p, K.ML.p);
+    if (K.nb > 0)
+      k_kron_right<<<(K.nb + 255) / 256, 256, 0, st>>>(kb, S->terms.p, dright.p, (int)right.size(), S->ops.p,
+                                                       S->ele_ptrs.p, K.MR.p);
+    HIPCHK(hipGetLastError());
+    K.z.resize(std::max<int32_t>(n, 1));
+    HIPCHK(hipStreamSynchronize(st));   // the build's own buffers go out of scope
+    *out = H.release();
+  });
+}
+
+int dmrgx_kron_info(const dmrgx_kron *K, int32_t *n, int32_t *sectors, int32_t *coupling_terms, double *flops) {
+  if (!K) return DMRGX_ERR_ARG;
+  if (n) *n = K->K.n;
+  if (sectors) *sectors = K->K.nsec;
+  if (coupling_terms) *coupling_terms = K->K.ncross;
+  if (flops) *flops = K->K.flops;
+  return DMRGX_OK;
+}
+
+int dmrgx_kron_spmv(const dmrgx_kron *K, const double *x, double *y) {
+  if (!K || !x || !y) return DMRGX_ERR_ARG;
+  return run_status([&] {
+    const int32_t n = K->K.n;
+    DevBuf<double> dx(n);
+    ctx().copy_in(dx.p, x, sizeof(double) * n);
+    K->K.product(dx.p, ctx().stream);
+    ctx().copy_out(y, K->K.z.p, sizeof(double) * n);
+  });
+}
+
+void dmrgx_kron_free(dmrgx_kron *K) {
+  if (!K) return;
+  (void)hipStreamSynchronize(ctx().stream);
+  delete K;
+}
+
+}  // extern "C"

@@ -643,6 +643,31 @@ int dmrgx_superblock_n(const dmrgx_superblock *S, int32_t *n);
 int dmrgx_superblock_assemble(dmrgx_superblock *S, int64_t *nnz_lower, dmrgx_matrix **full_out);
 int dmrgx_superblock_download_lower(const dmrgx_superblock *S, int64_t *row, int32_t *col, double *val);
 void dmrgx_superblock_free(dmrgx_superblock *S);
+
+/* ---- Kronecker form of a superblock Hamiltonian: the operator of the CRS that
+ * dmrgx_superblock_assemble builds (GET_HAM_LLLRRRRL, main/dmrg/<model>_vf/src/allocations/
+ * GET_HAM_LLLRRRRL.c:11-114), applied without assembling it.  No reference symbol has this
+ * form; it applies the superblock operator as the reference's matrix-free product does
+ * (dmrg/DMRG_MATRIX_VECTOR_PRODUCT_OBC.c:18-83: sub-Hamiltonians of the left and right
+ * halves plus the terms that couple them), built from the recorded term list instead of
+ * the program's GET_HAM_LLLR/RRRL.  Per sector of the superblock basis (one LLLR quantum
+ * number: a dense block of LLLR by RRRL states) the LL/LR/LLLR terms form a dense matrix
+ * acting on the block's rows and the RR/RL/RRRL terms one acting on its columns; the
+ * coupling terms are applied row by row through the superblock inverse.  Results agree with
+ * the CRS product to rounding (the sums are in an order of their own).  The superblock S
+ * must outlive the handle.  DMRGX_ERR_ARG if the basis is not a product of LLLR and RRRL
+ * states in every sector (a basis from GET_BASIS_SUPERBLOCK always is). */
+typedef struct dmrgx_kron dmrgx_kron;
+int dmrgx_superblock_kron(const dmrgx_superblock *S, dmrgx_kron **out);
+/* dimension, sectors, coupling terms, floating-point operations of the dense products per
+ * product (any pointer may be null) */
+int dmrgx_kron_info(const dmrgx_kron *K, int32_t *n, int32_t *sectors, int32_t *coupling_terms, double *flops);
+/* y = H x (host arrays of length n) */
+int dmrgx_kron_spmv(const dmrgx_kron *K, const double *x, double *y);
+/* dmrgx_ground_state on the Kronecker form (precision 0 only) */
+int dmrgx_kron_ground_state(const dmrgx_kron *K, const dmrgx_solve_params *p, double *vec_inout,
+                            dmrgx_solve_stats *stats);
+void dmrgx_kron_free(dmrgx_kron *K);
 /* Drop-in body of GET_HAM_LLLRRRRL (main/dmrg/<model>_vf/src/allocations/GET_HAM_LLLRRRRL.c:11):
  * assembles on the device, returns a malloc'ed CRS1 (free with FREE_CRS1) holding the
  * reference-layout lower triangle, and keeps the expanded device copy for the next

@@ -30,7 +30,8 @@ def test_gpu_schedule_argument_checks():
     import dmrg_c_amd as D
     A, _ = _matrix()
     try:
-        for bad in ((0, 0, 0), (3, 0, 0), (128, 0, 0), (8, -1, 0), (8, 0, 24), (8, 0, 32), (2, 0, 40), (8, 0, -1)):
+        for bad in ((0, 0, 0), (3, 0, 0), (128, 0, 0), (8, -1, 0), (8, 0, 24), (8, 0, 32), (2, 0, 40), (8, 0, -1),
+                    (8, 0, 64), (2, 0, 64 | 33), (2, 0, 64 | 40), (2, 0, 128)):
             with pytest.raises(D.DmrgxError):
                 A.set_schedule(*bad)
         A.set_schedule(16, 2048, 23)
@@ -76,6 +77,24 @@ def test_gpu_load_variants_bit_identical():
 
 
 @pytest.mark.gpu
+def test_gpu_sliced_loads_ahead_bit_identical():
+    """Bit 6 (matrix loads one pass group ahead of the x gathers) changes no arithmetic:
+    the sliced form with 16-bit columns gives the same product and solves with and
+    without it, plain or non-temporal, with any lanes per row and row mapping."""
+    A, e_ref = _matrix()
+    try:
+        x = np.random.default_rng(4).standard_normal(A.n)
+        for g in (1, 2, 4):
+            for v in (25, 27, 29, 31):
+                for cap in (4096, 0):
+                    ref = _solves(A, (g, cap, v), x)
+                    _same(ref, _solves(A, (g, cap, v | 64), x))
+            assert abs(ref[1][0] - e_ref) <= 1e-10 * abs(e_ref)
+    finally:
+        A.free()
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("sched", [(1, 0, 0), (4, 1024, 5), (16, 0, 10), (32, 8192, 15), (64, 2048, 12), (8, 16384, 6), (2, 0, 18), (8, 2048, 23),
                                    (1, 0, 24), (2, 4096, 24), (4, 8192, 28), (2, 16384, 31)])
 def test_gpu_schedules_agree(sched):
@@ -100,8 +119,9 @@ def test_gpu_autotune_sets_schedule():
         sched, us = A.autotune()
         assert A.schedule == sched and us > 0
         g, cap, v = sched
-        assert g & (g - 1) == 0 and 1 <= g <= 64 and cap >= 0 and v in range(40)
-        assert (v >> 3) < 3 or g <= 4
+        assert g & (g - 1) == 0 and 1 <= g <= 64 and cap >= 0 and (v & ~64) in range(40)
+        assert ((v >> 3) & 7) < 3 or g <= 4
+        assert not v & 64 or ((v >> 3) & 7) == 3   # loads ahead: sliced form only
         E, _, _ = A.ground_state()
         assert abs(E - e_ref) <= 1e-10 * abs(e_ref)
     finally:

@@ -27,7 +27,8 @@ import oracle_py
 from conftest import GOLDEN, REPO
 
 QN_FILES = sorted(glob.glob(os.path.join(GOLDEN, "qn_*.npz")))
-SMALL = [f for f in QN_FILES if "m400" not in f]
+# the production-size steps (m=400, m=800) run on the GPU only (the m=800 one in test_klm_m800.py)
+SMALL = [f for f in QN_FILES if "m400" not in f and "m800" not in f]
 REF_LIB = os.path.join(REPO, "oracle", "_ref", "libdmrgref.so")
 SLOTS = ("LL", "LR", "RR", "RL")
 
