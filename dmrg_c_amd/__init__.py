@@ -715,6 +715,7 @@ class KronOperator:
         L.dmrgx_kron_info.argtypes = [vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int32),
                                       C.POINTER(C.c_double)]
         L.dmrgx_kron_spmv.argtypes = [vp, vp, vp]
+        L.dmrgx_kron_set_matrix_cores.argtypes = [vp, C.c_int32]
         L.dmrgx_kron_ground_state.argtypes = [vp, C.POINTER(SolveParams), vp, C.POINTER(SolveStats)]
         L.dmrgx_kron_free.argtypes = [vp]
         L.dmrgx_kron_free.restype = None
@@ -725,6 +726,9 @@ class KronOperator:
         n, ns, nc, fl = C.c_int32(), C.c_int32(), C.c_int32(), C.c_double()
         check(L.dmrgx_kron_info(h, C.byref(n), C.byref(ns), C.byref(nc), C.byref(fl)))
         self.n, self.sectors, self.coupling_terms, self.flops = n.value, ns.value, nc.value, fl.value
+
+    def set_matrix_cores(self, on: bool):
+        check(lib().dmrgx_kron_set_matrix_cores(self.handle, 1 if on else 0))
 
     def matvec(self, x: np.ndarray) -> np.ndarray:
         xx = np.ascontiguousarray(x, np.float64)

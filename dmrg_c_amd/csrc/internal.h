@@ -843,6 +843,7 @@ struct KronDev {
   DevBuf<int32_t> cross;                  // indices of the coupling terms in the term list
   mutable DevBuf<double> z;               // H x of the last product
   double flops = 0.0;                     // of the two dense products per H x
+  bool mfma = true;                       // dense products on the FP64 matrix cores
   // z = H x on stream s
   void product(const double *x, hipStream_t s) const;
 };

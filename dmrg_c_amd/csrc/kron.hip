@@ -231,6 +231,83 @@ __global__ __launch_bounds__(256) void k_kron_tile(const KronSector *sec, const 
   }
 }
 
+// k_kron_tile on the FP64 matrix cores: each of the four wavefronts computes a 32 x 32
+// quarter of the 64 x 64 tile as 2 x 2 blocks of v_mfma_f64_16x16x4 (A fragment: lane l
+// holds A[l % 16][l / 16]; B fragment: B[l / 16][l % 16]; result register v of lane l:
+// D[4 v + l / 16][l % 16], as a one-hot scan of the instruction on gfx950 shows), k-slabs of
+// 16 staged in LDS as in k_kron_tile.
+typedef double kron_v4d __attribute__((ext_vector_type(4)));
+__global__ __launch_bounds__(256) void k_kron_tile_mfma(const KronSector *sec, const int32_t *tiles,
+                                                        const double *__restrict__ ML, const double *__restrict__ MR,
+                                                        const double *__restrict__ x, double *__restrict__ z) {
+  __shared__ double As[kKt][kTile + 1];   // As[k][row]
+  __shared__ double Bs[kKt][kTile + 1];   // Bs[k][col]
+  const int32_t *tl = tiles + 3 * (int64_t)blockIdx.x;
+  const KronSector S = sec[tl[0]];
+  const int32_t r0 = tl[1] * kTile, c0 = tl[2] * kTile;
+  const int wave = threadIdx.x / 64, l = threadIdx.x % 64;
+  const int wr = (wave / 2) * 32, wc = (wave % 2) * 32;   // this wavefront's quarter
+  const int li = l % 16, lk = l / 16;
+  const double *X = x + S.off;
+  const double *L = ML + S.ml;
+  const double *Rm = MR + S.mr;
+  kron_v4d acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; i++)
+#pragma unroll
+    for (int j = 0; j < 2; j++) acc[i][j] = kron_v4d{0.0, 0.0, 0.0, 0.0};
+  auto slab = [&]() {
+#pragma unroll
+    for (int kk = 0; kk < kKt; kk += 4) {
+      const double a0 = As[kk + lk][wr + li], a1 = As[kk + lk][wr + 16 + li];
+      const double b0 = Bs[kk + lk][wc + li], b1 = Bs[kk + lk][wc + 16 + li];
+      acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
+      acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
+      acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
+      acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+    }
+  };
+  // M_L X: As[k][i] = M_L[r0 + i][k0 + k], Bs[k][j] = X[k0 + k][c0 + j]
+  for (int32_t k0 = 0; k0 < S.da; k0 += kKt) {
+    for (int e = threadIdx.x; e < kKt * kTile; e += 256) {
+      const int i = e / kKt, k = e % kKt;
+      const int32_t rr = r0 + i, kk = k0 + k;
+      As[k][i] = (rr < S.da && kk < S.da) ? L[(int64_t)rr * S.da + kk] : 0.0;
+      const int k2 = e / kTile, j = e % kTile;
+      const int32_t kk2 = k0 + k2, cc = c0 + j;
+      Bs[k2][j] = (kk2 < S.da && cc < S.db) ? X[(int64_t)kk2 * S.db + cc] : 0.0;
+    }
+    __syncthreads();
+    slab();
+    __syncthreads();
+  }
+  // X M_R^T: As[k][i] = X[r0 + i][k0 + k], Bs[k][j] = M_R[c0 + j][k0 + k]
+  for (int32_t k0 = 0; k0 < S.db; k0 += kKt) {
+    for (int e = threadIdx.x; e < kKt * kTile; e += 256) {
+      const int i = e / kKt, k = e % kKt;
+      const int32_t rr = r0 + i, kk = k0 + k, cc = c0 + i;
+      As[k][i] = (rr < S.da && kk < S.db) ? X[(int64_t)rr * S.db + kk] : 0.0;
+      Bs[k][i] = (cc < S.db && kk < S.db) ? Rm[(int64_t)cc * S.db + kk] : 0.0;
+    }
+    __syncthreads();
+    slab();
+    __syncthreads();
+  }
+  double *Z = z + S.off;
+#pragma unroll
+  for (int bi = 0; bi < 2; bi++)
+#pragma unroll
+    for (int bj = 0; bj < 2; bj++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int32_t rr = r0 + wr + 16 * bi + 4 * v + lk, cc = c0 + wc + 16 * bj + li;
+        if (rr < S.da && cc < S.db) {
+          const int64_t at = (int64_t)rr * S.db + cc;
+          Z[at] = Z[at] + acc[bi][bj][v];
+        }
+      }
+}
+
 __global__ __launch_bounds__(256) void k_zero(int64_t n, double *p) {
   for (int64_t i = blockIdx.x * 256ll + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) p[i] = 0.0;
 }
@@ -244,7 +321,10 @@ void KronDev::product(const double *x, hipStream_t s) const {
   if (z.count < (size_t)n) const_cast<DevBuf<double> &>(z).resize(n);
   const SbView v = sb->view();
   k_kron_cross<<<(n + 255) / 256, 256, 0, s>>>(v, cross.p, ncross, x, z.p);
-  if (ntiles > 0) k_kron_tile<<<ntiles, 256, 0, s>>>(sec.p, tiles.p, ML.p, MR.p, x, z.p);
+  if (ntiles > 0) {
+    if (mfma) k_kron_tile_mfma<<<ntiles, 256, 0, s>>>(sec.p, tiles.p, ML.p, MR.p, x, z.p);
+    else k_kron_tile<<<ntiles, 256, 0, s>>>(sec.p, tiles.p, ML.p, MR.p, x, z.p);
+  }
   HIPCHK(hipGetLastError());
 }
 
@@ -399,6 +479,12 @@ int dmrgx_kron_info(const dmrgx_kron *K, int32_t *n, int32_t *sectors, int32_t *
   if (sectors) *sectors = K->K.nsec;
   if (coupling_terms) *coupling_terms = K->K.ncross;
   if (flops) *flops = K->K.flops;
+  return DMRGX_OK;
+}
+
+int dmrgx_kron_set_matrix_cores(dmrgx_kron *K, int32_t on) {
+  if (!K || (on != 0 && on != 1)) return DMRGX_ERR_ARG;
+  K->K.mfma = on != 0;
   return DMRGX_OK;
 }
 

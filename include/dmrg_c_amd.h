@@ -662,6 +662,8 @@ int dmrgx_superblock_kron(const dmrgx_superblock *S, dmrgx_kron **out);
 /* dimension, sectors, coupling terms, floating-point operations of the dense products per
  * product (any pointer may be null) */
 int dmrgx_kron_info(const dmrgx_kron *K, int32_t *n, int32_t *sectors, int32_t *coupling_terms, double *flops);
+/* dense products on the FP64 matrix cores (1, the default) or on the vector units (0) */
+int dmrgx_kron_set_matrix_cores(dmrgx_kron *K, int32_t on);
 /* y = H x (host arrays of length n) */
 int dmrgx_kron_spmv(const dmrgx_kron *K, const double *x, double *y);
 /* dmrgx_ground_state on the Kronecker form (precision 0 only) */

@@ -24,13 +24,15 @@ def _ref_full(s):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("cores", [True, False], ids=["mfma", "vector"])
 @pytest.mark.parametrize("path", SNAPS + BIG, ids=golden_ids(SNAPS + BIG))
-def test_gpu_kron_product_equals_reference_crs(path):
+def test_gpu_kron_product_equals_reference_crs(path, cores):
     import dmrg_c_amd as D
     from dmrg_c_amd import snapshot
     s = snapshot.load(path)
     sb = D.Superblock(s)
     K = sb.kron()
+    K.set_matrix_cores(cores)
     try:
         x = np.random.default_rng(11).standard_normal(s.n)
         y = K.matvec(x)
