@@ -679,7 +679,10 @@ int ground_state_op(const Operator &op, const dmrgx_matrix *A, const dmrgx_solve
         const double one = 1.0;
         v.up(&one);
         op.kron->product(v.d.p, st);
+        double hl;
         HIPCHK(hipMemcpy(&h, op.kron->z.p, sizeof(double), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(&hl, op.kron->zl.p, sizeof(double), hipMemcpyDeviceToHost));
+        h = h + hl;
       }
       out.eig = h;
       const double one = 1.0;

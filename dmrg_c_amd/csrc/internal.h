@@ -839,9 +839,11 @@ struct KronDev {
   std::vector<KronSector> hsec;
   DevBuf<KronSector> sec;
   DevBuf<double> ML, MR;                  // dense sector blocks, row-major
-  DevBuf<int32_t> tiles;                  // (sector, tile row, tile column) of the product
+  DevBuf<int32_t> tiles;                  // (sector, tile row, tile column, left/right product)
   DevBuf<int32_t> cross;                  // indices of the coupling terms in the term list
-  mutable DevBuf<double> z;               // H x of the last product
+  // flat LLLR / RRRL state of (LL d + LR) / (RR d + RL) (-1: none) and each flat state's sector
+  DevBuf<int32_t> aidx, bidx, asec, bsec;
+  mutable DevBuf<double> z, zl;           // H x of the last product is z + zl
   double flops = 0.0;                     // of the two dense products per H x
   bool mfma = true;                       // dense products on the FP64 matrix cores
   // z = H x on stream s

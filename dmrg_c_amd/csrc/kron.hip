@@ -17,7 +17,8 @@
 // dmrg/DMRG_MATRIX_VECTOR_PRODUCT_OBC.c:18-83 applies its M_LRRL.
 //
 // A product is two kernels: the coupling terms write z row by row, then one LDS-tiled FP64
-// kernel per 64 x 64 tile of every sector adds M_L X + X M_R^T.  The sums are in an order of
+// kernel whose workgroups each take a 64 x 64 tile of M_L X (written to zl) or of X M_R^T
+// (added to z); the solvers' epilogues read z + zl.  The sums are in an order of
 // their own (the CRS sums every term's contributions to an entry first), so results agree
 // with the CRS product to rounding.
 
@@ -118,18 +119,32 @@ __global__ __launch_bounds__(256) void k_kron_right(KronBuild kb, const DTerm *t
   }
 }
 
-// z = coupling terms applied to x, one thread per superblock row (the element makers'
-// enumeration, the superblock inverse for the column, x gathered).
-__global__ __launch_bounds__(256) void k_kron_cross(SbView sb, const int32_t *ids, int nt, const double *__restrict__ x,
-                                                    double *__restrict__ z) {
+// z = coupling terms applied to x, one thread per superblock row: the element makers'
+// enumeration of each term's operator rows; the column state's superblock index from the
+// sector tables (flat LLLR and RRRL state of the column, both in one sector, then
+// off + a db + b) instead of the dense superblock inverse (a few kB of tables in the caches
+// against the inverse's dim_LL d dim_RR d entries); a state pair outside the superblock basis
+// is dropped, as the inverse's -1 is.
+struct KronCross {
+  int32_t n, d;
+  const int16_t *LL, *LR, *RR, *RL;
+  const int32_t *aidx, *bidx, *asec, *bsec;
+  const KronSector *sec;
+  const DTerm *terms;
+  const DOp *ops;
+  const int32_t *const *eles;
+  const int32_t *ids;
+  int nt;
+};
+__global__ __launch_bounds__(256) void k_kron_cross(KronCross kc, const double *__restrict__ x, double *__restrict__ z) {
   const int32_t r = blockIdx.x * 256 + threadIdx.x;
-  if (r >= sb.n) return;
-  const int q[4] = {sb.LL[r], sb.LR[r], sb.RR[r], sb.RL[r]};
+  if (r >= kc.n) return;
+  const int q[4] = {kc.LL[r], kc.LR[r], kc.RR[r], kc.RL[r]};
   double s = 0.0;
-  for (int u = 0; u < nt; u++) {
-    const DTerm t = sb.terms[ids[u]];
-    const DOp a = sb.ops[t.op1], b = sb.ops[t.op2];
-    const double sc = (double)term_sign(t, sb.eles, q) * t.coeff;
+  for (int u = 0; u < kc.nt; u++) {
+    const DTerm t = kc.terms[kc.ids[u]];
+    const DOp a = kc.ops[t.op1], b = kc.ops[t.op2];
+    const double sc = (double)term_sign(t, kc.eles, q) * t.coeff;
     const int64_t b1 = a.row[q[t.s1]], e1 = a.row[q[t.s1] + 1];
     const int64_t b2 = b.row[q[t.s2]], e2 = b.row[q[t.s2] + 1];
     for (int64_t i1 = b1; i1 < e1; i1++) {
@@ -139,173 +154,143 @@ __global__ __launch_bounds__(256) void k_kron_cross(SbView sb, const int32_t *id
       for (int64_t i2 = b2; i2 < e2; i2++) {
         qq[t.s2] = b.col[i2];
         const double p = p1 * b.val[i2];
-        const int32_t c = sb.inv[sb_index(qq[0], qq[1], qq[2], qq[3], sb.dim_RR, sb.d)];
-        if (c >= 0 && fabs(p) > 1.0e-15) s += p * x[c];
+        const int32_t fa = kc.aidx[qq[0] * kc.d + qq[1]], fb = kc.bidx[qq[2] * kc.d + qq[3]];
+        if (fa < 0 || fb < 0 || !(fabs(p) > 1.0e-15)) continue;
+        const int sa = kc.asec[fa];
+        if (kc.bsec[fb] != sa) continue;
+        const KronSector S = kc.sec[sa];
+        s += p * x[S.off + (int64_t)(fa - S.a0) * S.db + (fb - S.b0)];
       }
     }
   }
   z[r] = s;
 }
 
-// One 64 x 64 tile of a sector's Z = M_L X + X M_R^T, added to z (which holds the coupling
-// terms).  256 threads, each a 4 x 4 block of outputs (rows ty + 16 i, columns tx + 16 j);
-// k-slabs of 16 staged in LDS, zero-padded past the sector's edges.
+// One 64 x 64 tile of one of a sector's two dense products: the left one (M_L X, k over
+// the sector's LLLR states, written to zl) or the right one (X M_R^T, k over its RRRL
+// states, added to z, which holds the coupling terms).  k-slabs of kKt staged in LDS, double
+// buffered: the next slab's global loads are issued before the current slab's products.
+// Outside the sector the operands are zero.  MFMA: each of the four wavefronts computes a
+// 32 x 32 quarter as 2 x 2 blocks of v_mfma_f64_16x16x4 (A fragment: lane l holds
+// A[l % 16][l / 16]; B fragment: B[l / 16][l % 16]; result register v of lane l:
+// D[4 v + l / 16][l % 16], as a one-hot scan of the instruction on gfx950 shows).  Without:
+// each thread a 4 x 4 block of FP64 fused multiply-adds (rows ty + 16 i, columns tx + 16 j).
+typedef double kron_v4d __attribute__((ext_vector_type(4)));
+constexpr int kPer = kKt * kTile / 256;   // staged elements per thread per operand
+
+template <bool MFMA>
 __global__ __launch_bounds__(256) void k_kron_tile(const KronSector *sec, const int32_t *tiles,
                                                    const double *__restrict__ ML, const double *__restrict__ MR,
-                                                   const double *__restrict__ x, double *__restrict__ z) {
-  __shared__ double As[kKt][kTile + 1];   // As[k][row]
-  __shared__ double Bs[kKt][kTile + 1];   // Bs[k][col]
-  const int32_t *tl = tiles + 3 * (int64_t)blockIdx.x;
+                                                   const double *__restrict__ x, double *__restrict__ z,
+                                                   double *__restrict__ zl) {
+  __shared__ double As[2][kKt][kTile + 1];   // As[buffer][k][row]
+  __shared__ double Bs[2][kKt][kTile + 1];   // Bs[buffer][k][col]
+  const int32_t *tl = tiles + 4 * (int64_t)blockIdx.x;
   const KronSector S = sec[tl[0]];
   const int32_t r0 = tl[1] * kTile, c0 = tl[2] * kTile;
-  const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;
+  const bool right = tl[3] != 0;
+  const int32_t K = right ? S.db : S.da;
   const double *X = x + S.off;
-  const double *L = ML + S.ml;
-  const double *Rm = MR + S.mr;
-  double acc[4][4];
+  const double *Aop = right ? X : ML + S.ml;           // A[i][k] = Aop[(r0 + i) lda + k]
+  const int64_t lda = right ? S.db : S.da;
+  const double *Bop = right ? MR + S.mr : X;           // left: B[k][j] = X[k db + c0 + j]
+  double ra[kPer], rb[kPer];                           // right: B[k][j] = M_R[(c0 + j) db + k]
+  auto fetch = [&](int32_t k0) {
 #pragma unroll
-  for (int i = 0; i < 4; i++)
-#pragma unroll
-    for (int j = 0; j < 4; j++) acc[i][j] = 0.0;
-  // M_L X: As[k][i] = M_L[r0 + i][k0 + k], Bs[k][j] = X[k0 + k][c0 + j]
-  for (int32_t k0 = 0; k0 < S.da; k0 += kKt) {
-    for (int e = threadIdx.x; e < kKt * kTile; e += 256) {
-      const int i = e / kKt, k = e % kKt;   // consecutive threads: consecutive k of one row of M_L
+    for (int u = 0; u < kPer; u++) {
+      const int e = threadIdx.x + 256 * u;
+      const int i = e / kKt, k = e % kKt;              // consecutive threads: consecutive k of a row
       const int32_t rr = r0 + i, kk = k0 + k;
-      As[k][i] = (rr < S.da && kk < S.da) ? L[(int64_t)rr * S.da + kk] : 0.0;
-      const int k2 = e / kTile, j = e % kTile;   // consecutive threads: consecutive columns of X
-      const int32_t kk2 = k0 + k2, cc = c0 + j;
-      Bs[k2][j] = (kk2 < S.da && cc < S.db) ? X[(int64_t)kk2 * S.db + cc] : 0.0;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < kKt; k++) {
-      double a[4], b[4];
-#pragma unroll
-      for (int i = 0; i < 4; i++) a[i] = As[k][ty + 16 * i];
-#pragma unroll
-      for (int j = 0; j < 4; j++) b[j] = Bs[k][tx + 16 * j];
-#pragma unroll
-      for (int i = 0; i < 4; i++)
-#pragma unroll
-        for (int j = 0; j < 4; j++) acc[i][j] = fma(a[i], b[j], acc[i][j]);
-    }
-    __syncthreads();
-  }
-  // X M_R^T: As[k][i] = X[r0 + i][k0 + k], Bs[k][j] = M_R[c0 + j][k0 + k]
-  for (int32_t k0 = 0; k0 < S.db; k0 += kKt) {
-    for (int e = threadIdx.x; e < kKt * kTile; e += 256) {
-      const int i = e / kKt, k = e % kKt;
-      const int32_t rr = r0 + i, kk = k0 + k, cc = c0 + i;
-      As[k][i] = (rr < S.da && kk < S.db) ? X[(int64_t)rr * S.db + kk] : 0.0;
-      Bs[k][i] = (cc < S.db && kk < S.db) ? Rm[(int64_t)cc * S.db + kk] : 0.0;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < kKt; k++) {
-      double a[4], b[4];
-#pragma unroll
-      for (int i = 0; i < 4; i++) a[i] = As[k][ty + 16 * i];
-#pragma unroll
-      for (int j = 0; j < 4; j++) b[j] = Bs[k][tx + 16 * j];
-#pragma unroll
-      for (int i = 0; i < 4; i++)
-#pragma unroll
-        for (int j = 0; j < 4; j++) acc[i][j] = fma(a[i], b[j], acc[i][j]);
-    }
-    __syncthreads();
-  }
-  double *Z = z + S.off;
-#pragma unroll
-  for (int i = 0; i < 4; i++) {
-    const int32_t rr = r0 + ty + 16 * i;
-    if (rr >= S.da) continue;
-#pragma unroll
-    for (int j = 0; j < 4; j++) {
-      const int32_t cc = c0 + tx + 16 * j;
-      if (cc < S.db) {
-        const int64_t at = (int64_t)rr * S.db + cc;
-        Z[at] = Z[at] + acc[i][j];
+      ra[u] = (rr < S.da && kk < K) ? Aop[rr * lda + kk] : 0.0;
+      if (right) {
+        const int32_t cc = c0 + i;
+        rb[u] = (cc < S.db && kk < K) ? Bop[(int64_t)cc * S.db + kk] : 0.0;
+      } else {
+        const int k2 = e / kTile, j = e % kTile;       // consecutive threads: consecutive columns of X
+        const int32_t kk2 = k0 + k2, cc = c0 + j;
+        rb[u] = (kk2 < K && cc < S.db) ? Bop[(int64_t)kk2 * S.db + cc] : 0.0;
       }
     }
-  }
-}
-
-// k_kron_tile on the FP64 matrix cores: each of the four wavefronts computes a 32 x 32
-// quarter of the 64 x 64 tile as 2 x 2 blocks of v_mfma_f64_16x16x4 (A fragment: lane l
-// holds A[l % 16][l / 16]; B fragment: B[l / 16][l % 16]; result register v of lane l:
-// D[4 v + l / 16][l % 16], as a one-hot scan of the instruction on gfx950 shows), k-slabs of
-// 16 staged in LDS as in k_kron_tile.
-typedef double kron_v4d __attribute__((ext_vector_type(4)));
-__global__ __launch_bounds__(256) void k_kron_tile_mfma(const KronSector *sec, const int32_t *tiles,
-                                                        const double *__restrict__ ML, const double *__restrict__ MR,
-                                                        const double *__restrict__ x, double *__restrict__ z) {
-  __shared__ double As[kKt][kTile + 1];   // As[k][row]
-  __shared__ double Bs[kKt][kTile + 1];   // Bs[k][col]
-  const int32_t *tl = tiles + 3 * (int64_t)blockIdx.x;
-  const KronSector S = sec[tl[0]];
-  const int32_t r0 = tl[1] * kTile, c0 = tl[2] * kTile;
+  };
+  auto stage = [&](int buf) {
+#pragma unroll
+    for (int u = 0; u < kPer; u++) {
+      const int e = threadIdx.x + 256 * u;
+      As[buf][e % kKt][e / kKt] = ra[u];
+      if (right) Bs[buf][e % kKt][e / kKt] = rb[u];
+      else Bs[buf][e / kTile][e % kTile] = rb[u];
+    }
+  };
   const int wave = threadIdx.x / 64, l = threadIdx.x % 64;
-  const int wr = (wave / 2) * 32, wc = (wave % 2) * 32;   // this wavefront's quarter
+  const int wr = (wave / 2) * 32, wc = (wave % 2) * 32;   // MFMA: this wavefront's quarter
   const int li = l % 16, lk = l / 16;
-  const double *X = x + S.off;
-  const double *L = ML + S.ml;
-  const double *Rm = MR + S.mr;
-  kron_v4d acc[2][2];
+  const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;
+  kron_v4d macc[2][2];
+  double vacc[4][4];
 #pragma unroll
   for (int i = 0; i < 2; i++)
 #pragma unroll
-    for (int j = 0; j < 2; j++) acc[i][j] = kron_v4d{0.0, 0.0, 0.0, 0.0};
-  auto slab = [&]() {
+    for (int j = 0; j < 2; j++) macc[i][j] = kron_v4d{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-    for (int kk = 0; kk < kKt; kk += 4) {
-      const double a0 = As[kk + lk][wr + li], a1 = As[kk + lk][wr + 16 + li];
-      const double b0 = Bs[kk + lk][wc + li], b1 = Bs[kk + lk][wc + 16 + li];
-      acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
-      acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
-      acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
-      acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+  for (int i = 0; i < 4; i++)
+#pragma unroll
+    for (int j = 0; j < 4; j++) vacc[i][j] = 0.0;
+  fetch(0);
+  stage(0);
+  __syncthreads();
+  int buf = 0;
+  for (int32_t k0 = 0; k0 < K; k0 += kKt) {
+    const bool more = k0 + kKt < K;
+    if (more) fetch(k0 + kKt);
+    if constexpr (MFMA) {
+#pragma unroll
+      for (int kk = 0; kk < kKt; kk += 4) {
+        const double a0 = As[buf][kk + lk][wr + li], a1 = As[buf][kk + lk][wr + 16 + li];
+        const double b0 = Bs[buf][kk + lk][wc + li], b1 = Bs[buf][kk + lk][wc + 16 + li];
+        macc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, macc[0][0], 0, 0, 0);
+        macc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, macc[0][1], 0, 0, 0);
+        macc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, macc[1][0], 0, 0, 0);
+        macc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, macc[1][1], 0, 0, 0);
+      }
+    } else {
+#pragma unroll 2
+      for (int k = 0; k < kKt; k++) {
+        double av[4], bv[4];
+#pragma unroll
+        for (int i = 0; i < 4; i++) av[i] = As[buf][k][ty + 16 * i];
+#pragma unroll
+        for (int j = 0; j < 4; j++) bv[j] = Bs[buf][k][tx + 16 * j];
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+          for (int j = 0; j < 4; j++) vacc[i][j] = fma(av[i], bv[j], vacc[i][j]);
+      }
+    }
+    if (more) stage(buf ^ 1);
+    __syncthreads();
+    buf ^= 1;
+  }
+  double *Z = right ? z + S.off : zl + S.off;
+  auto put = [&](int32_t rr, int32_t cc, double v) {
+    if (rr < S.da && cc < S.db) {
+      const int64_t at = (int64_t)rr * S.db + cc;
+      Z[at] = right ? Z[at] + v : v;
     }
   };
-  // M_L X: As[k][i] = M_L[r0 + i][k0 + k], Bs[k][j] = X[k0 + k][c0 + j]
-  for (int32_t k0 = 0; k0 < S.da; k0 += kKt) {
-    for (int e = threadIdx.x; e < kKt * kTile; e += 256) {
-      const int i = e / kKt, k = e % kKt;
-      const int32_t rr = r0 + i, kk = k0 + k;
-      As[k][i] = (rr < S.da && kk < S.da) ? L[(int64_t)rr * S.da + kk] : 0.0;
-      const int k2 = e / kTile, j = e % kTile;
-      const int32_t kk2 = k0 + k2, cc = c0 + j;
-      Bs[k2][j] = (kk2 < S.da && cc < S.db) ? X[(int64_t)kk2 * S.db + cc] : 0.0;
-    }
-    __syncthreads();
-    slab();
-    __syncthreads();
+  if constexpr (MFMA) {
+#pragma unroll
+    for (int bi = 0; bi < 2; bi++)
+#pragma unroll
+      for (int bj = 0; bj < 2; bj++)
+#pragma unroll
+        for (int v = 0; v < 4; v++)
+          put(r0 + wr + 16 * bi + 4 * v + lk, c0 + wc + 16 * bj + li, macc[bi][bj][v]);
+  } else {
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+      for (int j = 0; j < 4; j++) put(r0 + ty + 16 * i, c0 + tx + 16 * j, vacc[i][j]);
   }
-  // X M_R^T: As[k][i] = X[r0 + i][k0 + k], Bs[k][j] = M_R[c0 + j][k0 + k]
-  for (int32_t k0 = 0; k0 < S.db; k0 += kKt) {
-    for (int e = threadIdx.x; e < kKt * kTile; e += 256) {
-      const int i = e / kKt, k = e % kKt;
-      const int32_t rr = r0 + i, kk = k0 + k, cc = c0 + i;
-      As[k][i] = (rr < S.da && kk < S.db) ? X[(int64_t)rr * S.db + kk] : 0.0;
-      Bs[k][i] = (cc < S.db && kk < S.db) ? Rm[(int64_t)cc * S.db + kk] : 0.0;
-    }
-    __syncthreads();
-    slab();
-    __syncthreads();
-  }
-  double *Z = z + S.off;
-#pragma unroll
-  for (int bi = 0; bi < 2; bi++)
-#pragma unroll
-    for (int bj = 0; bj < 2; bj++)
-#pragma unroll
-      for (int v = 0; v < 4; v++) {
-        const int32_t rr = r0 + wr + 16 * bi + 4 * v + lk, cc = c0 + wc + 16 * bj + li;
-        if (rr < S.da && cc < S.db) {
-          const int64_t at = (int64_t)rr * S.db + cc;
-          Z[at] = Z[at] + acc[bi][bj][v];
-        }
-      }
 }
 
 __global__ __launch_bounds__(256) void k_zero(int64_t n, double *p) {
@@ -318,12 +303,12 @@ bool right_kind(int k) { return k == K_RR || k == K_RL || k == K_RRRL; }
 }  // namespace
 
 void KronDev::product(const double *x, hipStream_t s) const {
-  if (z.count < (size_t)n) const_cast<DevBuf<double> &>(z).resize(n);
-  const SbView v = sb->view();
-  k_kron_cross<<<(n + 255) / 256, 256, 0, s>>>(v, cross.p, ncross, x, z.p);
+  const KronCross kc{n,      sb->d,  sb->LL.p, sb->LR.p, sb->RR.p, sb->RL.p,    aidx.p, bidx.p, asec.p,
+                     bsec.p, sec.p,  sb->terms.p, sb->ops.p, sb->ele_ptrs.p, cross.p, ncross};
+  k_kron_cross<<<(n + 255) / 256, 256, 0, s>>>(kc, x, z.p);
   if (ntiles > 0) {
-    if (mfma) k_kron_tile_mfma<<<ntiles, 256, 0, s>>>(sec.p, tiles.p, ML.p, MR.p, x, z.p);
-    else k_kron_tile<<<ntiles, 256, 0, s>>>(sec.p, tiles.p, ML.p, MR.p, x, z.p);
+    if (mfma) k_kron_tile<true><<<ntiles, 256, 0, s>>>(sec.p, tiles.p, ML.p, MR.p, x, z.p, zl.p);
+    else k_kron_tile<false><<<ntiles, 256, 0, s>>>(sec.p, tiles.p, ML.p, MR.p, x, z.p, zl.p);
   }
   HIPCHK(hipGetLastError());
 }
@@ -418,14 +403,16 @@ int dmrgx_superblock_kron(const dmrgx_superblock *S, dmrgx_kron **out) {
       ml += (int64_t)sc.da * sc.da;
       mr += (int64_t)sc.db * sc.db;
       K.flops += 2.0 * sc.da * sc.db * (double)(sc.da + sc.db);
-      for (int32_t ti = 0; ti * kTile < sc.da; ti++)
-        for (int32_t tj = 0; tj * kTile < sc.db; tj++) {
-          tiles.push_back(s);
-          tiles.push_back(ti);
-          tiles.push_back(tj);
-        }
+      for (int ph = 0; ph < 2; ph++)   // the left and the right product of each tile
+        for (int32_t ti = 0; ti * kTile < sc.da; ti++)
+          for (int32_t tj = 0; tj * kTile < sc.db; tj++) {
+            tiles.push_back(s);
+            tiles.push_back(ti);
+            tiles.push_back(tj);
+            tiles.push_back(ph);
+          }
     }
-    K.ntiles = (int32_t)(tiles.size() / 3);
+    K.ntiles = (int32_t)(tiles.size() / 4);
     // the term list split by the halves it acts on
     std::vector<int32_t> left, right, cross;
     for (size_t i = 0; i < S->hterms.size(); i++) {
@@ -444,22 +431,22 @@ int dmrgx_superblock_kron(const dmrgx_superblock *S, dmrgx_kron **out) {
     up(K.tiles, tiles);
     up(K.cross, cross);
     DevBuf<int16_t> daLL, daLR, dbRR, dbRL;
-    DevBuf<int32_t> dasec, dbsec, daidx, dbidx, dleft, dright;
+    DevBuf<int32_t> dleft, dright;
     up(daLL, aLL);
     up(daLR, aLR);
     up(dbRR, bRR);
     up(dbRL, bRL);
-    up(dasec, asec);
-    up(dbsec, bsec);
-    up(daidx, aidx);
-    up(dbidx, bidx);
+    up(K.asec, asec);
+    up(K.bsec, bsec);
+    up(K.aidx, aidx);
+    up(K.bidx, bidx);
     up(dleft, left);
     up(dright, right);
     K.ML.resize(std::max<int64_t>(ml, 1));
     K.MR.resize(std::max<int64_t>(mr, 1));
     k_zero<<<1024, 256, 0, st>>>(ml, K.ML.p);
     k_zero<<<1024, 256, 0, st>>>(mr, K.MR.p);
-    const KronBuild kb{d, K.na, K.nb, daLL.p, daLR.p, dbRR.p, dbRL.p, dasec.p, dbsec.p, daidx.p, dbidx.p, K.sec.p};
+    const KronBuild kb{d, K.na, K.nb, daLL.p, daLR.p, dbRR.p, dbRL.p, K.asec.p, K.bsec.p, K.aidx.p, K.bidx.p, K.sec.p};
     if (K.na > 0)
       k_kron_left<<<(K.na + 255) / 256, 256, 0, st>>>(kb, S->terms.p, dleft.p, (int)left.size(), S->ops.p,
                                                       S->ele_ptrs.p, K.ML.p);
@@ -468,6 +455,7 @@ int dmrgx_superblock_kron(const dmrgx_superblock *S, dmrgx_kron **out) {
                                                        S->ele_ptrs.p, K.MR.p);
     HIPCHK(hipGetLastError());
     K.z.resize(std::max<int32_t>(n, 1));
+    K.zl.resize(std::max<int32_t>(n, 1));
     HIPCHK(hipStreamSynchronize(st));   // the build's own buffers go out of scope
     *out = H.release();
   });
@@ -495,7 +483,10 @@ int dmrgx_kron_spmv(const dmrgx_kron *K, const double *x, double *y) {
     DevBuf<double> dx(n);
     ctx().copy_in(dx.p, x, sizeof(double) * n);
     K->K.product(dx.p, ctx().stream);
+    std::vector<double> zl(n);
     ctx().copy_out(y, K->K.z.p, sizeof(double) * n);
+    ctx().copy_out(zl.data(), K->K.zl.p, sizeof(double) * n);
+    for (int32_t i = 0; i < n; i++) y[i] = y[i] + zl[i];
   });
 }
 
