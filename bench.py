@@ -8,16 +8,23 @@ t=1, J_K=2, h_z=0.19109 (drawn from seed 0 in [0, 0.3]), m=800 (last infinite-sy
 Hubbard L=64 N=64 U=4 h=0.1 m=400 superblock (tests/golden/hubbard_L64_m400_s31.npz) is the
 headline and otherwise a secondary line.  Inputs are uploaded to HBM before timing.
 
-One step = one ground-state solve as GET_GROUND_STATE does it: device assembly of the
-superblock CRS (bit-identical to GET_HAM_LLLRRRRL), expansion to the full matrix, Lanczos
-(reference defaults of main/dmrg/<model>_vf/src/main/main.c: "Lanczos", "Normal",
-diag_acc 1e-14), inverse iteration with CG (cg_acc 1e-7, ii_acc 1e-9), vector to host.
+One step = one ground-state solve as GET_GROUND_STATE does it (Lanczos with the reference
+defaults of main/dmrg/<model>_vf/src/main/main.c: "Lanczos", "Normal", diag_acc 1e-14, then
+inverse iteration with CG, cg_acc 1e-7, ii_acc 1e-9; vector to host), applying the superblock
+Hamiltonian in its Kronecker form: per sector of the superblock basis the LL/LR/LLLR terms as
+a dense matrix on the rows of the sector's block of the vector and the RR/RL/RRRL terms as one
+on its columns (FP64 matrix-core products), the coupling terms row by row; the form is built
+on the device from the recorded term list in every step.  The same solve on the assembled CRS
+(device assembly bit-identical to GET_HAM_LLLRRRRL, expansion, the CRS SpMV) is reported
+beside it as forms.crs_fp64, with its SpMV roofline.
 
 Reported beside it:
-* roofline of the dominant kernel (CSR SpMV inside Lanczos/CG), timed live with HIP events
-  on the library's stream during the timed steps; algorithmic bytes per launch
-  nnz*(8+4) + (n+1)*8 + 2*n*8 (CSR); MI355X HBM peak 8 TB/s;
-* traffic: HBM bytes per SpMV launch from the committed rocprofv3 --pmc summary;
+* roofline of the dominant kernel (the dense sector products), timed live with HIP events on
+  the library's stream during the timed steps: floating-point operations per product
+  sum over sectors of 2 da db (da + db) against the MI355X dense FP64 matrix peak (78.6
+  TFLOP/s); spmv_roofline: the CRS SpMV of forms.crs_fp64, algorithmic bytes per launch
+  against the 8 TB/s HBM peak, and its HBM traffic per launch from the committed rocprofv3
+  --pmc summary;
 * cpu_baseline: the reference's own code (oracle/_ref/libdmrgref.so, built from the
   reference sources, OpenMP, pinned to the process's cores): for the Hubbard superblock its
   DMRG_DIAGONALIZE_SUPERBLOCK_SYM, for the KLM superblock (one reference solve takes minutes)
@@ -51,6 +58,9 @@ SUPERBLOCKS = {
 WORKLOAD = SUPERBLOCKS["hubbard_L64_m400"][0]
 PROFILE_EVERY = 16
 PEAK_HBM_GBS = 8000.0
+PEAK_FP64_MFMA_TFLOPS = 78.6   # MI355X dense FP64 matrix peak
+KRON_VARIANTS = ["FP64 vector units, 64 x 64 tiles", "FP64 MFMA, 64 x 64 tiles", "FP64 MFMA, 128 x 128 tiles",
+                 "FP64 MFMA, 64 x 128 tiles", "FP64 MFMA, 128 x 64 tiles"]
 SPMV_FORMATS = [("CSR", "k_apply<CsrRows<double>> (CSR SpMV + fused Lanczos/CG epilogue)"),
                 ("value dictionary", "k_apply<DictRows> (value-dictionary CRS SpMV + fused Lanczos/CG epilogue)"),
                 ("delta columns", "k_apply<DeltaRows<double>> (delta-column CRS SpMV + fused Lanczos/CG epilogue)"),
@@ -818,6 +828,94 @@ def run_serve(D, rate, duration, seed=0, chunk_ticks=1, precision=1, snapshots=N
                     "assembled on the device at arrival"}
 
 
+def run_kron(D, L, name, steps, warmup):
+    """Solves/s of the superblock workload on the Kronecker form of its Hamiltonian: per solve
+    the form's build from the resident superblock (sector tables, the dense LLLR and RRRL
+    sector matrices from the term list) + Lanczos + inverse iteration/CG (the reference's
+    algorithm and parameters, as the CRS path) + vector to host; the live roofline of the
+    dense sector products (HIP events on the library stream, one in PROFILE_EVERY)."""
+    from dmrg_c_amd import snapshot
+    path, desc, _ = SUPERBLOCKS[name]
+    snap = snapshot.load(path)
+    e_ref = float(snap.meta["gs_energy"][0])
+    params = dict(diag_acc=1e-14, diag_max_step=800, cg_acc=1e-7, cg_max_step=2000, ii_acc=1e-9,
+                  ii_diag_add=1e-11, ii_max_step=3, seed=1)
+    sb = D.Superblock(snap)
+    K = sb.kron()
+    variant, tuned_us = K.autotune()
+    K.free()
+    log(f"kron {name}: {K.sectors} sectors, {K.coupling_terms} coupling terms, {K.flops / 1e9:.3f} GFLOP per "
+        f"product; autotune: variant {variant} ({tuned_us:.1f} us per product)")
+
+    def solve():
+        Kf = sb.kron()
+        Kf.set_kernel(variant)
+        E, vec, st = Kf.ground_state(**params)
+        Kf.free()
+        return E, st
+
+    for i in range(warmup):
+        E, st = solve()
+        log(f"kron warmup {i}: E={E:.15f} lanczos={st.lanczos_steps} cg={st.cg_steps}")
+    D.check(L.dmrgx_profile_begin_sampled(PROFILE_EVERY))
+    lat, errs, lan, cg, gerr = [], [], [], [], []
+    t0 = time.perf_counter()
+    for i in range(steps):
+        ts = time.perf_counter()
+        E, st = solve()
+        lat.append(time.perf_counter() - ts)
+        errs.append(abs(E - e_ref) / abs(e_ref))
+        lan.append(st.lanczos_steps)
+        cg.append(st.cg_steps)
+        gerr.append(st.gs_error)
+        log(f"kron step {i}: {lat[-1] * 1e3:.2f} ms  E={E:.15f}  dE/E={errs[-1]:.2e}")
+    total = time.perf_counter() - t0
+    ms, launches, flops = C.c_double(), C.c_int64(), C.c_double()
+    L.dmrgx_profile_end.argtypes = [C.POINTER(C.c_double), C.POINTER(C.c_int64), C.POINTER(C.c_double)]
+    D.check(L.dmrgx_profile_end(C.byref(ms), C.byref(launches), C.byref(flops)))
+    sb.free()
+    avg_s = ms.value / 1e3 / max(launches.value, 1)
+    fl = flops.value / max(launches.value, 1)
+    achieved = fl / avg_s / 1e12 if avg_s > 0 else 0.0
+    v = variant % len(KRON_VARIANTS)
+    return {
+        "value": round(steps / total, 4),
+        "ms_per_step": round(total / steps * 1e3, 3),
+        "data": f"superblock snapshot from the reference's own DMRG run (tests/golden/{os.path.basename(path)})",
+        "config": {
+            "workload": f"{desc}, superblock n={snap.n}: Kronecker form of the superblock Hamiltonian built on the "
+                        "device from the recorded term list (no CRS assembly) + Lanczos + inverse iteration/CG "
+                        "per solve",
+            "n": snap.n,
+            "sectors": K.sectors,
+            "coupling_terms": K.coupling_terms,
+            "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 3),
+            "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 3),
+            "max_rel_energy_error": float(max(errs)),
+            "max_gs_error": float(max(gerr)),
+            "lanczos_steps": int(np.median(lan)),
+            "cg_steps": int(np.median(cg)),
+            "product_kernel": {"variant": variant, "form": KRON_VARIANTS[v],
+                               "xcd_contiguous_tiles": variant >= len(KRON_VARIANTS)},
+        },
+        "roofline": {
+            "bound": "mfma",
+            "kernel": "k_kron_tile<64, 64, true> (dense sector products M_L X and X M_R^T on v_mfma_f64_16x16x4)"
+                      if v == 1 else f"k_kron_tile ({KRON_VARIANTS[v]})",
+            "achieved": round(achieved, 2),
+            "peak": PEAK_FP64_MFMA_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": round(achieved / PEAK_FP64_MFMA_TFLOPS, 4),
+            "traffic": None,
+            "alg_flops_per_launch": int(fl),
+            "avg_launch_us": round(avg_s * 1e6, 2),
+            "launches": int(launches.value),
+            "launches_sampled": f"1 in {PROFILE_EVERY} of the timed solves' products, HIP events on the library "
+                                "stream around the dense-product kernel",
+        },
+    }
+
+
 def run_superblock(D, L, name, steps, warmup, fixed, cpu_budget):
     """Solves/s of one superblock workload: device assembly + Lanczos + inverse iteration/CG +
     vector to host per solve, p50/p99, the live SpMV roofline (HIP events on the library
@@ -1072,35 +1170,46 @@ def main():
         else:
             spmv_sweep(D, sb, snap)
         return
-    head = run_superblock(D, L, wl, args.steps, args.warmup, fixed,
+    kron = run_kron(D, L, wl, args.steps, args.warmup)
+    head = run_superblock(D, L, wl, max(2, args.steps // 2) if not args.traffic_probe else args.steps, 1, fixed,
                           None if args.no_cpu_baseline else args.cpu_budget)
+    crs = {k: head[k] for k in ("value", "ms_per_step", "config", "roofline")}
+    crs["note"] = ("the reference's own structure: device assembly of the CRS (bit-exact with GET_HAM_LLLRRRRL), "
+                   "lower -> full expansion, sliced-form build and the CRS SpMV per product; the same Lanczos + "
+                   "inverse iteration/CG")
+    crs["over"] = "kron value / crs value = " + str(round(kron["value"] / head["value"], 2))
     line = {
         "metric": "ground-state solves/s + p99 latency at |dE/E|<=1e-9; SpMV HBM GB/s vs peak",
-        "value": head["value"],
+        "value": kron["value"],
         "unit": "solves/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": head["ms_per_step"],
+        "ms_per_step": kron["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": head["data"],
-        "config": head["config"],
-        "forms": {"fp32_refined": head["fp32_refined"],
-                  "fp32_refined_inverse_iteration": head["fp32_refined_inverse_iteration"]},
-        "roofline": head["roofline"],
+        "data": kron["data"],
+        "config": kron["config"],
+        "forms": {"crs_fp64": crs,
+                  "fp32_refined": dict(head["fp32_refined"], over="CRS FP64 path"),
+                  "fp32_refined_inverse_iteration": dict(head["fp32_refined_inverse_iteration"],
+                                                         over="CRS FP64 path")},
+        "roofline": kron["roofline"],
+        "spmv_roofline": head["roofline"],
         "cpu_baseline": head["cpu_baseline"],
     }
     if not args.headline_only:
         # the other superblock as a secondary measurement (same solve, its own roofline)
         for other in SUPERBLOCKS:
             if other != wl and os.path.exists(SUPERBLOCKS[other][0]):
+                ksec = run_kron(D, L, other, max(4, args.steps // 2), 1)
                 sec = run_superblock(D, L, other, max(4, args.steps // 2), 1, None, None)
                 line.setdefault("secondary_workloads", {})[other] = {
-                    k: sec[k] for k in ("value", "ms_per_step", "config", "fp32_refined",
-                                        "fp32_refined_inverse_iteration", "roofline")}
+                    "kron": {k: ksec[k] for k in ("value", "ms_per_step", "config", "roofline")},
+                    "crs_fp64": {k: sec[k] for k in ("value", "ms_per_step", "config", "fp32_refined",
+                                                     "fp32_refined_inverse_iteration", "roofline")}}
         log("forms: batched ED (256 seeded problems)")
         line["forms"]["batched"] = batched_form(D)
         log("forms: served stream")

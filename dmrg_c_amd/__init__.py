@@ -715,7 +715,8 @@ class KronOperator:
         L.dmrgx_kron_info.argtypes = [vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int32),
                                       C.POINTER(C.c_double)]
         L.dmrgx_kron_spmv.argtypes = [vp, vp, vp]
-        L.dmrgx_kron_set_matrix_cores.argtypes = [vp, C.c_int32]
+        L.dmrgx_kron_set_kernel.argtypes = [vp, C.c_int32]
+        L.dmrgx_kron_autotune.argtypes = [vp, C.POINTER(C.c_int32), C.POINTER(C.c_double)]
         L.dmrgx_kron_ground_state.argtypes = [vp, C.POINTER(SolveParams), vp, C.POINTER(SolveStats)]
         L.dmrgx_kron_free.argtypes = [vp]
         L.dmrgx_kron_free.restype = None
@@ -727,8 +728,15 @@ class KronOperator:
         check(L.dmrgx_kron_info(h, C.byref(n), C.byref(ns), C.byref(nc), C.byref(fl)))
         self.n, self.sectors, self.coupling_terms, self.flops = n.value, ns.value, nc.value, fl.value
 
-    def set_matrix_cores(self, on: bool):
-        check(lib().dmrgx_kron_set_matrix_cores(self.handle, 1 if on else 0))
+    VARIANTS = 10
+
+    def set_kernel(self, variant: int):
+        check(lib().dmrgx_kron_set_kernel(self.handle, variant))
+
+    def autotune(self):
+        v, us = C.c_int32(), C.c_double()
+        check(lib().dmrgx_kron_autotune(self.handle, C.byref(v), C.byref(us)))
+        return v.value, us.value
 
     def matvec(self, x: np.ndarray) -> np.ndarray:
         xx = np.ascontiguousarray(x, np.float64)

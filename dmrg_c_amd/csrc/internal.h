@@ -833,19 +833,29 @@ struct KronSector {
   int64_t ml, mr;     // offsets of the sector's M_L and M_R in KronDev::ML / MR
   int32_t a0, b0;     // first of its LLLR / RRRL states in the flat state lists
 };
+// Tile shapes of the dense sector products and the product kernels over them: variant 0
+// the FP64 vector form on 64 x 64 tiles, 1-4 the FP64 matrix cores on 64 x 64, 128 x 128,
+// 64 x 128 and 128 x 64 tiles.
+constexpr int kKronShapes = 4;
+inline constexpr int kKronTM[kKronShapes] = {64, 128, 64, 128};
+inline constexpr int kKronTN[kKronShapes] = {64, 128, 128, 64};
+constexpr int kKronVariants = 5;
+inline constexpr int kKronShape[kKronVariants] = {0, 0, 1, 2, 3};
 struct KronDev {
-  int32_t n = 0, nsec = 0, ntiles = 0, ncross = 0, na = 0, nb = 0;
+  int32_t n = 0, nsec = 0, ncross = 0, na = 0, nb = 0;
+  int32_t ntiles[kKronShapes] = {0, 0, 0, 0};
+  int variant = 1;                        // product kernel (kKronShape of it picks the tiles)
+  int xcd = 0;                            // 1: consecutive tiles to the blocks of one XCD
   const dmrgx_superblock *sb = nullptr;   // borrowed: basis, inverse, operators, term list
   std::vector<KronSector> hsec;
   DevBuf<KronSector> sec;
   DevBuf<double> ML, MR;                  // dense sector blocks, row-major
-  DevBuf<int32_t> tiles;                  // (sector, tile row, tile column, left/right product)
+  DevBuf<int32_t> tiles[kKronShapes];     // per shape: (sector, tile row, tile column, left/right)
   DevBuf<int32_t> cross;                  // indices of the coupling terms in the term list
   // flat LLLR / RRRL state of (LL d + LR) / (RR d + RL) (-1: none) and each flat state's sector
   DevBuf<int32_t> aidx, bidx, asec, bsec;
   mutable DevBuf<double> z, zl;           // H x of the last product is z + zl
   double flops = 0.0;                     // of the two dense products per H x
-  bool mfma = true;                       // dense products on the FP64 matrix cores
   // z = H x on stream s
   void product(const double *x, hipStream_t s) const;
 };

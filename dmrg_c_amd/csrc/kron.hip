@@ -25,6 +25,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstring>
 
@@ -35,7 +36,6 @@
 namespace dmrgx {
 namespace {
 
-constexpr int kTile = 64;   // output tile of the dense products
 constexpr int kKt = 16;     // k-slab staged in LDS
 
 struct KronBuild {
@@ -166,71 +166,86 @@ __global__ __launch_bounds__(256) void k_kron_cross(KronCross kc, const double *
   z[r] = s;
 }
 
-// One 64 x 64 tile of one of a sector's two dense products: the left one (M_L X, k over
+// One TM x TN tile of one of a sector's two dense products: the left one (M_L X, k over
 // the sector's LLLR states, written to zl) or the right one (X M_R^T, k over its RRRL
 // states, added to z, which holds the coupling terms).  k-slabs of kKt staged in LDS, double
 // buffered: the next slab's global loads are issued before the current slab's products.
-// Outside the sector the operands are zero.  MFMA: each of the four wavefronts computes a
-// 32 x 32 quarter as 2 x 2 blocks of v_mfma_f64_16x16x4 (A fragment: lane l holds
-// A[l % 16][l / 16]; B fragment: B[l / 16][l % 16]; result register v of lane l:
-// D[4 v + l / 16][l % 16], as a one-hot scan of the instruction on gfx950 shows).  Without:
-// each thread a 4 x 4 block of FP64 fused multiply-adds (rows ty + 16 i, columns tx + 16 j).
+// Outside the sector the operands are zero.  MFMA: the four wavefronts split the tile 2 x 2,
+// each computing its (TM/2) x (TN/2) part as blocks of v_mfma_f64_16x16x4 (A fragment: lane
+// l holds A[l % 16][l / 16]; B fragment: B[l / 16][l % 16]; result register v of lane l:
+// D[4 v + l / 16][l % 16], as a one-hot scan of the instruction on gfx950 shows).  Without
+// (64 x 64 only): each thread a 4 x 4 block of FP64 fused multiply-adds (rows ty + 16 i,
+// columns tx + 16 j).
 typedef double kron_v4d __attribute__((ext_vector_type(4)));
-constexpr int kPer = kKt * kTile / 256;   // staged elements per thread per operand
 
-template <bool MFMA>
+template <int TM, int TN, bool MFMA>
 __global__ __launch_bounds__(256) void k_kron_tile(const KronSector *sec, const int32_t *tiles,
                                                    const double *__restrict__ ML, const double *__restrict__ MR,
                                                    const double *__restrict__ x, double *__restrict__ z,
-                                                   double *__restrict__ zl) {
-  __shared__ double As[2][kKt][kTile + 1];   // As[buffer][k][row]
-  __shared__ double Bs[2][kKt][kTile + 1];   // Bs[buffer][k][col]
-  const int32_t *tl = tiles + 4 * (int64_t)blockIdx.x;
+                                                   double *__restrict__ zl, int xcd) {
+  static_assert(MFMA || (TM == 64 && TN == 64), "the vector form computes 64 x 64 tiles");
+  constexpr int PA = kKt * TM / 256, PB = kKt * TN / 256;   // staged elements per thread
+  constexpr int BM = TM / 32, BN = TN / 32;                 // 16 x 16 blocks per wavefront
+  __shared__ double As[2][kKt][TM + 1];   // As[buffer][k][row]
+  __shared__ double Bs[2][kKt][TN + 1];   // Bs[buffer][k][col]
+  // xcd: consecutive tiles (one sector's, in list order) to the blocks of one XCD, so that
+  // a sector's operands are fetched into one XCD's L2
+  const int32_t lt = xcd ? xcd_logical(blockIdx.x, gridDim.x) : (int32_t)blockIdx.x;
+  const int32_t *tl = tiles + 4 * (int64_t)lt;
   const KronSector S = sec[tl[0]];
-  const int32_t r0 = tl[1] * kTile, c0 = tl[2] * kTile;
+  const int32_t r0 = tl[1] * TM, c0 = tl[2] * TN;
   const bool right = tl[3] != 0;
   const int32_t K = right ? S.db : S.da;
   const double *X = x + S.off;
   const double *Aop = right ? X : ML + S.ml;           // A[i][k] = Aop[(r0 + i) lda + k]
   const int64_t lda = right ? S.db : S.da;
   const double *Bop = right ? MR + S.mr : X;           // left: B[k][j] = X[k db + c0 + j]
-  double ra[kPer], rb[kPer];                           // right: B[k][j] = M_R[(c0 + j) db + k]
+  double ra[PA], rb[PB];                               // right: B[k][j] = M_R[(c0 + j) db + k]
   auto fetch = [&](int32_t k0) {
 #pragma unroll
-    for (int u = 0; u < kPer; u++) {
+    for (int u = 0; u < PA; u++) {
       const int e = threadIdx.x + 256 * u;
       const int i = e / kKt, k = e % kKt;              // consecutive threads: consecutive k of a row
       const int32_t rr = r0 + i, kk = k0 + k;
       ra[u] = (rr < S.da && kk < K) ? Aop[rr * lda + kk] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < PB; u++) {
+      const int e = threadIdx.x + 256 * u;
       if (right) {
-        const int32_t cc = c0 + i;
+        const int j = e / kKt, k = e % kKt;
+        const int32_t cc = c0 + j, kk = k0 + k;
         rb[u] = (cc < S.db && kk < K) ? Bop[(int64_t)cc * S.db + kk] : 0.0;
       } else {
-        const int k2 = e / kTile, j = e % kTile;       // consecutive threads: consecutive columns of X
-        const int32_t kk2 = k0 + k2, cc = c0 + j;
-        rb[u] = (kk2 < K && cc < S.db) ? Bop[(int64_t)kk2 * S.db + cc] : 0.0;
+        const int k = e / TN, j = e % TN;              // consecutive threads: consecutive columns of X
+        const int32_t kk = k0 + k, cc = c0 + j;
+        rb[u] = (kk < K && cc < S.db) ? Bop[(int64_t)kk * S.db + cc] : 0.0;
       }
     }
   };
   auto stage = [&](int buf) {
 #pragma unroll
-    for (int u = 0; u < kPer; u++) {
+    for (int u = 0; u < PA; u++) {
       const int e = threadIdx.x + 256 * u;
       As[buf][e % kKt][e / kKt] = ra[u];
+    }
+#pragma unroll
+    for (int u = 0; u < PB; u++) {
+      const int e = threadIdx.x + 256 * u;
       if (right) Bs[buf][e % kKt][e / kKt] = rb[u];
-      else Bs[buf][e / kTile][e % kTile] = rb[u];
+      else Bs[buf][e / TN][e % TN] = rb[u];
     }
   };
   const int wave = threadIdx.x / 64, l = threadIdx.x % 64;
-  const int wr = (wave / 2) * 32, wc = (wave % 2) * 32;   // MFMA: this wavefront's quarter
+  const int wr = (wave / 2) * (TM / 2), wc = (wave % 2) * (TN / 2);   // MFMA: this wavefront's part
   const int li = l % 16, lk = l / 16;
   const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;
-  kron_v4d macc[2][2];
+  kron_v4d macc[BM][BN];
   double vacc[4][4];
 #pragma unroll
-  for (int i = 0; i < 2; i++)
+  for (int i = 0; i < BM; i++)
 #pragma unroll
-    for (int j = 0; j < 2; j++) macc[i][j] = kron_v4d{0.0, 0.0, 0.0, 0.0};
+    for (int j = 0; j < BN; j++) macc[i][j] = kron_v4d{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
   for (int i = 0; i < 4; i++)
 #pragma unroll
@@ -245,12 +260,15 @@ __global__ __launch_bounds__(256) void k_kron_tile(const KronSector *sec, const 
     if constexpr (MFMA) {
 #pragma unroll
       for (int kk = 0; kk < kKt; kk += 4) {
-        const double a0 = As[buf][kk + lk][wr + li], a1 = As[buf][kk + lk][wr + 16 + li];
-        const double b0 = Bs[buf][kk + lk][wc + li], b1 = Bs[buf][kk + lk][wc + 16 + li];
-        macc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, macc[0][0], 0, 0, 0);
-        macc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, macc[0][1], 0, 0, 0);
-        macc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, macc[1][0], 0, 0, 0);
-        macc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, macc[1][1], 0, 0, 0);
+        double a[BM], b[BN];
+#pragma unroll
+        for (int i = 0; i < BM; i++) a[i] = As[buf][kk + lk][wr + 16 * i + li];
+#pragma unroll
+        for (int j = 0; j < BN; j++) b[j] = Bs[buf][kk + lk][wc + 16 * j + li];
+#pragma unroll
+        for (int i = 0; i < BM; i++)
+#pragma unroll
+          for (int j = 0; j < BN; j++) macc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], macc[i][j], 0, 0, 0);
       }
     } else {
 #pragma unroll 2
@@ -279,12 +297,11 @@ __global__ __launch_bounds__(256) void k_kron_tile(const KronSector *sec, const 
   };
   if constexpr (MFMA) {
 #pragma unroll
-    for (int bi = 0; bi < 2; bi++)
+    for (int bi = 0; bi < BM; bi++)
 #pragma unroll
-      for (int bj = 0; bj < 2; bj++)
+      for (int bj = 0; bj < BN; bj++)
 #pragma unroll
-        for (int v = 0; v < 4; v++)
-          put(r0 + wr + 16 * bi + 4 * v + lk, c0 + wc + 16 * bj + li, macc[bi][bj][v]);
+        for (int v = 0; v < 4; v++) put(r0 + wr + 16 * bi + 4 * v + lk, c0 + wc + 16 * bj + li, macc[bi][bj][v]);
   } else {
 #pragma unroll
     for (int i = 0; i < 4; i++)
@@ -306,9 +323,27 @@ void KronDev::product(const double *x, hipStream_t s) const {
   const KronCross kc{n,      sb->d,  sb->LL.p, sb->LR.p, sb->RR.p, sb->RL.p,    aidx.p, bidx.p, asec.p,
                      bsec.p, sec.p,  sb->terms.p, sb->ops.p, sb->ele_ptrs.p, cross.p, ncross};
   k_kron_cross<<<(n + 255) / 256, 256, 0, s>>>(kc, x, z.p);
-  if (ntiles > 0) {
-    if (mfma) k_kron_tile<true><<<ntiles, 256, 0, s>>>(sec.p, tiles.p, ML.p, MR.p, x, z.p, zl.p);
-    else k_kron_tile<false><<<ntiles, 256, 0, s>>>(sec.p, tiles.p, ML.p, MR.p, x, z.p, zl.p);
+  const int shape = kKronShape[variant];
+  const int nt = ntiles[shape];
+  if (nt > 0) {
+    const int32_t *tl = tiles[shape].p;
+    // live timing (dmrgx_profile_begin_sampled): HIP events around the dense products, their
+    // floating-point operations counted where the CRS products count bytes
+    Context &C = ctx();
+    std::pair<hipEvent_t, hipEvent_t> *ev = nullptr;
+    if (C.prof_take()) {
+      ev = C.prof_slot();
+      HIPCHK(hipEventRecord(ev->first, s));
+      C.prof_bytes += flops;
+    }
+    switch (variant) {
+      case 0: k_kron_tile<64, 64, false><<<nt, 256, 0, s>>>(sec.p, tl, ML.p, MR.p, x, z.p, zl.p, xcd); break;
+      case 1: k_kron_tile<64, 64, true><<<nt, 256, 0, s>>>(sec.p, tl, ML.p, MR.p, x, z.p, zl.p, xcd); break;
+      case 2: k_kron_tile<128, 128, true><<<nt, 256, 0, s>>>(sec.p, tl, ML.p, MR.p, x, z.p, zl.p, xcd); break;
+      case 3: k_kron_tile<64, 128, true><<<nt, 256, 0, s>>>(sec.p, tl, ML.p, MR.p, x, z.p, zl.p, xcd); break;
+      default: k_kron_tile<128, 64, true><<<nt, 256, 0, s>>>(sec.p, tl, ML.p, MR.p, x, z.p, zl.p, xcd); break;
+    }
+    if (ev) HIPCHK(hipEventRecord(ev->second, s));
   }
   HIPCHK(hipGetLastError());
 }
@@ -395,7 +430,7 @@ int dmrgx_superblock_kron(const dmrgx_superblock *S, dmrgx_kron **out) {
     K.na = (int32_t)aLL.size();
     K.nb = (int32_t)bRR.size();
     int64_t ml = 0, mr = 0;
-    std::vector<int32_t> tiles;
+    std::vector<int32_t> tiles[kKronShapes];
     for (int s = 0; s < K.nsec; s++) {
       KronSector &sc = K.hsec[s];
       sc.ml = ml;
@@ -403,16 +438,26 @@ int dmrgx_superblock_kron(const dmrgx_superblock *S, dmrgx_kron **out) {
       ml += (int64_t)sc.da * sc.da;
       mr += (int64_t)sc.db * sc.db;
       K.flops += 2.0 * sc.da * sc.db * (double)(sc.da + sc.db);
-      for (int ph = 0; ph < 2; ph++)   // the left and the right product of each tile
-        for (int32_t ti = 0; ti * kTile < sc.da; ti++)
-          for (int32_t tj = 0; tj * kTile < sc.db; tj++) {
-            tiles.push_back(s);
-            tiles.push_back(ti);
-            tiles.push_back(tj);
-            tiles.push_back(ph);
-          }
+      for (int sh = 0; sh < kKronShapes; sh++)
+        for (int ph = 0; ph < 2; ph++)   // the left and the right product of each tile
+          for (int32_t ti = 0; ti * kKronTM[sh] < sc.da; ti++)
+            for (int32_t tj = 0; tj * kKronTN[sh] < sc.db; tj++) {
+              tiles[sh].push_back(s);
+              tiles[sh].push_back(ti);
+              tiles[sh].push_back(tj);
+              tiles[sh].push_back(ph);
+            }
     }
-    K.ntiles = (int32_t)(tiles.size() / 4);
+    // longest first (k extent of the tile's product: da for the left, db for the right one),
+    // so that the short tiles fill in behind the long ones instead of the long ones ending last
+    for (int sh = 0; sh < kKronShapes; sh++) {
+      std::vector<std::array<int32_t, 4>> t(tiles[sh].size() / 4);
+      std::memcpy(t.data(), tiles[sh].data(), sizeof(int32_t) * tiles[sh].size());
+      auto klen = [&](const std::array<int32_t, 4> &a) { return a[3] ? K.hsec[a[0]].db : K.hsec[a[0]].da; };
+      std::stable_sort(t.begin(), t.end(), [&](const auto &a, const auto &b) { return klen(a) > klen(b); });
+      std::memcpy(tiles[sh].data(), t.data(), sizeof(int32_t) * tiles[sh].size());
+      K.ntiles[sh] = (int32_t)t.size();
+    }
     // the term list split by the halves it acts on
     std::vector<int32_t> left, right, cross;
     for (size_t i = 0; i < S->hterms.size(); i++) {
@@ -428,7 +473,7 @@ int dmrgx_superblock_kron(const dmrgx_superblock *S, dmrgx_kron **out) {
       if (!h.empty()) ctx().copy_in(buf.p, h.data(), sizeof(T) * h.size());
     };
     up(K.sec, K.hsec);
-    up(K.tiles, tiles);
+    for (int sh = 0; sh < kKronShapes; sh++) up(K.tiles[sh], tiles[sh]);
     up(K.cross, cross);
     DevBuf<int16_t> daLL, daLR, dbRR, dbRL;
     DevBuf<int32_t> dleft, dright;
@@ -470,10 +515,49 @@ int dmrgx_kron_info(const dmrgx_kron *K, int32_t *n, int32_t *sectors, int32_t *
   return DMRGX_OK;
 }
 
-int dmrgx_kron_set_matrix_cores(dmrgx_kron *K, int32_t on) {
-  if (!K || (on != 0 && on != 1)) return DMRGX_ERR_ARG;
-  K->K.mfma = on != 0;
+int dmrgx_kron_set_kernel(dmrgx_kron *K, int32_t variant) {
+  if (!K || variant < 0 || variant >= 2 * kKronVariants) return DMRGX_ERR_ARG;
+  K->K.variant = variant % kKronVariants;
+  K->K.xcd = variant / kKronVariants;
   return DMRGX_OK;
+}
+
+int dmrgx_kron_autotune(dmrgx_kron *K, int32_t *variant_out, double *us_out) {
+  if (!K) return DMRGX_ERR_ARG;
+  return run_status([&] {
+    KronDev &D = K->K;
+    hipStream_t st = ctx().stream;
+    DevBuf<double> x(std::max<int32_t>(D.n, 1));
+    launch_start_vector(D.n, 7, x.p, st);
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    int best = D.variant, best_xcd = D.xcd;
+    float best_ms = 1e30f;
+    for (int c = 0; c < 2 * kKronVariants; c++) {
+      const int v = c % kKronVariants, xm = c / kKronVariants;
+      D.variant = v;
+      D.xcd = xm;
+      D.product(x.p, st);
+      HIPCHK(hipEventRecord(e0, st));
+      for (int r = 0; r < 5; r++) D.product(x.p, st);
+      HIPCHK(hipEventRecord(e1, st));
+      HIPCHK(hipEventSynchronize(e1));
+      float ms = 0;
+      HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+      if (ms < best_ms) {
+        best_ms = ms;
+        best = v;
+        best_xcd = xm;
+      }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    D.variant = best;
+    D.xcd = best_xcd;
+    if (variant_out) *variant_out = best + kKronVariants * best_xcd;
+    if (us_out) *us_out = best_ms * 1000.0 / 5;
+  });
 }
 
 int dmrgx_kron_spmv(const dmrgx_kron *K, const double *x, double *y) {

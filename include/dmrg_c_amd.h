@@ -662,8 +662,13 @@ int dmrgx_superblock_kron(const dmrgx_superblock *S, dmrgx_kron **out);
 /* dimension, sectors, coupling terms, floating-point operations of the dense products per
  * product (any pointer may be null) */
 int dmrgx_kron_info(const dmrgx_kron *K, int32_t *n, int32_t *sectors, int32_t *coupling_terms, double *flops);
-/* dense products on the FP64 matrix cores (1, the default) or on the vector units (0) */
-int dmrgx_kron_set_matrix_cores(dmrgx_kron *K, int32_t on);
+/* kernel of the dense sector products: 0 FP64 vector units on 64 x 64 tiles; FP64 matrix
+ * cores on 1: 64 x 64, 2: 128 x 128, 3: 64 x 128, 4: 128 x 64 tiles (default 1); plus 5 for
+ * the same kernels with consecutive tiles (one sector's) dealt to the workgroups of one XCD.
+ * The results agree to rounding between the variants. */
+int dmrgx_kron_set_kernel(dmrgx_kron *K, int32_t variant);
+/* times every variant (5 products each after one warm-up) and keeps the fastest */
+int dmrgx_kron_autotune(dmrgx_kron *K, int32_t *variant_out, double *us_out);
 /* y = H x (host arrays of length n) */
 int dmrgx_kron_spmv(const dmrgx_kron *K, const double *x, double *y);
 /* dmrgx_ground_state on the Kronecker form (precision 0 only) */

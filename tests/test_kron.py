@@ -24,15 +24,17 @@ def _ref_full(s):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("cores", [True, False], ids=["mfma", "vector"])
+@pytest.mark.parametrize("variant", range(10), ids=["vec64", "mfma64", "mfma128", "mfma64x128", "mfma128x64",
+                                                     "vec64-xcd", "mfma64-xcd", "mfma128-xcd", "mfma64x128-xcd",
+                                                     "mfma128x64-xcd"])
 @pytest.mark.parametrize("path", SNAPS + BIG, ids=golden_ids(SNAPS + BIG))
-def test_gpu_kron_product_equals_reference_crs(path, cores):
+def test_gpu_kron_product_equals_reference_crs(path, variant):
     import dmrg_c_amd as D
     from dmrg_c_amd import snapshot
     s = snapshot.load(path)
     sb = D.Superblock(s)
     K = sb.kron()
-    K.set_matrix_cores(cores)
+    K.set_kernel(variant)
     try:
         x = np.random.default_rng(11).standard_normal(s.n)
         y = K.matvec(x)
@@ -68,6 +70,27 @@ def test_gpu_kron_ground_state(path):
     if "gs_vec" in s.meta:
         ref = s.meta["gs_vec"]
         assert np.abs(vec - np.sign(vec @ ref) * ref).max() <= 1e-8
+
+
+@pytest.mark.gpu
+def test_gpu_kron_autotune_and_bad_variant():
+    import dmrg_c_amd as D
+    from dmrg_c_amd import snapshot
+    s = snapshot.load(BIG[0])
+    sb = D.Superblock(s)
+    K = sb.kron()
+    try:
+        v, us = K.autotune()
+        assert 0 <= v < K.VARIANTS and us > 0
+        for bad in (-1, K.VARIANTS):
+            with pytest.raises(D.DmrgxError):
+                K.set_kernel(bad)
+        E, _, _ = K.ground_state()
+        e_ref = float(s.meta["gs_energy"][0])
+        assert abs(E - e_ref) <= 1e-10 * abs(e_ref)
+    finally:
+        K.free()
+        sb.free()
 
 
 @pytest.mark.gpu
