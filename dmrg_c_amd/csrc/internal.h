@@ -852,6 +852,10 @@ struct KronDev {
   DevBuf<double> ML, MR;                  // dense sector blocks, row-major
   DevBuf<int32_t> tiles[kKronShapes];     // per shape: (sector, tile row, tile column, left/right)
   DevBuf<int32_t> cross;                  // indices of the coupling terms in the term list
+  // per coupling term: its factor, and the LLLR / RRRL states each state reaches with the
+  // term's operator on that half, with coefficients (k_kron_cross)
+  DevBuf<double> ct, lval, rval;
+  DevBuf<int32_t> lptr, lidx, rptr, ridx;
   // flat LLLR / RRRL state of (LL d + LR) / (RR d + RL) (-1: none) and each flat state's sector
   DevBuf<int32_t> aidx, bidx, asec, bsec;
   mutable DevBuf<double> z, zl;           // H x of the last product is z + zl

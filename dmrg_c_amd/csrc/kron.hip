@@ -119,51 +119,50 @@ __global__ __launch_bounds__(256) void k_kron_right(KronBuild kb, const DTerm *t
   }
 }
 
-// z = coupling terms applied to x, one thread per superblock row: the element makers'
-// enumeration of each term's operator rows; the column state's superblock index from the
-// sector tables (flat LLLR and RRRL state of the column, both in one sector, then
-// off + a db + b) instead of the dense superblock inverse (a few kB of tables in the caches
-// against the inverse's dim_LL d dim_RR d entries); a state pair outside the superblock basis
-// is dropped, as the inverse's -1 is.
+// z = coupling terms applied to x.  A coupling term has one operator on each half (op1 on
+// LL or LR, op2 on RR or RL) and a sign that is a product of a factor of the row's LLLR state
+// and one of its RRRL state, so per term the builder lists, for every LLLR state a', the
+// states a it reaches with op1 and their coefficients (left parity sign x op1 value), and
+// likewise for every RRRL state b' (right parity sign x op2 value); the term's own factor is
+// its coefficient times the sign rule's constant.  One workgroup per LLLR state a' of the
+// output, its threads over the sector's RRRL states b': for each term (in term order), op1
+// entry and op2 entry (in row order, as the element makers enumerate them) the contribution
+// c_t c_l c_r x[a, b] is added when (a, b) is a superblock state (a and b in one sector) and
+// |c_t c_l c_r| > 1e-15, as the element makers drop it.
 struct KronCross {
-  int32_t n, d;
-  const int16_t *LL, *LR, *RR, *RL;
-  const int32_t *aidx, *bidx, *asec, *bsec;
+  int nt, na, nb;
+  const double *ct;                         // per term
+  const int32_t *lptr, *lidx, *rptr, *ridx;  // term t: lptr[t (na + 1) + a'] ... into lidx / lval
+  const double *lval, *rval;
+  const int32_t *asec, *bsec;
   const KronSector *sec;
-  const DTerm *terms;
-  const DOp *ops;
-  const int32_t *const *eles;
-  const int32_t *ids;
-  int nt;
 };
 __global__ __launch_bounds__(256) void k_kron_cross(KronCross kc, const double *__restrict__ x, double *__restrict__ z) {
-  const int32_t r = blockIdx.x * 256 + threadIdx.x;
-  if (r >= kc.n) return;
-  const int q[4] = {kc.LL[r], kc.LR[r], kc.RR[r], kc.RL[r]};
-  double s = 0.0;
-  for (int u = 0; u < kc.nt; u++) {
-    const DTerm t = kc.terms[kc.ids[u]];
-    const DOp a = kc.ops[t.op1], b = kc.ops[t.op2];
-    const double sc = (double)term_sign(t, kc.eles, q) * t.coeff;
-    const int64_t b1 = a.row[q[t.s1]], e1 = a.row[q[t.s1] + 1];
-    const int64_t b2 = b.row[q[t.s2]], e2 = b.row[q[t.s2] + 1];
-    for (int64_t i1 = b1; i1 < e1; i1++) {
-      int qq[4] = {q[0], q[1], q[2], q[3]};
-      qq[t.s1] = a.col[i1];
-      const double p1 = sc * a.val[i1];
-      for (int64_t i2 = b2; i2 < e2; i2++) {
-        qq[t.s2] = b.col[i2];
-        const double p = p1 * b.val[i2];
-        const int32_t fa = kc.aidx[qq[0] * kc.d + qq[1]], fb = kc.bidx[qq[2] * kc.d + qq[3]];
-        if (fa < 0 || fb < 0 || !(fabs(p) > 1.0e-15)) continue;
-        const int sa = kc.asec[fa];
-        if (kc.bsec[fb] != sa) continue;
-        const KronSector S = kc.sec[sa];
-        s += p * x[S.off + (int64_t)(fa - S.a0) * S.db + (fb - S.b0)];
+  const int32_t i = blockIdx.x;   // flat LLLR state of the output rows
+  const KronSector S = kc.sec[kc.asec[i]];
+  double *zr = z + S.off + (int64_t)(i - S.a0) * S.db;
+  for (int32_t bl = threadIdx.x; bl < S.db; bl += 256) {
+    const int32_t bf = S.b0 + bl;
+    double acc = 0.0;
+    for (int t = 0; t < kc.nt; t++) {
+      const double c = kc.ct[t];
+      const int32_t *lp = kc.lptr + (int64_t)t * (kc.na + 1), *rp = kc.rptr + (int64_t)t * (kc.nb + 1);
+      for (int32_t e = lp[i]; e < lp[i + 1]; e++) {
+        const int32_t a = kc.lidx[e];
+        const int sa = kc.asec[a];
+        const double pl = c * kc.lval[e];
+        for (int32_t f = rp[bf]; f < rp[bf + 1]; f++) {
+          const int32_t b = kc.ridx[f];
+          if (kc.bsec[b] != sa) continue;
+          const double p = pl * kc.rval[f];
+          if (!(fabs(p) > 1.0e-15)) continue;
+          const KronSector T = kc.sec[sa];
+          acc += p * x[T.off + (int64_t)(a - T.a0) * T.db + (b - T.b0)];
+        }
       }
     }
+    zr[bl] = acc;
   }
-  z[r] = s;
 }
 
 // One TM x TN tile of one of a sector's two dense products: the left one (M_L X, k over
@@ -320,9 +319,8 @@ bool right_kind(int k) { return k == K_RR || k == K_RL || k == K_RRRL; }
 }  // namespace
 
 void KronDev::product(const double *x, hipStream_t s) const {
-  const KronCross kc{n,      sb->d,  sb->LL.p, sb->LR.p, sb->RR.p, sb->RL.p,    aidx.p, bidx.p, asec.p,
-                     bsec.p, sec.p,  sb->terms.p, sb->ops.p, sb->ele_ptrs.p, cross.p, ncross};
-  k_kron_cross<<<(n + 255) / 256, 256, 0, s>>>(kc, x, z.p);
+  const KronCross kc{ncross, na, nb, ct.p, lptr.p, lidx.p, rptr.p, ridx.p, lval.p, rval.p, asec.p, bsec.p, sec.p};
+  if (na > 0) k_kron_cross<<<na, 256, 0, s>>>(kc, x, z.p);
   const int shape = kKronShape[variant];
   const int nt = ntiles[shape];
   if (nt > 0) {
@@ -467,6 +465,97 @@ int dmrgx_superblock_kron(const dmrgx_superblock *S, dmrgx_kron **out) {
       else cross.push_back((int32_t)i);
     }
     K.ncross = (int32_t)cross.size();
+    // the coupling terms' per-state lists (see k_kron_cross), from host copies of the operators
+    {
+      auto op_host = [&](int id, std::vector<int64_t> &r, std::vector<int32_t> &c, std::vector<double> &v) {
+        r.resize(S->op_row[id].count);
+        ctx().copy_out(r.data(), S->op_row[id].p, sizeof(int64_t) * r.size());
+        const int64_t nnz = r.back();
+        c.resize(std::max<int64_t>(nnz, 1));
+        v.resize(std::max<int64_t>(nnz, 1));
+        if (nnz > 0) {
+          ctx().copy_out(c.data(), S->op_col[id].p, sizeof(int32_t) * nnz);
+          ctx().copy_out(v.data(), S->op_val[id].p, sizeof(double) * nnz);
+        }
+      };
+      std::vector<double> ct;
+      std::vector<int32_t> lptr, lidx, rptr, ridx;
+      std::vector<double> lval, rval;
+      for (int32_t ci : cross) {
+        const DTerm &t = S->hterms[ci];
+        std::vector<int64_t> r1, r2;
+        std::vector<int32_t> c1, c2;
+        std::vector<double> v1, v2;
+        op_host(t.op1, r1, c1, v1);
+        op_host(t.op2, r2, c2, v2);
+        auto par = [&](int e, int state) { return S->hele[e][state]; };
+        // parity of the row's LLLR part and of its RRRL part in the term's sign rule
+        auto left_parity = [&](int ll, int lr) {
+          if (t.sign_mode == 0) return 0;
+          switch (t.kind) {
+            case K_LLRL: return par(t.e0, ll) + par(t.e2, lr);
+            case K_LLRR: return par(t.e0, ll) + par(t.e1, lr);
+            case K_LRRL: return par(t.e1, lr);
+            case K_LRRR: return par(t.e0, lr);
+            default: return 0;
+          }
+        };
+        auto right_parity = [&](int rr) {
+          if (t.sign_mode == 0) return 0;
+          switch (t.kind) {
+            case K_LLRL: return par(t.e1, rr);
+            case K_LRRL: return par(t.e0, rr);
+            default: return 0;
+          }
+        };
+        ct.push_back((t.sign_mode == 1 ? -1.0 : 1.0) * t.coeff);
+        const int32_t base_l = (int32_t)lidx.size(), base_r = (int32_t)ridx.size();
+        (void)base_l;
+        (void)base_r;
+        for (int32_t i = 0; i < K.na; i++) {
+          lptr.push_back((int32_t)lidx.size());
+          const int q[2] = {aLL[i], aLR[i]};
+          const double sg = (left_parity(q[0], q[1]) % 2 == 0) ? 1.0 : -1.0;
+          for (int64_t e = r1[q[t.s1]]; e < r1[q[t.s1] + 1]; e++) {
+            int qq[2] = {q[0], q[1]};
+            qq[t.s1] = c1[e];
+            const int32_t a = aidx[(size_t)qq[0] * d + qq[1]];
+            if (a < 0) continue;
+            lidx.push_back(a);
+            lval.push_back(sg * v1[e]);
+          }
+        }
+        lptr.push_back((int32_t)lidx.size());
+        for (int32_t i = 0; i < K.nb; i++) {
+          rptr.push_back((int32_t)ridx.size());
+          const int q[4] = {0, 0, bRR[i], bRL[i]};
+          const double sg = (right_parity(q[2]) % 2 == 0) ? 1.0 : -1.0;
+          for (int64_t e = r2[q[t.s2]]; e < r2[q[t.s2] + 1]; e++) {
+            int qq[4] = {q[0], q[1], q[2], q[3]};
+            qq[t.s2] = c2[e];
+            const int32_t b = bidx[(size_t)qq[2] * d + qq[3]];
+            if (b < 0) continue;
+            ridx.push_back(b);
+            rval.push_back(sg * v2[e]);
+          }
+        }
+        rptr.push_back((int32_t)ridx.size());
+      }
+      if ((int64_t)lidx.size() > INT32_MAX / 2 || (int64_t)ridx.size() > INT32_MAX / 2)
+        throw Failure(DMRGX_ERR_ARG, "coupling-term lists too long");
+      auto upl = [](auto &buf, const auto &h) {
+        using T = typename std::decay_t<decltype(h)>::value_type;
+        buf.resize(std::max<size_t>(h.size(), 1));
+        if (!h.empty()) ctx().copy_in(buf.p, h.data(), sizeof(T) * h.size());
+      };
+      upl(K.ct, ct);
+      upl(K.lptr, lptr);
+      upl(K.lidx, lidx);
+      upl(K.lval, lval);
+      upl(K.rptr, rptr);
+      upl(K.ridx, ridx);
+      upl(K.rval, rval);
+    }
     auto up = [](auto &buf, const auto &h) {
       using T = typename std::decay_t<decltype(h)>::value_type;
       buf.resize(std::max<size_t>(h.size(), 1));
