@@ -835,12 +835,13 @@ struct KronSector {
 };
 // Tile shapes of the dense sector products and the product kernels over them: variant 0
 // the FP64 vector form on 64 x 64 tiles, 1-4 the FP64 matrix cores on 64 x 64, 128 x 128,
-// 64 x 128 and 128 x 64 tiles.
+// 64 x 128 and 128 x 64 tiles, 5 the matrix cores on 64 x 64 tiles with k-slabs of 32, 6 as 1
+// held to four wavefronts per SIMD.
 constexpr int kKronShapes = 4;
 inline constexpr int kKronTM[kKronShapes] = {64, 128, 64, 128};
 inline constexpr int kKronTN[kKronShapes] = {64, 128, 128, 64};
-constexpr int kKronVariants = 5;
-inline constexpr int kKronShape[kKronVariants] = {0, 0, 1, 2, 3};
+constexpr int kKronVariants = 7;
+inline constexpr int kKronShape[kKronVariants] = {0, 0, 1, 2, 3, 0, 0};
 struct KronDev {
   int32_t n = 0, nsec = 0, ncross = 0, na = 0, nb = 0;
   int32_t ntiles[kKronShapes] = {0, 0, 0, 0};

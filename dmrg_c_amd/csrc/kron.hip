@@ -36,7 +36,6 @@
 namespace dmrgx {
 namespace {
 
-constexpr int kKt = 16;     // k-slab staged in LDS
 
 struct KronBuild {
   int32_t d, nstate_a, nstate_b;
@@ -168,7 +167,8 @@ __global__ __launch_bounds__(256) void k_kron_cross(KronCross kc, const double *
 // One TM x TN tile of one of a sector's two dense products: the left one (M_L X, k over
 // the sector's LLLR states, written to zl) or the right one (X M_R^T, k over its RRRL
 // states, added to z, which holds the coupling terms).  k-slabs of kKt staged in LDS, double
-// buffered: the next slab's global loads are issued before the current slab's products.
+// buffered: the next slab's global loads are issued before the current slab's products
+// (kKt 16, or 32 for a longer product per slab behind the loads).
 // Outside the sector the operands are zero.  MFMA: the four wavefronts split the tile 2 x 2,
 // each computing its (TM/2) x (TN/2) part as blocks of v_mfma_f64_16x16x4 (A fragment: lane
 // l holds A[l % 16][l / 16]; B fragment: B[l / 16][l % 16]; result register v of lane l:
@@ -177,11 +177,11 @@ __global__ __launch_bounds__(256) void k_kron_cross(KronCross kc, const double *
 // columns tx + 16 j).
 typedef double kron_v4d __attribute__((ext_vector_type(4)));
 
-template <int TM, int TN, bool MFMA>
-__global__ __launch_bounds__(256) void k_kron_tile(const KronSector *sec, const int32_t *tiles,
-                                                   const double *__restrict__ ML, const double *__restrict__ MR,
-                                                   const double *__restrict__ x, double *__restrict__ z,
-                                                   double *__restrict__ zl, int xcd) {
+template <int TM, int TN, bool MFMA, int kKt>
+__device__ __forceinline__ void kron_tile_body(const KronSector *sec, const int32_t *tiles,
+                                               const double *__restrict__ ML, const double *__restrict__ MR,
+                                               const double *__restrict__ x, double *__restrict__ z,
+                                               double *__restrict__ zl, int xcd) {
   static_assert(MFMA || (TM == 64 && TN == 64), "the vector form computes 64 x 64 tiles");
   constexpr int PA = kKt * TM / 256, PB = kKt * TN / 256;   // staged elements per thread
   constexpr int BM = TM / 32, BN = TN / 32;                 // 16 x 16 blocks per wavefront
@@ -309,6 +309,19 @@ __global__ __launch_bounds__(256) void k_kron_tile(const KronSector *sec, const 
   }
 }
 
+template <int TM, int TN, bool MFMA, int kKt = 16>
+__global__ __launch_bounds__(256) void k_kron_tile(const KronSector *sec, const int32_t *tiles, const double *ML,
+                                                   const double *MR, const double *x, double *z, double *zl, int xcd) {
+  kron_tile_body<TM, TN, MFMA, kKt>(sec, tiles, ML, MR, x, z, zl, xcd);
+}
+// the 64 x 64 matrix-core tile held to at most 128 registers per lane, four wavefronts per
+// SIMD
+__global__ __launch_bounds__(256, 4) void k_kron_tile_o4(const KronSector *sec, const int32_t *tiles, const double *ML,
+                                                         const double *MR, const double *x, double *z, double *zl,
+                                                         int xcd) {
+  kron_tile_body<64, 64, true, 16>(sec, tiles, ML, MR, x, z, zl, xcd);
+}
+
 __global__ __launch_bounds__(256) void k_zero(int64_t n, double *p) {
   for (int64_t i = blockIdx.x * 256ll + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) p[i] = 0.0;
 }
@@ -339,7 +352,9 @@ void KronDev::product(const double *x, hipStream_t s) const {
       case 1: k_kron_tile<64, 64, true><<<nt, 256, 0, s>>>(sec.p, tl, ML.p, MR.p, x, z.p, zl.p, xcd); break;
       case 2: k_kron_tile<128, 128, true><<<nt, 256, 0, s>>>(sec.p, tl, ML.p, MR.p, x, z.p, zl.p, xcd); break;
       case 3: k_kron_tile<64, 128, true><<<nt, 256, 0, s>>>(sec.p, tl, ML.p, MR.p, x, z.p, zl.p, xcd); break;
-      default: k_kron_tile<128, 64, true><<<nt, 256, 0, s>>>(sec.p, tl, ML.p, MR.p, x, z.p, zl.p, xcd); break;
+      case 4: k_kron_tile<128, 64, true><<<nt, 256, 0, s>>>(sec.p, tl, ML.p, MR.p, x, z.p, zl.p, xcd); break;
+      case 5: k_kron_tile<64, 64, true, 32><<<nt, 256, 0, s>>>(sec.p, tl, ML.p, MR.p, x, z.p, zl.p, xcd); break;
+      default: k_kron_tile_o4<<<nt, 256, 0, s>>>(sec.p, tl, ML.p, MR.p, x, z.p, zl.p, xcd); break;
     }
     if (ev) HIPCHK(hipEventRecord(ev->second, s));
   }

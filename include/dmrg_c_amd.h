@@ -663,9 +663,10 @@ int dmrgx_superblock_kron(const dmrgx_superblock *S, dmrgx_kron **out);
  * product (any pointer may be null) */
 int dmrgx_kron_info(const dmrgx_kron *K, int32_t *n, int32_t *sectors, int32_t *coupling_terms, double *flops);
 /* kernel of the dense sector products: 0 FP64 vector units on 64 x 64 tiles; FP64 matrix
- * cores on 1: 64 x 64, 2: 128 x 128, 3: 64 x 128, 4: 128 x 64 tiles (default 1); plus 5 for
- * the same kernels with consecutive tiles (one sector's) dealt to the workgroups of one XCD.
- * The results agree to rounding between the variants. */
+ * cores on 1: 64 x 64, 2: 128 x 128, 3: 64 x 128, 4: 128 x 64 tiles, 5: 64 x 64 tiles with
+ * k-slabs of 32, 6: as 1 with at most 128 registers per lane (default 1); plus 7 for the same
+ * kernels with consecutive tiles (one sector's) dealt to the workgroups of one XCD.  The
+ * results agree to rounding between the variants. */
 int dmrgx_kron_set_kernel(dmrgx_kron *K, int32_t variant);
 /* times every variant (5 products each after one warm-up) and keeps the fastest */
 int dmrgx_kron_autotune(dmrgx_kron *K, int32_t *variant_out, double *us_out);
