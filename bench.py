@@ -61,7 +61,7 @@ PEAK_HBM_GBS = 8000.0
 PEAK_FP64_MFMA_TFLOPS = 78.6   # MI355X dense FP64 matrix peak
 KRON_VARIANTS = ["FP64 vector units, 64 x 64 tiles", "FP64 MFMA, 64 x 64 tiles", "FP64 MFMA, 128 x 128 tiles",
                  "FP64 MFMA, 64 x 128 tiles", "FP64 MFMA, 128 x 64 tiles", "FP64 MFMA, 64 x 64 tiles, k-slabs of 32",
-                 "FP64 MFMA, 64 x 64 tiles, four wavefronts per SIMD"]
+                 "FP64 MFMA, 64 x 64 tiles, four wavefronts per SIMD", "FP64 MFMA, 128 x 128 tiles, two wavefronts per SIMD"]
 SPMV_FORMATS = [("CSR", "k_apply<CsrRows<double>> (CSR SpMV + fused Lanczos/CG epilogue)"),
                 ("value dictionary", "k_apply<DictRows> (value-dictionary CRS SpMV + fused Lanczos/CG epilogue)"),
                 ("delta columns", "k_apply<DeltaRows<double>> (delta-column CRS SpMV + fused Lanczos/CG epilogue)"),

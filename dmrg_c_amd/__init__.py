@@ -728,7 +728,7 @@ class KronOperator:
         check(L.dmrgx_kron_info(h, C.byref(n), C.byref(ns), C.byref(nc), C.byref(fl)))
         self.n, self.sectors, self.coupling_terms, self.flops = n.value, ns.value, nc.value, fl.value
 
-    VARIANTS = 14
+    VARIANTS = 16
 
     def set_kernel(self, variant: int):
         check(lib().dmrgx_kron_set_kernel(self.handle, variant))

@@ -660,6 +660,14 @@ int ground_state_op(const Operator &op, const dmrgx_matrix *A, const dmrgx_solve
     o.min_step = p->diag_min_step;
     o.max_step = p->diag_max_step;
     o.type = p->lan_type;
+    if (p->lan_type < 0 || p->lan_type > 2) throw Failure(DMRGX_ERR_ARG, "lan_type must be 0 (Normal), 1 (Tight) or 2 (Residual)");
+    if (p->lan_type == 2) {
+      // "Residual": stop on the Ritz pair's residual bound, at which the L1 residual test
+      // (at most sqrt(n) times the 2-norm) passes with room to spare
+      o.type = 0;
+      o.res_tol = 0.5 * p->ii_acc / std::sqrt((double)std::max<int32_t>(op.n, 1));
+      o.res_abs = true;
+    }
     const bool mixed = p->precision == 1;
     const bool two_phase = p->precision == 2;
     if (p->precision != 0 && !mixed && !two_phase)
@@ -670,6 +678,7 @@ int ground_state_op(const Operator &op, const dmrgx_matrix *A, const dmrgx_solve
       ensure_fp32(*A);
       o.fp32 = true;
     }
+    if (two_phase && p->lan_type == 2) throw Failure(DMRGX_ERR_ARG, "precision 2 has its own stopping rule (lan_type 0)");
     if (two_phase) o.res_tol = p->mixed_switch > 0.0 ? p->mixed_switch : 1e-3;
     if (op.n == 1) {
       double h;

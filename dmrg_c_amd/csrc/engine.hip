@@ -1235,7 +1235,8 @@ int dmrgx_engine_create(const dmrgx_engine_opts *o, dmrgx_engine **out) {
 int dmrgx_engine_submit(dmrgx_engine *E, dmrgx_matrix *A, int take_ownership, const dmrgx_solve_params *p,
                         uint64_t tag, int64_t *request_id) {
   if (!E || !A || !p) return DMRGX_ERR_ARG;
-  if (p->diag_max_step <= p->diag_min_step || (p->precision != 0 && p->precision != 1) || p->guess)
+  if (p->diag_max_step <= p->diag_min_step || (p->precision != 0 && p->precision != 1) || p->guess ||
+      p->lan_type < 0 || p->lan_type > 1)
     return DMRGX_ERR_ARG;
   return run_status([&] {
     auto s = std::make_unique<Slot>();

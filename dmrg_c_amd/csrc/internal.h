@@ -836,12 +836,12 @@ struct KronSector {
 // Tile shapes of the dense sector products and the product kernels over them: variant 0
 // the FP64 vector form on 64 x 64 tiles, 1-4 the FP64 matrix cores on 64 x 64, 128 x 128,
 // 64 x 128 and 128 x 64 tiles, 5 the matrix cores on 64 x 64 tiles with k-slabs of 32, 6 as 1
-// held to four wavefronts per SIMD.
+// held to four wavefronts per SIMD, 7 as 2 held to two.
 constexpr int kKronShapes = 4;
 inline constexpr int kKronTM[kKronShapes] = {64, 128, 64, 128};
 inline constexpr int kKronTN[kKronShapes] = {64, 128, 128, 64};
-constexpr int kKronVariants = 7;
-inline constexpr int kKronShape[kKronVariants] = {0, 0, 1, 2, 3, 0, 0};
+constexpr int kKronVariants = 8;
+inline constexpr int kKronShape[kKronVariants] = {0, 0, 1, 2, 3, 0, 0, 1};
 struct KronDev {
   int32_t n = 0, nsec = 0, ncross = 0, na = 0, nb = 0;
   int32_t ntiles[kKronShapes] = {0, 0, 0, 0};
@@ -1015,8 +1015,9 @@ struct LanczosOpts {
   bool fp32 = false;
   bool dense_small = true;  // n < 1000: residual-bound stop standing in for the dense dsyev_ branch
   // > 0: stop instead when the Ritz pair's residual bound beta_{k+1}|s_k| falls to
-  // res_tol * max(1, |E_k|) (type and acc are then not used)
+  // res_tol * max(1, |E_k|), or to res_tol with res_abs (type and acc are then not used)
   double res_tol = 0.0;
+  bool res_abs = false;
 };
 // vec: device in/out (start on input, Ritz vector on output)
 void lanczos(const Operator &A, const LanczosOpts &o, double *vec, SolveOut &out);

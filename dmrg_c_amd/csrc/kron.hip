@@ -322,6 +322,14 @@ __global__ __launch_bounds__(256, 4) void k_kron_tile_o4(const KronSector *sec, 
   kron_tile_body<64, 64, true, 16>(sec, tiles, ML, MR, x, z, zl, xcd);
 }
 
+// the 128 x 128 matrix-core tile held to two wavefronts per SIMD (at most 256 registers per
+// lane): half the operand traffic per output of the 64 x 64 tile
+__global__ __launch_bounds__(256, 2) void k_kron_tile_o2(const KronSector *sec, const int32_t *tiles, const double *ML,
+                                                         const double *MR, const double *x, double *z, double *zl,
+                                                         int xcd) {
+  kron_tile_body<128, 128, true, 16>(sec, tiles, ML, MR, x, z, zl, xcd);
+}
+
 __global__ __launch_bounds__(256) void k_zero(int64_t n, double *p) {
   for (int64_t i = blockIdx.x * 256ll + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) p[i] = 0.0;
 }
@@ -354,7 +362,8 @@ void KronDev::product(const double *x, hipStream_t s) const {
       case 3: k_kron_tile<64, 128, true><<<nt, 256, 0, s>>>(sec.p, tl, ML.p, MR.p, x, z.p, zl.p, xcd); break;
       case 4: k_kron_tile<128, 64, true><<<nt, 256, 0, s>>>(sec.p, tl, ML.p, MR.p, x, z.p, zl.p, xcd); break;
       case 5: k_kron_tile<64, 64, true, 32><<<nt, 256, 0, s>>>(sec.p, tl, ML.p, MR.p, x, z.p, zl.p, xcd); break;
-      default: k_kron_tile_o4<<<nt, 256, 0, s>>>(sec.p, tl, ML.p, MR.p, x, z.p, zl.p, xcd); break;
+      case 6: k_kron_tile_o4<<<nt, 256, 0, s>>>(sec.p, tl, ML.p, MR.p, x, z.p, zl.p, xcd); break;
+      default: k_kron_tile_o2<<<nt, 256, 0, s>>>(sec.p, tl, ML.p, MR.p, x, z.p, zl.p, xcd); break;
     }
     if (ev) HIPCHK(hipEventRecord(ev->second, s));
   }

@@ -171,11 +171,12 @@ void lanczos(const Operator &A, const LanczosOpts &o, double *vec, SolveOut &out
         } else if (o.res_tol > 0.0) {
           tridiag_vector(k + 1, ha.data(), hb.data() + 1, lam, s.data());
           const double res = bnext * std::fabs(s[k]);
-          conv = res <= o.res_tol * tscale;
+          const double target = o.res_abs ? o.res_tol : o.res_tol * tscale;
+          conv = res <= target;
           if (!conv) {   // progress measure of the chunk prediction below
             if (n_dE++ == 0) dE_first = res;
             dE_last = res;
-            res_target = o.res_tol * tscale;
+            res_target = target;
           }
         } else if (o.type == 1) {
           tridiag_vector(k + 1, ha.data(), hb.data() + 1, lam, s.data());

@@ -550,7 +550,10 @@ typedef struct {
    double diag_acc;         /* Lanczos convergence threshold                     */
    int diag_min_step;
    int diag_max_step;
-   int lan_type;            /* 0 "Normal" |E_k - E_k-1| < acc, 1 "Tight" |s_k| < acc */
+   int lan_type;            /* 0 "Normal" |E_k - E_k-1| < acc, 1 "Tight" |s_k| < acc,
+                                2 "Residual" (no reference counterpart): the Ritz pair's
+                                residual bound beta_{k+1}|s_k| < ii_acc/(2 sqrt(n)), so that
+                                the inverse iteration's residual test usually passes at once */
    int guess;               /* 0: counter-hash start vector (seed), 1: vec_inout    */
    uint64_t seed;
    double ii_acc;           /* inverse iteration: stop when sum|Hv - Ev| < ii_acc */
@@ -664,9 +667,9 @@ int dmrgx_superblock_kron(const dmrgx_superblock *S, dmrgx_kron **out);
 int dmrgx_kron_info(const dmrgx_kron *K, int32_t *n, int32_t *sectors, int32_t *coupling_terms, double *flops);
 /* kernel of the dense sector products: 0 FP64 vector units on 64 x 64 tiles; FP64 matrix
  * cores on 1: 64 x 64, 2: 128 x 128, 3: 64 x 128, 4: 128 x 64 tiles, 5: 64 x 64 tiles with
- * k-slabs of 32, 6: as 1 with at most 128 registers per lane (default 1); plus 7 for the same
- * kernels with consecutive tiles (one sector's) dealt to the workgroups of one XCD.  The
- * results agree to rounding between the variants. */
+ * k-slabs of 32, 6: as 1 with at most 128 registers per lane, 7: as 2 with at most 256
+ * (default 1); plus 8 for the same kernels with consecutive tiles (one sector's) dealt to
+ * the workgroups of one XCD.  The results agree to rounding between the variants. */
 int dmrgx_kron_set_kernel(dmrgx_kron *K, int32_t variant);
 /* times every variant (5 products each after one warm-up) and keeps the fastest */
 int dmrgx_kron_autotune(dmrgx_kron *K, int32_t *variant_out, double *us_out);

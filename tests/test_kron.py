@@ -24,10 +24,10 @@ def _ref_full(s):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("variant", range(14), ids=["vec64", "mfma64", "mfma128", "mfma64x128", "mfma128x64", "mfma64k32",
-                                                     "mfma64o4", "vec64-xcd", "mfma64-xcd", "mfma128-xcd",
+@pytest.mark.parametrize("variant", range(16), ids=["vec64", "mfma64", "mfma128", "mfma64x128", "mfma128x64", "mfma64k32",
+                                                     "mfma64o4", "mfma128o2", "vec64-xcd", "mfma64-xcd", "mfma128-xcd",
                                                      "mfma64x128-xcd", "mfma128x64-xcd", "mfma64k32-xcd",
-                                                     "mfma64o4-xcd"])
+                                                     "mfma64o4-xcd", "mfma128o2-xcd"])
 @pytest.mark.parametrize("path", SNAPS + BIG, ids=golden_ids(SNAPS + BIG))
 def test_gpu_kron_product_equals_reference_crs(path, variant):
     import dmrg_c_amd as D
