@@ -829,7 +829,7 @@ def run_serve(D, rate, duration, seed=0, chunk_ticks=1, precision=1, snapshots=N
                     "assembled on the device at arrival"}
 
 
-def run_kron(D, L, name, steps, warmup):
+def run_kron(D, L, name, steps, warmup, lan_type=0, profile=True):
     """Solves/s of the superblock workload on the Kronecker form of its Hamiltonian: per solve
     the form's build from the resident superblock (sector tables, the dense LLLR and RRRL
     sector matrices from the term list) + Lanczos + inverse iteration/CG (the reference's
@@ -840,7 +840,7 @@ def run_kron(D, L, name, steps, warmup):
     snap = snapshot.load(path)
     e_ref = float(snap.meta["gs_energy"][0])
     params = dict(diag_acc=1e-14, diag_max_step=800, cg_acc=1e-7, cg_max_step=2000, ii_acc=1e-9,
-                  ii_diag_add=1e-11, ii_max_step=3, seed=1)
+                  ii_diag_add=1e-11, ii_max_step=3, seed=1, lan_type=lan_type)
     sb = D.Superblock(snap)
     K = sb.kron()
     variant, tuned_us = K.autotune()
@@ -858,7 +858,8 @@ def run_kron(D, L, name, steps, warmup):
     for i in range(warmup):
         E, st = solve()
         log(f"kron warmup {i}: E={E:.15f} lanczos={st.lanczos_steps} cg={st.cg_steps}")
-    D.check(L.dmrgx_profile_begin_sampled(PROFILE_EVERY))
+    if profile:
+        D.check(L.dmrgx_profile_begin_sampled(PROFILE_EVERY))
     lat, errs, lan, cg, gerr = [], [], [], [], []
     t0 = time.perf_counter()
     for i in range(steps):
@@ -872,9 +873,14 @@ def run_kron(D, L, name, steps, warmup):
         log(f"kron step {i}: {lat[-1] * 1e3:.2f} ms  E={E:.15f}  dE/E={errs[-1]:.2e}")
     total = time.perf_counter() - t0
     ms, launches, flops = C.c_double(), C.c_int64(), C.c_double()
-    L.dmrgx_profile_end.argtypes = [C.POINTER(C.c_double), C.POINTER(C.c_int64), C.POINTER(C.c_double)]
-    D.check(L.dmrgx_profile_end(C.byref(ms), C.byref(launches), C.byref(flops)))
+    if profile:
+        L.dmrgx_profile_end.argtypes = [C.POINTER(C.c_double), C.POINTER(C.c_int64), C.POINTER(C.c_double)]
+        D.check(L.dmrgx_profile_end(C.byref(ms), C.byref(launches), C.byref(flops)))
     sb.free()
+    if not profile:
+        return {"solves_per_s": round(steps / total, 4), "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 3),
+                "max_rel_energy_error": float(max(errs)), "max_gs_error": float(max(gerr)),
+                "lanczos_steps": int(np.median(lan)), "cg_steps": int(np.median(cg))}
     avg_s = ms.value / 1e3 / max(launches.value, 1)
     fl = flops.value / max(launches.value, 1)
     achieved = fl / avg_s / 1e12 if avg_s > 0 else 0.0
@@ -1179,6 +1185,7 @@ def main():
                    "lower -> full expansion, sliced-form build and the CRS SpMV per product; the same Lanczos + "
                    "inverse iteration/CG")
     crs["over"] = "kron value / crs value = " + str(round(kron["value"] / head["value"], 2))
+    crs["spmv_schedule_note"] = "the SpMV roofline of this form is spmv_roofline"
     line = {
         "metric": "ground-state solves/s + p99 latency at |dE/E|<=1e-9; SpMV HBM GB/s vs peak",
         "value": kron["value"],
@@ -1194,6 +1201,11 @@ def main():
         "data": kron["data"],
         "config": kron["config"],
         "forms": {"crs_fp64": crs,
+                  "kron_residual_stop": dict(run_kron(D, L, wl, max(2, args.steps // 2), 1, lan_type=2, profile=False),
+                                             note="the Kronecker form with lan_type 2 (no reference counterpart): "
+                                                  "Lanczos until the Ritz pair's residual bound is below "
+                                                  "ii_acc/(2 sqrt(n)), then the reference's L1 residual test (CG only "
+                                                  "if it fails); FP64"),
                   "fp32_refined": dict(head["fp32_refined"], over="CRS FP64 path"),
                   "fp32_refined_inverse_iteration": dict(head["fp32_refined_inverse_iteration"],
                                                          over="CRS FP64 path")},

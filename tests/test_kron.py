@@ -107,3 +107,31 @@ def test_gpu_kron_rejects_fp32():
     finally:
         K.free()
         sb.free()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("path", SNAPS[-3:] + BIG[:1], ids=golden_ids(SNAPS[-3:] + BIG[:1]))
+def test_gpu_residual_stop(path):
+    """lan_type 2 ("Residual", no reference counterpart): Lanczos until the Ritz pair's residual
+    bound is below ii_acc / (2 sqrt(n)); the reference's L1 residual test then decides.  The
+    energy within the FP64 bound of the reference's, on the Kronecker form and the CRS."""
+    import dmrg_c_amd as D
+    from dmrg_c_amd import snapshot
+    s = snapshot.load(path)
+    e_ref = float(s.meta["gs_energy"][0])
+    sb = D.Superblock(s)
+    K = sb.kron()
+    A = sb.assemble()
+    try:
+        for solver in (K, A):
+            E, vec, st = solver.ground_state(lan_type=2)
+            assert abs(E - e_ref) <= 1e-10 * max(1.0, abs(e_ref)), (E, e_ref)
+            assert st.gs_error < 1e-9
+            _, _, st0 = solver.ground_state()
+            assert st.spmv_calls <= st0.spmv_calls + 2
+        with pytest.raises(D.DmrgxError):
+            A.ground_state(lan_type=3)
+    finally:
+        K.free()
+        A.free()
+        sb.free()
