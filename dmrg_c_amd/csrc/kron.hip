@@ -200,25 +200,30 @@ __device__ __forceinline__ void kron_tile_body(const KronSector *sec, const int3
   const int64_t lda = right ? S.db : S.da;
   const double *Bop = right ? MR + S.mr : X;           // left: B[k][j] = X[k db + c0 + j]
   double ra[PA], rb[PB];                               // right: B[k][j] = M_R[(c0 + j) db + k]
+  // staging geometry, fixed over the k loop: A element u of this thread is row
+  // tid / kKt + (256 / kKt) u of the tile at column tid % kKt of the slab (right B the same
+  // with rows of M_R); left B element u is k-row tid / TN + (256 / TN) u at column tid % TN
+  const int kA = threadIdx.x % kKt, iA = threadIdx.x / kKt;
+  const double *pA = Aop + (int64_t)(r0 + iA) * lda + kA;
+  const int64_t sA = (int64_t)(256 / kKt) * lda;
+  const int kB = right ? kA : threadIdx.x / TN;
+  const int jB = right ? iA : threadIdx.x % TN;
+  const double *pB = right ? Bop + (int64_t)(c0 + jB) * S.db + kA : Bop + (int64_t)kB * S.db + c0 + jB;
+  const int64_t sB = right ? (int64_t)(256 / kKt) * S.db : (int64_t)(256 / TN) * S.db;
   auto fetch = [&](int32_t k0) {
 #pragma unroll
     for (int u = 0; u < PA; u++) {
-      const int e = threadIdx.x + 256 * u;
-      const int i = e / kKt, k = e % kKt;              // consecutive threads: consecutive k of a row
-      const int32_t rr = r0 + i, kk = k0 + k;
-      ra[u] = (rr < S.da && kk < K) ? Aop[rr * lda + kk] : 0.0;
+      const bool ok = r0 + iA + (256 / kKt) * u < S.da && k0 + kA < K;
+      ra[u] = ok ? pA[u * sA + k0] : 0.0;
     }
 #pragma unroll
     for (int u = 0; u < PB; u++) {
-      const int e = threadIdx.x + 256 * u;
       if (right) {
-        const int j = e / kKt, k = e % kKt;
-        const int32_t cc = c0 + j, kk = k0 + k;
-        rb[u] = (cc < S.db && kk < K) ? Bop[(int64_t)cc * S.db + kk] : 0.0;
+        const bool ok = c0 + jB + (256 / kKt) * u < S.db && k0 + kA < K;
+        rb[u] = ok ? pB[u * sB + k0] : 0.0;
       } else {
-        const int k = e / TN, j = e % TN;              // consecutive threads: consecutive columns of X
-        const int32_t kk = k0 + k, cc = c0 + j;
-        rb[u] = (kk < K && cc < S.db) ? Bop[(int64_t)kk * S.db + cc] : 0.0;
+        const bool ok = c0 + jB < S.db && k0 + kB + (256 / TN) * u < K;
+        rb[u] = ok ? pB[u * sB + (int64_t)k0 * S.db] : 0.0;
       }
     }
   };
