@@ -629,8 +629,10 @@ def run_density(D, reps: int, cpu: bool):
         L.dmrgx_free_system_info(out)
     tr = {"max_dim": 400, "dim_renorm": info["dim_renorm"], "tr_error": info["tr_error"],
           "calls_per_s": round(1.0 / float(np.median(ts)), 2), "ms_per_call": round(1e3 * float(np.median(ts)), 3),
-          "note": "density matrices, rocSOLVER dsyevd per block, selection and transformation matrices; "
-                  "host-inclusive, result back in host memory"}
+          "note": "density matrices, block eigenpairs (blocks up to 192 states: the library's own "
+                  "solver, eigenvalues of all states and eigenvectors of the kept ones; larger: "
+                  "rocSOLVER dsyevd), selection and transformation matrices; host-inclusive, result back "
+                  "in host memory"}
     ref_lib = os.path.join(REPO, "oracle", "_ref", "libdmrgref.so")
     if cpu and os.path.exists(ref_lib):
         os.environ["MKL_NUM_THREADS"] = "1"   # read by LAPACK at its first call: one core, as reported

@@ -545,6 +545,14 @@ def system_info(B: "DMRG_BASIS", n_qn: int, q1, q2, vec: np.ndarray, max_dim: in
         L.dmrgx_free_system_info(out)
 
 
+def set_block_eigensolver(mode: int) -> None:
+    """dmrgx_set_block_eigensolver: the truncation's eigensolver for blocks up to 192 states,
+    0 the library's own (default), 1 rocSOLVER dsyevd.  Process-wide."""
+    L = lib()
+    L.dmrgx_set_block_eigensolver.argtypes = [C.c_int]
+    check(L.dmrgx_set_block_eigensolver(mode))
+
+
 class WholeBasis:
     """dmrgx_whole_basis: the whole-space inverse table of DMRG_WHOLE_BASIS_Q1..Q4 on the
     device plus the sectors registered with add_sector(); v_m() is DMRG_V_M_*_Q1..Q4."""

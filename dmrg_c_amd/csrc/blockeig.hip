@@ -1,0 +1,646 @@
+// Eigenpairs of the block density matrices of a truncation step, for blocks up to
+// kBlockEigMax (dmrg/DMRG_GET_SYSTEM_INFO_Q2.c:22-46 calls LAPACK's dsyev per block through
+// sml/LAPACK_DSYEV.c and keeps the eigenvectors of the largest eigenvalues only).  The route
+// is LAPACK's dsyevx one (dsytrd, dstebz, dstein, dormtr), with eigenvectors formed for the
+// kept states only, in two launches around the host's selection:
+//  1. k_eig_values, one workgroup per block.  The block's lower triangle, packed, in LDS;
+//     Householder reduction to tridiagonal form, last row first: for i = n-1 .. 2 the
+//     reflector H_i = I - v v^T / h, v from row i's first i entries, zeroes them below the
+//     subdiagonal, and the leading i x i part B becomes B - v w^T - w v^T with p = B v / h,
+//     w = p - (v.p / 2h) v; v stays in row i.  Then every eigenvalue of the tridiagonal T by
+//     bisection on the Sturm count (pivots of the LDL^T factorisation of T - x I, floored
+//     at LAPACK's pivmin as in dstebz), one eigenvalue per thread, each step counting at
+//     the three quarter points, to an absolute width of 2 eps ||T||.  Out: the reflectors,
+//     T's diagonal and subdiagonal, the eigenvalues ascending.
+//  2. k_eig_vectors, one workgroup per block with kept states.  Per kept eigenvalue (one
+//     thread each): three steps of inverse iteration on T - lambda I (Gaussian elimination
+//     with partial pivoting, pivots below eps ||T|| raised to it, as LAPACK's dlagtf/dlagts),
+//     from a fixed pseudo-random start; modified Gram-Schmidt over the block's kept vectors
+//     in ascending eigenvalue order (a second pass when a vector lost more than half its
+//     norm to it), then x = H_{n-1} ... H_2 y, both with one wavefront per vector.
+// Eigenvalues agree with dsyev's to ~eps ||block||; eigenvectors up to sign, and up to a
+// rotation within a set of (nearly) equal eigenvalues.
+
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <cmath>
+
+#include "internal.h"
+
+namespace dmrgx {
+namespace {
+
+constexpr int kEigThreads = 256;
+constexpr int kLdsDoubles = 160 * 1024 / 8;             // LDS of one workgroup, in doubles
+constexpr int kVecFixed = 4 * kBlockEigMax + 8;         // stage 2's LDS besides the vectors (d, e, nrm, onrm, red)
+constexpr int64_t kVecLdsMax = kLdsDoubles - kVecFixed;  // vectors (n x k) held in LDS up to this
+static_assert(kVecFixed + kBlockEigMax * (kBlockEigMax + 1) / 2 <= kLdsDoubles, "reflectors must fit in LDS");
+
+__device__ __forceinline__ int tri(int r) { return r * (r + 1) / 2; }
+
+// max of v over the workgroup (every thread gets it); red: kEigThreads / 64 doubles
+__device__ double block_max_all(double v, double *red) {
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off));
+  __syncthreads();
+  if (threadIdx.x % 64 == 0) red[threadIdx.x / 64] = v;
+  __syncthreads();
+  double t = red[0];
+  for (int k = 1; k < kEigThreads / 64; k++) t = fmax(t, red[k]);
+  return t;
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  return v;
+}
+
+// Eight sums over the wavefront at once: in, lane l's partial sums of eight quantities; out,
+// the full sum of quantity ((l >> 5) & 1) * 4 + ((l >> 4) & 1) * 2 + ((l >> 3) & 1), held by
+// lane l (lanes 32 a + 16 b + 8 c + 0..7 hold quantity 4 a + 2 b + c).  Halving the set at
+// each of the first three exchanges: 10 exchanges instead of 48.
+__device__ __forceinline__ double wave_sum8(const double *x, int lane) {
+  const bool b5 = lane & 32, b4 = lane & 16, b3 = lane & 8;
+  double e4[4], e2[2];
+#pragma unroll
+  for (int q = 0; q < 4; q++) e4[q] = (b5 ? x[q + 4] : x[q]) + __shfl_xor(b5 ? x[q] : x[q + 4], 32);
+#pragma unroll
+  for (int q = 0; q < 2; q++) e2[q] = (b4 ? e4[q + 2] : e4[q]) + __shfl_xor(b4 ? e4[q] : e4[q + 2], 16);
+  double e1 = (b3 ? e2[1] : e2[0]) + __shfl_xor(b3 ? e2[0] : e2[1], 8);
+  e1 += __shfl_xor(e1, 4);
+  e1 += __shfl_xor(e1, 2);
+  e1 += __shfl_xor(e1, 1);
+  return e1;
+}
+
+// the value of x in lane l (uniform l)
+__device__ __forceinline__ double read_lane(double x, int l) {
+  const long long b = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_readlane((int)b, l), hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+
+// 1 / q to ~1 ulp: the hardware estimate and one Newton step (|q| >= the pivot floor, so
+// no overflow)
+__device__ __forceinline__ double recip(double q) {
+  double r = __builtin_amdgcn_rcp(q);
+  const double e = fma(-q, r, 1.0);
+  return fma(r, e, r);
+}
+
+struct ValJob {
+  int32_t n;
+  int64_t a_off;   // the block's row-major matrix in A
+  int64_t p_off;   // its packed lower triangle (reflectors) in H: tri(n) entries
+  int64_t r_off;   // its first row in D, E, HV, W
+};
+
+// Stage 1's threads as a 16 x 16 grid, (tr, tc) = (tid / 16, tid % 16): thread (tr, tc) holds
+// the block's lower-triangle entries (tr + 16 a, tc + 16 b), b <= a, in registers, slot
+// a (a + 1) / 2 + b; the 16 threads of one tr are one row of 16 lanes of a wavefront.
+constexpr int kGrid = 16;
+constexpr int kSlabs = kBlockEigMax / kGrid;
+constexpr int kSlots = kSlabs * (kSlabs + 1) / 2;
+static_assert(kBlockEigMax % kGrid == 0 && kGrid * kGrid == kEigThreads, "stage 1 thread grid");
+__host__ __device__ constexpr int slot(int a, int b) { return a * (a + 1) / 2 + b; }
+
+template <int kCtrl>
+__device__ __forceinline__ double dpp(double x) {
+  const long long v = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_mov_dpp((int)v, kCtrl, 0xF, 0xF, true);
+  const int hi = __builtin_amdgcn_mov_dpp((int)(v >> 32), kCtrl, 0xF, 0xF, true);
+  return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+
+// sum over the 16 lanes of a row of the wavefront (every lane of the row gets it): lane
+// pairs l^1, l^2, then the mirrors within 8 and within 16 lanes
+__device__ __forceinline__ double sum16(double x) {
+  x += dpp<0xB1>(x);    // quad_perm [1, 0, 3, 2]
+  x += dpp<0x4E>(x);    // quad_perm [2, 3, 0, 1]
+  x += dpp<0x141>(x);   // row_half_mirror
+  x += dpp<0x140>(x);   // row_mirror
+  return x;
+}
+
+__global__ __launch_bounds__(kEigThreads) void k_eig_values(const ValJob *jobs, const double *__restrict__ A,
+                                                            double *__restrict__ H, double *__restrict__ D,
+                                                            double *__restrict__ E, double *__restrict__ HV,
+                                                            double *__restrict__ W) {
+  extern __shared__ double sm[];
+  const ValJob J = jobs[blockIdx.x];
+  const int n = J.n, tid = threadIdx.x, lane = tid & 63;
+  const int tr = tid / kGrid, tc = tid % kGrid, na = (n + kGrid - 1) / kGrid;
+  // vb: row i of the current step (two buffers, alternating); pr, pc: partial products
+  double *d = sm, *e = d + n, *hv = e + n, *vb = hv + n, *pw = vb + 2 * n, *pr = pw + n, *pc = pr + n,
+         *red = pc + kGrid * n;
+  double Lr[kSlots];
+#pragma unroll
+  for (int a = 0; a < kSlabs; a++)
+#pragma unroll
+    for (int b = 0; b <= a; b++) {
+      const int r = tr + kGrid * a, c = tc + kGrid * b;
+      Lr[slot(a, b)] = (a < na && r < n && c <= r) ? A[J.a_off + (int64_t)r * n + c] : 0.0;
+    }
+  // weights of the a == b slots: the diagonal and below in the row products, strictly below
+  // in the column products (tr < tc: above the diagonal, not an entry)
+  const double fdiag = tr >= tc ? 1.0 : 0.0, fstrict = tr > tc ? 1.0 : 0.0;
+  for (int i = n - 1; i >= 2; i--) {
+    double *vv = vb + (i & 1) * n;
+    const int ai = i / kGrid, ti = i % kGrid;
+    if (tr == ti) {   // row i's entries c < i, from the threads holding them
+#pragma unroll
+      for (int a = 0; a < kSlabs; a++)
+        if (a == ai)
+#pragma unroll
+          for (int b = 0; b <= a; b++) {
+            const int c = tc + kGrid * b;
+            if (c < i) vv[c] = Lr[slot(a, b)];
+          }
+    }
+    __syncthreads();
+    double sg = 0.0;
+    for (int k = lane; k < i - 1; k += 64) sg += vv[k] * vv[k];
+    const double sig = wave_sum(sg);   // every wavefront, from LDS
+    const double xl = vv[i - 1];
+    if (sig == 0.0) {   // row i already tridiagonal
+      if (tid == 0) {
+        e[i] = xl;
+        hv[i] = 0.0;
+      }
+      continue;
+    }
+    const double alpha = xl >= 0.0 ? -sqrt(sig + xl * xl) : sqrt(sig + xl * xl);
+    const double h = alpha * alpha - xl * alpha, vl = xl - alpha;
+    if (tid == 0) {
+      e[i] = alpha;
+      hv[i] = h;
+    }
+    // v at this thread's rows and columns: row i's entries, v_{i-1} = xl - alpha, zero from i on
+    double vr[kSlabs], vc[kSlabs];
+#pragma unroll
+    for (int a = 0; a < kSlabs; a++) {
+      const int r = tr + kGrid * a, c = tc + kGrid * a;
+      vr[a] = (a < na && r < i) ? (r == i - 1 ? vl : vv[r]) : 0.0;
+      vc[a] = (a < na && c < i) ? (c == i - 1 ? vl : vv[c]) : 0.0;
+    }
+    if (tr == ti && tc == (i - 1) % kGrid) {   // v kept in row i
+#pragma unroll
+      for (int a = 0; a < kSlabs; a++)
+        if (a == ai)
+#pragma unroll
+          for (int b = 0; b <= a; b++)
+            if (b == (i - 1) / kGrid) Lr[slot(a, b)] = vl;
+    }
+    // p = B v / h, B = S + D + S^T: row sums over (D + S) and column sums over S, per thread,
+    // then the rows' over the 16 threads of a row group and the columns' over the 16 groups
+    double prow[kSlabs], pcol[kSlabs];
+#pragma unroll
+    for (int a = 0; a < kSlabs; a++) prow[a] = pcol[a] = 0.0;
+#pragma unroll
+    for (int a = 0; a < kSlabs; a++)
+#pragma unroll
+      for (int b = 0; b <= a; b++) {
+        if (a >= na) continue;
+        const double x = Lr[slot(a, b)];
+        if (a > b) {
+          prow[a] += x * vc[b];
+          pcol[b] += x * vr[a];
+        } else {
+          prow[a] += fdiag * x * vc[b];
+          pcol[b] += fstrict * x * vr[a];
+        }
+      }
+#pragma unroll
+    for (int a = 0; a < kSlabs; a++) {
+      if (a >= na) continue;
+      const double s = sum16(prow[a]);
+      const int r = tr + kGrid * a, c = tc + kGrid * a;
+      if (tc == 0 && r < i) pr[r] = s;
+      if (c < i) pc[tr * n + c] = pcol[a];
+    }
+    __syncthreads();
+    for (int j = tid; j < i; j += kEigThreads) {
+      double s = pr[j];
+      for (int t = 0; t < kGrid; t++) s += pc[t * n + j];
+      pw[j] = s / h;
+    }
+    __syncthreads();
+    // K = v.p / 2h (every wavefront), w = p - K v at this thread's rows and columns
+    double vp = 0.0;
+    for (int k = lane; k < i; k += 64) vp += (k == i - 1 ? vl : vv[k]) * pw[k];
+    const double K = wave_sum(vp) / (2.0 * h);
+    double wr[kSlabs], wc[kSlabs];
+#pragma unroll
+    for (int a = 0; a < kSlabs; a++) {
+      const int r = tr + kGrid * a, c = tc + kGrid * a;
+      wr[a] = (a < na && r < i) ? pw[r] - K * vr[a] : 0.0;
+      wc[a] = (a < na && c < i) ? pw[c] - K * vc[a] : 0.0;
+    }
+    // B -= v w^T + w v^T (rows and columns from i on: v = w = 0, unchanged)
+#pragma unroll
+    for (int a = 0; a < kSlabs; a++)
+#pragma unroll
+      for (int b = 0; b <= a; b++) {
+        if (a >= na) continue;
+        Lr[slot(a, b)] -= vr[a] * wc[b] + wr[a] * vc[b];
+      }
+  }
+  // T: diagonal d, e[k] = T[k][k-1] (k >= 1); the reflectors (rows) to H
+#pragma unroll
+  for (int a = 0; a < kSlabs; a++)
+#pragma unroll
+    for (int b = 0; b <= a; b++) {
+      const int r = tr + kGrid * a, c = tc + kGrid * b;
+      if (a >= na || r >= n || c > r) continue;
+      H[J.p_off + tri(r) + c] = Lr[slot(a, b)];
+      if (r == c) d[r] = Lr[slot(a, b)];
+      if (r == 1 && c == 0) e[1] = Lr[slot(a, b)];
+    }
+  if (tid == 0) {
+    e[0] = 0.0;
+    hv[0] = 0.0;
+    if (n > 1) hv[1] = 0.0;
+  }
+  __syncthreads();
+  for (int k = tid; k < n; k += kEigThreads) {
+    D[J.r_off + k] = d[k];
+    E[J.r_off + k] = e[k];
+    HV[J.r_off + k] = hv[k];
+  }
+  // Gershgorin interval, pivot floor, squared couplings (in pw)
+  double lo_k = DBL_MAX, hi_k = -DBL_MAX, e2max = 0.0;
+  for (int k = tid; k < n; k += kEigThreads) {
+    const double r = fabs(e[k]) + (k + 1 < n ? fabs(e[k + 1]) : 0.0);
+    lo_k = fmin(lo_k, d[k] - r);
+    hi_k = fmax(hi_k, d[k] + r);
+    pw[k] = e[k] * e[k];
+    e2max = fmax(e2max, pw[k]);
+  }
+  const double gl0 = -block_max_all(-lo_k, red), gu0 = block_max_all(hi_k, red);
+  e2max = block_max_all(e2max, red);   // ends with a barrier: pw complete
+  const double tnorm = fmax(fabs(gl0), fabs(gu0));
+  const double pivmin = DBL_MIN * fmax(1.0, e2max);
+  const double widen = 2.0 * DBL_EPSILON * tnorm * n + 2.0 * pivmin;
+  const double gl = gl0 - widen, gu = gu0 + widen;
+  const double atol = fmax(2.0 * DBL_EPSILON * tnorm, DBL_MIN);
+  // eigenvalue t: the smallest x with more than t eigenvalues at or below it; each step
+  // counts at the three quarter points of [lo, hi] (three independent pivot recurrences)
+  for (int t = tid; t < n; t += kEigThreads) {
+    double lo = gl, hi = gu;
+    while (true) {
+      const double w = hi - lo;
+      const double x1 = lo + 0.25 * w, x2 = lo + 0.5 * w, x3 = lo + 0.75 * w;
+      if (w <= atol || x1 <= lo || x3 >= hi) break;
+      double q1 = d[0] - x1, q2 = d[0] - x2, q3 = d[0] - x3;
+      if (fabs(q1) < pivmin) q1 = -pivmin;
+      if (fabs(q2) < pivmin) q2 = -pivmin;
+      if (fabs(q3) < pivmin) q3 = -pivmin;
+      int c1 = q1 < 0.0, c2 = q2 < 0.0, c3 = q3 < 0.0;
+      for (int k = 1; k < n; k++) {
+        const double dk = d[k], ek = pw[k];
+        q1 = (dk - x1) - ek * recip(q1);
+        q2 = (dk - x2) - ek * recip(q2);
+        q3 = (dk - x3) - ek * recip(q3);
+        if (fabs(q1) < pivmin) q1 = -pivmin;
+        if (fabs(q2) < pivmin) q2 = -pivmin;
+        if (fabs(q3) < pivmin) q3 = -pivmin;
+        c1 += q1 < 0.0;
+        c2 += q2 < 0.0;
+        c3 += q3 < 0.0;
+      }
+      if (c1 > t) hi = x1;
+      else if (c2 > t) {
+        lo = x1;
+        hi = x2;
+      } else if (c3 > t) {
+        lo = x2;
+        hi = x3;
+      } else lo = x3;
+    }
+    W[J.r_off + t] = 0.5 * (lo + hi);
+  }
+}
+
+struct VecJob {
+  int32_t n, k;
+  int64_t p_off, r_off;   // as ValJob
+  int64_t i_off;          // the block's k kept eigenvalue indices (ascending) in IDX
+  int64_t x_off;          // its k eigenvectors, row-major k x n, in X
+  int64_t w_off;          // workspace: 3 n k doubles in U, and n k in Y when not in LDS
+};
+
+// deterministic start vector entry in [-1, 1)
+__device__ __forceinline__ double start_entry(uint32_t c, uint32_t idx) {
+  uint32_t h = c * 0x9E3779B1u ^ (idx + 0x7F4A7C15u) * 0x85EBCA77u;
+  h ^= h >> 15;
+  h *= 0x2C1B3C6Du;
+  h ^= h >> 12;
+  h *= 0x297A2D39u;
+  h ^= h >> 15;
+  return (double)(h >> 8) * (1.0 / 8388608.0) - 1.0;
+}
+
+template <bool kInLds>
+__global__ __launch_bounds__(kEigThreads) void k_eig_vectors(const VecJob *jobs, const double *__restrict__ H,
+                                                             const double *__restrict__ D, const double *__restrict__ E,
+                                                             const double *__restrict__ HV, const double *__restrict__ W,
+                                                             const int32_t *__restrict__ IDX, double *__restrict__ U,
+                                                             double *__restrict__ Yg, double *__restrict__ X,
+                                                             int *fail) {
+  extern __shared__ double sm[];
+  const VecJob J = jobs[blockIdx.x];
+  const int n = J.n, k = J.k, tid = threadIdx.x;
+  double *d = sm, *e = d + n, *nrm = e + n, *red = nrm + 2 * kBlockEigMax;   // nrm, onrm: kBlockEigMax each
+  double *Y = kInLds ? red + 8 : Yg + J.w_off;   // Y[c * k + t]: component c of vector t
+  double *Ut = U + 3 * J.w_off;                  // U[(3c + j) k + t]: row c of vector t's factor
+  __shared__ int bad, again;
+  for (int c = tid; c < n; c += kEigThreads) {
+    d[c] = D[J.r_off + c];
+    e[c] = E[J.r_off + c];
+  }
+  if (tid == 0) {
+    bad = 0;
+    again = 0;
+  }
+  double tn = 0.0;
+  for (int c = tid; c < n; c += kEigThreads) tn = fmax(tn, fabs(d[c]) + fabs(e[c]) + (c + 1 < n ? fabs(e[c + 1]) : 0.0));
+  const double tiny = fmax(DBL_EPSILON * block_max_all(tn, red), DBL_MIN);   // barrier: d, e complete
+  // inverse iteration, one vector per thread
+  for (int t = tid; t < k; t += kEigThreads) {
+    const int idx = IDX[J.i_off + t];
+    const double lam = W[J.r_off + idx];
+    for (int c = 0; c < n; c++) Y[(int64_t)c * k + t] = start_entry((uint32_t)c, (uint32_t)idx);
+    for (int it = 0; it < 3; it++) {
+      double p0 = d[0] - lam, p1 = n > 1 ? e[1] : 0.0, r = Y[t];
+      for (int c = 0; c + 1 < n; c++) {
+        const double l = e[c + 1], a = d[c + 1] - lam, s = c + 2 < n ? e[c + 2] : 0.0;
+        const double rn = Y[(int64_t)(c + 1) * k + t];
+        double *u = Ut + (int64_t)3 * c * k + t;
+        if (fabs(p0) >= fabs(l)) {
+          const double piv = fabs(p0) < tiny ? copysign(tiny, p0) : p0;
+          const double m = l / piv;
+          u[0] = piv;
+          u[k] = p1;
+          u[2 * k] = 0.0;
+          Y[(int64_t)c * k + t] = r;
+          p0 = a - m * p1;
+          p1 = s;
+          r = rn - m * r;
+        } else {
+          const double m = p0 / l;
+          u[0] = fabs(l) < tiny ? copysign(tiny, l) : l;
+          u[k] = a;
+          u[2 * k] = s;
+          Y[(int64_t)c * k + t] = rn;
+          p0 = p1 - m * a;
+          p1 = -m * s;
+          r = r - m * rn;
+        }
+      }
+      // back substitution, then scale to max |y| = 1
+      double x1 = r / (fabs(p0) < tiny ? copysign(tiny, p0) : p0), x2 = 0.0;
+      Y[(int64_t)(n - 1) * k + t] = x1;
+      double big = fabs(x1);
+      for (int c = n - 2; c >= 0; c--) {
+        const double *u = Ut + (int64_t)3 * c * k + t;
+        const double x = (Y[(int64_t)c * k + t] - u[k] * x1 - u[2 * k] * x2) / u[0];
+        Y[(int64_t)c * k + t] = x;
+        big = fmax(big, fabs(x));
+        x2 = x1;
+        x1 = x;
+      }
+      if (!(big > 0.0) || !(big <= DBL_MAX)) {
+        atomicAdd(&bad, 1);
+        break;
+      }
+      const double sc = 1.0 / big;
+      for (int c = 0; c < n; c++) Y[(int64_t)c * k + t] *= sc;
+    }
+  }
+  __syncthreads();
+  if (bad) {
+    if (tid == 0) atomicAdd(fail, 1);
+    return;
+  }
+  // modified Gram-Schmidt in ascending eigenvalue order, one wavefront per vector (lane l
+  // holds components l, l + 64, l + 128); nrm[j]: |y_j|^2 once y_j is final, onrm[t]: |y_t|^2
+  // before the pass
+  const int lane = tid & 63, wv = __builtin_amdgcn_readfirstlane(tid >> 6);   // wavefront index, uniform
+  constexpr int kWaves = kEigThreads / 64;
+  double *onrm = nrm + kBlockEigMax;
+  auto load3 = [&](int t, double *y) {
+    for (int r = 0; r < 3; r++) {
+      const int c = lane + 64 * r;
+      y[r] = c < n ? Y[(int64_t)c * k + t] : 0.0;
+    }
+  };
+  auto store3 = [&](int t, const double *y) {
+    for (int r = 0; r < 3; r++) {
+      const int c = lane + 64 * r;
+      if (c < n) Y[(int64_t)c * k + t] = y[r];
+    }
+  };
+  for (int pass = 0; pass < 2; pass++) {
+    for (int t = wv; t < k; t += kWaves) {
+      double y[3];
+      load3(t, y);
+      const double s = wave_sum(y[0] * y[0] + y[1] * y[1] + y[2] * y[2]);
+      if (lane == 0) {
+        onrm[t] = s;
+        if (t == 0) nrm[0] = s;
+      }
+    }
+    __syncthreads();
+    for (int j = 0; j + 1 < k; j++) {
+      const double nj = nrm[j];
+      double yj[3];
+      load3(j, yj);
+      // this wavefront's vectors j + 1 + wv + 4 g, eight at a time
+      for (int g0 = 0; j + 1 + wv + kWaves * g0 < k; g0 += 8) {
+        double a[8][3], dt[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+          const int t = j + 1 + wv + kWaves * (g0 + u);
+          if (t < k) load3(t, a[u]);
+          else a[u][0] = a[u][1] = a[u][2] = 0.0;
+          dt[u] = yj[0] * a[u][0] + yj[1] * a[u][1] + yj[2] * a[u][2];
+        }
+        const double fl = wave_sum8(dt, lane) / nj;
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+          const int t = j + 1 + wv + kWaves * (g0 + u);
+          const double f = read_lane(fl, (u >> 2) * 32 + ((u >> 1) & 1) * 16 + (u & 1) * 8);
+          for (int r = 0; r < 3; r++) a[u][r] = a[u][r] - f * yj[r];
+          if (t < k) store3(t, a[u]);
+        }
+        if (wv == 0 && g0 == 0) {   // vector j + 1 is final: its norm, and what the pass took
+          const int t = j + 1;
+          const double sa = wave_sum(a[0][0] * a[0][0] + a[0][1] * a[0][1] + a[0][2] * a[0][2]);
+          if (lane == 0) {
+            nrm[t] = sa;
+            if (!(sa > 1e-20 * onrm[t])) atomicAdd(&bad, 1);
+            else if (sa < 0.5 * onrm[t]) again = 1;
+          }
+        }
+      }
+      __syncthreads();
+    }
+    if (bad || !again) break;
+    __syncthreads();
+    if (tid == 0) again = 0;
+  }
+  if (bad) {
+    if (tid == 0) atomicAdd(fail, 1);
+    return;
+  }
+  // unit length, out as row t of the block's k x n result in X; then the reflectors and h
+  // into LDS (over the vectors, and over d), and x = H_{n-1} ... H_2 y in place in X, each
+  // wavefront taking kChunk vectors at a time, held in registers over the reflectors
+  for (int t = wv; t < k; t += kWaves) {
+    double y[3];
+    load3(t, y);
+    const double sc = 1.0 / sqrt(wave_sum(y[0] * y[0] + y[1] * y[1] + y[2] * y[2]));
+    for (int r = 0; r < 3; r++) {
+      const int c = lane + 64 * r;
+      if (c < n) X[J.x_off + (int64_t)t * n + c] = y[r] * sc;
+    }
+  }
+  __syncthreads();
+  double *Hs = red + 8, *hs = d;
+  for (int q = tid; q < tri(n); q += kEigThreads) Hs[q] = H[J.p_off + q];
+  for (int c = tid; c < n; c += kEigThreads) hs[c] = HV[J.r_off + c];
+  __syncthreads();
+  constexpr int kChunk = 8;   // wave_sum8
+  for (int t0 = wv * kChunk; t0 < k; t0 += kWaves * kChunk) {
+    double y[kChunk][3];
+#pragma unroll
+    for (int u = 0; u < kChunk; u++)
+      for (int r = 0; r < 3; r++) {
+        const int c = lane + 64 * r;
+        y[u][r] = (t0 + u < k && c < n) ? X[J.x_off + (int64_t)(t0 + u) * n + c] : 0.0;
+      }
+    for (int i = 2; i < n; i++) {
+      const double h = hs[i];
+      if (h == 0.0) continue;
+      const double *v = Hs + tri(i);
+      double vr[3];
+      for (int r = 0; r < 3; r++) {
+        const int c = lane + 64 * r;
+        vr[r] = c < i ? v[c] : 0.0;
+      }
+      double dot[kChunk];
+#pragma unroll
+      for (int u = 0; u < kChunk; u++) dot[u] = vr[0] * y[u][0] + vr[1] * y[u][1] + vr[2] * y[u][2];
+      const double fl = wave_sum8(dot, lane) / h;
+#pragma unroll
+      for (int u = 0; u < kChunk; u++) {
+        const double f = read_lane(fl, (u >> 2) * 32 + ((u >> 1) & 1) * 16 + (u & 1) * 8);
+        for (int r = 0; r < 3; r++) y[u][r] -= f * vr[r];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kChunk; u++) {
+      if (t0 + u >= k) continue;
+      for (int r = 0; r < 3; r++) {
+        const int c = lane + 64 * r;
+        if (c < n) X[J.x_off + (int64_t)(t0 + u) * n + c] = y[u][r];
+      }
+    }
+  }
+}
+
+}  // namespace
+
+void BlockEig::values(int32_t count, const int32_t *dim, const int64_t *a_off, const double *A, hipStream_t s) {
+  nblk = count;
+  n.assign(dim, dim + count);
+  r_off.assign(count + 1, 0);
+  p_off.assign(count + 1, 0);
+  if (count <= 0) return;
+  std::vector<ValJob> jobs(count);
+  int32_t nmax = 1;
+  for (int32_t j = 0; j < count; j++) {
+    if (dim[j] < 1 || dim[j] > kBlockEigMax) throw Failure(DMRGX_ERR_ARG, "block eigensolver: dimension out of range");
+    r_off[j + 1] = r_off[j] + dim[j];
+    p_off[j + 1] = p_off[j] + (int64_t)dim[j] * (dim[j] + 1) / 2;
+    jobs[j] = ValJob{dim[j], a_off[j], p_off[j], r_off[j]};
+    nmax = std::max(nmax, dim[j]);
+  }
+  H.resize(p_off[count]);
+  Dg.resize(r_off[count]);
+  Eg.resize(r_off[count]);
+  HVg.resize(r_off[count]);
+  Wg.resize(r_off[count]);
+  DevBuf<ValJob> dj(count);
+  ctx().copy_in(dj.p, jobs.data(), sizeof(ValJob) * count);
+  const size_t lds = sizeof(double) * ((size_t)(7 + kGrid) * nmax + kEigThreads / 64);   // d e hv vb(2) pw pr pc(16) red
+  HIPCHK(hipFuncSetAttribute((const void *)k_eig_values, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  k_eig_values<<<count, kEigThreads, lds, s>>>(dj.p, A, H.p, Dg.p, Eg.p, HVg.p, Wg.p);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(s));   // the job list goes back to the allocation cache
+}
+
+bool BlockEig::vectors(const std::vector<std::vector<int32_t>> &kept, std::vector<double> &X,
+                       std::vector<int64_t> &x_off, hipStream_t s) {
+  x_off.assign(nblk + 1, 0);
+  std::vector<VecJob> lds_jobs, glob_jobs;
+  std::vector<int32_t> idx;
+  int64_t w = 0, wg = 0;
+  for (int32_t j = 0; j < nblk; j++) {
+    const int32_t nj = n[j], kj = (int32_t)kept[j].size();
+    x_off[j + 1] = x_off[j] + (int64_t)kj * nj;
+    if (kj == 0) continue;
+    VecJob V{nj, kj, p_off[j], r_off[j], (int64_t)idx.size(), x_off[j], 0};
+    idx.insert(idx.end(), kept[j].begin(), kept[j].end());
+    if ((int64_t)nj * kj <= kVecLdsMax) {
+      V.w_off = w;
+      w += (int64_t)nj * kj;
+      lds_jobs.push_back(V);
+    } else {
+      V.w_off = wg;
+      wg += (int64_t)nj * kj;
+      glob_jobs.push_back(V);
+    }
+  }
+  X.assign(x_off[nblk], 0.0);
+  if (idx.empty()) return true;
+  for (auto &V : glob_jobs) V.w_off += w;   // after the LDS jobs' factor workspace
+  DevBuf<int32_t> di(idx.size());
+  DevBuf<double> U(3 * (w + wg)), Yg(std::max<int64_t>(w + wg, 1)), Xd(x_off[nblk]);
+  DevBuf<int> fail(1);
+  Context &C = ctx();
+  C.copy_in(di.p, idx.data(), sizeof(int32_t) * idx.size());
+  HIPCHK(hipMemsetAsync(fail.p, 0, sizeof(int), s));
+  DevBuf<VecJob> dl(std::max<size_t>(lds_jobs.size(), 1)), dg(std::max<size_t>(glob_jobs.size(), 1));
+  // LDS: the fixed part, then the vectors (LDS form) and later the packed reflectors
+  auto area = [](const std::vector<VecJob> &js, bool vecs) {
+    int64_t m = 0;
+    for (const auto &V : js) m = std::max(m, std::max(vecs ? (int64_t)V.n * V.k : 0, (int64_t)V.n * (V.n + 1) / 2));
+    return sizeof(double) * (size_t)(kVecFixed + m);
+  };
+  if (!lds_jobs.empty()) {
+    const size_t lds = area(lds_jobs, true);
+    C.copy_in(dl.p, lds_jobs.data(), sizeof(VecJob) * lds_jobs.size());
+    HIPCHK(hipFuncSetAttribute((const void *)k_eig_vectors<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)lds));
+    k_eig_vectors<true><<<(unsigned)lds_jobs.size(), kEigThreads, lds, s>>>(dl.p, H.p, Dg.p, Eg.p, HVg.p, Wg.p,
+                                                                            di.p, U.p, Yg.p, Xd.p, fail.p);
+    HIPCHK(hipGetLastError());
+  }
+  if (!glob_jobs.empty()) {
+    C.copy_in(dg.p, glob_jobs.data(), sizeof(VecJob) * glob_jobs.size());
+    const size_t lds = area(glob_jobs, false);
+    HIPCHK(hipFuncSetAttribute((const void *)k_eig_vectors<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)lds));
+    k_eig_vectors<false><<<(unsigned)glob_jobs.size(), kEigThreads, lds, s>>>(
+        dg.p, H.p, Dg.p, Eg.p, HVg.p, Wg.p, di.p, U.p, Yg.p, Xd.p, fail.p);
+    HIPCHK(hipGetLastError());
+  }
+  int h = 0;
+  HIPCHK(hipMemcpyAsync(&h, fail.p, sizeof(int), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipMemcpyAsync(X.data(), Xd.p, sizeof(double) * X.size(), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  return h == 0;
+}
+
+}  // namespace dmrgx

@@ -14,6 +14,8 @@
 // is built without FMA contraction), so the result is the reference's bit for bit.
 #include <hip/hip_runtime.h>
 
+#include <atomic>
+
 #include <rocblas/rocblas.h>
 #include <rocsolver/rocsolver.h>
 
@@ -237,6 +239,11 @@ struct LaneGuard {
   }
 };
 
+// 0: blocks up to kBlockEigMax through the library's own eigensolver; 1: every block
+// through rocSOLVER (dmrgx_set_block_eigensolver)
+std::atomic<int> g_block_eig_mode{0};
+int block_eig_mode() { return g_block_eig_mode.load(std::memory_order_relaxed); }
+
 int32_t padded_block_size(int32_t d) {
   if (d <= 16) return 16;
   if (d <= 512) return (d + 31) / 32 * 32;
@@ -294,93 +301,159 @@ DMRG_SYSTEM_INFO *system_info(int32_t n_qn, const int32_t *Q1, const int32_t *Q2
   Context &C = ctx();
   std::vector<int64_t> row_base(nb + 1, 0);
   for (int32_t b = 0; b < nb; b++) row_base[b + 1] = row_base[b] + dim_block[b];
-  // eigenpairs by batches of equal padded size: one batched solve per size instead of one
-  // solve per block; sizes padded to 16, then to multiples of 32 up to 512 and of 64 above,
-  // so that a block just past a size step costs at most ~25 % more work, not the 8x of a
-  // power-of-two step
-  std::map<int32_t, std::vector<int32_t>> buckets;
-  for (int32_t b = 0; b < nb; b++) buckets[padded_block_size(dim_block[b])].push_back(b);
-  struct Batch {
-    int32_t p;
-    std::vector<int32_t> blocks;
-    DevBuf<double> a, w, e;
-    DevBuf<rocblas_int> info;
-  };
-  std::vector<std::unique_ptr<Batch>> batches;
-  // every batch padded on the library stream first
-  {
-    std::vector<std::unique_ptr<DevBuf<int64_t>>> keep_so;
-    std::vector<std::unique_ptr<DevBuf<int32_t>>> keep_sd;
-    for (const auto &kv : buckets) {
-      auto Bt = std::make_unique<Batch>();
-      Bt->p = kv.first;
-      Bt->blocks = kv.second;
-      const int32_t p = Bt->p, cnt = (int32_t)Bt->blocks.size();
-      std::vector<int64_t> so(cnt);
-      std::vector<int32_t> sd(cnt);
-      for (int32_t j = 0; j < cnt; j++) {
-        so[j] = off[Bt->blocks[j]];
-        sd[j] = dim_block[Bt->blocks[j]];
-      }
-      keep_so.push_back(std::make_unique<DevBuf<int64_t>>(cnt));
-      keep_sd.push_back(std::make_unique<DevBuf<int32_t>>(cnt));
-      C.copy_in(keep_so.back()->p, so.data(), sizeof(int64_t) * cnt);
-      C.copy_in(keep_sd.back()->p, sd.data(), sizeof(int32_t) * cnt);
-      Bt->a.resize((int64_t)cnt * p * p);
-      Bt->w.resize((int64_t)cnt * p);
-      Bt->e.resize((int64_t)cnt * p);
-      Bt->info.resize(cnt);
-      k_dm_pad<<<dim3((unsigned)((p * p + 255) / 256), (unsigned)cnt), 256, 0, C.stream>>>(
-          p, keep_so.back()->p, keep_sd.back()->p, dm.p, Bt->a.p);
-      HIPCHK(hipGetLastError());
-      batches.push_back(std::move(Bt));
-    }
-    // the offset and size arrays go back to the allocation cache after the padding kernels
-    HIPCHK(hipStreamSynchronize(C.stream));
-  }
-  // then the batched eigensolves side by side, batch i on side stream i mod kSolverStreams
-  LaneGuard lanes_used;   // declared after batches: destroyed (streams drained) before them
-  for (size_t i = 0; i < batches.size(); i++) {
-    Batch &Bt = *batches[i];
-    SolverLane &L = solver_lane((int)(i % kSolverStreams));
-    lanes_used.used = i + 1;
-    const int32_t p = Bt.p, cnt = (int32_t)Bt.blocks.size();
-    if (rocsolver_dsyevd_strided_batched(L.handle, rocblas_evect_original, rocblas_fill_upper, p, Bt.a.p, p,
-                                         (rocblas_stride)p * p, Bt.w.p, p, Bt.e.p, p, Bt.info.p,
-                                         cnt) != rocblas_status_success)
-      throw Failure(DMRGX_ERR_HIP, "rocsolver_dsyevd_strided_batched failed");
-    HIPCHK(hipEventRecord(L.done, L.stream));
-    HIPCHK(hipStreamWaitEvent(C.stream, L.done, 0));
-  }
-  // per block: eigenvalues ascending (hw, at row_base) and eigenvectors as rows (hv, at off)
+  // per block: eigenvalues ascending (hw, at row_base) and the eigenvectors the
+  // transformation matrix takes, as rows (hv, at off)
   std::vector<double> hw(dim_LLLR), hv(off.back());
-  for (const auto &Bt : batches) {
-    const int32_t p = Bt->p, cnt = (int32_t)Bt->blocks.size();
-    std::vector<double> ha((int64_t)cnt * p * p), hwp((int64_t)cnt * p);
-    std::vector<rocblas_int> hinfo(cnt);
-    C.copy_out(hinfo.data(), Bt->info.p, sizeof(rocblas_int) * cnt);
-    C.copy_out(hwp.data(), Bt->w.p, sizeof(double) * cnt * p);
-    C.copy_out(ha.data(), Bt->a.p, sizeof(double) * cnt * p * p);
-    for (int32_t j = 0; j < cnt; j++) {
-      if (hinfo[j] != 0) throw Failure(DMRGX_ERR_HIP, "block eigendecomposition did not converge");
-      const int32_t b = Bt->blocks[j], db = dim_block[b], pad = p - db;
-      const double *wj = hwp.data() + (int64_t)j * p, *aj = ha.data() + (int64_t)j * p * p;
-      // the pad eigenvalues (-1) come first; eigenvector i of the result is row i of the
-      // row-major view (column i of the column-major output)
-      for (int32_t i = 0; i < pad; i++)
-        if (wj[i] > -0.5) throw Failure(DMRGX_ERR_HIP, "padded eigenvalues out of place");
-      for (int32_t i = 0; i < db; i++) {
-        hw[row_base[b] + i] = wj[pad + i];
-        std::memcpy(hv.data() + off[b] + (int64_t)i * db, aj + (int64_t)(pad + i) * p, sizeof(double) * db);
+  // rocSOLVER for the listed blocks, by batches of equal padded size: one batched solve per
+  // size instead of one solve per block; sizes padded to 16, then to multiples of 32 up to
+  // 512 and of 64 above, so that a block just past a size step costs at most ~25 % more
+  // work, not the 8x of a power-of-two step
+  auto library_solve = [&](const std::vector<int32_t> &list) {
+    std::map<int32_t, std::vector<int32_t>> buckets;
+    for (int32_t b : list) buckets[padded_block_size(dim_block[b])].push_back(b);
+    struct Batch {
+      int32_t p;
+      std::vector<int32_t> blocks;
+      DevBuf<double> a, w, e;
+      DevBuf<rocblas_int> info;
+    };
+    std::vector<std::unique_ptr<Batch>> batches;
+    // every batch padded on the library stream first
+    {
+      std::vector<std::unique_ptr<DevBuf<int64_t>>> keep_so;
+      std::vector<std::unique_ptr<DevBuf<int32_t>>> keep_sd;
+      for (const auto &kv : buckets) {
+        auto Bt = std::make_unique<Batch>();
+        Bt->p = kv.first;
+        Bt->blocks = kv.second;
+        const int32_t p = Bt->p, cnt = (int32_t)Bt->blocks.size();
+        std::vector<int64_t> so(cnt);
+        std::vector<int32_t> sd(cnt);
+        for (int32_t j = 0; j < cnt; j++) {
+          so[j] = off[Bt->blocks[j]];
+          sd[j] = dim_block[Bt->blocks[j]];
+        }
+        keep_so.push_back(std::make_unique<DevBuf<int64_t>>(cnt));
+        keep_sd.push_back(std::make_unique<DevBuf<int32_t>>(cnt));
+        C.copy_in(keep_so.back()->p, so.data(), sizeof(int64_t) * cnt);
+        C.copy_in(keep_sd.back()->p, sd.data(), sizeof(int32_t) * cnt);
+        Bt->a.resize((int64_t)cnt * p * p);
+        Bt->w.resize((int64_t)cnt * p);
+        Bt->e.resize((int64_t)cnt * p);
+        Bt->info.resize(cnt);
+        k_dm_pad<<<dim3((unsigned)((p * p + 255) / 256), (unsigned)cnt), 256, 0, C.stream>>>(
+            p, keep_so.back()->p, keep_sd.back()->p, dm.p, Bt->a.p);
+        HIPCHK(hipGetLastError());
+        batches.push_back(std::move(Bt));
+      }
+      // the offset and size arrays go back to the allocation cache after the padding kernels
+      HIPCHK(hipStreamSynchronize(C.stream));
+    }
+    // then the batched eigensolves side by side, batch i on side stream i mod kSolverStreams
+    LaneGuard lanes_used;   // declared after batches: destroyed (streams drained) before them
+    for (size_t i = 0; i < batches.size(); i++) {
+      Batch &Bt = *batches[i];
+      SolverLane &L = solver_lane((int)(i % kSolverStreams));
+      lanes_used.used = i + 1;
+      const int32_t p = Bt.p, cnt = (int32_t)Bt.blocks.size();
+      if (rocsolver_dsyevd_strided_batched(L.handle, rocblas_evect_original, rocblas_fill_upper, p, Bt.a.p, p,
+                                           (rocblas_stride)p * p, Bt.w.p, p, Bt.e.p, p, Bt.info.p,
+                                           cnt) != rocblas_status_success)
+        throw Failure(DMRGX_ERR_HIP, "rocsolver_dsyevd_strided_batched failed");
+      HIPCHK(hipEventRecord(L.done, L.stream));
+      HIPCHK(hipStreamWaitEvent(C.stream, L.done, 0));
+    }
+    for (const auto &Bt : batches) {
+      const int32_t p = Bt->p, cnt = (int32_t)Bt->blocks.size();
+      std::vector<double> ha((int64_t)cnt * p * p), hwp((int64_t)cnt * p);
+      std::vector<rocblas_int> hinfo(cnt);
+      C.copy_out(hinfo.data(), Bt->info.p, sizeof(rocblas_int) * cnt);
+      C.copy_out(hwp.data(), Bt->w.p, sizeof(double) * cnt * p);
+      C.copy_out(ha.data(), Bt->a.p, sizeof(double) * cnt * p * p);
+      for (int32_t j = 0; j < cnt; j++) {
+        if (hinfo[j] != 0) throw Failure(DMRGX_ERR_HIP, "block eigendecomposition did not converge");
+        const int32_t b = Bt->blocks[j], db = dim_block[b], pad = p - db;
+        const double *wj = hwp.data() + (int64_t)j * p, *aj = ha.data() + (int64_t)j * p * p;
+        // the pad eigenvalues (-1) come first; eigenvector i of the result is row i of the
+        // row-major view (column i of the column-major output)
+        for (int32_t i = 0; i < pad; i++)
+          if (wj[i] > -0.5) throw Failure(DMRGX_ERR_HIP, "padded eigenvalues out of place");
+        for (int32_t i = 0; i < db; i++) {
+          hw[row_base[b] + i] = wj[pad + i];
+          std::memcpy(hv.data() + off[b] + (int64_t)i * db, aj + (int64_t)(pad + i) * p, sizeof(double) * db);
+        }
       }
     }
-  }
-  // DMRG_GET_SYSTEM_INFO_Q2.c:33-86: every eigenpair, stable descending order
+  };
+  // DMRG_GET_SYSTEM_INFO_Q2.c:33-112: every eigenpair in stable descending order, then the
+  // cut, moved down past (nearly) equal values
   std::vector<Pair> p;
-  p.reserve(dim_LLLR);
+  int32_t dim_renorm = 0;
+  const double zero = 1e-15;
+  auto select = [&]() {
+    p.clear();
+    p.reserve(dim_LLLR);
+    for (int32_t b = 0; b < nb; b++)
+      for (int32_t i = 0; i < dim_block[b]; i++) p.push_back(Pair{hw[row_base[b] + i], b, i, qb1[b], qb2[b]});
+    std::stable_sort(p.begin(), p.end(), [](const Pair &a, const Pair &c) { return a.val > c.val; });
+    dim_renorm = 0;
+    if (dim_LLLR <= max_dim) {
+      for (int32_t i = dim_LLLR; i >= 1; i--)
+        if (p[i - 1].val > zero) {
+          dim_renorm = i;
+          break;
+        }
+    } else {
+      for (int32_t i = max_dim; i >= 1; i--)
+        if (std::fabs(p[i - 1].val - p[i].val) > zero && p[i - 1].val > zero) {
+          dim_renorm = i;
+          break;
+        }
+    }
+  };
+  // blocks up to kBlockEigMax: the library's own solver (blockeig.hip), eigenvalues of every
+  // block first, eigenvectors of the kept states only; larger blocks (and every block when
+  // the own solver is switched off): rocSOLVER
+  std::vector<int32_t> small, large;
   for (int32_t b = 0; b < nb; b++)
-    for (int32_t i = 0; i < dim_block[b]; i++) p.push_back(Pair{hw[row_base[b] + i], b, i, qb1[b], qb2[b]});
-  std::stable_sort(p.begin(), p.end(), [](const Pair &a, const Pair &c) { return a.val > c.val; });
+    (dim_block[b] <= kBlockEigMax && block_eig_mode() == 0 ? small : large).push_back(b);
+  BlockEig own;
+  if (!small.empty()) {
+    std::vector<int32_t> sd(small.size());
+    std::vector<int64_t> so(small.size());
+    for (size_t j = 0; j < small.size(); j++) {
+      sd[j] = dim_block[small[j]];
+      so[j] = off[small[j]];
+    }
+    own.values((int32_t)small.size(), sd.data(), so.data(), dm.p, C.stream);
+    std::vector<double> ws(own.r_off.back());
+    C.copy_out(ws.data(), own.Wg.p, sizeof(double) * ws.size());
+    for (size_t j = 0; j < small.size(); j++)
+      std::memcpy(hw.data() + row_base[small[j]], ws.data() + own.r_off[j], sizeof(double) * sd[j]);
+  }
+  if (!large.empty()) library_solve(large);
+  select();
+  if (!small.empty()) {
+    std::vector<int32_t> slot(nb, -1);
+    for (size_t j = 0; j < small.size(); j++) slot[small[j]] = (int32_t)j;
+    std::vector<std::vector<int32_t>> kept(small.size());
+    for (int32_t i = 0; i < dim_renorm; i++)
+      if (slot[p[i].block] >= 0) kept[slot[p[i].block]].push_back(p[i].basis);
+    for (auto &k : kept) std::sort(k.begin(), k.end());
+    std::vector<double> X;
+    std::vector<int64_t> x_off;
+    if (own.vectors(kept, X, x_off, C.stream)) {
+      for (size_t j = 0; j < small.size(); j++) {
+        const int32_t b = small[j], db = dim_block[b];
+        for (size_t t = 0; t < kept[j].size(); t++)
+          std::memcpy(hv.data() + off[b] + (int64_t)kept[j][t] * db, X.data() + x_off[j] + (int64_t)t * db,
+                      sizeof(double) * db);
+      }
+    } else {   // a kept vector came out dependent on the others: rocSOLVER for these blocks
+      library_solve(small);
+      select();
+    }
+  }
   DMRG_SYSTEM_INFO *S = static_cast<DMRG_SYSTEM_INFO *>(std::calloc(1, sizeof(DMRG_SYSTEM_INFO)));
   if (!S) throw Failure(DMRGX_ERR_NOMEM, "host allocation failed");
   std::unique_ptr<DMRG_SYSTEM_INFO, void (*)(DMRG_SYSTEM_INFO *)> guard(S, dmrgx_free_system_info);
@@ -389,22 +462,6 @@ DMRG_SYSTEM_INFO *system_info(int32_t n_qn, const int32_t *Q1, const int32_t *Q2
   for (int32_t i = 0; i < dim_LLLR; i++) {
     S->Val_DM_Dist[i] = p[i].val;
     S->sum_val_dm = S->sum_val_dm + p[i].val;
-  }
-  // :91-112: the cut, moved down past (nearly) equal values
-  const double zero = 1e-15;
-  int32_t dim_renorm = 0;
-  if (dim_LLLR <= max_dim) {
-    for (int32_t i = dim_LLLR; i >= 1; i--)
-      if (p[i - 1].val > zero) {
-        dim_renorm = i;
-        break;
-      }
-  } else {
-    for (int32_t i = max_dim; i >= 1; i--)
-      if (std::fabs(p[i - 1].val - p[i].val) > zero && p[i - 1].val > zero) {
-        dim_renorm = i;
-        break;
-      }
   }
   if (dim_renorm <= 0) throw Failure(DMRGX_ERR_ARG, "no state kept (dim_renorm <= 0)");
   double kept = 0;
@@ -470,6 +527,13 @@ DMRG_SYSTEM_INFO *system_info(int32_t n_qn, const int32_t *Q1, const int32_t *Q2
 using namespace dmrgx;
 
 extern "C" {
+
+int dmrgx_set_block_eigensolver(int mode) {
+  return run_status([&] {
+    if (mode != 0 && mode != 1) throw Failure(DMRGX_ERR_ARG, "block eigensolver mode must be 0 or 1");
+    g_block_eig_mode.store(mode, std::memory_order_relaxed);
+  });
+}
 
 void dmrgx_free_system_info(DMRG_SYSTEM_INFO *Info) {
   if (!Info) return;

@@ -822,6 +822,25 @@ struct ObcDev {
   }
 };
 
+// Eigenpairs of a truncation step's block density matrices up to kBlockEigMax (blockeig.hip),
+// one workgroup per block, in two stages around the selection of the kept states.
+// values(): block j (dimension dim[j]) row-major at a_off[j] in A (read only); launches the
+// tridiagonal reduction and the eigenvalues and waits for them: block j's eigenvalues
+// ascending at Wg.p + r_off[j].  vectors(): the eigenvectors of eigenvalues kept[j] (indices
+// into block j's ascending list, ascending), as rows: X[x_off[j] + t * dim[j] ...] for the
+// t-th of kept[j].  False when a vector came out dependent on the others (the caller solves
+// another way).
+constexpr int32_t kBlockEigMax = 192;   // packed lower triangle, n (n + 1) / 2 doubles, in LDS
+struct BlockEig {
+  int32_t nblk = 0;
+  std::vector<int32_t> n;
+  std::vector<int64_t> r_off, p_off;
+  DevBuf<double> H, Dg, Eg, HVg, Wg;   // reflectors (packed), T's diagonal and subdiagonal, h, eigenvalues
+  void values(int32_t count, const int32_t *dim, const int64_t *a_off, const double *A, hipStream_t s);
+  bool vectors(const std::vector<std::vector<int32_t>> &kept, std::vector<double> &X, std::vector<int64_t> &x_off,
+               hipStream_t s);
+};
+
 // Kronecker form of a superblock Hamiltonian (kron.hip): per sector of the superblock basis
 // (rows with one LLLR quantum number: a dense da x db block of the vector, LLLR states by
 // RRRL states) the LLLR-only terms as a dense da x da matrix M_L and the RRRL-only terms as

@@ -742,8 +742,11 @@ double **DMRG_GET_DENSITY_MATRIX(int block, int *Dim_Block, DMRG_BASIS *Dmrg_Bas
 /* ---- truncation: density matrices, their block eigendecompositions and the kept states
  * include/dmrg.h:390-391 (dmrg/DMRG_GET_SYSTEM_INFO_Q1.c, DMRG_GET_SYSTEM_INFO_Q2.c:7-236):
  * blocks of equal quantum numbers (DMRG_GET_DENSITY_MATRIX_INFO_Q1/Q2 and their order
- * checks), each block's density matrix and its eigendecomposition on the device
- * (rocSOLVER dsyevd in place of LAPACK_DSYEV), then the reference's selection: all
+ * checks), each block's density matrix and its eigendecomposition on the device in place
+ * of LAPACK_DSYEV (blocks up to 192 states: the library's own solver, one workgroup per
+ * block -- Householder tridiagonalisation in LDS, every eigenvalue by Sturm bisection, and
+ * inverse iteration for the kept states' eigenvectors only; larger blocks: rocSOLVER
+ * dsyevd), then the reference's selection: all
  * eigenvalues sorted descending (stable), the cut at max_dim moved down past values that
  * differ by 1e-15 or less, the truncation error, the kept states re-sorted by Q_Number2 and
  * then Q_Number1 (stable), and the transformation matrix (CCS1, columns = kept
@@ -759,6 +762,12 @@ DMRG_SYSTEM_INFO *DMRG_GET_SYSTEM_INFO_Q2(int *Q_Number1, int *Q_Number2, DMRG_B
 /* The same with a status code: n_qn 1 (Q_Number2 ignored) or 2; *out = NULL on error. */
 int dmrgx_get_system_info(int32_t n_qn, const int32_t *Q_Number1, const int32_t *Q_Number2, const DMRG_BASIS *B,
                           const double *Vec, int32_t max_dim, DMRG_SYSTEM_INFO **out);
+/* No counterpart in the reference: which eigensolver the truncation calls above use for
+ * blocks up to 192 states -- 0 (default) the library's own, 1 rocSOLVER dsyevd for every
+ * block.  Process-wide; DMRGX_ERR_ARG for any other mode.  The own solver hands a block to
+ * rocSOLVER by itself when one of its kept eigenvectors comes out linearly dependent on the
+ * others. */
+int dmrgx_set_block_eigensolver(int mode);
 /* Frees what the three calls above return, as DMRG_FREE_SYSTEM_INFO (include/dmrg.h:357,
  * dmrg/DMRG_FREE_SYSTEM_INFO.c) does; the reference's own function frees it as well. */
 void dmrgx_free_system_info(DMRG_SYSTEM_INFO *Info);

@@ -160,6 +160,40 @@ def test_device_matches_oracle(case):
     assert_same_info(got, want, rho, base)
 
 
+BIG_CASES = [(S, md) for S in [Sub(f, 9) for f in QN_FILES if "m400" in f] for md in (400, 1000)]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", BIG_CASES, ids=[f"{S.name}-max{md}" for S, md in BIG_CASES])
+def test_device_library_solver_matches_oracle(case):
+    """The same selection through rocSOLVER dsyevd for every block (mode 1), and the own
+    solver's result (mode 0) against it: equal cut, eigenvalues within 1e-14, equal kept
+    subspaces per block."""
+    import dmrg_c_amd as D
+    S, max_dim = case
+    print(f"system info, rocSOLVER and own solver, {S.name} max_dim={max_dim}", flush=True)
+    B = S.basis_struct(D)
+    q1, q2 = qns(S)
+    D.set_block_eigensolver(1)
+    try:
+        lib_info = D.system_info(B, S.n_qn, q1, q2, S.vec, max_dim)
+    finally:
+        D.set_block_eigensolver(0)
+    own = D.system_info(B, S.n_qn, q1, q2, S.vec, max_dim)
+    want, rho, base = oracle_info(S, max_dim)
+    assert_same_info(lib_info, want, rho, base)
+    assert_same_info(own, lib_info, rho, base)
+
+
+def test_block_eigensolver_mode_checked():
+    import dmrg_c_amd as D
+    L = D.lib()
+    L.dmrgx_set_block_eigensolver.argtypes = [C.c_int]
+    assert L.dmrgx_set_block_eigensolver(2) == -1   # DMRGX_ERR_ARG
+    assert L.dmrgx_set_block_eigensolver(-1) == -1
+    assert L.dmrgx_set_block_eigensolver(0) == 0
+
+
 @pytest.mark.gpu
 def test_device_order_error():
     import dmrg_c_amd as D
