@@ -7,11 +7,11 @@
 //     Householder reduction to tridiagonal form, last row first: for i = n-1 .. 2 the
 //     reflector H_i = I - v v^T / h, v from row i's first i entries, zeroes them below the
 //     subdiagonal, and the leading i x i part B becomes B - v w^T - w v^T with p = B v / h,
-//     w = p - (v.p / 2h) v; v stays in row i.  Then every eigenvalue of the tridiagonal T by
-//     bisection on the Sturm count (pivots of the LDL^T factorisation of T - x I, floored
-//     at LAPACK's pivmin as in dstebz), one eigenvalue per thread, each step counting at
-//     the three quarter points, to an absolute width of 2 eps ||T||.  Out: the reflectors,
-//     T's diagonal and subdiagonal, the eigenvalues ascending.
+//     w = p - (v.p / 2h) v; v stays in row i.  Out: the reflectors, T's diagonal and
+//     subdiagonal, its Gershgorin interval.  Then k_eig_bisect, one wavefront per
+//     eigenvalue over all blocks: bisection on the Sturm count (pivots of the LDL^T
+//     factorisation of T - x I, floored at LAPACK's pivmin as in dstebz), 64 counts per
+//     step, to an absolute width of 2 eps ||T||: the eigenvalues ascending.
 //  2. k_eig_vectors, one workgroup per block with kept states.  Per kept eigenvalue (one
 //     thread each): three steps of inverse iteration on T - lambda I (Gaussian elimination
 //     with partial pivoting, pivots below eps ||T|| raised to it, as LAPACK's dlagtf/dlagts),
@@ -125,7 +125,7 @@ __device__ __forceinline__ double sum16(double x) {
 __global__ __launch_bounds__(kEigThreads) void k_eig_values(const ValJob *jobs, const double *__restrict__ A,
                                                             double *__restrict__ H, double *__restrict__ D,
                                                             double *__restrict__ E, double *__restrict__ HV,
-                                                            double *__restrict__ W) {
+                                                            double *__restrict__ BND) {
   extern __shared__ double sm[];
   const ValJob J = jobs[blockIdx.x];
   const int n = J.n, tid = threadIdx.x, lane = tid & 63;
@@ -267,58 +267,79 @@ __global__ __launch_bounds__(kEigThreads) void k_eig_values(const ValJob *jobs, 
     E[J.r_off + k] = e[k];
     HV[J.r_off + k] = hv[k];
   }
-  // Gershgorin interval, pivot floor, squared couplings (in pw)
+  // Gershgorin interval, pivot floor and bisection width for k_eig_bisect
   double lo_k = DBL_MAX, hi_k = -DBL_MAX, e2max = 0.0;
   for (int k = tid; k < n; k += kEigThreads) {
     const double r = fabs(e[k]) + (k + 1 < n ? fabs(e[k + 1]) : 0.0);
     lo_k = fmin(lo_k, d[k] - r);
     hi_k = fmax(hi_k, d[k] + r);
-    pw[k] = e[k] * e[k];
-    e2max = fmax(e2max, pw[k]);
+    e2max = fmax(e2max, e[k] * e[k]);
   }
   const double gl0 = -block_max_all(-lo_k, red), gu0 = block_max_all(hi_k, red);
-  e2max = block_max_all(e2max, red);   // ends with a barrier: pw complete
-  const double tnorm = fmax(fabs(gl0), fabs(gu0));
-  const double pivmin = DBL_MIN * fmax(1.0, e2max);
-  const double widen = 2.0 * DBL_EPSILON * tnorm * n + 2.0 * pivmin;
-  const double gl = gl0 - widen, gu = gu0 + widen;
-  const double atol = fmax(2.0 * DBL_EPSILON * tnorm, DBL_MIN);
-  // eigenvalue t: the smallest x with more than t eigenvalues at or below it; each step
-  // counts at the three quarter points of [lo, hi] (three independent pivot recurrences)
-  for (int t = tid; t < n; t += kEigThreads) {
-    double lo = gl, hi = gu;
-    while (true) {
-      const double w = hi - lo;
-      const double x1 = lo + 0.25 * w, x2 = lo + 0.5 * w, x3 = lo + 0.75 * w;
-      if (w <= atol || x1 <= lo || x3 >= hi) break;
-      double q1 = d[0] - x1, q2 = d[0] - x2, q3 = d[0] - x3;
-      if (fabs(q1) < pivmin) q1 = -pivmin;
-      if (fabs(q2) < pivmin) q2 = -pivmin;
-      if (fabs(q3) < pivmin) q3 = -pivmin;
-      int c1 = q1 < 0.0, c2 = q2 < 0.0, c3 = q3 < 0.0;
-      for (int k = 1; k < n; k++) {
-        const double dk = d[k], ek = pw[k];
-        q1 = (dk - x1) - ek * recip(q1);
-        q2 = (dk - x2) - ek * recip(q2);
-        q3 = (dk - x3) - ek * recip(q3);
-        if (fabs(q1) < pivmin) q1 = -pivmin;
-        if (fabs(q2) < pivmin) q2 = -pivmin;
-        if (fabs(q3) < pivmin) q3 = -pivmin;
-        c1 += q1 < 0.0;
-        c2 += q2 < 0.0;
-        c3 += q3 < 0.0;
-      }
-      if (c1 > t) hi = x1;
-      else if (c2 > t) {
-        lo = x1;
-        hi = x2;
-      } else if (c3 > t) {
-        lo = x2;
-        hi = x3;
-      } else lo = x3;
-    }
-    W[J.r_off + t] = 0.5 * (lo + hi);
+  e2max = block_max_all(e2max, red);
+  if (tid == 0) {
+    const double tnorm = fmax(fabs(gl0), fabs(gu0));
+    const double pivmin = DBL_MIN * fmax(1.0, e2max);
+    const double widen = 2.0 * DBL_EPSILON * tnorm * n + 2.0 * pivmin;
+    double *bnd = BND + 4 * (int64_t)blockIdx.x;
+    bnd[0] = gl0 - widen;
+    bnd[1] = gu0 + widen;
+    bnd[2] = pivmin;
+    bnd[3] = fmax(2.0 * DBL_EPSILON * tnorm, DBL_MIN);
   }
+}
+
+// Every eigenvalue of the tridiagonals, one wavefront per eigenvalue (kEigThreads / 64 per
+// workgroup): eigenvalue t of block j is the smallest x with more than t eigenvalues at or
+// below it.  Each step counts at 64 equally spaced interior points of [lo, hi] (lane l at
+// lo + (l + 1) (hi - lo) / 65, each lane its own pivot recurrence, pivots floored at the
+// block's pivmin), and the first point whose count exceeds t and the one before it bracket
+// the eigenvalue: ~9 steps to a width of 2 eps ||T||.
+constexpr int kBisWaves = kEigThreads / 64;
+__global__ __launch_bounds__(kEigThreads) void k_eig_bisect(const int32_t *__restrict__ EJ, int64_t total,
+                                                            const ValJob *__restrict__ jobs,
+                                                            const double *__restrict__ D, const double *__restrict__ E,
+                                                            const double *__restrict__ BND, double *__restrict__ W) {
+  __shared__ double sd[kBisWaves][kBlockEigMax], se[kBisWaves][kBlockEigMax];
+  const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int64_t g = (int64_t)blockIdx.x * kBisWaves + wv;   // this wavefront's eigenvalue
+  const bool live = g < total;
+  int32_t j = 0, n = 0, t = 0;
+  if (live) {
+    j = EJ[g];
+    n = jobs[j].n;
+    t = (int32_t)(g - jobs[j].r_off);
+    for (int k = lane; k < n; k += 64) {
+      sd[wv][k] = D[jobs[j].r_off + k];
+      const double ek = E[jobs[j].r_off + k];
+      se[wv][k] = ek * ek;
+    }
+  }
+  __syncthreads();
+  if (!live) return;
+  const double *d = sd[wv], *e2 = se[wv];
+  double lo = BND[4 * (int64_t)j], hi = BND[4 * (int64_t)j + 1];
+  const double pivmin = BND[4 * (int64_t)j + 2], atol = BND[4 * (int64_t)j + 3];
+  for (int it = 0; it < 64; it++) {
+    const double w = hi - lo;
+    if (w <= atol || !(lo + w * (1.0 / 65.0) > lo)) break;
+    const double x = lo + w * ((double)(lane + 1) * (1.0 / 65.0));
+    double q = d[0] - x;
+    if (fabs(q) < pivmin) q = -pivmin;
+    int cnt = q < 0.0;
+    for (int k = 1; k < n; k++) {
+      q = (d[k] - x) - e2[k] * recip(q);
+      if (fabs(q) < pivmin) q = -pivmin;
+      cnt += q < 0.0;
+    }
+    const unsigned long long above = __ballot(cnt > t);
+    const int f = above ? __ffsll((long long)above) - 1 : 64;   // first point past eigenvalue t
+    const double nlo = f > 0 ? __shfl(x, f - 1) : lo;
+    const double nhi = f < 64 ? __shfl(x, f) : hi;
+    lo = fmin(nlo, nhi);
+    hi = fmax(nhi, lo);
+  }
+  if (lane == 0) W[jobs[j].r_off + t] = 0.5 * (lo + hi);
 }
 
 struct VecJob {
@@ -575,7 +596,16 @@ void BlockEig::values(int32_t count, const int32_t *dim, const int64_t *a_off, c
   ctx().copy_in(dj.p, jobs.data(), sizeof(ValJob) * count);
   const size_t lds = sizeof(double) * ((size_t)(7 + kGrid) * nmax + kEigThreads / 64);   // d e hv vb(2) pw pr pc(16) red
   HIPCHK(hipFuncSetAttribute((const void *)k_eig_values, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  k_eig_values<<<count, kEigThreads, lds, s>>>(dj.p, A, H.p, Dg.p, Eg.p, HVg.p, Wg.p);
+  DevBuf<double> bnd(4 * (size_t)count);
+  k_eig_values<<<count, kEigThreads, lds, s>>>(dj.p, A, H.p, Dg.p, Eg.p, HVg.p, bnd.p);
+  HIPCHK(hipGetLastError());
+  std::vector<int32_t> ej(r_off[count]);
+  for (int32_t j = 0; j < count; j++) std::fill(ej.begin() + r_off[j], ej.begin() + r_off[j + 1], j);
+  DevBuf<int32_t> dej(ej.size());
+  ctx().copy_in(dej.p, ej.data(), sizeof(int32_t) * ej.size());
+  const int64_t total = r_off[count];
+  k_eig_bisect<<<(unsigned)((total + kBisWaves - 1) / kBisWaves), kEigThreads, 0, s>>>(dej.p, total, dj.p, Dg.p, Eg.p,
+                                                                                      bnd.p, Wg.p);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(s));   // the job list goes back to the allocation cache
 }
