@@ -17,7 +17,8 @@
 //     with partial pivoting, pivots below eps ||T|| raised to it, as LAPACK's dlagtf/dlagts),
 //     from a fixed pseudo-random start; modified Gram-Schmidt over the block's kept vectors
 //     in ascending eigenvalue order (a second pass when a vector lost more than half its
-//     norm to it), then x = H_{n-1} ... H_2 y, both with one wavefront per vector.
+//     norm to it), one wavefront per vector; then k_eig_back: x = H_{n-1} ... H_2 y, one
+//     wavefront per eight vectors, the reflectors in LDS.
 // Eigenvalues agree with dsyev's to ~eps ||block||; eigenvectors up to sign, and up to a
 // rotation within a set of (nearly) equal eigenvalues.
 
@@ -35,7 +36,7 @@ constexpr int kEigThreads = 256;
 constexpr int kLdsDoubles = 160 * 1024 / 8;             // LDS of one workgroup, in doubles
 constexpr int kVecFixed = 4 * kBlockEigMax + 8;         // stage 2's LDS besides the vectors (d, e, nrm, onrm, red)
 constexpr int64_t kVecLdsMax = kLdsDoubles - kVecFixed;  // vectors (n x k) held in LDS up to this
-static_assert(kVecFixed + kBlockEigMax * (kBlockEigMax + 1) / 2 <= kLdsDoubles, "reflectors must fit in LDS");
+static_assert(kBlockEigMax * (kBlockEigMax + 1) / 2 + kBlockEigMax <= kLdsDoubles, "reflectors must fit in LDS");
 
 __device__ __forceinline__ int tri(int r) { return r * (r + 1) / 2; }
 
@@ -514,9 +515,8 @@ __global__ __launch_bounds__(kEigThreads) void k_eig_vectors(const VecJob *jobs,
     if (tid == 0) atomicAdd(fail, 1);
     return;
   }
-  // unit length, out as row t of the block's k x n result in X; then the reflectors and h
-  // into LDS (over the vectors, and over d), and x = H_{n-1} ... H_2 y in place in X, each
-  // wavefront taking kChunk vectors at a time, held in registers over the reflectors
+  // unit length, out as row t of the block's k x n result in X (k_eig_back then applies the
+  // reflectors there)
   for (int t = wv; t < k; t += kWaves) {
     double y[3];
     load3(t, y);
@@ -526,46 +526,59 @@ __global__ __launch_bounds__(kEigThreads) void k_eig_vectors(const VecJob *jobs,
       if (c < n) X[J.x_off + (int64_t)t * n + c] = y[r] * sc;
     }
   }
+}
+
+// x = H_{n-1} ... H_2 y in place in X for kChunk of a block's kept vectors: one wavefront per
+// workgroup, the block's reflectors and h in LDS, the vectors in registers over the
+// reflectors (lane l: components l, l + 64, l + 128)
+constexpr int kChunk = 8;   // wave_sum8
+struct BackJob {
+  int32_t job, t0;   // the block (index into the VecJob list) and its first vector
+};
+
+__global__ __launch_bounds__(64) void k_eig_back(const BackJob *__restrict__ bj, const VecJob *__restrict__ jobs,
+                                                 const double *__restrict__ H, const double *__restrict__ HV,
+                                                 double *__restrict__ X) {
+  extern __shared__ double sm[];
+  const BackJob B = bj[blockIdx.x];
+  const VecJob J = jobs[B.job];
+  const int n = J.n, k = J.k, lane = threadIdx.x, t0 = B.t0;
+  double *Hs = sm, *hs = sm + tri(n);
+  for (int q = lane; q < tri(n); q += 64) Hs[q] = H[J.p_off + q];
+  for (int c = lane; c < n; c += 64) hs[c] = HV[J.r_off + c];
   __syncthreads();
-  double *Hs = red + 8, *hs = d;
-  for (int q = tid; q < tri(n); q += kEigThreads) Hs[q] = H[J.p_off + q];
-  for (int c = tid; c < n; c += kEigThreads) hs[c] = HV[J.r_off + c];
-  __syncthreads();
-  constexpr int kChunk = 8;   // wave_sum8
-  for (int t0 = wv * kChunk; t0 < k; t0 += kWaves * kChunk) {
-    double y[kChunk][3];
+  double y[kChunk][3];
 #pragma unroll
-    for (int u = 0; u < kChunk; u++)
-      for (int r = 0; r < 3; r++) {
-        const int c = lane + 64 * r;
-        y[u][r] = (t0 + u < k && c < n) ? X[J.x_off + (int64_t)(t0 + u) * n + c] : 0.0;
-      }
-    for (int i = 2; i < n; i++) {
-      const double h = hs[i];
-      if (h == 0.0) continue;
-      const double *v = Hs + tri(i);
-      double vr[3];
-      for (int r = 0; r < 3; r++) {
-        const int c = lane + 64 * r;
-        vr[r] = c < i ? v[c] : 0.0;
-      }
-      double dot[kChunk];
-#pragma unroll
-      for (int u = 0; u < kChunk; u++) dot[u] = vr[0] * y[u][0] + vr[1] * y[u][1] + vr[2] * y[u][2];
-      const double fl = wave_sum8(dot, lane) / h;
-#pragma unroll
-      for (int u = 0; u < kChunk; u++) {
-        const double f = read_lane(fl, (u >> 2) * 32 + ((u >> 1) & 1) * 16 + (u & 1) * 8);
-        for (int r = 0; r < 3; r++) y[u][r] -= f * vr[r];
-      }
+  for (int u = 0; u < kChunk; u++)
+    for (int r = 0; r < 3; r++) {
+      const int c = lane + 64 * r;
+      y[u][r] = (t0 + u < k && c < n) ? X[J.x_off + (int64_t)(t0 + u) * n + c] : 0.0;
     }
+  for (int i = 2; i < n; i++) {
+    const double h = hs[i];
+    if (h == 0.0) continue;
+    const double *v = Hs + tri(i);
+    double vr[3];
+    for (int r = 0; r < 3; r++) {
+      const int c = lane + 64 * r;
+      vr[r] = c < i ? v[c] : 0.0;
+    }
+    double dot[kChunk];
+#pragma unroll
+    for (int u = 0; u < kChunk; u++) dot[u] = vr[0] * y[u][0] + vr[1] * y[u][1] + vr[2] * y[u][2];
+    const double fl = wave_sum8(dot, lane) / h;
 #pragma unroll
     for (int u = 0; u < kChunk; u++) {
-      if (t0 + u >= k) continue;
-      for (int r = 0; r < 3; r++) {
-        const int c = lane + 64 * r;
-        if (c < n) X[J.x_off + (int64_t)(t0 + u) * n + c] = y[u][r];
-      }
+      const double f = read_lane(fl, (u >> 2) * 32 + ((u >> 1) & 1) * 16 + (u & 1) * 8);
+      for (int r = 0; r < 3; r++) y[u][r] -= f * vr[r];
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < kChunk; u++) {
+    if (t0 + u >= k) continue;
+    for (int r = 0; r < 3; r++) {
+      const int c = lane + 64 * r;
+      if (c < n) X[J.x_off + (int64_t)(t0 + u) * n + c] = y[u][r];
     }
   }
 }
@@ -642,10 +655,10 @@ bool BlockEig::vectors(const std::vector<std::vector<int32_t>> &kept, std::vecto
   C.copy_in(di.p, idx.data(), sizeof(int32_t) * idx.size());
   HIPCHK(hipMemsetAsync(fail.p, 0, sizeof(int), s));
   DevBuf<VecJob> dl(std::max<size_t>(lds_jobs.size(), 1)), dg(std::max<size_t>(glob_jobs.size(), 1));
-  // LDS: the fixed part, then the vectors (LDS form) and later the packed reflectors
+  // LDS: the fixed part, then the vectors (LDS form)
   auto area = [](const std::vector<VecJob> &js, bool vecs) {
     int64_t m = 0;
-    for (const auto &V : js) m = std::max(m, std::max(vecs ? (int64_t)V.n * V.k : 0, (int64_t)V.n * (V.n + 1) / 2));
+    for (const auto &V : js) m = std::max(m, vecs ? (int64_t)V.n * V.k : 0);
     return sizeof(double) * (size_t)(kVecFixed + m);
   };
   if (!lds_jobs.empty()) {
@@ -666,6 +679,23 @@ bool BlockEig::vectors(const std::vector<std::vector<int32_t>> &kept, std::vecto
         dg.p, H.p, Dg.p, Eg.p, HVg.p, Wg.p, di.p, U.p, Yg.p, Xd.p, fail.p);
     HIPCHK(hipGetLastError());
   }
+  // the reflectors, kChunk vectors per workgroup
+  std::vector<VecJob> all(lds_jobs);
+  all.insert(all.end(), glob_jobs.begin(), glob_jobs.end());
+  std::vector<BackJob> back;
+  int32_t nmax = 1;
+  for (size_t j = 0; j < all.size(); j++) {
+    nmax = std::max(nmax, all[j].n);
+    for (int32_t t0 = 0; t0 < all[j].k; t0 += kChunk) back.push_back(BackJob{(int32_t)j, t0});
+  }
+  DevBuf<VecJob> da(all.size());
+  DevBuf<BackJob> db(back.size());
+  C.copy_in(da.p, all.data(), sizeof(VecJob) * all.size());
+  C.copy_in(db.p, back.data(), sizeof(BackJob) * back.size());
+  const size_t lds_b = sizeof(double) * ((size_t)nmax * (nmax + 1) / 2 + nmax);
+  HIPCHK(hipFuncSetAttribute((const void *)k_eig_back, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_b));
+  k_eig_back<<<(unsigned)back.size(), 64, lds_b, s>>>(db.p, da.p, H.p, HVg.p, Xd.p);
+  HIPCHK(hipGetLastError());
   int h = 0;
   HIPCHK(hipMemcpyAsync(&h, fail.p, sizeof(int), hipMemcpyDeviceToHost, s));
   HIPCHK(hipMemcpyAsync(X.data(), Xd.p, sizeof(double) * X.size(), hipMemcpyDeviceToHost, s));
