@@ -645,7 +645,7 @@ bool BlockEig::vectors(const std::vector<std::vector<int32_t>> &kept, std::vecto
       glob_jobs.push_back(V);
     }
   }
-  X.assign(x_off[nblk], 0.0);
+  X.resize(x_off[nblk]);
   if (idx.empty()) return true;
   for (auto &V : glob_jobs) V.w_off += w;   // after the LDS jobs' factor workspace
   DevBuf<int32_t> di(idx.size());
@@ -697,9 +697,8 @@ bool BlockEig::vectors(const std::vector<std::vector<int32_t>> &kept, std::vecto
   k_eig_back<<<(unsigned)back.size(), 64, lds_b, s>>>(db.p, da.p, H.p, HVg.p, Xd.p);
   HIPCHK(hipGetLastError());
   int h = 0;
-  HIPCHK(hipMemcpyAsync(&h, fail.p, sizeof(int), hipMemcpyDeviceToHost, s));
-  HIPCHK(hipMemcpyAsync(X.data(), Xd.p, sizeof(double) * X.size(), hipMemcpyDeviceToHost, s));
-  HIPCHK(hipStreamSynchronize(s));
+  C.copy_out(&h, fail.p, sizeof(int));   // through pinned memory, after the kernels
+  if (h == 0) C.copy_out(X.data(), Xd.p, sizeof(double) * X.size());
   return h == 0;
 }
 

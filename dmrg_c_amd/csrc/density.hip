@@ -301,9 +301,11 @@ DMRG_SYSTEM_INFO *system_info(int32_t n_qn, const int32_t *Q1, const int32_t *Q2
   Context &C = ctx();
   std::vector<int64_t> row_base(nb + 1, 0);
   for (int32_t b = 0; b < nb; b++) row_base[b + 1] = row_base[b] + dim_block[b];
-  // per block: eigenvalues ascending (hw, at row_base) and the eigenvectors the
-  // transformation matrix takes, as rows (hv, at off)
-  std::vector<double> hw(dim_LLLR), hv(off.back());
+  // per block: eigenvalues ascending (hw, at row_base); the eigenvectors the transformation
+  // matrix takes: vec[row_base[b] + i] points at eigenvector i of block b (rows of hv for
+  // the rocSOLVER blocks, of X for the own solver's kept states)
+  std::vector<double> hw(dim_LLLR), hv;
+  std::vector<const double *> vec(dim_LLLR, nullptr);
   // rocSOLVER for the listed blocks, by batches of equal padded size: one batched solve per
   // size instead of one solve per block; sizes padded to 16, then to multiples of 32 up to
   // 512 and of 64 above, so that a block just past a size step costs at most ~25 % more
@@ -363,6 +365,7 @@ DMRG_SYSTEM_INFO *system_info(int32_t n_qn, const int32_t *Q1, const int32_t *Q2
       HIPCHK(hipEventRecord(L.done, L.stream));
       HIPCHK(hipStreamWaitEvent(C.stream, L.done, 0));
     }
+    hv.resize(off.back());
     for (const auto &Bt : batches) {
       const int32_t p = Bt->p, cnt = (int32_t)Bt->blocks.size();
       std::vector<double> ha((int64_t)cnt * p * p), hwp((int64_t)cnt * p);
@@ -381,6 +384,7 @@ DMRG_SYSTEM_INFO *system_info(int32_t n_qn, const int32_t *Q1, const int32_t *Q2
         for (int32_t i = 0; i < db; i++) {
           hw[row_base[b] + i] = wj[pad + i];
           std::memcpy(hv.data() + off[b] + (int64_t)i * db, aj + (int64_t)(pad + i) * p, sizeof(double) * db);
+          vec[row_base[b] + i] = hv.data() + off[b] + (int64_t)i * db;
         }
       }
     }
@@ -433,6 +437,8 @@ DMRG_SYSTEM_INFO *system_info(int32_t n_qn, const int32_t *Q1, const int32_t *Q2
   }
   if (!large.empty()) library_solve(large);
   select();
+  std::vector<double> X;   // the own solver's kept eigenvectors (rows), pointed at by vec
+  std::vector<int64_t> x_off;
   if (!small.empty()) {
     std::vector<int32_t> slot(nb, -1);
     for (size_t j = 0; j < small.size(); j++) slot[small[j]] = (int32_t)j;
@@ -440,14 +446,11 @@ DMRG_SYSTEM_INFO *system_info(int32_t n_qn, const int32_t *Q1, const int32_t *Q2
     for (int32_t i = 0; i < dim_renorm; i++)
       if (slot[p[i].block] >= 0) kept[slot[p[i].block]].push_back(p[i].basis);
     for (auto &k : kept) std::sort(k.begin(), k.end());
-    std::vector<double> X;
-    std::vector<int64_t> x_off;
     if (own.vectors(kept, X, x_off, C.stream)) {
       for (size_t j = 0; j < small.size(); j++) {
         const int32_t b = small[j], db = dim_block[b];
         for (size_t t = 0; t < kept[j].size(); t++)
-          std::memcpy(hv.data() + off[b] + (int64_t)kept[j][t] * db, X.data() + x_off[j] + (int64_t)t * db,
-                      sizeof(double) * db);
+          vec[row_base[b] + kept[j][t]] = X.data() + x_off[j] + (int64_t)t * db;
       }
     } else {   // a kept vector came out dependent on the others: rocSOLVER for these blocks
       library_solve(small);
@@ -503,11 +506,11 @@ DMRG_SYSTEM_INFO *system_info(int32_t n_qn, const int32_t *Q1, const int32_t *Q2
   int64_t k = 0;
   for (int32_t i = 0; i < dim_renorm; i++) {
     const int32_t b = p[i].block, db = dim_block[b];
-    const double *vec = hv.data() + off[b] + (int64_t)p[i].basis * db;
+    const double *v = vec[row_base[b] + p[i].basis];
     for (int32_t j = 0; j < db; j++, k++) {
-      T->Val[k] = vec[j];
+      T->Val[k] = v[j];
       T->Row[k] = (int)(row_base[b] + j);
-      Td->Val[k] = vec[j];
+      Td->Val[k] = v[j];
       Td->Col[k] = (int)(row_base[b] + j);
     }
     T->Col[i + 1] = k;
