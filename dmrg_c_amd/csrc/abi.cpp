@@ -688,10 +688,11 @@ int ground_state_op(const Operator &op, const dmrgx_matrix *A, const dmrgx_solve
         const double one = 1.0;
         v.up(&one);
         op.kron->product(v.d.p, st);
-        double hl;
+        double hl, hc;
         HIPCHK(hipMemcpy(&h, op.kron->z.p, sizeof(double), hipMemcpyDeviceToHost));
         HIPCHK(hipMemcpy(&hl, op.kron->zl.p, sizeof(double), hipMemcpyDeviceToHost));
-        h = h + hl;
+        HIPCHK(hipMemcpy(&hc, op.kron->zc.p, sizeof(double), hipMemcpyDeviceToHost));
+        h = (h + hl) + hc;
       }
       out.eig = h;
       const double one = 1.0;

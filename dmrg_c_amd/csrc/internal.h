@@ -878,7 +878,20 @@ struct KronDev {
   DevBuf<int32_t> lptr, lidx, rptr, ridx;
   // flat LLLR / RRRL state of (LL d + LR) / (RR d + RL) (-1: none) and each flat state's sector
   DevBuf<int32_t> aidx, bidx, asec, bsec;
-  mutable DevBuf<double> z, zl;           // H x of the last product is z + zl
+  mutable DevBuf<double> z, zl, zc;       // H x of the last product is (z + zl) + zc: right, left, coupling
+  mutable hipStream_t side = nullptr;     // the coupling terms' stream (default priority)
+  mutable hipEvent_t fork = nullptr, join = nullptr;
+  KronDev() = default;
+  KronDev(const KronDev &) = delete;
+  KronDev &operator=(const KronDev &) = delete;
+  ~KronDev() {
+    if (side) {
+      (void)hipStreamSynchronize(side);
+      (void)hipStreamDestroy(side);
+    }
+    if (fork) (void)hipEventDestroy(fork);
+    if (join) (void)hipEventDestroy(join);
+  }
   double flops = 0.0;                     // of the two dense products per H x
   // z = H x on stream s
   void product(const double *x, hipStream_t s) const;

@@ -578,12 +578,12 @@ int apply_sell(const Operator &op, bool fp32, const double *x, const Epi &epi, R
 // Rows of a product computed beforehand (the Kronecker form writes H x into z): one lane per
 // row returns z[r], so that the solvers' fused epilogues run on it unchanged.
 struct PreRows {
-  const double *z, *zl;   // the product is z[r] + zl[r]
+  const double *z, *zl, *zc;   // the product is (z[r] + zl[r]) + zc[r]
   static constexpr bool kTimed = false;
   double alg_bytes(int32_t) const { return 0.0; }
   template <int G>
   __device__ __forceinline__ double lane_sum(int32_t r, int lane, const double *__restrict__) const {
-    return lane == 0 ? z[r] + zl[r] : 0.0;
+    return lane == 0 ? (z[r] + zl[r]) + zc[r] : 0.0;
   }
 };
 
@@ -594,7 +594,7 @@ int apply_op(const Operator &op, bool fp32, const double *x, const Epi &epi, Red
   if (op.kron) {
     if (fp32) throw Failure(DMRGX_ERR_ARG, "the Kronecker form has no FP32 values");
     op.kron->product(x, s);
-    return apply_rows(op.n, 1, op.grid, PreRows{op.kron->z.p, op.kron->zl.p}, x, epi, R, s, 0, fin);
+    return apply_rows(op.n, 1, op.grid, PreRows{op.kron->z.p, op.kron->zl.p, op.kron->zc.p}, x, epi, R, s, 0, fin);
   }
   const int fmt = load_format(op.load);
   if (fmt == kFmtSellIdx && ensure_sell(*op.A, op.lanes, false)) {

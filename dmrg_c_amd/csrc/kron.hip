@@ -136,9 +136,9 @@ struct KronCross {
   const int32_t *asec, *bsec;
   const KronSector *sec;
 };
-__global__ __launch_bounds__(256) void k_kron_cross(KronCross kc, const double *__restrict__ x, double *__restrict__ z) {
-  const int32_t i = blockIdx.x;   // flat LLLR state of the output rows
-  const KronSector S = kc.sec[kc.asec[i]];
+__device__ __forceinline__ void kron_cross_row(const KronCross &kc, int32_t i, const double *__restrict__ x,
+                                               double *__restrict__ z) {
+  const KronSector S = kc.sec[kc.asec[i]];   // i: flat LLLR state of the output rows
   double *zr = z + S.off + (int64_t)(i - S.a0) * S.db;
   for (int32_t bl = threadIdx.x; bl < S.db; bl += 256) {
     const int32_t bf = S.b0 + bl;
@@ -164,9 +164,13 @@ __global__ __launch_bounds__(256) void k_kron_cross(KronCross kc, const double *
   }
 }
 
+__global__ __launch_bounds__(256) void k_kron_cross(KronCross kc, const double *__restrict__ x, double *__restrict__ z) {
+  kron_cross_row(kc, blockIdx.x, x, z);
+}
+
 // One TM x TN tile of one of a sector's two dense products: the left one (M_L X, k over
 // the sector's LLLR states, written to zl) or the right one (X M_R^T, k over its RRRL
-// states, added to z, which holds the coupling terms).  k-slabs of kKt staged in LDS, double
+// states, written to z).  k-slabs of kKt staged in LDS, double
 // buffered: the next slab's global loads are issued before the current slab's products
 // (kKt 16, or 32 for a longer product per slab behind the loads).
 // Outside the sector the operands are zero.  MFMA: the four wavefronts split the tile 2 x 2,
@@ -178,7 +182,7 @@ __global__ __launch_bounds__(256) void k_kron_cross(KronCross kc, const double *
 typedef double kron_v4d __attribute__((ext_vector_type(4)));
 
 template <int TM, int TN, bool MFMA, int kKt>
-__device__ __forceinline__ void kron_tile_body(const KronSector *sec, const int32_t *tiles,
+__device__ __forceinline__ void kron_tile_body(const KronSector *sec, const int32_t *tiles, int32_t nt,
                                                const double *__restrict__ ML, const double *__restrict__ MR,
                                                const double *__restrict__ x, double *__restrict__ z,
                                                double *__restrict__ zl, int xcd) {
@@ -189,7 +193,7 @@ __device__ __forceinline__ void kron_tile_body(const KronSector *sec, const int3
   __shared__ double Bs[2][kKt][TN + 1];   // Bs[buffer][k][col]
   // xcd: consecutive tiles (one sector's, in list order) to the blocks of one XCD, so that
   // a sector's operands are fetched into one XCD's L2
-  const int32_t lt = xcd ? xcd_logical(blockIdx.x, gridDim.x) : (int32_t)blockIdx.x;
+  const int32_t lt = xcd ? xcd_logical(blockIdx.x, nt) : (int32_t)blockIdx.x;
   const int32_t *tl = tiles + 4 * (int64_t)lt;
   const KronSector S = sec[tl[0]];
   const int32_t r0 = tl[1] * TM, c0 = tl[2] * TN;
@@ -296,7 +300,7 @@ __device__ __forceinline__ void kron_tile_body(const KronSector *sec, const int3
   auto put = [&](int32_t rr, int32_t cc, double v) {
     if (rr < S.da && cc < S.db) {
       const int64_t at = (int64_t)rr * S.db + cc;
-      Z[at] = right ? Z[at] + v : v;
+      Z[at] = v;
     }
   };
   if constexpr (MFMA) {
@@ -314,25 +318,24 @@ __device__ __forceinline__ void kron_tile_body(const KronSector *sec, const int3
   }
 }
 
+#define KRON_LAUNCH_ARGS                                                                     \
+  const KronSector *sec, const int32_t *tiles, int32_t nt, const double *ML, const double *MR, \
+      const double *x, double *z, double *zl, int xcd
+
 template <int TM, int TN, bool MFMA, int kKt = 16>
-__global__ __launch_bounds__(256) void k_kron_tile(const KronSector *sec, const int32_t *tiles, const double *ML,
-                                                   const double *MR, const double *x, double *z, double *zl, int xcd) {
-  kron_tile_body<TM, TN, MFMA, kKt>(sec, tiles, ML, MR, x, z, zl, xcd);
+__global__ __launch_bounds__(256) void k_kron_tile(KRON_LAUNCH_ARGS) {
+  kron_tile_body<TM, TN, MFMA, kKt>(sec, tiles, nt, ML, MR, x, z, zl, xcd);
 }
 // the 64 x 64 matrix-core tile held to at most 128 registers per lane, four wavefronts per
 // SIMD
-__global__ __launch_bounds__(256, 4) void k_kron_tile_o4(const KronSector *sec, const int32_t *tiles, const double *ML,
-                                                         const double *MR, const double *x, double *z, double *zl,
-                                                         int xcd) {
-  kron_tile_body<64, 64, true, 16>(sec, tiles, ML, MR, x, z, zl, xcd);
+__global__ __launch_bounds__(256, 4) void k_kron_tile_o4(KRON_LAUNCH_ARGS) {
+  kron_tile_body<64, 64, true, 16>(sec, tiles, nt, ML, MR, x, z, zl, xcd);
 }
 
 // the 128 x 128 matrix-core tile held to two wavefronts per SIMD (at most 256 registers per
 // lane): half the operand traffic per output of the 64 x 64 tile
-__global__ __launch_bounds__(256, 2) void k_kron_tile_o2(const KronSector *sec, const int32_t *tiles, const double *ML,
-                                                         const double *MR, const double *x, double *z, double *zl,
-                                                         int xcd) {
-  kron_tile_body<128, 128, true, 16>(sec, tiles, ML, MR, x, z, zl, xcd);
+__global__ __launch_bounds__(256, 2) void k_kron_tile_o2(KRON_LAUNCH_ARGS) {
+  kron_tile_body<128, 128, true, 16>(sec, tiles, nt, ML, MR, x, z, zl, xcd);
 }
 
 __global__ __launch_bounds__(256) void k_zero(int64_t n, double *p) {
@@ -344,11 +347,25 @@ bool right_kind(int k) { return k == K_RR || k == K_RL || k == K_RRRL; }
 
 }  // namespace
 
+// The coupling terms (to zc) on a side stream, beside the dense tiles (to z and zl) on the
+// caller's stream: the tiles hold four workgroups per CU and leave room for the coupling
+// terms' workgroups, which need no LDS, from the start.
 void KronDev::product(const double *x, hipStream_t s) const {
   const KronCross kc{ncross, na, nb, ct.p, lptr.p, lidx.p, rptr.p, ridx.p, lval.p, rval.p, asec.p, bsec.p, sec.p};
-  if (na > 0) k_kron_cross<<<na, 256, 0, s>>>(kc, x, z.p);
   const int shape = kKronShape[variant];
   const int nt = ntiles[shape];
+  if (!side) {
+    HIPCHK(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
+    HIPCHK(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&join, hipEventDisableTiming));
+  }
+  if (na > 0) {
+    HIPCHK(hipEventRecord(fork, s));   // x is ready, and the last product's zc has been read
+    HIPCHK(hipStreamWaitEvent(side, fork, 0));
+    k_kron_cross<<<na, 256, 0, side>>>(kc, x, zc.p);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(join, side));
+  }
   if (nt > 0) {
     const int32_t *tl = tiles[shape].p;
     // live timing (dmrgx_profile_begin_sampled): HIP events around the dense products, their
@@ -360,19 +377,22 @@ void KronDev::product(const double *x, hipStream_t s) const {
       HIPCHK(hipEventRecord(ev->first, s));
       C.prof_bytes += flops;
     }
+#define KRON_ARGS sec.p, tl, nt, ML.p, MR.p, x, z.p, zl.p, xcd
     switch (variant) {
-      case 0: k_kron_tile<64, 64, false><<<nt, 256, 0, s>>>(sec.p, tl, ML.p, MR.p, x, z.p, zl.p, xcd); break;
-      case 1: k_kron_tile<64, 64, true><<<nt, 256, 0, s>>>(sec.p, tl, ML.p, MR.p, x, z.p, zl.p, xcd); break;
-      case 2: k_kron_tile<128, 128, true><<<nt, 256, 0, s>>>(sec.p, tl, ML.p, MR.p, x, z.p, zl.p, xcd); break;
-      case 3: k_kron_tile<64, 128, true><<<nt, 256, 0, s>>>(sec.p, tl, ML.p, MR.p, x, z.p, zl.p, xcd); break;
-      case 4: k_kron_tile<128, 64, true><<<nt, 256, 0, s>>>(sec.p, tl, ML.p, MR.p, x, z.p, zl.p, xcd); break;
-      case 5: k_kron_tile<64, 64, true, 32><<<nt, 256, 0, s>>>(sec.p, tl, ML.p, MR.p, x, z.p, zl.p, xcd); break;
-      case 6: k_kron_tile_o4<<<nt, 256, 0, s>>>(sec.p, tl, ML.p, MR.p, x, z.p, zl.p, xcd); break;
-      default: k_kron_tile_o2<<<nt, 256, 0, s>>>(sec.p, tl, ML.p, MR.p, x, z.p, zl.p, xcd); break;
+      case 0: k_kron_tile<64, 64, false><<<nt, 256, 0, s>>>(KRON_ARGS); break;
+      case 1: k_kron_tile<64, 64, true><<<nt, 256, 0, s>>>(KRON_ARGS); break;
+      case 2: k_kron_tile<128, 128, true><<<nt, 256, 0, s>>>(KRON_ARGS); break;
+      case 3: k_kron_tile<64, 128, true><<<nt, 256, 0, s>>>(KRON_ARGS); break;
+      case 4: k_kron_tile<128, 64, true><<<nt, 256, 0, s>>>(KRON_ARGS); break;
+      case 5: k_kron_tile<64, 64, true, 32><<<nt, 256, 0, s>>>(KRON_ARGS); break;
+      case 6: k_kron_tile_o4<<<nt, 256, 0, s>>>(KRON_ARGS); break;
+      default: k_kron_tile_o2<<<nt, 256, 0, s>>>(KRON_ARGS); break;
     }
+#undef KRON_ARGS
     if (ev) HIPCHK(hipEventRecord(ev->second, s));
+    HIPCHK(hipGetLastError());
   }
-  HIPCHK(hipGetLastError());
+  if (na > 0) HIPCHK(hipStreamWaitEvent(s, join, 0));   // zc complete before the epilogue
 }
 
 }  // namespace dmrgx
@@ -619,6 +639,7 @@ int dmrgx_superblock_kron(const dmrgx_superblock *S, dmrgx_kron **out) {
     HIPCHK(hipGetLastError());
     K.z.resize(std::max<int32_t>(n, 1));
     K.zl.resize(std::max<int32_t>(n, 1));
+    K.zc.resize(std::max<int32_t>(n, 1));
     HIPCHK(hipStreamSynchronize(st));   // the build's own buffers go out of scope
     *out = H.release();
   });
@@ -685,10 +706,11 @@ int dmrgx_kron_spmv(const dmrgx_kron *K, const double *x, double *y) {
     DevBuf<double> dx(n);
     ctx().copy_in(dx.p, x, sizeof(double) * n);
     K->K.product(dx.p, ctx().stream);
-    std::vector<double> zl(n);
+    std::vector<double> zl(n), zc(n);
     ctx().copy_out(y, K->K.z.p, sizeof(double) * n);
     ctx().copy_out(zl.data(), K->K.zl.p, sizeof(double) * n);
-    for (int32_t i = 0; i < n; i++) y[i] = y[i] + zl[i];
+    ctx().copy_out(zc.data(), K->K.zc.p, sizeof(double) * n);
+    for (int32_t i = 0; i < n; i++) y[i] = (y[i] + zl[i]) + zc[i];   // as the solvers' epilogue sums them
   });
 }
 
