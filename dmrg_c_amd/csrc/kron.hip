@@ -348,8 +348,8 @@ bool right_kind(int k) { return k == K_RR || k == K_RL || k == K_RRRL; }
 }  // namespace
 
 // The coupling terms (to zc) on a side stream, beside the dense tiles (to z and zl) on the
-// caller's stream: the tiles hold four workgroups per CU and leave room for the coupling
-// terms' workgroups, which need no LDS, from the start.
+// caller's stream: the tiles are queued first and hold four workgroups per CU, the coupling
+// terms' workgroups, which need no LDS, fill the room beside them.
 void KronDev::product(const double *x, hipStream_t s) const {
   const KronCross kc{ncross, na, nb, ct.p, lptr.p, lidx.p, rptr.p, ridx.p, lval.p, rval.p, asec.p, bsec.p, sec.p};
   const int shape = kKronShape[variant];
@@ -359,13 +359,7 @@ void KronDev::product(const double *x, hipStream_t s) const {
     HIPCHK(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
     HIPCHK(hipEventCreateWithFlags(&join, hipEventDisableTiming));
   }
-  if (na > 0) {
-    HIPCHK(hipEventRecord(fork, s));   // x is ready, and the last product's zc has been read
-    HIPCHK(hipStreamWaitEvent(side, fork, 0));
-    k_kron_cross<<<na, 256, 0, side>>>(kc, x, zc.p);
-    HIPCHK(hipGetLastError());
-    HIPCHK(hipEventRecord(join, side));
-  }
+  if (na > 0) HIPCHK(hipEventRecord(fork, s));   // x is ready, and the last product's zc has been read
   if (nt > 0) {
     const int32_t *tl = tiles[shape].p;
     // live timing (dmrgx_profile_begin_sampled): HIP events around the dense products, their
@@ -392,7 +386,13 @@ void KronDev::product(const double *x, hipStream_t s) const {
     if (ev) HIPCHK(hipEventRecord(ev->second, s));
     HIPCHK(hipGetLastError());
   }
-  if (na > 0) HIPCHK(hipStreamWaitEvent(s, join, 0));   // zc complete before the epilogue
+  if (na > 0) {
+    HIPCHK(hipStreamWaitEvent(side, fork, 0));
+    k_kron_cross<<<na, 256, 0, side>>>(kc, x, zc.p);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(join, side));
+    HIPCHK(hipStreamWaitEvent(s, join, 0));   // zc complete before the epilogue
+  }
 }
 
 }  // namespace dmrgx
