@@ -14,7 +14,8 @@
 //     step, to an absolute width of 2 eps ||T||: the eigenvalues ascending.
 //  2. k_eig_vectors, one workgroup per block with kept states.  Per kept eigenvalue (one
 //     thread each): three steps of inverse iteration on T - lambda I (Gaussian elimination
-//     with partial pivoting, pivots below eps ||T|| raised to it, as LAPACK's dlagtf/dlagts),
+//     with partial pivoting, pivots below eps ||T|| raised to it, as LAPACK's dlagtf/dlagts;
+//     the pivots kept as reciprocals),
 //     from a fixed pseudo-random start; modified Gram-Schmidt over the block's kept vectors
 //     in ascending eigenvalue order (a second pass when a vector lost more than half its
 //     norm to it), one wavefront per vector; then k_eig_back: x = H_{n-1} ... H_2 y, one
@@ -51,26 +52,70 @@ __device__ double block_max_all(double v, double *red) {
   return t;
 }
 
+template <int kCtrl>
+__device__ __forceinline__ double dpp(double x) {
+  const long long v = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_mov_dpp((int)v, kCtrl, 0xF, 0xF, true);
+  const int hi = __builtin_amdgcn_mov_dpp((int)(v >> 32), kCtrl, 0xF, 0xF, true);
+  return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+
+// sum over the 16 lanes of a row of the wavefront (every lane of the row gets it): lane
+// pairs l^1, l^2, then the mirrors within 8 and within 16 lanes
+__device__ __forceinline__ double sum16(double x) {
+  x += dpp<0xB1>(x);    // quad_perm [1, 0, 3, 2]
+  x += dpp<0x4E>(x);    // quad_perm [2, 3, 0, 1]
+  x += dpp<0x141>(x);   // row_half_mirror
+  x += dpp<0x140>(x);   // row_mirror
+  return x;
+}
+
+// the value of x in lane l ^ 32 (l ^ 16): the gfx950 lane swaps, which exchange the upper
+// half of their first operand with the lower half of their second (rows 1 and 3 with rows 0
+// and 2 of 16 lanes); both operands x, the partner's value is the second result below the
+// swapped boundary and the first above it
+__device__ __forceinline__ double swap32(double x) {
+  const long long v = __double_as_longlong(x);
+  const auto lo = __builtin_amdgcn_permlane32_swap((int)v, (int)v, false, false);
+  const auto hi = __builtin_amdgcn_permlane32_swap((int)(v >> 32), (int)(v >> 32), false, false);
+  const bool up = threadIdx.x & 32;
+  const int l = up ? lo[0] : lo[1], h = up ? hi[0] : hi[1];
+  return __longlong_as_double((long long)(((unsigned long long)(unsigned)h << 32) | (unsigned)l));
+}
+
+__device__ __forceinline__ double swap16(double x) {
+  const long long v = __double_as_longlong(x);
+  const auto lo = __builtin_amdgcn_permlane16_swap((int)v, (int)v, false, false);
+  const auto hi = __builtin_amdgcn_permlane16_swap((int)(v >> 32), (int)(v >> 32), false, false);
+  const bool up = threadIdx.x & 16;
+  const int l = up ? lo[0] : lo[1], h = up ? hi[0] : hi[1];
+  return __longlong_as_double((long long)(((unsigned long long)(unsigned)h << 32) | (unsigned)l));
+}
+
+// sum over the wavefront, every lane gets it (all 64 lanes active)
 __device__ __forceinline__ double wave_sum(double v) {
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  v = sum16(v);
+  v += swap16(v);
+  v += swap32(v);
   return v;
 }
 
 // Eight sums over the wavefront at once: in, lane l's partial sums of eight quantities; out,
 // the full sum of quantity ((l >> 5) & 1) * 4 + ((l >> 4) & 1) * 2 + ((l >> 3) & 1), held by
-// lane l (lanes 32 a + 16 b + 8 c + 0..7 hold quantity 4 a + 2 b + c).  Halving the set at
-// each of the first three exchanges: 10 exchanges instead of 48.
+// lane l (lanes 32 a + 16 b + 8 c + 0..7 hold quantity 4 a + 2 b + c).  The set is halved at
+// the first three exchanges (lanes l ^ 32, l ^ 16, and l with 15 - l in a row of 16), the
+// rest summed within 8 lanes (all 64 lanes active).
 __device__ __forceinline__ double wave_sum8(const double *x, int lane) {
   const bool b5 = lane & 32, b4 = lane & 16, b3 = lane & 8;
   double e4[4], e2[2];
 #pragma unroll
-  for (int q = 0; q < 4; q++) e4[q] = (b5 ? x[q + 4] : x[q]) + __shfl_xor(b5 ? x[q] : x[q + 4], 32);
+  for (int q = 0; q < 4; q++) e4[q] = (b5 ? x[q + 4] : x[q]) + swap32(b5 ? x[q] : x[q + 4]);
 #pragma unroll
-  for (int q = 0; q < 2; q++) e2[q] = (b4 ? e4[q + 2] : e4[q]) + __shfl_xor(b4 ? e4[q] : e4[q + 2], 16);
-  double e1 = (b3 ? e2[1] : e2[0]) + __shfl_xor(b3 ? e2[0] : e2[1], 8);
-  e1 += __shfl_xor(e1, 4);
-  e1 += __shfl_xor(e1, 2);
-  e1 += __shfl_xor(e1, 1);
+  for (int q = 0; q < 2; q++) e2[q] = (b4 ? e4[q + 2] : e4[q]) + swap16(b4 ? e4[q] : e4[q + 2]);
+  double e1 = (b3 ? e2[1] : e2[0]) + dpp<0x140>(b3 ? e2[0] : e2[1]);   // row_mirror: l and 15 - l
+  e1 += dpp<0xB1>(e1);
+  e1 += dpp<0x4E>(e1);
+  e1 += dpp<0x141>(e1);   // row_half_mirror: l and 7 - l
   return e1;
 }
 
@@ -104,24 +149,6 @@ constexpr int kSlabs = kBlockEigMax / kGrid;
 constexpr int kSlots = kSlabs * (kSlabs + 1) / 2;
 static_assert(kBlockEigMax % kGrid == 0 && kGrid * kGrid == kEigThreads, "stage 1 thread grid");
 __host__ __device__ constexpr int slot(int a, int b) { return a * (a + 1) / 2 + b; }
-
-template <int kCtrl>
-__device__ __forceinline__ double dpp(double x) {
-  const long long v = __double_as_longlong(x);
-  const int lo = __builtin_amdgcn_mov_dpp((int)v, kCtrl, 0xF, 0xF, true);
-  const int hi = __builtin_amdgcn_mov_dpp((int)(v >> 32), kCtrl, 0xF, 0xF, true);
-  return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
-}
-
-// sum over the 16 lanes of a row of the wavefront (every lane of the row gets it): lane
-// pairs l^1, l^2, then the mirrors within 8 and within 16 lanes
-__device__ __forceinline__ double sum16(double x) {
-  x += dpp<0xB1>(x);    // quad_perm [1, 0, 3, 2]
-  x += dpp<0x4E>(x);    // quad_perm [2, 3, 0, 1]
-  x += dpp<0x141>(x);   // row_half_mirror
-  x += dpp<0x140>(x);   // row_mirror
-  return x;
-}
 
 __global__ __launch_bounds__(kEigThreads) void k_eig_values(const ValJob *jobs, const double *__restrict__ A,
                                                             double *__restrict__ H, double *__restrict__ D,
@@ -399,9 +426,9 @@ __global__ __launch_bounds__(kEigThreads) void k_eig_vectors(const VecJob *jobs,
         const double rn = Y[(int64_t)(c + 1) * k + t];
         double *u = Ut + (int64_t)3 * c * k + t;
         if (fabs(p0) >= fabs(l)) {
-          const double piv = fabs(p0) < tiny ? copysign(tiny, p0) : p0;
-          const double m = l / piv;
-          u[0] = piv;
+          const double rp = recip(fabs(p0) < tiny ? copysign(tiny, p0) : p0);
+          const double m = l * rp;
+          u[0] = rp;   // the pivot's reciprocal
           u[k] = p1;
           u[2 * k] = 0.0;
           Y[(int64_t)c * k + t] = r;
@@ -409,8 +436,9 @@ __global__ __launch_bounds__(kEigThreads) void k_eig_vectors(const VecJob *jobs,
           p1 = s;
           r = rn - m * r;
         } else {
-          const double m = p0 / l;
-          u[0] = fabs(l) < tiny ? copysign(tiny, l) : l;
+          const double rl = recip(fabs(l) < tiny ? copysign(tiny, l) : l);
+          const double m = p0 * rl;
+          u[0] = rl;
           u[k] = a;
           u[2 * k] = s;
           Y[(int64_t)c * k + t] = rn;
@@ -420,12 +448,12 @@ __global__ __launch_bounds__(kEigThreads) void k_eig_vectors(const VecJob *jobs,
         }
       }
       // back substitution, then scale to max |y| = 1
-      double x1 = r / (fabs(p0) < tiny ? copysign(tiny, p0) : p0), x2 = 0.0;
+      double x1 = r * recip(fabs(p0) < tiny ? copysign(tiny, p0) : p0), x2 = 0.0;
       Y[(int64_t)(n - 1) * k + t] = x1;
       double big = fabs(x1);
       for (int c = n - 2; c >= 0; c--) {
         const double *u = Ut + (int64_t)3 * c * k + t;
-        const double x = (Y[(int64_t)c * k + t] - u[k] * x1 - u[2 * k] * x2) / u[0];
+        const double x = (Y[(int64_t)c * k + t] - u[k] * x1 - u[2 * k] * x2) * u[0];
         Y[(int64_t)c * k + t] = x;
         big = fmax(big, fabs(x));
         x2 = x1;
