@@ -42,13 +42,14 @@ static_assert(kBlockEigMax * (kBlockEigMax + 1) / 2 + kBlockEigMax <= kLdsDouble
 __device__ __forceinline__ int tri(int r) { return r * (r + 1) / 2; }
 
 // max of v over the workgroup (every thread gets it); red: kEigThreads / 64 doubles
+template <int kThreads = kEigThreads>
 __device__ double block_max_all(double v, double *red) {
   for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off));
   __syncthreads();
   if (threadIdx.x % 64 == 0) red[threadIdx.x / 64] = v;
   __syncthreads();
   double t = red[0];
-  for (int k = 1; k < kEigThreads / 64; k++) t = fmax(t, red[k]);
+  for (int k = 1; k < kThreads / 64; k++) t = fmax(t, red[k]);
   return t;
 }
 
@@ -141,47 +142,50 @@ struct ValJob {
   int64_t r_off;   // its first row in D, E, HV, W
 };
 
-// Stage 1's threads as a 16 x 16 grid, (tr, tc) = (tid / 16, tid % 16): thread (tr, tc) holds
-// the block's lower-triangle entries (tr + 16 a, tc + 16 b), b <= a, in registers, slot
-// a (a + 1) / 2 + b; the 16 threads of one tr are one row of 16 lanes of a wavefront.
-constexpr int kGrid = 16;
-constexpr int kSlabs = kBlockEigMax / kGrid;
-constexpr int kSlots = kSlabs * (kSlabs + 1) / 2;
-static_assert(kBlockEigMax % kGrid == 0 && kGrid * kGrid == kEigThreads, "stage 1 thread grid");
-__host__ __device__ constexpr int slot(int a, int b) { return a * (a + 1) / 2 + b; }
+// Stage 1's threads as a 32 x 16 grid, (tr, tc) = (tid / 16, tid % 16): thread (tr, tc)
+// holds the block's entries (tr + 32 a, tc + 16 b), b <= 2 a + 1, in registers, slot
+// a (a + 1) + b: below the diagonal for b < 2 a, and for b = 2 a (b = 2 a + 1) when tr >= tc
+// (tr >= tc + 16); the 16 threads of one tr are one row of 16 lanes of a wavefront.
+constexpr int kValThreads = 512;
+constexpr int kGridR = 32, kGridC = 16;
+constexpr int kSlabs = kBlockEigMax / kGridR;          // row slabs (column slabs: 2 kSlabs)
+constexpr int kSlots = kSlabs * (kSlabs + 1);
+static_assert(kBlockEigMax % kGridR == 0 && kGridR * kGridC == kValThreads, "stage 1 thread grid");
+__host__ __device__ constexpr int slot(int a, int b) { return a * (a + 1) + b; }
 
-__global__ __launch_bounds__(kEigThreads) void k_eig_values(const ValJob *jobs, const double *__restrict__ A,
+__global__ __launch_bounds__(kValThreads) void k_eig_values(const ValJob *jobs, const double *__restrict__ A,
                                                             double *__restrict__ H, double *__restrict__ D,
                                                             double *__restrict__ E, double *__restrict__ HV,
                                                             double *__restrict__ BND) {
   extern __shared__ double sm[];
   const ValJob J = jobs[blockIdx.x];
   const int n = J.n, tid = threadIdx.x, lane = tid & 63;
-  const int tr = tid / kGrid, tc = tid % kGrid, na = (n + kGrid - 1) / kGrid;
+  const int tr = tid / kGridC, tc = tid % kGridC, na = (n + kGridR - 1) / kGridR;
   // vb: row i of the current step (two buffers, alternating); pr, pc: partial products
   double *d = sm, *e = d + n, *hv = e + n, *vb = hv + n, *pw = vb + 2 * n, *pr = pw + n, *pc = pr + n,
-         *red = pc + kGrid * n;
+         *red = pc + kGridR * n;
   double Lr[kSlots];
 #pragma unroll
   for (int a = 0; a < kSlabs; a++)
 #pragma unroll
-    for (int b = 0; b <= a; b++) {
-      const int r = tr + kGrid * a, c = tc + kGrid * b;
+    for (int b = 0; b <= 2 * a + 1; b++) {
+      const int r = tr + kGridR * a, c = tc + kGridC * b;
       Lr[slot(a, b)] = (a < na && r < n && c <= r) ? A[J.a_off + (int64_t)r * n + c] : 0.0;
     }
-  // weights of the a == b slots: the diagonal and below in the row products, strictly below
-  // in the column products (tr < tc: above the diagonal, not an entry)
-  const double fdiag = tr >= tc ? 1.0 : 0.0, fstrict = tr > tc ? 1.0 : 0.0;
+  // weights of the slots b = 2 a and b = 2 a + 1: the diagonal and below in the row
+  // products, strictly below in the column products (else above the diagonal, not an entry)
+  const double fd0 = tr >= tc ? 1.0 : 0.0, fs0 = tr > tc ? 1.0 : 0.0;
+  const double fd1 = tr >= tc + kGridC ? 1.0 : 0.0, fs1 = tr > tc + kGridC ? 1.0 : 0.0;
   for (int i = n - 1; i >= 2; i--) {
     double *vv = vb + (i & 1) * n;
-    const int ai = i / kGrid, ti = i % kGrid;
+    const int ai = i / kGridR, ti = i % kGridR;
     if (tr == ti) {   // row i's entries c < i, from the threads holding them
 #pragma unroll
       for (int a = 0; a < kSlabs; a++)
         if (a == ai)
 #pragma unroll
-          for (int b = 0; b <= a; b++) {
-            const int c = tc + kGrid * b;
+          for (int b = 0; b <= 2 * a + 1; b++) {
+            const int c = tc + kGridC * b;
             if (c < i) vv[c] = Lr[slot(a, b)];
           }
     }
@@ -204,52 +208,55 @@ __global__ __launch_bounds__(kEigThreads) void k_eig_values(const ValJob *jobs, 
       hv[i] = h;
     }
     // v at this thread's rows and columns: row i's entries, v_{i-1} = xl - alpha, zero from i on
-    double vr[kSlabs], vc[kSlabs];
+    double vr[kSlabs], vc[2 * kSlabs];
 #pragma unroll
     for (int a = 0; a < kSlabs; a++) {
-      const int r = tr + kGrid * a, c = tc + kGrid * a;
+      const int r = tr + kGridR * a;
       vr[a] = (a < na && r < i) ? (r == i - 1 ? vl : vv[r]) : 0.0;
-      vc[a] = (a < na && c < i) ? (c == i - 1 ? vl : vv[c]) : 0.0;
     }
-    if (tr == ti && tc == (i - 1) % kGrid) {   // v kept in row i
+#pragma unroll
+    for (int b = 0; b < 2 * kSlabs; b++) {
+      const int c = tc + kGridC * b;
+      vc[b] = c < i ? (c == i - 1 ? vl : vv[c]) : 0.0;
+    }
+    if (tr == ti && tc == (i - 1) % kGridC) {   // v kept in row i
 #pragma unroll
       for (int a = 0; a < kSlabs; a++)
         if (a == ai)
 #pragma unroll
-          for (int b = 0; b <= a; b++)
-            if (b == (i - 1) / kGrid) Lr[slot(a, b)] = vl;
+          for (int b = 0; b <= 2 * a + 1; b++)
+            if (b == (i - 1) / kGridC) Lr[slot(a, b)] = vl;
     }
     // p = B v / h, B = S + D + S^T: row sums over (D + S) and column sums over S, per thread,
-    // then the rows' over the 16 threads of a row group and the columns' over the 16 groups
-    double prow[kSlabs], pcol[kSlabs];
+    // then the rows' over the 16 threads of a row group and the columns' over the 32 groups
+    double pcol[2 * kSlabs];
 #pragma unroll
-    for (int a = 0; a < kSlabs; a++) prow[a] = pcol[a] = 0.0;
-#pragma unroll
-    for (int a = 0; a < kSlabs; a++)
-#pragma unroll
-      for (int b = 0; b <= a; b++) {
-        if (a >= na) continue;
-        const double x = Lr[slot(a, b)];
-        if (a > b) {
-          prow[a] += x * vc[b];
-          pcol[b] += x * vr[a];
-        } else {
-          prow[a] += fdiag * x * vc[b];
-          pcol[b] += fstrict * x * vr[a];
-        }
-      }
+    for (int b = 0; b < 2 * kSlabs; b++) pcol[b] = 0.0;
 #pragma unroll
     for (int a = 0; a < kSlabs; a++) {
       if (a >= na) continue;
-      const double s = sum16(prow[a]);
-      const int r = tr + kGrid * a, c = tc + kGrid * a;
+      double prow = 0.0;
+#pragma unroll
+      for (int b = 0; b <= 2 * a + 1; b++) {
+        const double x = Lr[slot(a, b)];
+        const double wd = b < 2 * a ? 1.0 : (b == 2 * a ? fd0 : fd1);
+        const double ws = b < 2 * a ? 1.0 : (b == 2 * a ? fs0 : fs1);
+        prow += wd * x * vc[b];
+        pcol[b] += ws * x * vr[a];
+      }
+      const double s = sum16(prow);
+      const int r = tr + kGridR * a;
       if (tc == 0 && r < i) pr[r] = s;
-      if (c < i) pc[tr * n + c] = pcol[a];
+    }
+#pragma unroll
+    for (int b = 0; b < 2 * kSlabs; b++) {
+      const int c = tc + kGridC * b;
+      if (c < i) pc[tr * n + c] = pcol[b];
     }
     __syncthreads();
-    for (int j = tid; j < i; j += kEigThreads) {
+    for (int j = tid; j < i; j += kValThreads) {
       double s = pr[j];
-      for (int t = 0; t < kGrid; t++) s += pc[t * n + j];
+      for (int t = 0; t < kGridR; t++) s += pc[t * n + j];
       pw[j] = s / h;
     }
     __syncthreads();
@@ -257,28 +264,28 @@ __global__ __launch_bounds__(kEigThreads) void k_eig_values(const ValJob *jobs, 
     double vp = 0.0;
     for (int k = lane; k < i; k += 64) vp += (k == i - 1 ? vl : vv[k]) * pw[k];
     const double K = wave_sum(vp) / (2.0 * h);
-    double wr[kSlabs], wc[kSlabs];
+    double wc[2 * kSlabs];
 #pragma unroll
-    for (int a = 0; a < kSlabs; a++) {
-      const int r = tr + kGrid * a, c = tc + kGrid * a;
-      wr[a] = (a < na && r < i) ? pw[r] - K * vr[a] : 0.0;
-      wc[a] = (a < na && c < i) ? pw[c] - K * vc[a] : 0.0;
+    for (int b = 0; b < 2 * kSlabs; b++) {
+      const int c = tc + kGridC * b;
+      wc[b] = c < i ? pw[c] - K * vc[b] : 0.0;
     }
     // B -= v w^T + w v^T (rows and columns from i on: v = w = 0, unchanged)
 #pragma unroll
-    for (int a = 0; a < kSlabs; a++)
+    for (int a = 0; a < kSlabs; a++) {
+      if (a >= na) continue;
+      const int r = tr + kGridR * a;
+      const double wr = r < i ? pw[r] - K * vr[a] : 0.0;
 #pragma unroll
-      for (int b = 0; b <= a; b++) {
-        if (a >= na) continue;
-        Lr[slot(a, b)] -= vr[a] * wc[b] + wr[a] * vc[b];
-      }
+      for (int b = 0; b <= 2 * a + 1; b++) Lr[slot(a, b)] -= vr[a] * wc[b] + wr * vc[b];
+    }
   }
   // T: diagonal d, e[k] = T[k][k-1] (k >= 1); the reflectors (rows) to H
 #pragma unroll
   for (int a = 0; a < kSlabs; a++)
 #pragma unroll
-    for (int b = 0; b <= a; b++) {
-      const int r = tr + kGrid * a, c = tc + kGrid * b;
+    for (int b = 0; b <= 2 * a + 1; b++) {
+      const int r = tr + kGridR * a, c = tc + kGridC * b;
       if (a >= na || r >= n || c > r) continue;
       H[J.p_off + tri(r) + c] = Lr[slot(a, b)];
       if (r == c) d[r] = Lr[slot(a, b)];
@@ -290,21 +297,21 @@ __global__ __launch_bounds__(kEigThreads) void k_eig_values(const ValJob *jobs, 
     if (n > 1) hv[1] = 0.0;
   }
   __syncthreads();
-  for (int k = tid; k < n; k += kEigThreads) {
+  for (int k = tid; k < n; k += kValThreads) {
     D[J.r_off + k] = d[k];
     E[J.r_off + k] = e[k];
     HV[J.r_off + k] = hv[k];
   }
   // Gershgorin interval, pivot floor and bisection width for k_eig_bisect
   double lo_k = DBL_MAX, hi_k = -DBL_MAX, e2max = 0.0;
-  for (int k = tid; k < n; k += kEigThreads) {
+  for (int k = tid; k < n; k += kValThreads) {
     const double r = fabs(e[k]) + (k + 1 < n ? fabs(e[k + 1]) : 0.0);
     lo_k = fmin(lo_k, d[k] - r);
     hi_k = fmax(hi_k, d[k] + r);
     e2max = fmax(e2max, e[k] * e[k]);
   }
-  const double gl0 = -block_max_all(-lo_k, red), gu0 = block_max_all(hi_k, red);
-  e2max = block_max_all(e2max, red);
+  const double gl0 = -block_max_all<kValThreads>(-lo_k, red), gu0 = block_max_all<kValThreads>(hi_k, red);
+  e2max = block_max_all<kValThreads>(e2max, red);
   if (tid == 0) {
     const double tnorm = fmax(fabs(gl0), fabs(gu0));
     const double pivmin = DBL_MIN * fmax(1.0, e2max);
@@ -635,10 +642,10 @@ void BlockEig::values(int32_t count, const int32_t *dim, const int64_t *a_off, c
   Wg.resize(r_off[count]);
   DevBuf<ValJob> dj(count);
   ctx().copy_in(dj.p, jobs.data(), sizeof(ValJob) * count);
-  const size_t lds = sizeof(double) * ((size_t)(7 + kGrid) * nmax + kEigThreads / 64);   // d e hv vb(2) pw pr pc(16) red
+  const size_t lds = sizeof(double) * ((size_t)(7 + kGridR) * nmax + kValThreads / 64);   // d e hv vb(2) pw pr pc(32) red
   HIPCHK(hipFuncSetAttribute((const void *)k_eig_values, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   DevBuf<double> bnd(4 * (size_t)count);
-  k_eig_values<<<count, kEigThreads, lds, s>>>(dj.p, A, H.p, Dg.p, Eg.p, HVg.p, bnd.p);
+  k_eig_values<<<count, kValThreads, lds, s>>>(dj.p, A, H.p, Dg.p, Eg.p, HVg.p, bnd.p);
   HIPCHK(hipGetLastError());
   std::vector<int32_t> ej(r_off[count]);
   for (int32_t j = 0; j < count; j++) std::fill(ej.begin() + r_off[j], ej.begin() + r_off[j + 1], j);
