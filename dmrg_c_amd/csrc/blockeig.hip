@@ -27,6 +27,7 @@
 
 #include <cfloat>
 #include <cmath>
+#include <cstring>
 
 #include "internal.h"
 
@@ -640,19 +641,23 @@ void BlockEig::values(int32_t count, const int32_t *dim, const int64_t *a_off, c
   Eg.resize(r_off[count]);
   HVg.resize(r_off[count]);
   Wg.resize(r_off[count]);
-  DevBuf<ValJob> dj(count);
-  ctx().copy_in(dj.p, jobs.data(), sizeof(ValJob) * count);
+  // one upload: the job list, then the map from eigenvalue to block
+  const size_t oe = (sizeof(ValJob) * count + 255) / 256 * 256;
+  std::vector<char> up(oe + sizeof(int32_t) * r_off[count]);
+  std::memcpy(up.data(), jobs.data(), sizeof(ValJob) * count);
+  int32_t *ej = reinterpret_cast<int32_t *>(up.data() + oe);
+  for (int32_t j = 0; j < count; j++) std::fill(ej + r_off[j], ej + r_off[j + 1], j);
+  DevBuf<char> dup(up.size());
+  ctx().copy_in(dup.p, up.data(), up.size());
+  const ValJob *dj = reinterpret_cast<const ValJob *>(dup.p);
+  const int32_t *dej = reinterpret_cast<const int32_t *>(dup.p + oe);
   const size_t lds = sizeof(double) * ((size_t)(7 + kGridR) * nmax + kValThreads / 64);   // d e hv vb(2) pw pr pc(32) red
   HIPCHK(hipFuncSetAttribute((const void *)k_eig_values, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   DevBuf<double> bnd(4 * (size_t)count);
-  k_eig_values<<<count, kValThreads, lds, s>>>(dj.p, A, H.p, Dg.p, Eg.p, HVg.p, bnd.p);
+  k_eig_values<<<count, kValThreads, lds, s>>>(dj, A, H.p, Dg.p, Eg.p, HVg.p, bnd.p);
   HIPCHK(hipGetLastError());
-  std::vector<int32_t> ej(r_off[count]);
-  for (int32_t j = 0; j < count; j++) std::fill(ej.begin() + r_off[j], ej.begin() + r_off[j + 1], j);
-  DevBuf<int32_t> dej(ej.size());
-  ctx().copy_in(dej.p, ej.data(), sizeof(int32_t) * ej.size());
   const int64_t total = r_off[count];
-  k_eig_bisect<<<(unsigned)((total + kBisWaves - 1) / kBisWaves), kEigThreads, 0, s>>>(dej.p, total, dj.p, Dg.p, Eg.p,
+  k_eig_bisect<<<(unsigned)((total + kBisWaves - 1) / kBisWaves), kEigThreads, 0, s>>>(dej, total, dj, Dg.p, Eg.p,
                                                                                       bnd.p, Wg.p);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(s));   // the job list goes back to the allocation cache
@@ -683,38 +688,7 @@ bool BlockEig::vectors(const std::vector<std::vector<int32_t>> &kept, std::vecto
   X.resize(x_off[nblk]);
   if (idx.empty()) return true;
   for (auto &V : glob_jobs) V.w_off += w;   // after the LDS jobs' factor workspace
-  DevBuf<int32_t> di(idx.size());
-  DevBuf<double> U(3 * (w + wg)), Yg(std::max<int64_t>(w + wg, 1)), Xd(x_off[nblk]);
-  DevBuf<int> fail(1);
-  Context &C = ctx();
-  C.copy_in(di.p, idx.data(), sizeof(int32_t) * idx.size());
-  HIPCHK(hipMemsetAsync(fail.p, 0, sizeof(int), s));
-  DevBuf<VecJob> dl(std::max<size_t>(lds_jobs.size(), 1)), dg(std::max<size_t>(glob_jobs.size(), 1));
-  // LDS: the fixed part, then the vectors (LDS form)
-  auto area = [](const std::vector<VecJob> &js, bool vecs) {
-    int64_t m = 0;
-    for (const auto &V : js) m = std::max(m, vecs ? (int64_t)V.n * V.k : 0);
-    return sizeof(double) * (size_t)(kVecFixed + m);
-  };
-  if (!lds_jobs.empty()) {
-    const size_t lds = area(lds_jobs, true);
-    C.copy_in(dl.p, lds_jobs.data(), sizeof(VecJob) * lds_jobs.size());
-    HIPCHK(hipFuncSetAttribute((const void *)k_eig_vectors<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                               (int)lds));
-    k_eig_vectors<true><<<(unsigned)lds_jobs.size(), kEigThreads, lds, s>>>(dl.p, H.p, Dg.p, Eg.p, HVg.p, Wg.p,
-                                                                            di.p, U.p, Yg.p, Xd.p, fail.p);
-    HIPCHK(hipGetLastError());
-  }
-  if (!glob_jobs.empty()) {
-    C.copy_in(dg.p, glob_jobs.data(), sizeof(VecJob) * glob_jobs.size());
-    const size_t lds = area(glob_jobs, false);
-    HIPCHK(hipFuncSetAttribute((const void *)k_eig_vectors<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                               (int)lds));
-    k_eig_vectors<false><<<(unsigned)glob_jobs.size(), kEigThreads, lds, s>>>(
-        dg.p, H.p, Dg.p, Eg.p, HVg.p, Wg.p, di.p, U.p, Yg.p, Xd.p, fail.p);
-    HIPCHK(hipGetLastError());
-  }
-  // the reflectors, kChunk vectors per workgroup
+  // the reflectors' jobs, kChunk vectors per workgroup
   std::vector<VecJob> all(lds_jobs);
   all.insert(all.end(), glob_jobs.begin(), glob_jobs.end());
   std::vector<BackJob> back;
@@ -723,17 +697,61 @@ bool BlockEig::vectors(const std::vector<std::vector<int32_t>> &kept, std::vecto
     nmax = std::max(nmax, all[j].n);
     for (int32_t t0 = 0; t0 < all[j].k; t0 += kChunk) back.push_back(BackJob{(int32_t)j, t0});
   }
-  DevBuf<VecJob> da(all.size());
-  DevBuf<BackJob> db(back.size());
-  C.copy_in(da.p, all.data(), sizeof(VecJob) * all.size());
-  C.copy_in(db.p, back.data(), sizeof(BackJob) * back.size());
+  // one upload for every job list and the kept indices
+  std::vector<char> up;
+  auto put = [&](const void *src, size_t bytes) {
+    const size_t o = (up.size() + 255) / 256 * 256;
+    up.resize(o + std::max<size_t>(bytes, 1));
+    if (bytes) std::memcpy(up.data() + o, src, bytes);
+    return o;
+  };
+  const size_t o_idx = put(idx.data(), sizeof(int32_t) * idx.size());
+  const size_t o_l = put(lds_jobs.data(), sizeof(VecJob) * lds_jobs.size());
+  const size_t o_g = put(glob_jobs.data(), sizeof(VecJob) * glob_jobs.size());
+  const size_t o_a = put(all.data(), sizeof(VecJob) * all.size());
+  const size_t o_b = put(back.data(), sizeof(BackJob) * back.size());
+  DevBuf<char> dup(up.size());
+  Context &C = ctx();
+  C.copy_in(dup.p, up.data(), up.size());
+  const int32_t *di = reinterpret_cast<const int32_t *>(dup.p + o_idx);
+  const VecJob *dl = reinterpret_cast<const VecJob *>(dup.p + o_l), *dg = reinterpret_cast<const VecJob *>(dup.p + o_g);
+  const VecJob *da = reinterpret_cast<const VecJob *>(dup.p + o_a);
+  const BackJob *db = reinterpret_cast<const BackJob *>(dup.p + o_b);
+  // X, then one double whose first word counts the failed blocks: one read-back for both
+  DevBuf<double> U(3 * (w + wg)), Yg(std::max<int64_t>(w + wg, 1)), Xd(x_off[nblk] + 1);
+  int *fail = reinterpret_cast<int *>(Xd.p + x_off[nblk]);
+  HIPCHK(hipMemsetAsync(fail, 0, sizeof(double), s));
+  // LDS: the fixed part, then the vectors (LDS form)
+  auto area = [](const std::vector<VecJob> &js, bool vecs) {
+    int64_t m = 0;
+    for (const auto &V : js) m = std::max(m, vecs ? (int64_t)V.n * V.k : 0);
+    return sizeof(double) * (size_t)(kVecFixed + m);
+  };
+  if (!lds_jobs.empty()) {
+    const size_t lds = area(lds_jobs, true);
+    HIPCHK(hipFuncSetAttribute((const void *)k_eig_vectors<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)lds));
+    k_eig_vectors<true><<<(unsigned)lds_jobs.size(), kEigThreads, lds, s>>>(dl, H.p, Dg.p, Eg.p, HVg.p, Wg.p, di,
+                                                                            U.p, Yg.p, Xd.p, fail);
+    HIPCHK(hipGetLastError());
+  }
+  if (!glob_jobs.empty()) {
+    const size_t lds = area(glob_jobs, false);
+    HIPCHK(hipFuncSetAttribute((const void *)k_eig_vectors<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)lds));
+    k_eig_vectors<false><<<(unsigned)glob_jobs.size(), kEigThreads, lds, s>>>(dg, H.p, Dg.p, Eg.p, HVg.p, Wg.p, di,
+                                                                             U.p, Yg.p, Xd.p, fail);
+    HIPCHK(hipGetLastError());
+  }
   const size_t lds_b = sizeof(double) * ((size_t)nmax * (nmax + 1) / 2 + nmax);
   HIPCHK(hipFuncSetAttribute((const void *)k_eig_back, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_b));
-  k_eig_back<<<(unsigned)back.size(), 64, lds_b, s>>>(db.p, da.p, H.p, HVg.p, Xd.p);
+  k_eig_back<<<(unsigned)back.size(), 64, lds_b, s>>>(db, da, H.p, HVg.p, Xd.p);
   HIPCHK(hipGetLastError());
+  X.resize(x_off[nblk] + 1);
+  C.copy_out(X.data(), Xd.p, sizeof(double) * X.size());   // through pinned memory, after the kernels
   int h = 0;
-  C.copy_out(&h, fail.p, sizeof(int));   // through pinned memory, after the kernels
-  if (h == 0) C.copy_out(X.data(), Xd.p, sizeof(double) * X.size());
+  std::memcpy(&h, X.data() + x_off[nblk], sizeof(int));
+  X.resize(x_off[nblk]);
   return h == 0;
 }
 
