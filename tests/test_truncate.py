@@ -185,6 +185,42 @@ def test_device_library_solver_matches_oracle(case):
     assert_same_info(own, lib_info, rho, base)
 
 
+M800 = [f for f in QN_FILES if "m800" in f]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("path", M800, ids=[os.path.basename(f)[3:-4] for f in M800])
+def test_device_mixed_solvers_m800(path):
+    """The KLM m=800 step: 61 blocks, 13 of them above 192 states (rocSOLVER) and 48 at or
+    below (the own solver) in one call, against rocSOLVER for every block: equal cut and
+    quantum numbers, eigenvalues within 1e-14, equal kept subspaces per block, orthonormal
+    columns."""
+    import dmrg_c_amd as D
+    S = Sub(path, 4)
+    assert (S.dim_block > 192).any() and (S.dim_block <= 192).any()
+    print(f"system info {S.name} dim_LLLR={S.dim_LLLR} blocks={len(S.dim_block)} max_dim=800", flush=True)
+    B = S.basis_struct(D)
+    q1, q2 = qns(S)
+    D.set_block_eigensolver(1)
+    try:
+        want = D.system_info(B, S.n_qn, q1, q2, S.vec, 800)
+    finally:
+        D.set_block_eigensolver(0)
+    got = D.system_info(B, S.n_qn, q1, q2, S.vec, 800)
+    assert got["dim_renorm"] == want["dim_renorm"]
+    np.testing.assert_allclose(got["val_dist"], want["val_dist"], rtol=0, atol=1e-14)
+    assert abs(got["tr_error"] - want["tr_error"]) < 1e-13
+    np.testing.assert_array_equal(got["q1"], want["q1"])
+    np.testing.assert_array_equal(got["q2"], want["q2"])
+    T, Tw = got["T"], want["T"]
+    k = T.shape[1]
+    np.testing.assert_allclose(T.T @ T, np.eye(k), atol=1e-12)
+    base = np.r_[0, np.cumsum(S.dim_block)]
+    for b in range(len(base) - 1):
+        sl = slice(base[b], base[b + 1])
+        np.testing.assert_allclose(T[sl] @ T[sl].T, Tw[sl] @ Tw[sl].T, atol=1e-9)
+
+
 def test_block_eigensolver_mode_checked():
     import dmrg_c_amd as D
     L = D.lib()
