@@ -1111,6 +1111,9 @@ def main():
     ap.add_argument("--traffic-probe", action="store_true",
                     help="run 5 plain SpMV launches on the workload and one copy kernel of 2^26 doubles "
                          "(traffic calibration under rocprofv3 --pmc), then exit")
+    ap.add_argument("--kron-probe", type=int, default=-1, metavar="VARIANT",
+                    help="build the workload's Kronecker form, apply product-kernel VARIANT to a seeded vector "
+                         "5 times (for rocprofv3 --pmc), then exit")
     ap.add_argument("--schedule", default="",
                     help="SpMV schedule 'lanes,cap,variant' instead of autotuning (bench and --traffic-probe)")
     ap.add_argument("--spmv-sweep", action="store_true",
@@ -1171,6 +1174,19 @@ def main():
                                                    else "hubbard_L64_m400")
     path = SUPERBLOCKS[wl][0]
     fixed = tuple(int(v) for v in args.schedule.split(",")) if args.schedule else None
+    if args.kron_probe >= 0:
+        snap = snapshot.load(path)
+        sb = D.Superblock(snap)
+        K = sb.kron()
+        K.set_kernel(args.kron_probe)
+        x = np.random.default_rng(1).standard_normal(K.n)
+        for _ in range(5):
+            y = K.matvec(x)
+        print(json.dumps({"kron_products": 5, "variant": args.kron_probe, "n": K.n, "flops_per_product": K.flops,
+                          "y0": float(y[0])}), flush=True)
+        K.free()
+        sb.free()
+        return
     if args.traffic_probe or args.spmv_sweep:
         snap = snapshot.load(path)
         sb = D.Superblock(snap)
