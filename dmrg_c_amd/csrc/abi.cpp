@@ -683,11 +683,13 @@ int ground_state_op(const Operator &op, const dmrgx_matrix *A, const dmrgx_solve
     if (op.n == 1) {
       double h;
       if (A) {
+        HIPCHK(hipStreamSynchronize(st));   // the values may still be written on the library stream
         HIPCHK(hipMemcpy(&h, A->val.p, sizeof(double), hipMemcpyDeviceToHost));
       } else {   // the 1 x 1 operator applied to 1
         const double one = 1.0;
         v.up(&one);
         op.kron->product(v.d.p, st);
+        HIPCHK(hipStreamSynchronize(st));   // the library's streams do not block on the null stream
         double hl, hc;
         HIPCHK(hipMemcpy(&h, op.kron->z.p, sizeof(double), hipMemcpyDeviceToHost));
         HIPCHK(hipMemcpy(&hl, op.kron->zl.p, sizeof(double), hipMemcpyDeviceToHost));

@@ -1177,6 +1177,7 @@ void admit(dmrgx_engine &E) {
       // 1x1: the eigenpair is the element itself (as dmrgx_ground_state)
       sp->vec.resize(1);
       double h;
+      HIPCHK(hipStreamSynchronize(ctx().stream));   // the library's streams do not block on the null stream
       HIPCHK(hipMemcpy(&h, sp->A->val.p, sizeof(double), hipMemcpyDeviceToHost));
       const double one = 1.0;
       HIPCHK(hipMemcpy(sp->vec.p, &one, sizeof(double), hipMemcpyHostToDevice));
