@@ -163,6 +163,39 @@ void Context::copy_in(void *dev_dst, const void *src, size_t bytes) {
   if (chunks > 1) HIPCHK(hipEventSynchronize(xfer_ev[chunks & 1]));
 }
 
+void Context::copy_in_many(const std::vector<CopySeg> &segs) {
+  size_t total = 0;
+  for (const CopySeg &g : segs) total += g.bytes;
+  if (total == 0) return;
+  xfer_init(*this);
+  // the segments' bytes in order, kXferChunk at a time; a chunk's pieces go to their own
+  // destinations
+  size_t seg = 0, in_seg = 0;   // next byte to stage: segment seg, offset in_seg
+  const size_t chunks = (total + kXferChunk - 1) / kXferChunk;
+  for (size_t k = 0; k < chunks; k++) {
+    if (k >= 2) HIPCHK(hipEventSynchronize(xfer_ev[k & 1]));   // chunk k - 2's DMAs have read the buffer
+    char *pin = xfer_pin[k & 1];
+    size_t fill = 0;
+    while (fill < kXferChunk && seg < segs.size()) {
+      const CopySeg &g = segs[seg];
+      const size_t len = std::min(kXferChunk - fill, g.bytes - in_seg);
+      if (len > 0) {
+        std::memcpy(pin + fill, static_cast<const char *>(g.src) + in_seg, len);
+        HIPCHK(hipMemcpyAsync(static_cast<char *>(g.dev_dst) + in_seg, pin + fill, len, hipMemcpyHostToDevice, stream));
+        fill += len;
+        in_seg += len;
+      }
+      if (in_seg == g.bytes) {
+        seg++;
+        in_seg = 0;
+      }
+    }
+    HIPCHK(hipEventRecord(xfer_ev[k & 1], stream));
+  }
+  HIPCHK(hipEventSynchronize(xfer_ev[(chunks - 1) & 1]));
+  if (chunks > 1) HIPCHK(hipEventSynchronize(xfer_ev[chunks & 1]));
+}
+
 hipEvent_t *Context::step_events() {
   if (!step_ev[0])
     for (int i = 0; i < 2; i++) HIPCHK(hipEventCreateWithFlags(&step_ev[i], hipEventDisableTiming));

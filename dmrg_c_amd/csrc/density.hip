@@ -168,9 +168,9 @@ void density_device(const DMRG_BASIS &B, int32_t b0, int32_t b1, const int32_t *
   DevBuf<int32_t> didx(n_psi > 0 ? n_psi : 1);
   DevBuf<DmTile> dt(tiles.size());
   dout.resize(n_out > 0 ? n_out : 1);
-  C.copy_in(vec.p, Vec, sizeof(double) * n);
-  if (n_psi > 0) C.copy_in(didx.p, idx.data(), sizeof(int32_t) * n_psi);
-  C.copy_in(dt.p, tiles.data(), sizeof(DmTile) * tiles.size());
+  C.copy_in_many({{vec.p, Vec, sizeof(double) * n},
+                  {didx.p, idx.data(), sizeof(int32_t) * (size_t)std::max<int64_t>(n_psi, 0)},
+                  {dt.p, tiles.data(), sizeof(DmTile) * tiles.size()}});
   if (n_psi > 0) k_dm_gather<<<(unsigned)((n_psi + 255) / 256), 256, 0, st>>>(n_psi, didx.p, vec.p, psi.p);
   // an empty partner list leaves its block's matrix zero, as the reference's does
   k_dm_syrk<<<(unsigned)tiles.size(), 256, 0, st>>>(dt.p, psi.p, dout.p);

@@ -62,6 +62,14 @@ struct Context {
   // landed (copy_in: on the device, in stream order).
   void copy_out(void *dst, const void *dev_src, size_t bytes);
   void copy_in(void *dev_dst, const void *src, size_t bytes);
+  // several host arrays to their device destinations through the same pinned chunks, as one
+  // pipelined transfer (one wait at the end instead of one per array)
+  struct CopySeg {
+    void *dev_dst;
+    const void *src;
+    size_t bytes;
+  };
+  void copy_in_many(const std::vector<CopySeg> &segs);
   char *xfer_pin[2] = {nullptr, nullptr};
   hipEvent_t xfer_ev[2] = {nullptr, nullptr};
   // two events for the solvers' pipelined chunk read-backs
