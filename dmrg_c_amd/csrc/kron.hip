@@ -439,13 +439,16 @@ int dmrgx_superblock_kron(const dmrgx_superblock *S, dmrgx_kron **out) {
       while (a < n && a + sc.db <= n && RR[a] == RR[r] && RL[a] == RL[r]) a += sc.db;
       sc.da = (int32_t)((a - r) / sc.db);
       // every row of the sector must be (LLLR state i, RRRL state j) at off + i db + j
-      for (int32_t i = 0; i < sc.da; i++)
-        for (int32_t j = 0; j < sc.db; j++) {
-          const int64_t q = r + (int64_t)i * sc.db + j;
-          if (LL[q] != LL[r + (int64_t)i * sc.db] || LR[q] != LR[r + (int64_t)i * sc.db] || RR[q] != RR[r + j] ||
-              RL[q] != RL[r + j])
-            throw Failure(DMRGX_ERR_ARG, "superblock basis is not a product of LLLR and RRRL states per sector");
-        }
+      // the product check: every row run of the sector has one (LL, LR) and the first run's
+      // RRRL sequence
+      for (int32_t i = 0; i < sc.da; i++) {
+        const int64_t q0 = r + (int64_t)i * sc.db;
+        int same = std::memcmp(&RR[q0], &RR[r], sizeof(int16_t) * sc.db) == 0 &&
+                   std::memcmp(&RL[q0], &RL[r], sizeof(int16_t) * sc.db) == 0;
+        const int16_t ll = LL[q0], lr = LR[q0];
+        for (int32_t j = 1; j < sc.db; j++) same &= (int)(LL[q0 + j] == ll) & (int)(LR[q0 + j] == lr);
+        if (!same) throw Failure(DMRGX_ERR_ARG, "superblock basis is not a product of LLLR and RRRL states per sector");
+      }
       sc.a0 = (int32_t)aLL.size();
       sc.b0 = (int32_t)bRR.size();
       for (int32_t i = 0; i < sc.da; i++) {
