@@ -110,7 +110,7 @@ def autotune(D, sb):
 
 def cpu_env():
     """OpenMP placement for the reference on the host: one thread per core of this process's
-    CPU set (at most 16, the GPU box's share), bound close on cores.  Read by libgomp when it
+    CPU set (at most 16: the host cores that go with one GPU), bound close on cores.  Read by libgomp when it
     starts, so set before any OpenMP library is loaded."""
     cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
     threads = max(1, min(16, cores))
