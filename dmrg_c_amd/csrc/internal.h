@@ -868,11 +868,12 @@ constexpr int kKronShapes = 4;
 inline constexpr int kKronTM[kKronShapes] = {64, 128, 64, 128};
 inline constexpr int kKronTN[kKronShapes] = {64, 128, 128, 64};
 constexpr int kKronVariants = 8;
+constexpr int kKronDefault = 1;   // 64 x 64 tiles, FP64 matrix cores, default occupancy, no XCD mapping
 inline constexpr int kKronShape[kKronVariants] = {0, 0, 1, 2, 3, 0, 0, 1};
 struct KronDev {
   int32_t n = 0, nsec = 0, ncross = 0, na = 0, nb = 0;
   int32_t ntiles[kKronShapes] = {0, 0, 0, 0};
-  int variant = 1;                        // product kernel (kKronShape of it picks the tiles)
+  int variant = kKronDefault;             // product kernel (kKronShape of it picks the tiles)
   int xcd = 0;                            // 1: consecutive tiles to the blocks of one XCD
   const dmrgx_superblock *sb = nullptr;   // borrowed: basis, inverse, operators, term list
   std::vector<KronSector> hsec;

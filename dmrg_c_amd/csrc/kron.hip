@@ -675,7 +675,7 @@ int dmrgx_kron_autotune(dmrgx_kron *K, int32_t *variant_out, double *us_out) {
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
     int best = D.variant, best_xcd = D.xcd;
-    float best_ms = 1e30f;
+    float best_ms = 1e30f, dflt_ms = 1e30f;
     for (int c = 0; c < 2 * kKronVariants; c++) {
       const int v = c % kKronVariants, xm = c / kKronVariants;
       D.variant = v;
@@ -687,6 +687,7 @@ int dmrgx_kron_autotune(dmrgx_kron *K, int32_t *variant_out, double *us_out) {
       HIPCHK(hipEventSynchronize(e1));
       float ms = 0;
       HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+      if (c == kKronDefault) dflt_ms = ms;
       if (ms < best_ms) {
         best_ms = ms;
         best = v;
@@ -695,6 +696,14 @@ int dmrgx_kron_autotune(dmrgx_kron *K, int32_t *variant_out, double *us_out) {
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
+    // the default kernel stays unless another is more than 2 % faster: the five products'
+    // timing varies by about 1 %, and between the two 64 x 64 kernels, which time within
+    // that, whole solves favour the default-occupancy one
+    if (dflt_ms <= 1.02f * best_ms) {
+      best_ms = dflt_ms;
+      best = kKronDefault % kKronVariants;
+      best_xcd = kKronDefault / kKronVariants;
+    }
     D.variant = best;
     D.xcd = best_xcd;
     if (variant_out) *variant_out = best + kKronVariants * best_xcd;
